@@ -1,0 +1,190 @@
+"""DROP-iteration benchmark on one MI355X.
+
+A step is one DROP iteration (MLP voxel lists of every history + the
+sequential block update, blocks of 3200 in storage order) over the
+reconstruction histories of a synthetic scan resident in HBM. Workload: the
+200x200x32 grid of 1x1x2.5 mm voxels with 20 M histories over 90 angles
+(the plain FP64-MLP GPU configuration); pass --config cfg1 for 100x100x16 /
+2 M histories.
+
+Prints one JSON line with the metric (histories/s per DROP iteration), the
+roofline of the dominant phase measured with HIP events on the stream the
+kernels run on, and the CPU oracle timed on a bounded sample on the host.
+
+  python bench.py [--gpus 1] [--steps K] [--warmup W] [--config cfg2]
+"""
+import argparse
+import json
+import os
+import sys
+import time
+
+ROOT = os.path.dirname(os.path.abspath(__file__))
+sys.path.insert(0, ROOT)
+
+HBM_PEAK_GBS = 8000.0        # MI355X HBM3E, GB/s
+FP64_PEAK_TFLOPS = 78.6      # MI355X FP64, TFLOP/s
+# FP64 operations of one MLP step of the reference loop body (.cu:5376-5491):
+# +,-,*,/ and each log/pow counted once ("Measurement").
+MLP_FLOPS_PER_STEP = 235
+
+
+def log(msg):
+    print(msg, file=sys.stderr, flush=True)
+
+
+def main():
+    ap = argparse.ArgumentParser()
+    ap.add_argument("--gpus", type=int, default=1)
+    ap.add_argument("--steps", type=int, default=3)
+    ap.add_argument("--warmup", type=int, default=1)
+    ap.add_argument("--config", default="cfg2")
+    ap.add_argument("--histories", type=int, default=None, help="override the scan size")
+    ap.add_argument("--cpu-seconds", type=float, default=15.0, help="target CPU time of the oracle sample")
+    ap.add_argument("--no-cpu-baseline", action="store_true")
+    args = ap.parse_args()
+
+    if args.gpus != 1:
+        print(json.dumps({"error": "this build runs on one GPU only; multi-GPU sharding is described in "
+                                   "DESIGN.md but not built", "n_gpus": args.gpus}))
+        sys.exit(2)
+
+    import numpy as np
+    import torch
+
+    import pct_amd
+    from pct_amd import configs
+
+    if not torch.cuda.is_available():
+        print(json.dumps({"error": "no GPU"}))
+        sys.exit(1)
+    dev = torch.device("cuda:0")
+    pre = configs.PRESETS[args.config]
+    p = configs.params(args.config)
+    scfg = configs.synth_cfg(args.config, n_histories=args.histories)
+    ctx = pct_amd.Context(p, dev)
+
+    # ---- setup (untimed): scan -> intersections -> hull -> endpoints ----------
+    t0 = time.time()
+    raw = ctx.generate(scfg)
+    hist, missed = ctx.recon_volume_intersections(raw)
+    del raw
+    hist = ctx.compact(hist, (missed == 0).to(torch.uint8))
+    hull = ctx.hull_init()
+    inter = ctx.mlp_endpoints(hist, hull)
+    hist = ctx.compact(hist, inter)
+    ctx.sync()
+    n_r = hist.n
+    V = p.columns * p.rows * p.slices
+    bits = hull.cpu().numpy().view(np.uint32)
+    idx = np.arange(V)
+    x0_np = ((bits[idx >> 5] >> (idx & 31)) & 1).astype(np.float32)
+    x = torch.from_numpy(x0_np).to(dev)
+    B = p.drop_block_size
+    sum_n, sum_vb = pct_amd.drop_stats(ctx, hist, B)
+    _, total_slots = ctx.mlp_capacity(hist)
+    sum_steps = total_slots - n_r
+    log(f"setup {time.time() - t0:.1f}s: N={scfg.n_histories} N_r={n_r} mean n={sum_n / n_r:.1f} "
+        f"sum|V_B|={sum_vb} steps={sum_steps}")
+
+    # ---- timed DROP iterations ----------------------------------------------
+    pct_amd.set_profiling(ctx, True)
+    for _ in range(args.warmup):
+        ctx.drop_iteration(hist, x, B, p.lambda_)
+    ctx.sync()
+    torch.cuda.synchronize()
+    phases = []
+    t_start = time.perf_counter()
+    ev0 = torch.cuda.Event(enable_timing=True)
+    ev1 = torch.cuda.Event(enable_timing=True)
+    ev0.record(ctx.stream)
+    for _ in range(args.steps):
+        ctx.drop_iteration(hist, x, B, p.lambda_)
+        phases.append(pct_amd.last_timings(ctx))
+    ev1.record(ctx.stream)
+    torch.cuda.synchronize()
+    wall = time.perf_counter() - t_start
+    gpu_ms = ev0.elapsed_time(ev1)
+    ms_per_step = gpu_ms / args.steps
+    value = n_r * args.steps / (gpu_ms / 1e3)
+    ph = np.array(phases)
+    cap_ms, mlp_ms, blk_ms = ph.mean(axis=0)
+    log(f"timed {args.steps} iterations: {ms_per_step:.2f} ms/iter (wall {wall / args.steps * 1e3:.2f} ms); "
+        f"capacity {cap_ms:.2f} ms, MLP {mlp_ms:.2f} ms, blocks {blk_ms:.2f} ms")
+
+    # algorithmic work per iteration (DESIGN.md "Measurement")
+    bytes_update = 48.0 * n_r + 8.0 * sum_n + 24.0 * sum_vb
+    upd_gbs = bytes_update / (blk_ms / 1e3) / 1e9
+    mlp_tflops = MLP_FLOPS_PER_STEP * float(sum_steps) / (mlp_ms / 1e3) / 1e12
+    roof_update = {"bound": "hbm", "achieved": round(upd_gbs, 2), "peak": HBM_PEAK_GBS, "unit": "GB/s",
+                   "frac": round(upd_gbs / HBM_PEAK_GBS, 4), "traffic": None,
+                   "kernel": "k_drop_scatter+k_drop_apply (block loop)", "ms": round(float(blk_ms), 3),
+                   "bytes_alg": bytes_update}
+    roof_mlp = {"bound": "mfma", "achieved": round(mlp_tflops, 3), "peak": FP64_PEAK_TFLOPS, "unit": "TFLOP/s",
+                "frac": round(mlp_tflops / FP64_PEAK_TFLOPS, 4), "traffic": None,
+                "kernel": "k_mlp_paths (FP64 VALU; peak = MI355X FP64 78.6 TFLOP/s)", "ms": round(float(mlp_ms), 3),
+                "flops_alg": MLP_FLOPS_PER_STEP * float(sum_steps)}
+    dominant = roof_mlp if mlp_ms >= blk_ms else roof_update
+
+    # ---- CPU oracle on a bounded sample, and RMSE of the GPU vs the oracle ----
+    cpu = None
+    rmse = None
+    if not args.no_cpu_baseline:
+        import oracle
+        hh = hist.numpy()
+        # calibrate on one block, then size the sample to ~cpu_seconds
+        m0 = min(B, n_r)
+        hs = oracle.HostHistories(arrays={k: v[:m0] for k, v in hh.items()})
+        t = time.perf_counter()
+        oracle.drop(p, hs, x0_np, iterations=1)
+        per_hist = (time.perf_counter() - t) / m0
+        m = int(min(n_r, max(m0, args.cpu_seconds / per_hist)))
+        m = max(B, (m // B) * B) if m >= B else m
+        hs = oracle.HostHistories(arrays={k: v[:m] for k, v in hh.items()})
+        t = time.perf_counter()
+        x_cpu = oracle.drop(p, hs, x0_np, iterations=1)
+        cpu_t = time.perf_counter() - t
+        cpu = {"value": round(m / cpu_t, 1), "unit": "histories/s per DROP iteration", "cores": 1, "kind": "port",
+               "sample": f"first {m} reconstruction histories ({m // B} DROP blocks of {B}), one iteration from the "
+                         f"same x0, FP64 oracle (MLP + DROP_blocks/DROP_update), single thread, {cpu_t:.1f} s"}
+        # the GPU on the same sample from the same x0
+        x_g = torch.from_numpy(x0_np).to(dev)
+        ctx.drop_iteration(hist.head(m), x_g, B, p.lambda_)
+        diff = x_g.cpu().numpy().astype(np.float64) - x_cpu.astype(np.float64)
+        touched = x_cpu != x0_np
+        rmse = {"rmse": float(np.sqrt(np.mean(diff[touched] ** 2))) if touched.any() else 0.0,
+                "max_abs": float(np.max(np.abs(diff))), "voxels": int(touched.sum()),
+                "sample": f"one iteration over the first {m} histories, GPU FP32 image vs FP64 oracle"}
+
+    out = {
+        "metric": "histories/s per DROP iteration (MLP+update), RSP RMSE vs FP64 oracle, 1 GPU",
+        "value": round(value, 1),
+        "unit": "histories/s",
+        "n_gpus": 1,
+        "steps": args.steps,
+        "warmup": args.warmup,
+        "ms_per_step": round(ms_per_step, 3),
+        "higher_is_better": True,
+        "scaling": "weak",
+        "vs_baseline": None,
+        "dtype": "f64",
+        "data": "synthetic",
+        "config": {"workload": f"{args.config}: {p.columns}x{p.rows}x{p.slices} voxels, "
+                               f"{scfg.n_histories} histories / 90 angles, DROP block {B}, lambda {p.lambda_:g}, "
+                               f"exact FP64 MLP, FP32 image, cylinder hull",
+                   "histories": scfg.n_histories, "reconstruction_histories": n_r,
+                   "mean_path_voxels": round(sum_n / n_r, 2), "sum_block_voxels": sum_vb,
+                   "drop_blocks": (n_r + B - 1) // B},
+        "phases_ms": {"capacity": round(float(cap_ms), 3), "mlp": round(float(mlp_ms), 3),
+                      "blocks": round(float(blk_ms), 3)},
+        "roofline": dominant,
+        "roofline_update": roof_update,
+        "roofline_mlp": roof_mlp,
+        "rmse_vs_oracle": rmse,
+        "cpu_baseline": cpu,
+    }
+    print(json.dumps(out), flush=True)
+
+
+if __name__ == "__main__":
+    main()

@@ -1,0 +1,184 @@
+/*
+ * pct.h -- C ABI of the MI355X proton-CT reconstruction engine.
+ *
+ * The reference (lynch829/pCT) has no plugin or FFI layer: its "interface" is a
+ * set of host stage functions that read and write global device arrays and
+ * launch CUDA kernels with positional SoA pointers
+ * (include/pCT_functions.h:35-70, 180-237). This header is the drop-in
+ * replacement for those stage/kernel boundaries on the DROP hot path: every
+ * entry point below names the reference function it replaces.
+ *
+ * Conventions
+ *   - All array pointers are DEVICE pointers owned by the caller unless stated.
+ *   - Every call is ordered on the context's HIP stream and returns an int
+ *     status: 0 = ok, negative = error (PCT_E_*); pct_last_error() gives text.
+ *   - A context is not thread-safe; use one context per host thread.
+ *   - Units are the reference's: cm, radians, gantry angle in integer degrees.
+ */
+#ifndef PCT_H
+#define PCT_H
+
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+/* ------------------------------------------------------------------------- */
+/* Parameters: the reference's compile-time constants as one runtime struct.  */
+/* Field names follow include/pCT_data.h and include/pCT_config.h.            */
+/* ------------------------------------------------------------------------- */
+typedef struct pct_params {
+    int32_t columns, rows, slices;           /* COLUMNS, ROWS, SLICES (data.h:264-266)          */
+    int32_t t_bins, v_bins, angular_bins;    /* T_BINS, V_BINS, ANGULAR_BINS (data.h:238-243)   */
+    double  recon_cyl_radius;                /* RECON_CYL_RADIUS [cm] (data.h:255)              */
+    double  recon_cyl_height;                /* RECON_CYL_HEIGHT [cm] (data.h:257)              */
+    double  slice_thickness;                 /* SLICE_THICKNESS [cm] (data.h:271)               */
+    double  t_bin_size, v_bin_size;          /* T_BIN_SIZE, V_BIN_SIZE [cm] (data.h:237,239)     */
+    double  angular_bin_size;                /* ANGULAR_BIN_SIZE [deg] (data.h:241)             */
+    double  ssd_t_size, ssd_v_size;          /* SSD_T_SIZE, SSD_V_SIZE [cm] (data.h:227-228)     */
+    double  source_radius;                   /* SOURCE_RADIUS [cm] (data.h:222)                 */
+    double  gantry_angle_interval;           /* GANTRY_ANGLE_INTERVAL [deg] (data.h:223)        */
+    double  sigmas_to_keep;                  /* SIGMAS_TO_KEEP (data.h:244)                     */
+    double  fbp_threshold;                   /* FBP_THRESHOLD (data.h:248)                      */
+    double  rsp_air;                         /* RSP_AIR (data.h:297)                            */
+    double  sc_threshold, msc_threshold;     /* SC_THRESHOLD, MSC_THRESHOLD (data.h:282-283)    */
+    double  hull_filter_threshold;           /* HULL_FILTER_THRESHOLD (data.h:287)              */
+    int32_t hull_filter_radius;              /* HULL_FILTER_RADIUS (data.h:288)                 */
+    int32_t msc_diff_thresh;                 /* MSC_DIFF_THRESH (data.h:281)                    */
+    double  e_0, x_0;                        /* E_0 [MeV/c], X_0 [cm] (data.h:294-295)           */
+    double  a[6];                            /* A_0..A_5 (data.h:306-311), folded to doubles    */
+    int32_t max_intersections;               /* MAX_INTERSECTIONS (data.h:299)                  */
+    int32_t drop_block_size;                 /* DROP_BLOCK_SIZE / BLOCK_SIZE_RECON (data.h:377-378) */
+    float   lambda;                          /* LAMBDA (data.h:374)                             */
+    int32_t iterations;                      /* ITERATIONS (cfg.h:5)                            */
+} pct_params;
+
+/* Raw tracker data of one read chunk (data.h:497-508): 4 tracker planes in the
+ * gantry frame (u along the beam, t lateral, v vertical) + WEPL. */
+typedef struct pct_raw_soa {
+    const int32_t* gantry_angle;                          /* [deg]   */
+    const float *t_in_1, *t_in_2, *t_out_1, *t_out_2;
+    const float *u_in_1, *u_in_2, *u_out_1, *u_out_2;
+    const float *v_in_1, *v_in_2, *v_out_1, *v_out_2;
+    const float* wepl;
+    int64_t n;
+} pct_raw_soa;
+
+/* Per-history record after intersection (data.h:577-590, same field set and
+ * order as the reference's *_vector SoA). */
+typedef struct pct_hist_soa {
+    float *x_entry, *y_entry, *z_entry;
+    float *x_exit, *y_exit, *z_exit;
+    float *xy_entry_angle, *xz_entry_angle;
+    float *xy_exit_angle, *xz_exit_angle;
+    float* wepl;
+    int32_t* bin_num;
+    uint32_t* first_voxel;                               /* first_MLP_voxel (data.h:590) */
+    int64_t n;
+} pct_hist_soa;
+
+/* Status codes */
+#define PCT_OK              0
+#define PCT_E_INVALID     (-1)   /* bad argument / size                         */
+#define PCT_E_HIP         (-2)   /* HIP runtime error                          */
+#define PCT_E_OVERFLOW    (-3)   /* a path longer than max_intersections        */
+#define PCT_E_NOMEM       (-4)   /* device allocation failed                    */
+
+/* MLP evaluation modes for the path and DROP entry points */
+#define PCT_MLP_FP64      0      /* exact FP64 Schulte MLP (.cu:5279-5501)      */
+
+typedef struct pct_ctx pct_ctx;
+
+/* Context: owns a HIP stream (or borrows `stream` if non-NULL) and the
+ * workspaces of the DROP loop. Replaces the global device state of
+ * data.h:512-573 and cudaSetDevice(0) in image_reconstruction_GPU (.cu:6543). */
+pct_ctx*    pct_create(const pct_params* params, void* stream);
+void        pct_destroy(pct_ctx* ctx);
+int         pct_sync(pct_ctx* ctx);
+const char* pct_last_error(const pct_ctx* ctx);
+/* Fill `out` with the reference defaults (pCT_data.h / pCT_config.h). */
+void        pct_default_params(pct_params* out);
+
+/* (a) Recon-volume intersection: replaces recon_volume_intersections_GPU
+ * (.cu:2169-2300) and calculate_intercepts (.cu:2301-2387). Writes the entry
+ * and exit points/angles of every history and missed[i] = 1 when the path
+ * misses the reconstruction cylinder. out->n must equal raw->n; out->wepl is
+ * copied from raw->wepl, out->bin_num / first_voxel are left untouched. */
+int pct_recon_volume_intersections(pct_ctx* ctx, const pct_raw_soa* raw,
+                                   const pct_hist_soa* out, uint8_t* missed);
+
+/* (c) Cylinder hull: replaces initialize_hull_GPU (.cu:3316-3326). The hull is
+ * bit-packed, one bit per voxel, voxel v at word v>>5, bit v&31;
+ * ceil(V/32) words. */
+int pct_hull_init(pct_ctx* ctx, uint32_t* hull_bits);
+
+/* (d) MLP endpoints: replaces collect_MLP_endpoints_GPU (.cu:5729-5770) with
+ * find_MLP_endpoints_GPU (.cu:4997-5277). For histories that enter and exit
+ * the hull, overwrites x/y/z entry/exit with the hull surface points and sets
+ * first_voxel; intersected[i] = 1 for those, 0 otherwise. */
+int pct_mlp_endpoints(pct_ctx* ctx, const pct_hist_soa* hist,
+                      const uint32_t* hull_bits, uint8_t* intersected);
+
+/* Stable, order-preserving compaction of every field of `src` whose keep[i]
+ * is nonzero into `dst` (dst arrays sized >= src->n). Replaces the host
+ * push_back loops of .cu:2443-2477, 2777-2809, 6277-6298. The number of kept
+ * histories is written to *n_kept (host pointer) and to dst->n is NOT
+ * written (the struct is const); the caller sets it. */
+int pct_compact(pct_ctx* ctx, const pct_hist_soa* src, const uint8_t* keep,
+                const pct_hist_soa* dst, int64_t* n_kept);
+
+/* (d) MLP voxel lists for histories [start, start+count): replaces the path
+ * part of find_MLP_path_GPU (.cu:5279-5501). Path of history h is written to
+ * voxels[offsets[h-start] ...] (offsets is a device array of count+1 int64
+ * prefix offsets from pct_mlp_capacity), its length to lengths[h-start].
+ * Returns PCT_E_OVERFLOW if a path exceeds max_intersections. */
+int pct_mlp_capacity(pct_ctx* ctx, const pct_hist_soa* hist, int64_t start,
+                     int64_t count, int64_t* offsets, int64_t* total);
+int pct_mlp_paths(pct_ctx* ctx, const pct_hist_soa* hist, int64_t start,
+                  int64_t count, const int64_t* offsets, uint32_t* voxels,
+                  uint32_t* lengths, int mode);
+
+/* (e) One DROP block: replaces block_update_GPU (.cu:5774-5841): for every
+ * history of [start, start+count) computes its MLP path, a_i.x, the residual
+ * and scatters count[v] += 1, x_update[v] += delta_i. */
+int pct_drop_block_update(pct_ctx* ctx, const pct_hist_soa* hist, int64_t start,
+                          int64_t count, const float* x, float* x_update,
+                          uint32_t* counts, float lambda, int mode);
+
+/* (e) Image update: replaces image_update_GPU (.cu:5900-5938):
+ * x[v] += x_update[v] / counts[v] where counts[v] > 0, then clears both. */
+int pct_drop_image_update(pct_ctx* ctx, float* x, float* x_update, uint32_t* counts);
+
+/* (e) One whole DROP iteration over all histories in storage order, blocks of
+ * `block_size`: replaces the body of the iteration loop of
+ * image_reconstruction_GPU (.cu:6795-6857). Equivalent to the sequence of
+ * pct_drop_block_update + pct_drop_image_update per block. */
+int pct_drop_iteration(pct_ctx* ctx, const pct_hist_soa* hist, float* x,
+                       int block_size, float lambda, int mode);
+
+/* Measurement (counterpart of the reference's per-iteration clock() and
+ * cudaEvent timing, .cu:6773-6849): when profiling is on, pct_drop_iteration
+ * records HIP events on the context stream and pct_last_timings returns the
+ * last iteration's phase durations in ms: [0] path capacities (prefix sum),
+ * [1] MLP voxel lists of all histories, [2] the sequential DROP block loop. */
+int pct_set_profiling(pct_ctx* ctx, int on);
+int pct_last_timings(pct_ctx* ctx, double* ms3);
+/* Work statistics of one DROP iteration (x is not modified): *sum_path =
+ * sum over histories of the path length n_i, *sum_touched = sum over blocks of
+ * the number of distinct voxels the block touches (|V_B|). */
+int pct_drop_stats(pct_ctx* ctx, const pct_hist_soa* hist, int block_size, int64_t* sum_path, int64_t* sum_touched);
+
+/* Diagnostics (no counterpart in the reference): evaluates one function of
+ * the shared FP64/FP32 math library (pct_math.h) on the device for n inputs,
+ * so host/device bit equality can be checked directly.
+ * func: 0 log, 1 sin, 2 cos, 3 tan, 4 atan, 5 atan2(in[i], in2[i]),
+ *       6 sinf, 7 cosf, 8 atan2f, 9 sqrt, 10 division in[i] / in2[i].
+ * Single-precision functions read and write the low 32 bits as float. */
+int pct_selftest_math(pct_ctx* ctx, int func, const double* in, const double* in2, double* out, int64_t n);
+
+#ifdef __cplusplus
+}
+#endif
+
+#endif /* PCT_H */

@@ -1,0 +1,183 @@
+"""CPU oracle -- TEST INFRASTRUCTURE ONLY.
+
+ctypes wrapper of oracle/liboracle.so, the plain C++ restatement of the
+reference (lynch829/pCT) built by oracle/build.py. Used only by tests/,
+__graft_entry__.smoke() and bench.py's cpu_baseline leg, as the checker. The
+product package pct_amd never imports this module.
+
+All arrays are host numpy arrays; the structs are the ABI structs of
+include/pct.h (shared with the product's binding).
+"""
+import ctypes
+import os
+
+import numpy as np
+
+from pct_amd import _lib as abi
+
+HERE = os.path.dirname(os.path.abspath(__file__))
+LIB_PATH = os.path.join(HERE, "liboracle.so")
+
+_L = None
+
+
+def lib():
+    global _L
+    if _L is None:
+        if not os.path.exists(LIB_PATH):
+            from . import build as _b
+            _b.build()
+        L = ctypes.CDLL(LIB_PATH)
+        P = ctypes.POINTER
+        vp = ctypes.c_void_p
+        L.oracle_recon_volume_intersections.argtypes = [P(abi.PctParams), P(abi.PctRawSoa), P(abi.PctHistSoa), vp]
+        L.oracle_hull_init.argtypes = [P(abi.PctParams), vp]
+        L.oracle_mlp_endpoints.argtypes = [P(abi.PctParams), P(abi.PctHistSoa), vp, vp]
+        L.oracle_chord_length.argtypes = [P(abi.PctParams), ctypes.c_float, ctypes.c_float, ctypes.c_float, ctypes.c_float]
+        L.oracle_chord_length.restype = ctypes.c_double
+        L.oracle_mlp_path.argtypes = [P(abi.PctParams), P(abi.PctHistSoa), ctypes.c_int64, vp, P(ctypes.c_int)]
+        L.oracle_mlp_paths.argtypes = [P(abi.PctParams), P(abi.PctHistSoa), ctypes.c_int64, ctypes.c_int64, vp,
+                                       ctypes.c_int64, vp]
+        L.oracle_drop.argtypes = [P(abi.PctParams), P(abi.PctHistSoa), vp, ctypes.c_int, ctypes.c_float, ctypes.c_int,
+                                  ctypes.c_int64]
+        L.oracle_math.argtypes = [ctypes.c_int, vp, vp, vp, ctypes.c_int64]
+        L.oracle_math.restype = None
+        _L = L
+    return _L
+
+
+MATH_FUNCS = {"log": 0, "sin": 1, "cos": 2, "tan": 3, "atan": 4, "atan2": 5, "sinf": 6, "cosf": 7, "atan2f": 8,
+              "sqrt": 9, "div": 10}
+
+
+def math_eval(func, a, b=None):
+    """Host evaluation of a pct_math.h function (see MATH_FUNCS)."""
+    f = MATH_FUNCS[func] if isinstance(func, str) else func
+    a = np.ascontiguousarray(a, dtype=np.float64)
+    b = None if b is None else np.ascontiguousarray(b, dtype=np.float64)
+    out = np.empty_like(a)
+    lib().oracle_math(f, _p(a), _p(b), _p(out), a.size)
+    return out
+
+
+def _p(a):
+    return ctypes.c_void_p(a.ctypes.data) if a is not None else ctypes.c_void_p(0)
+
+
+class HostHistories:
+    """numpy SoA mirror of pct_hist_soa."""
+
+    def __init__(self, n=0, arrays=None):
+        if arrays is not None:
+            self.n = len(arrays["wepl"])
+            for f in abi.HIST_FLOAT_FIELDS:
+                setattr(self, f, np.ascontiguousarray(arrays[f], dtype=np.float32))
+            self.bin_num = np.ascontiguousarray(arrays.get("bin_num", np.zeros(self.n)), dtype=np.int32)
+            self.first_voxel = np.ascontiguousarray(arrays.get("first_voxel", np.zeros(self.n)), dtype=np.uint32)
+        else:
+            self.n = n
+            for f in abi.HIST_FLOAT_FIELDS:
+                setattr(self, f, np.zeros(n, np.float32))
+            self.bin_num = np.zeros(n, np.int32)
+            self.first_voxel = np.zeros(n, np.uint32)
+
+    def soa(self):
+        s = abi.PctHistSoa()
+        for f in abi.HIST_FLOAT_FIELDS:
+            setattr(s, f, getattr(self, f).ctypes.data)
+        s.bin_num = self.bin_num.ctypes.data
+        s.first_voxel = self.first_voxel.ctypes.data
+        s.n = self.n
+        return s
+
+    def select(self, mask):
+        h = HostHistories(int(mask.sum()))
+        for f in abi.HIST_FLOAT_FIELDS + ("bin_num", "first_voxel"):
+            setattr(h, f, np.ascontiguousarray(getattr(self, f)[mask]))
+        return h
+
+    def head(self, n):
+        h = HostHistories(n)
+        for f in abi.HIST_FLOAT_FIELDS + ("bin_num", "first_voxel"):
+            setattr(h, f, np.ascontiguousarray(getattr(self, f)[:n]))
+        return h
+
+    def arrays(self):
+        d = {f: getattr(self, f) for f in abi.HIST_FLOAT_FIELDS}
+        d["bin_num"] = self.bin_num
+        d["first_voxel"] = self.first_voxel
+        return d
+
+
+def raw_soa(raw: dict):
+    """pct_raw_soa over a dict of numpy arrays (keeps them alive in the dict)."""
+    s = abi.PctRawSoa()
+    raw["gantry_angle"] = np.ascontiguousarray(raw["gantry_angle"], dtype=np.int32)
+    s.gantry_angle = raw["gantry_angle"].ctypes.data
+    for f in abi.RAW_FLOAT_FIELDS:
+        raw[f] = np.ascontiguousarray(raw[f], dtype=np.float32)
+        setattr(s, f, raw[f].ctypes.data)
+    s.n = len(raw["wepl"])
+    return s
+
+
+def recon_volume_intersections(p, raw: dict):
+    s = raw_soa(raw)
+    h = HostHistories(s.n)
+    missed = np.zeros(s.n, np.uint8)
+    hs = h.soa()
+    lib().oracle_recon_volume_intersections(ctypes.byref(p), ctypes.byref(s), ctypes.byref(hs), _p(missed))
+    return h, missed
+
+
+def hull_init(p):
+    hull = np.zeros(p.columns * p.rows * p.slices, np.uint8)
+    lib().oracle_hull_init(ctypes.byref(p), _p(hull))
+    return hull
+
+
+def mlp_endpoints(p, h: HostHistories, hull: np.ndarray):
+    inter = np.zeros(h.n, np.uint8)
+    hs = h.soa()
+    lib().oracle_mlp_endpoints(ctypes.byref(p), ctypes.byref(hs), _p(np.ascontiguousarray(hull, np.uint8)), _p(inter))
+    return inter
+
+
+def chord_length(p, xy_in, xy_out, xz_in, xz_out):
+    return lib().oracle_chord_length(ctypes.byref(p), xy_in, xy_out, xz_in, xz_out)
+
+
+def mlp_path(p, h: HostHistories, i: int):
+    buf = np.zeros(p.max_intersections + 1, np.uint32)
+    skipped = ctypes.c_int(0)
+    hs = h.soa()
+    n = lib().oracle_mlp_path(ctypes.byref(p), ctypes.byref(hs), i, _p(buf), ctypes.byref(skipped))
+    if n < 0:
+        raise OverflowError("MLP path longer than max_intersections")
+    return buf[:n].copy(), skipped.value
+
+
+def mlp_paths(p, h: HostHistories, start=0, count=None, stride=None):
+    """Voxel lists of histories [start, start+count) as a list of arrays."""
+    count = h.n - start if count is None else count
+    stride = p.max_intersections if stride is None else stride
+    vox = np.zeros(count * stride, np.uint32)
+    lens = np.zeros(count, np.uint32)
+    hs = h.soa()
+    rc = lib().oracle_mlp_paths(ctypes.byref(p), ctypes.byref(hs), start, count, _p(vox), stride, _p(lens))
+    if rc != 0:
+        raise OverflowError("MLP path overflow in the oracle")
+    return [vox[k * stride:k * stride + lens[k]].copy() for k in range(count)]
+
+
+def drop(p, h: HostHistories, x0: np.ndarray, iterations=1, block_size=None, lam=None, max_histories=-1):
+    """Plain DROP (E6 semantics) from x0; returns the new float32 image."""
+    x = np.ascontiguousarray(x0, dtype=np.float32).copy()
+    block_size = p.drop_block_size if block_size is None else block_size
+    lam = p.lambda_ if lam is None else lam
+    hs = h.soa()
+    rc = lib().oracle_drop(ctypes.byref(p), ctypes.byref(hs), _p(x), int(block_size), float(lam), int(iterations),
+                           int(max_histories))
+    if rc != 0:
+        raise OverflowError("MLP path overflow in the oracle DROP")
+    return x

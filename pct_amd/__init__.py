@@ -1,0 +1,263 @@
+"""pct_amd -- MI355X-native proton-CT reconstruction engine (DROP hot path).
+
+Host-side mirror of the reference's stage functions (lynch829/pCT,
+src/pCT_Reconstruction_Data_Segments.cu) over the C ABI of libpct.so
+(include/pct.h). Device memory is held in torch tensors (plumbing only); all
+compute runs in the hand-written HIP kernels of libpct.so. There is no CPU
+fallback: every entry point raises if libpct.so is missing or a call fails.
+"""
+import ctypes
+
+from . import _lib
+from . import configs
+from ._lib import PCT_MLP_FP64, PctHistSoa, PctParams, PctRawSoa, PctSynthCfg, PctSynthSoa
+
+__all__ = ["Context", "Histories", "RawScan", "configs", "PCT_MLP_FP64", "generate_host"]
+
+
+class PctError(RuntimeError):
+    pass
+
+
+def _ptr(t):
+    return ctypes.c_void_p(t.data_ptr()) if t is not None else ctypes.c_void_p(0)
+
+
+class RawScan:
+    """Raw tracker arrays (pct_raw_soa) in torch tensors."""
+
+    def __init__(self, n, device):
+        import torch
+        self.n = n
+        self.gantry_angle = torch.empty(n, dtype=torch.int32, device=device)
+        for f in _lib.RAW_FLOAT_FIELDS:
+            setattr(self, f, torch.empty(n, dtype=torch.float32, device=device))
+
+    def soa(self):
+        s = PctRawSoa()
+        s.gantry_angle = self.gantry_angle.data_ptr()
+        for f in _lib.RAW_FLOAT_FIELDS:
+            setattr(s, f, getattr(self, f).data_ptr())
+        s.n = self.n
+        return s
+
+    def synth_soa(self):
+        s = PctSynthSoa()
+        s.gantry_angle = self.gantry_angle.data_ptr()
+        for f in _lib.RAW_FLOAT_FIELDS:
+            setattr(s, f, getattr(self, f).data_ptr())
+        return s
+
+    def to(self, device):
+        r = RawScan.__new__(RawScan)
+        r.n = self.n
+        r.gantry_angle = self.gantry_angle.to(device)
+        for f in _lib.RAW_FLOAT_FIELDS:
+            setattr(r, f, getattr(self, f).to(device))
+        return r
+
+    def numpy(self):
+        d = {"gantry_angle": self.gantry_angle.cpu().numpy()}
+        for f in _lib.RAW_FLOAT_FIELDS:
+            d[f] = getattr(self, f).cpu().numpy()
+        return d
+
+
+class Histories:
+    """Per-history SoA (pct_hist_soa, data.h:577-590) in torch tensors."""
+
+    def __init__(self, n, device):
+        import torch
+        self.n = n
+        for f in _lib.HIST_FLOAT_FIELDS:
+            setattr(self, f, torch.zeros(n, dtype=torch.float32, device=device))
+        self.bin_num = torch.zeros(n, dtype=torch.int32, device=device)
+        self.first_voxel = torch.zeros(n, dtype=torch.int32, device=device)   # u32 bit pattern
+
+    def soa(self):
+        s = PctHistSoa()
+        for f in _lib.HIST_FLOAT_FIELDS:
+            setattr(s, f, getattr(self, f).data_ptr())
+        s.bin_num = self.bin_num.data_ptr()
+        s.first_voxel = self.first_voxel.data_ptr()
+        s.n = self.n
+        return s
+
+    def to(self, device):
+        h = Histories.__new__(Histories)
+        h.n = self.n
+        for f in _lib.HIST_FLOAT_FIELDS + ("bin_num", "first_voxel"):
+            setattr(h, f, getattr(self, f).to(device))
+        return h
+
+    def head(self, n):
+        h = Histories.__new__(Histories)
+        h.n = n
+        for f in _lib.HIST_FLOAT_FIELDS + ("bin_num", "first_voxel"):
+            setattr(h, f, getattr(self, f)[:n])
+        return h
+
+    def numpy(self):
+        d = {f: getattr(self, f).cpu().numpy() for f in _lib.HIST_FLOAT_FIELDS}
+        d["bin_num"] = self.bin_num.cpu().numpy()
+        d["first_voxel"] = self.first_voxel.cpu().numpy().view("uint32")
+        return d
+
+
+class Context:
+    """A pct_ctx on the current HIP device (pct_create, include/pct.h)."""
+
+    def __init__(self, params: PctParams, device="cuda"):
+        import torch
+        self.torch = torch
+        self.device = torch.device(device)
+        if self.device.type != "cuda":
+            raise PctError("pct_amd.Context needs a GPU device")
+        torch.cuda.set_device(self.device)
+        self.params = params.copy()
+        self.L = _lib.lib()
+        self.stream = torch.cuda.current_stream(self.device)
+        self._ctx = self.L.pct_create(ctypes.byref(self.params), ctypes.c_void_p(self.stream.cuda_stream))
+        if not self._ctx:
+            raise PctError("pct_create failed")
+        self.V = params.columns * params.rows * params.slices
+
+    def close(self):
+        if getattr(self, "_ctx", None):
+            self.L.pct_destroy(self._ctx)
+            self._ctx = None
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+    def _check(self, rc, what):
+        if rc != 0:
+            msg = self.L.pct_last_error(self._ctx)
+            raise PctError(f"{what} failed ({rc}): {msg.decode() if msg else ''}")
+
+    def sync(self):
+        self._check(self.L.pct_sync(self._ctx), "pct_sync")
+
+    # -- synthetic input ----------------------------------------------------
+    def generate(self, cfg: PctSynthCfg, start=0, count=None) -> RawScan:
+        count = cfg.n_histories - start if count is None else count
+        raw = RawScan(count, self.device)
+        soa = raw.synth_soa()
+        self._check(self.L.pct_synth_generate(self._ctx, ctypes.byref(cfg), start, count, ctypes.byref(soa)),
+                    "pct_synth_generate")
+        return raw
+
+    # -- (a) recon-volume intersection (.cu:2066-2168 / 2169-2387) ------------
+    def recon_volume_intersections(self, raw: RawScan):
+        torch = self.torch
+        hist = Histories(raw.n, self.device)
+        missed = torch.empty(raw.n, dtype=torch.uint8, device=self.device)
+        rs, hs = raw.soa(), hist.soa()
+        self._check(self.L.pct_recon_volume_intersections(self._ctx, ctypes.byref(rs), ctypes.byref(hs), _ptr(missed)),
+                    "pct_recon_volume_intersections")
+        return hist, missed
+
+    # -- (c) hull ---------------------------------------------------------------
+    def hull_init(self):
+        words = (self.V + 31) // 32
+        bits = self.torch.empty(words, dtype=self.torch.int32, device=self.device)
+        self._check(self.L.pct_hull_init(self._ctx, _ptr(bits)), "pct_hull_init")
+        return bits
+
+    # -- (d) endpoints + compaction (reconstruction_cuts, .cu:6142-6300) --------
+    def mlp_endpoints(self, hist: Histories, hull_bits):
+        inter = self.torch.empty(hist.n, dtype=self.torch.uint8, device=self.device)
+        hs = hist.soa()
+        self._check(self.L.pct_mlp_endpoints(self._ctx, ctypes.byref(hs), _ptr(hull_bits), _ptr(inter)),
+                    "pct_mlp_endpoints")
+        return inter
+
+    def compact(self, hist: Histories, keep) -> Histories:
+        out = Histories(hist.n, self.device)
+        n_kept = ctypes.c_int64(0)
+        s, d = hist.soa(), out.soa()
+        self._check(self.L.pct_compact(self._ctx, ctypes.byref(s), _ptr(keep), ctypes.byref(d), ctypes.byref(n_kept)),
+                    "pct_compact")
+        return out.head(n_kept.value)
+
+    # -- (d) MLP paths ----------------------------------------------------------
+    def mlp_capacity(self, hist: Histories, start=0, count=None):
+        count = hist.n - start if count is None else count
+        offs = self.torch.empty(count + 1, dtype=self.torch.int64, device=self.device)
+        total = ctypes.c_int64(0)
+        hs = hist.soa()
+        self._check(self.L.pct_mlp_capacity(self._ctx, ctypes.byref(hs), start, count, _ptr(offs), ctypes.byref(total)),
+                    "pct_mlp_capacity")
+        return offs, total.value
+
+    def mlp_paths(self, hist: Histories, start=0, count=None, mode=PCT_MLP_FP64):
+        count = hist.n - start if count is None else count
+        offs, total = self.mlp_capacity(hist, start, count)
+        vox = self.torch.zeros(max(total, 1), dtype=self.torch.int32, device=self.device)
+        lens = self.torch.zeros(max(count, 1), dtype=self.torch.int32, device=self.device)
+        hs = hist.soa()
+        self._check(self.L.pct_mlp_paths(self._ctx, ctypes.byref(hs), start, count, _ptr(offs), _ptr(vox), _ptr(lens),
+                                         mode), "pct_mlp_paths")
+        return offs, vox, lens[:count]
+
+    # -- (e) DROP -----------------------------------------------------------------
+    def drop_block_update(self, hist, start, count, x, x_update, counts, lam, mode=PCT_MLP_FP64):
+        hs = hist.soa()
+        self._check(self.L.pct_drop_block_update(self._ctx, ctypes.byref(hs), start, count, _ptr(x), _ptr(x_update),
+                                                 _ptr(counts), float(lam), mode), "pct_drop_block_update")
+
+    def drop_image_update(self, x, x_update, counts):
+        self._check(self.L.pct_drop_image_update(self._ctx, _ptr(x), _ptr(x_update), _ptr(counts)),
+                    "pct_drop_image_update")
+
+    def drop_iteration(self, hist, x, block_size=None, lam=None, mode=PCT_MLP_FP64):
+        block_size = self.params.drop_block_size if block_size is None else block_size
+        lam = self.params.lambda_ if lam is None else lam
+        hs = hist.soa()
+        self._check(self.L.pct_drop_iteration(self._ctx, ctypes.byref(hs), _ptr(x), int(block_size), float(lam), mode),
+                    "pct_drop_iteration")
+
+
+def set_profiling(ctx: "Context", on: bool):
+    ctx._check(ctx.L.pct_set_profiling(ctx._ctx, 1 if on else 0), "pct_set_profiling")
+
+
+def last_timings(ctx: "Context"):
+    """[capacity ms, MLP ms, block loop ms] of the last profiled iteration."""
+    out = (ctypes.c_double * 3)()
+    ctx._check(ctx.L.pct_last_timings(ctx._ctx, out), "pct_last_timings")
+    return list(out)
+
+
+def drop_stats(ctx: "Context", hist: "Histories", block_size: int):
+    """(sum of path lengths, sum over blocks of touched voxels) of one iteration."""
+    a, b = ctypes.c_int64(0), ctypes.c_int64(0)
+    hs = hist.soa()
+    ctx._check(ctx.L.pct_drop_stats(ctx._ctx, ctypes.byref(hs), int(block_size), ctypes.byref(a), ctypes.byref(b)),
+               "pct_drop_stats")
+    return a.value, b.value
+
+
+def selftest_math(ctx: "Context", func: int, a, b=None):
+    """Device evaluation of one pct_math.h function (diagnostics)."""
+    import torch
+    a = torch.as_tensor(a, dtype=torch.float64).to(ctx.device).contiguous()
+    b = None if b is None else torch.as_tensor(b, dtype=torch.float64).to(ctx.device).contiguous()
+    out = torch.empty_like(a)
+    ctx._check(ctx.L.pct_selftest_math(ctx._ctx, func, _ptr(a), _ptr(b), _ptr(out), a.numel()), "pct_selftest_math")
+    ctx.sync()
+    return out.cpu().numpy()
+
+
+def generate_host(cfg: PctSynthCfg, start=0, count=None) -> RawScan:
+    """The synthetic scan generated on the host (bit-identical to the device)."""
+    count = cfg.n_histories - start if count is None else count
+    raw = RawScan(count, "cpu")
+    soa = raw.synth_soa()
+    rc = _lib.lib().pct_synth_generate_host(ctypes.byref(cfg), start, count, ctypes.byref(soa))
+    if rc != 0:
+        raise PctError(f"pct_synth_generate_host failed ({rc})")
+    return raw

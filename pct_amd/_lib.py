@@ -1,0 +1,120 @@
+"""ctypes binding of libpct.so (the C ABI declared in include/pct.h and
+include/pct_synth_api.h).
+
+The library is loaded from this package directory. There is no fallback: if
+libpct.so is missing the import of the GPU entry points fails loudly.
+"""
+import ctypes
+import os
+
+_HERE = os.path.dirname(os.path.abspath(__file__))
+LIB_PATH = os.path.join(_HERE, "libpct.so")
+
+c_i32, c_i64, c_u64, c_f32, c_f64, c_vp = (ctypes.c_int32, ctypes.c_int64, ctypes.c_uint64, ctypes.c_float,
+                                          ctypes.c_double, ctypes.c_void_p)
+
+
+class PctParams(ctypes.Structure):
+    _fields_ = [
+        ("columns", c_i32), ("rows", c_i32), ("slices", c_i32),
+        ("t_bins", c_i32), ("v_bins", c_i32), ("angular_bins", c_i32),
+        ("recon_cyl_radius", c_f64), ("recon_cyl_height", c_f64), ("slice_thickness", c_f64),
+        ("t_bin_size", c_f64), ("v_bin_size", c_f64), ("angular_bin_size", c_f64),
+        ("ssd_t_size", c_f64), ("ssd_v_size", c_f64), ("source_radius", c_f64),
+        ("gantry_angle_interval", c_f64), ("sigmas_to_keep", c_f64), ("fbp_threshold", c_f64),
+        ("rsp_air", c_f64), ("sc_threshold", c_f64), ("msc_threshold", c_f64),
+        ("hull_filter_threshold", c_f64), ("hull_filter_radius", c_i32), ("msc_diff_thresh", c_i32),
+        ("e_0", c_f64), ("x_0", c_f64), ("a", c_f64 * 6),
+        ("max_intersections", c_i32), ("drop_block_size", c_i32), ("lambda_", c_f32), ("iterations", c_i32),
+    ]
+
+    def copy(self):
+        c = PctParams()
+        ctypes.pointer(c)[0] = self
+        return c
+
+
+class PctRawSoa(ctypes.Structure):
+    _fields_ = [("gantry_angle", c_vp)] + [(n, c_vp) for n in (
+        "t_in_1", "t_in_2", "t_out_1", "t_out_2", "u_in_1", "u_in_2", "u_out_1", "u_out_2",
+        "v_in_1", "v_in_2", "v_out_1", "v_out_2", "wepl")] + [("n", c_i64)]
+
+
+HIST_FLOAT_FIELDS = ("x_entry", "y_entry", "z_entry", "x_exit", "y_exit", "z_exit",
+                     "xy_entry_angle", "xz_entry_angle", "xy_exit_angle", "xz_exit_angle", "wepl")
+RAW_FLOAT_FIELDS = ("t_in_1", "t_in_2", "t_out_1", "t_out_2", "u_in_1", "u_in_2", "u_out_1", "u_out_2",
+                    "v_in_1", "v_in_2", "v_out_1", "v_out_2", "wepl")
+
+
+class PctHistSoa(ctypes.Structure):
+    _fields_ = [(n, c_vp) for n in HIST_FLOAT_FIELDS] + [("bin_num", c_vp), ("first_voxel", c_vp), ("n", c_i64)]
+
+
+SYNTH_MAX_CYL = 16
+
+
+class PctSynthCfg(ctypes.Structure):
+    _fields_ = [
+        ("seed", c_u64), ("n_histories", c_i64), ("n_angles", c_i32), ("angle_step_deg", c_f64),
+        ("u_in_1", c_f64), ("u_in_2", c_f64), ("u_out_1", c_f64), ("u_out_2", c_f64),
+        ("t_half", c_f64), ("v_half", c_f64), ("source_radius", c_f64), ("beta_cp", c_f64),
+        ("x0", c_f64), ("straggle", c_f64), ("n_cyl", c_i32),
+        ("cyl_x", c_f64 * SYNTH_MAX_CYL), ("cyl_y", c_f64 * SYNTH_MAX_CYL),
+        ("cyl_r", c_f64 * SYNTH_MAX_CYL), ("cyl_rsp", c_f64 * SYNTH_MAX_CYL),
+    ]
+
+
+class PctSynthSoa(ctypes.Structure):
+    _fields_ = [("gantry_angle", c_vp)] + [(n, c_vp) for n in RAW_FLOAT_FIELDS]
+
+
+# name -> (restype, argtypes) of every exported entry point of include/pct.h and
+# include/pct_synth_api.h
+P = ctypes.POINTER
+SIGNATURES = {
+    "pct_create": (c_vp, [P(PctParams), c_vp]),
+    "pct_destroy": (None, [c_vp]),
+    "pct_sync": (ctypes.c_int, [c_vp]),
+    "pct_last_error": (ctypes.c_char_p, [c_vp]),
+    "pct_default_params": (None, [P(PctParams)]),
+    "pct_recon_volume_intersections": (ctypes.c_int, [c_vp, P(PctRawSoa), P(PctHistSoa), c_vp]),
+    "pct_hull_init": (ctypes.c_int, [c_vp, c_vp]),
+    "pct_mlp_endpoints": (ctypes.c_int, [c_vp, P(PctHistSoa), c_vp, c_vp]),
+    "pct_compact": (ctypes.c_int, [c_vp, P(PctHistSoa), c_vp, P(PctHistSoa), P(c_i64)]),
+    "pct_mlp_capacity": (ctypes.c_int, [c_vp, P(PctHistSoa), c_i64, c_i64, c_vp, P(c_i64)]),
+    "pct_mlp_paths": (ctypes.c_int, [c_vp, P(PctHistSoa), c_i64, c_i64, c_vp, c_vp, c_vp, ctypes.c_int]),
+    "pct_drop_block_update": (ctypes.c_int, [c_vp, P(PctHistSoa), c_i64, c_i64, c_vp, c_vp, c_vp, c_f32, ctypes.c_int]),
+    "pct_drop_image_update": (ctypes.c_int, [c_vp, c_vp, c_vp, c_vp]),
+    "pct_drop_iteration": (ctypes.c_int, [c_vp, P(PctHistSoa), c_vp, ctypes.c_int, c_f32, ctypes.c_int]),
+    "pct_set_profiling": (ctypes.c_int, [c_vp, ctypes.c_int]),
+    "pct_last_timings": (ctypes.c_int, [c_vp, c_vp]),
+    "pct_drop_stats": (ctypes.c_int, [c_vp, P(PctHistSoa), ctypes.c_int, P(c_i64), P(c_i64)]),
+    "pct_selftest_math": (ctypes.c_int, [c_vp, ctypes.c_int, c_vp, c_vp, c_vp, c_i64]),
+    "pct_synth_generate": (ctypes.c_int, [c_vp, P(PctSynthCfg), c_i64, c_i64, P(PctSynthSoa)]),
+    "pct_synth_generate_host": (ctypes.c_int, [P(PctSynthCfg), c_i64, c_i64, P(PctSynthSoa)]),
+}
+
+PCT_MLP_FP64 = 0
+
+_lib = None
+
+
+def lib():
+    """The loaded libpct.so (raises if it was never built)."""
+    global _lib
+    if _lib is None:
+        if not os.path.exists(LIB_PATH):
+            raise RuntimeError(f"libpct.so not found at {LIB_PATH}: run __graft_entry__.build() / python -m pct_amd.build")
+        L = ctypes.CDLL(LIB_PATH)
+        for name, (res, args) in SIGNATURES.items():
+            f = getattr(L, name)
+            f.restype = res
+            f.argtypes = args
+        _lib = L
+    return _lib
+
+
+def default_params() -> PctParams:
+    p = PctParams()
+    lib().pct_default_params(ctypes.byref(p))
+    return p

@@ -1,0 +1,108 @@
+"""Scan and reconstruction presets.
+
+Each preset gives the reconstruction parameters (the reference's
+pCT_config.h / pCT_data.h set, include/pct.h pct_params) and the synthetic
+scan that feeds them (include/pct_synth_api.h pct_synth_cfg).
+
+  mini : 40x40x8 voxels of 5x5x2.5 mm, 20 000 histories over 90 angles
+         (small enough for the pure-Python checks and committed fixtures)
+  cfg1 : 100x100x16 voxels of 2x2x2.5 mm, 2 M histories, lambda 1e-4
+         (the reference's CPU-runnable case)
+  cfg2 : 200x200x32 voxels of 1x1x2.5 mm (the reference defaults), 20 M
+         histories, lambda 1e-4 (the single-GPU benchmark workload)
+
+Common synthetic geometry: tracker planes at u = -25, -15, +15, +25 cm, entry
+hits uniform in |t| < 11 cm and |v| < 2 cm at u = -15 cm, cone beam from a
+point source at SOURCE_RADIUS, 200 MeV protons (beta c p = 364.86 MeV),
+phantom = 20 cm diameter water cylinder (RSP 1.0) with four 2 cm diameter
+inserts (RSP 0.3, 1.1, 1.3, 1.6) centred 5 cm off axis at 45/135/225/315 deg,
+1 % Gaussian WEPL straggling, 90 angles at 4 deg, seed 0x5EED2026 + config id.
+"""
+import math
+from dataclasses import dataclass
+
+from . import _lib
+
+BETA_CP_200MEV = 364.86   # MeV: T = 200 MeV, m_p = 938.272 MeV -> p = 644.4 MeV/c, beta = 0.5662
+
+INSERT_RSP = (0.3, 1.1, 1.3, 1.6)
+INSERT_ANGLES_DEG = (45.0, 135.0, 225.0, 315.0)
+
+
+@dataclass
+class Preset:
+    name: str
+    columns: int
+    rows: int
+    slices: int
+    slice_thickness: float
+    n_histories: int
+    lam: float = 1e-4
+    block_size: int = 3200
+    iterations: int = 12
+    v_half: float = 2.0
+    config_id: int = 0
+
+
+PRESETS = {
+    "mini": Preset("mini", 40, 40, 8, 0.25, 20_000, v_half=1.0, config_id=9),
+    "cfg1": Preset("cfg1", 100, 100, 16, 0.25, 2_000_000, config_id=1),
+    "cfg2": Preset("cfg2", 200, 200, 32, 0.25, 20_000_000, config_id=2),
+}
+
+SEED_BASE = 0x5EED2026
+
+
+def params(preset: str, **overrides) -> _lib.PctParams:
+    """pct_params for a preset (reference defaults for everything else)."""
+    pr = PRESETS[preset]
+    p = _lib.default_params()
+    p.columns, p.rows, p.slices = pr.columns, pr.rows, pr.slices
+    p.slice_thickness = pr.slice_thickness
+    p.recon_cyl_height = pr.slices * pr.slice_thickness
+    p.lambda_ = pr.lam
+    p.drop_block_size = pr.block_size
+    p.iterations = pr.iterations
+    for k, v in overrides.items():
+        setattr(p, k, v)
+    return p
+
+
+def synth_cfg(preset: str, n_histories: int = None, seed: int = None) -> _lib.PctSynthCfg:
+    pr = PRESETS[preset]
+    c = _lib.PctSynthCfg()
+    c.seed = SEED_BASE + pr.config_id if seed is None else seed
+    c.n_histories = pr.n_histories if n_histories is None else n_histories
+    c.n_angles = 90
+    c.angle_step_deg = 4.0
+    c.u_in_1, c.u_in_2, c.u_out_1, c.u_out_2 = -25.0, -15.0, 15.0, 25.0
+    c.t_half = 11.0
+    c.v_half = pr.v_half
+    c.source_radius = 260.7
+    c.beta_cp = BETA_CP_200MEV
+    c.x0 = 36.08
+    c.straggle = 0.01
+    cyl = [(0.0, 0.0, 10.0, 1.0)]
+    for rsp, ang in zip(INSERT_RSP, INSERT_ANGLES_DEG):
+        a = math.radians(ang)
+        cyl.append((5.0 * math.cos(a), 5.0 * math.sin(a), 1.0, rsp))
+    c.n_cyl = len(cyl)
+    for k, (x, y, r, rsp) in enumerate(cyl):
+        c.cyl_x[k], c.cyl_y[k], c.cyl_r[k], c.cyl_rsp[k] = x, y, r, rsp
+    return c
+
+
+def phantom_rsp(cfg: _lib.PctSynthCfg, p: _lib.PctParams):
+    """Ground-truth RSP at voxel centres (numpy, [slices, rows, columns])."""
+    import numpy as np
+    C, R, S = p.columns, p.rows, p.slices
+    w = 2 * p.recon_cyl_radius / C
+    h = 2 * p.recon_cyl_radius / R
+    xs = -p.recon_cyl_radius + (np.arange(C) + 0.5) * w
+    ys = p.recon_cyl_radius - (np.arange(R) + 0.5) * h
+    X, Y = np.meshgrid(xs, ys)
+    img = np.zeros((R, C))
+    for k in range(cfg.n_cyl):
+        inside = (X - cfg.cyl_x[k]) ** 2 + (Y - cfg.cyl_y[k]) ** 2 < cfg.cyl_r[k] ** 2
+        img[inside] = cfg.cyl_rsp[k]
+    return np.broadcast_to(img, (S, R, C)).copy()

@@ -1,0 +1,95 @@
+/*
+ * pct_internal.h -- internal definitions of libpct (context, derived geometry,
+ * launch helpers). Not part of the public C ABI (include/pct.h).
+ */
+#ifndef PCT_INTERNAL_H
+#define PCT_INTERNAL_H
+
+#include <hip/hip_runtime.h>
+#include <stdint.h>
+#include <string>
+#include <vector>
+#include "../../include/pct.h"
+#include "pct_math.h"
+
+/* Derived constants of the reference's macros (data.h:264-348), computed once
+ * on the host and passed to kernels by value. */
+struct pct_geo {
+    int C, R, S;
+    int64_t V;
+    double voxel_width, voxel_height, voxel_thickness;   /* data.h:268-270 */
+    double x_zero, y_zero, z_zero;                       /* data.h:272-274 */
+    double mlp_u_step;                                   /* data.h:300     */
+    double radius, height;
+    double e_0, x_0;
+    double A0;
+    /* A_i / k folds (data.h:322-348) */
+    double a0_2, a0_3, a1_2, a1_3, a1_4, a1_6, a1_12, a2_3, a2_4, a2_5, a2_12, a2_30;
+    double a3_4, a3_5, a3_6, a3_20, a3_60, a4_5, a4_6, a4_7, a4_30, a4_105;
+    double a5_6, a5_7, a5_8, a5_42, a5_168;
+    int max_intersections;
+    int64_t hull_words;                                  /* ceil(V / 32)   */
+};
+
+pct_geo pct_make_geo(const pct_params& p);
+
+/* Per-depth FP64 table of the u_1-only MLP terms (see pct_mlp.hip). Field f of
+ * step k is at tab[f * K + k]. */
+enum { MLP_TAB_U1 = 0, MLP_TAB_S1I0, MLP_TAB_S1I1, MLP_TAB_S1I3, MLP_TAB_C1, MLP_TAB_C2, MLP_TAB_C3, MLP_TAB_FIELDS };
+
+struct pct_buffer {
+    void* ptr = nullptr;
+    size_t bytes = 0;
+};
+
+struct pct_ctx {
+    pct_params p;
+    pct_geo g;
+    hipStream_t stream = nullptr;
+    bool own_stream = false;
+    int device = 0;
+    std::string err;
+    /* device-side status word written by kernels (overflow flags etc.) */
+    int* d_status = nullptr;
+    /* MLP per-depth table */
+    double* d_mlp_tab = nullptr;
+    int mlp_tab_k = 0;
+    /* workspaces, grown on demand */
+    pct_buffer scan_tmp, steps, offsets, path_store, path_len, chord, xupd, counts;
+    pct_buffer blk_offsets, blk_voxels, blk_lengths, blk_chord;
+    /* phase timing of pct_drop_iteration (HIP events on `stream`) */
+    bool profile = false;
+    hipEvent_t ev[4] = {nullptr, nullptr, nullptr, nullptr};
+    bool ev_recorded = false;
+    /* statistics counters (device, 2 x int64) */
+    unsigned long long* d_stats = nullptr;
+};
+
+/* error helpers */
+int pct_fail(pct_ctx* ctx, int code, const char* fmt, ...);
+#define PCT_HIP_CHECK(ctx, expr)                                                        \
+    do {                                                                                \
+        hipError_t e_ = (expr);                                                         \
+        if (e_ != hipSuccess)                                                           \
+            return pct_fail((ctx), PCT_E_HIP, "%s failed: %s (%s:%d)", #expr,           \
+                            hipGetErrorString(e_), __FILE__, __LINE__);                 \
+    } while (0)
+
+int pct_ensure(pct_ctx* ctx, pct_buffer& b, size_t bytes);
+/* reads and clears the device status word; returns nonzero flags */
+int pct_take_status(pct_ctx* ctx, int* flags);
+
+/* status flags */
+#define PCT_STATUS_OVERFLOW   1   /* path longer than max_intersections       */
+#define PCT_STATUS_CAPACITY   2   /* path longer than its precomputed capacity */
+#define PCT_STATUS_TABLE      4   /* chord longer than the per-depth table     */
+
+/* exclusive prefix sum of n int32 values into n+1 int64 offsets (device) */
+int pct_exclusive_scan_i32(pct_ctx* ctx, const int32_t* in, int64_t n, int64_t* out, int64_t* total_host);
+
+static inline unsigned pct_grid(int64_t n, int threads) {
+    int64_t b = (n + threads - 1) / threads;
+    return (unsigned)(b > 0 ? b : 1);
+}
+
+#endif

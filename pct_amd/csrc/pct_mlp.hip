@@ -1,0 +1,499 @@
+/*
+ * pct_mlp.hip -- (d) MLP endpoints and the exact FP64 most-likely-path walk.
+ *
+ * Endpoints: replaces collect_MLP_endpoints_GPU (.cu:5729-5770) and
+ * find_MLP_endpoints_GPU (.cu:4997-5277): one thread per history, two
+ * straight-line voxel walks on a bit-packed hull (1 bit per voxel instead of
+ * the reference's bool array: 8x less hull traffic, 160 KB at 200x200x32).
+ *
+ * MLP: replaces the path part of find_MLP_path_GPU (.cu:5279-5501). The
+ * reference runs one thread per history through ~2u2/w sequential steps and
+ * appends the step's voxel when it differs from the previous one. The steps
+ * are independent of each other except for that dedup, so here one wavefront
+ * takes one history at a time and its 64 lanes evaluate 64 consecutive steps;
+ * the dedup becomes a ballot + prefix popcount and the path is written
+ * coalesced. Terms that depend only on the depth u_1 (sigma_1 and its
+ * inverse, the three common sigma_2 polynomials) are read from a per-depth
+ * FP64 table built on the host with the very same expressions and the same
+ * accumulated u_1 sequence (u_1 += MLP_U_STEP, .cu:5490), so every step value
+ * is bit-identical to the reference's inline evaluation; only the u_2
+ * dependent terms (one log, the Sigma_2 inverse, the 2x2 solve, the rotation
+ * and the voxel index) are computed per step.
+ */
+#include <algorithm>
+#include "pct_internal.h"
+
+namespace {
+
+/* ------------------------------------------------------------------------- */
+/* voxel helpers (.cu:8508-8510, 8721-8752)                                   */
+/* ------------------------------------------------------------------------- */
+__device__ __forceinline__ int calc_voxel(double zero, double pos, double size) {
+    double q = pct_fabs(pos - zero) / size;
+    return q < 1.0e9 ? (int)q : -1;
+}
+__device__ __forceinline__ double edge_coord(double zero, int voxel_entered, double size, int inc, int step) {
+    int on_edge = (step == inc) ? voxel_entered : voxel_entered + 1;
+    return zero + (inc * on_edge * size);
+}
+__device__ __forceinline__ double dist_remaining(double zero, double pos, int inc, int step, double size, int cur) {
+    return pct_fabs(edge_coord(zero, cur + inc * step, size, inc, step) - pos);
+}
+__device__ __forceinline__ double corr_coord(double m, double x, double x0, double y0) { return m * (x - x0) + y0; }
+__device__ __forceinline__ bool hull_bit(const uint32_t* __restrict__ hull, int64_t v) {
+    return (hull[v >> 5] >> (v & 31)) & 1u;
+}
+
+/* one endpoint walk (.cu:4997-5277); returns hit_hull */
+__device__ bool endpoint_walk(const pct_geo& g, const uint32_t* __restrict__ hull, double x_start, double y_start,
+                              double z_start, double xy_angle, double xz_angle, double& x_obj, double& y_obj,
+                              double& z_obj, int& vx, int& vy, int& vz, bool entering) {
+    const int C = g.C, R = g.R, S = g.S;
+    const int X_INC = 1, Y_INC = -1, Z_INC = -1;
+    double x = x_start, y = y_start, z = z_start;
+    if (!entering) xy_angle += PCT_PI;
+    double sxy, cxy, sxz, cxz;
+    pct_sincos(xy_angle, &sxy, &cxy);
+    pct_sincos(xz_angle, &sxz, &cxz);
+    int xm = (cxy >= 0) - (cxy <= 0);
+    int ym = (sxy >= 0) - (sxy <= 0);
+    int zm = (sxz >= 0) - (sxz <= 0);
+    if (xm < 0) zm *= -1;
+    vx = calc_voxel(g.x_zero, x, g.voxel_width);
+    vy = calc_voxel(g.y_zero, y, g.voxel_height);
+    vz = calc_voxel(g.z_zero, z, g.voxel_thickness);
+    double x_to_go = dist_remaining(g.x_zero, x, X_INC, xm, g.voxel_width, vx);
+    double y_to_go = dist_remaining(g.y_zero, y, Y_INC, ym, g.voxel_height, vy);
+    double z_to_go = dist_remaining(g.z_zero, z, Z_INC, zm, g.voxel_thickness, vz);
+    double txy = sxy / cxy, txz = sxz / cxz;
+    double delta_yx = pct_fabs(txy), delta_zx = pct_fabs(txz), delta_zy = pct_fabs(txz / txy);
+    double dy_dx = txy, dz_dx = txz, dz_dy = txz / txy, dx_dy = 1.0 / txy, dx_dz = 1.0 / txz, dy_dz = txy / txz;
+    bool hit = false;
+    bool outside = (vx >= C) || (vy >= R) || (vz >= S) || (vx < 0) || (vy < 0) || (vz < 0);
+    if (!outside) hit = hull_bit(hull, vx + (int64_t)vy * C + (int64_t)vz * C * R);
+    bool end = outside || hit;
+    int guard = 4 * (C + R + S) + 16;
+    if (zm != 0) {
+        while (!end && guard-- > 0) {
+            double x_ext = delta_zx * x_to_go;
+            double y_ext = delta_zy * y_to_go;
+            if ((z_to_go <= x_ext) && (z_to_go <= y_ext)) {
+                vz -= zm;
+                z = edge_coord(g.z_zero, vz, g.voxel_thickness, Z_INC, zm);
+                x = corr_coord(dx_dz, z, z_start, x_start);
+                y = corr_coord(dy_dz, z, z_start, y_start);
+                x_to_go = dist_remaining(g.x_zero, x, X_INC, xm, g.voxel_width, vx);
+                y_to_go = dist_remaining(g.y_zero, y, Y_INC, ym, g.voxel_height, vy);
+                z_to_go = g.voxel_thickness;
+            } else if (x_ext <= y_ext) {
+                vx += xm;
+                x = edge_coord(g.x_zero, vx, g.voxel_width, X_INC, xm);
+                y = corr_coord(dy_dx, x, x_start, y_start);
+                z = corr_coord(dz_dx, x, x_start, z_start);
+                x_to_go = g.voxel_width;
+                y_to_go = dist_remaining(g.y_zero, y, Y_INC, ym, g.voxel_height, vy);
+                z_to_go = dist_remaining(g.z_zero, z, Z_INC, zm, g.voxel_thickness, vz);
+            } else {
+                vy -= ym;
+                y = edge_coord(g.y_zero, vy, g.voxel_height, Y_INC, ym);
+                x = corr_coord(dx_dy, y, y_start, x_start);
+                z = corr_coord(dz_dy, y, y_start, z_start);
+                x_to_go = dist_remaining(g.x_zero, x, X_INC, xm, g.voxel_width, vx);
+                y_to_go = g.voxel_height;
+                z_to_go = dist_remaining(g.z_zero, z, Z_INC, zm, g.voxel_thickness, vz);
+            }
+            if (x_to_go == 0) { x_to_go = g.voxel_width; vx += xm; }
+            if (y_to_go == 0) { y_to_go = g.voxel_height; vy -= ym; }
+            if (z_to_go == 0) { z_to_go = g.voxel_thickness; vz -= zm; }
+            vz = vz > 0 ? vz : 0;
+            outside = (vx >= C) || (vy >= R) || (vz >= S) || (vx < 0) || (vy < 0) || (vz < 0);
+            if (!outside) hit = hull_bit(hull, vx + (int64_t)vy * C + (int64_t)vz * C * R);
+            end = outside || hit;
+        }
+    } else {
+        while (!end && guard-- > 0) {
+            double y_ext = y_to_go / delta_yx;
+            if (x_to_go <= y_ext) {
+                vx += xm;
+                x = edge_coord(g.x_zero, vx, g.voxel_width, X_INC, xm);
+                y = corr_coord(dy_dx, x, x_start, y_start);
+                x_to_go = g.voxel_width;
+                y_to_go = dist_remaining(g.y_zero, y, Y_INC, ym, g.voxel_height, vy);
+            } else {
+                vy -= ym;
+                y = edge_coord(g.y_zero, vy, g.voxel_height, Z_INC, ym);
+                x = corr_coord(dx_dy, y, y_start, x_start);
+                x_to_go = dist_remaining(g.x_zero, x, X_INC, xm, g.voxel_width, vx);
+                y_to_go = g.voxel_height;
+            }
+            if (x_to_go == 0) { x_to_go = g.voxel_width; vx += xm; }
+            if (y_to_go == 0) { y_to_go = g.voxel_height; vy -= ym; }
+            outside = (vx >= C) || (vy >= R) || (vz >= S) || (vx < 0) || (vy < 0) || (vz < 0);
+            if (!outside) hit = hull_bit(hull, vx + (int64_t)vy * C + (int64_t)vz * C * R);
+            end = outside || hit;
+        }
+    }
+    if (!end) hit = false;
+    if (hit) { x_obj = x; y_obj = y; z_obj = z; }
+    return hit;
+}
+
+__global__ void __launch_bounds__(256)
+k_mlp_endpoints(pct_geo g, pct_hist_soa h, const uint32_t* __restrict__ hull, uint8_t* __restrict__ intersected) {
+    int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= h.n) return;
+    double xi = 0, yi = 0, zi = 0, xo = 0, yo = 0, zo = 0;
+    int vx, vy, vz, vxo, vyo, vzo;
+    bool en = endpoint_walk(g, hull, h.x_entry[i], h.y_entry[i], h.z_entry[i], h.xy_entry_angle[i], h.xz_entry_angle[i],
+                            xi, yi, zi, vx, vy, vz, true);
+    bool ex = endpoint_walk(g, hull, h.x_exit[i], h.y_exit[i], h.z_exit[i], h.xy_exit_angle[i], h.xz_exit_angle[i],
+                            xo, yo, zo, vxo, vyo, vzo, false);
+    uint32_t first = 0;
+    if (en && ex) {
+        first = (uint32_t)(vx + g.C * vy + g.R * g.C * vz);
+        h.x_entry[i] = (float)xi; h.y_entry[i] = (float)yi; h.z_entry[i] = (float)zi;
+        h.x_exit[i] = (float)xo;  h.y_exit[i] = (float)yo;  h.z_exit[i] = (float)zo;
+    }
+    h.first_voxel[i] = first;
+    intersected[i] = (uint8_t)(en && ex);
+}
+
+/* ------------------------------------------------------------------------- */
+/* D3: EffectiveChordLength_GPU (.cu:4974-4995). sinf/cosf receive a float.   */
+/* ------------------------------------------------------------------------- */
+__device__ __forceinline__ double chord_length(const pct_geo& g, double abs_angle_t, double abs_angle_v) {
+    double eff_angle_t = abs_angle_t - ((int)(abs_angle_t / (PCT_PI / 2))) * (PCT_PI / 2);
+    double eff_angle_v = pct_fabs(abs_angle_v);
+    double step_fraction = g.mlp_u_step / g.voxel_width;
+    float s2t = pct_sinf((float)(2 * eff_angle_t));
+    float cs_t = pct_cosf((float)eff_angle_t) + pct_sinf((float)eff_angle_t);
+    double chord_2d = (1 / 3.0) * ((step_fraction * step_fraction * s2t - 6) / (step_fraction * s2t - 2 * cs_t)
+                                   + step_fraction * step_fraction * s2t / (2 * cs_t));
+    double mean_pixel_width = g.voxel_width / cs_t;
+    double height_fraction = g.voxel_thickness / mean_pixel_width;
+    step_fraction = g.mlp_u_step / mean_pixel_width;
+    float s2v = pct_sinf((float)(2 * eff_angle_v));
+    double hcs = height_fraction * pct_cosf((float)eff_angle_v) + pct_sinf((float)eff_angle_v);
+    double chord_3d = (1 / 3.0) * ((step_fraction * step_fraction * s2v - 6 * height_fraction) / (step_fraction * s2v - 2 * hcs)
+                                   + step_fraction * step_fraction * s2v / (2 * hcs));
+    return g.voxel_width * chord_2d * chord_3d;
+}
+
+/* per-history MLP constants (.cu:5306-5374) */
+struct mlp_hist {
+    double ca, sa, u_in, t_in, t_out, dT, v_in, v_out, dV, u_2, pre1, pre2, pre3;
+    double chord;
+    int S;
+};
+
+__device__ __forceinline__ int steps_below(const double* __restrict__ u1tab, int K, double lim) {
+    /* number of leading k with u1tab[k] < lim (u1tab strictly increasing);
+     * 0 for NaN, K when every entry is below */
+    int lo = 0, hi = K;
+    while (lo < hi) {
+        int mid = (lo + hi) >> 1;
+        if (u1tab[mid] < lim) lo = mid + 1; else hi = mid;
+    }
+    return lo;
+}
+
+__device__ __forceinline__ void mlp_setup(const pct_geo& g, const pct_hist_soa& h, int64_t i,
+                                          const double* __restrict__ u1tab, int K, mlp_hist& m) {
+    double x_in = h.x_entry[i], y_in = h.y_entry[i], z_in = h.z_entry[i];
+    double x_out = h.x_exit[i], y_out = h.y_exit[i], z_out = h.z_exit[i];
+    double xy_in = h.xy_entry_angle[i], xz_in = h.xz_entry_angle[i];
+    double xy_out = h.xy_exit_angle[i], xz_out = h.xz_exit_angle[i];
+    m.chord = chord_length(g, (xy_in + xy_out) / 2.0, (xz_in + xz_out) / 2.0);
+    double sa, ca;
+    pct_sincos(xy_in, &sa, &ca);
+    double u_in = (ca * x_in) + (sa * y_in);
+    double u_out = (ca * x_out) + (sa * y_out);
+    double t_in = (ca * y_in) - (sa * x_in);
+    double t_out = (ca * y_out) - (sa * x_out);
+    if (u_in > u_out) {
+        xy_in += PCT_PI;
+        xy_out += PCT_PI;
+        pct_sincos(xy_in, &sa, &ca);
+        u_in = (ca * x_in) + (sa * y_in);
+        u_out = (ca * x_out) + (sa * y_out);
+        t_in = (ca * y_in) - (sa * x_in);
+        t_out = (ca * y_out) - (sa * x_out);
+    }
+    m.ca = ca; m.sa = sa;
+    m.u_in = u_in; m.t_in = t_in; m.t_out = t_out; m.dT = xy_out - xy_in;
+    m.v_in = z_in; m.v_out = z_out; m.dV = xz_out - xz_in;
+    double u_2 = pct_fabs(u_out - u_in);
+    m.u_2 = u_2;
+    m.pre1 = u_2 * u_2 * u_2 * (g.a0_3 + u_2 * (g.a1_12 + u_2 * (g.a2_30 + u_2 * (g.a3_60 + u_2 * (g.a4_105 + u_2 * g.a5_168)))));
+    m.pre2 = u_2 * u_2 * (g.a0_2 + u_2 * (g.a1_6 + u_2 * (g.a2_12 + u_2 * (g.a3_20 + u_2 * (g.a4_30 + u_2 * g.a5_42)))));
+    m.pre3 = u_2 * (g.A0 + u_2 * (g.a1_2 + u_2 * (g.a2_3 + u_2 * (g.a3_4 + u_2 * (g.a4_5 + u_2 * g.a5_6)))));
+    m.S = steps_below(u1tab, K, u_2 - g.mlp_u_step);
+}
+
+/* one MLP step (.cu:5376-5489) at table index k; returns the voxel index or -1
+ * when it falls outside the grid */
+__device__ __forceinline__ int64_t mlp_step(const pct_geo& g, const mlp_hist& m, const double* __restrict__ tab, int K, int k) {
+    double u_1 = tab[MLP_TAB_U1 * K + k];
+    double S1I0 = tab[MLP_TAB_S1I0 * K + k];
+    double S1I1 = tab[MLP_TAB_S1I1 * K + k];
+    double S1I3 = tab[MLP_TAB_S1I3 * K + k];
+    double cst1 = tab[MLP_TAB_C1 * K + k];
+    double cst2 = tab[MLP_TAB_C2 * K + k];
+    double cst3 = tab[MLP_TAB_C3 * K + k];
+    double S1I2 = S1I1;
+    double d = m.u_2 - u_1;                       /* R_1[1] = R_1T[2] */
+    double l2 = g.e_0 * (1 + 0.038 * pct_log((m.u_2 - u_1) / g.x_0));
+    double c2 = l2 * l2 / g.x_0;
+    double sigma_t2 = c2 * (m.pre1 - m.u_2 * m.u_2 * cst1 + 2 * m.u_2 * cst2 - cst3);
+    double sigma_t2_theta2 = c2 * (m.pre2 - m.u_2 * cst1 + cst2);
+    double sigma_theta2 = c2 * (m.pre3 - cst1);
+    double det2 = sigma_t2 * sigma_theta2 - sigma_t2_theta2 * sigma_t2_theta2;
+    double S2I0 = sigma_theta2 / det2;
+    double S2I1 = -sigma_t2_theta2 / det2;
+    double S2I2 = S2I1;
+    double S2I3 = sigma_t2 / det2;
+    /* R_1 = {1, d, 0, 1}, R_1T = {1, 0, d, 1} */
+    double f13_1 = S2I0 * 1.0 + S2I1 * 0.0;
+    double f13_2 = S2I2 * 1.0 + S2I3 * 0.0;
+    double f24_1 = S2I0 * d + S2I1 * 1.0;
+    double f24_2 = S2I2 * d + S2I3 * 1.0;
+    double F0 = S1I0 + 1.0 * f13_1 + 0.0 * f13_2;
+    double F1 = S1I1 + 1.0 * f24_1 + 0.0 * f24_2;
+    double F2 = S1I2 + d * f13_1 + 1.0 * f13_2;
+    double F3 = S1I3 + d * f24_1 + 1.0 * f24_2;
+    double detF = F0 * F3 - F1 * F2;
+    double I0 = F3 / detF;
+    double I1 = -F1 / detF;
+    /* R_0 = {1, u_1, 0, 1}, T_0 = {t_in, 0} */
+    double a1 = 1.0 * m.t_in + u_1 * 0.0;
+    double a2 = 0.0 * m.t_in + 1.0 * 0.0;
+    double a3 = S2I0 * m.t_out + S2I1 * m.dT;
+    double a4 = S2I2 * m.t_out + S2I3 * m.dT;
+    double s0 = S1I0 * a1 + S1I1 * a2 + 1.0 * a3 + 0.0 * a4;
+    double s1 = S1I2 * a1 + S1I3 * a2 + d * a3 + 1.0 * a4;
+    double t_1 = I0 * s0 + I1 * s1;
+    double b1 = 1.0 * m.v_in + u_1 * 0.0;
+    double b2 = 0.0 * m.v_in + 1.0 * 0.0;
+    double b3 = S2I0 * m.v_out + S2I1 * m.dV;
+    double b4 = S2I2 * m.v_out + S2I3 * m.dV;
+    double r0 = S1I0 * b1 + S1I1 * b2 + 1.0 * b3 + 0.0 * b4;
+    double r1 = S1I2 * b1 + S1I3 * b2 + d * b3 + 1.0 * b4;
+    double v_1 = I0 * r0 + I1 * r1;
+    double x_1 = (m.ca * (m.u_in + u_1)) - (m.sa * t_1);
+    double y_1 = (m.sa * (m.u_in + u_1)) + (m.ca * t_1);
+    int vx = calc_voxel(g.x_zero, x_1, g.voxel_width);
+    int vy = calc_voxel(g.y_zero, y_1, g.voxel_height);
+    int vz = calc_voxel(g.z_zero, v_1, g.voxel_thickness);
+    int64_t voxel = (int64_t)vx + (int64_t)vy * g.C + (int64_t)vz * g.C * g.R;
+    if (vx < 0 || vy < 0 || vz < 0 || voxel >= g.V) return -1;
+    return voxel;
+}
+
+__device__ __forceinline__ double bcast_d(double v, int lane) {
+    int lo = __builtin_amdgcn_readlane(__double2loint(v), lane);
+    int hi = __builtin_amdgcn_readlane(__double2hiint(v), lane);
+    return __hiloint2double(hi, lo);
+}
+__device__ __forceinline__ int bcast_i(int v, int lane) { return __builtin_amdgcn_readlane(v, lane); }
+__device__ __forceinline__ int64_t bcast_l(int64_t v, int lane) {
+    uint32_t lo = (uint32_t)__builtin_amdgcn_readlane((int)(uint32_t)v, lane);
+    uint32_t hi = (uint32_t)__builtin_amdgcn_readlane((int)(uint32_t)((uint64_t)v >> 32), lane);
+    return (int64_t)(((uint64_t)hi << 32) | lo);
+}
+
+/* capacity pass: steps[k] = S + 1 (upper bound of the path length) */
+__global__ void __launch_bounds__(256)
+k_mlp_steps(pct_geo g, pct_hist_soa h, int64_t start, int64_t count, const double* __restrict__ tab, int K,
+            int32_t* __restrict__ steps) {
+    int64_t k = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (k >= count) return;
+    mlp_hist m;
+    mlp_setup(g, h, start + k, tab + MLP_TAB_U1 * K, K, m);
+    steps[k] = m.S + 1;
+}
+
+/* MLP paths. Each wavefront owns 64 consecutive histories: lane j computes the
+ * per-history constants of history j, then the wave walks the histories one
+ * by one with the 64 lanes on 64 consecutive steps. */
+__global__ void __launch_bounds__(256)
+k_mlp_paths(pct_geo g, pct_hist_soa h, int64_t start, int64_t count, const double* __restrict__ tab, int K,
+            const int64_t* __restrict__ offsets, uint32_t* __restrict__ voxels, uint32_t* __restrict__ lengths,
+            double* __restrict__ chord_out, int* __restrict__ status) {
+    const int lane = threadIdx.x & 63;
+    const int64_t base = ((int64_t)blockIdx.x * blockDim.x + (threadIdx.x & ~63));
+    if (base >= count) return;
+    const int64_t mine = base + lane;
+    mlp_hist m;
+    uint32_t first = 0;
+    int64_t off = 0, cap = 0;
+    if (mine < count) {
+        mlp_setup(g, h, start + mine, tab + MLP_TAB_U1 * K, K, m);
+        first = h.first_voxel[start + mine];
+        off = offsets[mine];
+        cap = offsets[mine + 1] - off;
+        if (chord_out) chord_out[mine] = m.chord;
+        if (m.S >= K) atomicOr(status, PCT_STATUS_TABLE);
+    } else {
+        m.S = 0;
+    }
+    const int nh = (int)((count - base) < 64 ? (count - base) : 64);
+    const uint64_t lt_mask = (lane == 0) ? 0ull : (~0ull >> (64 - lane));
+    int local_flags = 0;
+    for (int j = 0; j < nh; ++j) {
+        mlp_hist b;
+        b.ca = bcast_d(m.ca, j); b.sa = bcast_d(m.sa, j); b.u_in = bcast_d(m.u_in, j);
+        b.t_in = bcast_d(m.t_in, j); b.t_out = bcast_d(m.t_out, j); b.dT = bcast_d(m.dT, j);
+        b.v_in = bcast_d(m.v_in, j); b.v_out = bcast_d(m.v_out, j); b.dV = bcast_d(m.dV, j);
+        b.u_2 = bcast_d(m.u_2, j); b.pre1 = bcast_d(m.pre1, j); b.pre2 = bcast_d(m.pre2, j);
+        b.pre3 = bcast_d(m.pre3, j);
+        int S = bcast_i(m.S, j);
+        if (S > K) S = K;
+        uint32_t carry = (uint32_t)bcast_i((int)first, j);
+        int64_t hoff = bcast_l(off, j);
+        int64_t hcap = bcast_l(cap, j);
+        if (lane == 0 && hcap > 0) voxels[hoff] = carry;
+        int64_t n = 1;
+        for (int c = 0; c < S; c += 64) {
+            int k = c + lane;
+            int64_t vox = -1;
+            if (k < S) vox = mlp_step(g, b, tab, K, k);
+            bool valid = vox >= 0;
+            uint64_t vmask = __ballot(valid);
+            uint64_t below = vmask & lt_mask;
+            int src = below ? 63 - __clzll(below) : 0;
+            uint32_t pv = (uint32_t)__shfl((int)(uint32_t)vox, src, 64);
+            uint32_t prev = below ? pv : carry;
+            bool is_new = valid && (uint32_t)vox != prev;
+            uint64_t nmask = __ballot(is_new);
+            if (is_new) {
+                int64_t pos = n + __popcll(nmask & lt_mask);
+                if (pos < hcap) voxels[hoff + pos] = (uint32_t)vox;
+            }
+            n += __popcll(nmask);
+            if (vmask) carry = (uint32_t)__shfl((int)(uint32_t)vox, 63 - __clzll(vmask), 64);
+        }
+        if (n > hcap) local_flags |= PCT_STATUS_CAPACITY;
+        if (n > g.max_intersections) local_flags |= PCT_STATUS_OVERFLOW;
+        if (lane == 0) lengths[base + j] = (uint32_t)(n < hcap ? n : hcap);
+    }
+    if (lane == 0 && local_flags) atomicOr(status, local_flags);
+}
+
+}  // namespace
+
+/* ------------------------------------------------------------------------- */
+/* host side                                                                  */
+/* ------------------------------------------------------------------------- */
+
+/* Per-depth table: the reference's u_1-only expressions (.cu:5382-5402) at the
+ * accumulated depths u_1 = w/2, w/2 + w/2, ... (repeated addition as in the
+ * reference loop). Built once per context with the same operation order. */
+static int ensure_mlp_table(pct_ctx* ctx) {
+    if (ctx->d_mlp_tab) return PCT_OK;
+    const pct_geo& g = ctx->g;
+    /* a chord between two points of the image square is at most 2*sqrt(2)*R */
+    int K = (int)(2.0 * 1.4142135623730951 * g.radius / g.mlp_u_step) + 8;
+    std::vector<double> tab((size_t)MLP_TAB_FIELDS * K);
+    double u_0 = 0.0, u_1 = g.mlp_u_step;
+    for (int k = 0; k < K; ++k) {
+        double l1 = g.e_0 * (1 + 0.038 * pct_log((u_1 - u_0) / g.x_0));
+        double c1 = l1 * l1 / g.x_0;
+        double sigma_t1 = c1 * (u_1 * u_1 * u_1 * (g.a0_3 + u_1 * (g.a1_12 + u_1 * (g.a2_30 + u_1 * (g.a3_60 + u_1 * (g.a4_105 + u_1 * g.a5_168))))));
+        double sigma_t1_theta1 = c1 * (u_1 * u_1 * (g.a0_2 + u_1 * (g.a1_6 + u_1 * (g.a2_12 + u_1 * (g.a3_20 + u_1 * (g.a4_30 + u_1 * g.a5_42))))));
+        double sigma_theta1 = c1 * (u_1 * (g.A0 + u_1 * (g.a1_2 + u_1 * (g.a2_3 + u_1 * (g.a3_4 + u_1 * (g.a4_5 + u_1 * g.a5_6))))));
+        double det1 = sigma_t1 * sigma_theta1 - sigma_t1_theta1 * sigma_t1_theta1;
+        tab[MLP_TAB_U1 * K + k] = u_1;
+        tab[MLP_TAB_S1I0 * K + k] = sigma_theta1 / det1;
+        tab[MLP_TAB_S1I1 * K + k] = -sigma_t1_theta1 / det1;
+        tab[MLP_TAB_S1I3 * K + k] = sigma_t1 / det1;
+        tab[MLP_TAB_C1 * K + k] = u_1 * (g.A0 + u_1 * (g.a1_2 + u_1 * (g.a2_3 + u_1 * (g.a3_4 + u_1 * (g.a4_5 + u_1 * g.a5_6)))));
+        tab[MLP_TAB_C2 * K + k] = u_1 * u_1 * (g.a0_2 + u_1 * (g.a1_3 + u_1 * (g.a2_4 + u_1 * (g.a3_5 + u_1 * (g.a4_6 + u_1 * g.a5_7)))));
+        tab[MLP_TAB_C3 * K + k] = u_1 * u_1 * u_1 * (g.a0_3 + u_1 * (g.a1_4 + u_1 * (g.a2_5 + u_1 * (g.a3_6 + u_1 * (g.a4_7 + u_1 * g.a5_8)))));
+        u_1 += g.mlp_u_step;
+    }
+    double* d = nullptr;
+    if (hipMalloc(&d, sizeof(double) * tab.size()) != hipSuccess) return pct_fail(ctx, PCT_E_NOMEM, "MLP table alloc");
+    hipError_t e = hipMemcpy(d, tab.data(), sizeof(double) * tab.size(), hipMemcpyHostToDevice);
+    if (e != hipSuccess) { (void)hipFree(d); return pct_fail(ctx, PCT_E_HIP, "MLP table copy: %s", hipGetErrorString(e)); }
+    ctx->d_mlp_tab = d;
+    ctx->mlp_tab_k = K;
+    return PCT_OK;
+}
+
+static int check_hist(pct_ctx* ctx, const pct_hist_soa* h, int64_t start, int64_t count, const char* who) {
+    if (!ctx || !h) return pct_fail(ctx, PCT_E_INVALID, "%s: null argument", who);
+    if (start < 0 || count < 0 || start + count > h->n)
+        return pct_fail(ctx, PCT_E_INVALID, "%s: range [%lld, %lld) outside [0, %lld)", who, (long long)start,
+                        (long long)(start + count), (long long)h->n);
+    if (!h->x_entry || !h->y_entry || !h->z_entry || !h->x_exit || !h->y_exit || !h->z_exit || !h->xy_entry_angle ||
+        !h->xz_entry_angle || !h->xy_exit_angle || !h->xz_exit_angle || !h->first_voxel)
+        return pct_fail(ctx, PCT_E_INVALID, "%s: missing history field", who);
+    return PCT_OK;
+}
+
+extern "C" int pct_mlp_endpoints(pct_ctx* ctx, const pct_hist_soa* hist, const uint32_t* hull_bits, uint8_t* intersected) {
+    int rc = check_hist(ctx, hist, 0, hist ? hist->n : 0, "pct_mlp_endpoints");
+    if (rc) return rc;
+    if (!hull_bits || !intersected) return pct_fail(ctx, PCT_E_INVALID, "pct_mlp_endpoints: null hull/intersected");
+    if (hist->n == 0) return PCT_OK;
+    hipLaunchKernelGGL(k_mlp_endpoints, dim3(pct_grid(hist->n, 256)), dim3(256), 0, ctx->stream, ctx->g, *hist,
+                       hull_bits, intersected);
+    PCT_HIP_CHECK(ctx, hipGetLastError());
+    return PCT_OK;
+}
+
+/* internal: capacity offsets into `offsets` (count+1), total path slots */
+int pct_mlp_capacity_impl(pct_ctx* ctx, const pct_hist_soa* hist, int64_t start, int64_t count, int64_t* offsets,
+                          int64_t* total) {
+    int rc = ensure_mlp_table(ctx);
+    if (rc) return rc;
+    rc = pct_ensure(ctx, ctx->steps, sizeof(int32_t) * (count > 0 ? count : 1));
+    if (rc) return rc;
+    int32_t* steps = (int32_t*)ctx->steps.ptr;
+    if (count > 0)
+        hipLaunchKernelGGL(k_mlp_steps, dim3(pct_grid(count, 256)), dim3(256), 0, ctx->stream, ctx->g, *hist, start, count,
+                           (const double*)ctx->d_mlp_tab, ctx->mlp_tab_k, steps);
+    PCT_HIP_CHECK(ctx, hipGetLastError());
+    return pct_exclusive_scan_i32(ctx, steps, count, offsets, total);
+}
+
+int pct_mlp_paths_impl(pct_ctx* ctx, const pct_hist_soa* hist, int64_t start, int64_t count, const int64_t* offsets,
+                       uint32_t* voxels, uint32_t* lengths, double* chord) {
+    int rc = ensure_mlp_table(ctx);
+    if (rc) return rc;
+    if (count == 0) return PCT_OK;
+    hipLaunchKernelGGL(k_mlp_paths, dim3(pct_grid(count, 256)), dim3(256), 0, ctx->stream, ctx->g, *hist, start, count,
+                       (const double*)ctx->d_mlp_tab, ctx->mlp_tab_k, offsets, voxels, lengths, chord, ctx->d_status);
+    PCT_HIP_CHECK(ctx, hipGetLastError());
+    return PCT_OK;
+}
+
+int pct_status_to_rc(pct_ctx* ctx, int flags) {
+    if (flags & PCT_STATUS_TABLE) return pct_fail(ctx, PCT_E_OVERFLOW, "MLP chord longer than the per-depth table");
+    if (flags & PCT_STATUS_CAPACITY) return pct_fail(ctx, PCT_E_OVERFLOW, "MLP path longer than its capacity");
+    if (flags & PCT_STATUS_OVERFLOW)
+        return pct_fail(ctx, PCT_E_OVERFLOW, "MLP path longer than max_intersections (%d)", ctx->g.max_intersections);
+    return PCT_OK;
+}
+
+extern "C" int pct_mlp_capacity(pct_ctx* ctx, const pct_hist_soa* hist, int64_t start, int64_t count, int64_t* offsets,
+                                int64_t* total) {
+    int rc = check_hist(ctx, hist, start, count, "pct_mlp_capacity");
+    if (rc) return rc;
+    if (!offsets) return pct_fail(ctx, PCT_E_INVALID, "pct_mlp_capacity: null offsets");
+    return pct_mlp_capacity_impl(ctx, hist, start, count, offsets, total);
+}
+
+extern "C" int pct_mlp_paths(pct_ctx* ctx, const pct_hist_soa* hist, int64_t start, int64_t count, const int64_t* offsets,
+                             uint32_t* voxels, uint32_t* lengths, int mode) {
+    int rc = check_hist(ctx, hist, start, count, "pct_mlp_paths");
+    if (rc) return rc;
+    if (mode != PCT_MLP_FP64) return pct_fail(ctx, PCT_E_INVALID, "pct_mlp_paths: unknown mode %d", mode);
+    if (!offsets || !voxels || !lengths) return pct_fail(ctx, PCT_E_INVALID, "pct_mlp_paths: null output");
+    rc = pct_mlp_paths_impl(ctx, hist, start, count, offsets, voxels, lengths, nullptr);
+    if (rc) return rc;
+    int flags = 0;
+    rc = pct_take_status(ctx, &flags);
+    if (rc) return rc;
+    return pct_status_to_rc(ctx, flags);
+}

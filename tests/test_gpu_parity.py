@@ -1,0 +1,165 @@
+"""GPU parity of the DROP hot path against the CPU oracle (through the C ABI).
+
+Bit-exact: the shared math library (host vs device), the synthetic generator,
+recon-volume intersections, the cylinder hull, MLP endpoints, compaction and
+MLP voxel lists. Floating point: the DROP image (float atomics on the GPU vs
+double accumulation in storage order in the oracle).
+"""
+import numpy as np
+import pytest
+
+pytestmark = pytest.mark.gpu
+
+
+@pytest.fixture(scope="module")
+def ctx_mini(mini_host):
+    import pct_amd
+    return pct_amd.Context(mini_host["p"])
+
+
+def _to_dev(h_host, device="cuda"):
+    import torch
+    import pct_amd
+    d = pct_amd.Histories(h_host.n, device)
+    for f, arr in h_host.arrays().items():
+        t = torch.from_numpy(np.ascontiguousarray(arr).view(np.int32) if arr.dtype == np.uint32 else arr)
+        getattr(d, f).copy_(t.to(device))
+    return d
+
+
+@pytest.mark.parametrize("func", ["log", "sin", "cos", "tan", "atan", "atan2", "sinf", "cosf", "atan2f", "sqrt", "div"])
+def test_math_host_device_bit_equal(ctx_mini, func):
+    import oracle
+    import pct_amd
+    rng = np.random.default_rng(7)
+    n = 200_000
+    if func == "log":
+        a = np.concatenate([rng.uniform(1e-4, 40, n // 2), 10 ** rng.uniform(-300, 300, n // 2)])
+    elif func in ("sqrt",):
+        a = 10 ** rng.uniform(-300, 300, n)
+    elif func in ("atan",):
+        a = np.concatenate([rng.uniform(-3, 3, n // 2), 10 ** rng.uniform(-8, 8, n // 2)])
+    else:
+        a = rng.uniform(-13, 13, n)
+    b = rng.uniform(-13, 13, n) if func in ("atan2", "atan2f", "div") else None
+    host = oracle.math_eval(func, a, b)
+    dev = pct_amd.selftest_math(ctx_mini, oracle.MATH_FUNCS[func], a, b)
+    assert np.array_equal(host.view(np.uint64), dev.view(np.uint64)), \
+        f"{func}: {np.count_nonzero(host.view(np.uint64) != dev.view(np.uint64))} of {n} differ"
+
+
+def test_synth_device_equals_host(ctx_mini, mini_host):
+    raw_dev = ctx_mini.generate(mini_host["cfg"]).numpy()
+    for k, v in mini_host["raw"].items():
+        assert np.array_equal(np.asarray(v).view(np.uint32), raw_dev[k].view(np.uint32)), k
+
+
+def test_recon_volume_intersections_bit_exact(ctx_mini, mini_host):
+    raw = ctx_mini.generate(mini_host["cfg"])
+    hist, missed = ctx_mini.recon_volume_intersections(raw)
+    got = hist.numpy()
+    exp = mini_host["h_all"].arrays()
+    assert np.array_equal(missed.cpu().numpy(), mini_host["missed"])
+    keep = mini_host["missed"] == 0
+    for f in ("x_entry", "y_entry", "z_entry", "x_exit", "y_exit", "z_exit", "xy_entry_angle", "xz_entry_angle",
+              "xy_exit_angle", "xz_exit_angle", "wepl"):
+        a, b = exp[f][keep].view(np.uint32), got[f][keep].view(np.uint32)
+        assert np.array_equal(a, b), f"{f}: {np.count_nonzero(a != b)} differ"
+
+
+def test_hull_init_bit_exact(ctx_mini, mini_host):
+    bits = ctx_mini.hull_init().cpu().numpy().view(np.uint32)
+    V = mini_host["hull"].size
+    unpacked = ((bits[np.arange(V) >> 5] >> (np.arange(V) & 31)) & 1).astype(np.uint8)
+    assert np.array_equal(unpacked, mini_host["hull"])
+
+
+def test_endpoints_and_compaction_bit_exact(ctx_mini, mini_host):
+    d = _to_dev(mini_host["h_pre"])
+    bits = ctx_mini.hull_init()
+    inter = ctx_mini.mlp_endpoints(d, bits).cpu().numpy()
+    assert np.array_equal(inter, mini_host["inter"])
+    comp = ctx_mini.compact(d, ctx_mini.torch.from_numpy(inter).cuda())
+    got = comp.numpy()
+    exp = mini_host["hr"].arrays()
+    assert comp.n == mini_host["hr"].n
+    for f in ("x_entry", "y_entry", "z_entry", "x_exit", "y_exit", "z_exit", "first_voxel", "wepl"):
+        assert np.array_equal(np.asarray(exp[f]).view(np.uint32), np.asarray(got[f]).view(np.uint32)), f
+
+
+def test_mlp_paths_bit_exact(ctx_mini, mini_host):
+    import oracle
+    hr = mini_host["hr"]
+    d = _to_dev(hr)
+    offs, vox, lens = ctx_mini.mlp_paths(d)
+    offs, vox, lens = offs.cpu().numpy(), vox.cpu().numpy().view(np.uint32), lens.cpu().numpy()
+    exp = oracle.mlp_paths(mini_host["p"], hr)
+    bad = 0
+    for k, e in enumerate(exp):
+        g = vox[offs[k]:offs[k] + lens[k]]
+        if len(g) != len(e) or not np.array_equal(g, e):
+            bad += 1
+    assert bad == 0, f"{bad} of {len(exp)} voxel lists differ"
+
+
+def test_mlp_paths_subrange(ctx_mini, mini_host):
+    import oracle
+    hr = mini_host["hr"]
+    d = _to_dev(hr)
+    start, count = 1234, 777
+    offs, vox, lens = ctx_mini.mlp_paths(d, start, count)
+    offs, vox, lens = offs.cpu().numpy(), vox.cpu().numpy().view(np.uint32), lens.cpu().numpy()
+    exp = oracle.mlp_paths(mini_host["p"], hr, start, count)
+    for k, e in enumerate(exp):
+        assert np.array_equal(vox[offs[k]:offs[k] + lens[k]], e), k
+
+
+def _rmse(a, b):
+    return float(np.sqrt(np.mean((a.astype(np.float64) - b.astype(np.float64)) ** 2)))
+
+
+def test_drop_iteration_matches_oracle(ctx_mini, mini_host):
+    import oracle
+    torch = ctx_mini.torch
+    p = mini_host["p"]
+    hr = mini_host["hr"]
+    x0 = mini_host["hull"].astype(np.float32)
+    lam = 1e-2   # large relaxation so that the update is far above float rounding
+    exp1 = oracle.drop(p, hr, x0, iterations=1, lam=lam)
+    exp3 = oracle.drop(p, hr, x0, iterations=3, lam=lam)
+    d = _to_dev(hr)
+    x = torch.from_numpy(x0.copy()).cuda()
+    ctx_mini.drop_iteration(d, x, lam=lam)
+    got1 = x.cpu().numpy()
+    ctx_mini.drop_iteration(d, x, lam=lam)
+    ctx_mini.drop_iteration(d, x, lam=lam)
+    got3 = x.cpu().numpy()
+    step1 = _rmse(exp1, x0)
+    assert step1 > 1e-3
+    # float atomics (any order) vs double accumulation in storage order
+    assert _rmse(got1, exp1) < 1e-4 * max(step1, 1.0)
+    assert np.max(np.abs(got1 - exp1)) < 1e-4
+    assert _rmse(got3, exp3) < 1e-4
+    # same voxels touched
+    assert np.array_equal(got1 != x0, exp1 != x0)
+
+
+def test_drop_block_api_matches_iteration(ctx_mini, mini_host):
+    """pct_drop_block_update + pct_drop_image_update per block == pct_drop_iteration."""
+    torch = ctx_mini.torch
+    hr = mini_host["hr"]
+    d = _to_dev(hr)
+    x0 = mini_host["hull"].astype(np.float32)
+    lam = 1e-2
+    x_it = torch.from_numpy(x0.copy()).cuda()
+    ctx_mini.drop_iteration(d, x_it, lam=lam)
+    x = torch.from_numpy(x0.copy()).cuda()
+    xu = torch.zeros_like(x)
+    cnt = torch.zeros(x.numel(), dtype=torch.int32, device="cuda")
+    B = mini_host["p"].drop_block_size
+    for s in range(0, hr.n, B):
+        ctx_mini.drop_block_update(d, s, min(B, hr.n - s), x, xu, cnt, lam)
+        ctx_mini.drop_image_update(x, xu, cnt)
+    a, b = x.cpu().numpy(), x_it.cpu().numpy()
+    assert np.max(np.abs(a - b)) < 1e-5
+    assert int(cnt.abs().sum()) == 0 and float(xu.abs().sum()) == 0.0
