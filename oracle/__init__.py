@@ -40,6 +40,10 @@ def lib():
                                        ctypes.c_int64, vp]
         L.oracle_drop.argtypes = [P(abi.PctParams), P(abi.PctHistSoa), vp, ctypes.c_int, ctypes.c_float, ctypes.c_int,
                                   ctypes.c_int64]
+        L.oracle_mlp_trace.argtypes = [P(abi.PctParams), P(abi.PctHistSoa), ctypes.c_int64, vp, vp, vp, vp, vp,
+                                       ctypes.c_int, P(ctypes.c_int)]
+        L.oracle_drop_lists.argtypes = [ctypes.c_int64, ctypes.c_int64, vp, vp, vp, vp, vp, ctypes.c_int,
+                                        ctypes.c_float, ctypes.c_int]
         L.oracle_math.argtypes = [ctypes.c_int, vp, vp, vp, ctypes.c_int64]
         L.oracle_math.restype = None
         _L = L
@@ -168,6 +172,27 @@ def mlp_paths(p, h: HostHistories, start=0, count=None, stride=None):
     if rc != 0:
         raise OverflowError("MLP path overflow in the oracle")
     return [vox[k * stride:k * stride + lens[k]].copy() for k in range(count)]
+
+
+def mlp_trace(p, h: HostHistories, i: int, cap=4096):
+    """Per-step (u_1, t_1, v_1, x_1, y_1) of the MLP loop of history i, and the path length."""
+    arrs = [np.zeros(cap) for _ in range(5)]
+    npath = ctypes.c_int(0)
+    hs = h.soa()
+    k = lib().oracle_mlp_trace(ctypes.byref(p), ctypes.byref(hs), i, *[_p(a) for a in arrs], cap, ctypes.byref(npath))
+    u1, t1, v1, x1, y1 = (a[:k] for a in arrs)
+    return dict(u1=u1, t1=t1, v1=v1, x1=x1, y1=y1, n_path=npath.value)
+
+
+def drop_lists(V, lists, chord, b, x0, block_size, lam, iterations=1):
+    """DROP on explicit voxel lists (see oracle_drop_lists)."""
+    offsets = np.zeros(len(lists) + 1, np.int64)
+    offsets[1:] = np.cumsum([len(a) for a in lists])
+    vox = np.concatenate([np.asarray(a, np.uint32) for a in lists]) if lists else np.zeros(1, np.uint32)
+    x = np.ascontiguousarray(x0, np.float32).copy()
+    lib().oracle_drop_lists(V, len(lists), _p(offsets), _p(vox), _p(np.ascontiguousarray(chord, np.float64)),
+                            _p(np.ascontiguousarray(b, np.float32)), _p(x), int(block_size), float(lam), int(iterations))
+    return x
 
 
 def drop(p, h: HostHistories, x0: np.ndarray, iterations=1, block_size=None, lam=None, max_histories=-1):

@@ -251,11 +251,21 @@ double effective_chord_length(const Geo& g, double abs_angle_t, double abs_angle
 /* D2: find_MLP_path (.cu:4439-4633; device twin .cu:5279-5501)              */
 /* Returns the path length, or -1 when it would exceed max_intersections.   */
 /* ------------------------------------------------------------------------ */
+struct MlpTrace {            /* optional per-step record for the tests */
+    double* u1;
+    double* t1;
+    double* v1;
+    double* x1;
+    double* y1;
+    int cap;
+    int n;
+};
+
 int find_MLP_path(const Geo& g, uint32_t* path, uint32_t first_voxel,
                   double x_in_object, double y_in_object, double z_in_object,
                   double x_out_object, double y_out_object, double z_out_object,
                   double xy_entry_angle, double xz_entry_angle, double xy_exit_angle, double xz_exit_angle,
-                  int* skipped) {
+                  int* skipped, MlpTrace* trace = nullptr) {
     const int C = g.C, R = g.R;
     const double STEP = g.mlp_u_step;
     int num_intersections = 0;
@@ -371,6 +381,11 @@ int find_MLP_path(const Geo& g, uint32_t* path, uint32_t first_voxel,
         double x_1 = (ca * (u_in_object + u_1)) - (sa * t_1);
         double y_1 = (sa * (u_in_object + u_1)) + (ca * t_1);
         double z_1 = v_1;
+        if (trace && trace->n < trace->cap) {
+            trace->u1[trace->n] = u_1; trace->t1[trace->n] = t_1; trace->v1[trace->n] = v_1;
+            trace->x1[trace->n] = x_1; trace->y1[trace->n] = y_1;
+            trace->n++;
+        }
 
         int voxel_x = calculate_voxel(g.x_zero, x_1, g.voxel_width);
         int voxel_y = calculate_voxel(g.y_zero, y_1, g.voxel_height);
@@ -531,6 +546,58 @@ int oracle_mlp_paths(const pct_params* p, const pct_hist_soa* h, int64_t start, 
         lengths[k] = (uint32_t)n;
     }
     return rc;
+}
+
+/* D2 with the per-step record (u_1, t_1, v_1, x_1, y_1) of every loop
+ * iteration; returns the number of steps recorded (path length in *n_path). */
+int oracle_mlp_trace(const pct_params* p, const pct_hist_soa* h, int64_t i, double* u1, double* t1, double* v1,
+                     double* x1, double* y1, int cap, int* n_path) {
+    Geo g = make_geo(*p);
+    std::vector<uint32_t> path(g.max_intersections + 1);
+    MlpTrace tr{u1, t1, v1, x1, y1, cap, 0};
+    int n = find_MLP_path(g, path.data(), h->first_voxel[i], h->x_entry[i], h->y_entry[i], h->z_entry[i],
+                          h->x_exit[i], h->y_exit[i], h->z_exit[i], h->xy_entry_angle[i], h->xz_entry_angle[i],
+                          h->xy_exit_angle[i], h->xz_exit_angle[i], nullptr, &tr);
+    if (n_path) *n_path = n;
+    return tr.n;
+}
+
+/* DROP on explicit voxel lists (DROP_blocks .cu:8023-8043 + DROP_update
+ * .cu:8044-8055 with the per-history expression of .cu:5822-5834): list k is
+ * voxels[offsets[k] .. offsets[k+1]), chord[k] its effective chord length,
+ * b[k] its WEPL. Blocks of block_size in list order, `iterations` passes. */
+int oracle_drop_lists(int64_t V, int64_t n_lists, const int64_t* offsets, const uint32_t* voxels, const double* chord,
+                      const float* b, float* x, int block_size, float lambda, int iterations) {
+    std::vector<double> x_update(V, 0.0);
+    std::vector<uint32_t> counts(V, 0);
+    std::vector<uint32_t> touched;
+    for (int it = 0; it < iterations; ++it) {
+        for (int64_t start = 0; start < n_lists; start += block_size) {
+            int64_t end = start + block_size < n_lists ? start + block_size : n_lists;
+            touched.clear();
+            for (int64_t k = start; k < end; ++k) {
+                int64_t n = offsets[k + 1] - offsets[k];
+                const uint32_t* a = voxels + offsets[k];
+                double L = chord[k];
+                double sum = 0.0;
+                for (int64_t j = 0; j < n; ++j) sum += x[a[j]];
+                double a_i_dot_x_k = sum * L;
+                double a_i_dot_a_i = (L * L) * n;
+                double update = L * ((b[k] - a_i_dot_x_k) / a_i_dot_a_i) * lambda;
+                for (int64_t j = 0; j < n; ++j) {
+                    if (counts[a[j]] == 0) touched.push_back(a[j]);
+                    x_update[a[j]] += update;
+                    counts[a[j]] += 1;
+                }
+            }
+            for (uint32_t v : touched) {
+                x[v] = (float)(x[v] + x_update[v] / counts[v]);
+                x_update[v] = 0.0;
+                counts[v] = 0;
+            }
+        }
+    }
+    return 0;
 }
 
 /* E6 + E4 semantics: `iterations` DROP iterations over all histories in

@@ -113,6 +113,8 @@ class Context:
         self.device = torch.device(device)
         if self.device.type != "cuda":
             raise PctError("pct_amd.Context needs a GPU device")
+        if self.device.index is None:
+            self.device = torch.device("cuda", torch.cuda.current_device())
         torch.cuda.set_device(self.device)
         self.params = params.copy()
         self.L = _lib.lib()
