@@ -1,0 +1,148 @@
+"""CPU checks of the C ABI library and the host-side logic (no GPU compute):
+
+  - libpct.so loads and exports every function declared in include/*.h;
+  - the ctypes structs have the C layout (sizes/offsets from a compiled probe);
+  - the product fails loudly when libpct.so is missing;
+  - the shared math library is within a few ulp of numpy/glibc;
+  - the synthetic generator: angle grouping, exact line integrals, misses,
+    determinism and independence of the generated range.
+"""
+import ctypes
+import os
+import re
+import subprocess
+
+import numpy as np
+import pytest
+
+import oracle
+import pct_amd
+from pct_amd import _lib, configs
+
+ROOT = os.path.dirname(os.path.dirname(os.path.abspath(__file__)))
+
+
+def _declared_functions():
+    names = set()
+    for h in ("pct.h", "pct_synth_api.h"):
+        src = open(os.path.join(ROOT, "include", h)).read()
+        src = re.sub(r"/\*.*?\*/", "", src, flags=re.S)
+        for m in re.finditer(r"\b(pct_[a-z0-9_]+)\s*\(", src):
+            names.add(m.group(1))
+    return names
+
+
+def test_library_exports_every_declared_symbol():
+    L = ctypes.CDLL(_lib.LIB_PATH)
+    declared = _declared_functions()
+    assert len(declared) >= 15
+    missing = [n for n in declared if not hasattr(L, n)]
+    assert not missing, missing
+    # and the binding declares a signature for each of them
+    assert declared <= set(_lib.SIGNATURES), declared - set(_lib.SIGNATURES)
+
+
+def test_struct_layout_matches_c(tmp_path):
+    src = tmp_path / "probe.c"
+    src.write_text(f"""
+#include <stddef.h>
+#include "{ROOT}/include/pct.h"
+#include "{ROOT}/include/pct_synth_api.h"
+long v[] = {{ sizeof(pct_params), offsetof(pct_params, a), offsetof(pct_params, max_intersections),
+             offsetof(pct_params, lambda), sizeof(pct_hist_soa), sizeof(pct_raw_soa), sizeof(pct_synth_cfg),
+             offsetof(pct_synth_cfg, cyl_rsp), sizeof(pct_synth_soa) }};
+""")
+    so = tmp_path / "probe.so"
+    subprocess.run(["gcc", "-shared", "-fPIC", "-o", str(so), str(src)], check=True)
+    v = (ctypes.c_long * 9).in_dll(ctypes.CDLL(str(so)), "v")
+    exp = [ctypes.sizeof(_lib.PctParams), _lib.PctParams.a.offset, _lib.PctParams.max_intersections.offset,
+           _lib.PctParams.lambda_.offset, ctypes.sizeof(_lib.PctHistSoa), ctypes.sizeof(_lib.PctRawSoa),
+           ctypes.sizeof(_lib.PctSynthCfg), _lib.PctSynthCfg.cyl_rsp.offset, ctypes.sizeof(_lib.PctSynthSoa)]
+    assert list(v) == exp
+
+
+def test_default_params_are_the_reference_defaults():
+    p = _lib.default_params()
+    assert (p.columns, p.rows, p.slices) == (200, 200, 32)           # data.h:264-266
+    assert (p.t_bins, p.v_bins, p.angular_bins) == (350, 36, 90)     # data.h:238-243
+    assert p.max_intersections == 512 and p.drop_block_size == 3200
+    assert p.lambda_ == pytest.approx(0.001)
+    assert p.a[0] == 7.457 * float(np.float32(1e-6))                 # data.h:306 (float pow)
+    assert p.e_0 == 13.6 and p.x_0 == 36.08
+
+
+def test_product_fails_loudly_without_library(monkeypatch):
+    monkeypatch.setattr(_lib, "LIB_PATH", "/nonexistent/libpct.so")
+    monkeypatch.setattr(_lib, "_lib", None)
+    with pytest.raises(RuntimeError):
+        _lib.lib()
+
+
+def _ulp(a, b):
+    return np.abs(a - b) / np.spacing(np.abs(b))
+
+
+@pytest.mark.parametrize("func,ref,lo,hi,tol", [
+    ("log", np.log, 1e-6, 60.0, 1), ("sin", np.sin, -13, 13, 1), ("cos", np.cos, -13, 13, 1),
+    ("tan", np.tan, -1.5, 1.5, 4), ("atan", np.arctan, -50, 50, 1)])
+def test_math_accuracy(func, ref, lo, hi, tol):
+    rng = np.random.default_rng(1)
+    x = rng.uniform(lo, hi, 100_000)
+    got = oracle.math_eval(func, x)
+    exp = ref(x)
+    keep = np.abs(exp) > 1e-6
+    assert _ulp(got[keep], exp[keep]).max() <= tol
+
+
+def test_math_special_values():
+    assert oracle.math_eval("log", [1.0])[0] == 0.0
+    assert oracle.math_eval("log", [np.e])[0] == pytest.approx(1.0, rel=2e-16)
+    assert oracle.math_eval("atan2", [0.0], [-1.0])[0] == np.pi
+    assert oracle.math_eval("atan2", [1.0], [0.0])[0] == np.pi / 2
+    assert oracle.math_eval("sin", [0.0])[0] == 0.0 and oracle.math_eval("cos", [0.0])[0] == 1.0
+    assert np.isnan(oracle.math_eval("log", [-1.0])[0])
+    x = np.array([1e-310, 5e-324])
+    np.testing.assert_allclose(oracle.math_eval("log", x), np.log(x), rtol=1e-15)
+
+
+def test_synth_angles_and_wepl():
+    cfg = configs.synth_cfg("mini", n_histories=9020)
+    raw = pct_amd.generate_host(cfg).numpy()
+    ga = raw["gantry_angle"]
+    # 9020 = 90 * 100 + 20: the first 20 angles get 101 histories
+    counts = np.bincount(ga // 4, minlength=90)
+    assert counts[:20].tolist() == [101] * 20 and counts[20:].tolist() == [100] * 70
+    assert np.all(np.diff(ga) >= 0)
+    w = raw["wepl"]
+    assert np.all(w >= 0)
+    assert 0.5 < np.mean(w > 0) < 1.0
+    # straight-line WEPL through the water cylinder never exceeds its diameter
+    # by more than the inserts and the noise allow
+    assert w.max() < 20.0 * 1.6 * 1.05
+
+
+def test_synth_line_integral_closed_form():
+    """a central track at gantry 0 crosses 20 cm of water; the straggling is
+    1 %: the mean WEPL of tracks with |t| < 0.2 cm is 20 cm within noise"""
+    cfg = configs.synth_cfg("mini", n_histories=90 * 4000)
+    cfg.n_cyl = 1           # water only
+    raw = pct_amd.generate_host(cfg, 0, 4000).numpy()
+    t = raw["t_in_2"]
+    sel = np.abs(t) < 0.2
+    assert sel.sum() > 20
+    # chord of a line at lateral t through a circle of radius 10: 2 sqrt(100 - t^2),
+    # scaled by the small cone-beam obliquity
+    chord = 2 * np.sqrt(100 - t[sel].astype(np.float64) ** 2)
+    rel = raw["wepl"][sel] / chord - 1
+    assert abs(rel.mean()) < 0.01 and rel.std() < 0.02
+
+
+def test_synth_range_independence():
+    cfg = configs.synth_cfg("mini", n_histories=5000)
+    full = pct_amd.generate_host(cfg).numpy()
+    part = pct_amd.generate_host(cfg, 1234, 777).numpy()
+    for k in full:
+        np.testing.assert_array_equal(full[k][1234:1234 + 777], part[k])
+    again = pct_amd.generate_host(cfg).numpy()
+    for k in full:
+        np.testing.assert_array_equal(full[k], again[k])

@@ -1,0 +1,225 @@
+"""Pins the oracle's MLP (find_MLP_path, .cu:4439-4633 / 5279-5501) and chord
+length (EffectiveChordLength, .cu:4974-4995) against what the mathematics
+fixes, independently of the reference's expression order:
+
+  - an independent numpy implementation of the Schulte most-likely path in
+    matrix form, y1 = (S1^-1 + R1^T S2^-1 R1)^-1 (S1^-1 R0 y0 + R1^T S2^-1 y2),
+    with the scattering matrices from exact polynomial integrals of the
+    1/(beta^2 p^2) fit (numpy.polynomial) and spot checks by scipy quadrature;
+  - a straight track (equal lateral positions, zero angles) stays straight;
+  - mirror symmetry t -> -t;
+  - the chord length closed form at zero angles (= VOXEL_WIDTH).
+"""
+import math
+
+import numpy as np
+import pytest
+from numpy.polynomial import Polynomial as Poly
+
+import oracle
+from pct_amd import configs
+
+
+def _params(preset="mini"):
+    return configs.params(preset)
+
+
+def _sigma_matrices(p, u0, u1, u2):
+    A = Poly(list(p.a))
+    U = Poly([0.0, 1.0])
+
+    def integ(poly, a, b):
+        P = poly.integ()
+        return P(b) - P(a)
+
+    c1 = (p.e_0 * (1 + 0.038 * math.log((u1 - u0) / p.x_0))) ** 2 / p.x_0
+    c2 = (p.e_0 * (1 + 0.038 * math.log((u2 - u1) / p.x_0))) ** 2 / p.x_0
+    S1 = c1 * np.array([[integ((u1 - U) ** 2 * A, u0, u1), integ((u1 - U) * A, u0, u1)],
+                        [integ((u1 - U) * A, u0, u1), integ(A, u0, u1)]])
+    S2 = c2 * np.array([[integ((u2 - U) ** 2 * A, u1, u2), integ((u2 - U) * A, u1, u2)],
+                        [integ((u2 - U) * A, u1, u2), integ(A, u1, u2)]])
+    return S1, S2
+
+
+def mlp_numpy(p, rec):
+    """Independent MLP: returns (u1 list, t1, v1, x1, y1) for one history."""
+    vw = 2 * p.recon_cyl_radius / p.columns
+    step = vw / 2
+    xi, yi, zi = (float(rec[k]) for k in ("x_entry", "y_entry", "z_entry"))
+    xo, yo, zo = (float(rec[k]) for k in ("x_exit", "y_exit", "z_exit"))
+    a_in, a_out = float(rec["xy_entry_angle"]), float(rec["xy_exit_angle"])
+    az_in, az_out = float(rec["xz_entry_angle"]), float(rec["xz_exit_angle"])
+    ca, sa = math.cos(a_in), math.sin(a_in)
+    u_in, u_out = ca * xi + sa * yi, ca * xo + sa * yo
+    if u_in > u_out:
+        a_in += math.pi
+        a_out += math.pi
+        ca, sa = math.cos(a_in), math.sin(a_in)
+        u_in, u_out = ca * xi + sa * yi, ca * xo + sa * yo
+    t_in, t_out = ca * yi - sa * xi, ca * yo - sa * xo
+    u2 = abs(u_out - u_in)
+    res = {k: [] for k in ("u1", "t1", "v1", "x1", "y1")}
+    u1 = step
+    while u1 < u2 - step:
+        S1, S2 = _sigma_matrices(p, 0.0, u1, u2)
+        S1i, S2i = np.linalg.inv(S1), np.linalg.inv(S2)
+        R0 = np.array([[1.0, u1], [0.0, 1.0]])
+        R1 = np.array([[1.0, u2 - u1], [0.0, 1.0]])
+        M = np.linalg.inv(S1i + R1.T @ S2i @ R1)
+        yt = M @ (S1i @ R0 @ np.array([t_in, 0.0]) + R1.T @ S2i @ np.array([t_out, a_out - a_in]))
+        yv = M @ (S1i @ R0 @ np.array([zi, 0.0]) + R1.T @ S2i @ np.array([zo, az_out - az_in]))
+        res["u1"].append(u1)
+        res["t1"].append(yt[0])
+        res["v1"].append(yv[0])
+        res["x1"].append(ca * (u_in + u1) - sa * yt[0])
+        res["y1"].append(sa * (u_in + u1) + ca * yt[0])
+        u1 += step
+    return {k: np.array(v) for k, v in res.items()}
+
+
+def test_sigma_integrals_match_quadrature():
+    from scipy.integrate import quad
+    p = _params()
+    A = list(p.a)
+    f = lambda u: sum(A[k] * u ** k for k in range(6))  # noqa: E731
+    for u1, u2 in [(0.3, 12.0), (5.0, 17.5), (11.9, 12.0)]:
+        S1, S2 = _sigma_matrices(p, 0.0, u1, u2)
+        c1 = (p.e_0 * (1 + 0.038 * math.log(u1 / p.x_0))) ** 2 / p.x_0
+        c2 = (p.e_0 * (1 + 0.038 * math.log((u2 - u1) / p.x_0))) ** 2 / p.x_0
+        q = lambda g, a, b: quad(g, a, b, epsabs=0, epsrel=1e-13)[0]  # noqa: E731
+        assert S1[0, 0] == pytest.approx(c1 * q(lambda u: (u1 - u) ** 2 * f(u), 0, u1), rel=1e-10)
+        assert S1[0, 1] == pytest.approx(c1 * q(lambda u: (u1 - u) * f(u), 0, u1), rel=1e-10)
+        assert S1[1, 1] == pytest.approx(c1 * q(f, 0, u1), rel=1e-10)
+        assert S2[0, 0] == pytest.approx(c2 * q(lambda u: (u2 - u) ** 2 * f(u), u1, u2), rel=1e-10)
+        assert S2[0, 1] == pytest.approx(c2 * q(lambda u: (u2 - u) * f(u), u1, u2), rel=1e-10)
+        assert S2[1, 1] == pytest.approx(c2 * q(f, u1, u2), rel=1e-10)
+
+
+def test_mlp_trace_matches_independent_formula(mini_host):
+    p = mini_host["p"]
+    hr = mini_host["hr"]
+    rng = np.random.default_rng(3)
+    idx = rng.choice(hr.n, 40, replace=False)
+    arrays = hr.arrays()
+    for i in idx:
+        tr = oracle.mlp_trace(p, hr, int(i))
+        ref = mlp_numpy(p, {k: arrays[k][i] for k in arrays})
+        assert len(tr["u1"]) == len(ref["u1"])
+        assert np.array_equal(tr["u1"], ref["u1"])
+        scale = 1.0 + np.abs(ref["t1"])
+        assert np.max(np.abs(tr["t1"] - ref["t1"]) / scale) < 1e-9
+        assert np.max(np.abs(tr["v1"] - ref["v1"]) / (1.0 + np.abs(ref["v1"]))) < 1e-9
+
+
+def _voxels_of(p, x, y, z):
+    vw = 2 * p.recon_cyl_radius / p.columns
+    vh = 2 * p.recon_cyl_radius / p.rows
+    vt = p.slice_thickness
+    vx = np.floor(np.abs(x + p.recon_cyl_radius) / vw).astype(np.int64)
+    vy = np.floor(np.abs(y - p.recon_cyl_radius) / vh).astype(np.int64)
+    vz = np.floor(np.abs(z - p.recon_cyl_height / 2) / vt).astype(np.int64)
+    return vx + vy * p.columns + vz * p.columns * p.rows
+
+
+def test_mlp_voxel_lists_match_independent_formula(mini_host):
+    """Voxel lists of the oracle equal those of the independent formula,
+    except where the independent position lies within 1e-9 cm of a voxel
+    face (rounding decides there)."""
+    p = mini_host["p"]
+    hr = mini_host["hr"]
+    arrays = hr.arrays()
+    vw = 2 * p.recon_cyl_radius / p.columns
+    rng = np.random.default_rng(4)
+    for i in rng.choice(hr.n, 40, replace=False):
+        got, _ = oracle.mlp_path(p, hr, int(i))
+        ref = mlp_numpy(p, {k: arrays[k][i] for k in arrays})
+        vox = _voxels_of(p, ref["x1"], ref["y1"], ref["v1"])
+        exp = [int(hr.first_voxel[i])]
+        for v in vox:
+            if v != exp[-1]:
+                exp.append(int(v))
+        if not np.array_equal(np.array(exp, np.uint32), got):
+            fx = np.abs(((ref["x1"] + p.recon_cyl_radius) / vw) - np.round((ref["x1"] + p.recon_cyl_radius) / vw))
+            fy = np.abs(((ref["y1"] - p.recon_cyl_radius) / vw) - np.round((ref["y1"] - p.recon_cyl_radius) / vw))
+            assert min(fx.min(), fy.min()) * vw < 1e-9, f"history {i}: lists differ away from voxel faces"
+
+
+def _single(p, xi, yi, zi, xo, yo, zo, axy_in=0.0, axy_out=0.0, axz_in=0.0, axz_out=0.0, first=0):
+    h = oracle.HostHistories(1)
+    for k, v in dict(x_entry=xi, y_entry=yi, z_entry=zi, x_exit=xo, y_exit=yo, z_exit=zo,
+                     xy_entry_angle=axy_in, xy_exit_angle=axy_out, xz_entry_angle=axz_in,
+                     xz_exit_angle=axz_out, wepl=10.0).items():
+        getattr(h, k)[0] = v
+    h.first_voxel[0] = first
+    return h
+
+
+def test_straight_track_stays_straight():
+    p = _params()
+    for t in (-3.3, 0.0, 2.55, 7.1):
+        h = _single(p, -8.0, t, 0.3, 8.0, t, 0.3)
+        tr = oracle.mlp_trace(p, h, 0)
+        assert len(tr["t1"]) > 10
+        np.testing.assert_allclose(tr["t1"], np.float32(t), rtol=0, atol=1e-12)
+        np.testing.assert_allclose(tr["v1"], np.float32(0.3), rtol=0, atol=1e-12)
+
+
+def test_mirror_symmetry():
+    p = _params()
+    a = _single(p, -8.0, 1.37, 0.2, 7.5, -0.42, -0.1, 0.031, -0.017, 0.004, -0.012)
+    b = _single(p, -8.0, -1.37, 0.2, 7.5, 0.42, -0.1, -0.031, 0.017, 0.004, -0.012)
+    ta, tb = oracle.mlp_trace(p, a, 0), oracle.mlp_trace(p, b, 0)
+    np.testing.assert_array_equal(ta["u1"], tb["u1"])
+    np.testing.assert_allclose(ta["t1"], -tb["t1"], rtol=0, atol=1e-12)
+    np.testing.assert_allclose(ta["v1"], tb["v1"], rtol=0, atol=1e-12)
+
+
+def test_endpoint_continuity():
+    """t1 approaches the entry value at u1 -> 0 and the exit value at u1 -> u2."""
+    p = _params()
+    h = _single(p, -8.0, 1.0, 0.2, 8.0, 2.0, -0.3, 0.0, 0.05, 0.0, -0.02)
+    tr = oracle.mlp_trace(p, h, 0)
+    step = tr["u1"][0]
+    assert abs(tr["t1"][0] - 1.0) < 0.02 * 1 + 2 * step * 0.1
+    assert abs(tr["t1"][-1] - 2.0) < 0.05
+    assert abs(tr["v1"][0] - np.float32(0.2)) < 0.01 and abs(tr["v1"][-1] - np.float32(-0.3)) < 0.02
+
+
+def test_reversed_track_uses_flipped_frame():
+    """u_in > u_out (.cu:4488-4500): the entry angle is turned by pi; the
+    path is that of the same chord walked in +u."""
+    p = _params()
+    h = _single(p, 8.0, 0.5, 0.0, -8.0, 0.5, 0.0, 0.0, 0.0, 0.0, 0.0)
+    tr = oracle.mlp_trace(p, h, 0)
+    assert len(tr["u1"]) > 10
+    np.testing.assert_allclose(tr["y1"], 0.5, atol=1e-9)
+    assert np.all(np.diff(tr["x1"]) < 0)   # walks from x=+8 towards x=-8
+
+
+def test_chord_length_closed_form():
+    p = _params()
+    vw = 2 * p.recon_cyl_radius / p.columns
+    assert oracle.chord_length(p, 0.0, 0.0, 0.0, 0.0) == vw
+    # a second closed form: the reference expression evaluated with float32
+    # trig at 30 degrees
+    th = math.radians(30.0)
+    s2 = float(np.float32(math.sin(float(np.float32(2 * th)))))
+    cs = float(np.float32(np.float32(math.cos(float(np.float32(th)))) + np.float32(math.sin(float(np.float32(th))))))
+    sf = 0.5
+    c2d = (1 / 3.0) * ((sf * sf * s2 - 6) / (sf * s2 - 2 * cs) + sf * sf * s2 / (2 * cs))
+    mpw = vw / cs
+    hf = p.slice_thickness / mpw
+    sf2 = (vw / 2) / mpw
+    c3d = (1 / 3.0) * ((0 - 6 * hf) / (0 - 2 * (hf * 1.0 + 0.0)) + 0.0)
+    exp = vw * c2d * c3d
+    got = oracle.chord_length(p, np.float32(th), np.float32(th), 0.0, 0.0)
+    assert got == pytest.approx(exp, rel=1e-6)
+    assert sf2 > 0
+
+
+def test_overflow_is_reported():
+    p = _params()
+    p.max_intersections = 8
+    h = _single(p, -8.0, 0.3, 0.1, 8.0, 0.3, 0.1)
+    with pytest.raises(OverflowError):
+        oracle.mlp_path(p, h, 0)
