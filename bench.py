@@ -81,36 +81,46 @@ def main():
     x0_np = ((bits[idx >> 5] >> (idx & 31)) & 1).astype(np.float32)
     x = torch.from_numpy(x0_np).to(dev)
     B = p.drop_block_size
+    t1 = time.time()
+    ctx.drop_prepare(hist, B)
+    ctx.sync()
+    t_plan = time.time() - t1
     sum_n, sum_vb = pct_amd.drop_stats(ctx, hist, B)
     _, total_slots = ctx.mlp_capacity(hist)
     sum_steps = total_slots - n_r
-    log(f"setup {time.time() - t0:.1f}s: N={scfg.n_histories} N_r={n_r} mean n={sum_n / n_r:.1f} "
-        f"sum|V_B|={sum_vb} steps={sum_steps}")
+    log(f"setup {time.time() - t0:.1f}s (plan {t_plan:.2f}s): N={scfg.n_histories} N_r={n_r} "
+        f"mean n={sum_n / n_r:.1f} sum|V_B|={sum_vb} steps={sum_steps}")
 
-    # ---- timed DROP iterations ----------------------------------------------
+    def timed(mode, steps):
+        phases = []
+        torch.cuda.synchronize()
+        ev0 = torch.cuda.Event(enable_timing=True)
+        ev1 = torch.cuda.Event(enable_timing=True)
+        ev0.record(ctx.stream)
+        for _ in range(steps):
+            ctx.drop_iteration(hist, x, B, p.lambda_, mode)
+            phases.append(pct_amd.last_timings(ctx))
+        ev1.record(ctx.stream)
+        torch.cuda.synchronize()
+        return ev0.elapsed_time(ev1), np.array(phases)
+
+    # ---- timed DROP iterations (default: MLP recomputed every iteration) ----
     pct_amd.set_profiling(ctx, True)
     for _ in range(args.warmup):
         ctx.drop_iteration(hist, x, B, p.lambda_)
     ctx.sync()
-    torch.cuda.synchronize()
-    phases = []
     t_start = time.perf_counter()
-    ev0 = torch.cuda.Event(enable_timing=True)
-    ev1 = torch.cuda.Event(enable_timing=True)
-    ev0.record(ctx.stream)
-    for _ in range(args.steps):
-        ctx.drop_iteration(hist, x, B, p.lambda_)
-        phases.append(pct_amd.last_timings(ctx))
-    ev1.record(ctx.stream)
-    torch.cuda.synchronize()
+    gpu_ms, ph = timed(0, args.steps)
     wall = time.perf_counter() - t_start
-    gpu_ms = ev0.elapsed_time(ev1)
     ms_per_step = gpu_ms / args.steps
     value = n_r * args.steps / (gpu_ms / 1e3)
-    ph = np.array(phases)
-    cap_ms, mlp_ms, blk_ms = ph.mean(axis=0)
+    _, mlp_ms, blk_ms = ph.mean(axis=0)
     log(f"timed {args.steps} iterations: {ms_per_step:.2f} ms/iter (wall {wall / args.steps * 1e3:.2f} ms); "
-        f"capacity {cap_ms:.2f} ms, MLP {mlp_ms:.2f} ms, blocks {blk_ms:.2f} ms")
+        f"MLP {mlp_ms:.2f} ms, blocks {blk_ms:.2f} ms")
+    # other forms of the same iteration, for comparison
+    ms_cached, _ = timed(pct_amd.PCT_DROP_CACHED, 1)
+    ms_atomic, _ = timed(pct_amd.PCT_DROP_ATOMIC, 1)
+    log(f"cached-list iteration {ms_cached:.2f} ms, reference-structure (atomic) iteration {ms_atomic:.2f} ms")
 
     # algorithmic work per iteration (DESIGN.md "Measurement")
     bytes_update = 48.0 * n_r + 8.0 * sum_n + 24.0 * sum_vb
@@ -118,7 +128,7 @@ def main():
     mlp_tflops = MLP_FLOPS_PER_STEP * float(sum_steps) / (mlp_ms / 1e3) / 1e12
     roof_update = {"bound": "hbm", "achieved": round(upd_gbs, 2), "peak": HBM_PEAK_GBS, "unit": "GB/s",
                    "frac": round(upd_gbs / HBM_PEAK_GBS, 4), "traffic": None,
-                   "kernel": "k_drop_scatter+k_drop_apply (block loop)", "ms": round(float(blk_ms), 3),
+                   "kernel": "k_block_delta+k_block_apply (block loop, hipGraph)", "ms": round(float(blk_ms), 3),
                    "bytes_alg": bytes_update}
     roof_mlp = {"bound": "mfma", "achieved": round(mlp_tflops, 3), "peak": FP64_PEAK_TFLOPS, "unit": "TFLOP/s",
                 "frac": round(mlp_tflops / FP64_PEAK_TFLOPS, 4), "traffic": None,
@@ -175,8 +185,12 @@ def main():
                    "histories": scfg.n_histories, "reconstruction_histories": n_r,
                    "mean_path_voxels": round(sum_n / n_r, 2), "sum_block_voxels": sum_vb,
                    "drop_blocks": (n_r + B - 1) // B},
-        "phases_ms": {"capacity": round(float(cap_ms), 3), "mlp": round(float(mlp_ms), 3),
-                      "blocks": round(float(blk_ms), 3)},
+        "phases_ms": {"mlp": round(float(mlp_ms), 3), "blocks": round(float(blk_ms), 3),
+                      "plan_setup_once": round(t_plan * 1e3, 1)},
+        "other_forms": {"cached_lists_histories_per_s": round(n_r / (ms_cached / 1e3), 1),
+                        "cached_lists_ms": round(ms_cached, 3),
+                        "atomic_reference_structure_histories_per_s": round(n_r / (ms_atomic / 1e3), 1),
+                        "atomic_reference_structure_ms": round(ms_atomic, 3)},
         "roofline": dominant,
         "roofline_update": roof_update,
         "roofline_mlp": roof_mlp,

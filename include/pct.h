@@ -87,6 +87,13 @@ typedef struct pct_hist_soa {
 
 /* MLP evaluation modes for the path and DROP entry points */
 #define PCT_MLP_FP64      0      /* exact FP64 Schulte MLP (.cu:5279-5501)      */
+#define PCT_MLP_MODE_MASK 0xff
+/* flags of pct_drop_iteration's mode, OR-ed with the MLP mode */
+#define PCT_DROP_CACHED    0x100 /* reuse the plan's voxel lists instead of
+                                    recomputing the MLP in this iteration      */
+#define PCT_DROP_ATOMIC    0x200 /* reference structure: atomic scatter and a
+                                    volume scan per block, no plan             */
+#define PCT_DROP_NO_GRAPH  0x400 /* launch the block loop eagerly (no hipGraph) */
 
 typedef struct pct_ctx pct_ctx;
 
@@ -150,10 +157,18 @@ int pct_drop_block_update(pct_ctx* ctx, const pct_hist_soa* hist, int64_t start,
  * x[v] += x_update[v] / counts[v] where counts[v] > 0, then clears both. */
 int pct_drop_image_update(pct_ctx* ctx, float* x, float* x_update, uint32_t* counts);
 
+/* (e) Per-scan DROP plan: computes every MLP voxel list of `hist` and the
+ * per-block voxel->history transpose used by pct_drop_iteration. Replaces the
+ * set-up part of image_reconstruction_GPU (.cu:6764-6782). Called implicitly
+ * by pct_drop_iteration when the histories or the block size change. */
+int pct_drop_prepare(pct_ctx* ctx, const pct_hist_soa* hist, int block_size, int mode);
+
 /* (e) One whole DROP iteration over all histories in storage order, blocks of
  * `block_size`: replaces the body of the iteration loop of
  * image_reconstruction_GPU (.cu:6795-6857). Equivalent to the sequence of
- * pct_drop_block_update + pct_drop_image_update per block. */
+ * pct_drop_block_update + pct_drop_image_update per block. By default the
+ * MLP lists are recomputed (and checked against the plan) in every iteration,
+ * as the reference does; see the PCT_DROP_* flags. */
 int pct_drop_iteration(pct_ctx* ctx, const pct_hist_soa* hist, float* x,
                        int block_size, float lambda, int mode);
 

@@ -10,7 +10,8 @@ import ctypes
 
 from . import _lib
 from . import configs
-from ._lib import PCT_MLP_FP64, PctHistSoa, PctParams, PctRawSoa, PctSynthCfg, PctSynthSoa
+from ._lib import (PCT_DROP_ATOMIC, PCT_DROP_CACHED, PCT_DROP_NO_GRAPH, PCT_MLP_FP64, PctHistSoa, PctParams,
+                   PctRawSoa, PctSynthCfg, PctSynthSoa)
 
 __all__ = ["Context", "Histories", "RawScan", "configs", "PCT_MLP_FP64", "generate_host"]
 
@@ -215,7 +216,13 @@ class Context:
         self._check(self.L.pct_drop_image_update(self._ctx, _ptr(x), _ptr(x_update), _ptr(counts)),
                     "pct_drop_image_update")
 
+    def drop_prepare(self, hist, block_size=None, mode=PCT_MLP_FP64):
+        block_size = self.params.drop_block_size if block_size is None else block_size
+        hs = hist.soa()
+        self._check(self.L.pct_drop_prepare(self._ctx, ctypes.byref(hs), int(block_size), mode), "pct_drop_prepare")
+
     def drop_iteration(self, hist, x, block_size=None, lam=None, mode=PCT_MLP_FP64):
+        """One DROP iteration (mode: MLP mode | PCT_DROP_* flags)."""
         block_size = self.params.drop_block_size if block_size is None else block_size
         lam = self.params.lambda_ if lam is None else lam
         hs = hist.soa()

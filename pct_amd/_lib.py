@@ -86,6 +86,7 @@ SIGNATURES = {
     "pct_drop_block_update": (ctypes.c_int, [c_vp, P(PctHistSoa), c_i64, c_i64, c_vp, c_vp, c_vp, c_f32, ctypes.c_int]),
     "pct_drop_image_update": (ctypes.c_int, [c_vp, c_vp, c_vp, c_vp]),
     "pct_drop_iteration": (ctypes.c_int, [c_vp, P(PctHistSoa), c_vp, ctypes.c_int, c_f32, ctypes.c_int]),
+    "pct_drop_prepare": (ctypes.c_int, [c_vp, P(PctHistSoa), ctypes.c_int, ctypes.c_int]),
     "pct_set_profiling": (ctypes.c_int, [c_vp, ctypes.c_int]),
     "pct_last_timings": (ctypes.c_int, [c_vp, c_vp]),
     "pct_drop_stats": (ctypes.c_int, [c_vp, P(PctHistSoa), ctypes.c_int, P(c_i64), P(c_i64)]),
@@ -95,6 +96,9 @@ SIGNATURES = {
 }
 
 PCT_MLP_FP64 = 0
+PCT_DROP_CACHED = 0x100
+PCT_DROP_ATOMIC = 0x200
+PCT_DROP_NO_GRAPH = 0x400
 
 _lib = None
 

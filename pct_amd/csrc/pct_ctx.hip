@@ -13,6 +13,7 @@
 #include <stdio.h>
 #include <string.h>
 #include "pct_internal.h"
+#include "pct_plan.h"
 
 /* ------------------------------------------------------------------------- */
 /* parameters                                                                 */
@@ -153,6 +154,7 @@ extern "C" void pct_destroy(pct_ctx* ctx) {
     if (ctx->d_status) (void)hipFree(ctx->d_status);
     if (ctx->d_mlp_tab) (void)hipFree(ctx->d_mlp_tab);
     if (ctx->d_stats) (void)hipFree(ctx->d_stats);
+    pct_plan_destroy(ctx->plan);
     for (hipEvent_t& e : ctx->ev)
         if (e) (void)hipEventDestroy(e);
     if (ctx->own_stream && ctx->stream) (void)hipStreamDestroy(ctx->stream);

@@ -25,7 +25,8 @@
 int pct_mlp_capacity_impl(pct_ctx* ctx, const pct_hist_soa* hist, int64_t start, int64_t count, int64_t* offsets,
                           int64_t* total);
 int pct_mlp_paths_impl(pct_ctx* ctx, const pct_hist_soa* hist, int64_t start, int64_t count, const int64_t* offsets,
-                       uint32_t* voxels, uint32_t* lengths, double* chord);
+                       uint32_t* voxels, uint32_t* lengths, double* chord, uint32_t* hash_out,
+                       const uint32_t* hash_expect);
 int pct_status_to_rc(pct_ctx* ctx, int flags);
 
 namespace {
@@ -209,7 +210,7 @@ extern "C" int pct_drop_block_update(pct_ctx* ctx, const pct_hist_soa* hist, int
     if ((rc = pct_ensure(ctx, ctx->blk_lengths, sizeof(uint32_t) * count))) return rc;
     if ((rc = pct_ensure(ctx, ctx->blk_chord, sizeof(double) * count))) return rc;
     rc = pct_mlp_paths_impl(ctx, hist, start, count, (const int64_t*)ctx->blk_offsets.ptr, (uint32_t*)ctx->blk_voxels.ptr,
-                            (uint32_t*)ctx->blk_lengths.ptr, (double*)ctx->blk_chord.ptr);
+                            (uint32_t*)ctx->blk_lengths.ptr, (double*)ctx->blk_chord.ptr, nullptr, nullptr);
     if (rc) return rc;
     rc = launch_scatter(ctx, (const uint32_t*)ctx->blk_voxels.ptr, (const int64_t*)ctx->blk_offsets.ptr,
                         (const uint32_t*)ctx->blk_lengths.ptr, (const double*)ctx->blk_chord.ptr, hist->wepl + start,
@@ -240,17 +241,17 @@ static int build_path_store(pct_ctx* ctx, const pct_hist_soa* hist) {
     if ((rc = pct_ensure(ctx, ctx->chord, sizeof(double) * n))) return rc;
     if (ctx->profile) PCT_HIP_CHECK(ctx, hipEventRecord(ctx->ev[1], ctx->stream));
     rc = pct_mlp_paths_impl(ctx, hist, 0, n, (const int64_t*)ctx->offsets.ptr, (uint32_t*)ctx->path_store.ptr,
-                            (uint32_t*)ctx->path_len.ptr, (double*)ctx->chord.ptr);
+                            (uint32_t*)ctx->path_len.ptr, (double*)ctx->chord.ptr, nullptr, nullptr);
     if (rc) return rc;
     if (ctx->profile) PCT_HIP_CHECK(ctx, hipEventRecord(ctx->ev[2], ctx->stream));
     return PCT_OK;
 }
 
-extern "C" int pct_drop_iteration(pct_ctx* ctx, const pct_hist_soa* hist, float* x, int block_size, float lambda, int mode) {
+/* the reference's structure on the GPU: all MLP lists, then per block an
+ * atomic scatter (block_update_GPU) and a volume scan (image_update_GPU) */
+int pct_drop_iteration_atomic(pct_ctx* ctx, const pct_hist_soa* hist, float* x, int block_size, float lambda) {
     int rc = check_drop_hist(ctx, hist, "pct_drop_iteration");
     if (rc) return rc;
-    if (mode != PCT_MLP_FP64) return pct_fail(ctx, PCT_E_INVALID, "pct_drop_iteration: unknown mode %d", mode);
-    if (!x || block_size <= 0) return pct_fail(ctx, PCT_E_INVALID, "pct_drop_iteration: bad x or block size");
     const int64_t n = hist->n;
     if (n <= 0) return PCT_OK;
     if ((rc = build_path_store(ctx, hist))) return rc;
@@ -296,39 +297,3 @@ extern "C" int pct_last_timings(pct_ctx* ctx, double* ms3) {
     return PCT_OK;
 }
 
-extern "C" int pct_drop_stats(pct_ctx* ctx, const pct_hist_soa* hist, int block_size, int64_t* sum_path,
-                              int64_t* sum_touched) {
-    int rc = check_drop_hist(ctx, hist, "pct_drop_stats");
-    if (rc) return rc;
-    if (block_size <= 0 || !sum_path || !sum_touched) return pct_fail(ctx, PCT_E_INVALID, "pct_drop_stats: bad argument");
-    const int64_t n = hist->n;
-    *sum_path = 0;
-    *sum_touched = 0;
-    if (n <= 0) return PCT_OK;
-    if (!ctx->d_stats && hipMalloc(&ctx->d_stats, 2 * sizeof(long long)) != hipSuccess)
-        return pct_fail(ctx, PCT_E_NOMEM, "stats alloc");
-    PCT_HIP_CHECK(ctx, hipMemsetAsync(ctx->d_stats, 0, 2 * sizeof(long long), ctx->stream));
-    bool prof = ctx->profile;
-    ctx->profile = false;
-    rc = build_path_store(ctx, hist);
-    ctx->profile = prof;
-    if (rc) return rc;
-    const int64_t* offs = (const int64_t*)ctx->offsets.ptr;
-    const uint32_t* vox = (const uint32_t*)ctx->path_store.ptr;
-    const uint32_t* len = (const uint32_t*)ctx->path_len.ptr;
-    uint32_t* cn = (uint32_t*)ctx->counts.ptr;
-    for (int64_t s = 0; s < n; s += block_size) {
-        int64_t cnt = (n - s) < block_size ? (n - s) : block_size;
-        hipLaunchKernelGGL(k_drop_count, dim3((unsigned)((cnt + 3) / 4)), dim3(256), 0, ctx->stream, vox, offs + s,
-                           len + s, cnt, cn, ctx->d_stats);
-        if ((rc = launch_apply(ctx, nullptr, nullptr, cn, true))) return rc;
-    }
-    long long h[2];
-    PCT_HIP_CHECK(ctx, hipMemcpyAsync(h, ctx->d_stats, sizeof(h), hipMemcpyDeviceToHost, ctx->stream));
-    PCT_HIP_CHECK(ctx, hipStreamSynchronize(ctx->stream));
-    *sum_path = h[0];
-    *sum_touched = h[1];
-    int flags = 0;
-    if ((rc = pct_take_status(ctx, &flags))) return rc;
-    return pct_status_to_rc(ctx, flags);
-}

@@ -37,6 +37,8 @@ pct_geo pct_make_geo(const pct_params& p);
  * step k is at tab[f * K + k]. */
 enum { MLP_TAB_U1 = 0, MLP_TAB_S1I0, MLP_TAB_S1I1, MLP_TAB_S1I3, MLP_TAB_C1, MLP_TAB_C2, MLP_TAB_C3, MLP_TAB_FIELDS };
 
+struct pct_plan;
+
 struct pct_buffer {
     void* ptr = nullptr;
     size_t bytes = 0;
@@ -63,6 +65,8 @@ struct pct_ctx {
     bool ev_recorded = false;
     /* statistics counters (device, 2 x int64) */
     unsigned long long* d_stats = nullptr;
+    /* DROP plan of the last scan (pct_plan.hip) */
+    pct_plan* plan = nullptr;
 };
 
 /* error helpers */
@@ -83,6 +87,7 @@ int pct_take_status(pct_ctx* ctx, int* flags);
 #define PCT_STATUS_OVERFLOW   1   /* path longer than max_intersections       */
 #define PCT_STATUS_CAPACITY   2   /* path longer than its precomputed capacity */
 #define PCT_STATUS_TABLE      4   /* chord longer than the per-depth table     */
+#define PCT_STATUS_PLAN       8   /* recomputed lists differ from the DROP plan */
 
 /* exclusive prefix sum of n int32 values into n+1 int64 offsets (device) */
 int pct_exclusive_scan_i32(pct_ctx* ctx, const int32_t* in, int64_t n, int64_t* out, int64_t* total_host);

@@ -289,6 +289,14 @@ __device__ __forceinline__ int64_t mlp_step(const pct_geo& g, const mlp_hist& m,
     return voxel;
 }
 
+/* position-aware mix of one list element; a list's hash is the sum over its
+ * elements plus a length term, so it can be formed by a wave reduction */
+__device__ __forceinline__ uint32_t pct_list_mix(uint32_t v, uint32_t pos) {
+    uint32_t h = (v ^ (pos * 0x9E3779B9u)) * 0x85EBCA6Bu;
+    h ^= h >> 13;
+    return h * 0xC2B2AE35u;
+}
+
 __device__ __forceinline__ double bcast_d(double v, int lane) {
     int lo = __builtin_amdgcn_readlane(__double2loint(v), lane);
     int hi = __builtin_amdgcn_readlane(__double2hiint(v), lane);
@@ -318,7 +326,8 @@ k_mlp_steps(pct_geo g, pct_hist_soa h, int64_t start, int64_t count, const doubl
 __global__ void __launch_bounds__(256)
 k_mlp_paths(pct_geo g, pct_hist_soa h, int64_t start, int64_t count, const double* __restrict__ tab, int K,
             const int64_t* __restrict__ offsets, uint32_t* __restrict__ voxels, uint32_t* __restrict__ lengths,
-            double* __restrict__ chord_out, int* __restrict__ status) {
+            double* __restrict__ chord_out, int* __restrict__ status, uint32_t* __restrict__ hash_out,
+            const uint32_t* __restrict__ hash_expect) {
     const int lane = threadIdx.x & 63;
     const int64_t base = ((int64_t)blockIdx.x * blockDim.x + (threadIdx.x & ~63));
     if (base >= count) return;
@@ -353,6 +362,7 @@ k_mlp_paths(pct_geo g, pct_hist_soa h, int64_t start, int64_t count, const doubl
         int64_t hcap = bcast_l(cap, j);
         if (lane == 0 && hcap > 0) voxels[hoff] = carry;
         int64_t n = 1;
+        uint32_t hsh = lane == 0 ? pct_list_mix(carry, 0) : 0u;
         for (int c = 0; c < S; c += 64) {
             int k = c + lane;
             int64_t vox = -1;
@@ -368,6 +378,7 @@ k_mlp_paths(pct_geo g, pct_hist_soa h, int64_t start, int64_t count, const doubl
             if (is_new) {
                 int64_t pos = n + __popcll(nmask & lt_mask);
                 if (pos < hcap) voxels[hoff + pos] = (uint32_t)vox;
+                hsh += pct_list_mix((uint32_t)vox, (uint32_t)pos);
             }
             n += __popcll(nmask);
             if (vmask) carry = (uint32_t)__shfl((int)(uint32_t)vox, 63 - __clzll(vmask), 64);
@@ -375,6 +386,15 @@ k_mlp_paths(pct_geo g, pct_hist_soa h, int64_t start, int64_t count, const doubl
         if (n > hcap) local_flags |= PCT_STATUS_CAPACITY;
         if (n > g.max_intersections) local_flags |= PCT_STATUS_OVERFLOW;
         if (lane == 0) lengths[base + j] = (uint32_t)(n < hcap ? n : hcap);
+        if (hash_out || hash_expect) {
+#pragma unroll
+            for (int d = 32; d >= 1; d >>= 1) hsh += (uint32_t)__shfl_xor((int)hsh, d, 64);
+            hsh += (uint32_t)n * 0x27d4eb2fu;
+            if (lane == 0) {
+                if (hash_out) hash_out[base + j] = hsh;
+                if (hash_expect && hash_expect[base + j] != hsh) local_flags |= PCT_STATUS_PLAN;
+            }
+        }
     }
     if (lane == 0 && local_flags) atomicOr(status, local_flags);
 }
@@ -458,17 +478,21 @@ int pct_mlp_capacity_impl(pct_ctx* ctx, const pct_hist_soa* hist, int64_t start,
 }
 
 int pct_mlp_paths_impl(pct_ctx* ctx, const pct_hist_soa* hist, int64_t start, int64_t count, const int64_t* offsets,
-                       uint32_t* voxels, uint32_t* lengths, double* chord) {
+                       uint32_t* voxels, uint32_t* lengths, double* chord, uint32_t* hash_out,
+                       const uint32_t* hash_expect) {
     int rc = ensure_mlp_table(ctx);
     if (rc) return rc;
     if (count == 0) return PCT_OK;
     hipLaunchKernelGGL(k_mlp_paths, dim3(pct_grid(count, 256)), dim3(256), 0, ctx->stream, ctx->g, *hist, start, count,
-                       (const double*)ctx->d_mlp_tab, ctx->mlp_tab_k, offsets, voxels, lengths, chord, ctx->d_status);
+                       (const double*)ctx->d_mlp_tab, ctx->mlp_tab_k, offsets, voxels, lengths, chord, ctx->d_status,
+                       hash_out, hash_expect);
     PCT_HIP_CHECK(ctx, hipGetLastError());
     return PCT_OK;
 }
 
 int pct_status_to_rc(pct_ctx* ctx, int flags) {
+    if (flags & PCT_STATUS_PLAN)
+        return pct_fail(ctx, PCT_E_INVALID, "recomputed MLP voxel lists differ from the DROP plan (inputs changed?)");
     if (flags & PCT_STATUS_TABLE) return pct_fail(ctx, PCT_E_OVERFLOW, "MLP chord longer than the per-depth table");
     if (flags & PCT_STATUS_CAPACITY) return pct_fail(ctx, PCT_E_OVERFLOW, "MLP path longer than its capacity");
     if (flags & PCT_STATUS_OVERFLOW)
@@ -490,7 +514,7 @@ extern "C" int pct_mlp_paths(pct_ctx* ctx, const pct_hist_soa* hist, int64_t sta
     if (rc) return rc;
     if (mode != PCT_MLP_FP64) return pct_fail(ctx, PCT_E_INVALID, "pct_mlp_paths: unknown mode %d", mode);
     if (!offsets || !voxels || !lengths) return pct_fail(ctx, PCT_E_INVALID, "pct_mlp_paths: null output");
-    rc = pct_mlp_paths_impl(ctx, hist, start, count, offsets, voxels, lengths, nullptr);
+    rc = pct_mlp_paths_impl(ctx, hist, start, count, offsets, voxels, lengths, nullptr, nullptr, nullptr);
     if (rc) return rc;
     int flags = 0;
     rc = pct_take_status(ctx, &flags);
