@@ -115,6 +115,34 @@ void        pct_default_params(pct_params* out);
 int pct_recon_volume_intersections(pct_ctx* ctx, const pct_raw_soa* raw,
                                    const pct_hist_soa* out, uint8_t* missed);
 
+/* Per-bin statistics of the (t, v, angle) binning (data.h:522-531 arrays).
+ * Every array has t_bins * v_bins * angular_bins entries, bin
+ * t + angle * T_BINS + v * T_BINS * ANGULAR_BINS. */
+typedef struct pct_bin_stats {
+    int32_t* counts;                                   /* bin_counts          */
+    float *mean_wepl, *mean_rel_ut, *mean_rel_uv;      /* mean_* arrays       */
+    float *stddev_wepl, *stddev_rel_ut, *stddev_rel_uv;/* stddev_* arrays     */
+} pct_bin_stats;
+
+/* (a) Binning: replaces binning_GPU (.cu:2507-2578). Writes hist->bin_num
+ * (0 for a history whose bin is out of range, as the reference's calloc'd
+ * array leaves it) and in_range[i]; orders the non-missed histories by bin
+ * (stably) for the two calls below. */
+int pct_binning(pct_ctx* ctx, const pct_hist_soa* hist, const uint8_t* missed, uint8_t* in_range);
+/* (a) Per-bin count and means of WEPL and relative angles over in-range
+ * non-missed histories, squared deviations over all non-missed ones, standard
+ * deviations: replaces the binning sums + calculate_means_GPU (.cu:2618) +
+ * sum_squared_deviations_GPU (.cu:2671) + calculate_standard_deviations_GPU
+ * (.cu:2718). Sums are formed in storage order (deterministic). */
+int pct_bin_statistics(pct_ctx* ctx, const pct_hist_soa* hist, const uint8_t* missed, const uint8_t* in_range,
+                       const pct_bin_stats* stats);
+/* (a) 3-sigma cuts + sinogram: replaces statistical_cuts_GPU (.cu:2835-2869)
+ * and construct_sinogram_GPU (.cu:2912-2918). failed[i] = 1 for histories
+ * that fail a cut or are missed; stats->counts becomes the survivors' count
+ * and sinogram[bin] their mean WEPL. */
+int pct_statistical_cuts(pct_ctx* ctx, const pct_hist_soa* hist, const uint8_t* missed, const pct_bin_stats* stats,
+                         uint8_t* failed, float* sinogram);
+
 /* (c) Cylinder hull: replaces initialize_hull_GPU (.cu:3316-3326). The hull is
  * bit-packed, one bit per voxel, voxel v at word v>>5, bit v&31;
  * ceil(V/32) words. */

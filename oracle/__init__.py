@@ -44,6 +44,10 @@ def lib():
                                        ctypes.c_int, P(ctypes.c_int)]
         L.oracle_drop_lists.argtypes = [ctypes.c_int64, ctypes.c_int64, vp, vp, vp, vp, vp, ctypes.c_int,
                                         ctypes.c_float, ctypes.c_int]
+        L.oracle_binning.argtypes = [P(abi.PctParams), P(abi.PctHistSoa), vp, vp, vp, vp, vp, vp]
+        L.oracle_bin_statistics.argtypes = [P(abi.PctParams), P(abi.PctHistSoa), vp, vp, vp, vp, vp, vp, vp, vp]
+        L.oracle_statistical_cuts.argtypes = [P(abi.PctParams), P(abi.PctHistSoa), vp, vp, vp, vp, vp, vp, vp, vp,
+                                              vp, vp]
         L.oracle_math.argtypes = [ctypes.c_int, vp, vp, vp, ctypes.c_int64]
         L.oracle_math.restype = None
         _L = L
@@ -132,6 +136,31 @@ def recon_volume_intersections(p, raw: dict):
     hs = h.soa()
     lib().oracle_recon_volume_intersections(ctypes.byref(p), ctypes.byref(s), ctypes.byref(hs), _p(missed))
     return h, missed
+
+
+def bin_stats_and_cuts(p, h: HostHistories, missed: np.ndarray):
+    """A2-A7 on all histories of h (bin_num written into h). Returns dict with
+    in_range, counts (of the statistics), mean_*, stddev_*, failed, sinogram,
+    cut_counts."""
+    nb = p.t_bins * p.v_bins * p.angular_bins
+    missed = np.ascontiguousarray(missed, np.uint8)
+    in_range = np.zeros(h.n, np.uint8)
+    counts = np.zeros(nb, np.int32)
+    mw, mu, mv = (np.zeros(nb, np.float32) for _ in range(3))
+    sw, su, sv = (np.zeros(nb, np.float32) for _ in range(3))
+    hs = h.soa()
+    L = lib()
+    L.oracle_binning(ctypes.byref(p), ctypes.byref(hs), _p(missed), _p(in_range), _p(counts), _p(mw), _p(mu), _p(mv))
+    L.oracle_bin_statistics(ctypes.byref(p), ctypes.byref(hs), _p(missed), _p(counts), _p(mw), _p(mu), _p(mv),
+                            _p(sw), _p(su), _p(sv))
+    stat_counts = counts.copy()
+    failed = np.zeros(h.n, np.uint8)
+    sino = np.zeros(nb, np.float32)
+    L.oracle_statistical_cuts(ctypes.byref(p), ctypes.byref(hs), _p(missed), _p(counts), _p(mw), _p(mu), _p(mv),
+                              _p(sw), _p(su), _p(sv), _p(failed), _p(sino))
+    return dict(in_range=in_range, counts=stat_counts, mean_wepl=mw, mean_rel_ut=mu, mean_rel_uv=mv,
+                stddev_wepl=sw, stddev_rel_ut=su, stddev_rel_uv=sv, failed=failed, sinogram=sino,
+                cut_counts=counts)
 
 
 def hull_init(p):

@@ -163,6 +163,49 @@ class Context:
                     "pct_recon_volume_intersections")
         return hist, missed
 
+    # -- (a) binning, statistics, cuts (.cu:2388-2918) -----------------------------
+    @property
+    def n_bins(self):
+        p = self.params
+        return p.t_bins * p.v_bins * p.angular_bins
+
+    def new_bin_stats(self):
+        torch = self.torch
+        st = {"counts": torch.zeros(self.n_bins, dtype=torch.int32, device=self.device)}
+        for f in _lib.BIN_STAT_FIELDS[1:]:
+            st[f] = torch.zeros(self.n_bins, dtype=torch.float32, device=self.device)
+        return st
+
+    @staticmethod
+    def _bin_soa(st):
+        s = _lib.PctBinStats()
+        for f in _lib.BIN_STAT_FIELDS:
+            setattr(s, f, st[f].data_ptr())
+        return s
+
+    def binning(self, hist, missed):
+        """binning_GPU: writes hist.bin_num, returns in_range (uint8)."""
+        in_range = self.torch.empty(hist.n, dtype=self.torch.uint8, device=self.device)
+        hs = hist.soa()
+        self._check(self.L.pct_binning(self._ctx, ctypes.byref(hs), _ptr(missed), _ptr(in_range)), "pct_binning")
+        return in_range
+
+    def bin_statistics(self, hist, missed, in_range):
+        st = self.new_bin_stats()
+        hs, ss = hist.soa(), self._bin_soa(st)
+        self._check(self.L.pct_bin_statistics(self._ctx, ctypes.byref(hs), _ptr(missed), _ptr(in_range),
+                                              ctypes.byref(ss)), "pct_bin_statistics")
+        return st
+
+    def statistical_cuts(self, hist, missed, st):
+        """3-sigma cuts + sinogram; returns (failed uint8, sinogram f32). st['counts'] becomes the survivors' count."""
+        failed = self.torch.empty(hist.n, dtype=self.torch.uint8, device=self.device)
+        sino = self.torch.zeros(self.n_bins, dtype=self.torch.float32, device=self.device)
+        hs, ss = hist.soa(), self._bin_soa(st)
+        self._check(self.L.pct_statistical_cuts(self._ctx, ctypes.byref(hs), _ptr(missed), ctypes.byref(ss),
+                                                _ptr(failed), _ptr(sino)), "pct_statistical_cuts")
+        return failed, sino
+
     # -- (c) hull ---------------------------------------------------------------
     def hull_init(self):
         words = (self.V + 31) // 32

@@ -50,6 +50,14 @@ class PctHistSoa(ctypes.Structure):
     _fields_ = [(n, c_vp) for n in HIST_FLOAT_FIELDS] + [("bin_num", c_vp), ("first_voxel", c_vp), ("n", c_i64)]
 
 
+class PctBinStats(ctypes.Structure):
+    _fields_ = [(n, c_vp) for n in ("counts", "mean_wepl", "mean_rel_ut", "mean_rel_uv", "stddev_wepl",
+                                    "stddev_rel_ut", "stddev_rel_uv")]
+
+
+BIN_STAT_FIELDS = ("counts", "mean_wepl", "mean_rel_ut", "mean_rel_uv", "stddev_wepl", "stddev_rel_ut",
+                   "stddev_rel_uv")
+
 SYNTH_MAX_CYL = 16
 
 
@@ -79,6 +87,9 @@ SIGNATURES = {
     "pct_default_params": (None, [P(PctParams)]),
     "pct_recon_volume_intersections": (ctypes.c_int, [c_vp, P(PctRawSoa), P(PctHistSoa), c_vp]),
     "pct_hull_init": (ctypes.c_int, [c_vp, c_vp]),
+    "pct_binning": (ctypes.c_int, [c_vp, P(PctHistSoa), c_vp, c_vp]),
+    "pct_bin_statistics": (ctypes.c_int, [c_vp, P(PctHistSoa), c_vp, c_vp, P(PctBinStats)]),
+    "pct_statistical_cuts": (ctypes.c_int, [c_vp, P(PctHistSoa), c_vp, P(PctBinStats), c_vp, c_vp]),
     "pct_mlp_endpoints": (ctypes.c_int, [c_vp, P(PctHistSoa), c_vp, c_vp]),
     "pct_compact": (ctypes.c_int, [c_vp, P(PctHistSoa), c_vp, P(PctHistSoa), P(c_i64)]),
     "pct_mlp_capacity": (ctypes.c_int, [c_vp, P(PctHistSoa), c_i64, c_i64, c_vp, P(c_i64)]),

@@ -1,0 +1,292 @@
+/*
+ * pct_bins.hip -- (a) binning, per-bin statistics, 3-sigma cuts, sinogram.
+ *
+ * Replaces binning_GPU (.cu:2507-2578), calculate_means_GPU (.cu:2618),
+ * sum_squared_deviations_GPU (.cu:2671), calculate_standard_deviations_GPU
+ * (.cu:2718), statistical_cuts_GPU (.cu:2835) and construct_sinogram_GPU
+ * (.cu:2912). The reference forms every per-bin sum with float atomics, so
+ * its statistics -- and therefore which histories the cuts keep -- depend on
+ * thread scheduling. Here the sums are formed in storage order: the histories
+ * are stably sorted by bin once (rocPRIM radix sort on the bin number), and one
+ * thread per bin walks its members in storage order, accumulating in float as
+ * the reference's arrays do. The result is deterministic and bit-identical to a
+ * sequential pass; the decision paths (means, deviations, cut masks) are exact.
+ *
+ * Reference quirks kept: an out-of-range history keeps bin 0 and is kept
+ * (it enters bin 0's squared deviations and cuts, not its mean); a bin with
+ * at most one member keeps its raw sum of squares as its deviation; angle cuts
+ * are strict, the WEPL cut inclusive.
+ */
+#include <rocprim/device/device_radix_sort.hpp>
+#include "pct_internal.h"
+
+namespace {
+
+struct bin_geo {
+    int T, V, A;
+    double t_size, v_size, a_size;
+    int64_t bins;
+};
+
+__device__ __forceinline__ double wrap_pi(double a) {
+    if (a > PCT_PI) a -= 2 * PCT_PI;
+    if (a < -PCT_PI) a += 2 * PCT_PI;
+    return a;
+}
+
+/* bin number of every history; sort key = bin for kept (not missed)
+ * histories, `bins` (sorts last) for missed ones */
+__global__ void __launch_bounds__(256)
+k_binning(bin_geo b, pct_hist_soa h, const uint8_t* __restrict__ missed, uint8_t* __restrict__ in_range,
+          uint32_t* __restrict__ keys, uint32_t* __restrict__ vals) {
+    int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= h.n) return;
+    const double RADIANS_TO_ANGLE = 180.0 / PCT_PI, ANGLE_TO_RADIANS = PCT_PI / 180.0;
+    float xe = h.x_entry[i], ye = h.y_entry[i], ze = h.z_entry[i];
+    float xx = h.x_exit[i], yx = h.y_exit[i], zx = h.z_exit[i];
+    double x_mid = (xe + xx) / 2;
+    double y_mid = (ye + yx) / 2;
+    double z_mid = (ze + zx) / 2;
+    double path_angle = pct_atan2f(yx - ye, xx - xe);
+    if (path_angle < 0) path_angle += 2 * PCT_PI;
+    int angle_bin = (int)((path_angle * RADIANS_TO_ANGLE / b.a_size) + 0.5) % b.A;
+    double angle = angle_bin * b.a_size * ANGLE_TO_RADIANS;
+    double s, c;
+    pct_sincos(angle, &s, &c);
+    double t = y_mid * c - x_mid * s;
+    int t_bin = (int)((t / b.t_size) + b.T / 2);
+    int v_bin = (int)((z_mid / b.v_size) + b.V / 2);
+    bool ok = (t_bin >= 0) && (v_bin >= 0) && (t_bin < b.T) && (v_bin < b.V);
+    int32_t bin = ok ? t_bin + angle_bin * b.T + v_bin * b.T * b.A : 0;
+    h.bin_num[i] = bin;
+    in_range[i] = ok;
+    keys[i] = missed[i] ? (uint32_t)b.bins : (uint32_t)bin;
+    vals[i] = (uint32_t)i;
+}
+
+/* [first, last) member range of every bin in the sorted order */
+__global__ void k_bin_ranges(const uint32_t* __restrict__ keys, int64_t n, int64_t bins, uint32_t* __restrict__ first,
+                             uint32_t* __restrict__ last) {
+    int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= n) return;
+    uint32_t k = keys[i];
+    if (k >= bins) return;
+    if (i == 0 || keys[i - 1] != k) first[k] = (uint32_t)i;
+    if (i == n - 1 || keys[i + 1] != k) last[k] = (uint32_t)(i + 1);
+}
+
+__global__ void k_fill_u32(uint32_t* __restrict__ a, int64_t n, uint32_t v) {
+    int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (i < n) a[i] = v;
+}
+
+/* A2 sums + A3 means + A4 squared deviations + A5 deviations, one thread per bin */
+__global__ void __launch_bounds__(256)
+k_bin_stats(int64_t bins, const uint32_t* __restrict__ first, const uint32_t* __restrict__ last,
+            const uint32_t* __restrict__ order, pct_hist_soa h, const uint8_t* __restrict__ in_range,
+            pct_bin_stats st) {
+    int64_t bin = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (bin >= bins) return;
+    uint32_t a = first[bin], e = last[bin];
+    int32_t count = 0;
+    float sw = 0.0f, su = 0.0f, sv = 0.0f;
+    for (uint32_t k = a; k < e; ++k) {
+        uint32_t i = order[k];
+        if (!in_range[i]) continue;
+        double rel_ut = wrap_pi(h.xy_exit_angle[i] - h.xy_entry_angle[i]);
+        double rel_uv = wrap_pi(h.xz_exit_angle[i] - h.xz_entry_angle[i]);
+        count += 1;
+        sw += h.wepl[i];
+        su += (float)rel_ut;
+        sv += (float)rel_uv;
+    }
+    if (count > 0) {
+        sw /= count;
+        su /= count;
+        sv /= count;
+    }
+    float qw = 0.0f, qu = 0.0f, qv = 0.0f;
+    for (uint32_t k = a; k < e; ++k) {
+        uint32_t i = order[k];
+        double rel_ut = wrap_pi(h.xy_exit_angle[i] - h.xy_entry_angle[i]);
+        double rel_uv = wrap_pi(h.xz_exit_angle[i] - h.xz_entry_angle[i]);
+        double dw = h.wepl[i] - sw;
+        double du = rel_ut - su;
+        double dv = rel_uv - sv;
+        qw += (float)(dw * dw);
+        qu += (float)(du * du);
+        qv += (float)(dv * dv);
+    }
+    if (count > 1) {
+        qw = pct_sqrtf(qw / (count - 1));
+        qu = pct_sqrtf(qu / (count - 1));
+        qv = pct_sqrtf(qv / (count - 1));
+    }
+    st.counts[bin] = count;
+    st.mean_wepl[bin] = sw; st.mean_rel_ut[bin] = su; st.mean_rel_uv[bin] = sv;
+    st.stddev_wepl[bin] = qw; st.stddev_rel_ut[bin] = qu; st.stddev_rel_uv[bin] = qv;
+}
+
+__global__ void k_copy_u8(const uint8_t* __restrict__ a, uint8_t* __restrict__ b, int64_t n) {
+    int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (i < n) b[i] = a[i];
+}
+
+/* A6 + A7, one thread per bin */
+__global__ void __launch_bounds__(256)
+k_bin_cuts(int64_t bins, const uint32_t* __restrict__ first, const uint32_t* __restrict__ last,
+           const uint32_t* __restrict__ order, pct_hist_soa h, pct_bin_stats st, float sig,
+           uint8_t* __restrict__ failed, float* __restrict__ sinogram) {
+    int64_t bin = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (bin >= bins) return;
+    uint32_t a = first[bin], e = last[bin];
+    float mw = st.mean_wepl[bin], mu = st.mean_rel_ut[bin], mv = st.mean_rel_uv[bin];
+    float cw = sig * st.stddev_wepl[bin], cu = sig * st.stddev_rel_ut[bin], cv = sig * st.stddev_rel_uv[bin];
+    int32_t count = 0;
+    float sum = 0.0f;
+    for (uint32_t k = a; k < e; ++k) {
+        uint32_t i = order[k];
+        double rel_ut = wrap_pi(h.xy_exit_angle[i] - h.xy_entry_angle[i]);
+        double rel_uv = wrap_pi(h.xz_exit_angle[i] - h.xz_entry_angle[i]);
+        float w = h.wepl[i];
+        bool pu = pct_fabs(rel_ut - mu) < (double)cu;
+        bool pv = pct_fabs(rel_uv - mv) < (double)cv;
+        bool pw = pct_fabsf(mw - w) <= cw;
+        bool f = !pu || !pv || !pw;
+        failed[i] = f;
+        if (!f) {
+            count += 1;
+            sum += w;
+        }
+    }
+    if (count > 0) sum /= count;
+    st.counts[bin] = count;
+    sinogram[bin] = sum;
+}
+
+}  // namespace
+
+/* sorted order of the kept histories by bin, cached per history array */
+struct pct_bin_order {
+    pct_hist_soa key{};
+    int64_t n = 0;
+    pct_buffer keys_in, keys_out, vals_in, vals_out, first, last, tmp;
+};
+
+static bin_geo make_bin_geo(const pct_params& p) {
+    bin_geo b;
+    b.T = p.t_bins; b.V = p.v_bins; b.A = p.angular_bins;
+    b.t_size = p.t_bin_size; b.v_size = p.v_bin_size; b.a_size = p.angular_bin_size;
+    b.bins = (int64_t)b.T * b.V * b.A;
+    return b;
+}
+
+static int bins_bits(int64_t bins) {
+    int b = 1;
+    while (((int64_t)1 << b) <= bins) ++b;
+    return b;
+}
+
+static int check_bin_args(pct_ctx* ctx, const pct_hist_soa* h, const uint8_t* missed, const char* who) {
+    if (!ctx || !h || !missed) return pct_fail(ctx, PCT_E_INVALID, "%s: null argument", who);
+    if (!h->bin_num || !h->wepl || !h->x_entry || !h->xy_entry_angle || !h->xz_entry_angle)
+        return pct_fail(ctx, PCT_E_INVALID, "%s: missing history field", who);
+    if (h->n < 0 || h->n >= ((int64_t)1 << 32)) return pct_fail(ctx, PCT_E_INVALID, "%s: bad history count", who);
+    return PCT_OK;
+}
+
+static pct_bin_order* order_of(pct_ctx* ctx) {
+    if (!ctx->bin_order) ctx->bin_order = new pct_bin_order();
+    return ctx->bin_order;
+}
+
+void pct_bin_order_destroy(pct_bin_order* o) {
+    if (!o) return;
+    for (pct_buffer* b : {&o->keys_in, &o->keys_out, &o->vals_in, &o->vals_out, &o->first, &o->last, &o->tmp})
+        if (b->ptr) (void)hipFree(b->ptr);
+    delete o;
+}
+
+/* binning + stable sort by bin */
+static int bin_and_sort(pct_ctx* ctx, const pct_hist_soa* h, const uint8_t* missed, uint8_t* in_range) {
+    bin_geo b = make_bin_geo(ctx->p);
+    pct_bin_order* o = order_of(ctx);
+    const int64_t n = h->n;
+    int rc;
+    size_t nn = (size_t)(n > 0 ? n : 1);
+    if ((rc = pct_ensure(ctx, o->keys_in, 4 * nn)) || (rc = pct_ensure(ctx, o->keys_out, 4 * nn)) ||
+        (rc = pct_ensure(ctx, o->vals_in, 4 * nn)) || (rc = pct_ensure(ctx, o->vals_out, 4 * nn)) ||
+        (rc = pct_ensure(ctx, o->first, 4 * (size_t)b.bins)) || (rc = pct_ensure(ctx, o->last, 4 * (size_t)b.bins)))
+        return rc;
+    if (n > 0)
+        hipLaunchKernelGGL(k_binning, dim3(pct_grid(n, 256)), dim3(256), 0, ctx->stream, b, *h, missed, in_range,
+                           (uint32_t*)o->keys_in.ptr, (uint32_t*)o->vals_in.ptr);
+    size_t tb = 0;
+    int endbit = bins_bits(b.bins);
+    PCT_HIP_CHECK(ctx, rocprim::radix_sort_pairs(nullptr, tb, (uint32_t*)nullptr, (uint32_t*)nullptr, (uint32_t*)nullptr,
+                                                 (uint32_t*)nullptr, nn, 0, endbit, ctx->stream));
+    if ((rc = pct_ensure(ctx, o->tmp, tb))) return rc;
+    tb = o->tmp.bytes;
+    if (n > 0)
+        PCT_HIP_CHECK(ctx, rocprim::radix_sort_pairs(o->tmp.ptr, tb, (uint32_t*)o->keys_in.ptr, (uint32_t*)o->keys_out.ptr,
+                                                     (uint32_t*)o->vals_in.ptr, (uint32_t*)o->vals_out.ptr, (size_t)n, 0,
+                                                     endbit, ctx->stream));
+    hipLaunchKernelGGL(k_fill_u32, dim3(pct_grid(b.bins, 256)), dim3(256), 0, ctx->stream, (uint32_t*)o->first.ptr,
+                       b.bins, 0u);
+    hipLaunchKernelGGL(k_fill_u32, dim3(pct_grid(b.bins, 256)), dim3(256), 0, ctx->stream, (uint32_t*)o->last.ptr,
+                       b.bins, 0u);
+    if (n > 0)
+        hipLaunchKernelGGL(k_bin_ranges, dim3(pct_grid(n, 256)), dim3(256), 0, ctx->stream, (const uint32_t*)o->keys_out.ptr,
+                           n, b.bins, (uint32_t*)o->first.ptr, (uint32_t*)o->last.ptr);
+    PCT_HIP_CHECK(ctx, hipGetLastError());
+    o->key = *h;
+    o->n = n;
+    return PCT_OK;
+}
+
+extern "C" int pct_binning(pct_ctx* ctx, const pct_hist_soa* hist, const uint8_t* missed, uint8_t* in_range) {
+    int rc = check_bin_args(ctx, hist, missed, "pct_binning");
+    if (rc) return rc;
+    if (!in_range) return pct_fail(ctx, PCT_E_INVALID, "pct_binning: null in_range");
+    return bin_and_sort(ctx, hist, missed, in_range);
+}
+
+static bool same_arrays(const pct_hist_soa& a, const pct_hist_soa& b) {
+    return a.n == b.n && a.bin_num == b.bin_num && a.wepl == b.wepl && a.x_entry == b.x_entry;
+}
+
+extern "C" int pct_bin_statistics(pct_ctx* ctx, const pct_hist_soa* hist, const uint8_t* missed, const uint8_t* in_range,
+                                  const pct_bin_stats* stats) {
+    int rc = check_bin_args(ctx, hist, missed, "pct_bin_statistics");
+    if (rc) return rc;
+    if (!in_range || !stats || !stats->counts || !stats->mean_wepl || !stats->mean_rel_ut || !stats->mean_rel_uv ||
+        !stats->stddev_wepl || !stats->stddev_rel_ut || !stats->stddev_rel_uv)
+        return pct_fail(ctx, PCT_E_INVALID, "pct_bin_statistics: null output");
+    pct_bin_order* o = order_of(ctx);
+    if (!same_arrays(o->key, *hist))
+        return pct_fail(ctx, PCT_E_INVALID, "pct_bin_statistics: call pct_binning on these histories first");
+    bin_geo b = make_bin_geo(ctx->p);
+    hipLaunchKernelGGL(k_bin_stats, dim3(pct_grid(b.bins, 256)), dim3(256), 0, ctx->stream, b.bins,
+                       (const uint32_t*)o->first.ptr, (const uint32_t*)o->last.ptr, (const uint32_t*)o->vals_out.ptr,
+                       *hist, in_range, *stats);
+    PCT_HIP_CHECK(ctx, hipGetLastError());
+    return PCT_OK;
+}
+
+extern "C" int pct_statistical_cuts(pct_ctx* ctx, const pct_hist_soa* hist, const uint8_t* missed,
+                                    const pct_bin_stats* stats, uint8_t* failed, float* sinogram) {
+    int rc = check_bin_args(ctx, hist, missed, "pct_statistical_cuts");
+    if (rc) return rc;
+    if (!stats || !failed || !sinogram) return pct_fail(ctx, PCT_E_INVALID, "pct_statistical_cuts: null argument");
+    pct_bin_order* o = order_of(ctx);
+    if (!same_arrays(o->key, *hist))
+        return pct_fail(ctx, PCT_E_INVALID, "pct_statistical_cuts: call pct_binning on these histories first");
+    bin_geo b = make_bin_geo(ctx->p);
+    if (hist->n > 0)
+        hipLaunchKernelGGL(k_copy_u8, dim3(pct_grid(hist->n, 256)), dim3(256), 0, ctx->stream, missed, failed, hist->n);
+    hipLaunchKernelGGL(k_bin_cuts, dim3(pct_grid(b.bins, 256)), dim3(256), 0, ctx->stream, b.bins,
+                       (const uint32_t*)o->first.ptr, (const uint32_t*)o->last.ptr, (const uint32_t*)o->vals_out.ptr,
+                       *hist, *stats, (float)ctx->p.sigmas_to_keep, failed, sinogram);
+    PCT_HIP_CHECK(ctx, hipGetLastError());
+    return PCT_OK;
+}

@@ -155,6 +155,7 @@ extern "C" void pct_destroy(pct_ctx* ctx) {
     if (ctx->d_mlp_tab) (void)hipFree(ctx->d_mlp_tab);
     if (ctx->d_stats) (void)hipFree(ctx->d_stats);
     pct_plan_destroy(ctx->plan);
+    pct_bin_order_destroy(ctx->bin_order);
     for (hipEvent_t& e : ctx->ev)
         if (e) (void)hipEventDestroy(e);
     if (ctx->own_stream && ctx->stream) (void)hipStreamDestroy(ctx->stream);

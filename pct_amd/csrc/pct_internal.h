@@ -38,6 +38,7 @@ pct_geo pct_make_geo(const pct_params& p);
 enum { MLP_TAB_U1 = 0, MLP_TAB_S1I0, MLP_TAB_S1I1, MLP_TAB_S1I3, MLP_TAB_C1, MLP_TAB_C2, MLP_TAB_C3, MLP_TAB_FIELDS };
 
 struct pct_plan;
+struct pct_bin_order;
 
 struct pct_buffer {
     void* ptr = nullptr;
@@ -67,7 +68,11 @@ struct pct_ctx {
     unsigned long long* d_stats = nullptr;
     /* DROP plan of the last scan (pct_plan.hip) */
     pct_plan* plan = nullptr;
+    /* stable order of the histories by bin (pct_bins.hip) */
+    pct_bin_order* bin_order = nullptr;
 };
+
+void pct_bin_order_destroy(pct_bin_order* o);
 
 /* error helpers */
 int pct_fail(pct_ctx* ctx, int code, const char* fmt, ...);
