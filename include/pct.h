@@ -148,6 +148,36 @@ int pct_statistical_cuts(pct_ctx* ctx, const pct_hist_soa* hist, const uint8_t* 
  * ceil(V/32) words. */
 int pct_hull_init(pct_ctx* ctx, uint32_t* hull_bits);
 
+/* (c) Space carving: replaces SC_GPU (.cu:3338-3444). Clears the bit of every
+ * voxel on the straight entry->exit walk (take_3D/2D_step_GPU, .cu:8754-8867)
+ * of each non-missed history with WEPL <= SC_THRESHOLD. */
+int pct_space_carve(pct_ctx* ctx, const pct_hist_soa* hist, const uint8_t* missed, uint32_t* hull_bits);
+/* (c) Modified space carving: replaces initialize_hull_GPU<int> on the MSC
+ * counts (.cu:3303) and MSC_GPU (.cu:3456-3558): counts[v] starts at 1 inside
+ * the cylinder, 0 outside, and is incremented for every walk voxel of each
+ * non-missed history with WEPL <= MSC_THRESHOLD. */
+int pct_msc_init(pct_ctx* ctx, int32_t* counts);
+int pct_msc_count(pct_ctx* ctx, const pct_hist_soa* hist, const uint8_t* missed, int32_t* counts);
+/* (c) MSC edge detection: replaces MSC_edge_detection_GPU (.cu:3569-3596),
+ * out of place (out != counts): out[v] = 0 where the in-slice 3x3 count drop
+ * exceeds MSC_DIFF_THRESH or the voxel is on the cylinder rim, else 1. */
+int pct_msc_edge(pct_ctx* ctx, const int32_t* counts, int32_t* out);
+/* (c) Hull selection + averaging filter: replaces hull_selection (.cu:3954)
+ * and averaging_filter_GPU (.cu:4007-4029): hull = counts != 0, then a voxel
+ * is kept when its in-slice (2r+1)^2 neighbourhood sum exceeds
+ * HULL_FILTER_THRESHOLD times the neighbourhood size. Bit-packed output. */
+int pct_hull_select(pct_ctx* ctx, const int32_t* counts, uint32_t* hull_bits);
+
+/* (b) FBP: replaces filter_GPU (.cu:3036-3069, Shepp-Logan), backprojection_GPU
+ * (.cu:3155-3230) and FBP_image_2_hull_GPU (.cu:3250-3261). Sinograms have
+ * t_bins * v_bins * angular_bins entries, images columns * rows * slices. */
+int pct_fbp_filter(pct_ctx* ctx, const float* sinogram, float* filtered);
+int pct_backproject(pct_ctx* ctx, const float* filtered, float* image);
+int pct_fbp_threshold(pct_ctx* ctx, const float* image, uint32_t* hull_bits);
+/* (e) Initial iterate x0 = FBP image * hull (X_K0 = HYBRID,
+ * define_initial_iterate .cu:7191-7224). */
+int pct_initial_iterate(pct_ctx* ctx, const float* fbp_image, const uint32_t* hull_bits, float* x0);
+
 /* (d) MLP endpoints: replaces collect_MLP_endpoints_GPU (.cu:5729-5770) with
  * find_MLP_endpoints_GPU (.cu:4997-5277). For histories that enter and exit
  * the hull, overwrites x/y/z entry/exit with the hull surface points and sets

@@ -48,6 +48,11 @@ def lib():
         L.oracle_bin_statistics.argtypes = [P(abi.PctParams), P(abi.PctHistSoa), vp, vp, vp, vp, vp, vp, vp, vp]
         L.oracle_statistical_cuts.argtypes = [P(abi.PctParams), P(abi.PctHistSoa), vp, vp, vp, vp, vp, vp, vp, vp,
                                               vp, vp]
+        for name, n in (("oracle_space_carve", 2), ("oracle_msc_count", 2)):
+            getattr(L, name).argtypes = [P(abi.PctParams), P(abi.PctHistSoa)] + [vp] * n
+        for name, n in (("oracle_msc_edge", 2), ("oracle_hull_select", 2), ("oracle_fbp_filter", 2),
+                        ("oracle_backprojection", 2), ("oracle_fbp_hull", 2), ("oracle_initial_iterate", 3)):
+            getattr(L, name).argtypes = [P(abi.PctParams)] + [vp] * n
         L.oracle_math.argtypes = [ctypes.c_int, vp, vp, vp, ctypes.c_int64]
         L.oracle_math.restype = None
         _L = L
@@ -161,6 +166,57 @@ def bin_stats_and_cuts(p, h: HostHistories, missed: np.ndarray):
     return dict(in_range=in_range, counts=stat_counts, mean_wepl=mw, mean_rel_ut=mu, mean_rel_uv=mv,
                 stddev_wepl=sw, stddev_rel_ut=su, stddev_rel_uv=sv, failed=failed, sinogram=sino,
                 cut_counts=counts)
+
+
+def space_carve(p, h: HostHistories, missed, hull):
+    hull = np.ascontiguousarray(hull, np.uint8).copy()
+    hs = h.soa()
+    lib().oracle_space_carve(ctypes.byref(p), ctypes.byref(hs), _p(np.ascontiguousarray(missed, np.uint8)), _p(hull))
+    return hull
+
+
+def msc_counts(p, h: HostHistories, missed):
+    counts = hull_init(p).astype(np.int32)
+    hs = h.soa()
+    lib().oracle_msc_count(ctypes.byref(p), ctypes.byref(hs), _p(np.ascontiguousarray(missed, np.uint8)), _p(counts))
+    return counts
+
+
+def msc_edge(p, counts):
+    out = np.zeros_like(counts, dtype=np.int32)
+    lib().oracle_msc_edge(ctypes.byref(p), _p(np.ascontiguousarray(counts, np.int32)), _p(out))
+    return out
+
+
+def hull_select(p, counts):
+    hull = np.zeros(counts.size, np.uint8)
+    lib().oracle_hull_select(ctypes.byref(p), _p(np.ascontiguousarray(counts, np.int32)), _p(hull))
+    return hull
+
+
+def fbp_filter(p, sino):
+    out = np.zeros_like(sino, dtype=np.float32)
+    lib().oracle_fbp_filter(ctypes.byref(p), _p(np.ascontiguousarray(sino, np.float32)), _p(out))
+    return out
+
+
+def backprojection(p, filtered):
+    img = np.zeros(p.columns * p.rows * p.slices, np.float32)
+    lib().oracle_backprojection(ctypes.byref(p), _p(np.ascontiguousarray(filtered, np.float32)), _p(img))
+    return img
+
+
+def fbp_hull(p, image):
+    hull = np.zeros(image.size, np.uint8)
+    lib().oracle_fbp_hull(ctypes.byref(p), _p(np.ascontiguousarray(image, np.float32)), _p(hull))
+    return hull
+
+
+def initial_iterate(p, fbp, hull):
+    x0 = np.zeros(fbp.size, np.float32)
+    lib().oracle_initial_iterate(ctypes.byref(p), _p(np.ascontiguousarray(fbp, np.float32)),
+                                 _p(np.ascontiguousarray(hull, np.uint8)), _p(x0))
+    return x0
 
 
 def hull_init(p):

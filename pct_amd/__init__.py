@@ -213,6 +213,53 @@ class Context:
         self._check(self.L.pct_hull_init(self._ctx, _ptr(bits)), "pct_hull_init")
         return bits
 
+    def space_carve(self, hist, missed, hull_bits):
+        hs = hist.soa()
+        self._check(self.L.pct_space_carve(self._ctx, ctypes.byref(hs), _ptr(missed), _ptr(hull_bits)),
+                    "pct_space_carve")
+        return hull_bits
+
+    def msc_counts(self, hist, missed, counts=None):
+        """MSC counts (initialized to the cylinder when counts is None)."""
+        if counts is None:
+            counts = self.torch.empty(self.V, dtype=self.torch.int32, device=self.device)
+            self._check(self.L.pct_msc_init(self._ctx, _ptr(counts)), "pct_msc_init")
+        hs = hist.soa()
+        self._check(self.L.pct_msc_count(self._ctx, ctypes.byref(hs), _ptr(missed), _ptr(counts)), "pct_msc_count")
+        return counts
+
+    def msc_edge(self, counts):
+        out = self.torch.empty_like(counts)
+        self._check(self.L.pct_msc_edge(self._ctx, _ptr(counts), _ptr(out)), "pct_msc_edge")
+        return out
+
+    def hull_select(self, counts):
+        bits = self.torch.empty((self.V + 31) // 32, dtype=self.torch.int32, device=self.device)
+        self._check(self.L.pct_hull_select(self._ctx, _ptr(counts), _ptr(bits)), "pct_hull_select")
+        return bits
+
+    # -- (b) FBP (.cu:2919-3261) and the initial iterate (.cu:7191) -------------
+    def fbp_filter(self, sinogram):
+        out = self.torch.empty_like(sinogram)
+        self._check(self.L.pct_fbp_filter(self._ctx, _ptr(sinogram), _ptr(out)), "pct_fbp_filter")
+        return out
+
+    def backproject(self, filtered):
+        img = self.torch.empty(self.V, dtype=self.torch.float32, device=self.device)
+        self._check(self.L.pct_backproject(self._ctx, _ptr(filtered), _ptr(img)), "pct_backproject")
+        return img
+
+    def fbp_threshold(self, image):
+        bits = self.torch.empty((self.V + 31) // 32, dtype=self.torch.int32, device=self.device)
+        self._check(self.L.pct_fbp_threshold(self._ctx, _ptr(image), _ptr(bits)), "pct_fbp_threshold")
+        return bits
+
+    def initial_iterate(self, fbp_image, hull_bits):
+        x0 = self.torch.empty(self.V, dtype=self.torch.float32, device=self.device)
+        self._check(self.L.pct_initial_iterate(self._ctx, _ptr(fbp_image), _ptr(hull_bits), _ptr(x0)),
+                    "pct_initial_iterate")
+        return x0
+
     # -- (d) endpoints + compaction (reconstruction_cuts, .cu:6142-6300) --------
     def mlp_endpoints(self, hist: Histories, hull_bits):
         inter = self.torch.empty(hist.n, dtype=self.torch.uint8, device=self.device)

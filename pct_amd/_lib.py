@@ -87,6 +87,15 @@ SIGNATURES = {
     "pct_default_params": (None, [P(PctParams)]),
     "pct_recon_volume_intersections": (ctypes.c_int, [c_vp, P(PctRawSoa), P(PctHistSoa), c_vp]),
     "pct_hull_init": (ctypes.c_int, [c_vp, c_vp]),
+    "pct_space_carve": (ctypes.c_int, [c_vp, P(PctHistSoa), c_vp, c_vp]),
+    "pct_msc_init": (ctypes.c_int, [c_vp, c_vp]),
+    "pct_msc_count": (ctypes.c_int, [c_vp, P(PctHistSoa), c_vp, c_vp]),
+    "pct_msc_edge": (ctypes.c_int, [c_vp, c_vp, c_vp]),
+    "pct_hull_select": (ctypes.c_int, [c_vp, c_vp, c_vp]),
+    "pct_fbp_filter": (ctypes.c_int, [c_vp, c_vp, c_vp]),
+    "pct_backproject": (ctypes.c_int, [c_vp, c_vp, c_vp]),
+    "pct_fbp_threshold": (ctypes.c_int, [c_vp, c_vp, c_vp]),
+    "pct_initial_iterate": (ctypes.c_int, [c_vp, c_vp, c_vp, c_vp]),
     "pct_binning": (ctypes.c_int, [c_vp, P(PctHistSoa), c_vp, c_vp]),
     "pct_bin_statistics": (ctypes.c_int, [c_vp, P(PctHistSoa), c_vp, c_vp, P(PctBinStats)]),
     "pct_statistical_cuts": (ctypes.c_int, [c_vp, P(PctHistSoa), c_vp, P(PctBinStats), c_vp, c_vp]),

@@ -1,0 +1,247 @@
+/*
+ * pct_hull.hip -- (c) space carving, modified space carving, MSC edge
+ * detection and hull selection with the averaging filter.
+ *
+ * Replaces SC_GPU / MSC_GPU (.cu:3338-3558, straight voxel walk
+ * take_3D_step_GPU / take_2D_step_GPU .cu:8754-8867), MSC_edge_detection_GPU
+ * (.cu:3569-3596), hull_selection (.cu:3954-3990) and averaging_filter_GPU
+ * (.cu:4007-4029). Hulls are bit-packed (1 bit per voxel): SC clears bits with
+ * atomicAnd, MSC counts with integer atomics (both order-independent, so the
+ * result is deterministic), the filtered hull is packed with a wavefront
+ * ballot. The reference's edge detection rewrites the counts in place while
+ * other workgroups still read them (.cu:3582 vs 3589); here it reads the counts
+ * and writes a separate array, the race-free meaning of that kernel.
+ */
+#include "pct_internal.h"
+
+namespace {
+
+__device__ __forceinline__ int calc_voxel(double zero, double pos, double size) {
+    double q = pct_fabs(pos - zero) / size;
+    return q < 1.0e9 ? (int)q : -1;
+}
+__device__ __forceinline__ double edge_coord(double zero, int voxel_entered, double size, int inc, int step) {
+    int on_edge = (step == inc) ? voxel_entered : voxel_entered + 1;
+    return zero + (inc * on_edge * size);
+}
+__device__ __forceinline__ double dist_rem(double zero, double pos, int inc, int step, double size, int cur) {
+    return pct_fabs(edge_coord(zero, cur + inc * step, size, inc, step) - pos);
+}
+__device__ __forceinline__ double corr(double m, double x, double x0, double y0) { return m * (x - x0) + y0; }
+
+/* the carving walk from entry to exit: visit() for every voxel before the exit voxel */
+template <class F>
+__device__ void carve_walk(const pct_geo& g, float xe, float ye, float ze, float xx, float yx, float zx, F visit) {
+    const int C = g.C, R = g.R, S = g.S;
+    const int X_INC = 1, Y_INC = -1, Z_INC = -1;
+    int xm = (xe <= xx) - (xe >= xx);
+    int ym = (ye <= yx) - (ye >= yx);
+    int zm = (ze <= zx) - (ze >= zx);
+    double x = xe, y = ye, z = ze;
+    int vx = calc_voxel(g.x_zero, xe, g.voxel_width);
+    int vy = calc_voxel(g.y_zero, ye, g.voxel_height);
+    int vz = calc_voxel(g.z_zero, ze, g.voxel_thickness);
+    int64_t voxel = vx + (int64_t)vy * C + (int64_t)vz * C * R;
+    double x_to_go = dist_rem(g.x_zero, x, X_INC, xm, g.voxel_width, vx);
+    double y_to_go = dist_rem(g.y_zero, y, Y_INC, ym, g.voxel_height, vy);
+    double z_to_go = dist_rem(g.z_zero, z, Z_INC, zm, g.voxel_thickness, vz);
+    double dy_dx = (yx - ye) / (xx - xe);
+    double dz_dx = (zx - ze) / (xx - xe);
+    double dz_dy = (zx - ze) / (yx - ye);
+    double dx_dy = (xx - xe) / (yx - ye);
+    double dx_dz = (xx - xe) / (zx - ze);
+    double dy_dz = (yx - ye) / (zx - ze);
+    int vxo = calc_voxel(g.x_zero, xx, g.voxel_width);
+    int vyo = calc_voxel(g.y_zero, yx, g.voxel_height);
+    int vzo = calc_voxel(g.z_zero, zx, g.voxel_thickness);
+    int64_t voxel_out = vxo + (int64_t)vyo * C + (int64_t)vzo * C * R;
+    bool end_walk = (voxel == voxel_out) || (vx >= C) || (vy >= R) || (vz >= S) || vx < 0 || vy < 0 || vz < 0;
+    if (!end_walk) visit(voxel);
+    int guard = 4 * (C + R + S) + 16;
+    while (!end_walk && guard-- > 0) {
+        if (zm != 0) {
+            double x_ext = pct_fabs(dz_dx) * x_to_go;
+            double y_ext = pct_fabs(dz_dy) * y_to_go;
+            if ((z_to_go <= x_ext) && (z_to_go <= y_ext)) {
+                vz -= zm;
+                z = edge_coord(g.z_zero, vz, g.voxel_thickness, Z_INC, zm);
+                x = corr(dx_dz, z, ze, xe);
+                y = corr(dy_dz, z, ze, ye);
+                x_to_go = dist_rem(g.x_zero, x, X_INC, xm, g.voxel_width, vx);
+                y_to_go = dist_rem(g.y_zero, y, Y_INC, ym, g.voxel_height, vy);
+                z_to_go = g.voxel_thickness;
+            } else if (x_ext <= y_ext) {
+                vx += xm;
+                x = edge_coord(g.x_zero, vx, g.voxel_width, X_INC, xm);
+                y = corr(dy_dx, x, xe, ye);
+                z = corr(dz_dx, x, xe, ze);
+                x_to_go = g.voxel_width;
+                y_to_go = dist_rem(g.y_zero, y, Y_INC, ym, g.voxel_height, vy);
+                z_to_go = dist_rem(g.z_zero, z, Z_INC, zm, g.voxel_thickness, vz);
+            } else {
+                vy -= ym;
+                y = edge_coord(g.y_zero, vy, g.voxel_height, Y_INC, ym);
+                x = corr(dx_dy, y, ye, xe);
+                z = corr(dz_dy, y, ye, ze);
+                x_to_go = dist_rem(g.x_zero, x, X_INC, xm, g.voxel_width, vx);
+                y_to_go = g.voxel_height;
+                z_to_go = dist_rem(g.z_zero, z, Z_INC, zm, g.voxel_thickness, vz);
+            }
+            if (x_to_go == 0) { x_to_go = g.voxel_width; vx += xm; }
+            if (y_to_go == 0) { y_to_go = g.voxel_height; vy -= ym; }
+            if (z_to_go == 0) { z_to_go = g.voxel_thickness; vz -= zm; }
+        } else {
+            double y_ext = pct_fabs(dx_dy) * y_to_go;
+            if (x_to_go <= y_ext) {
+                vx += xm;
+                x = edge_coord(g.x_zero, vx, g.voxel_width, X_INC, xm);
+                y = corr(dy_dx, x, xe, ye);
+                x_to_go = g.voxel_width;
+                y_to_go = dist_rem(g.y_zero, y, Y_INC, ym, g.voxel_height, vy);
+            } else {
+                vy -= ym;
+                y = edge_coord(g.y_zero, vy, g.voxel_height, Y_INC, ym);
+                x = corr(dx_dy, y, ye, xe);
+                x_to_go = dist_rem(g.x_zero, x, X_INC, xm, g.voxel_width, vx);
+                y_to_go = g.voxel_height;
+            }
+            if (x_to_go == 0) { x_to_go = g.voxel_width; vx += xm; }
+            if (y_to_go == 0) { y_to_go = g.voxel_height; vy -= ym; }
+        }
+        vz = vz > 0 ? vz : 0;
+        voxel = vx + (int64_t)vy * C + (int64_t)vz * C * R;
+        end_walk = (voxel == voxel_out) || (vx >= C) || (vy >= R) || (vz >= S) || vx < 0 || vy < 0 || vz < 0;
+        if (!end_walk) visit(voxel);
+    }
+}
+
+__global__ void __launch_bounds__(256)
+k_space_carve(pct_geo g, pct_hist_soa h, const uint8_t* __restrict__ missed, double threshold,
+              uint32_t* __restrict__ bits) {
+    int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= h.n || missed[i] || !(h.wepl[i] <= threshold)) return;
+    carve_walk(g, h.x_entry[i], h.y_entry[i], h.z_entry[i], h.x_exit[i], h.y_exit[i], h.z_exit[i], [&](int64_t v) {
+        if (v >= 0 && v < g.V) atomicAnd(&bits[v >> 5], ~(1u << (v & 31)));
+    });
+}
+
+__global__ void __launch_bounds__(256)
+k_msc_count(pct_geo g, pct_hist_soa h, const uint8_t* __restrict__ missed, double threshold,
+            int32_t* __restrict__ counts) {
+    int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= h.n || missed[i] || !(h.wepl[i] <= threshold)) return;
+    carve_walk(g, h.x_entry[i], h.y_entry[i], h.z_entry[i], h.x_exit[i], h.y_exit[i], h.z_exit[i], [&](int64_t v) {
+        if (v >= 0 && v < g.V) atomicAdd(&counts[v], 1);
+    });
+}
+
+/* initialize_hull_GPU<int> (.cu:3316) for the MSC counts */
+__global__ void k_counts_init(pct_geo g, int32_t* __restrict__ counts) {
+    int64_t v = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (v >= g.V) return;
+    int column = (int)(v % g.C), row = (int)((v / g.C) % g.R);
+    double x = (column - g.C / 2 + 0.5) * g.voxel_width;
+    double y = (g.R / 2 - row - 0.5) * g.voxel_height;
+    counts[v] = (x * x + y * y < g.radius * g.radius) ? 1 : 0;
+}
+
+__global__ void k_msc_edge(pct_geo g, int diff_thresh, const int32_t* __restrict__ counts, int32_t* __restrict__ out) {
+    int64_t voxel = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (voxel >= g.V) return;
+    const int C = g.C, R = g.R;
+    int column = (int)(voxel % C), row = (int)((voxel / C) % R);
+    int64_t slice_base = voxel - column - (int64_t)row * C;
+    float x = (float)((column - C / 2 + 0.5) * g.voxel_width);
+    float y = (float)((R / 2 - row - 0.5) * g.voxel_height);
+    int max_difference = 0;
+    if (row != 0 && row != R - 1 && column != 0 && column != C - 1) {
+        int c0 = counts[voxel];
+        for (int cr = row - 1; cr <= row + 1; ++cr)
+            for (int cc = column - 1; cc <= column + 1; ++cc) {
+                int d = c0 - counts[slice_base + cc + (int64_t)cr * C];
+                if (d > max_difference) max_difference = d;
+            }
+    }
+    int v = max_difference > diff_thresh ? 0 : 1;
+    double rr = g.radius - (g.voxel_width > g.voxel_height ? g.voxel_width : g.voxel_height) / 2;
+    float s2 = x * x + y * y;
+    if ((double)s2 >= rr * rr) v = 0;
+    out[voxel] = v;
+}
+
+/* hull = (counts != 0) then the averaging filter with threshold; one voxel per
+ * lane, 64 consecutive voxels per wave packed into two words */
+__global__ void __launch_bounds__(256)
+k_hull_select(pct_geo g, int radius, double threshold, const int32_t* __restrict__ counts, uint32_t* __restrict__ bits) {
+    int64_t voxel = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    bool on = false;
+    if (voxel < g.V) {
+        const int C = g.C, R = g.R;
+        int vx = (int)(voxel % C), vy = (int)((voxel / C) % R);
+        int64_t slice_base = voxel - vx - (int64_t)vy * C;
+        unsigned left = vx - radius > 0 ? vx - radius : 0, right = vx + radius < C - 1 ? vx + radius : C - 1;
+        unsigned top = vy - radius > 0 ? vy - radius : 0, bottom = vy + radius < R - 1 ? vy + radius : R - 1;
+        int nvox = (int)((right - left + 1) * (bottom - top + 1));
+        double sum_threshold = nvox * threshold;
+        double sum = 0.0;
+        for (int c = (int)left; c <= (int)right; ++c)
+            for (int r = (int)top; r <= (int)bottom; ++r) sum += (counts[slice_base + c + (int64_t)r * C] != 0) ? 1.0 : 0.0;
+        on = sum > sum_threshold;
+    }
+    uint64_t m = __ballot(on);
+    int lane = threadIdx.x & 63;
+    int64_t w0 = voxel >> 5;       /* lane 0 and lane 32 write their word */
+    if ((lane & 31) == 0 && voxel < g.V) bits[w0] = (uint32_t)(lane == 0 ? m : (m >> 32));
+}
+
+}  // namespace
+
+static int check_carve(pct_ctx* ctx, const pct_hist_soa* h, const uint8_t* missed, const void* out, const char* who) {
+    if (!ctx || !h || !missed || !out) return pct_fail(ctx, PCT_E_INVALID, "%s: null argument", who);
+    if (!h->x_entry || !h->x_exit || !h->wepl) return pct_fail(ctx, PCT_E_INVALID, "%s: missing history field", who);
+    return PCT_OK;
+}
+
+extern "C" int pct_space_carve(pct_ctx* ctx, const pct_hist_soa* hist, const uint8_t* missed, uint32_t* hull_bits) {
+    int rc = check_carve(ctx, hist, missed, hull_bits, "pct_space_carve");
+    if (rc) return rc;
+    if (hist->n > 0)
+        hipLaunchKernelGGL(k_space_carve, dim3(pct_grid(hist->n, 256)), dim3(256), 0, ctx->stream, ctx->g, *hist, missed,
+                           ctx->p.sc_threshold, hull_bits);
+    PCT_HIP_CHECK(ctx, hipGetLastError());
+    return PCT_OK;
+}
+
+extern "C" int pct_msc_init(pct_ctx* ctx, int32_t* counts) {
+    if (!ctx || !counts) return pct_fail(ctx, PCT_E_INVALID, "pct_msc_init: null argument");
+    hipLaunchKernelGGL(k_counts_init, dim3(pct_grid(ctx->g.V, 256)), dim3(256), 0, ctx->stream, ctx->g, counts);
+    PCT_HIP_CHECK(ctx, hipGetLastError());
+    return PCT_OK;
+}
+
+extern "C" int pct_msc_count(pct_ctx* ctx, const pct_hist_soa* hist, const uint8_t* missed, int32_t* counts) {
+    int rc = check_carve(ctx, hist, missed, counts, "pct_msc_count");
+    if (rc) return rc;
+    if (hist->n > 0)
+        hipLaunchKernelGGL(k_msc_count, dim3(pct_grid(hist->n, 256)), dim3(256), 0, ctx->stream, ctx->g, *hist, missed,
+                           ctx->p.msc_threshold, counts);
+    PCT_HIP_CHECK(ctx, hipGetLastError());
+    return PCT_OK;
+}
+
+extern "C" int pct_msc_edge(pct_ctx* ctx, const int32_t* counts, int32_t* out) {
+    if (!ctx || !counts || !out || counts == out) return pct_fail(ctx, PCT_E_INVALID, "pct_msc_edge: bad arguments");
+    hipLaunchKernelGGL(k_msc_edge, dim3(pct_grid(ctx->g.V, 256)), dim3(256), 0, ctx->stream, ctx->g,
+                       ctx->p.msc_diff_thresh, counts, out);
+    PCT_HIP_CHECK(ctx, hipGetLastError());
+    return PCT_OK;
+}
+
+extern "C" int pct_hull_select(pct_ctx* ctx, const int32_t* counts, uint32_t* hull_bits) {
+    if (!ctx || !counts || !hull_bits) return pct_fail(ctx, PCT_E_INVALID, "pct_hull_select: null argument");
+    if (ctx->p.hull_filter_radius < 0) return pct_fail(ctx, PCT_E_INVALID, "pct_hull_select: negative radius");
+    hipLaunchKernelGGL(k_hull_select, dim3(pct_grid(ctx->g.V, 256)), dim3(256), 0, ctx->stream, ctx->g,
+                       ctx->p.hull_filter_radius, ctx->p.hull_filter_threshold, counts, hull_bits);
+    PCT_HIP_CHECK(ctx, hipGetLastError());
+    return PCT_OK;
+}

@@ -39,3 +39,63 @@ def test_binning_stats_cuts_bit_exact(cfg1_sample):
     assert np.array_equal(sino.cpu().numpy().view(np.uint32), r["sinogram"].view(np.uint32))
     assert np.array_equal(st["counts"].cpu().numpy(), r["cut_counts"])
     ctx.close()
+
+
+@pytest.fixture(scope="module")
+def hull_sample():
+    """cfg1 geometry, 600 k histories, a 16 cm water cylinder (so that
+    protons pass through air inside the reconstruction cylinder)"""
+    import oracle
+    import pct_amd
+    from pct_amd import configs
+    p = configs.params("cfg1")
+    cfg = configs.synth_cfg("cfg1", n_histories=600_000)
+    cfg.cyl_r[0] = 8.0
+    raw = pct_amd.generate_host(cfg).numpy()
+    h, missed = oracle.recon_volume_intersections(p, dict(raw))
+    r = oracle.bin_stats_and_cuts(p, h, missed)
+    counts = oracle.msc_counts(p, h, missed)
+    edge = oracle.msc_edge(p, counts)
+    hull = oracle.hull_select(p, edge)
+    sc = oracle.space_carve(p, h, missed, oracle.hull_init(p))
+    filt = oracle.fbp_filter(p, r["sinogram"])
+    fbp = oracle.backprojection(p, filt)
+    fh = oracle.fbp_hull(p, fbp)
+    x0 = oracle.initial_iterate(p, fbp, hull)
+    return dict(p=p, cfg=cfg, missed=missed, counts=counts, edge=edge, hull=hull, sc=sc, sino=r["sinogram"],
+                filt=filt, fbp=fbp, fh=fh, x0=x0)
+
+
+def _unpack(bits, V):
+    b = bits.cpu().numpy().view(np.uint32)
+    idx = np.arange(V)
+    return ((b[idx >> 5] >> (idx & 31)) & 1).astype(np.uint8)
+
+
+def test_hull_and_fbp_bit_exact(hull_sample):
+    import pct_amd
+    s = hull_sample
+    p = s["p"]
+    V = p.columns * p.rows * p.slices
+    ctx = pct_amd.Context(p)
+    torch = ctx.torch
+    raw = ctx.generate(s["cfg"])
+    hist, missed = ctx.recon_volume_intersections(raw)
+    counts = ctx.msc_counts(hist, missed)
+    assert (s["counts"] > 1).sum() > 1000               # carving happened
+    assert np.array_equal(counts.cpu().numpy(), s["counts"])
+    edge = ctx.msc_edge(counts)
+    assert np.array_equal(edge.cpu().numpy(), s["edge"])
+    hull = ctx.hull_select(edge)
+    assert np.array_equal(_unpack(hull, V), s["hull"])
+    sc = ctx.space_carve(hist, missed, ctx.hull_init())
+    assert np.array_equal(_unpack(sc, V), s["sc"])
+    sino = torch.from_numpy(s["sino"]).cuda()
+    filt = ctx.fbp_filter(sino)
+    assert np.array_equal(filt.cpu().numpy().view(np.uint32), s["filt"].view(np.uint32))
+    fbp = ctx.backproject(filt)
+    assert np.array_equal(fbp.cpu().numpy().view(np.uint32), s["fbp"].view(np.uint32))
+    assert np.array_equal(_unpack(ctx.fbp_threshold(fbp), V), s["fh"])
+    x0 = ctx.initial_iterate(fbp, hull)
+    assert np.array_equal(x0.cpu().numpy().view(np.uint32), s["x0"].view(np.uint32))
+    ctx.close()
