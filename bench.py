@@ -33,6 +33,16 @@ def log(msg):
     print(msg, file=sys.stderr, flush=True)
 
 
+def _cpu_model():
+    try:
+        for line in open("/proc/cpuinfo"):
+            if line.startswith("model name"):
+                return line.split(":", 1)[1].strip()
+    except OSError:
+        pass
+    return "unknown"
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -64,22 +74,22 @@ def main():
     scfg = configs.synth_cfg(args.config, n_histories=args.histories)
     ctx = pct_amd.Context(p, dev)
 
-    # ---- setup (untimed): scan -> intersections -> hull -> endpoints ----------
+    # ---- setup (untimed): the reference's main() up to the DROP loop ---------
+    # intersections, binning, 3-sigma cuts, FBP, MSC hull, x0 = FBP * hull,
+    # MLP endpoints -- every stage on the GPU (pct_amd.pipeline)
+    from pct_amd import pipeline
     t0 = time.time()
     raw = ctx.generate(scfg)
-    hist, missed = ctx.recon_volume_intersections(raw)
+    torch.cuda.synchronize()
+    t_pre = time.time()
+    pre = pipeline.preprocess(ctx, raw)
+    torch.cuda.synchronize()
+    t_pre = time.time() - t_pre
     del raw
-    hist = ctx.compact(hist, (missed == 0).to(torch.uint8))
-    hull = ctx.hull_init()
-    inter = ctx.mlp_endpoints(hist, hull)
-    hist = ctx.compact(hist, inter)
-    ctx.sync()
+    hist = pre.hist
     n_r = hist.n
-    V = p.columns * p.rows * p.slices
-    bits = hull.cpu().numpy().view(np.uint32)
-    idx = np.arange(V)
-    x0_np = ((bits[idx >> 5] >> (idx & 31)) & 1).astype(np.float32)
-    x = torch.from_numpy(x0_np).to(dev)
+    x = pre.x0.clone()
+    x0_np = pre.x0.cpu().numpy()
     B = p.drop_block_size
     t1 = time.time()
     ctx.drop_prepare(hist, B)
@@ -88,7 +98,7 @@ def main():
     sum_n, sum_vb = pct_amd.drop_stats(ctx, hist, B)
     _, total_slots = ctx.mlp_capacity(hist)
     sum_steps = total_slots - n_r
-    log(f"setup {time.time() - t0:.1f}s (plan {t_plan:.2f}s): N={scfg.n_histories} N_r={n_r} "
+    log(f"setup {time.time() - t0:.1f}s (preprocess {t_pre:.2f}s, plan {t_plan:.2f}s): N={scfg.n_histories} N_r={n_r} "
         f"mean n={sum_n / n_r:.1f} sum|V_B|={sum_vb} steps={sum_steps}")
 
     def timed(mode, steps):
@@ -156,7 +166,15 @@ def main():
         cpu_t = time.perf_counter() - t
         cpu = {"value": round(m / cpu_t, 1), "unit": "histories/s per DROP iteration", "cores": 1, "kind": "port",
                "sample": f"first {m} reconstruction histories ({m // B} DROP blocks of {B}), one iteration from the "
-                         f"same x0, FP64 oracle (MLP + DROP_blocks/DROP_update), single thread, {cpu_t:.1f} s"}
+                         f"same x0, FP64 oracle (MLP + DROP_blocks/DROP_update), single thread, {cpu_t:.1f} s",
+               "cpu_model": _cpu_model()}
+        # the same sample with the per-block MLP work spread over the host cores
+        threads = min(16, os.cpu_count() or 1)
+        t = time.perf_counter()
+        x_mt = oracle.drop(p, hs, x0_np, iterations=1, threads=threads)
+        mt_t = time.perf_counter() - t
+        cpu["multithreaded"] = {"value": round(m / mt_t, 1), "cores": threads, "seconds": round(mt_t, 2),
+                                "identical_to_single_thread": bool(np.array_equal(x_mt, x_cpu))}
         # the GPU on the same sample from the same x0
         x_g = torch.from_numpy(x0_np).to(dev)
         ctx.drop_iteration(hist.head(m), x_g, B, p.lambda_)
@@ -181,12 +199,13 @@ def main():
         "data": "synthetic",
         "config": {"workload": f"{args.config}: {p.columns}x{p.rows}x{p.slices} voxels, "
                                f"{scfg.n_histories} histories / 90 angles, DROP block {B}, lambda {p.lambda_:g}, "
-                               f"exact FP64 MLP, FP32 image, cylinder hull",
-                   "histories": scfg.n_histories, "reconstruction_histories": n_r,
+                               f"exact FP64 MLP, FP32 image, x0 = FBP * MSC hull",
+                   "histories": scfg.n_histories, "post_cut_histories": pre.counts["post_cut"],
+                   "reconstruction_histories": n_r,
                    "mean_path_voxels": round(sum_n / n_r, 2), "sum_block_voxels": sum_vb,
                    "drop_blocks": (n_r + B - 1) // B},
         "phases_ms": {"mlp": round(float(mlp_ms), 3), "blocks": round(float(blk_ms), 3),
-                      "plan_setup_once": round(t_plan * 1e3, 1)},
+                      "plan_setup_once": round(t_plan * 1e3, 1), "preprocess_once": round(t_pre * 1e3, 1)},
         "other_forms": {"cached_lists_histories_per_s": round(n_r / (ms_cached / 1e3), 1),
                         "cached_lists_ms": round(ms_cached, 3),
                         "atomic_reference_structure_histories_per_s": round(n_r / (ms_atomic / 1e3), 1),

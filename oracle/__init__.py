@@ -40,6 +40,8 @@ def lib():
                                        ctypes.c_int64, vp]
         L.oracle_drop.argtypes = [P(abi.PctParams), P(abi.PctHistSoa), vp, ctypes.c_int, ctypes.c_float, ctypes.c_int,
                                   ctypes.c_int64]
+        L.oracle_drop_mt.argtypes = [P(abi.PctParams), P(abi.PctHistSoa), vp, ctypes.c_int, ctypes.c_float,
+                                     ctypes.c_int, ctypes.c_int64, ctypes.c_int]
         L.oracle_mlp_trace.argtypes = [P(abi.PctParams), P(abi.PctHistSoa), ctypes.c_int64, vp, vp, vp, vp, vp,
                                        ctypes.c_int, P(ctypes.c_int)]
         L.oracle_drop_lists.argtypes = [ctypes.c_int64, ctypes.c_int64, vp, vp, vp, vp, vp, ctypes.c_int,
@@ -219,6 +221,27 @@ def initial_iterate(p, fbp, hull):
     return x0
 
 
+def preprocess(p, raw: dict, hull_kind="msc"):
+    """The reference's main() up to the DROP loop (.cu:24-226) on the CPU:
+    returns (reconstruction histories, x0, hull byte mask, fbp image)."""
+    h, missed = recon_volume_intersections(p, dict(raw))
+    r = bin_stats_and_cuts(p, h, missed)
+    if hull_kind == "msc":
+        hull = hull_select(p, msc_edge(p, msc_counts(p, h, missed)))
+    elif hull_kind == "sc":
+        hull = space_carve(p, h, missed, hull_init(p))
+    elif hull_kind == "cylinder":
+        hull = hull_init(p)
+    post = h.select(r["failed"] == 0)
+    fbp = backprojection(p, fbp_filter(p, r["sinogram"]))
+    if hull_kind == "fbp":
+        hull = fbp_hull(p, fbp)
+    x0 = initial_iterate(p, fbp, hull)
+    inter = mlp_endpoints(p, post, hull)
+    recon = post.select(inter == 1)
+    return recon, x0, hull, fbp
+
+
 def hull_init(p):
     hull = np.zeros(p.columns * p.rows * p.slices, np.uint8)
     lib().oracle_hull_init(ctypes.byref(p), _p(hull))
@@ -280,14 +303,17 @@ def drop_lists(V, lists, chord, b, x0, block_size, lam, iterations=1):
     return x
 
 
-def drop(p, h: HostHistories, x0: np.ndarray, iterations=1, block_size=None, lam=None, max_histories=-1):
-    """Plain DROP (E6 semantics) from x0; returns the new float32 image."""
+def drop(p, h: HostHistories, x0: np.ndarray, iterations=1, block_size=None, lam=None, max_histories=-1,
+         threads=1):
+    """Plain DROP (E6 semantics) from x0; returns the new float32 image. The
+    result does not depend on `threads` (paths in parallel, accumulation in
+    storage order)."""
     x = np.ascontiguousarray(x0, dtype=np.float32).copy()
     block_size = p.drop_block_size if block_size is None else block_size
     lam = p.lambda_ if lam is None else lam
     hs = h.soa()
-    rc = lib().oracle_drop(ctypes.byref(p), ctypes.byref(hs), _p(x), int(block_size), float(lam), int(iterations),
-                           int(max_histories))
+    rc = lib().oracle_drop_mt(ctypes.byref(p), ctypes.byref(hs), _p(x), int(block_size), float(lam), int(iterations),
+                              int(max_histories), int(threads))
     if rc != 0:
         raise OverflowError("MLP path overflow in the oracle DROP")
     return x

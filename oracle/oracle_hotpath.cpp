@@ -603,12 +603,22 @@ int oracle_drop_lists(int64_t V, int64_t n_lists, const int64_t* offsets, const 
 /* E6 + E4 semantics: `iterations` DROP iterations over all histories in
  * storage order with blocks of `block_size`; x is updated in place.
  * x_update is double (data.h:558) and counts unsigned (DROP_blocks). */
-int oracle_drop(const pct_params* p, const pct_hist_soa* h, float* x, int block_size, float lambda,
-                int iterations, int64_t max_histories) {
+int oracle_drop_mt(const pct_params* p, const pct_hist_soa* h, float* x, int block_size, float lambda,
+                   int iterations, int64_t max_histories, int threads) {
+    /* Per block the histories' paths and update values are independent of
+     * each other (they read x as it was at the start of the block), so they
+     * may be computed by several threads; the accumulation into x_update and
+     * the image update then run sequentially in storage order, exactly as in
+     * the single-threaded DROP_blocks/DROP_update, so the result does not
+     * depend on the thread count. */
     Geo g = make_geo(*p);
+    if (threads < 1) threads = 1;
     std::vector<double> x_update(g.V, 0.0);
     std::vector<uint32_t> counts(g.V, 0);
-    std::vector<uint32_t> path(g.max_intersections + 1);
+    const int stride = g.max_intersections + 1;
+    std::vector<uint32_t> paths((size_t)block_size * stride);
+    std::vector<int> lens(block_size);
+    std::vector<double> upd(block_size);
     std::vector<uint32_t> touched;
     int64_t n_hist = h->n;
     if (max_histories >= 0 && max_histories < n_hist) n_hist = max_histories;
@@ -616,12 +626,16 @@ int oracle_drop(const pct_params* p, const pct_hist_soa* h, float* x, int block_
     for (int it = 0; it < iterations; ++it) {
         for (int64_t start = 0; start < n_hist; start += block_size) {
             int64_t end = start + block_size < n_hist ? start + block_size : n_hist;
-            touched.clear();
-            for (int64_t i = start; i < end; ++i) {
-                int n = find_MLP_path(g, path.data(), h->first_voxel[i], h->x_entry[i], h->y_entry[i], h->z_entry[i],
+            int64_t cnt = end - start;
+#pragma omp parallel for num_threads(threads) schedule(dynamic, 16)
+            for (int64_t k = 0; k < cnt; ++k) {
+                int64_t i = start + k;
+                uint32_t* path = paths.data() + k * stride;
+                int n = find_MLP_path(g, path, h->first_voxel[i], h->x_entry[i], h->y_entry[i], h->z_entry[i],
                                       h->x_exit[i], h->y_exit[i], h->z_exit[i], h->xy_entry_angle[i],
                                       h->xz_entry_angle[i], h->xy_exit_angle[i], h->xz_exit_angle[i], nullptr);
-                if (n < 0) { rc = -3; continue; }
+                lens[k] = n;
+                if (n < 0) continue;
                 double L = effective_chord_length(g, ((double)h->xy_entry_angle[i] + (double)h->xy_exit_angle[i]) / 2.0,
                                                   ((double)h->xz_entry_angle[i] + (double)h->xz_exit_angle[i]) / 2.0);
                 double sum = 0.0;
@@ -629,11 +643,17 @@ int oracle_drop(const pct_params* p, const pct_hist_soa* h, float* x, int block_
                 double a_i_dot_x_k = sum * L;
                 double a_i_dot_a_i = (L * L) * n;
                 double b_i = h->wepl[i];
-                double update = L * ((b_i - a_i_dot_x_k) / a_i_dot_a_i) * lambda;
+                upd[k] = L * ((b_i - a_i_dot_x_k) / a_i_dot_a_i) * lambda;
+            }
+            touched.clear();
+            for (int64_t k = 0; k < cnt; ++k) {
+                int n = lens[k];
+                if (n < 0) { rc = -3; continue; }
+                const uint32_t* path = paths.data() + k * stride;
                 for (int j = 0; j < n; ++j) {
                     uint32_t v = path[j];
                     if (counts[v] == 0) touched.push_back(v);
-                    x_update[v] += update;
+                    x_update[v] += upd[k];
                     counts[v] += 1;
                 }
             }
@@ -646,6 +666,11 @@ int oracle_drop(const pct_params* p, const pct_hist_soa* h, float* x, int block_
         }
     }
     return rc;
+}
+
+int oracle_drop(const pct_params* p, const pct_hist_soa* h, float* x, int block_size, float lambda,
+                int iterations, int64_t max_histories) {
+    return oracle_drop_mt(p, h, x, block_size, lambda, iterations, max_histories, 1);
 }
 
 }  // extern "C"

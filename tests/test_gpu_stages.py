@@ -99,3 +99,32 @@ def test_hull_and_fbp_bit_exact(hull_sample):
     x0 = ctx.initial_iterate(fbp, hull)
     assert np.array_equal(x0.cpu().numpy().view(np.uint32), s["x0"].view(np.uint32))
     ctx.close()
+
+
+def test_full_pipeline_matches_oracle():
+    """raw scan -> cuts -> FBP -> MSC hull -> x0 -> endpoints -> 3 DROP
+    iterations, GPU (pct_amd.pipeline) vs the oracle's main()-equivalent"""
+    import oracle
+    import pct_amd
+    from pct_amd import configs, pipeline
+    p = configs.params("cfg1")
+    p.lambda_ = 1e-3
+    cfg = configs.synth_cfg("cfg1", n_histories=300_000)
+    cfg.cyl_r[0] = 8.0
+    raw = pct_amd.generate_host(cfg).numpy()
+    recon, x0, hull, fbp = oracle.preprocess(p, raw)
+    ctx = pct_amd.Context(p)
+    pre = pipeline.preprocess(ctx, ctx.generate(cfg))
+    V = p.columns * p.rows * p.slices
+    assert pre.hist.n == recon.n
+    assert np.array_equal(_unpack(pre.hull, V), hull)
+    assert np.array_equal(pre.x0.cpu().numpy().view(np.uint32), x0.view(np.uint32))
+    got = pre.hist.numpy()
+    for f in ("x_entry", "y_entry", "z_exit", "first_voxel", "wepl", "bin_num"):
+        assert np.array_equal(np.asarray(got[f]).view(np.uint32), np.asarray(recon.arrays()[f]).view(np.uint32)), f
+    x = pipeline.image_reconstruction(ctx, pre, iterations=3)
+    exp = oracle.drop(p, recon, x0, iterations=3, threads=8)
+    d = np.abs(x.cpu().numpy().astype(np.float64) - exp)
+    assert d.max() <= 1e-5
+    assert np.mean(d == 0) > 0.99
+    ctx.close()

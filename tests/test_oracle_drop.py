@@ -91,3 +91,12 @@ def test_partial_last_block_is_applied(mini_host):
     full = oracle.drop(p, hr, x0, iterations=1, lam=1e-2)
     first = oracle.drop(p, hr.head(p.drop_block_size), x0, iterations=1, lam=1e-2)
     assert np.any(full != first)
+
+
+def test_threaded_drop_equals_single_thread(mini_host):
+    p = mini_host["p"]
+    hr = mini_host["hr"]
+    x0 = mini_host["hull"].astype(np.float32)
+    a = oracle.drop(p, hr, x0, iterations=2, lam=1e-2, threads=1)
+    b = oracle.drop(p, hr, x0, iterations=2, lam=1e-2, threads=8)
+    np.testing.assert_array_equal(a, b)
