@@ -87,6 +87,9 @@ typedef struct pct_hist_soa {
 
 /* MLP evaluation modes for the path and DROP entry points */
 #define PCT_MLP_FP64      0      /* exact FP64 Schulte MLP (.cu:5279-5501)      */
+#define PCT_MLP_TABLE     1      /* precision form, after the tabulated MLP
+                                    (.cu:5502-5656): FP64 per-depth sigma
+                                    tables and 2x2 inversions, FP32 stepping  */
 #define PCT_MLP_MODE_MASK 0xff
 /* flags of pct_drop_iteration's mode, OR-ed with the MLP mode */
 #define PCT_DROP_CACHED    0x100 /* reuse the plan's voxel lists instead of
@@ -94,6 +97,9 @@ typedef struct pct_hist_soa {
 #define PCT_DROP_ATOMIC    0x200 /* reference structure: atomic scatter and a
                                     volume scan per block, no plan             */
 #define PCT_DROP_NO_GRAPH  0x400 /* launch the block loop eagerly (no hipGraph) */
+#define PCT_DROP_F32       0x800 /* precision form of the update: a_i.x, the
+                                    residual and the per-voxel sums in FP32,
+                                    sums Kahan-compensated                      */
 
 typedef struct pct_ctx pct_ctx;
 

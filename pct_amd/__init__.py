@@ -105,6 +105,33 @@ class Histories:
         return d
 
 
+class _StreamOrdered:
+    """libpct's functions, each call ordered after the work already queued on
+    torch's current stream and before whatever torch queues there next.
+
+    The context runs on a stream of its own (torch's default stream is the
+    null stream, which cannot be captured into a hipGraph). Tensors are
+    allocated and read back on torch's current stream, so every library call
+    waits for that stream and that stream waits for the call: an event record
+    and a device-side wait on each side, no host synchronisation."""
+
+    def __init__(self, lib, stream, torch):
+        self._lib, self._stream, self._torch = lib, stream, torch
+
+    def __getattr__(self, name):
+        fn = getattr(self._lib, name)
+        stream, torch = self._stream, self._torch
+
+        def call(*args):
+            cur = torch.cuda.current_stream(stream.device)
+            stream.wait_stream(cur)
+            try:
+                return fn(*args)
+            finally:
+                cur.wait_stream(stream)
+        return call
+
+
 class Context:
     """A pct_ctx on the current HIP device (pct_create, include/pct.h)."""
 
@@ -118,16 +145,17 @@ class Context:
             self.device = torch.device("cuda", torch.cuda.current_device())
         torch.cuda.set_device(self.device)
         self.params = params.copy()
-        self.L = _lib.lib()
-        self.stream = torch.cuda.current_stream(self.device)
-        self._ctx = self.L.pct_create(ctypes.byref(self.params), ctypes.c_void_p(self.stream.cuda_stream))
+        lib = _lib.lib()
+        self.stream = torch.cuda.Stream(self.device)
+        self._ctx = lib.pct_create(ctypes.byref(self.params), ctypes.c_void_p(self.stream.cuda_stream))
         if not self._ctx:
             raise PctError("pct_create failed")
+        self.L = _StreamOrdered(lib, self.stream, torch)
         self.V = params.columns * params.rows * params.slices
 
     def close(self):
         if getattr(self, "_ctx", None):
-            self.L.pct_destroy(self._ctx)
+            self.L._lib.pct_destroy(self._ctx)
             self._ctx = None
 
     def __del__(self):

@@ -153,6 +153,7 @@ extern "C" void pct_destroy(pct_ctx* ctx) {
         if (b->ptr) (void)hipFree(b->ptr);
     if (ctx->d_status) (void)hipFree(ctx->d_status);
     if (ctx->d_mlp_tab) (void)hipFree(ctx->d_mlp_tab);
+    if (ctx->d_scat_tab) (void)hipFree(ctx->d_scat_tab);
     if (ctx->d_stats) (void)hipFree(ctx->d_stats);
     pct_plan_destroy(ctx->plan);
     pct_bin_order_destroy(ctx->bin_order);

@@ -26,7 +26,7 @@ int pct_mlp_capacity_impl(pct_ctx* ctx, const pct_hist_soa* hist, int64_t start,
                           int64_t* total);
 int pct_mlp_paths_impl(pct_ctx* ctx, const pct_hist_soa* hist, int64_t start, int64_t count, const int64_t* offsets,
                        uint32_t* voxels, uint32_t* lengths, double* chord, uint32_t* hash_out,
-                       const uint32_t* hash_expect);
+                       const uint32_t* hash_expect, int mode);
 int pct_status_to_rc(pct_ctx* ctx, int flags);
 
 namespace {
@@ -196,7 +196,8 @@ extern "C" int pct_drop_block_update(pct_ctx* ctx, const pct_hist_soa* hist, int
                                      const float* x, float* x_update, uint32_t* counts, float lambda, int mode) {
     int rc = check_drop_hist(ctx, hist, "pct_drop_block_update");
     if (rc) return rc;
-    if (mode != PCT_MLP_FP64) return pct_fail(ctx, PCT_E_INVALID, "pct_drop_block_update: unknown mode %d", mode);
+    if (mode != PCT_MLP_FP64 && mode != PCT_MLP_TABLE)
+        return pct_fail(ctx, PCT_E_INVALID, "pct_drop_block_update: unknown mode %d", mode);
     if (!x || !x_update || !counts) return pct_fail(ctx, PCT_E_INVALID, "pct_drop_block_update: null image array");
     if (start < 0 || count < 0 || start + count > hist->n)
         return pct_fail(ctx, PCT_E_INVALID, "pct_drop_block_update: bad range");
@@ -210,7 +211,7 @@ extern "C" int pct_drop_block_update(pct_ctx* ctx, const pct_hist_soa* hist, int
     if ((rc = pct_ensure(ctx, ctx->blk_lengths, sizeof(uint32_t) * count))) return rc;
     if ((rc = pct_ensure(ctx, ctx->blk_chord, sizeof(double) * count))) return rc;
     rc = pct_mlp_paths_impl(ctx, hist, start, count, (const int64_t*)ctx->blk_offsets.ptr, (uint32_t*)ctx->blk_voxels.ptr,
-                            (uint32_t*)ctx->blk_lengths.ptr, (double*)ctx->blk_chord.ptr, nullptr, nullptr);
+                            (uint32_t*)ctx->blk_lengths.ptr, (double*)ctx->blk_chord.ptr, nullptr, nullptr, mode);
     if (rc) return rc;
     rc = launch_scatter(ctx, (const uint32_t*)ctx->blk_voxels.ptr, (const int64_t*)ctx->blk_offsets.ptr,
                         (const uint32_t*)ctx->blk_lengths.ptr, (const double*)ctx->blk_chord.ptr, hist->wepl + start,
@@ -228,7 +229,7 @@ extern "C" int pct_drop_image_update(pct_ctx* ctx, float* x, float* x_update, ui
 }
 
 /* phase 1 of an iteration: capacity offsets + path store of all histories */
-static int build_path_store(pct_ctx* ctx, const pct_hist_soa* hist) {
+static int build_path_store(pct_ctx* ctx, const pct_hist_soa* hist, int mode) {
     const int64_t n = hist->n;
     int rc;
     if ((rc = ensure_accumulators(ctx))) return rc;
@@ -241,7 +242,7 @@ static int build_path_store(pct_ctx* ctx, const pct_hist_soa* hist) {
     if ((rc = pct_ensure(ctx, ctx->chord, sizeof(double) * n))) return rc;
     if (ctx->profile) PCT_HIP_CHECK(ctx, hipEventRecord(ctx->ev[1], ctx->stream));
     rc = pct_mlp_paths_impl(ctx, hist, 0, n, (const int64_t*)ctx->offsets.ptr, (uint32_t*)ctx->path_store.ptr,
-                            (uint32_t*)ctx->path_len.ptr, (double*)ctx->chord.ptr, nullptr, nullptr);
+                            (uint32_t*)ctx->path_len.ptr, (double*)ctx->chord.ptr, nullptr, nullptr, mode);
     if (rc) return rc;
     if (ctx->profile) PCT_HIP_CHECK(ctx, hipEventRecord(ctx->ev[2], ctx->stream));
     return PCT_OK;
@@ -249,12 +250,13 @@ static int build_path_store(pct_ctx* ctx, const pct_hist_soa* hist) {
 
 /* the reference's structure on the GPU: all MLP lists, then per block an
  * atomic scatter (block_update_GPU) and a volume scan (image_update_GPU) */
-int pct_drop_iteration_atomic(pct_ctx* ctx, const pct_hist_soa* hist, float* x, int block_size, float lambda) {
+int pct_drop_iteration_atomic(pct_ctx* ctx, const pct_hist_soa* hist, float* x, int block_size, float lambda,
+                              int mode) {
     int rc = check_drop_hist(ctx, hist, "pct_drop_iteration");
     if (rc) return rc;
     const int64_t n = hist->n;
     if (n <= 0) return PCT_OK;
-    if ((rc = build_path_store(ctx, hist))) return rc;
+    if ((rc = build_path_store(ctx, hist, mode))) return rc;
     const int64_t* offs = (const int64_t*)ctx->offsets.ptr;
     const uint32_t* vox = (const uint32_t*)ctx->path_store.ptr;
     const uint32_t* len = (const uint32_t*)ctx->path_len.ptr;

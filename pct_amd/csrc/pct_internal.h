@@ -57,6 +57,9 @@ struct pct_ctx {
     /* MLP per-depth table */
     double* d_mlp_tab = nullptr;
     int mlp_tab_k = 0;
+    /* precision form: sigma_2 scattering coefficient per remaining depth */
+    double* d_scat_tab = nullptr;
+    int scat_q = 0;
     /* workspaces, grown on demand */
     pct_buffer scan_tmp, steps, offsets, path_store, path_len, chord, xupd, counts;
     pct_buffer blk_offsets, blk_voxels, blk_lengths, blk_chord;

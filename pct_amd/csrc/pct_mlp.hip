@@ -320,42 +320,188 @@ k_mlp_steps(pct_geo g, pct_hist_soa h, int64_t start, int64_t count, const doubl
     steps[k] = m.S + 1;
 }
 
-/* MLP paths. Each wavefront owns 64 consecutive histories: lane j computes the
- * per-history constants of history j, then the wave walks the histories one
- * by one with the 64 lanes on 64 consecutive steps. */
-__global__ void __launch_bounds__(256)
-k_mlp_paths(pct_geo g, pct_hist_soa h, int64_t start, int64_t count, const double* __restrict__ tab, int K,
-            const int64_t* __restrict__ offsets, uint32_t* __restrict__ voxels, uint32_t* __restrict__ lengths,
-            double* __restrict__ chord_out, int* __restrict__ status, uint32_t* __restrict__ hash_out,
-            const uint32_t* __restrict__ hash_expect) {
-    const int lane = threadIdx.x & 63;
-    const int64_t base = ((int64_t)blockIdx.x * blockDim.x + (threadIdx.x & ~63));
-    if (base >= count) return;
-    const int64_t mine = base + lane;
-    mlp_hist m;
-    uint32_t first = 0;
-    int64_t off = 0, cap = 0;
-    if (mine < count) {
-        mlp_setup(g, h, start + mine, tab + MLP_TAB_U1 * K, K, m);
-        first = h.first_voxel[start + mine];
-        off = offsets[mine];
-        cap = offsets[mine + 1] - off;
-        if (chord_out) chord_out[mine] = m.chord;
-        if (m.S >= K) atomicOr(status, PCT_STATUS_TABLE);
-    } else {
-        m.S = 0;
+__device__ __forceinline__ float bcast_f(float v, int lane) {
+    return __int_as_float(__builtin_amdgcn_readlane(__float_as_int(v), lane));
+}
+
+/* ------------------------------------------------------------------------- */
+/* Precision form: find_MLP_path_GPU_tabulated (.cu:5502-5656).               */
+/*                                                                            */
+/* The sigma integrals and both 2x2 inversions stay FP64: the u_1-only terms  */
+/* come from the per-depth table above, the sigma_2 scattering coefficient    */
+/* (E_0 (1 + 0.038 ln(d / X_0)))^2 / X_0 from a per-depth table over the      */
+/* remaining depth d = u_2 - u_1 at SCAT_SUB entries per MLP step (5e-5 cm at */
+/* 1 mm voxels, the reference's DEPTH_TABLE_STEP), indexed like the reference */
+/* (round((u_2 - u_1) / step) at the first step, then minus SCAT_SUB per      */
+/* step, .cu:5552 and 5651). The 2x2 solve is reduced to the three            */
+/* coefficients t_1 = alpha t_in + beta t_out + gamma (theta_out - theta_in)  */
+/* (v_1 likewise), which need one reciprocal per inverse instead of the five  */
+/* divisions of the literal expressions. The stepping -- t_1, v_1, the        */
+/* rotation back to x/y and the voxel index -- runs in FP32.                  */
+/*                                                                            */
+/* The depth table is stored as [r][q] with j = q SCAT_SUB + r: the 64 lanes  */
+/* of a wave, on 64 consecutive steps of one history, read 64 consecutive     */
+/* doubles (one row r, columns q_0 - k).                                      */
+/* ------------------------------------------------------------------------- */
+#define SCAT_SUB 1000
+
+struct mlp_thist {
+    double u_2, u2sq, two_u2, pre1, pre2, pre3;
+    float ca, sa, u_in, t_in, t_out, dT, v_in, v_out, dV;
+    int q0, r;
+};
+
+__device__ __forceinline__ int calc_voxel_f(float zero, float pos, float inv_size) {
+    float q = __builtin_fabsf(pos - zero) * inv_size;
+    return q < 1.0e9f ? (int)q : -1;
+}
+
+/* floats of the precision form's voxel index */
+struct grid_f {
+    float x_zero, y_zero, z_zero, inv_w, inv_h, inv_t;
+};
+
+__device__ __forceinline__ void mlp_table_setup(const pct_geo& g, const mlp_hist& m, mlp_thist& t) {
+    t.u_2 = m.u_2;
+    t.u2sq = m.u_2 * m.u_2;
+    t.two_u2 = 2 * m.u_2;
+    t.pre1 = m.pre1; t.pre2 = m.pre2; t.pre3 = m.pre3;
+    t.ca = (float)m.ca; t.sa = (float)m.sa; t.u_in = (float)m.u_in;
+    t.t_in = (float)m.t_in; t.t_out = (float)m.t_out; t.dT = (float)m.dT;
+    t.v_in = (float)m.v_in; t.v_out = (float)m.v_out; t.dV = (float)m.dV;
+    double j0 = (m.u_2 - g.mlp_u_step) / (g.mlp_u_step / SCAT_SUB) + 0.5;
+    int j = j0 > 0 ? (j0 < 2.0e9 ? (int)j0 : 2000000000) : 0;
+    t.r = j % SCAT_SUB;
+    t.q0 = j / SCAT_SUB;
+}
+
+__device__ __forceinline__ int64_t mlp_step_table(const pct_geo& g, const grid_f& gf, const mlp_thist& m,
+                                                  const double* __restrict__ tab, int K, const double* __restrict__ scat,
+                                                  int Q, int k) {
+    double u_1 = tab[MLP_TAB_U1 * K + k];
+    double S1I0 = tab[MLP_TAB_S1I0 * K + k];
+    double S1I1 = tab[MLP_TAB_S1I1 * K + k];
+    double S1I3 = tab[MLP_TAB_S1I3 * K + k];
+    double cst1 = tab[MLP_TAB_C1 * K + k];
+    double cst2 = tab[MLP_TAB_C2 * K + k];
+    double cst3 = tab[MLP_TAB_C3 * K + k];
+    int q = m.q0 - k;
+    q = q < 0 ? 0 : (q >= Q ? Q - 1 : q);
+    double c2 = scat[(int64_t)m.r * Q + q];
+    double d = m.u_2 - u_1;
+    double st2 = c2 * (m.pre1 - m.u2sq * cst1 + m.two_u2 * cst2 - cst3);
+    double stt2 = c2 * (m.pre2 - m.u_2 * cst1 + cst2);
+    double sth2 = c2 * (m.pre3 - cst1);
+    double r2 = 1.0 / (st2 * sth2 - stt2 * stt2);
+    double A = sth2 * r2, Bq = -stt2 * r2, C = st2 * r2;    /* Sigma_2^-1 = {A, Bq; Bq, C} */
+    double e = A * d + Bq;
+    double f = Bq * d + C;
+    double F0 = S1I0 + A;
+    double F1 = S1I1 + e;
+    double F3 = S1I3 + d * e + f;
+    double rF = 1.0 / (F0 * F3 - F1 * F1);
+    double I0 = F3 * rF, I1 = -F1 * rF;
+    float alpha = (float)(I0 * S1I0 + I1 * S1I1);
+    float beta = (float)(I0 * A + I1 * e);
+    float gamma = (float)(I0 * Bq + I1 * f);
+    float t_1 = alpha * m.t_in + beta * m.t_out + gamma * m.dT;
+    float v_1 = alpha * m.v_in + beta * m.v_out + gamma * m.dV;
+    float us = m.u_in + (float)u_1;
+    float x_1 = m.ca * us - m.sa * t_1;
+    float y_1 = m.sa * us + m.ca * t_1;
+    int vx = calc_voxel_f(gf.x_zero, x_1, gf.inv_w);
+    int vy = calc_voxel_f(gf.y_zero, y_1, gf.inv_h);
+    int vz = calc_voxel_f(gf.z_zero, v_1, gf.inv_t);
+    int64_t voxel = (int64_t)vx + (int64_t)vy * g.C + (int64_t)vz * g.C * g.R;
+    if (vx < 0 || vy < 0 || vz < 0 || voxel >= g.V) return -1;
+    return voxel;
+}
+
+/* The two walks as policies of one paths kernel: setup of lane j's history,
+ * its broadcast to the wave, and one step. */
+struct exact_walk {
+    const double* tab;
+    int K;
+    typedef mlp_hist H;
+    __device__ void setup(const pct_geo& g, const pct_hist_soa& h, int64_t i, H& m, double& chord, int& S) const {
+        mlp_setup(g, h, i, tab + MLP_TAB_U1 * K, K, m);
+        chord = m.chord;
+        S = m.S;
     }
-    const int nh = (int)((count - base) < 64 ? (count - base) : 64);
-    const uint64_t lt_mask = (lane == 0) ? 0ull : (~0ull >> (64 - lane));
-    int local_flags = 0;
-    for (int j = 0; j < nh; ++j) {
-        mlp_hist b;
+    __device__ H bcast(const H& m, int j) const {
+        H b;
         b.ca = bcast_d(m.ca, j); b.sa = bcast_d(m.sa, j); b.u_in = bcast_d(m.u_in, j);
         b.t_in = bcast_d(m.t_in, j); b.t_out = bcast_d(m.t_out, j); b.dT = bcast_d(m.dT, j);
         b.v_in = bcast_d(m.v_in, j); b.v_out = bcast_d(m.v_out, j); b.dV = bcast_d(m.dV, j);
         b.u_2 = bcast_d(m.u_2, j); b.pre1 = bcast_d(m.pre1, j); b.pre2 = bcast_d(m.pre2, j);
         b.pre3 = bcast_d(m.pre3, j);
-        int S = bcast_i(m.S, j);
+        return b;
+    }
+    __device__ int64_t step(const pct_geo& g, const H& b, int k) const { return mlp_step(g, b, tab, K, k); }
+};
+
+struct table_walk {
+    const double* tab;
+    int K;
+    const double* scat;
+    int Q;
+    grid_f gf;
+    typedef mlp_thist H;
+    __device__ void setup(const pct_geo& g, const pct_hist_soa& h, int64_t i, H& t, double& chord, int& S) const {
+        mlp_hist m;
+        mlp_setup(g, h, i, tab + MLP_TAB_U1 * K, K, m);
+        mlp_table_setup(g, m, t);
+        chord = m.chord;
+        S = m.S;
+    }
+    __device__ H bcast(const H& m, int j) const {
+        H b;
+        b.u_2 = bcast_d(m.u_2, j); b.u2sq = bcast_d(m.u2sq, j); b.two_u2 = bcast_d(m.two_u2, j);
+        b.pre1 = bcast_d(m.pre1, j); b.pre2 = bcast_d(m.pre2, j); b.pre3 = bcast_d(m.pre3, j);
+        b.ca = bcast_f(m.ca, j); b.sa = bcast_f(m.sa, j); b.u_in = bcast_f(m.u_in, j);
+        b.t_in = bcast_f(m.t_in, j); b.t_out = bcast_f(m.t_out, j); b.dT = bcast_f(m.dT, j);
+        b.v_in = bcast_f(m.v_in, j); b.v_out = bcast_f(m.v_out, j); b.dV = bcast_f(m.dV, j);
+        b.q0 = bcast_i(m.q0, j); b.r = bcast_i(m.r, j);
+        return b;
+    }
+    __device__ int64_t step(const pct_geo& g, const H& b, int k) const {
+        return mlp_step_table(g, gf, b, tab, K, scat, Q, k);
+    }
+};
+
+/* MLP paths. Each wavefront owns 64 consecutive histories: lane j computes the
+ * per-history constants of history j, then the wave walks the histories one
+ * by one with the 64 lanes on 64 consecutive steps. */
+template <class W>
+__global__ void __launch_bounds__(256)
+k_mlp_paths(pct_geo g, pct_hist_soa h, int64_t start, int64_t count, W walk,
+            const int64_t* __restrict__ offsets, uint32_t* __restrict__ voxels, uint32_t* __restrict__ lengths,
+            double* __restrict__ chord_out, int* __restrict__ status, uint32_t* __restrict__ hash_out,
+            const uint32_t* __restrict__ hash_expect) {
+    const int K = walk.K;
+    const int lane = threadIdx.x & 63;
+    const int64_t base = ((int64_t)blockIdx.x * blockDim.x + (threadIdx.x & ~63));
+    if (base >= count) return;
+    const int64_t mine = base + lane;
+    typename W::H m;
+    int mS = 0;
+    uint32_t first = 0;
+    int64_t off = 0, cap = 0;
+    if (mine < count) {
+        double chord;
+        walk.setup(g, h, start + mine, m, chord, mS);
+        first = h.first_voxel[start + mine];
+        off = offsets[mine];
+        cap = offsets[mine + 1] - off;
+        if (chord_out) chord_out[mine] = chord;
+        if (mS >= K) atomicOr(status, PCT_STATUS_TABLE);
+    }
+    const int nh = (int)((count - base) < 64 ? (count - base) : 64);
+    const uint64_t lt_mask = (lane == 0) ? 0ull : (~0ull >> (64 - lane));
+    int local_flags = 0;
+    for (int j = 0; j < nh; ++j) {
+        typename W::H b = walk.bcast(m, j);
+        int S = bcast_i(mS, j);
         if (S > K) S = K;
         uint32_t carry = (uint32_t)bcast_i((int)first, j);
         int64_t hoff = bcast_l(off, j);
@@ -366,7 +512,7 @@ k_mlp_paths(pct_geo g, pct_hist_soa h, int64_t start, int64_t count, const doubl
         for (int c = 0; c < S; c += 64) {
             int k = c + lane;
             int64_t vox = -1;
-            if (k < S) vox = mlp_step(g, b, tab, K, k);
+            if (k < S) vox = walk.step(g, b, k);
             bool valid = vox >= 0;
             uint64_t vmask = __ballot(valid);
             uint64_t below = vmask & lt_mask;
@@ -440,6 +586,32 @@ static int ensure_mlp_table(pct_ctx* ctx) {
     return PCT_OK;
 }
 
+/* Precision form: sigma_2 scattering coefficient at the remaining depth
+ * j * step / SCAT_SUB (generate_scattering_coefficient_table, .cu:4830-4861),
+ * stored as [r][q] with j = q SCAT_SUB + r. Q covers the per-depth table's
+ * K steps plus two. */
+static int ensure_scat_table(pct_ctx* ctx) {
+    if (ctx->d_scat_tab) return PCT_OK;
+    const pct_geo& g = ctx->g;
+    const int Q = ctx->mlp_tab_k + 2;
+    const double dstep = g.mlp_u_step / SCAT_SUB;
+    std::vector<double> tab((size_t)SCAT_SUB * Q);
+    for (int q = 0; q < Q; ++q)
+        for (int r = 0; r < SCAT_SUB; ++r) {
+            int64_t j = (int64_t)q * SCAT_SUB + r;
+            double depth = (j > 0 ? j : 1) * dstep;
+            double l = g.e_0 * (1 + 0.038 * pct_log(depth / g.x_0));
+            tab[(size_t)r * Q + q] = l * l / g.x_0;
+        }
+    double* d = nullptr;
+    if (hipMalloc(&d, sizeof(double) * tab.size()) != hipSuccess) return pct_fail(ctx, PCT_E_NOMEM, "depth table alloc");
+    hipError_t e = hipMemcpy(d, tab.data(), sizeof(double) * tab.size(), hipMemcpyHostToDevice);
+    if (e != hipSuccess) { (void)hipFree(d); return pct_fail(ctx, PCT_E_HIP, "depth table copy: %s", hipGetErrorString(e)); }
+    ctx->d_scat_tab = d;
+    ctx->scat_q = Q;
+    return PCT_OK;
+}
+
 static int check_hist(pct_ctx* ctx, const pct_hist_soa* h, int64_t start, int64_t count, const char* who) {
     if (!ctx || !h) return pct_fail(ctx, PCT_E_INVALID, "%s: null argument", who);
     if (start < 0 || count < 0 || start + count > h->n)
@@ -479,13 +651,25 @@ int pct_mlp_capacity_impl(pct_ctx* ctx, const pct_hist_soa* hist, int64_t start,
 
 int pct_mlp_paths_impl(pct_ctx* ctx, const pct_hist_soa* hist, int64_t start, int64_t count, const int64_t* offsets,
                        uint32_t* voxels, uint32_t* lengths, double* chord, uint32_t* hash_out,
-                       const uint32_t* hash_expect) {
+                       const uint32_t* hash_expect, int mode) {
     int rc = ensure_mlp_table(ctx);
     if (rc) return rc;
+    if (mode != PCT_MLP_FP64 && mode != PCT_MLP_TABLE)
+        return pct_fail(ctx, PCT_E_INVALID, "unknown MLP mode %d", mode);
+    if (mode == PCT_MLP_TABLE && (rc = ensure_scat_table(ctx))) return rc;
     if (count == 0) return PCT_OK;
-    hipLaunchKernelGGL(k_mlp_paths, dim3(pct_grid(count, 256)), dim3(256), 0, ctx->stream, ctx->g, *hist, start, count,
-                       (const double*)ctx->d_mlp_tab, ctx->mlp_tab_k, offsets, voxels, lengths, chord, ctx->d_status,
-                       hash_out, hash_expect);
+    if (mode == PCT_MLP_FP64) {
+        exact_walk w{(const double*)ctx->d_mlp_tab, ctx->mlp_tab_k};
+        hipLaunchKernelGGL(k_mlp_paths<exact_walk>, dim3(pct_grid(count, 256)), dim3(256), 0, ctx->stream, ctx->g, *hist,
+                           start, count, w, offsets, voxels, lengths, chord, ctx->d_status, hash_out, hash_expect);
+    } else {
+        const pct_geo& g = ctx->g;
+        grid_f gf{(float)g.x_zero, (float)g.y_zero, (float)g.z_zero, (float)(1.0 / g.voxel_width),
+                  (float)(1.0 / g.voxel_height), (float)(1.0 / g.voxel_thickness)};
+        table_walk w{(const double*)ctx->d_mlp_tab, ctx->mlp_tab_k, (const double*)ctx->d_scat_tab, ctx->scat_q, gf};
+        hipLaunchKernelGGL(k_mlp_paths<table_walk>, dim3(pct_grid(count, 256)), dim3(256), 0, ctx->stream, ctx->g, *hist,
+                           start, count, w, offsets, voxels, lengths, chord, ctx->d_status, hash_out, hash_expect);
+    }
     PCT_HIP_CHECK(ctx, hipGetLastError());
     return PCT_OK;
 }
@@ -512,9 +696,10 @@ extern "C" int pct_mlp_paths(pct_ctx* ctx, const pct_hist_soa* hist, int64_t sta
                              uint32_t* voxels, uint32_t* lengths, int mode) {
     int rc = check_hist(ctx, hist, start, count, "pct_mlp_paths");
     if (rc) return rc;
-    if (mode != PCT_MLP_FP64) return pct_fail(ctx, PCT_E_INVALID, "pct_mlp_paths: unknown mode %d", mode);
+    if (mode != PCT_MLP_FP64 && mode != PCT_MLP_TABLE)
+        return pct_fail(ctx, PCT_E_INVALID, "pct_mlp_paths: unknown mode %d", mode);
     if (!offsets || !voxels || !lengths) return pct_fail(ctx, PCT_E_INVALID, "pct_mlp_paths: null output");
-    rc = pct_mlp_paths_impl(ctx, hist, start, count, offsets, voxels, lengths, nullptr, nullptr, nullptr);
+    rc = pct_mlp_paths_impl(ctx, hist, start, count, offsets, voxels, lengths, nullptr, nullptr, nullptr, mode);
     if (rc) return rc;
     int flags = 0;
     rc = pct_take_status(ctx, &flags);

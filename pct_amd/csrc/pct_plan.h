@@ -27,6 +27,8 @@ struct pct_plan {
     hipGraphExec_t graph = nullptr;
     float* graph_x = nullptr;
     double graph_lambda = 0.0;
+    bool graph_f32 = false;
+    int mlp_mode = 0;              /* MLP mode the lists were built with    */
 };
 
 void pct_plan_destroy(pct_plan* pl);

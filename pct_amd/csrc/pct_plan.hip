@@ -38,9 +38,10 @@ int pct_mlp_capacity_impl(pct_ctx* ctx, const pct_hist_soa* hist, int64_t start,
                           int64_t* total);
 int pct_mlp_paths_impl(pct_ctx* ctx, const pct_hist_soa* hist, int64_t start, int64_t count, const int64_t* offsets,
                        uint32_t* voxels, uint32_t* lengths, double* chord, uint32_t* hash_out,
-                       const uint32_t* hash_expect);
+                       const uint32_t* hash_expect, int mode);
 int pct_status_to_rc(pct_ctx* ctx, int flags);
-int pct_drop_iteration_atomic(pct_ctx* ctx, const pct_hist_soa* hist, float* x, int block_size, float lambda);
+int pct_drop_iteration_atomic(pct_ctx* ctx, const pct_hist_soa* hist, float* x, int block_size, float lambda,
+                              int mode);
 
 namespace {
 
@@ -163,6 +164,62 @@ k_block_apply(const uint32_t* __restrict__ seg_voxel, const uint32_t* __restrict
     x[v] = (float)((double)x[v] + upd / (double)(e - a));
 }
 
+/* Precision form of the update: the same two kernels in FP32 with
+ * Kahan-compensated sums (a_i.x per history, x_update per voxel). */
+__device__ __forceinline__ void kahan_add(float& s, float& c, float v) {
+    float y = v - c;
+    float t = s + y;
+    c = (t - s) - y;
+    s = t;
+}
+
+__global__ void __launch_bounds__(64 * DELTA_WAVES)
+k_block_delta_f32(const uint32_t* __restrict__ voxels, const int64_t* __restrict__ cap_off,
+                  const uint32_t* __restrict__ len, const double* __restrict__ chord, const float* __restrict__ wepl,
+                  int64_t h0, int64_t count, const float* __restrict__ x, float lambda, float* __restrict__ delta) {
+    const int lane = threadIdx.x & 63;
+    const int64_t k = (int64_t)blockIdx.x * DELTA_WAVES + (threadIdx.x >> 6);
+    if (k >= count) return;
+    const int64_t h = h0 + k;
+    const int n = (int)len[h];
+    const uint32_t* p = voxels + cap_off[h];
+    float s = 0.0f, c = 0.0f;
+#pragma unroll
+    for (int r = 0; r < DELTA_SLOTS; ++r) {
+        int j = lane + 64 * r;
+        if (j < n) kahan_add(s, c, x[p[j]]);
+    }
+    for (int j = lane + 64 * DELTA_SLOTS; j < n; j += 64) kahan_add(s, c, x[p[j]]);
+#pragma unroll
+    for (int d = 32; d >= 1; d >>= 1) {
+        float os = __shfl_xor(s, d, 64), oc = __shfl_xor(c, d, 64);
+        c += oc;
+        kahan_add(s, c, os);
+    }
+    if (lane == 0) {
+        const float L = (float)chord[h];
+        const float a_i_dot_x_k = s * L;
+        const float a_i_dot_a_i = (L * L) * (float)n;
+        delta[k] = L * ((wepl[h] - a_i_dot_x_k) / a_i_dot_a_i) * lambda;
+    }
+}
+
+__global__ void __launch_bounds__(256)
+k_block_apply_f32(const uint32_t* __restrict__ seg_voxel, const uint32_t* __restrict__ seg_start,
+                  const uint16_t* __restrict__ csc, int64_t s0, int64_t ns, int64_t entry0, uint32_t n_entries,
+                  const float* __restrict__ delta, float* __restrict__ x) {
+    int64_t k = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (k >= ns) return;
+    int64_t s = s0 + k;
+    uint32_t a = seg_start[s];
+    uint32_t e = (k + 1 < ns) ? seg_start[s + 1] : n_entries;
+    const uint16_t* c = csc + entry0;
+    float upd = 0.0f, comp = 0.0f;
+    for (uint32_t i = a; i < e; ++i) kahan_add(upd, comp, delta[c[i]]);
+    uint32_t v = seg_voxel[s];
+    x[v] = x[v] + upd / (float)(e - a);
+}
+
 }  // namespace
 
 /* ------------------------------------------------------------------------- */
@@ -189,7 +246,7 @@ static int vbits_for(int64_t V) {
 }
 
 /* Build (or rebuild) the plan of `hist` with blocks of B histories. */
-static int plan_build(pct_ctx* ctx, const pct_hist_soa* hist, int B) {
+static int plan_build(pct_ctx* ctx, const pct_hist_soa* hist, int B, int mlp_mode) {
     const int64_t n = hist->n;
     if (B <= 0 || B > 65536) return pct_fail(ctx, PCT_E_INVALID, "DROP block size %d outside [1, 65536]", B);
     const int vbits = vbits_for(ctx->g.V);
@@ -211,7 +268,7 @@ static int plan_build(pct_ctx* ctx, const pct_hist_soa* hist, int B) {
     uint32_t* vox = (uint32_t*)ctx->path_store.ptr;
     uint32_t* len = (uint32_t*)ctx->path_len.ptr;
     if ((rc = pct_mlp_paths_impl(ctx, hist, 0, n, cap_off, vox, len, (double*)ctx->chord.ptr, (uint32_t*)pl->hash.ptr,
-                                 nullptr)))
+                                 nullptr, mlp_mode)))
         return rc;
     int flags = 0;
     if ((rc = pct_take_status(ctx, &flags))) return rc;
@@ -331,6 +388,7 @@ static int plan_build(pct_ctx* ctx, const pct_hist_soa* hist, int B) {
     pl->E = E;
     pl->S = S_total;
     pl->key = *hist;
+    pl->mlp_mode = mlp_mode;
     pl->valid = true;
     return PCT_OK;
 }
@@ -341,7 +399,7 @@ static bool same_hist(const pct_hist_soa& a, const pct_hist_soa& b) {
 }
 
 /* the block loop of one iteration on the plan */
-static int plan_block_loop(pct_ctx* ctx, pct_plan* pl, const pct_hist_soa* hist, float* x, double lambda) {
+static int plan_block_loop(pct_ctx* ctx, pct_plan* pl, const pct_hist_soa* hist, float* x, double lambda, bool f32) {
     const int64_t B = pl->B;
     const uint32_t* vox = (const uint32_t*)ctx->path_store.ptr;
     const int64_t* cap_off = (const int64_t*)ctx->offsets.ptr;
@@ -350,12 +408,20 @@ static int plan_block_loop(pct_ctx* ctx, pct_plan* pl, const pct_hist_soa* hist,
     double* delta = (double*)pl->delta.ptr;
     for (int64_t b = 0; b < pl->nb; ++b) {
         int64_t h0 = b * B, cnt = (pl->n - h0) < B ? (pl->n - h0) : B;
-        hipLaunchKernelGGL(k_block_delta, dim3((unsigned)((cnt + DELTA_WAVES - 1) / DELTA_WAVES)), dim3(64 * DELTA_WAVES), 0,
-                           ctx->stream, vox, cap_off, len, chord, (const float*)hist->wepl, h0, cnt, (const float*)x,
-                           lambda, delta);
+        dim3 dgrid((unsigned)((cnt + DELTA_WAVES - 1) / DELTA_WAVES));
+        if (f32)
+            hipLaunchKernelGGL(k_block_delta_f32, dgrid, dim3(64 * DELTA_WAVES), 0, ctx->stream, vox, cap_off, len, chord,
+                               (const float*)hist->wepl, h0, cnt, (const float*)x, (float)lambda, (float*)delta);
+        else
+            hipLaunchKernelGGL(k_block_delta, dgrid, dim3(64 * DELTA_WAVES), 0, ctx->stream, vox, cap_off, len, chord,
+                               (const float*)hist->wepl, h0, cnt, (const float*)x, lambda, delta);
         int64_t s0 = pl->blk_seg_h[b], ns = pl->blk_seg_h[b + 1] - s0;
         uint32_t ne = (uint32_t)(pl->blk_entry_h[b + 1] - pl->blk_entry_h[b]);
-        if (ns > 0)
+        if (ns > 0 && f32)
+            hipLaunchKernelGGL(k_block_apply_f32, dim3(pct_grid(ns, 256)), dim3(256), 0, ctx->stream,
+                               (const uint32_t*)pl->seg_voxel.ptr, (const uint32_t*)pl->seg_start.ptr,
+                               (const uint16_t*)pl->csc.ptr, s0, ns, pl->blk_entry_h[b], ne, (const float*)delta, x);
+        else if (ns > 0)
             hipLaunchKernelGGL(k_block_apply, dim3(pct_grid(ns, 256)), dim3(256), 0, ctx->stream,
                                (const uint32_t*)pl->seg_voxel.ptr, (const uint32_t*)pl->seg_start.ptr,
                                (const uint16_t*)pl->csc.ptr, s0, ns, pl->blk_entry_h[b], ne, (const double*)delta, x);
@@ -364,12 +430,12 @@ static int plan_block_loop(pct_ctx* ctx, pct_plan* pl, const pct_hist_soa* hist,
     return PCT_OK;
 }
 
-static int plan_run_graph(pct_ctx* ctx, pct_plan* pl, const pct_hist_soa* hist, float* x, double lambda) {
-    if (!pl->graph || pl->graph_x != x || pl->graph_lambda != lambda) {
+static int plan_run_graph(pct_ctx* ctx, pct_plan* pl, const pct_hist_soa* hist, float* x, double lambda, bool f32) {
+    if (!pl->graph || pl->graph_x != x || pl->graph_lambda != lambda || pl->graph_f32 != f32) {
         plan_release_graph(pl);
         hipGraph_t g = nullptr;
         PCT_HIP_CHECK(ctx, hipStreamBeginCapture(ctx->stream, hipStreamCaptureModeThreadLocal));
-        int rc = plan_block_loop(ctx, pl, hist, x, lambda);
+        int rc = plan_block_loop(ctx, pl, hist, x, lambda, f32);
         hipError_t e = hipStreamEndCapture(ctx->stream, &g);
         if (rc) { if (g) (void)hipGraphDestroy(g); return rc; }
         if (e != hipSuccess) return pct_fail(ctx, PCT_E_HIP, "graph capture: %s", hipGetErrorString(e));
@@ -378,6 +444,7 @@ static int plan_run_graph(pct_ctx* ctx, pct_plan* pl, const pct_hist_soa* hist, 
         if (e != hipSuccess) { pl->graph = nullptr; return pct_fail(ctx, PCT_E_HIP, "graph instantiate: %s", hipGetErrorString(e)); }
         pl->graph_x = x;
         pl->graph_lambda = lambda;
+        pl->graph_f32 = f32;
     }
     PCT_HIP_CHECK(ctx, hipGraphLaunch(pl->graph, ctx->stream));
     return PCT_OK;
@@ -386,23 +453,27 @@ static int plan_run_graph(pct_ctx* ctx, pct_plan* pl, const pct_hist_soa* hist, 
 extern "C" int pct_drop_prepare(pct_ctx* ctx, const pct_hist_soa* hist, int block_size, int mode) {
     if (!ctx || !hist || !hist->wepl || !hist->first_voxel || !hist->x_entry)
         return pct_fail(ctx, PCT_E_INVALID, "pct_drop_prepare: null argument");
-    if ((mode & PCT_MLP_MODE_MASK) != PCT_MLP_FP64) return pct_fail(ctx, PCT_E_INVALID, "pct_drop_prepare: unknown MLP mode");
+    const int mlp = mode & PCT_MLP_MODE_MASK;
+    if (mlp != PCT_MLP_FP64 && mlp != PCT_MLP_TABLE)
+        return pct_fail(ctx, PCT_E_INVALID, "pct_drop_prepare: unknown MLP mode %d", mlp);
     if (hist->n <= 0) return pct_fail(ctx, PCT_E_INVALID, "pct_drop_prepare: no histories");
-    return plan_build(ctx, hist, block_size);
+    return plan_build(ctx, hist, block_size, mlp);
 }
 
 extern "C" int pct_drop_iteration(pct_ctx* ctx, const pct_hist_soa* hist, float* x, int block_size, float lambda, int mode) {
     if (!ctx || !hist || !hist->wepl || !hist->first_voxel || !hist->x_entry)
         return pct_fail(ctx, PCT_E_INVALID, "pct_drop_iteration: null argument");
-    if ((mode & PCT_MLP_MODE_MASK) != PCT_MLP_FP64) return pct_fail(ctx, PCT_E_INVALID, "pct_drop_iteration: unknown MLP mode");
+    const int mlp = mode & PCT_MLP_MODE_MASK;
+    if (mlp != PCT_MLP_FP64 && mlp != PCT_MLP_TABLE)
+        return pct_fail(ctx, PCT_E_INVALID, "pct_drop_iteration: unknown MLP mode %d", mlp);
     if (!x || block_size <= 0) return pct_fail(ctx, PCT_E_INVALID, "pct_drop_iteration: bad x or block size");
     if (hist->n <= 0) return PCT_OK;
-    if (mode & PCT_DROP_ATOMIC) return pct_drop_iteration_atomic(ctx, hist, x, block_size, lambda);
+    if (mode & PCT_DROP_ATOMIC) return pct_drop_iteration_atomic(ctx, hist, x, block_size, lambda, mlp);
     int rc;
     pct_plan* pl = ctx->plan;
     bool fresh = false;
-    if (!pl || !pl->valid || pl->B != block_size || !same_hist(pl->key, *hist)) {
-        if ((rc = plan_build(ctx, hist, block_size))) return rc;
+    if (!pl || !pl->valid || pl->B != block_size || pl->mlp_mode != mlp || !same_hist(pl->key, *hist)) {
+        if ((rc = plan_build(ctx, hist, block_size, mlp))) return rc;
         pl = ctx->plan;
         fresh = true;
     }
@@ -413,12 +484,13 @@ extern "C" int pct_drop_iteration(pct_ctx* ctx, const pct_hist_soa* hist, float*
          * same and check the lists against the plan */
         rc = pct_mlp_paths_impl(ctx, hist, 0, hist->n, (const int64_t*)ctx->offsets.ptr, (uint32_t*)ctx->path_store.ptr,
                                 (uint32_t*)ctx->path_len.ptr, (double*)ctx->chord.ptr, nullptr,
-                                (const uint32_t*)pl->hash.ptr);
+                                (const uint32_t*)pl->hash.ptr, mlp);
         if (rc) return rc;
     }
     if (ctx->profile) PCT_HIP_CHECK(ctx, hipEventRecord(ctx->ev[2], ctx->stream));
-    if (mode & PCT_DROP_NO_GRAPH) rc = plan_block_loop(ctx, pl, hist, x, (double)lambda);
-    else rc = plan_run_graph(ctx, pl, hist, x, (double)lambda);
+    const bool f32 = (mode & PCT_DROP_F32) != 0;
+    if (mode & PCT_DROP_NO_GRAPH) rc = plan_block_loop(ctx, pl, hist, x, (double)lambda, f32);
+    else rc = plan_run_graph(ctx, pl, hist, x, (double)lambda, f32);
     if (rc) return rc;
     if (ctx->profile) {
         PCT_HIP_CHECK(ctx, hipEventRecord(ctx->ev[3], ctx->stream));
@@ -434,7 +506,7 @@ extern "C" int pct_drop_stats(pct_ctx* ctx, const pct_hist_soa* hist, int block_
     if (!ctx || !hist || !sum_path || !sum_touched) return pct_fail(ctx, PCT_E_INVALID, "pct_drop_stats: null argument");
     pct_plan* pl = ctx->plan;
     if (!pl || !pl->valid || pl->B != block_size || !same_hist(pl->key, *hist)) {
-        int rc = plan_build(ctx, hist, block_size);
+        int rc = plan_build(ctx, hist, block_size, pl ? pl->mlp_mode : PCT_MLP_FP64);
         if (rc) return rc;
         pl = ctx->plan;
     }
