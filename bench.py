@@ -25,7 +25,7 @@ sys.path.insert(0, ROOT)
 HBM_PEAK_GBS = 8000.0        # MI355X HBM3E, GB/s
 FP64_PEAK_TFLOPS = 78.6      # MI355X FP64, TFLOP/s
 # FP64 operations of one MLP step of the reference loop body (.cu:5376-5491):
-# +,-,*,/ and each log/pow counted once ("Measurement").
+# +,-,*,/ and each log/pow counted once.
 MLP_FLOPS_PER_STEP = 235
 
 
@@ -55,8 +55,8 @@ def main():
     args = ap.parse_args()
 
     if args.gpus != 1:
-        print(json.dumps({"error": "this build runs on one GPU only; multi-GPU sharding is described in "
-                                   "DESIGN.md but not built", "n_gpus": args.gpus}))
+        print(json.dumps({"error": "one GPU only: the multi-GPU sharding of the DROP blocks is designed but "
+                                   "not built", "n_gpus": args.gpus}))
         sys.exit(2)
 
     import numpy as np
@@ -101,14 +101,15 @@ def main():
     log(f"setup {time.time() - t0:.1f}s (preprocess {t_pre:.2f}s, plan {t_plan:.2f}s): N={scfg.n_histories} N_r={n_r} "
         f"mean n={sum_n / n_r:.1f} sum|V_B|={sum_vb} steps={sum_steps}")
 
-    def timed(mode, steps):
+    def timed(mode, steps, img=None):
+        img = x if img is None else img
         phases = []
         torch.cuda.synchronize()
         ev0 = torch.cuda.Event(enable_timing=True)
         ev1 = torch.cuda.Event(enable_timing=True)
         ev0.record(ctx.stream)
         for _ in range(steps):
-            ctx.drop_iteration(hist, x, B, p.lambda_, mode)
+            ctx.drop_iteration(hist, img, B, p.lambda_, mode)
             phases.append(pct_amd.last_timings(ctx))
         ev1.record(ctx.stream)
         torch.cuda.synchronize()
@@ -132,7 +133,46 @@ def main():
     ms_atomic, _ = timed(pct_amd.PCT_DROP_ATOMIC, 1)
     log(f"cached-list iteration {ms_cached:.2f} ms, reference-structure (atomic) iteration {ms_atomic:.2f} ms")
 
-    # algorithmic work per iteration (DESIGN.md "Measurement")
+    # ---- precision form: tabulated MLP + FP32 stepping + FP32/Kahan update ----
+    # RSP error vs the exact FP64 form after iterations 1, 6 and 12 from x0 (the
+    # exact form equals the CPU oracle bit for bit, see rmse_vs_oracle below)
+    snaps = (1, 6, 12)
+    ref = {}
+    xe = pre.x0.clone()
+    for it in range(1, snaps[-1] + 1):
+        ctx.drop_iteration(hist, xe, B, p.lambda_)
+        if it in snaps:
+            ref[it] = xe.cpu().numpy()
+    t1 = time.time()
+    ctx.drop_prepare(hist, B, pct_amd.PCT_MLP_TABLE)
+    ctx.sync()
+    t_plan_p = time.time() - t1
+    xs = pre.x0.clone()
+    for _ in range(args.warmup):
+        ctx.drop_iteration(hist, xs, B, p.lambda_, pct_amd.PCT_PRECISION)
+    ms_prec, ph_p = timed(pct_amd.PCT_PRECISION, args.steps, xs)
+    ms_prec /= args.steps
+    _, mlp_ms_p, blk_ms_p = ph_p.mean(axis=0)
+    in_hull = x0_np != 0
+    prec_err = {}
+    xp = pre.x0.clone()
+    for it in range(1, snaps[-1] + 1):
+        ctx.drop_iteration(hist, xp, B, p.lambda_, pct_amd.PCT_PRECISION)
+        if it in snaps:
+            d = xp.cpu().numpy().astype(np.float64) - ref[it].astype(np.float64)
+            r = float(np.sqrt(np.mean(d[in_hull] ** 2)))
+            prec_err[f"iteration_{it}"] = {"rmse": r, "rmse_rel": r / float(np.mean(np.abs(ref[it][in_hull]))),
+                                           "max_abs": float(np.max(np.abs(d)))}
+    log(f"precision form: {ms_prec:.2f} ms/iter (MLP {mlp_ms_p:.2f} ms, blocks {blk_ms_p:.2f} ms), error {prec_err}")
+    precision = {"histories_per_s": round(n_r / (ms_prec / 1e3), 1), "ms_per_iteration": round(ms_prec, 3),
+                 "phases_ms": {"mlp": round(float(mlp_ms_p), 3), "blocks": round(float(blk_ms_p), 3),
+                               "plan_setup_once": round(t_plan_p * 1e3, 1)},
+                 "speedup_vs_fp64_form": round(ms_per_step / ms_prec, 3),
+                 "error_vs_fp64": prec_err,
+                 "error_region": "voxels of the hull (x0 != 0); reference image: exact FP64 form"}
+
+    # algorithmic work per iteration: 48 B per history record, 4 B gather + 4 B
+    # scatter per path entry, 24 B per voxel touched by a block
     bytes_update = 48.0 * n_r + 8.0 * sum_n + 24.0 * sum_vb
     upd_gbs = bytes_update / (blk_ms / 1e3) / 1e9
     mlp_tflops = MLP_FLOPS_PER_STEP * float(sum_steps) / (mlp_ms / 1e3) / 1e12
@@ -206,6 +246,7 @@ def main():
                    "drop_blocks": (n_r + B - 1) // B},
         "phases_ms": {"mlp": round(float(mlp_ms), 3), "blocks": round(float(blk_ms), 3),
                       "plan_setup_once": round(t_plan * 1e3, 1), "preprocess_once": round(t_pre * 1e3, 1)},
+        "precision_form": precision,
         "other_forms": {"cached_lists_histories_per_s": round(n_r / (ms_cached / 1e3), 1),
                         "cached_lists_ms": round(ms_cached, 3),
                         "atomic_reference_structure_histories_per_s": round(n_r / (ms_atomic / 1e3), 1),

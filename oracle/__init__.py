@@ -38,12 +38,13 @@ def lib():
         L.oracle_mlp_path.argtypes = [P(abi.PctParams), P(abi.PctHistSoa), ctypes.c_int64, vp, P(ctypes.c_int)]
         L.oracle_mlp_paths.argtypes = [P(abi.PctParams), P(abi.PctHistSoa), ctypes.c_int64, ctypes.c_int64, vp,
                                        ctypes.c_int64, vp]
+        L.oracle_mlp_paths_table.argtypes = L.oracle_mlp_paths.argtypes
         L.oracle_drop.argtypes = [P(abi.PctParams), P(abi.PctHistSoa), vp, ctypes.c_int, ctypes.c_float, ctypes.c_int,
                                   ctypes.c_int64]
         L.oracle_drop_mt.argtypes = [P(abi.PctParams), P(abi.PctHistSoa), vp, ctypes.c_int, ctypes.c_float,
                                      ctypes.c_int, ctypes.c_int64, ctypes.c_int]
         L.oracle_mlp_trace.argtypes = [P(abi.PctParams), P(abi.PctHistSoa), ctypes.c_int64, vp, vp, vp, vp, vp,
-                                       ctypes.c_int, P(ctypes.c_int)]
+                                       ctypes.c_int, P(ctypes.c_int), ctypes.c_int]
         L.oracle_drop_lists.argtypes = [ctypes.c_int64, ctypes.c_int64, vp, vp, vp, vp, vp, ctypes.c_int,
                                         ctypes.c_float, ctypes.c_int]
         L.oracle_binning.argtypes = [P(abi.PctParams), P(abi.PctHistSoa), vp, vp, vp, vp, vp, vp]
@@ -269,25 +270,31 @@ def mlp_path(p, h: HostHistories, i: int):
     return buf[:n].copy(), skipped.value
 
 
-def mlp_paths(p, h: HostHistories, start=0, count=None, stride=None):
-    """Voxel lists of histories [start, start+count) as a list of arrays."""
+def mlp_paths(p, h: HostHistories, start=0, count=None, stride=None, table=False):
+    """Voxel lists of histories [start, start+count) as a list of arrays
+    (table=True: the precision form, D4)."""
     count = h.n - start if count is None else count
+    if start < 0 or count < 0 or start + count > h.n:
+        raise ValueError(f"histories [{start}, {start + count}) outside [0, {h.n})")
     stride = p.max_intersections if stride is None else stride
     vox = np.zeros(count * stride, np.uint32)
     lens = np.zeros(count, np.uint32)
     hs = h.soa()
-    rc = lib().oracle_mlp_paths(ctypes.byref(p), ctypes.byref(hs), start, count, _p(vox), stride, _p(lens))
+    fn = lib().oracle_mlp_paths_table if table else lib().oracle_mlp_paths
+    rc = fn(ctypes.byref(p), ctypes.byref(hs), start, count, _p(vox), stride, _p(lens))
     if rc != 0:
         raise OverflowError("MLP path overflow in the oracle")
     return [vox[k * stride:k * stride + lens[k]].copy() for k in range(count)]
 
 
-def mlp_trace(p, h: HostHistories, i: int, cap=4096):
-    """Per-step (u_1, t_1, v_1, x_1, y_1) of the MLP loop of history i, and the path length."""
+def mlp_trace(p, h: HostHistories, i: int, cap=4096, table=False):
+    """Per-step (u_1, t_1, v_1, x_1, y_1) of the MLP loop of history i, and the
+    path length (table=True: the precision form, whose t_1.. are FP32)."""
     arrs = [np.zeros(cap) for _ in range(5)]
     npath = ctypes.c_int(0)
     hs = h.soa()
-    k = lib().oracle_mlp_trace(ctypes.byref(p), ctypes.byref(hs), i, *[_p(a) for a in arrs], cap, ctypes.byref(npath))
+    k = lib().oracle_mlp_trace(ctypes.byref(p), ctypes.byref(hs), i, *[_p(a) for a in arrs], cap, ctypes.byref(npath),
+                               1 if table else 0)
     u1, t1, v1, x1, y1 = (a[:k] for a in arrs)
     return dict(u1=u1, t1=t1, v1=v1, x1=x1, y1=y1, n_path=npath.value)
 

@@ -402,6 +402,143 @@ int find_MLP_path(const Geo& g, uint32_t* path, uint32_t first_voxel,
     return num_intersections;
 }
 
+/* ------------------------------------------------------------------------ */
+/* D4: the precision form, after find_MLP_path_GPU_tabulated (.cu:5502-5656) */
+/*                                                                          */
+/* As in the tabulated reference kernel: the sigma integrals and the 2x2    */
+/* inversions are FP64; the sigma_2 scattering coefficient is read at the   */
+/* remaining depth rounded to a table of step STEP/1000 (5e-5 cm at 1 mm    */
+/* voxels, DEPTH_TABLE_STEP), the index being rounded once at the first     */
+/* step and decremented by 1000 per step (.cu:5552, 5651). Choices of this  */
+/* form that differ from the reference's tabulated kernel: sin/cos of the   */
+/* entry angle and the u_2 polynomials are evaluated exactly instead of     */
+/* from 0.001-degree and 5e-5 cm tables; the 2x2 solve is reduced to the    */
+/* coefficients alpha, beta, gamma of t_1 = alpha t_in + beta t_out + gamma */
+/* (theta_out - theta_in) (one reciprocal per inverse); t_1, v_1, the       */
+/* rotation back to x/y and the voxel index are FP32 (voxel index by        */
+/* multiplication with the FP32 reciprocal of the voxel size). The table    */
+/* index is clamped to the depths a chord inside the image can have.        */
+/* ------------------------------------------------------------------------ */
+const int SCAT_SUB = 1000;
+
+inline int calculate_voxel_f(float zero, float pos, float inv_size) {
+    float q = fabsf(pos - zero) * inv_size;
+    return q < 1.0e9f ? (int)q : -1;
+}
+
+inline double depth_coefficient(const Geo& g, int64_t j) {
+    double depth = (j > 0 ? j : 1) * (g.mlp_u_step / SCAT_SUB);
+    double l = g.e_0 * (1 + 0.038 * pct_log(depth / g.x_0));
+    return l * l / g.x_0;
+}
+
+int find_MLP_path_table(const Geo& g, uint32_t* path, uint32_t first_voxel,
+                        double x_in_object, double y_in_object, double z_in_object,
+                        double x_out_object, double y_out_object, double z_out_object,
+                        double xy_entry_angle, double xz_entry_angle, double xy_exit_angle, double xz_exit_angle,
+                        int* skipped, MlpTrace* trace = nullptr) {
+    const int C = g.C, R = g.R;
+    const double STEP = g.mlp_u_step;
+    const int K = (int)(2.0 * 1.4142135623730951 * g.radius / STEP) + 8;   /* steps a chord can take */
+    const int Q = K + 2;
+    const float x_zero = (float)g.x_zero, y_zero = (float)g.y_zero, z_zero = (float)g.z_zero;
+    const float inv_w = (float)(1.0 / g.voxel_width), inv_h = (float)(1.0 / g.voxel_height);
+    const float inv_t = (float)(1.0 / g.voxel_thickness);
+    int num_intersections = 0;
+    if (skipped) *skipped = 0;
+    path[num_intersections++] = first_voxel;
+
+    double ca = pct_cos(xy_entry_angle), sa = pct_sin(xy_entry_angle);
+    double u_in_object = (ca * x_in_object) + (sa * y_in_object);
+    double u_out_object = (ca * x_out_object) + (sa * y_out_object);
+    double t_in_object = (ca * y_in_object) - (sa * x_in_object);
+    double t_out_object = (ca * y_out_object) - (sa * x_out_object);
+    if (u_in_object > u_out_object) {
+        xy_entry_angle += PCT_PI;
+        xy_exit_angle += PCT_PI;
+        ca = pct_cos(xy_entry_angle); sa = pct_sin(xy_entry_angle);
+        u_in_object = (ca * x_in_object) + (sa * y_in_object);
+        u_out_object = (ca * x_out_object) + (sa * y_out_object);
+        t_in_object = (ca * y_in_object) - (sa * x_in_object);
+        t_out_object = (ca * y_out_object) - (sa * x_out_object);
+    }
+    const float f_ca = (float)ca, f_sa = (float)sa, f_u_in = (float)u_in_object;
+    const float f_t_in = (float)t_in_object, f_t_out = (float)t_out_object;
+    const float f_dT = (float)(xy_exit_angle - xy_entry_angle);
+    const float f_v_in = (float)z_in_object, f_v_out = (float)z_out_object;
+    const float f_dV = (float)(xz_exit_angle - xz_entry_angle);
+
+    double u_0 = 0.0, u_1 = STEP;
+    double u_2 = pct_fabs(u_out_object - u_in_object);
+    double sigma_2_pre_1 = u_2 * u_2 * u_2 * (g.a0_3 + u_2 * (g.a1_12 + u_2 * (g.a2_30 + u_2 * (g.a3_60 + u_2 * (g.a4_105 + u_2 * g.a5_168)))));
+    double sigma_2_pre_2 = u_2 * u_2 * (g.a0_2 + u_2 * (g.a1_6 + u_2 * (g.a2_12 + u_2 * (g.a3_20 + u_2 * (g.a4_30 + u_2 * g.a5_42)))));
+    double sigma_2_pre_3 = u_2 * (g.A0 + u_2 * (g.a1_2 + u_2 * (g.a2_3 + u_2 * (g.a3_4 + u_2 * (g.a4_5 + u_2 * g.a5_6)))));
+    double u_2_sq = u_2 * u_2, two_u_2 = 2 * u_2;
+    /* sigma_2 table index: rounded at the first step, then -SCAT_SUB per step */
+    double j_first = (u_2 - STEP) / (STEP / SCAT_SUB) + 0.5;
+    int j0 = j_first > 0 ? (j_first < 2.0e9 ? (int)j_first : 2000000000) : 0;
+    const int row = j0 % SCAT_SUB, q_first = j0 / SCAT_SUB;
+
+    for (int k = 0; u_1 < u_2 - STEP; ++k) {
+        double l1 = g.e_0 * (1 + 0.038 * pct_log((u_1 - u_0) / g.x_0));
+        double sigma_1_coefficient = l1 * l1 / g.x_0;
+        double sigma_t1 = sigma_1_coefficient * (u_1 * u_1 * u_1 * (g.a0_3 + u_1 * (g.a1_12 + u_1 * (g.a2_30 + u_1 * (g.a3_60 + u_1 * (g.a4_105 + u_1 * g.a5_168))))));
+        double sigma_t1_theta1 = sigma_1_coefficient * (u_1 * u_1 * (g.a0_2 + u_1 * (g.a1_6 + u_1 * (g.a2_12 + u_1 * (g.a3_20 + u_1 * (g.a4_30 + u_1 * g.a5_42))))));
+        double sigma_theta1 = sigma_1_coefficient * (u_1 * (g.A0 + u_1 * (g.a1_2 + u_1 * (g.a2_3 + u_1 * (g.a3_4 + u_1 * (g.a4_5 + u_1 * g.a5_6))))));
+        double determinant_Sigma_1 = sigma_t1 * sigma_theta1 - sigma_t1_theta1 * sigma_t1_theta1;
+        double S1I0 = sigma_theta1 / determinant_Sigma_1;
+        double S1I1 = -sigma_t1_theta1 / determinant_Sigma_1;
+        double S1I3 = sigma_t1 / determinant_Sigma_1;
+        double common_1 = u_1 * (g.A0 + u_1 * (g.a1_2 + u_1 * (g.a2_3 + u_1 * (g.a3_4 + u_1 * (g.a4_5 + u_1 * g.a5_6)))));
+        double common_2 = u_1 * u_1 * (g.a0_2 + u_1 * (g.a1_3 + u_1 * (g.a2_4 + u_1 * (g.a3_5 + u_1 * (g.a4_6 + u_1 * g.a5_7)))));
+        double common_3 = u_1 * u_1 * u_1 * (g.a0_3 + u_1 * (g.a1_4 + u_1 * (g.a2_5 + u_1 * (g.a3_6 + u_1 * (g.a4_7 + u_1 * g.a5_8)))));
+
+        int q = q_first - k;
+        q = q < 0 ? 0 : (q >= Q ? Q - 1 : q);
+        double c2 = depth_coefficient(g, (int64_t)q * SCAT_SUB + row);
+        double d = u_2 - u_1;
+        double sigma_t2 = c2 * (sigma_2_pre_1 - u_2_sq * common_1 + two_u_2 * common_2 - common_3);
+        double sigma_t2_theta2 = c2 * (sigma_2_pre_2 - u_2 * common_1 + common_2);
+        double sigma_theta2 = c2 * (sigma_2_pre_3 - common_1);
+        double r2 = 1.0 / (sigma_t2 * sigma_theta2 - sigma_t2_theta2 * sigma_t2_theta2);
+        double S2I0 = sigma_theta2 * r2, S2I1 = -sigma_t2_theta2 * r2, S2I3 = sigma_t2 * r2;
+        /* (Sigma_1^-1 + R_1^T Sigma_2^-1 R_1), R_1 = {1, d; 0, 1}; symmetric */
+        double e = S2I0 * d + S2I1;
+        double f = S2I1 * d + S2I3;
+        double F0 = S1I0 + S2I0;
+        double F1 = S1I1 + e;
+        double F3 = S1I3 + d * e + f;
+        double rF = 1.0 / (F0 * F3 - F1 * F1);
+        double I0 = F3 * rF, I1 = -F1 * rF;
+        float alpha = (float)(I0 * S1I0 + I1 * S1I1);
+        float beta = (float)(I0 * S2I0 + I1 * e);
+        float gamma = (float)(I0 * S2I1 + I1 * f);
+        float t_1 = alpha * f_t_in + beta * f_t_out + gamma * f_dT;
+        float v_1 = alpha * f_v_in + beta * f_v_out + gamma * f_dV;
+        float u_shifted = f_u_in + (float)u_1;
+        float x_1 = f_ca * u_shifted - f_sa * t_1;
+        float y_1 = f_sa * u_shifted + f_ca * t_1;
+        if (trace && trace->n < trace->cap) {
+            trace->u1[trace->n] = u_1; trace->t1[trace->n] = t_1; trace->v1[trace->n] = v_1;
+            trace->x1[trace->n] = x_1; trace->y1[trace->n] = y_1;
+            trace->n++;
+        }
+
+        int voxel_x = calculate_voxel_f(x_zero, x_1, inv_w);
+        int voxel_y = calculate_voxel_f(y_zero, y_1, inv_h);
+        int voxel_z = calculate_voxel_f(z_zero, v_1, inv_t);
+        int64_t voxel = (int64_t)voxel_x + (int64_t)voxel_y * C + (int64_t)voxel_z * C * R;
+        if (voxel_x < 0 || voxel_y < 0 || voxel_z < 0 || voxel >= g.V) {
+            if (skipped) (*skipped)++;
+        } else if ((uint32_t)voxel != path[num_intersections - 1]) {
+            if (num_intersections >= g.max_intersections) return -1;
+            path[num_intersections++] = (uint32_t)voxel;
+        }
+        u_1 += STEP;
+    }
+    return num_intersections;
+}
+
 }  // namespace
 
 /* ======================================================================== */
@@ -548,16 +685,37 @@ int oracle_mlp_paths(const pct_params* p, const pct_hist_soa* h, int64_t start, 
     return rc;
 }
 
-/* D2 with the per-step record (u_1, t_1, v_1, x_1, y_1) of every loop
- * iteration; returns the number of steps recorded (path length in *n_path). */
+/* D4 (precision form) for histories [start, start+count), same layout. */
+int oracle_mlp_paths_table(const pct_params* p, const pct_hist_soa* h, int64_t start, int64_t count,
+                           uint32_t* voxels, int64_t stride, uint32_t* lengths) {
+    Geo g = make_geo(*p);
+    std::vector<uint32_t> buf(g.max_intersections + 1);
+    int rc = 0;
+    for (int64_t k = 0; k < count; ++k) {
+        int64_t i = start + k;
+        int n = find_MLP_path_table(g, buf.data(), h->first_voxel[i], h->x_entry[i], h->y_entry[i], h->z_entry[i],
+                                    h->x_exit[i], h->y_exit[i], h->z_exit[i], h->xy_entry_angle[i],
+                                    h->xz_entry_angle[i], h->xy_exit_angle[i], h->xz_exit_angle[i], nullptr);
+        if (n < 0) { rc = -3; n = 0; }
+        if (n > stride) { rc = -3; n = (int)stride; }
+        memcpy(voxels + k * stride, buf.data(), sizeof(uint32_t) * n);
+        lengths[k] = (uint32_t)n;
+    }
+    return rc;
+}
+
+/* D2 (table=0) or D4 (table=1) with the per-step record (u_1, t_1, v_1, x_1,
+ * y_1) of every loop iteration; returns the number of steps recorded (path
+ * length in *n_path). */
 int oracle_mlp_trace(const pct_params* p, const pct_hist_soa* h, int64_t i, double* u1, double* t1, double* v1,
-                     double* x1, double* y1, int cap, int* n_path) {
+                     double* x1, double* y1, int cap, int* n_path, int table) {
     Geo g = make_geo(*p);
     std::vector<uint32_t> path(g.max_intersections + 1);
     MlpTrace tr{u1, t1, v1, x1, y1, cap, 0};
-    int n = find_MLP_path(g, path.data(), h->first_voxel[i], h->x_entry[i], h->y_entry[i], h->z_entry[i],
-                          h->x_exit[i], h->y_exit[i], h->z_exit[i], h->xy_entry_angle[i], h->xz_entry_angle[i],
-                          h->xy_exit_angle[i], h->xz_exit_angle[i], nullptr, &tr);
+    auto walk = table ? find_MLP_path_table : find_MLP_path;
+    int n = walk(g, path.data(), h->first_voxel[i], h->x_entry[i], h->y_entry[i], h->z_entry[i],
+                 h->x_exit[i], h->y_exit[i], h->z_exit[i], h->xy_entry_angle[i], h->xz_entry_angle[i],
+                 h->xy_exit_angle[i], h->xz_exit_angle[i], nullptr, &tr);
     if (n_path) *n_path = n;
     return tr.n;
 }

@@ -10,10 +10,11 @@ import ctypes
 
 from . import _lib
 from . import configs
-from ._lib import (PCT_DROP_ATOMIC, PCT_DROP_CACHED, PCT_DROP_NO_GRAPH, PCT_MLP_FP64, PctHistSoa, PctParams,
-                   PctRawSoa, PctSynthCfg, PctSynthSoa)
+from ._lib import (PCT_DROP_ATOMIC, PCT_DROP_CACHED, PCT_DROP_F32, PCT_DROP_NO_GRAPH, PCT_MLP_FP64, PCT_MLP_TABLE,
+                   PCT_PRECISION, PctHistSoa, PctParams, PctRawSoa, PctSynthCfg, PctSynthSoa)
 
-__all__ = ["Context", "Histories", "RawScan", "configs", "PCT_MLP_FP64", "generate_host"]
+__all__ = ["Context", "Histories", "RawScan", "configs", "PCT_MLP_FP64", "PCT_MLP_TABLE", "PCT_PRECISION",
+           "generate_host"]
 
 
 class PctError(RuntimeError):

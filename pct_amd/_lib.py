@@ -116,9 +116,13 @@ SIGNATURES = {
 }
 
 PCT_MLP_FP64 = 0
+PCT_MLP_TABLE = 1
 PCT_DROP_CACHED = 0x100
 PCT_DROP_ATOMIC = 0x200
 PCT_DROP_NO_GRAPH = 0x400
+PCT_DROP_F32 = 0x800
+# the precision form: tabulated MLP with FP32 stepping + FP32/Kahan update
+PCT_PRECISION = PCT_MLP_TABLE | PCT_DROP_F32
 
 _lib = None
 
