@@ -238,15 +238,30 @@ int pct_drop_iteration(pct_ctx* ctx, const pct_hist_soa* hist, float* x,
 
 /* Measurement (counterpart of the reference's per-iteration clock() and
  * cudaEvent timing, .cu:6773-6849): when profiling is on, pct_drop_iteration
- * records HIP events on the context stream and pct_last_timings returns the
- * last iteration's phase durations in ms: [0] path capacities (prefix sum),
- * [1] MLP voxel lists of all histories, [2] the sequential DROP block loop. */
+ * records HIP events and pct_last_timings returns, for the last iteration, in
+ * ms: [0] from its start until the block loop may start (the MLP lists of the
+ * first chunk of blocks), [1] from its start until the MLP lists of all
+ * histories are done, [2] from the start of the block loop to the end of the
+ * iteration. The MLP and the block loop overlap (chunk by chunk), so the
+ * iteration takes [0] + [2]; without a MLP pass [0] = [1] = 0. */
 int pct_set_profiling(pct_ctx* ctx, int on);
 int pct_last_timings(pct_ctx* ctx, double* ms3);
 /* Work statistics of one DROP iteration (x is not modified): *sum_path =
  * sum over histories of the path length n_i, *sum_touched = sum over blocks of
  * the number of distinct voxels the block touches (|V_B|). */
 int pct_drop_stats(pct_ctx* ctx, const pct_hist_soa* hist, int block_size, int64_t* sum_path, int64_t* sum_touched);
+
+/* Execution knobs of pct_drop_iteration (no counterpart in the reference,
+ * whose launch shapes are compile-time constants, data.h:377-378). They change
+ * scheduling only, never results. value <= 0 restores the default. */
+#define PCT_TUNE_MLP_WAVES     1  /* resident MLP wavefronts per SIMD while the
+                                     block loop runs beside it (default: the
+                                     kernel's occupancy limit)                 */
+#define PCT_TUNE_CHUNK_FIRST   2  /* blocks in the first chunk (default 16)    */
+#define PCT_TUNE_CHUNK_MAX     3  /* largest chunk in blocks (default 256)     */
+#define PCT_TUNE_OVERLAP       4  /* 1: MLP beside the block loop (default),
+                                     2: MLP first, then the block loop         */
+int pct_set_tuning(pct_ctx* ctx, int key, int value);
 
 /* Diagnostics (no counterpart in the reference): evaluates one function of
  * the shared FP64/FP32 math library (pct_math.h) on the device for n inputs,

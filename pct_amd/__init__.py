@@ -360,6 +360,19 @@ def last_timings(ctx: "Context"):
     return list(out)
 
 
+TUNING_KEYS = {"mlp_waves": _lib.PCT_TUNE_MLP_WAVES, "chunk_first": _lib.PCT_TUNE_CHUNK_FIRST,
+               "chunk_max": _lib.PCT_TUNE_CHUNK_MAX, "overlap": _lib.PCT_TUNE_OVERLAP}
+
+
+def set_tuning(ctx: "Context", **knobs):
+    """Execution knobs of the DROP iteration (pct_set_tuning): mlp_waves,
+    chunk_first, chunk_max, overlap (1 beside, 2 sequential); 0 = default."""
+    for k, v in knobs.items():
+        if k not in TUNING_KEYS:
+            raise PctError(f"unknown tuning knob {k}")
+        ctx._check(ctx.L.pct_set_tuning(ctx._ctx, TUNING_KEYS[k], int(v)), "pct_set_tuning")
+
+
 def drop_stats(ctx: "Context", hist: "Histories", block_size: int):
     """(sum of path lengths, sum over blocks of touched voxels) of one iteration."""
     a, b = ctypes.c_int64(0), ctypes.c_int64(0)

@@ -110,6 +110,7 @@ SIGNATURES = {
     "pct_set_profiling": (ctypes.c_int, [c_vp, ctypes.c_int]),
     "pct_last_timings": (ctypes.c_int, [c_vp, c_vp]),
     "pct_drop_stats": (ctypes.c_int, [c_vp, P(PctHistSoa), ctypes.c_int, P(c_i64), P(c_i64)]),
+    "pct_set_tuning": (ctypes.c_int, [c_vp, ctypes.c_int, ctypes.c_int]),
     "pct_selftest_math": (ctypes.c_int, [c_vp, ctypes.c_int, c_vp, c_vp, c_vp, c_i64]),
     "pct_synth_generate": (ctypes.c_int, [c_vp, P(PctSynthCfg), c_i64, c_i64, P(PctSynthSoa)]),
     "pct_synth_generate_host": (ctypes.c_int, [P(PctSynthCfg), c_i64, c_i64, P(PctSynthSoa)]),
@@ -121,6 +122,11 @@ PCT_DROP_CACHED = 0x100
 PCT_DROP_ATOMIC = 0x200
 PCT_DROP_NO_GRAPH = 0x400
 PCT_DROP_F32 = 0x800
+# pct_set_tuning keys
+PCT_TUNE_MLP_WAVES = 1
+PCT_TUNE_CHUNK_FIRST = 2
+PCT_TUNE_CHUNK_MAX = 3
+PCT_TUNE_OVERLAP = 4
 # the precision form: tabulated MLP with FP32 stepping + FP32/Kahan update
 PCT_PRECISION = PCT_MLP_TABLE | PCT_DROP_F32
 

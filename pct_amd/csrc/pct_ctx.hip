@@ -159,6 +159,10 @@ extern "C" void pct_destroy(pct_ctx* ctx) {
     pct_bin_order_destroy(ctx->bin_order);
     for (hipEvent_t& e : ctx->ev)
         if (e) (void)hipEventDestroy(e);
+    if (ctx->aux_stream) {
+        (void)hipStreamSynchronize(ctx->aux_stream);
+        (void)hipStreamDestroy(ctx->aux_stream);
+    }
     if (ctx->own_stream && ctx->stream) (void)hipStreamDestroy(ctx->stream);
     delete ctx;
 }

@@ -240,10 +240,11 @@ static int build_path_store(pct_ctx* ctx, const pct_hist_soa* hist, int mode) {
     if ((rc = pct_ensure(ctx, ctx->path_store, sizeof(uint32_t) * (total > 0 ? total : 1)))) return rc;
     if ((rc = pct_ensure(ctx, ctx->path_len, sizeof(uint32_t) * n))) return rc;
     if ((rc = pct_ensure(ctx, ctx->chord, sizeof(double) * n))) return rc;
-    if (ctx->profile) PCT_HIP_CHECK(ctx, hipEventRecord(ctx->ev[1], ctx->stream));
     rc = pct_mlp_paths_impl(ctx, hist, 0, n, (const int64_t*)ctx->offsets.ptr, (uint32_t*)ctx->path_store.ptr,
                             (uint32_t*)ctx->path_len.ptr, (double*)ctx->chord.ptr, nullptr, nullptr, mode);
     if (rc) return rc;
+    /* sequential phases: the block loop starts when all lists are done */
+    if (ctx->profile) PCT_HIP_CHECK(ctx, hipEventRecord(ctx->ev[1], ctx->stream));
     if (ctx->profile) PCT_HIP_CHECK(ctx, hipEventRecord(ctx->ev[2], ctx->stream));
     return PCT_OK;
 }
@@ -291,9 +292,10 @@ extern "C" int pct_last_timings(pct_ctx* ctx, double* ms3) {
     if (!ctx || !ms3) return PCT_E_INVALID;
     if (!ctx->ev_recorded) return pct_fail(ctx, PCT_E_INVALID, "pct_last_timings: no profiled iteration");
     PCT_HIP_CHECK(ctx, hipEventSynchronize(ctx->ev[3]));
+    const int from[3] = {0, 0, 1}, to[3] = {1, 2, 3};
     for (int k = 0; k < 3; ++k) {
         float ms = 0.f;
-        PCT_HIP_CHECK(ctx, hipEventElapsedTime(&ms, ctx->ev[k], ctx->ev[k + 1]));
+        PCT_HIP_CHECK(ctx, hipEventElapsedTime(&ms, ctx->ev[from[k]], ctx->ev[to[k]]));
         ms3[k] = ms;
     }
     return PCT_OK;

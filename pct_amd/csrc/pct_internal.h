@@ -50,6 +50,9 @@ struct pct_ctx {
     pct_geo g;
     hipStream_t stream = nullptr;
     bool own_stream = false;
+    /* second stream of the context (created on first use): the MLP lists of
+     * a DROP iteration run on it ahead of the block loop */
+    hipStream_t aux_stream = nullptr;
     int device = 0;
     std::string err;
     /* device-side status word written by kernels (overflow flags etc.) */
@@ -60,6 +63,12 @@ struct pct_ctx {
     /* precision form: sigma_2 scattering coefficient per remaining depth */
     double* d_scat_tab = nullptr;
     int scat_q = 0;
+    /* resident workgroups per CU of the MLP paths kernel per walk, and the
+     * CU count (0 = not queried) */
+    int mlp_occ[2] = {0, 0};
+    int cus = 0;
+    /* execution knobs (pct_set_tuning; 0 = default) */
+    int tune_mlp_waves = 0, tune_chunk_first = 0, tune_chunk_max = 0, tune_overlap = 0;
     /* workspaces, grown on demand */
     pct_buffer scan_tmp, steps, offsets, path_store, path_len, chord, xupd, counts;
     pct_buffer blk_offsets, blk_voxels, blk_lengths, blk_chord;

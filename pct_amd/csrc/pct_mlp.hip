@@ -480,8 +480,12 @@ k_mlp_paths(pct_geo g, pct_hist_soa h, int64_t start, int64_t count, W walk,
             const uint32_t* __restrict__ hash_expect) {
     const int K = walk.K;
     const int lane = threadIdx.x & 63;
-    const int64_t base = ((int64_t)blockIdx.x * blockDim.x + (threadIdx.x & ~63));
-    if (base >= count) return;
+    const uint64_t lt_mask = (lane == 0) ? 0ull : (~0ull >> (64 - lane));
+    int local_flags = 0;
+    /* grid-stride over groups of 64 histories: the grid is sized to the
+     * resident waves, so a launch has no partial last round */
+    for (int64_t base = ((int64_t)blockIdx.x * blockDim.x + (threadIdx.x & ~63)); base < count;
+         base += (int64_t)gridDim.x * blockDim.x) {
     const int64_t mine = base + lane;
     typename W::H m;
     int mS = 0;
@@ -497,8 +501,6 @@ k_mlp_paths(pct_geo g, pct_hist_soa h, int64_t start, int64_t count, W walk,
         if (mS >= K) atomicOr(status, PCT_STATUS_TABLE);
     }
     const int nh = (int)((count - base) < 64 ? (count - base) : 64);
-    const uint64_t lt_mask = (lane == 0) ? 0ull : (~0ull >> (64 - lane));
-    int local_flags = 0;
     for (int j = 0; j < nh; ++j) {
         typename W::H b = walk.bcast(m, j);
         int S = bcast_i(mS, j);
@@ -541,6 +543,7 @@ k_mlp_paths(pct_geo g, pct_hist_soa h, int64_t start, int64_t count, W walk,
                 if (hash_expect && hash_expect[base + j] != hsh) local_flags |= PCT_STATUS_PLAN;
             }
         }
+    }
     }
     if (lane == 0 && local_flags) atomicOr(status, local_flags);
 }
@@ -649,29 +652,50 @@ int pct_mlp_capacity_impl(pct_ctx* ctx, const pct_hist_soa* hist, int64_t start,
     return pct_exclusive_scan_i32(ctx, steps, count, offsets, total);
 }
 
-int pct_mlp_paths_impl(pct_ctx* ctx, const pct_hist_soa* hist, int64_t start, int64_t count, const int64_t* offsets,
-                       uint32_t* voxels, uint32_t* lengths, double* chord, uint32_t* hash_out,
-                       const uint32_t* hash_expect, int mode) {
+/* internal: the MLP lists of histories [start, start+count) on `stream`;
+ * offsets/lengths/chord/hash arrays are indexed from `start` (entry 0 is
+ * history start) */
+int pct_mlp_paths_on(pct_ctx* ctx, hipStream_t stream, const pct_hist_soa* hist, int64_t start, int64_t count,
+                     const int64_t* offsets, uint32_t* voxels, uint32_t* lengths, double* chord, uint32_t* hash_out,
+                     const uint32_t* hash_expect, int mode) {
     int rc = ensure_mlp_table(ctx);
     if (rc) return rc;
     if (mode != PCT_MLP_FP64 && mode != PCT_MLP_TABLE)
         return pct_fail(ctx, PCT_E_INVALID, "unknown MLP mode %d", mode);
     if (mode == PCT_MLP_TABLE && (rc = ensure_scat_table(ctx))) return rc;
     if (count == 0) return PCT_OK;
+    if (!ctx->cus) {
+        /* resident workgroups of each walk: the grid-stride launch size */
+        PCT_HIP_CHECK(ctx, hipDeviceGetAttribute(&ctx->cus, hipDeviceAttributeMultiprocessorCount, ctx->device));
+        PCT_HIP_CHECK(ctx, hipOccupancyMaxActiveBlocksPerMultiprocessor(&ctx->mlp_occ[0], k_mlp_paths<exact_walk>, 256, 0));
+        PCT_HIP_CHECK(ctx, hipOccupancyMaxActiveBlocksPerMultiprocessor(&ctx->mlp_occ[1], k_mlp_paths<table_walk>, 256, 0));
+        ctx->cus = std::max(1, ctx->cus);
+    }
+    /* a workgroup of 256 is one wavefront on each of the CU's 4 SIMDs */
+    int per_cu = std::max(1, ctx->mlp_occ[mode == PCT_MLP_FP64 ? 0 : 1]);
+    if (ctx->tune_mlp_waves > 0 && ctx->tune_mlp_waves < per_cu) per_cu = ctx->tune_mlp_waves;
+    const unsigned grid = std::min(pct_grid(count, 256), (unsigned)(ctx->cus * per_cu));
     if (mode == PCT_MLP_FP64) {
         exact_walk w{(const double*)ctx->d_mlp_tab, ctx->mlp_tab_k};
-        hipLaunchKernelGGL(k_mlp_paths<exact_walk>, dim3(pct_grid(count, 256)), dim3(256), 0, ctx->stream, ctx->g, *hist,
+        hipLaunchKernelGGL(k_mlp_paths<exact_walk>, dim3(grid), dim3(256), 0, stream, ctx->g, *hist,
                            start, count, w, offsets, voxels, lengths, chord, ctx->d_status, hash_out, hash_expect);
     } else {
         const pct_geo& g = ctx->g;
         grid_f gf{(float)g.x_zero, (float)g.y_zero, (float)g.z_zero, (float)(1.0 / g.voxel_width),
                   (float)(1.0 / g.voxel_height), (float)(1.0 / g.voxel_thickness)};
         table_walk w{(const double*)ctx->d_mlp_tab, ctx->mlp_tab_k, (const double*)ctx->d_scat_tab, ctx->scat_q, gf};
-        hipLaunchKernelGGL(k_mlp_paths<table_walk>, dim3(pct_grid(count, 256)), dim3(256), 0, ctx->stream, ctx->g, *hist,
+        hipLaunchKernelGGL(k_mlp_paths<table_walk>, dim3(grid), dim3(256), 0, stream, ctx->g, *hist,
                            start, count, w, offsets, voxels, lengths, chord, ctx->d_status, hash_out, hash_expect);
     }
     PCT_HIP_CHECK(ctx, hipGetLastError());
     return PCT_OK;
+}
+
+int pct_mlp_paths_impl(pct_ctx* ctx, const pct_hist_soa* hist, int64_t start, int64_t count, const int64_t* offsets,
+                       uint32_t* voxels, uint32_t* lengths, double* chord, uint32_t* hash_out,
+                       const uint32_t* hash_expect, int mode) {
+    return pct_mlp_paths_on(ctx, ctx->stream, hist, start, count, offsets, voxels, lengths, chord, hash_out,
+                            hash_expect, mode);
 }
 
 int pct_status_to_rc(pct_ctx* ctx, int flags) {

@@ -15,20 +15,37 @@ struct pct_plan {
     int64_t nb = 0;                /* blocks                                */
     int64_t E = 0;                 /* path entries (sum of list lengths)    */
     int64_t S = 0;                 /* segments = sum over blocks of |V_B|   */
+    int mlp_mode = 0;              /* MLP mode the lists were built with    */
+    /* path lists, block-strided: history k of block b at
+     * list[list_base[b] + k * stride[b]], unused slots hold the sentinel */
+    pct_buffer list;               /* u32                                   */
+    pct_buffer hist_off;           /* int64 per history+1: list offsets     */
+    pct_buffer len;                /* u32 per history: list length          */
+    pct_buffer chord;              /* f64 per history: effective chord      */
     pct_buffer hash;               /* u32 per history: hash of its list     */
-    pct_buffer ex_off;             /* int64 per history+1: exact offsets    */
-    pct_buffer blk_entry;          /* int64 per block+1: first entry        */
+    /* block transpose: per block the touched voxels sorted by voxel, each a
+     * 16-byte record {voxel, count | history indices}; csc holds all
+     * block-local history indices in (block, voxel, history) order for the
+     * records with more than the inline number */
+    pct_buffer segs;               /* uint4 per segment                     */
+    pct_buffer csc;                /* u16 per entry                         */
     pct_buffer blk_seg;            /* int64 per block+1: first segment      */
-    pct_buffer seg_voxel;          /* u32 per segment                       */
-    pct_buffer seg_start;          /* u32 per segment, block-relative entry */
-    pct_buffer csc;                /* u16 per entry: block-local history    */
+    pct_buffer blk_entry;          /* int64 per block+1: first csc entry    */
     pct_buffer delta;              /* f64 per history of one block          */
-    std::vector<int64_t> blk_entry_h, blk_seg_h;
-    hipGraphExec_t graph = nullptr;
+    std::vector<int64_t> blk_entry_h, blk_seg_h, list_base_h;
+    std::vector<int32_t> stride_h;
+    /* the block loop in chunks of blocks [chunk_b[c], chunk_b[c+1]), one
+     * hipGraph each; chunk c may start when the MLP lists of its histories are
+     * done (ev_chunk[c]) */
+    std::vector<int64_t> chunk_b;
+    int64_t chunk_first = 0, chunk_max = 0;
+    std::vector<hipGraphExec_t> graphs;
+    std::vector<hipEvent_t> ev_chunk;
+    hipEvent_t ev_start = nullptr;
     float* graph_x = nullptr;
+    const float* graph_wepl = nullptr;
     double graph_lambda = 0.0;
     bool graph_f32 = false;
-    int mlp_mode = 0;              /* MLP mode the lists were built with    */
 };
 
 void pct_plan_destroy(pct_plan* pl);

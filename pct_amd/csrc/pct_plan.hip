@@ -4,13 +4,16 @@
  * The voxel lists of the MLP do not depend on the image, so the structure of
  * every DROP block -- which voxels it touches, how many of its histories touch
  * each, and which ones -- is the same in every iteration. The plan stores it
- * once per scan, next to the path store:
+ * once per scan:
  *
- *   path store   CSR by history (u32 voxel ids, capacity offsets), lengths,
- *                effective chord lengths, a hash of every list;
- *   block CSC    per DROP block, the distinct voxels it touches sorted by
- *                voxel (u32), each with the block-local indices of the
- *                histories that touch it, in history order (u16).
+ *   path lists   block-strided: history k of block b at list_base[b] +
+ *                k * stride[b], stride[b] = the longest list of the block,
+ *                unused slots hold a sentinel; lengths, effective chord
+ *                lengths and a hash of every list;
+ *   block CSC    per DROP block the distinct voxels it touches, sorted by
+ *                voxel, as 16-byte records {voxel, count, up to 5 block-local
+ *                history indices in history order}; longer segments point
+ *                into a u16 array of all indices.
  *
  * It is built with one stable radix sort by (block, voxel) over chunks of
  * blocks. An iteration is then two gather-only kernels per block, with no
@@ -24,12 +27,23 @@
  *                  x += x_update / count (DROP_blocks + DROP_update,
  *                  .cu:8023-8055) -- the oracle's accumulation order.
  *
- * Segments are sorted by voxel, so the image reads/writes of k_block_apply are
- * monotone in address. The whole block loop of an iteration is captured once
- * into a hipGraph and replayed. In the default mode each iteration first
- * recomputes every MLP voxel list (as the reference does, .cu:5822) and checks
- * it against the plan's hash.
+ * The block loop is a chain of ~2 x 4.7k dependent launches per iteration at
+ * 200x200x32, so its cost is launch latency plus the dependent memory round
+ * trips of each kernel. The layout is chosen so that each kernel has exactly
+ * two: the delta kernel finds a history's list from the launch arguments
+ * (block base + k * stride) and reads the list and the history's scalars in
+ * one round, then gathers x; the apply kernel reads one record per voxel
+ * (voxel + inline history indices) and then the deltas and the old x[v]
+ * together. The block loop is captured once into hipGraphs of chunks of
+ * blocks and replayed. In the default mode every iteration recomputes every MLP
+ * voxel list (as the reference does, .cu:5822) into the plan's list store and
+ * checks it against the plan's hash; that FP64-bound pass runs on a second
+ * stream, chunk by chunk ahead of the latency-bound block loop, and the graph
+ * of a chunk waits for its own lists only. Blocks still run one after the
+ * other in storage order and every block reads the lists computed in the
+ * same iteration.
  */
+#include <algorithm>
 #include <rocprim/device/device_radix_sort.hpp>
 #include "pct_internal.h"
 #include "pct_plan.h"
@@ -42,6 +56,12 @@ int pct_mlp_paths_impl(pct_ctx* ctx, const pct_hist_soa* hist, int64_t start, in
 int pct_status_to_rc(pct_ctx* ctx, int flags);
 int pct_drop_iteration_atomic(pct_ctx* ctx, const pct_hist_soa* hist, float* x, int block_size, float lambda,
                               int mode);
+int pct_mlp_paths_on(pct_ctx* ctx, hipStream_t stream, const pct_hist_soa* hist, int64_t start, int64_t count,
+                     const int64_t* offsets, uint32_t* voxels, uint32_t* lengths, double* chord, uint32_t* hash_out,
+                     const uint32_t* hash_expect, int mode);
+
+#define LIST_SENTINEL 0xFFFFFFFFu
+#define SEG_INLINE 5
 
 namespace {
 
@@ -53,19 +73,56 @@ __global__ void k_lengths_i32(const uint32_t* __restrict__ len, int64_t n, int32
     if (i < n) out[i] = (int32_t)len[i];
 }
 
-/* block entry bases: blk_entry[b] = exact offset of history b*B */
-__global__ void k_block_bases(const int64_t* __restrict__ ex_off, int64_t n, int64_t B, int64_t nb,
-                              int64_t* __restrict__ blk_entry) {
-    int64_t b = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
-    if (b > nb) return;
-    int64_t h = b * B < n ? b * B : n;
-    blk_entry[b] = ex_off[h];
+/* longest list of each block: one workgroup of 256 per block */
+__global__ void __launch_bounds__(256)
+k_block_max_len(const uint32_t* __restrict__ len, int64_t n, int64_t B, int32_t* __restrict__ out) {
+    __shared__ uint32_t sm[256];
+    const int64_t b = blockIdx.x;
+    const int64_t h0 = b * B, h1 = (h0 + B) < n ? (h0 + B) : n;
+    uint32_t m = 0;
+    for (int64_t h = h0 + threadIdx.x; h < h1; h += blockDim.x) m = len[h] > m ? len[h] : m;
+    sm[threadIdx.x] = m;
+    __syncthreads();
+    for (int s = 128; s > 0; s >>= 1) {
+        if ((int)threadIdx.x < s) sm[threadIdx.x] = sm[threadIdx.x] > sm[threadIdx.x + s] ? sm[threadIdx.x] : sm[threadIdx.x + s];
+        __syncthreads();
+    }
+    if (threadIdx.x == 0) out[b] = (int32_t)sm[0];
+}
+
+/* hist_off[h] = list_base[b] + (h - b B) stride[b]; hist_off[n] = list size */
+__global__ void k_hist_offsets(const int64_t* __restrict__ list_base, const int32_t* __restrict__ stride, int64_t n,
+                               int64_t B, int64_t nb, int64_t* __restrict__ hist_off) {
+    int64_t h = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (h > n) return;
+    if (h == n) { hist_off[n] = list_base[nb]; return; }
+    int64_t b = h / B;
+    hist_off[h] = list_base[b] + (h - b * B) * stride[b];
+}
+
+__global__ void k_fill_u32(uint32_t* __restrict__ p, int64_t n, uint32_t v) {
+    int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (i < n) p[i] = v;
+}
+
+/* one wavefront per history: copy its list from the capacity-strided build
+ * store into the block-strided list store */
+__global__ void __launch_bounds__(256)
+k_copy_lists(const uint32_t* __restrict__ src, const int64_t* __restrict__ cap_off, const uint32_t* __restrict__ len,
+             const int64_t* __restrict__ hist_off, int64_t n, uint32_t* __restrict__ dst) {
+    int64_t h = (int64_t)blockIdx.x * 4 + (threadIdx.x >> 6);
+    int lane = threadIdx.x & 63;
+    if (h >= n) return;
+    int m = (int)len[h];
+    const uint32_t* s = src + cap_off[h];
+    uint32_t* d = dst + hist_off[h];
+    for (int j = lane; j < m; j += 64) d[j] = s[j];
 }
 
 /* one wavefront per history of the chunk: keys = (local block << vbits) | voxel,
  * values = block-local history index, at the history's exact entry offset */
 __global__ void __launch_bounds__(256)
-k_emit_keys(const uint32_t* __restrict__ voxels, const int64_t* __restrict__ cap_off,
+k_emit_keys(const uint32_t* __restrict__ list, const int64_t* __restrict__ hist_off,
             const uint32_t* __restrict__ len, const int64_t* __restrict__ ex_off, int64_t h0, int64_t h1,
             int64_t B, int64_t b0, int vbits, int64_t entry0, uint32_t* __restrict__ keys,
             uint32_t* __restrict__ vals) {
@@ -76,9 +133,9 @@ k_emit_keys(const uint32_t* __restrict__ voxels, const int64_t* __restrict__ cap
     uint32_t hi = (uint32_t)(b - b0) << vbits;
     uint32_t local = (uint32_t)(h - b * B);
     int n = (int)len[h];
-    int64_t src = cap_off[h], dst = ex_off[h] - entry0;
+    int64_t src = hist_off[h], dst = ex_off[h] - entry0;
     for (int j = lane; j < n; j += 64) {
-        keys[dst + j] = hi | voxels[src + j];
+        keys[dst + j] = hi | list[src + j];
         vals[dst + j] = local;
     }
 }
@@ -88,26 +145,39 @@ __global__ void k_head_flags(const uint32_t* __restrict__ keys, int64_t E, int32
     if (i < E) flags[i] = (i == 0 || keys[i] != keys[i - 1]) ? 1 : 0;
 }
 
-/* segments of the chunk: voxel and block-relative start of each, the CSC
- * values, and the first segment of each block */
+/* segments of the chunk: the csc values of every entry, and at each segment
+ * head the 16-byte record and, for a block's first segment, blk_seg */
 __global__ void k_write_segments(const uint32_t* __restrict__ keys, const uint32_t* __restrict__ vals,
                                  const int32_t* __restrict__ flags, const int64_t* __restrict__ segidx, int64_t E,
                                  int vbits, int64_t seg0, int64_t b0, int64_t entry0,
-                                 const int64_t* __restrict__ blk_entry, uint32_t* __restrict__ seg_voxel,
-                                 uint32_t* __restrict__ seg_start, uint16_t* __restrict__ csc,
-                                 int64_t* __restrict__ blk_seg) {
+                                 const int64_t* __restrict__ blk_entry, uint4* __restrict__ segs,
+                                 uint16_t* __restrict__ csc, int64_t* __restrict__ blk_seg) {
     int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
     if (i >= E) return;
     uint32_t k = keys[i];
     int64_t b = b0 + (int64_t)(k >> vbits);
     int64_t gi = entry0 + i;
     csc[gi] = (uint16_t)vals[i];
-    if (flags[i]) {
-        int64_t s = seg0 + segidx[i];
-        seg_voxel[s] = k & ((1u << vbits) - 1u);
-        seg_start[s] = (uint32_t)(gi - blk_entry[b]);
-        if (gi == blk_entry[b]) blk_seg[b] = s;
+    if (!flags[i]) return;
+    int64_t s = seg0 + segidx[i];
+    int64_t j = i + 1;
+    while (j < E && !flags[j]) ++j;               /* a new block always starts a new segment */
+    uint32_t c = (uint32_t)(j - i);
+    uint4 r;
+    r.x = k & ((1u << vbits) - 1u);
+    if (c <= SEG_INLINE) {
+        uint32_t id[SEG_INLINE] = {0u, 0u, 0u, 0u, 0u};
+        for (uint32_t q = 0; q < c; ++q) id[q] = vals[i + q] & 0xffffu;
+        r.y = c | (id[0] << 16);
+        r.z = id[1] | (id[2] << 16);
+        r.w = id[3] | (id[4] << 16);
+    } else {
+        r.y = 0u;                                   /* count 0 marks a record with its indices in csc */
+        r.z = (uint32_t)(gi - blk_entry[b]);        /* block-relative first entry */
+        r.w = c;
     }
+    segs[s] = r;
+    if (gi == blk_entry[b]) blk_seg[b] = s;
 }
 
 /* ------------------------------------------------------------------------- */
@@ -122,46 +192,73 @@ __device__ __forceinline__ double wave_sum_d(double v) {
     return v;
 }
 
+/* list points at the block's first list; history k's list is at k * stride */
 __global__ void __launch_bounds__(64 * DELTA_WAVES)
-k_block_delta(const uint32_t* __restrict__ voxels, const int64_t* __restrict__ cap_off,
-              const uint32_t* __restrict__ len, const double* __restrict__ chord, const float* __restrict__ wepl,
-              int64_t h0, int64_t count, const float* __restrict__ x, double lambda, double* __restrict__ delta) {
+k_block_delta(const uint32_t* __restrict__ list, int stride, const uint32_t* __restrict__ len,
+              const double* __restrict__ chord, const float* __restrict__ wepl, int64_t h0, int64_t count,
+              const float* __restrict__ x, double lambda, double* __restrict__ delta) {
+    __builtin_amdgcn_s_setprio(3);   /* latency-bound: issue ahead of the MLP waves sharing the CU */
     const int lane = threadIdx.x & 63;
     const int64_t k = (int64_t)blockIdx.x * DELTA_WAVES + (threadIdx.x >> 6);
     if (k >= count) return;
     const int64_t h = h0 + k;
-    const int n = (int)len[h];
-    const uint32_t* p = voxels + cap_off[h];
-    double s = 0.0;
+    const uint32_t n = len[h];
+    const double L = chord[h];
+    const float b_i = wepl[h];
+    const uint32_t* p = list + k * stride;
+    uint32_t v[DELTA_SLOTS];
 #pragma unroll
     for (int r = 0; r < DELTA_SLOTS; ++r) {
         int j = lane + 64 * r;
-        if (j < n) s += (double)x[p[j]];
+        v[r] = j < stride ? p[j] : LIST_SENTINEL;
     }
-    for (int j = lane + 64 * DELTA_SLOTS; j < n; j += 64) s += (double)x[p[j]];
+    double s = 0.0;
+#pragma unroll
+    for (int r = 0; r < DELTA_SLOTS; ++r)
+        if (v[r] != LIST_SENTINEL) s += (double)x[v[r]];
+    for (int j = lane + 64 * DELTA_SLOTS; j < stride; j += 64) {
+        uint32_t w = p[j];
+        if (w != LIST_SENTINEL) s += (double)x[w];
+    }
     s = wave_sum_d(s);
     if (lane == 0) {
-        const double L = chord[h];
         const double a_i_dot_x_k = s * L;
         const double a_i_dot_a_i = (L * L) * n;
-        delta[k] = L * ((wepl[h] - a_i_dot_x_k) / a_i_dot_a_i) * lambda;
+        delta[k] = L * ((b_i - a_i_dot_x_k) / a_i_dot_a_i) * lambda;
     }
 }
 
+/* csc points at the block's first entry */
 __global__ void __launch_bounds__(256)
-k_block_apply(const uint32_t* __restrict__ seg_voxel, const uint32_t* __restrict__ seg_start,
-              const uint16_t* __restrict__ csc, int64_t s0, int64_t ns, int64_t entry0, uint32_t n_entries,
+k_block_apply(const uint4* __restrict__ segs, int64_t s0, int64_t ns, const uint16_t* __restrict__ csc,
               const double* __restrict__ delta, float* __restrict__ x) {
+    __builtin_amdgcn_s_setprio(3);   /* latency-bound: issue ahead of the MLP waves sharing the CU */
     int64_t k = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
     if (k >= ns) return;
-    int64_t s = s0 + k;
-    uint32_t a = seg_start[s];
-    uint32_t e = (k + 1 < ns) ? seg_start[s + 1] : n_entries;
-    const uint16_t* c = csc + entry0;
+    const uint4 r = segs[s0 + k];
+    const uint32_t v = r.x;
+    const uint32_t c = r.y & 0xffffu;
+    const float xv = x[v];
     double upd = 0.0;
-    for (uint32_t i = a; i < e; ++i) upd += delta[c[i]];
-    uint32_t v = seg_voxel[s];
-    x[v] = (float)((double)x[v] + upd / (double)(e - a));
+    uint32_t cnt;
+    if (c != 0u) {
+        const double d0 = delta[r.y >> 16];
+        const double d1 = c > 1u ? delta[r.z & 0xffffu] : 0.0;
+        const double d2 = c > 2u ? delta[r.z >> 16] : 0.0;
+        const double d3 = c > 3u ? delta[r.w & 0xffffu] : 0.0;
+        const double d4 = c > 4u ? delta[r.w >> 16] : 0.0;
+        upd += d0;
+        if (c > 1u) upd += d1;
+        if (c > 2u) upd += d2;
+        if (c > 3u) upd += d3;
+        if (c > 4u) upd += d4;
+        cnt = c;
+    } else {
+        cnt = r.w;
+        const uint16_t* q = csc + r.z;
+        for (uint32_t i = 0; i < cnt; ++i) upd += delta[q[i]];
+    }
+    x[v] = (float)((double)xv + upd / (double)cnt);
 }
 
 /* Precision form of the update: the same two kernels in FP32 with
@@ -174,22 +271,32 @@ __device__ __forceinline__ void kahan_add(float& s, float& c, float v) {
 }
 
 __global__ void __launch_bounds__(64 * DELTA_WAVES)
-k_block_delta_f32(const uint32_t* __restrict__ voxels, const int64_t* __restrict__ cap_off,
-                  const uint32_t* __restrict__ len, const double* __restrict__ chord, const float* __restrict__ wepl,
-                  int64_t h0, int64_t count, const float* __restrict__ x, float lambda, float* __restrict__ delta) {
+k_block_delta_f32(const uint32_t* __restrict__ list, int stride, const uint32_t* __restrict__ len,
+                  const double* __restrict__ chord, const float* __restrict__ wepl, int64_t h0, int64_t count,
+                  const float* __restrict__ x, float lambda, float* __restrict__ delta) {
+    __builtin_amdgcn_s_setprio(3);   /* latency-bound: issue ahead of the MLP waves sharing the CU */
     const int lane = threadIdx.x & 63;
     const int64_t k = (int64_t)blockIdx.x * DELTA_WAVES + (threadIdx.x >> 6);
     if (k >= count) return;
     const int64_t h = h0 + k;
-    const int n = (int)len[h];
-    const uint32_t* p = voxels + cap_off[h];
-    float s = 0.0f, c = 0.0f;
+    const uint32_t n = len[h];
+    const float L = (float)chord[h];
+    const float b_i = wepl[h];
+    const uint32_t* p = list + k * stride;
+    uint32_t v[DELTA_SLOTS];
 #pragma unroll
     for (int r = 0; r < DELTA_SLOTS; ++r) {
         int j = lane + 64 * r;
-        if (j < n) kahan_add(s, c, x[p[j]]);
+        v[r] = j < stride ? p[j] : LIST_SENTINEL;
     }
-    for (int j = lane + 64 * DELTA_SLOTS; j < n; j += 64) kahan_add(s, c, x[p[j]]);
+    float s = 0.0f, c = 0.0f;
+#pragma unroll
+    for (int r = 0; r < DELTA_SLOTS; ++r)
+        if (v[r] != LIST_SENTINEL) kahan_add(s, c, x[v[r]]);
+    for (int j = lane + 64 * DELTA_SLOTS; j < stride; j += 64) {
+        uint32_t w = p[j];
+        if (w != LIST_SENTINEL) kahan_add(s, c, x[w]);
+    }
 #pragma unroll
     for (int d = 32; d >= 1; d >>= 1) {
         float os = __shfl_xor(s, d, 64), oc = __shfl_xor(c, d, 64);
@@ -197,27 +304,42 @@ k_block_delta_f32(const uint32_t* __restrict__ voxels, const int64_t* __restrict
         kahan_add(s, c, os);
     }
     if (lane == 0) {
-        const float L = (float)chord[h];
         const float a_i_dot_x_k = s * L;
         const float a_i_dot_a_i = (L * L) * (float)n;
-        delta[k] = L * ((wepl[h] - a_i_dot_x_k) / a_i_dot_a_i) * lambda;
+        delta[k] = L * ((b_i - a_i_dot_x_k) / a_i_dot_a_i) * lambda;
     }
 }
 
 __global__ void __launch_bounds__(256)
-k_block_apply_f32(const uint32_t* __restrict__ seg_voxel, const uint32_t* __restrict__ seg_start,
-                  const uint16_t* __restrict__ csc, int64_t s0, int64_t ns, int64_t entry0, uint32_t n_entries,
+k_block_apply_f32(const uint4* __restrict__ segs, int64_t s0, int64_t ns, const uint16_t* __restrict__ csc,
                   const float* __restrict__ delta, float* __restrict__ x) {
+    __builtin_amdgcn_s_setprio(3);   /* latency-bound: issue ahead of the MLP waves sharing the CU */
     int64_t k = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
     if (k >= ns) return;
-    int64_t s = s0 + k;
-    uint32_t a = seg_start[s];
-    uint32_t e = (k + 1 < ns) ? seg_start[s + 1] : n_entries;
-    const uint16_t* c = csc + entry0;
+    const uint4 r = segs[s0 + k];
+    const uint32_t v = r.x;
+    const uint32_t c = r.y & 0xffffu;
+    const float xv = x[v];
     float upd = 0.0f, comp = 0.0f;
-    for (uint32_t i = a; i < e; ++i) kahan_add(upd, comp, delta[c[i]]);
-    uint32_t v = seg_voxel[s];
-    x[v] = x[v] + upd / (float)(e - a);
+    uint32_t cnt;
+    if (c != 0u) {
+        const float d0 = delta[r.y >> 16];
+        const float d1 = c > 1u ? delta[r.z & 0xffffu] : 0.0f;
+        const float d2 = c > 2u ? delta[r.z >> 16] : 0.0f;
+        const float d3 = c > 3u ? delta[r.w & 0xffffu] : 0.0f;
+        const float d4 = c > 4u ? delta[r.w >> 16] : 0.0f;
+        kahan_add(upd, comp, d0);
+        if (c > 1u) kahan_add(upd, comp, d1);
+        if (c > 2u) kahan_add(upd, comp, d2);
+        if (c > 3u) kahan_add(upd, comp, d3);
+        if (c > 4u) kahan_add(upd, comp, d4);
+        cnt = c;
+    } else {
+        cnt = r.w;
+        const uint16_t* q = csc + r.z;
+        for (uint32_t i = 0; i < cnt; ++i) kahan_add(upd, comp, delta[q[i]]);
+    }
+    x[v] = xv + upd / (float)cnt;
 }
 
 }  // namespace
@@ -225,17 +347,33 @@ k_block_apply_f32(const uint32_t* __restrict__ seg_voxel, const uint32_t* __rest
 /* ------------------------------------------------------------------------- */
 /* host                                                                       */
 /* ------------------------------------------------------------------------- */
+/* chunks of the block loop (the unit of MLP / block-loop overlap): the first
+ * is small so that the block loop starts early, later ones double up to
+ * CHUNK_MAX blocks so that each MLP launch fills the GPU for many rounds */
+#define CHUNK_FIRST 16
+#define CHUNK_MAX 256
+
 static void plan_release_graph(pct_plan* pl) {
-    if (pl->graph) { (void)hipGraphExecDestroy(pl->graph); pl->graph = nullptr; }
+    for (hipGraphExec_t& g : pl->graphs)
+        if (g) (void)hipGraphExecDestroy(g);
+    pl->graphs.clear();
+}
+
+static void release(pct_buffer& b) {
+    if (b.ptr) (void)hipFree(b.ptr);
+    b.ptr = nullptr;
+    b.bytes = 0;
 }
 
 void pct_plan_destroy(pct_plan* pl) {
     if (!pl) return;
     plan_release_graph(pl);
-    pct_buffer* bufs[] = {&pl->hash, &pl->ex_off, &pl->blk_entry, &pl->blk_seg, &pl->seg_voxel, &pl->seg_start,
-                          &pl->csc, &pl->delta};
-    for (pct_buffer* b : bufs)
-        if (b->ptr) (void)hipFree(b->ptr);
+    pct_buffer* bufs[] = {&pl->list, &pl->hist_off, &pl->len, &pl->chord, &pl->hash, &pl->segs,
+                          &pl->csc, &pl->blk_seg, &pl->blk_entry, &pl->delta};
+    for (pct_buffer* b : bufs) release(*b);
+    for (hipEvent_t e : pl->ev_chunk)
+        if (e) (void)hipEventDestroy(e);
+    if (pl->ev_start) (void)hipEventDestroy(pl->ev_start);
     delete pl;
 }
 
@@ -256,43 +394,92 @@ static int plan_build(pct_ctx* ctx, const pct_hist_soa* hist, int B, int mlp_mod
     plan_release_graph(pl);
     pl->valid = false;
     int rc;
-    /* path store + hash */
+    const int64_t nb = (n + B - 1) / B;
+    /* 1. the MLP lists into a capacity-strided build store */
     if ((rc = pct_ensure(ctx, ctx->offsets, sizeof(int64_t) * (n + 1)))) return rc;
     int64_t total = 0;
     if ((rc = pct_mlp_capacity_impl(ctx, hist, 0, n, (int64_t*)ctx->offsets.ptr, &total))) return rc;
     if ((rc = pct_ensure(ctx, ctx->path_store, sizeof(uint32_t) * (total > 0 ? total : 1)))) return rc;
-    if ((rc = pct_ensure(ctx, ctx->path_len, sizeof(uint32_t) * n))) return rc;
-    if ((rc = pct_ensure(ctx, ctx->chord, sizeof(double) * n))) return rc;
+    if ((rc = pct_ensure(ctx, pl->len, sizeof(uint32_t) * n))) return rc;
+    if ((rc = pct_ensure(ctx, pl->chord, sizeof(double) * n))) return rc;
     if ((rc = pct_ensure(ctx, pl->hash, sizeof(uint32_t) * n))) return rc;
     const int64_t* cap_off = (const int64_t*)ctx->offsets.ptr;
-    uint32_t* vox = (uint32_t*)ctx->path_store.ptr;
-    uint32_t* len = (uint32_t*)ctx->path_len.ptr;
-    if ((rc = pct_mlp_paths_impl(ctx, hist, 0, n, cap_off, vox, len, (double*)ctx->chord.ptr, (uint32_t*)pl->hash.ptr,
-                                 nullptr, mlp_mode)))
+    uint32_t* len = (uint32_t*)pl->len.ptr;
+    if ((rc = pct_mlp_paths_impl(ctx, hist, 0, n, cap_off, (uint32_t*)ctx->path_store.ptr, len, (double*)pl->chord.ptr,
+                                 (uint32_t*)pl->hash.ptr, nullptr, mlp_mode)))
         return rc;
     int flags = 0;
     if ((rc = pct_take_status(ctx, &flags))) return rc;
     if ((rc = pct_status_to_rc(ctx, flags))) return rc;
-    /* exact entry offsets */
-    if ((rc = pct_ensure(ctx, pl->ex_off, sizeof(int64_t) * (n + 1)))) return rc;
-    if ((rc = pct_ensure(ctx, ctx->steps, sizeof(int32_t) * n))) return rc;
+    /* 2. block strides = longest list per block; block-strided list store */
+    if ((rc = pct_ensure(ctx, ctx->steps, sizeof(int32_t) * (n > nb ? n : nb)))) return rc;
+    int32_t* bmax = (int32_t*)ctx->steps.ptr;
+    hipLaunchKernelGGL(k_block_max_len, dim3((unsigned)nb), dim3(256), 0, ctx->stream, (const uint32_t*)len, n,
+                       (int64_t)B, bmax);
+    pl->stride_h.assign(nb, 0);
+    PCT_HIP_CHECK(ctx, hipMemcpyAsync(pl->stride_h.data(), bmax, sizeof(int32_t) * nb, hipMemcpyDeviceToHost, ctx->stream));
+    PCT_HIP_CHECK(ctx, hipStreamSynchronize(ctx->stream));
+    pl->list_base_h.assign(nb + 1, 0);
+    for (int64_t b = 0; b < nb; ++b) {
+        int64_t cnt = (n - b * B) < B ? (n - b * B) : B;
+        pl->list_base_h[b + 1] = pl->list_base_h[b] + cnt * pl->stride_h[b];
+    }
+    const int64_t list_size = pl->list_base_h[nb];
+    if ((rc = pct_ensure(ctx, pl->list, sizeof(uint32_t) * (list_size > 0 ? list_size : 1)))) return rc;
+    if ((rc = pct_ensure(ctx, pl->hist_off, sizeof(int64_t) * (n + 1)))) return rc;
+    pct_buffer tmp_base, tmp_stride;
+    auto free_tmp = [&]() { release(tmp_base); release(tmp_stride); };
+    if ((rc = pct_ensure(ctx, tmp_base, sizeof(int64_t) * (nb + 1))) || (rc = pct_ensure(ctx, tmp_stride, sizeof(int32_t) * nb))) {
+        free_tmp();
+        return rc;
+    }
+    PCT_HIP_CHECK(ctx, hipMemcpyAsync(tmp_base.ptr, pl->list_base_h.data(), sizeof(int64_t) * (nb + 1),
+                                      hipMemcpyHostToDevice, ctx->stream));
+    PCT_HIP_CHECK(ctx, hipMemcpyAsync(tmp_stride.ptr, pl->stride_h.data(), sizeof(int32_t) * nb, hipMemcpyHostToDevice,
+                                      ctx->stream));
+    hipLaunchKernelGGL(k_hist_offsets, dim3(pct_grid(n + 1, 256)), dim3(256), 0, ctx->stream,
+                       (const int64_t*)tmp_base.ptr, (const int32_t*)tmp_stride.ptr, n, (int64_t)B, nb,
+                       (int64_t*)pl->hist_off.ptr);
+    hipLaunchKernelGGL(k_fill_u32, dim3(pct_grid(list_size, 256)), dim3(256), 0, ctx->stream, (uint32_t*)pl->list.ptr,
+                       list_size, LIST_SENTINEL);
+    hipLaunchKernelGGL(k_copy_lists, dim3((unsigned)((n + 3) / 4)), dim3(256), 0, ctx->stream,
+                       (const uint32_t*)ctx->path_store.ptr, cap_off, (const uint32_t*)len,
+                       (const int64_t*)pl->hist_off.ptr, n, (uint32_t*)pl->list.ptr);
+    PCT_HIP_CHECK(ctx, hipGetLastError());
+    PCT_HIP_CHECK(ctx, hipStreamSynchronize(ctx->stream));
+    free_tmp();
+    release(ctx->path_store);          /* the build store is not needed any more */
+    const int64_t* hist_off = (const int64_t*)pl->hist_off.ptr;
+    const uint32_t* list = (const uint32_t*)pl->list.ptr;
+    /* 3. exact entry offsets (prefix sum of the lengths) */
+    pct_buffer ex_off;
+    if ((rc = pct_ensure(ctx, ex_off, sizeof(int64_t) * (n + 1)))) return rc;
     hipLaunchKernelGGL(k_lengths_i32, dim3(pct_grid(n, 256)), dim3(256), 0, ctx->stream, (const uint32_t*)len, n,
                        (int32_t*)ctx->steps.ptr);
     int64_t E = 0;
-    if ((rc = pct_exclusive_scan_i32(ctx, (const int32_t*)ctx->steps.ptr, n, (int64_t*)pl->ex_off.ptr, &E))) return rc;
-    const int64_t nb = (n + B - 1) / B;
-    if ((rc = pct_ensure(ctx, pl->blk_entry, sizeof(int64_t) * (nb + 1)))) return rc;
-    if ((rc = pct_ensure(ctx, pl->blk_seg, sizeof(int64_t) * (nb + 1)))) return rc;
-    hipLaunchKernelGGL(k_block_bases, dim3(pct_grid(nb + 1, 256)), dim3(256), 0, ctx->stream,
-                       (const int64_t*)pl->ex_off.ptr, n, (int64_t)B, nb, (int64_t*)pl->blk_entry.ptr);
-    std::vector<int64_t> blk_entry_h(nb + 1);
-    PCT_HIP_CHECK(ctx, hipMemcpyAsync(blk_entry_h.data(), pl->blk_entry.ptr, sizeof(int64_t) * (nb + 1),
-                                      hipMemcpyDeviceToHost, ctx->stream));
+    if ((rc = pct_exclusive_scan_i32(ctx, (const int32_t*)ctx->steps.ptr, n, (int64_t*)ex_off.ptr, &E))) {
+        release(ex_off);
+        return rc;
+    }
+    std::vector<int64_t> ex_h(n + 1);
+    PCT_HIP_CHECK(ctx, hipMemcpyAsync(ex_h.data(), ex_off.ptr, sizeof(int64_t) * (n + 1), hipMemcpyDeviceToHost,
+                                      ctx->stream));
     PCT_HIP_CHECK(ctx, hipStreamSynchronize(ctx->stream));
-    /* CSC values; segment arrays sized for the worst case (one segment per entry) */
-    if ((rc = pct_ensure(ctx, pl->csc, sizeof(uint16_t) * (E > 0 ? E : 1)))) return rc;
-    /* chunks of blocks: key = (local block << vbits) | voxel must fit 32 bits,
-     * and a chunk holds at most ~2^27 entries */
+    std::vector<int64_t> blk_entry_h(nb + 1);
+    for (int64_t b = 0; b <= nb; ++b) blk_entry_h[b] = ex_h[b * B < n ? b * B : n];
+    if ((rc = pct_ensure(ctx, pl->blk_entry, sizeof(int64_t) * (nb + 1))) ||
+        (rc = pct_ensure(ctx, pl->blk_seg, sizeof(int64_t) * (nb + 1)))) {
+        release(ex_off);
+        return rc;
+    }
+    PCT_HIP_CHECK(ctx, hipMemcpyAsync(pl->blk_entry.ptr, blk_entry_h.data(), sizeof(int64_t) * (nb + 1),
+                                      hipMemcpyHostToDevice, ctx->stream));
+    if ((rc = pct_ensure(ctx, pl->csc, sizeof(uint16_t) * (E > 0 ? E : 1)))) {
+        release(ex_off);
+        return rc;
+    }
+    /* 4. chunks of blocks: key = (local block << vbits) | voxel must fit 32
+     * bits, and a chunk holds at most ~2^27 entries */
     int64_t max_blocks = (int64_t)1 << (32 - vbits);
     int64_t avg = nb > 0 ? (E + nb - 1) / nb : 1;
     int64_t by_entries = ((int64_t)1 << 27) / (avg > 0 ? avg : 1);
@@ -301,8 +488,7 @@ static int plan_build(pct_ctx* ctx, const pct_hist_soa* hist, int B, int mlp_mod
     int64_t max_chunk_entries = 0;
     for (int64_t b0 = 0; b0 < nb; b0 += chunk) {
         int64_t b1 = b0 + chunk < nb ? b0 + chunk : nb;
-        int64_t e = blk_entry_h[b1] - blk_entry_h[b0];
-        if (e > max_chunk_entries) max_chunk_entries = e;
+        max_chunk_entries = std::max(max_chunk_entries, blk_entry_h[b1] - blk_entry_h[b0]);
     }
     size_t ce = (size_t)(max_chunk_entries > 0 ? max_chunk_entries : 1);
     size_t sort_tmp = 0;
@@ -313,8 +499,7 @@ static int plan_build(pct_ctx* ctx, const pct_hist_soa* hist, int B, int mlp_mod
     }
     pct_buffer kin, kout, vin, vout, fl, sidx, tmp;
     auto free_all = [&]() {
-        for (pct_buffer* b : {&kin, &kout, &vin, &vout, &fl, &sidx, &tmp})
-            if (b->ptr) (void)hipFree(b->ptr);
+        for (pct_buffer* b : {&kin, &kout, &vin, &vout, &fl, &sidx, &tmp, &ex_off}) release(*b);
     };
     if ((rc = pct_ensure(ctx, kin, 4 * ce)) || (rc = pct_ensure(ctx, kout, 4 * ce)) || (rc = pct_ensure(ctx, vin, 4 * ce)) ||
         (rc = pct_ensure(ctx, vout, 4 * ce)) || (rc = pct_ensure(ctx, fl, 4 * ce)) ||
@@ -322,13 +507,12 @@ static int plan_build(pct_ctx* ctx, const pct_hist_soa* hist, int B, int mlp_mod
         free_all();
         return rc;
     }
-    /* first pass: count segments per chunk to size the segment arrays */
+    /* first pass counts the segments per chunk, the second writes them */
     std::vector<int64_t> chunk_segs;
     int64_t S_total = 0;
     for (int pass = 0; pass < 2; ++pass) {
         int64_t seg0 = 0;
-        int ci = 0;
-        for (int64_t b0 = 0; b0 < nb; b0 += chunk, ++ci) {
+        for (int64_t b0 = 0; b0 < nb; b0 += chunk) {
             int64_t b1 = b0 + chunk < nb ? b0 + chunk : nb;
             int64_t h0 = b0 * B, h1 = b1 * B < n ? b1 * B : n;
             int64_t entry0 = blk_entry_h[b0];
@@ -336,9 +520,9 @@ static int plan_build(pct_ctx* ctx, const pct_hist_soa* hist, int B, int mlp_mod
             if (Ec == 0) { if (pass == 0) chunk_segs.push_back(0); continue; }
             int endbit = vbits + vbits_for(b1 - b0 + 1);
             if (endbit > 32) endbit = 32;
-            hipLaunchKernelGGL(k_emit_keys, dim3((unsigned)((h1 - h0 + 3) / 4)), dim3(256), 0, ctx->stream,
-                               (const uint32_t*)vox, cap_off, (const uint32_t*)len, (const int64_t*)pl->ex_off.ptr,
-                               h0, h1, (int64_t)B, b0, vbits, entry0, (uint32_t*)kin.ptr, (uint32_t*)vin.ptr);
+            hipLaunchKernelGGL(k_emit_keys, dim3((unsigned)((h1 - h0 + 3) / 4)), dim3(256), 0, ctx->stream, list,
+                               hist_off, (const uint32_t*)len, (const int64_t*)ex_off.ptr, h0, h1, (int64_t)B, b0, vbits,
+                               entry0, (uint32_t*)kin.ptr, (uint32_t*)vin.ptr);
             size_t tb = tmp.bytes;
             PCT_HIP_CHECK(ctx, rocprim::radix_sort_pairs(tmp.ptr, tb, (uint32_t*)kin.ptr, (uint32_t*)kout.ptr,
                                                          (uint32_t*)vin.ptr, (uint32_t*)vout.ptr, (size_t)Ec, 0, endbit,
@@ -357,16 +541,15 @@ static int plan_build(pct_ctx* ctx, const pct_hist_soa* hist, int B, int mlp_mod
             hipLaunchKernelGGL(k_write_segments, dim3(pct_grid(Ec, 256)), dim3(256), 0, ctx->stream,
                                (const uint32_t*)kout.ptr, (const uint32_t*)vout.ptr, (const int32_t*)fl.ptr,
                                (const int64_t*)sidx.ptr, Ec, vbits, seg0, b0, entry0,
-                               (const int64_t*)pl->blk_entry.ptr, (uint32_t*)pl->seg_voxel.ptr,
-                               (uint32_t*)pl->seg_start.ptr, (uint16_t*)pl->csc.ptr, (int64_t*)pl->blk_seg.ptr);
+                               (const int64_t*)pl->blk_entry.ptr, (uint4*)pl->segs.ptr, (uint16_t*)pl->csc.ptr,
+                               (int64_t*)pl->blk_seg.ptr);
             PCT_HIP_CHECK(ctx, hipGetLastError());
             seg0 += Sc;
         }
         if (pass == 0) {
             S_total = 0;
             for (int64_t s : chunk_segs) S_total += s;
-            if ((rc = pct_ensure(ctx, pl->seg_voxel, sizeof(uint32_t) * (S_total > 0 ? S_total : 1))) ||
-                (rc = pct_ensure(ctx, pl->seg_start, sizeof(uint32_t) * (S_total > 0 ? S_total : 1)))) {
+            if ((rc = pct_ensure(ctx, pl->segs, sizeof(uint4) * (S_total > 0 ? S_total : 1)))) {
                 free_all();
                 return rc;
             }
@@ -389,6 +572,7 @@ static int plan_build(pct_ctx* ctx, const pct_hist_soa* hist, int B, int mlp_mod
     pl->S = S_total;
     pl->key = *hist;
     pl->mlp_mode = mlp_mode;
+    pl->chunk_b.clear();
     pl->valid = true;
     return PCT_OK;
 }
@@ -398,55 +582,150 @@ static bool same_hist(const pct_hist_soa& a, const pct_hist_soa& b) {
            a.xy_entry_angle == b.xy_entry_angle && a.x_exit == b.x_exit;
 }
 
-/* the block loop of one iteration on the plan */
-static int plan_block_loop(pct_ctx* ctx, pct_plan* pl, const pct_hist_soa* hist, float* x, double lambda, bool f32) {
+/* the block loop over blocks [b_begin, b_end) on the plan */
+static int plan_block_loop(pct_ctx* ctx, pct_plan* pl, const pct_hist_soa* hist, float* x, double lambda, bool f32,
+                           int64_t b_begin, int64_t b_end) {
     const int64_t B = pl->B;
-    const uint32_t* vox = (const uint32_t*)ctx->path_store.ptr;
-    const int64_t* cap_off = (const int64_t*)ctx->offsets.ptr;
-    const uint32_t* len = (const uint32_t*)ctx->path_len.ptr;
-    const double* chord = (const double*)ctx->chord.ptr;
-    double* delta = (double*)pl->delta.ptr;
-    for (int64_t b = 0; b < pl->nb; ++b) {
+    const uint32_t* list = (const uint32_t*)pl->list.ptr;
+    const uint32_t* len = (const uint32_t*)pl->len.ptr;
+    const double* chord = (const double*)pl->chord.ptr;
+    const uint4* segs = (const uint4*)pl->segs.ptr;
+    const uint16_t* csc = (const uint16_t*)pl->csc.ptr;
+    void* delta = pl->delta.ptr;
+    for (int64_t b = b_begin; b < b_end; ++b) {
         int64_t h0 = b * B, cnt = (pl->n - h0) < B ? (pl->n - h0) : B;
+        const uint32_t* lb = list + pl->list_base_h[b];
+        int stride = pl->stride_h[b];
         dim3 dgrid((unsigned)((cnt + DELTA_WAVES - 1) / DELTA_WAVES));
         if (f32)
-            hipLaunchKernelGGL(k_block_delta_f32, dgrid, dim3(64 * DELTA_WAVES), 0, ctx->stream, vox, cap_off, len, chord,
+            hipLaunchKernelGGL(k_block_delta_f32, dgrid, dim3(64 * DELTA_WAVES), 0, ctx->stream, lb, stride, len, chord,
                                (const float*)hist->wepl, h0, cnt, (const float*)x, (float)lambda, (float*)delta);
         else
-            hipLaunchKernelGGL(k_block_delta, dgrid, dim3(64 * DELTA_WAVES), 0, ctx->stream, vox, cap_off, len, chord,
-                               (const float*)hist->wepl, h0, cnt, (const float*)x, lambda, delta);
+            hipLaunchKernelGGL(k_block_delta, dgrid, dim3(64 * DELTA_WAVES), 0, ctx->stream, lb, stride, len, chord,
+                               (const float*)hist->wepl, h0, cnt, (const float*)x, lambda, (double*)delta);
         int64_t s0 = pl->blk_seg_h[b], ns = pl->blk_seg_h[b + 1] - s0;
-        uint32_t ne = (uint32_t)(pl->blk_entry_h[b + 1] - pl->blk_entry_h[b]);
+        const uint16_t* cb = csc + pl->blk_entry_h[b];
         if (ns > 0 && f32)
-            hipLaunchKernelGGL(k_block_apply_f32, dim3(pct_grid(ns, 256)), dim3(256), 0, ctx->stream,
-                               (const uint32_t*)pl->seg_voxel.ptr, (const uint32_t*)pl->seg_start.ptr,
-                               (const uint16_t*)pl->csc.ptr, s0, ns, pl->blk_entry_h[b], ne, (const float*)delta, x);
+            hipLaunchKernelGGL(k_block_apply_f32, dim3(pct_grid(ns, 256)), dim3(256), 0, ctx->stream, segs, s0, ns, cb,
+                               (const float*)delta, x);
         else if (ns > 0)
-            hipLaunchKernelGGL(k_block_apply, dim3(pct_grid(ns, 256)), dim3(256), 0, ctx->stream,
-                               (const uint32_t*)pl->seg_voxel.ptr, (const uint32_t*)pl->seg_start.ptr,
-                               (const uint16_t*)pl->csc.ptr, s0, ns, pl->blk_entry_h[b], ne, (const double*)delta, x);
+            hipLaunchKernelGGL(k_block_apply, dim3(pct_grid(ns, 256)), dim3(256), 0, ctx->stream, segs, s0, ns, cb,
+                               (const double*)delta, x);
     }
     PCT_HIP_CHECK(ctx, hipGetLastError());
     return PCT_OK;
 }
 
-static int plan_run_graph(pct_ctx* ctx, pct_plan* pl, const pct_hist_soa* hist, float* x, double lambda, bool f32) {
-    if (!pl->graph || pl->graph_x != x || pl->graph_lambda != lambda || pl->graph_f32 != f32) {
-        plan_release_graph(pl);
+/* chunk boundaries from the current knobs; a change drops the graphs */
+static void plan_chunks(pct_ctx* ctx, pct_plan* pl) {
+    int64_t first = ctx->tune_chunk_first > 0 ? ctx->tune_chunk_first : CHUNK_FIRST;
+    int64_t most = ctx->tune_chunk_max > 0 ? ctx->tune_chunk_max : CHUNK_MAX;
+    if (most < first) most = first;
+    if (!pl->chunk_b.empty() && pl->chunk_first == first && pl->chunk_max == most) return;
+    plan_release_graph(pl);
+    pl->chunk_first = first;
+    pl->chunk_max = most;
+    pl->chunk_b.assign(1, 0);
+    for (int64_t size = first; pl->chunk_b.back() < pl->nb; size = std::min<int64_t>(2 * size, most))
+        pl->chunk_b.push_back(std::min(pl->nb, pl->chunk_b.back() + size));
+}
+
+static int plan_ensure_graphs(pct_ctx* ctx, pct_plan* pl, const pct_hist_soa* hist, float* x, double lambda,
+                              bool f32) {
+    const int64_t nchunks = (int64_t)pl->chunk_b.size() - 1;
+    if ((int64_t)pl->graphs.size() == nchunks && pl->graph_x == x && pl->graph_lambda == lambda &&
+        pl->graph_f32 == f32 && pl->graph_wepl == hist->wepl)
+        return PCT_OK;
+    plan_release_graph(pl);
+    for (int64_t c = 0; c < nchunks; ++c) {
+        int64_t b0 = pl->chunk_b[c], b1 = pl->chunk_b[c + 1];
         hipGraph_t g = nullptr;
         PCT_HIP_CHECK(ctx, hipStreamBeginCapture(ctx->stream, hipStreamCaptureModeThreadLocal));
-        int rc = plan_block_loop(ctx, pl, hist, x, lambda, f32);
+        int rc = plan_block_loop(ctx, pl, hist, x, lambda, f32, b0, b1);
         hipError_t e = hipStreamEndCapture(ctx->stream, &g);
-        if (rc) { if (g) (void)hipGraphDestroy(g); return rc; }
-        if (e != hipSuccess) return pct_fail(ctx, PCT_E_HIP, "graph capture: %s", hipGetErrorString(e));
-        e = hipGraphInstantiate(&pl->graph, g, nullptr, nullptr, 0);
+        if (rc) { if (g) (void)hipGraphDestroy(g); plan_release_graph(pl); return rc; }
+        if (e != hipSuccess) { plan_release_graph(pl); return pct_fail(ctx, PCT_E_HIP, "graph capture: %s", hipGetErrorString(e)); }
+        hipGraphExec_t ge = nullptr;
+        e = hipGraphInstantiate(&ge, g, nullptr, nullptr, 0);
         (void)hipGraphDestroy(g);
-        if (e != hipSuccess) { pl->graph = nullptr; return pct_fail(ctx, PCT_E_HIP, "graph instantiate: %s", hipGetErrorString(e)); }
-        pl->graph_x = x;
-        pl->graph_lambda = lambda;
-        pl->graph_f32 = f32;
+        if (e != hipSuccess) { plan_release_graph(pl); return pct_fail(ctx, PCT_E_HIP, "graph instantiate: %s", hipGetErrorString(e)); }
+        pl->graphs.push_back(ge);
     }
-    PCT_HIP_CHECK(ctx, hipGraphLaunch(pl->graph, ctx->stream));
+    pl->graph_x = x;
+    pl->graph_lambda = lambda;
+    pl->graph_f32 = f32;
+    pl->graph_wepl = hist->wepl;
+    return PCT_OK;
+}
+
+static int plan_ensure_events(pct_ctx* ctx, pct_plan* pl, int64_t nchunks) {
+    if (!ctx->aux_stream)
+        PCT_HIP_CHECK(ctx, hipStreamCreateWithFlags(&ctx->aux_stream, hipStreamNonBlocking));
+    if (!pl->ev_start) PCT_HIP_CHECK(ctx, hipEventCreateWithFlags(&pl->ev_start, hipEventDisableTiming));
+    while ((int64_t)pl->ev_chunk.size() < nchunks) {
+        hipEvent_t e = nullptr;
+        PCT_HIP_CHECK(ctx, hipEventCreateWithFlags(&e, hipEventDisableTiming));
+        pl->ev_chunk.push_back(e);
+    }
+    return PCT_OK;
+}
+
+/* One iteration on the plan. With `recompute`, the MLP lists of every chunk
+ * of blocks are recomputed on the context's second stream, ahead of the block
+ * loop, which takes chunk c as soon as its lists are done: the FP64-bound MLP
+ * of later chunks runs beside the latency-bound block loop of earlier ones. */
+static int plan_iterate(pct_ctx* ctx, pct_plan* pl, const pct_hist_soa* hist, float* x, double lambda, bool f32,
+                        int mlp, bool recompute, bool eager) {
+    const bool sequential = recompute && ctx->tune_overlap == 2;
+    int rc;
+    plan_chunks(ctx, pl);
+    const int64_t nchunks = (int64_t)pl->chunk_b.size() - 1;
+    if (!eager && (rc = plan_ensure_graphs(ctx, pl, hist, x, lambda, f32))) return rc;
+    if (ctx->profile) PCT_HIP_CHECK(ctx, hipEventRecord(ctx->ev[0], ctx->stream));
+    if (recompute && ctx->tune_overlap == 2) {
+        /* sequential: all lists first, then the block loop */
+        rc = pct_mlp_paths_on(ctx, ctx->stream, hist, 0, pl->n, (const int64_t*)pl->hist_off.ptr,
+                              (uint32_t*)pl->list.ptr, (uint32_t*)pl->len.ptr, (double*)pl->chord.ptr, nullptr,
+                              (const uint32_t*)pl->hash.ptr, mlp);
+        if (rc) return rc;
+        recompute = false;
+        if (ctx->profile) {
+            PCT_HIP_CHECK(ctx, hipEventRecord(ctx->ev[1], ctx->stream));
+            PCT_HIP_CHECK(ctx, hipEventRecord(ctx->ev[2], ctx->stream));
+        }
+    } else if (recompute) {
+        if ((rc = plan_ensure_events(ctx, pl, nchunks))) return rc;
+        hipStream_t aux = ctx->aux_stream;
+        PCT_HIP_CHECK(ctx, hipEventRecord(pl->ev_start, ctx->stream));
+        PCT_HIP_CHECK(ctx, hipStreamWaitEvent(aux, pl->ev_start, 0));
+        for (int64_t c = 0; c < nchunks; ++c) {
+            int64_t h0 = pl->chunk_b[c] * pl->B;
+            int64_t h1 = std::min(pl->n, pl->chunk_b[c + 1] * pl->B);
+            rc = pct_mlp_paths_on(ctx, aux, hist, h0, h1 - h0, (const int64_t*)pl->hist_off.ptr + h0,
+                                  (uint32_t*)pl->list.ptr, (uint32_t*)pl->len.ptr + h0, (double*)pl->chord.ptr + h0,
+                                  nullptr, (const uint32_t*)pl->hash.ptr + h0, mlp);
+            if (rc) return rc;
+            PCT_HIP_CHECK(ctx, hipEventRecord(pl->ev_chunk[c], aux));
+            if (ctx->profile && c == 0) PCT_HIP_CHECK(ctx, hipEventRecord(ctx->ev[1], aux));
+        }
+        if (ctx->profile) PCT_HIP_CHECK(ctx, hipEventRecord(ctx->ev[2], aux));
+    } else if (ctx->profile && !sequential) {
+        PCT_HIP_CHECK(ctx, hipEventRecord(ctx->ev[1], ctx->stream));
+        PCT_HIP_CHECK(ctx, hipEventRecord(ctx->ev[2], ctx->stream));
+    }
+    for (int64_t c = 0; c < nchunks; ++c) {
+        if (recompute) PCT_HIP_CHECK(ctx, hipStreamWaitEvent(ctx->stream, pl->ev_chunk[c], 0));
+        if (eager) {
+            int64_t b0 = pl->chunk_b[c], b1 = pl->chunk_b[c + 1];
+            if ((rc = plan_block_loop(ctx, pl, hist, x, lambda, f32, b0, b1))) return rc;
+        } else {
+            PCT_HIP_CHECK(ctx, hipGraphLaunch(pl->graphs[c], ctx->stream));
+        }
+    }
+    if (ctx->profile) {
+        PCT_HIP_CHECK(ctx, hipEventRecord(ctx->ev[3], ctx->stream));
+        ctx->ev_recorded = true;
+    }
     return PCT_OK;
 }
 
@@ -477,27 +756,18 @@ extern "C" int pct_drop_iteration(pct_ctx* ctx, const pct_hist_soa* hist, float*
         pl = ctx->plan;
         fresh = true;
     }
-    if (ctx->profile) PCT_HIP_CHECK(ctx, hipEventRecord(ctx->ev[0], ctx->stream));
-    if (ctx->profile) PCT_HIP_CHECK(ctx, hipEventRecord(ctx->ev[1], ctx->stream));
-    if (!(mode & PCT_DROP_CACHED) && !fresh) {
-        /* the reference recomputes every MLP path in every iteration; do the
-         * same and check the lists against the plan */
-        rc = pct_mlp_paths_impl(ctx, hist, 0, hist->n, (const int64_t*)ctx->offsets.ptr, (uint32_t*)ctx->path_store.ptr,
-                                (uint32_t*)ctx->path_len.ptr, (double*)ctx->chord.ptr, nullptr,
-                                (const uint32_t*)pl->hash.ptr, mlp);
-        if (rc) return rc;
-    }
-    if (ctx->profile) PCT_HIP_CHECK(ctx, hipEventRecord(ctx->ev[2], ctx->stream));
-    const bool f32 = (mode & PCT_DROP_F32) != 0;
-    if (mode & PCT_DROP_NO_GRAPH) rc = plan_block_loop(ctx, pl, hist, x, (double)lambda, f32);
-    else rc = plan_run_graph(ctx, pl, hist, x, (double)lambda, f32);
+    /* the reference recomputes every MLP path in every iteration; so does the
+     * default mode, into the plan's list store (a list's capacity is its
+     * block's stride), checking every list against the plan's hash */
+    const bool recompute = !(mode & PCT_DROP_CACHED) && !fresh;
+    rc = plan_iterate(ctx, pl, hist, x, (double)lambda, (mode & PCT_DROP_F32) != 0, mlp, recompute,
+                      (mode & PCT_DROP_NO_GRAPH) != 0);
     if (rc) return rc;
-    if (ctx->profile) {
-        PCT_HIP_CHECK(ctx, hipEventRecord(ctx->ev[3], ctx->stream));
-        ctx->ev_recorded = true;
-    }
     int flags = 0;
     if ((rc = pct_take_status(ctx, &flags))) return rc;
+    /* recomputed lists that differ from the plan (or overflow a stride) have
+     * overwritten the list store: the plan is rebuilt on the next call */
+    if (flags) pl->valid = false;
     return pct_status_to_rc(ctx, flags);
 }
 
@@ -512,5 +782,21 @@ extern "C" int pct_drop_stats(pct_ctx* ctx, const pct_hist_soa* hist, int block_
     }
     *sum_path = pl->E;
     *sum_touched = pl->S;
+    return PCT_OK;
+}
+
+extern "C" int pct_set_tuning(pct_ctx* ctx, int key, int value) {
+    if (!ctx) return PCT_E_INVALID;
+    if (value < 0) value = 0;
+    switch (key) {
+    case PCT_TUNE_MLP_WAVES: ctx->tune_mlp_waves = value; break;
+    case PCT_TUNE_CHUNK_FIRST: ctx->tune_chunk_first = value; break;
+    case PCT_TUNE_CHUNK_MAX: ctx->tune_chunk_max = value; break;
+    case PCT_TUNE_OVERLAP:
+        if (value > 2) return pct_fail(ctx, PCT_E_INVALID, "pct_set_tuning: overlap %d not in {1, 2}", value);
+        ctx->tune_overlap = value;
+        break;
+    default: return pct_fail(ctx, PCT_E_INVALID, "pct_set_tuning: unknown key %d", key);
+    }
     return PCT_OK;
 }

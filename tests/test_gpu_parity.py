@@ -216,3 +216,23 @@ def test_drop_block_api_matches_iteration(ctx_mini, mini_host):
     a, b = x.cpu().numpy(), x_it.cpu().numpy()
     assert np.max(np.abs(a - b)) < 1e-5
     assert int(cnt.abs().sum()) == 0 and float(xu.abs().sum()) == 0.0
+
+
+def test_tuning_knobs_do_not_change_results(ctx_mini, mini_host):
+    """overlapped / sequential MLP and any chunking of the block loop give the
+    same image bit for bit"""
+    import pct_amd
+    torch = ctx_mini.torch
+    d = _to_dev(mini_host["hr"])
+    x0 = mini_host["hull"].astype(np.float32)
+    out = []
+    for knobs in ({}, {"overlap": 2}, {"chunk_first": 1, "chunk_max": 2}, {"chunk_first": 3, "mlp_waves": 1}):
+        pct_amd.set_tuning(ctx_mini, overlap=0, chunk_first=0, chunk_max=0, mlp_waves=0)
+        pct_amd.set_tuning(ctx_mini, **knobs)
+        x = torch.from_numpy(x0.copy()).cuda()
+        for _ in range(3):
+            ctx_mini.drop_iteration(d, x, lam=1e-2)
+        out.append(x.cpu().numpy())
+    pct_amd.set_tuning(ctx_mini, overlap=0, chunk_first=0, chunk_max=0, mlp_waves=0)
+    for o in out[1:]:
+        assert np.array_equal(out[0], o)
