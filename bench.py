@@ -52,6 +52,13 @@ def main():
     ap.add_argument("--histories", type=int, default=None, help="override the scan size")
     ap.add_argument("--cpu-seconds", type=float, default=15.0, help="target CPU time of the oracle sample")
     ap.add_argument("--no-cpu-baseline", action="store_true")
+    ap.add_argument("--quick", action="store_true",
+                    help="only the timed iterations (for counter runs, where every dispatch is slow)")
+    ap.add_argument("--form", default="default", choices=["default", "cached", "sequential", "precision"],
+                    help="with --quick: the form of the timed iterations (default: MLP beside the block loop; "
+                         "cached: the block loop alone; sequential: MLP then block loop; precision)")
+    ap.add_argument("--copy-calibration", action="store_true",
+                    help="5 device copies of 256 MiB and nothing else: the reference kernel for the HBM counters")
     args = ap.parse_args()
 
     if args.gpus != 1:
@@ -68,6 +75,15 @@ def main():
     if not torch.cuda.is_available():
         print(json.dumps({"error": "no GPU"}))
         sys.exit(1)
+    if args.copy_calibration:
+        n = 64 << 20
+        src = torch.ones(n, dtype=torch.float32, device="cuda")
+        dst = torch.empty_like(src)
+        for _ in range(5):
+            dst.copy_(src)
+        torch.cuda.synchronize()
+        print(json.dumps({"copy_bytes_read": 4 * n, "copy_bytes_written": 4 * n, "copies": 5}), flush=True)
+        return
     dev = torch.device("cuda:0")
     pre = configs.PRESETS[args.config]
     p = configs.params(args.config)
@@ -117,21 +133,44 @@ def main():
 
     # ---- timed DROP iterations (default: MLP recomputed every iteration) ----
     pct_amd.set_profiling(ctx, True)
+    form_mode = 0
+    if args.quick and args.form == "cached":
+        form_mode = pct_amd.PCT_DROP_CACHED
+    elif args.quick and args.form == "sequential":
+        pct_amd.set_tuning(ctx, overlap=2)
+    elif args.quick and args.form == "precision":
+        form_mode = pct_amd.PCT_PRECISION
+        ctx.drop_prepare(hist, B, pct_amd.PCT_MLP_TABLE)
     for _ in range(args.warmup):
-        ctx.drop_iteration(hist, x, B, p.lambda_)
+        ctx.drop_iteration(hist, x, B, p.lambda_, form_mode)
     ctx.sync()
     t_start = time.perf_counter()
-    gpu_ms, ph = timed(0, args.steps)
+    gpu_ms, ph = timed(form_mode, args.steps)
     wall = time.perf_counter() - t_start
     ms_per_step = gpu_ms / args.steps
     value = n_r * args.steps / (gpu_ms / 1e3)
     _, mlp_ms, blk_ms = ph.mean(axis=0)
     log(f"timed {args.steps} iterations: {ms_per_step:.2f} ms/iter (wall {wall / args.steps * 1e3:.2f} ms); "
         f"MLP {mlp_ms:.2f} ms, blocks {blk_ms:.2f} ms")
-    # other forms of the same iteration, for comparison
+    if args.quick:
+        print(json.dumps({"metric": "histories/s per DROP iteration (MLP+update), RSP RMSE vs FP64 oracle, 1 GPU",
+                          "value": round(value, 1), "unit": "histories/s", "ms_per_step": round(ms_per_step, 3),
+                          "steps": args.steps, "quick": True, "form": args.form,
+                          "phases_ms": [round(float(v), 3) for v in ph.mean(axis=0)]}), flush=True)
+        return
+    # each phase alone, for its roofline: the block loop without the MLP pass
+    # (cached lists: k_block_delta + k_block_apply only), and the MLP pass
+    # followed by the block loop on one stream (no overlap)
     ms_cached, _ = timed(pct_amd.PCT_DROP_CACHED, 1)
+    pct_amd.set_tuning(ctx, overlap=2)
+    ms_seq, ph_seq = timed(0, 1)
+    pct_amd.set_tuning(ctx, overlap=0)
+    mlp_alone_ms = float(ph_seq[0][1])
+    blk_alone_ms = ms_cached
+    # the reference's structure (atomic scatter + volume scan per block), for comparison
     ms_atomic, _ = timed(pct_amd.PCT_DROP_ATOMIC, 1)
-    log(f"cached-list iteration {ms_cached:.2f} ms, reference-structure (atomic) iteration {ms_atomic:.2f} ms")
+    log(f"block loop alone {ms_cached:.2f} ms, MLP alone {mlp_alone_ms:.2f} ms, sequential iteration {ms_seq:.2f} ms, "
+        f"reference-structure (atomic) iteration {ms_atomic:.2f} ms")
 
     # ---- precision form: tabulated MLP + FP32 stepping + FP32/Kahan update ----
     # RSP error vs the exact FP64 form after iterations 1, 6 and 12 from x0 (the
@@ -174,17 +213,28 @@ def main():
     # algorithmic work per iteration: 48 B per history record, 4 B gather + 4 B
     # scatter per path entry, 24 B per voxel touched by a block
     bytes_update = 48.0 * n_r + 8.0 * sum_n + 24.0 * sum_vb
-    upd_gbs = bytes_update / (blk_ms / 1e3) / 1e9
-    mlp_tflops = MLP_FLOPS_PER_STEP * float(sum_steps) / (mlp_ms / 1e3) / 1e12
+    nb = (n_r + B - 1) // B
+    upd_gbs = bytes_update / (blk_alone_ms / 1e3) / 1e9
+    mlp_tflops = MLP_FLOPS_PER_STEP * float(sum_steps) / (mlp_alone_ms / 1e3) / 1e12
     roof_update = {"bound": "hbm", "achieved": round(upd_gbs, 2), "peak": HBM_PEAK_GBS, "unit": "GB/s",
                    "frac": round(upd_gbs / HBM_PEAK_GBS, 4), "traffic": None,
-                   "kernel": "k_block_delta+k_block_apply (block loop, hipGraph)", "ms": round(float(blk_ms), 3),
-                   "bytes_alg": bytes_update}
+                   "kernel": "k_block_delta+k_block_apply, one pair per DROP block (hipGraph replay)",
+                   "us_per_block": round(blk_alone_ms * 1e3 / nb, 3), "bytes_alg_per_block": round(bytes_update / nb),
+                   "ms_per_iteration": round(float(blk_alone_ms), 3), "bytes_alg": bytes_update}
     roof_mlp = {"bound": "mfma", "achieved": round(mlp_tflops, 3), "peak": FP64_PEAK_TFLOPS, "unit": "TFLOP/s",
                 "frac": round(mlp_tflops / FP64_PEAK_TFLOPS, 4), "traffic": None,
-                "kernel": "k_mlp_paths (FP64 VALU; peak = MI355X FP64 78.6 TFLOP/s)", "ms": round(float(mlp_ms), 3),
-                "flops_alg": MLP_FLOPS_PER_STEP * float(sum_steps)}
-    dominant = roof_mlp if mlp_ms >= blk_ms else roof_update
+                "kernel": "k_mlp_paths<exact_walk> (FP64 VALU; peak = MI355X FP64 78.6 TFLOP/s)",
+                "ms_per_iteration": round(mlp_alone_ms, 3), "flops_alg": MLP_FLOPS_PER_STEP * float(sum_steps)}
+    # measured HBM traffic of the block pair (FETCH_SIZE + WRITE_SIZE, scaled by
+    # a plain copy) from the committed counter profile of this geometry
+    tpath = os.path.join(ROOT, "profiles", f"hbm_traffic_{args.config}.json")
+    if os.path.exists(tpath):
+        tr = json.load(open(tpath))["kernels"]
+        per_block = sum(tr[k]["fetch_bytes_per_dispatch"] + tr[k]["write_bytes_per_dispatch"]
+                        for k in ("k_block_delta", "k_block_apply"))
+        roof_update["traffic"] = per_block
+        roof_update["traffic_source"] = os.path.relpath(tpath, ROOT) + " (bytes per block: delta + apply)"
+    dominant = roof_mlp if mlp_alone_ms >= blk_alone_ms else roof_update
 
     # ---- CPU oracle on a bounded sample, and RMSE of the GPU vs the oracle ----
     cpu = None
@@ -244,10 +294,13 @@ def main():
                    "reconstruction_histories": n_r,
                    "mean_path_voxels": round(sum_n / n_r, 2), "sum_block_voxels": sum_vb,
                    "drop_blocks": (n_r + B - 1) // B},
-        "phases_ms": {"mlp": round(float(mlp_ms), 3), "blocks": round(float(blk_ms), 3),
+        "phases_ms": {"mlp_span": round(float(mlp_ms), 3), "block_loop_span": round(float(blk_ms), 3),
+                      "note": "the MLP pass and the block loop overlap chunk by chunk; alone they take "
+                              f"{mlp_alone_ms:.2f} and {blk_alone_ms:.2f} ms",
                       "plan_setup_once": round(t_plan * 1e3, 1), "preprocess_once": round(t_pre * 1e3, 1)},
         "precision_form": precision,
-        "other_forms": {"cached_lists_histories_per_s": round(n_r / (ms_cached / 1e3), 1),
+        "other_forms": {"sequential_mlp_then_blocks_ms": round(ms_seq, 3),
+                        "cached_lists_histories_per_s": round(n_r / (ms_cached / 1e3), 1),
                         "cached_lists_ms": round(ms_cached, 3),
                         "atomic_reference_structure_histories_per_s": round(n_r / (ms_atomic / 1e3), 1),
                         "atomic_reference_structure_ms": round(ms_atomic, 3)},
