@@ -23,11 +23,11 @@ struct pct_plan {
     pct_buffer len;                /* u32 per history: list length          */
     pct_buffer chord;              /* f64 per history: effective chord      */
     pct_buffer hash;               /* u32 per history: hash of its list     */
-    /* block transpose: per block the touched voxels sorted by voxel, each a
-     * 16-byte record {voxel, count | history indices}; csc holds all
+    /* block transpose: per block the touched voxels sorted by voxel, each an
+     * 8-byte record {voxel, count | history indices}; csc holds all
      * block-local history indices in (block, voxel, history) order for the
      * records with more than the inline number */
-    pct_buffer segs;               /* uint4 per segment                     */
+    pct_buffer segs;               /* u64 per segment                       */
     pct_buffer csc;                /* u16 per entry                         */
     pct_buffer blk_seg;            /* int64 per block+1: first segment      */
     pct_buffer blk_entry;          /* int64 per block+1: first csc entry    */

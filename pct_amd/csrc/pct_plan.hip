@@ -11,7 +11,7 @@
  *                unused slots hold a sentinel; lengths, effective chord
  *                lengths and a hash of every list;
  *   block CSC    per DROP block the distinct voxels it touches, sorted by
- *                voxel, as 16-byte records {voxel, count, up to 5 block-local
+ *                voxel, as 8-byte records {voxel, count, up to 3 block-local
  *                history indices in history order}; longer segments point
  *                into a u16 array of all indices.
  *
@@ -32,7 +32,7 @@
  * trips of each kernel. The layout is chosen so that each kernel has exactly
  * two: the delta kernel finds a history's list from the launch arguments
  * (block base + k * stride) and reads the list and the history's scalars in
- * one round, then gathers x; the apply kernel reads one record per voxel
+ * one round, then gathers x; the apply kernel reads one 8-byte record per voxel
  * (voxel + inline history indices) and then the deltas and the old x[v]
  * together. The block loop is captured once into hipGraphs of chunks of
  * blocks and replayed. In the default mode every iteration recomputes every MLP
@@ -61,7 +61,16 @@ int pct_mlp_paths_on(pct_ctx* ctx, hipStream_t stream, const pct_hist_soa* hist,
                      const uint32_t* hash_expect, int mode);
 
 #define LIST_SENTINEL 0xFFFFFFFFu
-#define SEG_INLINE 5
+/* 8-byte segment record: bits 0-21 voxel, 22-23 inline count (1-3; 0 = the
+ * history indices are in csc), then either up to three 13-bit block-local
+ * history indices (bits 24-36, 37-49, 50-62) or the block-relative first csc
+ * entry (bits 24-50) and the count (bits 51-63). Hence V <= 2^22 voxels and
+ * blocks of at most 8191 histories. */
+#define SEG_VOX_BITS 22
+#define SEG_IDX_BITS 13
+#define SEG_INLINE 3
+#define SEG_MAX_V ((int64_t)1 << SEG_VOX_BITS)
+#define SEG_MAX_B (((int64_t)1 << SEG_IDX_BITS) - 1)
 
 namespace {
 
@@ -150,7 +159,7 @@ __global__ void k_head_flags(const uint32_t* __restrict__ keys, int64_t E, int32
 __global__ void k_write_segments(const uint32_t* __restrict__ keys, const uint32_t* __restrict__ vals,
                                  const int32_t* __restrict__ flags, const int64_t* __restrict__ segidx, int64_t E,
                                  int vbits, int64_t seg0, int64_t b0, int64_t entry0,
-                                 const int64_t* __restrict__ blk_entry, uint4* __restrict__ segs,
+                                 const int64_t* __restrict__ blk_entry, uint64_t* __restrict__ segs,
                                  uint16_t* __restrict__ csc, int64_t* __restrict__ blk_seg) {
     int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
     if (i >= E) return;
@@ -163,18 +172,13 @@ __global__ void k_write_segments(const uint32_t* __restrict__ keys, const uint32
     int64_t j = i + 1;
     while (j < E && !flags[j]) ++j;               /* a new block always starts a new segment */
     uint32_t c = (uint32_t)(j - i);
-    uint4 r;
-    r.x = k & ((1u << vbits) - 1u);
+    uint64_t r = (uint64_t)(k & ((1u << vbits) - 1u));
     if (c <= SEG_INLINE) {
-        uint32_t id[SEG_INLINE] = {0u, 0u, 0u, 0u, 0u};
-        for (uint32_t q = 0; q < c; ++q) id[q] = vals[i + q] & 0xffffu;
-        r.y = c | (id[0] << 16);
-        r.z = id[1] | (id[2] << 16);
-        r.w = id[3] | (id[4] << 16);
-    } else {
-        r.y = 0u;                                   /* count 0 marks a record with its indices in csc */
-        r.z = (uint32_t)(gi - blk_entry[b]);        /* block-relative first entry */
-        r.w = c;
+        r |= (uint64_t)c << SEG_VOX_BITS;
+        for (uint32_t q = 0; q < c; ++q) r |= (uint64_t)(vals[i + q] & 0x1fffu) << (24 + SEG_IDX_BITS * q);
+    } else {                                        /* inline count 0: indices in csc */
+        r |= (uint64_t)(gi - blk_entry[b]) << 24;   /* block-relative first entry */
+        r |= (uint64_t)c << 51;
     }
     segs[s] = r;
     if (gi == blk_entry[b]) blk_seg[b] = s;
@@ -230,32 +234,28 @@ k_block_delta(const uint32_t* __restrict__ list, int stride, const uint32_t* __r
 
 /* csc points at the block's first entry */
 __global__ void __launch_bounds__(256)
-k_block_apply(const uint4* __restrict__ segs, int64_t s0, int64_t ns, const uint16_t* __restrict__ csc,
+k_block_apply(const uint64_t* __restrict__ segs, int64_t s0, int64_t ns, const uint16_t* __restrict__ csc,
               const double* __restrict__ delta, float* __restrict__ x) {
     __builtin_amdgcn_s_setprio(3);   /* latency-bound: issue ahead of the MLP waves sharing the CU */
     int64_t k = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
     if (k >= ns) return;
-    const uint4 r = segs[s0 + k];
-    const uint32_t v = r.x;
-    const uint32_t c = r.y & 0xffffu;
+    const uint64_t r = segs[s0 + k];
+    const uint32_t v = (uint32_t)(r & (SEG_MAX_V - 1));
+    const uint32_t c = (uint32_t)(r >> SEG_VOX_BITS) & 3u;
     const float xv = x[v];
     double upd = 0.0;
     uint32_t cnt;
     if (c != 0u) {
-        const double d0 = delta[r.y >> 16];
-        const double d1 = c > 1u ? delta[r.z & 0xffffu] : 0.0;
-        const double d2 = c > 2u ? delta[r.z >> 16] : 0.0;
-        const double d3 = c > 3u ? delta[r.w & 0xffffu] : 0.0;
-        const double d4 = c > 4u ? delta[r.w >> 16] : 0.0;
+        const double d0 = delta[(r >> 24) & 0x1fffu];
+        const double d1 = c > 1u ? delta[(r >> 37) & 0x1fffu] : 0.0;
+        const double d2 = c > 2u ? delta[(r >> 50) & 0x1fffu] : 0.0;
         upd += d0;
         if (c > 1u) upd += d1;
         if (c > 2u) upd += d2;
-        if (c > 3u) upd += d3;
-        if (c > 4u) upd += d4;
         cnt = c;
     } else {
-        cnt = r.w;
-        const uint16_t* q = csc + r.z;
+        cnt = (uint32_t)(r >> 51);
+        const uint16_t* q = csc + ((r >> 24) & 0x7ffffffu);
         for (uint32_t i = 0; i < cnt; ++i) upd += delta[q[i]];
     }
     x[v] = (float)((double)xv + upd / (double)cnt);
@@ -311,32 +311,28 @@ k_block_delta_f32(const uint32_t* __restrict__ list, int stride, const uint32_t*
 }
 
 __global__ void __launch_bounds__(256)
-k_block_apply_f32(const uint4* __restrict__ segs, int64_t s0, int64_t ns, const uint16_t* __restrict__ csc,
+k_block_apply_f32(const uint64_t* __restrict__ segs, int64_t s0, int64_t ns, const uint16_t* __restrict__ csc,
                   const float* __restrict__ delta, float* __restrict__ x) {
     __builtin_amdgcn_s_setprio(3);   /* latency-bound: issue ahead of the MLP waves sharing the CU */
     int64_t k = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
     if (k >= ns) return;
-    const uint4 r = segs[s0 + k];
-    const uint32_t v = r.x;
-    const uint32_t c = r.y & 0xffffu;
+    const uint64_t r = segs[s0 + k];
+    const uint32_t v = (uint32_t)(r & (SEG_MAX_V - 1));
+    const uint32_t c = (uint32_t)(r >> SEG_VOX_BITS) & 3u;
     const float xv = x[v];
     float upd = 0.0f, comp = 0.0f;
     uint32_t cnt;
     if (c != 0u) {
-        const float d0 = delta[r.y >> 16];
-        const float d1 = c > 1u ? delta[r.z & 0xffffu] : 0.0f;
-        const float d2 = c > 2u ? delta[r.z >> 16] : 0.0f;
-        const float d3 = c > 3u ? delta[r.w & 0xffffu] : 0.0f;
-        const float d4 = c > 4u ? delta[r.w >> 16] : 0.0f;
+        const float d0 = delta[(r >> 24) & 0x1fffu];
+        const float d1 = c > 1u ? delta[(r >> 37) & 0x1fffu] : 0.0f;
+        const float d2 = c > 2u ? delta[(r >> 50) & 0x1fffu] : 0.0f;
         kahan_add(upd, comp, d0);
         if (c > 1u) kahan_add(upd, comp, d1);
         if (c > 2u) kahan_add(upd, comp, d2);
-        if (c > 3u) kahan_add(upd, comp, d3);
-        if (c > 4u) kahan_add(upd, comp, d4);
         cnt = c;
     } else {
-        cnt = r.w;
-        const uint16_t* q = csc + r.z;
+        cnt = (uint32_t)(r >> 51);
+        const uint16_t* q = csc + ((r >> 24) & 0x7ffffffu);
         for (uint32_t i = 0; i < cnt; ++i) kahan_add(upd, comp, delta[q[i]]);
     }
     x[v] = xv + upd / (float)cnt;
@@ -386,9 +382,12 @@ static int vbits_for(int64_t V) {
 /* Build (or rebuild) the plan of `hist` with blocks of B histories. */
 static int plan_build(pct_ctx* ctx, const pct_hist_soa* hist, int B, int mlp_mode) {
     const int64_t n = hist->n;
-    if (B <= 0 || B > 65536) return pct_fail(ctx, PCT_E_INVALID, "DROP block size %d outside [1, 65536]", B);
+    if (B <= 0 || B > SEG_MAX_B)
+        return pct_fail(ctx, PCT_E_INVALID, "DROP block size %d outside [1, %lld]", B, (long long)SEG_MAX_B);
     const int vbits = vbits_for(ctx->g.V);
-    if (vbits > 28) return pct_fail(ctx, PCT_E_INVALID, "image too large for the plan (%lld voxels)", (long long)ctx->g.V);
+    if (ctx->g.V > SEG_MAX_V)
+        return pct_fail(ctx, PCT_E_INVALID, "image too large for the plan (%lld voxels > %lld)", (long long)ctx->g.V,
+                        (long long)SEG_MAX_V);
     if (!ctx->plan) ctx->plan = new pct_plan();
     pct_plan* pl = ctx->plan;
     plan_release_graph(pl);
@@ -541,7 +540,7 @@ static int plan_build(pct_ctx* ctx, const pct_hist_soa* hist, int B, int mlp_mod
             hipLaunchKernelGGL(k_write_segments, dim3(pct_grid(Ec, 256)), dim3(256), 0, ctx->stream,
                                (const uint32_t*)kout.ptr, (const uint32_t*)vout.ptr, (const int32_t*)fl.ptr,
                                (const int64_t*)sidx.ptr, Ec, vbits, seg0, b0, entry0,
-                               (const int64_t*)pl->blk_entry.ptr, (uint4*)pl->segs.ptr, (uint16_t*)pl->csc.ptr,
+                               (const int64_t*)pl->blk_entry.ptr, (uint64_t*)pl->segs.ptr, (uint16_t*)pl->csc.ptr,
                                (int64_t*)pl->blk_seg.ptr);
             PCT_HIP_CHECK(ctx, hipGetLastError());
             seg0 += Sc;
@@ -549,7 +548,7 @@ static int plan_build(pct_ctx* ctx, const pct_hist_soa* hist, int B, int mlp_mod
         if (pass == 0) {
             S_total = 0;
             for (int64_t s : chunk_segs) S_total += s;
-            if ((rc = pct_ensure(ctx, pl->segs, sizeof(uint4) * (S_total > 0 ? S_total : 1)))) {
+            if ((rc = pct_ensure(ctx, pl->segs, sizeof(uint64_t) * (S_total > 0 ? S_total : 1)))) {
                 free_all();
                 return rc;
             }
@@ -589,7 +588,7 @@ static int plan_block_loop(pct_ctx* ctx, pct_plan* pl, const pct_hist_soa* hist,
     const uint32_t* list = (const uint32_t*)pl->list.ptr;
     const uint32_t* len = (const uint32_t*)pl->len.ptr;
     const double* chord = (const double*)pl->chord.ptr;
-    const uint4* segs = (const uint4*)pl->segs.ptr;
+    const uint64_t* segs = (const uint64_t*)pl->segs.ptr;
     const uint16_t* csc = (const uint16_t*)pl->csc.ptr;
     void* delta = pl->delta.ptr;
     for (int64_t b = b_begin; b < b_end; ++b) {
