@@ -263,6 +263,30 @@ int pct_drop_stats(pct_ctx* ctx, const pct_hist_soa* hist, int block_size, int64
                                      2: MLP first, then the block loop         */
 int pct_set_tuning(pct_ctx* ctx, int key, int value);
 
+/* (e) Batched form: the DROP iterations of several independent scans (each
+ * with its own context, i.e. its own grid and parameters) run together. The
+ * reference reconstructs one scan per process run (main, .cu:24-253); this
+ * entry point has no counterpart and serves callers with many scans. Blocks
+ * are still processed in storage order within every scan; block step j runs
+ * block j of every scan that has one in one pair of launches (one kernel for
+ * the deltas, one for the voxel updates of all those blocks), so the
+ * latency-bound chains of the scans share each launch. Each scan's MLP lists
+ * are recomputed every iteration on a second stream, chunk of steps by chunk
+ * of steps, ahead of the block steps that need them. The batch runs on the
+ * stream of scans[0]; results are identical to pct_drop_iteration per scan.
+ * At most PCT_BATCH_MAX scans. */
+#define PCT_BATCH_MAX 16
+typedef struct pct_batch pct_batch;
+pct_batch* pct_batch_create(pct_ctx* const* scans, int n_scans);
+void       pct_batch_destroy(pct_batch* b);
+/* builds (or keeps) each scan's plan for its histories and block size */
+int        pct_batch_prepare(pct_batch* b, const pct_hist_soa* hists, const int* block_sizes, int mode);
+/* one DROP iteration of every scan: xs[s] is scan s's image, lambdas[s] its
+ * relaxation; mode as pct_drop_iteration (MLP mode | PCT_DROP_F32 |
+ * PCT_DROP_CACHED | PCT_DROP_NO_GRAPH) */
+int        pct_batch_iteration(pct_batch* b, const pct_hist_soa* hists, float* const* xs, const int* block_sizes,
+                               const float* lambdas, int mode);
+
 /* Diagnostics (no counterpart in the reference): evaluates one function of
  * the shared FP64/FP32 math library (pct_math.h) on the device for n inputs,
  * so host/device bit equality can be checked directly.

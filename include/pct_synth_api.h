@@ -32,6 +32,13 @@ typedef struct pct_synth_cfg {
     int32_t  n_cyl;               /* cylinders along z, body first, inserts after  */
     double   cyl_x[PCT_SYNTH_MAX_CYL], cyl_y[PCT_SYNTH_MAX_CYL];
     double   cyl_r[PCT_SYNTH_MAX_CYL], cyl_rsp[PCT_SYNTH_MAX_CYL];
+    /* ellipsoids (used instead of the cylinders when n_ell > 0), body first:
+     * centre [cm], semi-axes [cm], rotation about z [rad], RSP; an ellipsoid
+     * replaces the RSP of the last listed one containing its centre */
+    int32_t  n_ell;
+    double   ell_x[PCT_SYNTH_MAX_CYL], ell_y[PCT_SYNTH_MAX_CYL], ell_z[PCT_SYNTH_MAX_CYL];
+    double   ell_a[PCT_SYNTH_MAX_CYL], ell_b[PCT_SYNTH_MAX_CYL], ell_c[PCT_SYNTH_MAX_CYL];
+    double   ell_phi[PCT_SYNTH_MAX_CYL], ell_rsp[PCT_SYNTH_MAX_CYL];
 } pct_synth_cfg;
 
 /* Raw tracker arrays written by the generator (same fields as pct_raw_soa). */

@@ -13,7 +13,7 @@ from . import configs
 from ._lib import (PCT_DROP_ATOMIC, PCT_DROP_CACHED, PCT_DROP_F32, PCT_DROP_NO_GRAPH, PCT_MLP_FP64, PCT_MLP_TABLE,
                    PCT_PRECISION, PctHistSoa, PctParams, PctRawSoa, PctSynthCfg, PctSynthSoa)
 
-__all__ = ["Context", "Histories", "RawScan", "configs", "PCT_MLP_FP64", "PCT_MLP_TABLE", "PCT_PRECISION",
+__all__ = ["Context", "Batch", "Histories", "RawScan", "configs", "PCT_MLP_FP64", "PCT_MLP_TABLE", "PCT_PRECISION",
            "generate_host"]
 
 
@@ -358,6 +358,69 @@ def last_timings(ctx: "Context"):
     out = (ctypes.c_double * 3)()
     ctx._check(ctx.L.pct_last_timings(ctx._ctx, out), "pct_last_timings")
     return list(out)
+
+
+class Batch:
+    """Several scans reconstructed together (pct_batch_*, include/pct.h): one
+    Context per scan (its own grid and parameters); the batch runs on the
+    first context's stream."""
+
+    def __init__(self, contexts):
+        self.contexts = list(contexts)
+        n = len(self.contexts)
+        self.L = _lib.lib()
+        arr = (ctypes.c_void_p * n)(*[c._ctx for c in self.contexts])
+        self._b = self.L.pct_batch_create(arr, n)
+        if not self._b:
+            raise PctError("pct_batch_create failed (1..16 contexts on one device)")
+
+    def close(self):
+        if getattr(self, "_b", None):
+            self.L.pct_batch_destroy(self._b)
+            self._b = None
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+    def _call(self, fn, *args, what=""):
+        c0 = self.contexts[0]
+        torch = c0.torch
+        cur = torch.cuda.current_stream(c0.device)
+        for c in self.contexts:
+            c.stream.wait_stream(cur)
+            if c is not c0:
+                c0.stream.wait_stream(c.stream)
+        try:
+            rc = fn(self._b, *args)
+        finally:
+            cur.wait_stream(c0.stream)
+            for c in self.contexts[1:]:
+                cur.wait_stream(c.stream)
+        if rc != 0:
+            msg = self.L.pct_last_error(c0._ctx)
+            raise PctError(f"{what} failed ({rc}): {msg.decode() if msg else ''}")
+
+    def _soas(self, hists):
+        n = len(self.contexts)
+        if len(hists) != n:
+            raise PctError(f"{len(hists)} history sets for {n} scans")
+        return (PctHistSoa * n)(*[h.soa() for h in hists])
+
+    def prepare(self, hists, block_sizes, mode=PCT_MLP_FP64):
+        n = len(self.contexts)
+        self._call(self.L.pct_batch_prepare, self._soas(hists), (ctypes.c_int * n)(*block_sizes), int(mode),
+                   what="pct_batch_prepare")
+
+    def iteration(self, hists, xs, block_sizes, lambdas=None, mode=PCT_MLP_FP64):
+        """One DROP iteration of every scan (xs: one float32 image per scan)."""
+        n = len(self.contexts)
+        lambdas = [c.params.lambda_ for c in self.contexts] if lambdas is None else lambdas
+        self._call(self.L.pct_batch_iteration, self._soas(hists), (ctypes.c_void_p * n)(*[x.data_ptr() for x in xs]),
+                   (ctypes.c_int * n)(*block_sizes), (ctypes.c_float * n)(*lambdas), int(mode),
+                   what="pct_batch_iteration")
 
 
 TUNING_KEYS = {"mlp_waves": _lib.PCT_TUNE_MLP_WAVES, "chunk_first": _lib.PCT_TUNE_CHUNK_FIRST,

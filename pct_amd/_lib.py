@@ -69,6 +69,10 @@ class PctSynthCfg(ctypes.Structure):
         ("x0", c_f64), ("straggle", c_f64), ("n_cyl", c_i32),
         ("cyl_x", c_f64 * SYNTH_MAX_CYL), ("cyl_y", c_f64 * SYNTH_MAX_CYL),
         ("cyl_r", c_f64 * SYNTH_MAX_CYL), ("cyl_rsp", c_f64 * SYNTH_MAX_CYL),
+        ("n_ell", c_i32),
+        ("ell_x", c_f64 * SYNTH_MAX_CYL), ("ell_y", c_f64 * SYNTH_MAX_CYL), ("ell_z", c_f64 * SYNTH_MAX_CYL),
+        ("ell_a", c_f64 * SYNTH_MAX_CYL), ("ell_b", c_f64 * SYNTH_MAX_CYL), ("ell_c", c_f64 * SYNTH_MAX_CYL),
+        ("ell_phi", c_f64 * SYNTH_MAX_CYL), ("ell_rsp", c_f64 * SYNTH_MAX_CYL),
     ]
 
 
@@ -111,6 +115,10 @@ SIGNATURES = {
     "pct_last_timings": (ctypes.c_int, [c_vp, c_vp]),
     "pct_drop_stats": (ctypes.c_int, [c_vp, P(PctHistSoa), ctypes.c_int, P(c_i64), P(c_i64)]),
     "pct_set_tuning": (ctypes.c_int, [c_vp, ctypes.c_int, ctypes.c_int]),
+    "pct_batch_create": (c_vp, [P(c_vp), ctypes.c_int]),
+    "pct_batch_destroy": (None, [c_vp]),
+    "pct_batch_prepare": (ctypes.c_int, [c_vp, P(PctHistSoa), P(ctypes.c_int), ctypes.c_int]),
+    "pct_batch_iteration": (ctypes.c_int, [c_vp, P(PctHistSoa), P(c_vp), P(ctypes.c_int), P(c_f32), ctypes.c_int]),
     "pct_selftest_math": (ctypes.c_int, [c_vp, ctypes.c_int, c_vp, c_vp, c_vp, c_i64]),
     "pct_synth_generate": (ctypes.c_int, [c_vp, P(PctSynthCfg), c_i64, c_i64, P(PctSynthSoa)]),
     "pct_synth_generate_host": (ctypes.c_int, [P(PctSynthCfg), c_i64, c_i64, P(PctSynthSoa)]),

@@ -9,7 +9,7 @@ import sys
 HERE = os.path.dirname(os.path.abspath(__file__))
 CSRC = os.path.join(HERE, "csrc")
 OUT = os.path.join(HERE, "libpct.so")
-SOURCES = ["pct_ctx.hip", "pct_intersect.hip", "pct_mlp.hip", "pct_drop.hip", "pct_plan.hip",
+SOURCES = ["pct_ctx.hip", "pct_intersect.hip", "pct_mlp.hip", "pct_drop.hip", "pct_plan.hip", "pct_batch.hip",
            "pct_bins.hip", "pct_hull.hip", "pct_fbp.hip"]
 
 HIPCC_FLAGS = [

@@ -106,3 +106,90 @@ def phantom_rsp(cfg: _lib.PctSynthCfg, p: _lib.PctParams):
         inside = (X - cfg.cyl_x[k]) ** 2 + (Y - cfg.cyl_y[k]) ** 2 < cfg.cyl_r[k] ** 2
         img[inside] = cfg.cyl_rsp[k]
     return np.broadcast_to(img, (S, R, C)).copy()
+
+
+# ---------------------------------------------------------------------------
+# Batched form: 12 independent scans of uneven size in one DROP iteration
+# ---------------------------------------------------------------------------
+BATCH_SCANS = 12
+BATCH_GRIDS_XY = (64, 96, 128, 160, 192, 224, 256)
+BATCH_SLICES = (8, 16, 24, 32, 40, 48)
+BATCH_HISTORIES = (500_000, 30_000_000)      # log-uniform range
+BATCH_BLOCK_SIZES = (800, 1600, 3200, 6400)  # the tuner's candidates
+BATCH_RSP = (0.2, 1.8)                       # uniform range of every body's RSP
+
+
+@dataclass
+class BatchScan:
+    name: str
+    params: object        # pct_params
+    synth: object         # pct_synth_cfg
+
+
+def _random_ellipsoids(rng, height):
+    """body + 3-8 non-overlapping inserts inside it (rejection sampling)"""
+    body = dict(x=rng.uniform(-0.5, 0.5), y=rng.uniform(-0.5, 0.5), z=0.0, a=rng.uniform(7.0, 9.5),
+                b=rng.uniform(6.0, 9.5), c=height / 2 + 2.0, phi=rng.uniform(0, math.pi), rsp=rng.uniform(*BATCH_RSP))
+    out = [body]
+    want = int(rng.integers(3, 9))
+    tries = 0
+    while len(out) < 1 + want and tries < 2000:
+        tries += 1
+        a, b, c = rng.uniform(0.5, 2.5), rng.uniform(0.5, 2.5), rng.uniform(0.5, max(0.6, height / 2))
+        r = max(a, b)
+        ang, rad = rng.uniform(0, 2 * math.pi), rng.uniform(0, min(body["a"], body["b"]) - r - 0.3)
+        if rad < 0:
+            continue
+        e = dict(x=body["x"] + rad * math.cos(ang), y=body["y"] + rad * math.sin(ang), z=rng.uniform(-0.5, 0.5),
+                 a=a, b=b, c=c, phi=rng.uniform(0, math.pi), rsp=rng.uniform(*BATCH_RSP))
+        if all(math.hypot(e["x"] - o["x"], e["y"] - o["y"]) > r + max(o["a"], o["b"]) + 0.2 for o in out[1:]):
+            out.append(e)
+    return out
+
+
+def batch_scans(seed: int = SEED_BASE + 4, scale: float = 1.0):
+    """The batched configuration: 12 scans, grids from 64x64x8 to 256x256x48
+    (the first and last scan take the two ends), histories log-uniform in
+    [0.5 M, 30 M] (times `scale`), random ellipsoid phantoms with RSP in
+    U[0.2, 1.8], 90 angles, max_intersections 1024 (paths up to ~450 voxels)."""
+    import numpy as np
+    rng = np.random.default_rng(seed)
+    out = []
+    for k in range(BATCH_SCANS):
+        if k == 0:
+            cxy, s = 64, 8
+        elif k == BATCH_SCANS - 1:
+            cxy, s = 256, 48
+        else:
+            cxy, s = int(rng.choice(BATCH_GRIDS_XY)), int(rng.choice(BATCH_SLICES))
+        lo, hi = math.log(BATCH_HISTORIES[0]), math.log(BATCH_HISTORIES[1])
+        n = int(round(math.exp(rng.uniform(lo, hi)) * scale))
+        p = _lib.default_params()
+        p.columns = p.rows = cxy
+        p.slices = s
+        p.slice_thickness = 0.25
+        p.recon_cyl_height = s * 0.25
+        p.lambda_ = 1e-4
+        p.drop_block_size = 3200
+        p.iterations = 12
+        p.max_intersections = 1024
+        c = _lib.PctSynthCfg()
+        c.seed = seed * 1000 + k
+        c.n_histories = n
+        c.n_angles = 90
+        c.angle_step_deg = 4.0
+        c.u_in_1, c.u_in_2, c.u_out_1, c.u_out_2 = -25.0, -15.0, 15.0, 25.0
+        c.t_half = 11.0
+        c.v_half = max(0.5, p.recon_cyl_height / 2 - 0.5)
+        c.source_radius = 260.7
+        c.beta_cp = BETA_CP_200MEV
+        c.x0 = 36.08
+        c.straggle = 0.01
+        c.n_cyl = 0
+        ells = _random_ellipsoids(rng, p.recon_cyl_height)
+        c.n_ell = len(ells)
+        for j, e in enumerate(ells):
+            for f in ("x", "y", "z", "a", "b", "c", "phi", "rsp"):
+                getattr(c, "ell_" + f)[j] = e[f]
+        out.append(BatchScan(f"scan{k:02d}_{cxy}x{cxy}x{s}_{n}", p, c))
+    return out

@@ -49,5 +49,8 @@ struct pct_plan {
 };
 
 void pct_plan_destroy(pct_plan* pl);
+/* (re)build the plan of ctx for hist with blocks of B and the given MLP mode */
+int pct_plan_build(pct_ctx* ctx, const pct_hist_soa* hist, int B, int mlp_mode);
+bool pct_plan_ready(const pct_ctx* ctx, const pct_hist_soa* hist, int B, int mlp_mode);
 
 #endif

@@ -47,6 +47,7 @@
 #include <rocprim/device/device_radix_sort.hpp>
 #include "pct_internal.h"
 #include "pct_plan.h"
+#include "pct_plan_kernels.h"
 
 int pct_mlp_capacity_impl(pct_ctx* ctx, const pct_hist_soa* hist, int64_t start, int64_t count, int64_t* offsets,
                           int64_t* total);
@@ -60,17 +61,7 @@ int pct_mlp_paths_on(pct_ctx* ctx, hipStream_t stream, const pct_hist_soa* hist,
                      const int64_t* offsets, uint32_t* voxels, uint32_t* lengths, double* chord, uint32_t* hash_out,
                      const uint32_t* hash_expect, int mode);
 
-#define LIST_SENTINEL 0xFFFFFFFFu
-/* 8-byte segment record: bits 0-21 voxel, 22-23 inline count (1-3; 0 = the
- * history indices are in csc), then either up to three 13-bit block-local
- * history indices (bits 24-36, 37-49, 50-62) or the block-relative first csc
- * entry (bits 24-50) and the count (bits 51-63). Hence V <= 2^22 voxels and
- * blocks of at most 8191 histories. */
-#define SEG_VOX_BITS 22
-#define SEG_IDX_BITS 13
-#define SEG_INLINE 3
-#define SEG_MAX_V ((int64_t)1 << SEG_VOX_BITS)
-#define SEG_MAX_B (((int64_t)1 << SEG_IDX_BITS) - 1)
+
 
 namespace {
 
@@ -187,155 +178,45 @@ __global__ void k_write_segments(const uint32_t* __restrict__ keys, const uint32
 /* ------------------------------------------------------------------------- */
 /* iteration kernels                                                          */
 /* ------------------------------------------------------------------------- */
-#define DELTA_WAVES 4
-#define DELTA_SLOTS 8
-
-__device__ __forceinline__ double wave_sum_d(double v) {
-#pragma unroll
-    for (int d = 32; d >= 1; d >>= 1) v += __shfl_xor(v, d, 64);
-    return v;
-}
-
-/* list points at the block's first list; history k's list is at k * stride */
+/* the block loop's kernels: bodies in pct_plan_kernels.h. Both raise their
+ * wave priority: they are latency-bound and share the CUs with the
+ * long-running MLP waves of the next chunks. */
 __global__ void __launch_bounds__(64 * DELTA_WAVES)
 k_block_delta(const uint32_t* __restrict__ list, int stride, const uint32_t* __restrict__ len,
               const double* __restrict__ chord, const float* __restrict__ wepl, int64_t h0, int64_t count,
               const float* __restrict__ x, double lambda, double* __restrict__ delta) {
-    __builtin_amdgcn_s_setprio(3);   /* latency-bound: issue ahead of the MLP waves sharing the CU */
-    const int lane = threadIdx.x & 63;
+    __builtin_amdgcn_s_setprio(3);
     const int64_t k = (int64_t)blockIdx.x * DELTA_WAVES + (threadIdx.x >> 6);
     if (k >= count) return;
-    const int64_t h = h0 + k;
-    const uint32_t n = len[h];
-    const double L = chord[h];
-    const float b_i = wepl[h];
-    const uint32_t* p = list + k * stride;
-    uint32_t v[DELTA_SLOTS];
-#pragma unroll
-    for (int r = 0; r < DELTA_SLOTS; ++r) {
-        int j = lane + 64 * r;
-        v[r] = j < stride ? p[j] : LIST_SENTINEL;
-    }
-    double s = 0.0;
-#pragma unroll
-    for (int r = 0; r < DELTA_SLOTS; ++r)
-        if (v[r] != LIST_SENTINEL) s += (double)x[v[r]];
-    for (int j = lane + 64 * DELTA_SLOTS; j < stride; j += 64) {
-        uint32_t w = p[j];
-        if (w != LIST_SENTINEL) s += (double)x[w];
-    }
-    s = wave_sum_d(s);
-    if (lane == 0) {
-        const double a_i_dot_x_k = s * L;
-        const double a_i_dot_a_i = (L * L) * n;
-        delta[k] = L * ((b_i - a_i_dot_x_k) / a_i_dot_a_i) * lambda;
-    }
-}
-
-/* csc points at the block's first entry */
-__global__ void __launch_bounds__(256)
-k_block_apply(const uint64_t* __restrict__ segs, int64_t s0, int64_t ns, const uint16_t* __restrict__ csc,
-              const double* __restrict__ delta, float* __restrict__ x) {
-    __builtin_amdgcn_s_setprio(3);   /* latency-bound: issue ahead of the MLP waves sharing the CU */
-    int64_t k = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
-    if (k >= ns) return;
-    const uint64_t r = segs[s0 + k];
-    const uint32_t v = (uint32_t)(r & (SEG_MAX_V - 1));
-    const uint32_t c = (uint32_t)(r >> SEG_VOX_BITS) & 3u;
-    const float xv = x[v];
-    double upd = 0.0;
-    uint32_t cnt;
-    if (c != 0u) {
-        const double d0 = delta[(r >> 24) & 0x1fffu];
-        const double d1 = c > 1u ? delta[(r >> 37) & 0x1fffu] : 0.0;
-        const double d2 = c > 2u ? delta[(r >> 50) & 0x1fffu] : 0.0;
-        upd += d0;
-        if (c > 1u) upd += d1;
-        if (c > 2u) upd += d2;
-        cnt = c;
-    } else {
-        cnt = (uint32_t)(r >> 51);
-        const uint16_t* q = csc + ((r >> 24) & 0x7ffffffu);
-        for (uint32_t i = 0; i < cnt; ++i) upd += delta[q[i]];
-    }
-    x[v] = (float)((double)xv + upd / (double)cnt);
-}
-
-/* Precision form of the update: the same two kernels in FP32 with
- * Kahan-compensated sums (a_i.x per history, x_update per voxel). */
-__device__ __forceinline__ void kahan_add(float& s, float& c, float v) {
-    float y = v - c;
-    float t = s + y;
-    c = (t - s) - y;
-    s = t;
+    pct_blk::delta_f64(list, stride, len, chord, wepl, h0, k, x, lambda, delta, threadIdx.x & 63);
 }
 
 __global__ void __launch_bounds__(64 * DELTA_WAVES)
 k_block_delta_f32(const uint32_t* __restrict__ list, int stride, const uint32_t* __restrict__ len,
                   const double* __restrict__ chord, const float* __restrict__ wepl, int64_t h0, int64_t count,
                   const float* __restrict__ x, float lambda, float* __restrict__ delta) {
-    __builtin_amdgcn_s_setprio(3);   /* latency-bound: issue ahead of the MLP waves sharing the CU */
-    const int lane = threadIdx.x & 63;
+    __builtin_amdgcn_s_setprio(3);
     const int64_t k = (int64_t)blockIdx.x * DELTA_WAVES + (threadIdx.x >> 6);
     if (k >= count) return;
-    const int64_t h = h0 + k;
-    const uint32_t n = len[h];
-    const float L = (float)chord[h];
-    const float b_i = wepl[h];
-    const uint32_t* p = list + k * stride;
-    uint32_t v[DELTA_SLOTS];
-#pragma unroll
-    for (int r = 0; r < DELTA_SLOTS; ++r) {
-        int j = lane + 64 * r;
-        v[r] = j < stride ? p[j] : LIST_SENTINEL;
-    }
-    float s = 0.0f, c = 0.0f;
-#pragma unroll
-    for (int r = 0; r < DELTA_SLOTS; ++r)
-        if (v[r] != LIST_SENTINEL) kahan_add(s, c, x[v[r]]);
-    for (int j = lane + 64 * DELTA_SLOTS; j < stride; j += 64) {
-        uint32_t w = p[j];
-        if (w != LIST_SENTINEL) kahan_add(s, c, x[w]);
-    }
-#pragma unroll
-    for (int d = 32; d >= 1; d >>= 1) {
-        float os = __shfl_xor(s, d, 64), oc = __shfl_xor(c, d, 64);
-        c += oc;
-        kahan_add(s, c, os);
-    }
-    if (lane == 0) {
-        const float a_i_dot_x_k = s * L;
-        const float a_i_dot_a_i = (L * L) * (float)n;
-        delta[k] = L * ((b_i - a_i_dot_x_k) / a_i_dot_a_i) * lambda;
-    }
+    pct_blk::delta_f32(list, stride, len, chord, wepl, h0, k, x, lambda, delta, threadIdx.x & 63);
+}
+
+__global__ void __launch_bounds__(256)
+k_block_apply(const uint64_t* __restrict__ segs, int64_t s0, int64_t ns, const uint16_t* __restrict__ csc,
+              const double* __restrict__ delta, float* __restrict__ x) {
+    __builtin_amdgcn_s_setprio(3);
+    int64_t k = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (k >= ns) return;
+    pct_blk::apply_f64(segs[s0 + k], csc, delta, x);
 }
 
 __global__ void __launch_bounds__(256)
 k_block_apply_f32(const uint64_t* __restrict__ segs, int64_t s0, int64_t ns, const uint16_t* __restrict__ csc,
                   const float* __restrict__ delta, float* __restrict__ x) {
-    __builtin_amdgcn_s_setprio(3);   /* latency-bound: issue ahead of the MLP waves sharing the CU */
+    __builtin_amdgcn_s_setprio(3);
     int64_t k = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
     if (k >= ns) return;
-    const uint64_t r = segs[s0 + k];
-    const uint32_t v = (uint32_t)(r & (SEG_MAX_V - 1));
-    const uint32_t c = (uint32_t)(r >> SEG_VOX_BITS) & 3u;
-    const float xv = x[v];
-    float upd = 0.0f, comp = 0.0f;
-    uint32_t cnt;
-    if (c != 0u) {
-        const float d0 = delta[(r >> 24) & 0x1fffu];
-        const float d1 = c > 1u ? delta[(r >> 37) & 0x1fffu] : 0.0f;
-        const float d2 = c > 2u ? delta[(r >> 50) & 0x1fffu] : 0.0f;
-        kahan_add(upd, comp, d0);
-        if (c > 1u) kahan_add(upd, comp, d1);
-        if (c > 2u) kahan_add(upd, comp, d2);
-        cnt = c;
-    } else {
-        cnt = (uint32_t)(r >> 51);
-        const uint16_t* q = csc + ((r >> 24) & 0x7ffffffu);
-        for (uint32_t i = 0; i < cnt; ++i) kahan_add(upd, comp, delta[q[i]]);
-    }
-    x[v] = xv + upd / (float)cnt;
+    pct_blk::apply_f32(segs[s0 + k], csc, delta, x);
 }
 
 }  // namespace
@@ -579,6 +460,15 @@ static int plan_build(pct_ctx* ctx, const pct_hist_soa* hist, int B, int mlp_mod
 static bool same_hist(const pct_hist_soa& a, const pct_hist_soa& b) {
     return a.n == b.n && a.x_entry == b.x_entry && a.wepl == b.wepl && a.first_voxel == b.first_voxel &&
            a.xy_entry_angle == b.xy_entry_angle && a.x_exit == b.x_exit;
+}
+
+/* internal entry points for the batched form (pct_batch.hip) */
+int pct_plan_build(pct_ctx* ctx, const pct_hist_soa* hist, int B, int mlp_mode) {
+    return plan_build(ctx, hist, B, mlp_mode);
+}
+bool pct_plan_ready(const pct_ctx* ctx, const pct_hist_soa* hist, int B, int mlp_mode) {
+    const pct_plan* pl = ctx->plan;
+    return pl && pl->valid && pl->B == B && pl->mlp_mode == mlp_mode && same_hist(pl->key, *hist);
 }
 
 /* the block loop over blocks [b_begin, b_end) on the plan */

@@ -1,0 +1,167 @@
+/*
+ * pct_plan_kernels.h -- the per-block DROP kernels' bodies (one history's
+ * delta, one voxel's update), shared by the single-scan block loop
+ * (pct_plan.hip) and the batched multi-scan block loop (pct_batch.hip).
+ *
+ * delta_i = L (b_i - L sum_{v in path} x_v) / (n L^2) lambda
+ *     (block_update_GPU, .cu:5822-5834), a.x summed per lane over the list
+ *     slots then by a butterfly over the wavefront;
+ * x_v += (sum of the deltas of the block's histories through v, in history
+ *     order) / count (DROP_blocks + DROP_update, .cu:8023-8055).
+ * FP64 as the oracle, or FP32 with Kahan-compensated sums (precision form).
+ */
+#ifndef PCT_PLAN_KERNELS_H
+#define PCT_PLAN_KERNELS_H
+
+#include <stdint.h>
+#include <hip/hip_runtime.h>
+
+#define LIST_SENTINEL 0xFFFFFFFFu
+/* 8-byte segment record: bits 0-21 voxel, 22-23 inline count (1-3; 0 = the
+ * history indices are in csc), then either up to three 13-bit block-local
+ * history indices (bits 24-36, 37-49, 50-62) or the block-relative first csc
+ * entry (bits 24-50) and the count (bits 51-63). Hence V <= 2^22 voxels and
+ * blocks of at most 8191 histories. */
+#define SEG_VOX_BITS 22
+#define SEG_IDX_BITS 13
+#define SEG_INLINE 3
+#define SEG_MAX_V ((int64_t)1 << SEG_VOX_BITS)
+#define SEG_MAX_B (((int64_t)1 << SEG_IDX_BITS) - 1)
+
+#define DELTA_WAVES 4
+#define DELTA_SLOTS 8
+
+namespace pct_blk {
+
+__device__ __forceinline__ double wave_sum_d(double v) {
+#pragma unroll
+    for (int d = 32; d >= 1; d >>= 1) v += __shfl_xor(v, d, 64);
+    return v;
+}
+
+__device__ __forceinline__ void kahan_add(float& s, float& c, float v) {
+    float y = v - c;
+    float t = s + y;
+    c = (t - s) - y;
+    s = t;
+}
+
+/* history k of the block (one wavefront); list points at the block's first
+ * list, history k's list is at k * stride; h0 = first history of the block */
+__device__ __forceinline__ void delta_f64(const uint32_t* __restrict__ list, int stride, const uint32_t* __restrict__ len,
+                                          const double* __restrict__ chord, const float* __restrict__ wepl, int64_t h0,
+                                          int64_t k, const float* __restrict__ x, double lambda,
+                                          double* __restrict__ delta, int lane) {
+    const int64_t h = h0 + k;
+    const uint32_t n = len[h];
+    const double L = chord[h];
+    const float b_i = wepl[h];
+    const uint32_t* p = list + k * stride;
+    uint32_t v[DELTA_SLOTS];
+#pragma unroll
+    for (int r = 0; r < DELTA_SLOTS; ++r) {
+        int j = lane + 64 * r;
+        v[r] = j < stride ? p[j] : LIST_SENTINEL;
+    }
+    double s = 0.0;
+#pragma unroll
+    for (int r = 0; r < DELTA_SLOTS; ++r)
+        if (v[r] != LIST_SENTINEL) s += (double)x[v[r]];
+    for (int j = lane + 64 * DELTA_SLOTS; j < stride; j += 64) {
+        uint32_t w = p[j];
+        if (w != LIST_SENTINEL) s += (double)x[w];
+    }
+    s = wave_sum_d(s);
+    if (lane == 0) {
+        const double a_i_dot_x_k = s * L;
+        const double a_i_dot_a_i = (L * L) * n;
+        delta[k] = L * ((b_i - a_i_dot_x_k) / a_i_dot_a_i) * lambda;
+    }
+}
+
+__device__ __forceinline__ void delta_f32(const uint32_t* __restrict__ list, int stride, const uint32_t* __restrict__ len,
+                                          const double* __restrict__ chord, const float* __restrict__ wepl, int64_t h0,
+                                          int64_t k, const float* __restrict__ x, float lambda,
+                                          float* __restrict__ delta, int lane) {
+    const int64_t h = h0 + k;
+    const uint32_t n = len[h];
+    const float L = (float)chord[h];
+    const float b_i = wepl[h];
+    const uint32_t* p = list + k * stride;
+    uint32_t v[DELTA_SLOTS];
+#pragma unroll
+    for (int r = 0; r < DELTA_SLOTS; ++r) {
+        int j = lane + 64 * r;
+        v[r] = j < stride ? p[j] : LIST_SENTINEL;
+    }
+    float s = 0.0f, c = 0.0f;
+#pragma unroll
+    for (int r = 0; r < DELTA_SLOTS; ++r)
+        if (v[r] != LIST_SENTINEL) kahan_add(s, c, x[v[r]]);
+    for (int j = lane + 64 * DELTA_SLOTS; j < stride; j += 64) {
+        uint32_t w = p[j];
+        if (w != LIST_SENTINEL) kahan_add(s, c, x[w]);
+    }
+#pragma unroll
+    for (int d = 32; d >= 1; d >>= 1) {
+        float os = __shfl_xor(s, d, 64), oc = __shfl_xor(c, d, 64);
+        c += oc;
+        kahan_add(s, c, os);
+    }
+    if (lane == 0) {
+        const float a_i_dot_x_k = s * L;
+        const float a_i_dot_a_i = (L * L) * (float)n;
+        delta[k] = L * ((b_i - a_i_dot_x_k) / a_i_dot_a_i) * lambda;
+    }
+}
+
+/* segment record r of a block; csc points at the block's first entry */
+__device__ __forceinline__ void apply_f64(uint64_t r, const uint16_t* __restrict__ csc,
+                                          const double* __restrict__ delta, float* __restrict__ x) {
+    const uint32_t v = (uint32_t)(r & (SEG_MAX_V - 1));
+    const uint32_t c = (uint32_t)(r >> SEG_VOX_BITS) & 3u;
+    const float xv = x[v];
+    double upd = 0.0;
+    uint32_t cnt;
+    if (c != 0u) {
+        const double d0 = delta[(r >> 24) & 0x1fffu];
+        const double d1 = c > 1u ? delta[(r >> 37) & 0x1fffu] : 0.0;
+        const double d2 = c > 2u ? delta[(r >> 50) & 0x1fffu] : 0.0;
+        upd += d0;
+        if (c > 1u) upd += d1;
+        if (c > 2u) upd += d2;
+        cnt = c;
+    } else {
+        cnt = (uint32_t)(r >> 51);
+        const uint16_t* q = csc + ((r >> 24) & 0x7ffffffu);
+        for (uint32_t i = 0; i < cnt; ++i) upd += delta[q[i]];
+    }
+    x[v] = (float)((double)xv + upd / (double)cnt);
+}
+
+__device__ __forceinline__ void apply_f32(uint64_t r, const uint16_t* __restrict__ csc,
+                                          const float* __restrict__ delta, float* __restrict__ x) {
+    const uint32_t v = (uint32_t)(r & (SEG_MAX_V - 1));
+    const uint32_t c = (uint32_t)(r >> SEG_VOX_BITS) & 3u;
+    const float xv = x[v];
+    float upd = 0.0f, comp = 0.0f;
+    uint32_t cnt;
+    if (c != 0u) {
+        const float d0 = delta[(r >> 24) & 0x1fffu];
+        const float d1 = c > 1u ? delta[(r >> 37) & 0x1fffu] : 0.0f;
+        const float d2 = c > 2u ? delta[(r >> 50) & 0x1fffu] : 0.0f;
+        kahan_add(upd, comp, d0);
+        if (c > 1u) kahan_add(upd, comp, d1);
+        if (c > 2u) kahan_add(upd, comp, d2);
+        cnt = c;
+    } else {
+        cnt = (uint32_t)(r >> 51);
+        const uint16_t* q = csc + ((r >> 24) & 0x7ffffffu);
+        for (uint32_t i = 0; i < cnt; ++i) kahan_add(upd, comp, delta[q[i]]);
+    }
+    x[v] = xv + upd / (float)cnt;
+}
+
+}  // namespace pct_blk
+
+#endif

@@ -15,9 +15,9 @@
  *     extra history.
  *   - Cone-beam entry: hit (t, v) uniform at the inner entry plane, direction
  *     from a point source at u = -SOURCE_RADIUS.
- *   - Phantom: cylinders along z (infinite height); RSP of a point = the RSP
- *     of the last listed cylinder containing it. Line integrals are exact
- *     chord lengths per cylinder.
+ *   - Phantom: cylinders along z (infinite height) or rotated ellipsoids;
+ *     RSP of a point = the RSP of the last listed body containing it (inserts
+ *     nested in the body). Line integrals are exact chord lengths per body.
  *   - Multiple Coulomb scattering: at the object exit, per projection plane,
  *     displacement and angle from the PDG joint-Gaussian prescription
  *     (y = L theta0 (z1/sqrt(12) + z2/2), theta = theta0 z2) with the Highland
@@ -84,6 +84,40 @@ PCT_HD int pct_synth_chord(double t0, double m, double cu, double ct, double r, 
     return 1;
 }
 
+/* chord interval [s0, s1] (in u) of the line (u, t0 + m_t u, v0 + m_v u) with
+ * ellipsoid k of the configuration, in the gantry frame at angle theta
+ * (cos ct, sin st); returns 0 when disjoint */
+PCT_HD int pct_synth_ell_chord(const pct_synth_cfg* g, int k, double ct, double st, double t0, double m_t, double v0,
+                               double m_v, double* s0, double* s1) {
+    double cu = ct * g->ell_x[k] + st * g->ell_y[k];
+    double ctt = ct * g->ell_y[k] - st * g->ell_x[k];
+    /* principal axes at angle phi - theta in the (u, t) plane */
+    double sa, ca;
+    pct_sincos(g->ell_phi[k] - pct_atan2(st, ct), &sa, &ca);
+    double pu = -cu, pt = t0 - ctt, pv = v0 - g->ell_z[k];
+    double q0x = (ca * pu + sa * pt) / g->ell_a[k], qdx = (ca + sa * m_t) / g->ell_a[k];
+    double q0y = (-sa * pu + ca * pt) / g->ell_b[k], qdy = (-sa + ca * m_t) / g->ell_b[k];
+    double q0z = pv / g->ell_c[k], qdz = m_v / g->ell_c[k];
+    double A = qdx * qdx + qdy * qdy + qdz * qdz;
+    double Bh = q0x * qdx + q0y * qdy + q0z * qdz;
+    double Cc = q0x * q0x + q0y * q0y + q0z * q0z - 1.0;
+    double disc = Bh * Bh - A * Cc;
+    if (!(disc > 0.0)) return 0;
+    double sq = pct_sqrt(disc);
+    *s0 = (-Bh - sq) / A;
+    *s1 = (-Bh + sq) / A;
+    return 1;
+}
+
+/* 1 when the centre of ellipsoid k lies inside ellipsoid j */
+PCT_HD int pct_synth_ell_contains(const pct_synth_cfg* g, int j, int k) {
+    double dx = g->ell_x[k] - g->ell_x[j], dy = g->ell_y[k] - g->ell_y[j], dz = g->ell_z[k] - g->ell_z[j];
+    double sp, cp;
+    pct_sincos(g->ell_phi[j], &sp, &cp);
+    double lx = (cp * dx + sp * dy) / g->ell_a[j], ly = (-sp * dx + cp * dy) / g->ell_b[j], lz = dz / g->ell_c[j];
+    return lx * lx + ly * ly + lz * lz <= 1.0;
+}
+
 typedef struct pct_synth_record {
     int32_t gantry_angle;
     float t_in_1, t_in_2, t_out_1, t_out_2;
@@ -112,10 +146,20 @@ PCT_HD void pct_synth_history(const pct_synth_cfg* g, int64_t i, pct_synth_recor
     double v0 = v2 - m_v * g->u_in_2;
     double ds_du = pct_sqrt(1.0 + m_t * m_t + m_v * m_v);
 
-    /* exact line integral: RSP of the last containing cylinder */
+    /* exact line integral: RSP of the last containing cylinder / ellipsoid */
     double wepl_true = 0.0, u_exit_obj = g->u_in_2, path_len = 0.0;
     int hit = 0;
-    for (int k = 0; k < g->n_cyl; ++k) {
+    for (int k = 0; k < g->n_ell; ++k) {
+        double s0, s1;
+        if (!pct_synth_ell_chord(g, k, ct, st, t0, m_t, v0, m_v, &s0, &s1)) continue;
+        double len = (s1 - s0) * ds_du;
+        double parent = 0.0;
+        for (int j = 0; j < k; ++j)
+            if (pct_synth_ell_contains(g, j, k)) parent = g->ell_rsp[j];
+        wepl_true += (g->ell_rsp[k] - parent) * len;
+        if (k == 0) { hit = 1; u_exit_obj = s1; path_len = len; }
+    }
+    for (int k = 0; g->n_ell <= 0 && k < g->n_cyl; ++k) {
         /* cylinder centre in the gantry frame: u = c x + s y, t = c y - s x */
         double cu = ct * g->cyl_x[k] + st * g->cyl_y[k];
         double ctt = ct * g->cyl_y[k] - st * g->cyl_x[k];

@@ -1,0 +1,77 @@
+"""GPU: the batched form (pct_batch_*) reproduces each scan's own
+pct_drop_iteration bit for bit -- different grids, block sizes and history
+counts in one batch, the FP64 and the precision forms, graph and eager
+launches, recomputed and cached lists."""
+import numpy as np
+import pytest
+
+pytestmark = pytest.mark.gpu
+
+
+def _small_scans(n_scans=4, n_hist=24_000):
+    """the first scans of the batched configuration with few histories:
+    intersections -> cylinder hull -> endpoints -> compaction, x0 = hull"""
+    import pct_amd
+    from pct_amd import configs
+    out = []
+    for k, sc in enumerate(configs.batch_scans()[:n_scans]):
+        sc.synth.n_histories = n_hist + 7000 * k
+        ctx = pct_amd.Context(sc.params)
+        hist, missed = ctx.recon_volume_intersections(ctx.generate(sc.synth))
+        hist = ctx.compact(hist, (missed == 0).to(ctx.torch.uint8))
+        hull = ctx.hull_init()
+        hist = ctx.compact(hist, ctx.mlp_endpoints(hist, hull))
+        x0 = ctx.initial_iterate(ctx.torch.ones(ctx.V, dtype=ctx.torch.float32, device=ctx.device), hull)
+        out.append((ctx, hist, x0))
+    return out
+
+
+@pytest.fixture(scope="module")
+def scans():
+    return _small_scans()
+
+
+@pytest.mark.parametrize("mode_name", ["fp64", "precision", "fp64_eager", "fp64_cached"])
+def test_batch_equals_single_scans(scans, mode_name):
+    import pct_amd
+    mode = {"fp64": 0, "precision": pct_amd.PCT_PRECISION, "fp64_eager": pct_amd.PCT_DROP_NO_GRAPH,
+            "fp64_cached": pct_amd.PCT_DROP_CACHED}[mode_name]
+    sizes = [800, 1600, 3200, 6400][:len(scans)]
+    lams = [1e-2, 2e-2, 5e-3, 1e-2][:len(scans)]
+    ref = []
+    for (ctx, hist, x0), B, lam in zip(scans, sizes, lams):
+        x = x0.clone()
+        for _ in range(3):
+            ctx.drop_iteration(hist, x, B, lam, mode)
+        ref.append(x.cpu().numpy())
+    batch = pct_amd.Batch([c for c, _, _ in scans])
+    hists = [h for _, h, _ in scans]
+    xs = [x0.clone() for _, _, x0 in scans]
+    batch.prepare(hists, sizes, mode & 0xff)
+    for _ in range(3):
+        batch.iteration(hists, xs, sizes, lams, mode)
+    for k, (x, r) in enumerate(zip(xs, ref)):
+        got = x.cpu().numpy()
+        assert np.any(got != scans[k][2].cpu().numpy())
+        assert np.array_equal(got, r), f"scan {k}: {np.count_nonzero(got != r)} voxels differ"
+    batch.close()
+
+
+def test_batch_replays_and_changed_block_sizes(scans):
+    """graph replays with eager work between them, then new block sizes
+    (plans rebuilt for the changed scans only) -- still equal to single scans"""
+    import pct_amd
+    batch = pct_amd.Batch([c for c, _, _ in scans])
+    hists = [h for _, h, _ in scans]
+    for sizes in ([3200] * len(scans), [6400, 800, 3200, 1600][:len(scans)]):
+        xs = [x0.clone() for _, _, x0 in scans]
+        for _ in range(2):
+            batch.iteration(hists, xs, sizes, None, 0)
+            for x in xs:
+                x.mul_(1.0)
+        for (ctx, hist, x0), B, x in zip(scans, sizes, xs):
+            r = x0.clone()
+            for _ in range(2):
+                ctx.drop_iteration(hist, r, B, None, 0)
+            assert np.array_equal(x.cpu().numpy(), r.cpu().numpy())
+    batch.close()
