@@ -43,6 +43,73 @@ def _cpu_model():
     return "unknown"
 
 
+def batched_form(log):
+    """The batched configuration: 12 scans of uneven size (grids 64x64x8 to
+    256x256x48, 0.5-30 M histories, random ellipsoid phantoms), each
+    preprocessed on the GPU, block sizes from the project's tuner, then timed
+    DROP iterations of the whole batch (FP64 and precision forms) and, for
+    comparison, the same scans' iterations one after the other."""
+    import torch
+    import pct_amd
+    from pct_amd import configs, pipeline, tuner
+    t0 = time.time()
+    scans = configs.batch_scans()
+    ctxs, hists, x0s, info = [], [], [], []
+    for sc in scans:
+        ctx = pct_amd.Context(sc.params)
+        pre = pipeline.preprocess(ctx, ctx.generate(sc.synth))
+        ctxs.append(ctx)
+        hists.append(pre.hist)
+        x0s.append(pre.x0)
+        info.append({"name": sc.name, "grid": [sc.params.columns, sc.params.rows, sc.params.slices],
+                     "histories": int(sc.synth.n_histories), "reconstruction_histories": int(pre.hist.n),
+                     "ellipsoids": int(sc.synth.n_ell)})
+        del pre
+    setup_s = time.time() - t0
+    batch = pct_amd.Batch(ctxs)
+    best, table = tuner.tune(batch, hists, x0s, configs.BATCH_BLOCK_SIZES, mode=0, log=log)
+    B = best["block_sizes"]
+    n = sum(h.n for h in hists)
+    res = {"scans": info, "setup_s": round(setup_s, 2), "tuner": table, "block_sizes": B,
+           "reconstruction_histories": n}
+
+    def timed_batch(mode, steps=3):
+        batch.prepare(hists, B, mode & 0xff)
+        xs = [x.clone() for x in x0s]
+        batch.iteration(hists, xs, B, None, mode)
+        torch.cuda.synchronize()
+        e0 = torch.cuda.Event(enable_timing=True)
+        e1 = torch.cuda.Event(enable_timing=True)
+        e0.record(ctxs[0].stream)
+        for _ in range(steps):
+            batch.iteration(hists, xs, B, None, mode)
+        e1.record(ctxs[0].stream)
+        torch.cuda.synchronize()
+        return e0.elapsed_time(e1) / steps
+
+    for name, mode in (("fp64", 0), ("precision", pct_amd.PCT_PRECISION)):
+        ms = timed_batch(mode)
+        res[name] = {"ms_per_iteration": round(ms, 3), "histories_per_s": round(n / (ms / 1e3), 1)}
+        log(f"batched {name}: {ms:.2f} ms/iteration, {n / (ms / 1e3) / 1e6:.1f} M histories/s")
+    seq = 0.0
+    for ctx, h, x0, b in zip(ctxs, hists, x0s, B):
+        x = x0.clone()
+        ctx.drop_iteration(h, x, b)
+        torch.cuda.synchronize()
+        e0 = torch.cuda.Event(enable_timing=True)
+        e1 = torch.cuda.Event(enable_timing=True)
+        e0.record(ctx.stream)
+        ctx.drop_iteration(h, x, b)
+        e1.record(ctx.stream)
+        torch.cuda.synchronize()
+        seq += e0.elapsed_time(e1)
+    res["scans_one_after_the_other_fp64_ms"] = round(seq, 3)
+    batch.close()
+    for c in ctxs:
+        c.close()
+    return res
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -52,6 +119,7 @@ def main():
     ap.add_argument("--histories", type=int, default=None, help="override the scan size")
     ap.add_argument("--cpu-seconds", type=float, default=15.0, help="target CPU time of the oracle sample")
     ap.add_argument("--no-cpu-baseline", action="store_true")
+    ap.add_argument("--no-batched", action="store_true", help="skip the batched 12-scan form")
     ap.add_argument("--quick", action="store_true",
                     help="only the timed iterations (for counter runs, where every dispatch is slow)")
     ap.add_argument("--form", default="default", choices=["default", "cached", "sequential", "precision"],
@@ -274,6 +342,19 @@ def main():
                 "max_abs": float(np.max(np.abs(diff))), "voxels": int(touched.sum()),
                 "sample": f"one iteration over the first {m} histories, GPU FP32 image vs FP64 oracle"}
 
+    # ---- batched form (12 scans), after releasing this scan's plan ----
+    batched = None
+    post_cut = pre.counts["post_cut"]
+    if not args.no_batched:
+        ctx.close()
+        del hist, pre, x
+        torch.cuda.empty_cache()
+        try:
+            batched = batched_form(log)
+        except Exception as e:        # e.g. device memory: reported, the headline stands
+            batched = {"error": f"{type(e).__name__}: {e}"}
+            log(f"batched form failed: {e}")
+
     out = {
         "metric": "histories/s per DROP iteration (MLP+update), RSP RMSE vs FP64 oracle, 1 GPU",
         "value": round(value, 1),
@@ -290,7 +371,7 @@ def main():
         "config": {"workload": f"{args.config}: {p.columns}x{p.rows}x{p.slices} voxels, "
                                f"{scfg.n_histories} histories / 90 angles, DROP block {B}, lambda {p.lambda_:g}, "
                                f"exact FP64 MLP, FP32 image, x0 = FBP * MSC hull",
-                   "histories": scfg.n_histories, "post_cut_histories": pre.counts["post_cut"],
+                   "histories": scfg.n_histories, "post_cut_histories": post_cut,
                    "reconstruction_histories": n_r,
                    "mean_path_voxels": round(sum_n / n_r, 2), "sum_block_voxels": sum_vb,
                    "drop_blocks": (n_r + B - 1) // B},
@@ -299,6 +380,7 @@ def main():
                               f"{mlp_alone_ms:.2f} and {blk_alone_ms:.2f} ms",
                       "plan_setup_once": round(t_plan * 1e3, 1), "preprocess_once": round(t_pre * 1e3, 1)},
         "precision_form": precision,
+        "batched_form": batched,
         "other_forms": {"sequential_mlp_then_blocks_ms": round(ms_seq, 3),
                         "cached_lists_histories_per_s": round(n_r / (ms_cached / 1e3), 1),
                         "cached_lists_ms": round(ms_cached, 3),
