@@ -19,6 +19,8 @@ import os
 import sys
 import time
 
+import numpy as np
+
 ROOT = os.path.dirname(os.path.abspath(__file__))
 sys.path.insert(0, ROOT)
 
@@ -110,6 +112,159 @@ def batched_form(log):
     return res
 
 
+def _rel_errors(x, refs, x0_np):
+    inside = x0_np != 0
+    out = {}
+    for it, r in refs.items():
+        d = x[it].astype(np.float64) - r.astype(np.float64)
+        rm = float(np.sqrt(np.mean(d[inside] ** 2)))
+        out[f"iteration_{it}"] = {"rmse": rm, "rmse_rel": rm / float(np.mean(np.abs(r[inside]))),
+                                  "max_abs": float(np.max(np.abs(d)))}
+    return out
+
+
+def learned_form(ctx, hist, x0, ref, B, p, ms_fp64, args, timed, log):
+    """The learned-path form on this scan: training set (10 M exact-MLP paths
+    of 8 random-ellipsoid phantom scans of other grids), training with the
+    library's kernels, calibration check on 1 in 64 histories (cells whose
+    99th-percentile path deviation exceeds 3 % of a voxel take the exact
+    walk), the agreement with the exact walk over every history, timed DROP
+    iterations (surrogate MLP recomputed every iteration + FP32/Kahan update)
+    and the RSP error against the exact FP64 form after iterations 1, 6, 12."""
+    import torch
+    import pct_amd
+    from pct_amd import configs, learned
+    res = {}
+    t0 = time.time()
+    scans = learned.training_scans(configs.LEARNED_TRAIN_SCANS, configs.LEARNED_TRAIN_HISTORIES_PER_SCAN)
+    F, Y = learned.training_set(configs.LEARNED_TRAIN_PATHS, scans)
+    torch.cuda.synchronize()
+    t_data = time.time() - t0
+    t0 = time.time()
+    e0, e1 = torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)
+    e0.record(ctx.stream)
+    net = learned.train_network(ctx, F, Y, epochs=20, batch=8192, lr=3e-3, seed=3)
+    e1.record(ctx.stream)
+    torch.cuda.synchronize()
+    t_train = time.time() - t0
+    steps = 20 * (F.shape[0] // 8192)
+    # MACs of one sample through the padded network (16x64 + 64x64 + 64x16),
+    # forward + the two backward products per layer
+    flops = 3 * 2 * (16 * 64 + 64 * 64 + 64 * 16) * 8192.0 * steps
+    train_ms = e0.elapsed_time(e1)
+    res["training"] = {"paths": int(F.shape[0]), "scans": [sc.name for sc in scans], "epochs": 20, "batch": 8192,
+                       "optimizer_steps": steps, "data_s": round(t_data, 2), "train_s": round(t_train, 2),
+                       "loss_first_epoch": net.epoch_loss[0], "loss_last_epoch": net.epoch_loss[-1],
+                       "mfma": {"achieved_tflops": round(flops / (train_ms / 1e3) / 1e12, 3), "peak_tflops": 2500.0,
+                                "frac": round(flops / (train_ms / 1e3) / 1e12 / 2500.0, 6),
+                                "note": "two launches per optimizer step of 8192 samples (128 workgroups): "
+                                        "latency-bound, not MFMA-bound"}}
+    del F, Y
+    log(f"learned: training set {t_data:.1f}s, training {t_train:.1f}s, loss {net.epoch_loss[0]:.3e} -> "
+        f"{net.epoch_loss[-1]:.3e}")
+    res.update(evaluate_learned(ctx, hist, x0, ref, B, p, net, timed, args.steps, log))
+    res["speedup_vs_fp64_form"] = round(ms_fp64 / res["ms_per_iteration"], 3)
+    return res, net
+
+
+def evaluate_learned(ctx, hist, x0, ref, B, p, net, timed, steps, log):
+    import torch
+    import pct_amd
+    from pct_amd import learned
+    res = {}
+    net.install(ctx)
+    vw = 2 * p.recon_cyl_radius / p.columns
+    tol = 0.03 * vw
+    cal = learned.calibrate(ctx, hist, stride=64, tol=tol)
+    full = learned.calibrate(ctx, hist, stride=1)
+    res["calibration"] = {"tol_cm": tol, "samples": cal["samples"], "cells_fallback": cal["cells_fallback"],
+                          "fallback_rate": round(cal["fallback_rate"], 5)}
+    res["vs_exact_walk_all_histories"] = {
+        "histories": full["samples"], "in_trained_range": full["in_range"],
+        "surrogate_histories": full["surrogate"], "fallback_rate": round(full["fallback_rate"], 5),
+        "list_mismatch_of_surrogate_histories": round(full["surrogate_mismatch"] / max(full["surrogate"], 1), 4),
+        "path_deviation_um": {"median": round(full["dev_median_cm"] * 1e4, 3),
+                              "p99": round(full["dev_p99_cm"] * 1e4, 3), "max": round(full["dev_max_cm"] * 1e4, 3)}}
+    log(f"learned: {res['vs_exact_walk_all_histories']}")
+    t0 = time.time()
+    ctx.drop_prepare(hist, B, pct_amd.PCT_MLP_SURROGATE)
+    ctx.sync()
+    res["plan_setup_once_ms"] = round((time.time() - t0) * 1e3, 1)
+    xs = x0.clone()
+    ctx.drop_iteration(hist, xs, B, p.lambda_, pct_amd.PCT_LEARNED)
+    learned.fallback_stats(ctx)
+    ms, ph = timed(pct_amd.PCT_LEARNED, steps, xs)
+    ms /= steps
+    fb, tot = learned.fallback_stats(ctx)
+    _, mlp_ms, blk_ms = ph.mean(axis=0)
+    pct_amd.set_tuning(ctx, overlap=2)
+    _, ph_seq = timed(pct_amd.PCT_LEARNED, 1, xs)
+    pct_amd.set_tuning(ctx, overlap=0)
+    n = hist.n
+    res.update({"histories_per_s": round(n / (ms / 1e3), 1), "ms_per_iteration": round(ms, 3),
+                "phases_ms": {"mlp_span": round(float(mlp_ms), 3), "block_loop_span": round(float(blk_ms), 3),
+                              "surrogate_mlp_alone": round(float(ph_seq[0][0]), 3)},
+                "fallback_rate_timed_iterations": round(fb / max(tot, 1), 5)})
+    if ref:
+        xl = x0.clone()
+        snaps = {}
+        for it in range(1, max(ref) + 1):
+            ctx.drop_iteration(hist, xl, B, p.lambda_, pct_amd.PCT_LEARNED)
+            if it in ref:
+                snaps[it] = xl.cpu().numpy()
+        res["error_vs_fp64"] = _rel_errors(snaps, ref, x0.cpu().numpy())
+        res["error_region"] = "voxels of the hull (x0 != 0); reference image: exact FP64 form"
+    log(f"learned form: {ms:.2f} ms/iteration, fallback {fb / max(tot, 1):.4f}, errors {res.get('error_vs_fp64')}")
+    return res
+
+
+def learned_large(net, steps, log):
+    """The learned-path form on the 256x256x48 / 50 M-history scan (random
+    ellipsoid phantom): the exact FP64 form gives the reference images after
+    iterations 1, 6 and 12, then the same network as on the 200x200x32 scan."""
+    import torch
+    import pct_amd
+    from pct_amd import configs, pipeline
+    sc = configs.learned_large()
+    ctx = pct_amd.Context(sc.params)
+    t0 = time.time()
+    pre = pipeline.preprocess(ctx, ctx.generate(sc.synth))
+    hist = pre.hist
+    B = sc.params.drop_block_size
+    ctx.drop_prepare(hist, B)
+    ctx.sync()
+    setup_s = time.time() - t0
+    pct_amd.set_profiling(ctx, True)
+
+    def timed(mode, k, img):
+        phases = []
+        torch.cuda.synchronize()
+        ev0, ev1 = torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)
+        ev0.record(ctx.stream)
+        for _ in range(k):
+            ctx.drop_iteration(hist, img, B, sc.params.lambda_, mode)
+            phases.append(pct_amd.last_timings(ctx))
+        ev1.record(ctx.stream)
+        torch.cuda.synchronize()
+        return ev0.elapsed_time(ev1), np.array(phases)
+
+    ref = {}
+    xe = pre.x0.clone()
+    t_ref = 0.0
+    for it in range(1, 13):
+        ms, _ = timed(0, 1, xe)
+        t_ref += ms
+        if it in (1, 6, 12):
+            ref[it] = xe.cpu().numpy()
+    res = {"scan": sc.name, "histories": int(sc.synth.n_histories), "reconstruction_histories": int(hist.n),
+           "setup_s": round(setup_s, 1), "fp64_form_ms_per_iteration": round(t_ref / 12, 3),
+           "fp64_form_histories_per_s": round(hist.n / (t_ref / 12 / 1e3), 1)}
+    res.update(evaluate_learned(ctx, hist, pre.x0, ref, B, sc.params, net, timed, steps, log))
+    res["speedup_vs_fp64_form"] = round(res["fp64_form_ms_per_iteration"] / res["ms_per_iteration"], 3)
+    ctx.close()
+    return res
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -120,6 +275,8 @@ def main():
     ap.add_argument("--cpu-seconds", type=float, default=15.0, help="target CPU time of the oracle sample")
     ap.add_argument("--no-cpu-baseline", action="store_true")
     ap.add_argument("--no-batched", action="store_true", help="skip the batched 12-scan form")
+    ap.add_argument("--no-learned", action="store_true", help="skip the learned-path form")
+    ap.add_argument("--no-large", action="store_true", help="skip the 256x256x48 / 50 M learned-form case")
     ap.add_argument("--quick", action="store_true",
                     help="only the timed iterations (for counter runs, where every dispatch is slow)")
     ap.add_argument("--form", default="default", choices=["default", "cached", "sequential", "precision"],
@@ -134,7 +291,6 @@ def main():
                                    "not built", "n_gpus": args.gpus}))
         sys.exit(2)
 
-    import numpy as np
     import torch
 
     import pct_amd
@@ -278,6 +434,15 @@ def main():
                  "error_vs_fp64": prec_err,
                  "error_region": "voxels of the hull (x0 != 0); reference image: exact FP64 form"}
 
+    # ---- learned-path form: network trained here on 10 M exact-MLP paths ----
+    learned_res, net = None, None
+    if not args.no_learned:
+        try:
+            learned_res, net = learned_form(ctx, hist, pre.x0, ref, B, p, ms_per_step, args, timed, log)
+        except Exception as e:
+            learned_res = {"error": f"{type(e).__name__}: {e}"}
+            log(f"learned form failed: {e}")
+
     # algorithmic work per iteration: 48 B per history record, 4 B gather + 4 B
     # scatter per path entry, 24 B per voxel touched by a block
     bytes_update = 48.0 * n_r + 8.0 * sum_n + 24.0 * sum_vb
@@ -355,6 +520,15 @@ def main():
             batched = {"error": f"{type(e).__name__}: {e}"}
             log(f"batched form failed: {e}")
 
+    if net is not None and not args.no_large and isinstance(learned_res, dict):
+        ctx.close()                     # the 200x200x32 plan is released first
+        torch.cuda.empty_cache()
+        try:
+            learned_res["large"] = learned_large(net, args.steps, log)
+        except Exception as e:
+            learned_res["large"] = {"error": f"{type(e).__name__}: {e}"}
+            log(f"learned large case failed: {e}")
+
     out = {
         "metric": "histories/s per DROP iteration (MLP+update), RSP RMSE vs FP64 oracle, 1 GPU",
         "value": round(value, 1),
@@ -381,6 +555,7 @@ def main():
                       "plan_setup_once": round(t_plan * 1e3, 1), "preprocess_once": round(t_pre * 1e3, 1)},
         "precision_form": precision,
         "batched_form": batched,
+        "learned_form": learned_res,
         "other_forms": {"sequential_mlp_then_blocks_ms": round(ms_seq, 3),
                         "cached_lists_histories_per_s": round(n_r / (ms_cached / 1e3), 1),
                         "cached_lists_ms": round(ms_cached, 3),

@@ -58,6 +58,10 @@ def lib():
             getattr(L, name).argtypes = [P(abi.PctParams)] + [vp] * n
         L.oracle_math.argtypes = [ctypes.c_int, vp, vp, vp, ctypes.c_int64]
         L.oracle_math.restype = None
+        L.oracle_mlp_points.argtypes = [P(abi.PctParams), P(abi.PctHistSoa), ctypes.c_int64, ctypes.c_int, vp, vp, vp]
+        L.oracle_net_features.argtypes = [P(abi.PctParams), P(abi.PctHistSoa), ctypes.c_int64, vp, vp, vp, vp]
+        L.oracle_surrogate_path.argtypes = [P(abi.PctParams), P(abi.PctHistSoa), ctypes.c_int64, vp, vp, vp, vp,
+                                            ctypes.c_int, P(ctypes.c_int)]
         _L = L
     return _L
 
@@ -324,3 +328,44 @@ def drop(p, h: HostHistories, x0: np.ndarray, iterations=1, block_size=None, lam
     if rc != 0:
         raise OverflowError("MLP path overflow in the oracle DROP")
     return x
+
+
+# ---------------------------------------------------------------------------
+# learned-path form (oracle_net.cpp; the network itself is oracle/net.py)
+# ---------------------------------------------------------------------------
+def mlp_points(p, h: HostHistories, i: int, u1):
+    """exact FP64 MLP (t, v) of history i at the depths u1"""
+    u1 = np.ascontiguousarray(u1, np.float64)
+    t = np.zeros(u1.size)
+    v = np.zeros(u1.size)
+    hs = h.soa()
+    lib().oracle_mlp_points(ctypes.byref(p), ctypes.byref(hs), i, u1.size, _p(u1), _p(t), _p(v))
+    return t, v
+
+
+def net_features(p, h: HostHistories, targets=True):
+    """(features[n][12], targets[n][8] or None, cells[n], u_2[n])"""
+    f = np.zeros((h.n, 12), np.float32)
+    t = np.zeros((h.n, 8), np.float32) if targets else None
+    cells = np.zeros(h.n, np.int32)
+    u2 = np.zeros(h.n)
+    hs = h.soa()
+    lib().oracle_net_features(ctypes.byref(p), ctypes.byref(hs), h.n, _p(f), _p(t), _p(cells), _p(u2))
+    return f, t, cells, u2
+
+
+def surrogate_path(p, h: HostHistories, i: int, out8, cap=4096):
+    """FP32 surrogate walk of history i for network outputs out8: (path,
+    per-step t, per-step v)"""
+    out8 = np.ascontiguousarray(out8, np.float32)
+    buf = np.zeros(p.max_intersections + 1, np.uint32)
+    tr = np.zeros(cap, np.float32)
+    vr = np.zeros(cap, np.float32)
+    nst = ctypes.c_int(0)
+    hs = h.soa()
+    n = lib().oracle_surrogate_path(ctypes.byref(p), ctypes.byref(hs), i, _p(out8), _p(buf), _p(tr), _p(vr), cap,
+                                    ctypes.byref(nst))
+    if n < 0:
+        raise OverflowError("surrogate path longer than max_intersections")
+    k = min(nst.value, cap)
+    return buf[:n].copy(), tr[:k].copy(), vr[:k].copy()

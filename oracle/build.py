@@ -11,7 +11,7 @@ import sys
 
 HERE = os.path.dirname(os.path.abspath(__file__))
 OUT = os.path.join(HERE, "liboracle.so")
-SOURCES = ["oracle_hotpath.cpp", "oracle_math.cpp", "oracle_bins.cpp", "oracle_hull_fbp.cpp"]
+SOURCES = ["oracle_hotpath.cpp", "oracle_math.cpp", "oracle_bins.cpp", "oracle_hull_fbp.cpp", "oracle_net.cpp"]
 FLAGS = ["-O2", "-std=c++17", "-fPIC", "-shared", "-ffp-contract=off", "-fno-fast-math", "-fopenmp"]
 
 

@@ -80,8 +80,24 @@ class PctSynthSoa(ctypes.Structure):
     _fields_ = [("gantry_angle", c_vp)] + [(n, c_vp) for n in RAW_FLOAT_FIELDS]
 
 
-# name -> (restype, argtypes) of every exported entry point of include/pct.h and
-# include/pct_synth_api.h
+class PctNetTrainCfg(ctypes.Structure):
+    _fields_ = [("epochs", c_i32), ("batch", c_i32), ("lr", c_f32), ("lr_min", c_f32), ("beta1", c_f32),
+                ("beta2", c_f32), ("eps", c_f32), ("seed", c_u64)]
+
+
+NET_CELLS = 64
+
+
+class PctNetCalib(ctypes.Structure):
+    _fields_ = [("samples", c_i64), ("list_mismatch", c_i64), ("in_range", c_i64), ("surrogate", c_i64),
+                ("surrogate_mismatch", c_i64), ("dev_median", c_f64),
+                ("dev_p99", c_f64), ("dev_max", c_f64), ("cells_fallback", c_i32),
+                ("cell_samples", c_i32 * NET_CELLS), ("cell_p99", c_f32 * NET_CELLS),
+                ("cell_fallback", ctypes.c_uint8 * NET_CELLS)]
+
+
+# name -> (restype, argtypes) of every exported entry point of include/pct.h,
+# include/pct_net.h and include/pct_synth_api.h
 P = ctypes.POINTER
 SIGNATURES = {
     "pct_create": (c_vp, [P(PctParams), c_vp]),
@@ -120,12 +136,22 @@ SIGNATURES = {
     "pct_batch_prepare": (ctypes.c_int, [c_vp, P(PctHistSoa), P(ctypes.c_int), ctypes.c_int]),
     "pct_batch_iteration": (ctypes.c_int, [c_vp, P(PctHistSoa), P(c_vp), P(ctypes.c_int), P(c_f32), ctypes.c_int]),
     "pct_selftest_math": (ctypes.c_int, [c_vp, ctypes.c_int, c_vp, c_vp, c_vp, c_i64]),
+    "pct_net_features": (ctypes.c_int, [c_vp, P(PctHistSoa), c_i64, c_i64, c_vp, c_vp]),
+    "pct_net_gradient": (ctypes.c_int, [c_vp, c_vp, c_vp, c_vp, c_i64, c_vp, P(c_f64)]),
+    "pct_net_train": (ctypes.c_int, [c_vp, c_vp, c_vp, c_vp, c_i64, P(PctNetTrainCfg), c_vp]),
+    "pct_net_set": (ctypes.c_int, [c_vp, c_vp, c_vp, c_vp]),
+    "pct_net_set_hermite": (ctypes.c_int, [c_vp, c_vp, c_vp]),
+    "pct_net_forward": (ctypes.c_int, [c_vp, c_vp, c_i64, c_vp]),
+    "pct_net_calibrate": (ctypes.c_int, [c_vp, P(PctHistSoa), c_i64, c_i64, c_i64, c_f32, c_i32, P(PctNetCalib)]),
+    "pct_net_fallback_stats": (ctypes.c_int, [c_vp, P(c_i64), P(c_i64)]),
     "pct_synth_generate": (ctypes.c_int, [c_vp, P(PctSynthCfg), c_i64, c_i64, P(PctSynthSoa)]),
     "pct_synth_generate_host": (ctypes.c_int, [P(PctSynthCfg), c_i64, c_i64, P(PctSynthSoa)]),
 }
 
 PCT_MLP_FP64 = 0
 PCT_MLP_TABLE = 1
+PCT_MLP_SURROGATE = 2
+NET_IN, NET_OUT, NET_PARAMS = 12, 8, 5512
 PCT_DROP_CACHED = 0x100
 PCT_DROP_ATOMIC = 0x200
 PCT_DROP_NO_GRAPH = 0x400
@@ -137,6 +163,8 @@ PCT_TUNE_CHUNK_MAX = 3
 PCT_TUNE_OVERLAP = 4
 # the precision form: tabulated MLP with FP32 stepping + FP32/Kahan update
 PCT_PRECISION = PCT_MLP_TABLE | PCT_DROP_F32
+# the learned-path form: surrogate MLP (exact fallback) + FP32/Kahan update
+PCT_LEARNED = PCT_MLP_SURROGATE | PCT_DROP_F32
 
 _lib = None
 

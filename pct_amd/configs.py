@@ -193,3 +193,47 @@ def batch_scans(seed: int = SEED_BASE + 4, scale: float = 1.0):
                 getattr(c, "ell_" + f)[j] = e[f]
         out.append(BatchScan(f"scan{k:02d}_{cxy}x{cxy}x{s}_{n}", p, c))
     return out
+
+
+# ---------------------------------------------------------------------------
+# Learned-path form: the large evaluation scan
+# ---------------------------------------------------------------------------
+LEARNED_TRAIN_PATHS = 10_000_000
+LEARNED_TRAIN_SCANS = 8
+LEARNED_TRAIN_HISTORIES_PER_SCAN = 1_600_000
+
+
+def learned_large(n_histories: int = 50_000_000, seed: int = SEED_BASE + 6):
+    """256x256x48 voxels (0.78 x 0.78 x 2.5 mm), n_histories over 90 angles, a
+    random ellipsoid phantom (RSP in U[0.2, 1.8]) of its own seed,
+    max_intersections 1024: the second evaluation case of the learned-path form"""
+    import numpy as np
+    rng = np.random.default_rng(seed)
+    p = _lib.default_params()
+    p.columns = p.rows = 256
+    p.slices = 48
+    p.slice_thickness = 0.25
+    p.recon_cyl_height = 48 * 0.25
+    p.lambda_ = 1e-4
+    p.drop_block_size = 3200
+    p.iterations = 12
+    p.max_intersections = 1024
+    c = _lib.PctSynthCfg()
+    c.seed = seed
+    c.n_histories = n_histories
+    c.n_angles = 90
+    c.angle_step_deg = 4.0
+    c.u_in_1, c.u_in_2, c.u_out_1, c.u_out_2 = -25.0, -15.0, 15.0, 25.0
+    c.t_half = 11.0
+    c.v_half = p.recon_cyl_height / 2 - 0.5
+    c.source_radius = 260.7
+    c.beta_cp = BETA_CP_200MEV
+    c.x0 = 36.08
+    c.straggle = 0.01
+    c.n_cyl = 0
+    ells = _random_ellipsoids(rng, p.recon_cyl_height)
+    c.n_ell = len(ells)
+    for j, e in enumerate(ells):
+        for f in ("x", "y", "z", "a", "b", "c", "phi", "rsp"):
+            getattr(c, "ell_" + f)[j] = e[f]
+    return BatchScan(f"large_256x256x48_{n_histories}", p, c)

@@ -143,7 +143,8 @@ static int check_batch_args(pct_batch* b, const pct_hist_soa* hists, const int* 
     pct_ctx* c0 = b->scans[0];
     if (!hists || !block_sizes) return pct_fail(c0, PCT_E_INVALID, "%s: null argument", who);
     const int mlp = mode & PCT_MLP_MODE_MASK;
-    if (mlp != PCT_MLP_FP64 && mlp != PCT_MLP_TABLE) return pct_fail(c0, PCT_E_INVALID, "%s: unknown MLP mode %d", who, mlp);
+    for (pct_ctx* c : b->scans)
+        if (!pct_mlp_mode_ok(c, mlp)) return pct_fail(c0, PCT_E_INVALID, "%s: MLP mode %d not available", who, mlp);
     if (mode & PCT_DROP_ATOMIC) return pct_fail(c0, PCT_E_INVALID, "%s: no atomic form in a batch", who);
     for (size_t s = 0; s < b->scans.size(); ++s) {
         const pct_hist_soa& h = hists[s];

@@ -155,6 +155,7 @@ extern "C" void pct_destroy(pct_ctx* ctx) {
     if (ctx->d_mlp_tab) (void)hipFree(ctx->d_mlp_tab);
     if (ctx->d_scat_tab) (void)hipFree(ctx->d_scat_tab);
     if (ctx->d_stats) (void)hipFree(ctx->d_stats);
+    (void)pct_net_release(ctx);
     pct_plan_destroy(ctx->plan);
     pct_bin_order_destroy(ctx->bin_order);
     for (hipEvent_t& e : ctx->ev)

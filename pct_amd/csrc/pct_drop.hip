@@ -196,8 +196,8 @@ extern "C" int pct_drop_block_update(pct_ctx* ctx, const pct_hist_soa* hist, int
                                      const float* x, float* x_update, uint32_t* counts, float lambda, int mode) {
     int rc = check_drop_hist(ctx, hist, "pct_drop_block_update");
     if (rc) return rc;
-    if (mode != PCT_MLP_FP64 && mode != PCT_MLP_TABLE)
-        return pct_fail(ctx, PCT_E_INVALID, "pct_drop_block_update: unknown mode %d", mode);
+    if (!pct_mlp_mode_ok(ctx, mode))
+        return pct_fail(ctx, PCT_E_INVALID, "pct_drop_block_update: mode %d not available", mode);
     if (!x || !x_update || !counts) return pct_fail(ctx, PCT_E_INVALID, "pct_drop_block_update: null image array");
     if (start < 0 || count < 0 || start + count > hist->n)
         return pct_fail(ctx, PCT_E_INVALID, "pct_drop_block_update: bad range");

@@ -10,6 +10,7 @@
 #include <string>
 #include <vector>
 #include "../../include/pct.h"
+#include "../../include/pct_net.h"
 #include "pct_math.h"
 
 /* Derived constants of the reference's macros (data.h:264-348), computed once
@@ -39,6 +40,7 @@ enum { MLP_TAB_U1 = 0, MLP_TAB_S1I0, MLP_TAB_S1I1, MLP_TAB_S1I3, MLP_TAB_C1, MLP
 
 struct pct_plan;
 struct pct_bin_order;
+struct pct_net_frag;
 
 struct pct_buffer {
     void* ptr = nullptr;
@@ -65,8 +67,20 @@ struct pct_ctx {
     int scat_q = 0;
     /* resident workgroups per CU of the MLP paths kernel per walk, and the
      * CU count (0 = not queried) */
-    int mlp_occ[2] = {0, 0};
+    int mlp_occ[3] = {0, 0, 0};
     int cus = 0;
+    /* learned-path form (pct_net.hip): installed network (inference operands),
+     * per-input trained range lo[12], hi[12], calibration cell flags (1 = exact
+     * walk), fallback/history counters, and a float copy of the u_1 column of
+     * the MLP table. net_version counts installs (0 = none) so that plans built
+     * with another network are rebuilt. */
+    pct_net_frag* d_net = nullptr;
+    float* d_net_range = nullptr;
+    uint8_t* d_net_cells = nullptr;
+    unsigned long long* d_net_counts = nullptr;
+    float* d_u1f = nullptr;
+    int net_version = 0;
+    pct_buffer net_partial, net_state, net_loss, net_calib;
     /* execution knobs (pct_set_tuning; 0 = default) */
     int tune_mlp_waves = 0, tune_chunk_first = 0, tune_chunk_max = 0, tune_overlap = 0;
     /* workspaces, grown on demand */
@@ -106,8 +120,17 @@ int pct_take_status(pct_ctx* ctx, int* flags);
 #define PCT_STATUS_TABLE      4   /* chord longer than the per-depth table     */
 #define PCT_STATUS_PLAN       8   /* recomputed lists differ from the DROP plan */
 
+/* MLP modes accepted by the path and DROP entry points (the surrogate only
+ * once a network is installed) */
+bool pct_mlp_mode_ok(const pct_ctx* ctx, int mlp);
+int pct_net_release(pct_ctx* ctx);
+
 /* exclusive prefix sum of n int32 values into n+1 int64 offsets (device) */
 int pct_exclusive_scan_i32(pct_ctx* ctx, const int32_t* in, int64_t n, int64_t* out, int64_t* total_host);
+
+/* workgroups of a grid-stride launch over a count that may exceed what one
+ * dispatch can hold (2^32 - 1 work-items) */
+#define PCT_GRID_CAP 65536u
 
 static inline unsigned pct_grid(int64_t n, int threads) {
     int64_t b = (n + threads - 1) / threads;

@@ -21,7 +21,10 @@
  * and the voxel index) are computed per step.
  */
 #include <algorithm>
+#include <vector>
 #include "pct_internal.h"
+#include "pct_net_basis.h"
+#include "pct_net_dev.h"
 
 namespace {
 
@@ -230,16 +233,11 @@ __device__ __forceinline__ void mlp_setup(const pct_geo& g, const pct_hist_soa& 
     m.S = steps_below(u1tab, K, u_2 - g.mlp_u_step);
 }
 
-/* one MLP step (.cu:5376-5489) at table index k; returns the voxel index or -1
- * when it falls outside the grid */
-__device__ __forceinline__ int64_t mlp_step(const pct_geo& g, const mlp_hist& m, const double* __restrict__ tab, int K, int k) {
-    double u_1 = tab[MLP_TAB_U1 * K + k];
-    double S1I0 = tab[MLP_TAB_S1I0 * K + k];
-    double S1I1 = tab[MLP_TAB_S1I1 * K + k];
-    double S1I3 = tab[MLP_TAB_S1I3 * K + k];
-    double cst1 = tab[MLP_TAB_C1 * K + k];
-    double cst2 = tab[MLP_TAB_C2 * K + k];
-    double cst3 = tab[MLP_TAB_C3 * K + k];
+/* the u_2-dependent part of one MLP step (.cu:5402-5472): (t_1, v_1) from the
+ * u_1-only terms S1I* (Sigma_1^-1) and cst* (the common sigma_2 polynomials) */
+__device__ __forceinline__ void mlp_tv_terms(const pct_geo& g, const mlp_hist& m, double u_1, double S1I0, double S1I1,
+                                             double S1I3, double cst1, double cst2, double cst3, double& t_out,
+                                             double& v_out) {
     double S1I2 = S1I1;
     double d = m.u_2 - u_1;                       /* R_1[1] = R_1T[2] */
     double l2 = g.e_0 * (1 + 0.038 * pct_log((m.u_2 - u_1) / g.x_0));
@@ -279,6 +277,39 @@ __device__ __forceinline__ int64_t mlp_step(const pct_geo& g, const mlp_hist& m,
     double r0 = S1I0 * b1 + S1I1 * b2 + 1.0 * b3 + 0.0 * b4;
     double r1 = S1I2 * b1 + S1I3 * b2 + d * b3 + 1.0 * b4;
     double v_1 = I0 * r0 + I1 * r1;
+    t_out = t_1;
+    v_out = v_1;
+}
+
+/* (t_1, v_1) of the MLP at table index k */
+__device__ __forceinline__ void mlp_tv(const pct_geo& g, const mlp_hist& m, const double* __restrict__ tab, int K, int k,
+                                       double& u_1, double& t_1, double& v_1) {
+    u_1 = tab[MLP_TAB_U1 * K + k];
+    mlp_tv_terms(g, m, u_1, tab[MLP_TAB_S1I0 * K + k], tab[MLP_TAB_S1I1 * K + k], tab[MLP_TAB_S1I3 * K + k],
+                 tab[MLP_TAB_C1 * K + k], tab[MLP_TAB_C2 * K + k], tab[MLP_TAB_C3 * K + k], t_1, v_1);
+}
+
+/* (t_1, v_1) of the MLP at an arbitrary depth u_1: the u_1-only terms
+ * evaluated directly with the expressions of the per-depth table */
+__device__ __forceinline__ void mlp_point_at(const pct_geo& g, const mlp_hist& m, double u_1, double& t_1, double& v_1) {
+    const double u_0 = 0.0;
+    double l1 = g.e_0 * (1 + 0.038 * pct_log((u_1 - u_0) / g.x_0));
+    double c1 = l1 * l1 / g.x_0;
+    double sigma_t1 = c1 * (u_1 * u_1 * u_1 * (g.a0_3 + u_1 * (g.a1_12 + u_1 * (g.a2_30 + u_1 * (g.a3_60 + u_1 * (g.a4_105 + u_1 * g.a5_168))))));
+    double sigma_t1_theta1 = c1 * (u_1 * u_1 * (g.a0_2 + u_1 * (g.a1_6 + u_1 * (g.a2_12 + u_1 * (g.a3_20 + u_1 * (g.a4_30 + u_1 * g.a5_42))))));
+    double sigma_theta1 = c1 * (u_1 * (g.A0 + u_1 * (g.a1_2 + u_1 * (g.a2_3 + u_1 * (g.a3_4 + u_1 * (g.a4_5 + u_1 * g.a5_6))))));
+    double det1 = sigma_t1 * sigma_theta1 - sigma_t1_theta1 * sigma_t1_theta1;
+    double cst1 = u_1 * (g.A0 + u_1 * (g.a1_2 + u_1 * (g.a2_3 + u_1 * (g.a3_4 + u_1 * (g.a4_5 + u_1 * g.a5_6)))));
+    double cst2 = u_1 * u_1 * (g.a0_2 + u_1 * (g.a1_3 + u_1 * (g.a2_4 + u_1 * (g.a3_5 + u_1 * (g.a4_6 + u_1 * g.a5_7)))));
+    double cst3 = u_1 * u_1 * u_1 * (g.a0_3 + u_1 * (g.a1_4 + u_1 * (g.a2_5 + u_1 * (g.a3_6 + u_1 * (g.a4_7 + u_1 * g.a5_8)))));
+    mlp_tv_terms(g, m, u_1, sigma_theta1 / det1, -sigma_t1_theta1 / det1, sigma_t1 / det1, cst1, cst2, cst3, t_1, v_1);
+}
+
+/* one MLP step (.cu:5376-5489) at table index k; returns the voxel index or -1
+ * when it falls outside the grid */
+__device__ __forceinline__ int64_t mlp_step(const pct_geo& g, const mlp_hist& m, const double* __restrict__ tab, int K, int k) {
+    double u_1, t_1, v_1;
+    mlp_tv(g, m, tab, K, k, u_1, t_1, v_1);
     double x_1 = (m.ca * (m.u_in + u_1)) - (m.sa * t_1);
     double y_1 = (m.sa * (m.u_in + u_1)) + (m.ca * t_1);
     int vx = calc_voxel(g.x_zero, x_1, g.voxel_width);
@@ -307,6 +338,44 @@ __device__ __forceinline__ int64_t bcast_l(int64_t v, int lane) {
     uint32_t lo = (uint32_t)__builtin_amdgcn_readlane((int)(uint32_t)v, lane);
     uint32_t hi = (uint32_t)__builtin_amdgcn_readlane((int)(uint32_t)((uint64_t)v >> 32), lane);
     return (int64_t)(((uint64_t)hi << 32) | lo);
+}
+
+/* The voxel list of one history from its S steps, one wavefront: 64 lanes
+ * evaluate 64 consecutive steps (step(k) returns the voxel or -1 outside the
+ * grid), a step is appended when it differs from the previous valid one (the
+ * dedup of .cu:5478-5485: a ballot and a prefix popcount across the wave), the
+ * list is written from voxels[hoff] on (the first voxel `carry` included)
+ * while it fits hcap, and its position-aware hash is accumulated. Returns the
+ * list length n (uniform) and this lane's part of the hash. */
+template <class StepF>
+__device__ __forceinline__ void walk_list(StepF step, int S, uint32_t carry, int64_t hoff, int64_t hcap,
+                                          uint32_t* __restrict__ voxels, int lane, uint64_t lt_mask, int64_t& n_out,
+                                          uint32_t& hsh_out) {
+    if (lane == 0 && hcap > 0) voxels[hoff] = carry;
+    int64_t n = 1;
+    uint32_t hsh = lane == 0 ? pct_list_mix(carry, 0) : 0u;
+    for (int c = 0; c < S; c += 64) {
+        int k = c + lane;
+        int64_t vox = -1;
+        if (k < S) vox = step(k);
+        bool valid = vox >= 0;
+        uint64_t vmask = __ballot(valid);
+        uint64_t below = vmask & lt_mask;
+        int src = below ? 63 - __clzll(below) : 0;
+        uint32_t pv = (uint32_t)__shfl((int)(uint32_t)vox, src, 64);
+        uint32_t prev = below ? pv : carry;
+        bool is_new = valid && (uint32_t)vox != prev;
+        uint64_t nmask = __ballot(is_new);
+        if (is_new) {
+            int64_t pos = n + __popcll(nmask & lt_mask);
+            if (pos < hcap) voxels[hoff + pos] = (uint32_t)vox;
+            hsh += pct_list_mix((uint32_t)vox, (uint32_t)pos);
+        }
+        n += __popcll(nmask);
+        if (vmask) carry = (uint32_t)__shfl((int)(uint32_t)vox, 63 - __clzll(vmask), 64);
+    }
+    n_out = n;
+    hsh_out = hsh;
 }
 
 /* capacity pass: steps[k] = S + 1 (upper bound of the path length) */
@@ -508,29 +577,9 @@ k_mlp_paths(pct_geo g, pct_hist_soa h, int64_t start, int64_t count, W walk,
         uint32_t carry = (uint32_t)bcast_i((int)first, j);
         int64_t hoff = bcast_l(off, j);
         int64_t hcap = bcast_l(cap, j);
-        if (lane == 0 && hcap > 0) voxels[hoff] = carry;
-        int64_t n = 1;
-        uint32_t hsh = lane == 0 ? pct_list_mix(carry, 0) : 0u;
-        for (int c = 0; c < S; c += 64) {
-            int k = c + lane;
-            int64_t vox = -1;
-            if (k < S) vox = walk.step(g, b, k);
-            bool valid = vox >= 0;
-            uint64_t vmask = __ballot(valid);
-            uint64_t below = vmask & lt_mask;
-            int src = below ? 63 - __clzll(below) : 0;
-            uint32_t pv = (uint32_t)__shfl((int)(uint32_t)vox, src, 64);
-            uint32_t prev = below ? pv : carry;
-            bool is_new = valid && (uint32_t)vox != prev;
-            uint64_t nmask = __ballot(is_new);
-            if (is_new) {
-                int64_t pos = n + __popcll(nmask & lt_mask);
-                if (pos < hcap) voxels[hoff + pos] = (uint32_t)vox;
-                hsh += pct_list_mix((uint32_t)vox, (uint32_t)pos);
-            }
-            n += __popcll(nmask);
-            if (vmask) carry = (uint32_t)__shfl((int)(uint32_t)vox, 63 - __clzll(vmask), 64);
-        }
+        int64_t n;
+        uint32_t hsh;
+        walk_list([&](int k) { return walk.step(g, b, k); }, S, carry, hoff, hcap, voxels, lane, lt_mask, n, hsh);
         if (n > hcap) local_flags |= PCT_STATUS_CAPACITY;
         if (n > g.max_intersections) local_flags |= PCT_STATUS_OVERFLOW;
         if (lane == 0) lengths[base + j] = (uint32_t)(n < hcap ? n : hcap);
@@ -546,6 +595,346 @@ k_mlp_paths(pct_geo g, pct_hist_soa h, int64_t start, int64_t count, W walk,
     }
     }
     if (lane == 0 && local_flags) atomicOr(status, local_flags);
+}
+
+/* ------------------------------------------------------------------------- */
+/* Learned-path form (include/pct_net.h): the network's inputs, the FP32      */
+/* surrogate path (cubic Hermite between the MLP's own boundary values plus   */
+/* the clamped cubic spline of the predicted knot offsets, per segment of the */
+/* normalised depth a cubic in tau whose coefficients are formed once per     */
+/* history), the guard deciding between the surrogate and the exact walk, and */
+/* the walk kernels.                                                          */
+/* ------------------------------------------------------------------------- */
+__device__ __forceinline__ void net_features(const mlp_hist& m, double xz_in, double xz_out, double wepl, float* f) {
+    f[0] = (float)((m.t_out - m.t_in) * 2.0);
+    f[1] = (float)((m.v_out - m.v_in) * 2.0);
+    f[2] = (float)(m.dT * 20.0);
+    f[3] = (float)(m.dV * 20.0);
+    f[4] = (float)(m.u_2 * 0.125 - 1.5);
+    f[5] = (float)(wepl * 0.125 - 1.5);
+    f[6] = (float)(m.t_in * 0.1);
+    f[7] = (float)(m.t_out * 0.1);
+    f[8] = (float)(m.v_in * 0.25);
+    f[9] = (float)(m.v_out * 0.25);
+    f[10] = (float)(xz_in * 10.0);
+    f[11] = (float)(xz_out * 10.0);
+}
+
+__device__ __forceinline__ int net_cell(const mlp_hist& m) {
+    int ub = (int)(m.u_2 / 3.0);
+    ub = ub > 7 ? 7 : (ub < 0 ? 0 : ub);
+    double mag = pct_fabs(m.t_out - m.t_in);
+    double a = pct_fabs(m.v_out - m.v_in), b = m.u_2 * pct_fabs(m.dT), c = m.u_2 * pct_fabs(m.dV);
+    mag = a > mag ? a : mag;
+    mag = b > mag ? b : mag;
+    mag = c > mag ? c : mag;
+    int mb = 0;
+    double thr = 0.02;
+    for (int k = 0; k < 7; ++k, thr *= 2)
+        if (mag >= thr) mb = k + 1;
+    return 8 * ub + mb;
+}
+
+__device__ __forceinline__ bool net_in_range(const float* f, const float* __restrict__ range) {
+    bool ok = true;
+#pragma unroll
+    for (int k = 0; k < PCT_NET_IN; ++k) ok = ok && f[k] >= range[k] && f[k] <= range[PCT_NET_IN + k];
+    return ok;
+}
+
+/* FP32 constants of one history's surrogate path */
+struct sur_hist {
+    float ca, sa, u_in, inv_u2;          /* rotation, entry depth, 1 / u_2             */
+    float t_in, dlt, wt, v_in, dlv, wv;  /* t_in, t_out - t_in, u_2 dtheta_t; v alike  */
+};
+
+__device__ __forceinline__ sur_hist sur_setup(const mlp_hist& m) {
+    sur_hist s;
+    s.ca = (float)m.ca; s.sa = (float)m.sa; s.u_in = (float)m.u_in; s.inv_u2 = (float)(1.0 / m.u_2);
+    s.t_in = (float)m.t_in; s.dlt = (float)(m.t_out - m.t_in); s.wt = (float)(m.u_2 * m.dT);
+    s.v_in = (float)m.v_in; s.dlv = (float)(m.v_out - m.v_in); s.wv = (float)(m.u_2 * m.dV);
+    return s;
+}
+
+__device__ __forceinline__ sur_hist sur_bcast(const sur_hist& s, int j) {
+    sur_hist b;
+    b.ca = bcast_f(s.ca, j); b.sa = bcast_f(s.sa, j); b.u_in = bcast_f(s.u_in, j); b.inv_u2 = bcast_f(s.inv_u2, j);
+    b.t_in = bcast_f(s.t_in, j); b.dlt = bcast_f(s.dlt, j); b.wt = bcast_f(s.wt, j);
+    b.v_in = bcast_f(s.v_in, j); b.dlv = bcast_f(s.dlv, j); b.wv = bcast_f(s.wv, j);
+    return b;
+}
+
+/* coefficient q (0..39) of the per-segment cubics: t for q < 20, v from 20 on;
+ * segment (q % 20) / 4, power q % 4; out8 are the network outputs */
+__device__ __forceinline__ float sur_coef(int q, const sur_hist& s, const float* out8) {
+    const bool isv = q >= 20;
+    const int r = q - (isv ? 20 : 0), i = r >> 2, k = r & 3;
+    const float base = isv ? s.v_in : s.t_in, dl = isv ? s.dlv : s.dlt, w = isv ? s.wv : s.wt;
+    const float* o = out8 + (isv ? 4 : 0);
+    float a = (k == 0 ? base : 0.0f) + dl * NET_H01[i][k];
+    a = a + w * NET_H11[i][k];
+#pragma unroll
+    for (int j = 0; j < 4; ++j) a = a + (o[j] * (float)PCT_NET_OUT_SCALE) * NET_PHI[j][i][k];
+    return a;
+}
+
+/* surrogate (t, v) at step k from the coefficients Q[40] */
+__device__ __forceinline__ void sur_pos(const sur_hist& s, const float* Q, float u1, float& t, float& v) {
+    float s5 = (u1 * s.inv_u2) * 5.0f;
+    int seg = (int)s5;
+    seg = seg > 4 ? 4 : seg;
+    float tau = s5 - (float)seg;
+    const float* qt = Q + 4 * seg;
+    const float* qv = Q + 20 + 4 * seg;
+    t = qt[0] + tau * (qt[1] + tau * (qt[2] + tau * qt[3]));
+    v = qv[0] + tau * (qv[1] + tau * (qv[2] + tau * qv[3]));
+}
+
+__device__ __forceinline__ int64_t sur_step(const pct_geo& g, const grid_f& gf, const sur_hist& s, const float* Q,
+                                            const float* __restrict__ u1f, int k) {
+    float u1 = u1f[k], t, v;
+    sur_pos(s, Q, u1, t, v);
+    float us = s.u_in + u1;
+    float x = s.ca * us - s.sa * t;
+    float y = s.sa * us + s.ca * t;
+    int vx = calc_voxel_f(gf.x_zero, x, gf.inv_w);
+    int vy = calc_voxel_f(gf.y_zero, y, gf.inv_h);
+    int vz = calc_voxel_f(gf.z_zero, v, gf.inv_t);
+    int64_t voxel = (int64_t)vx + (int64_t)vy * g.C + (int64_t)vz * g.C * g.R;
+    if (vx < 0 || vy < 0 || vz < 0 || voxel >= g.V) return -1;
+    return voxel;
+}
+
+__device__ __forceinline__ void wave_lds_sync() {
+    __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
+    __builtin_amdgcn_wave_barrier();
+    __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
+}
+
+/* per-wavefront LDS of the surrogate kernels */
+struct sur_lds {
+    uint16_t xs[64 * 16];     /* network inputs (bfloat16 bits) */
+    float out[64 * 8];        /* network outputs                */
+    float Q[40];              /* current history's cubics       */
+};
+
+/* setup of the 64 histories of a wavefront: lane j's MLP constants, its
+ * network inputs into LDS, then the network for the whole wave when any lane
+ * needs it */
+__device__ __forceinline__ void sur_wave_setup(const pct_geo& g, const pct_hist_soa& h, int64_t i, bool have,
+                                               const double* __restrict__ tab, int K, const pct_net_frag* __restrict__ nf,
+                                               const float* __restrict__ range, const uint8_t* __restrict__ cells,
+                                               bool use_cells, sur_lds& L, mlp_hist& m, bool& use, bool& in_range,
+                                               int& cell) {
+    const int lane = threadIdx.x & 63;
+    float f[PCT_NET_IN];
+#pragma unroll
+    for (int k = 0; k < PCT_NET_IN; ++k) f[k] = 0.0f;
+    use = in_range = false;
+    cell = 0;
+    if (have) {
+        mlp_setup(g, h, i, tab + MLP_TAB_U1 * K, K, m);
+        net_features(m, h.xz_entry_angle[i], h.xz_exit_angle[i], h.wepl[i], f);
+        in_range = net_in_range(f, range);
+        cell = net_cell(m);
+        use = in_range && (!use_cells || cells[cell] == 0);
+    }
+#pragma unroll
+    for (int k = 0; k < 16; ++k) L.xs[lane * 16 + k] = k < PCT_NET_IN ? (uint16_t)pct_f2bf(f[k]) : (uint16_t)0;
+    wave_lds_sync();
+    if (__ballot(use)) pct_net_infer_wave(nf, L.xs, L.out);
+    wave_lds_sync();
+}
+
+/* Surrogate MLP paths: like k_mlp_paths, one wavefront per 64 histories; a
+ * history the guard accepts is walked on its FP32 surrogate path, any other
+ * with the exact FP64 steps. counts[0] += fallbacks, counts[1] += histories. */
+__global__ void __launch_bounds__(256)
+k_mlp_paths_sur(pct_geo g, pct_hist_soa h, int64_t start, int64_t count, const double* __restrict__ tab, int K,
+                const float* __restrict__ u1f, grid_f gf, const pct_net_frag* __restrict__ nf,
+                const float* __restrict__ range, const uint8_t* __restrict__ cells, const int64_t* __restrict__ offsets,
+                uint32_t* __restrict__ voxels, uint32_t* __restrict__ lengths, double* __restrict__ chord_out,
+                int* __restrict__ status, uint32_t* __restrict__ hash_out, const uint32_t* __restrict__ hash_expect,
+                unsigned long long* __restrict__ counts) {
+    __shared__ __align__(16) sur_lds lds[4];
+    extern __shared__ float u1s[];          /* the K step depths, staged once per workgroup */
+    for (int k = threadIdx.x; k < K; k += blockDim.x) u1s[k] = u1f[k];
+    __syncthreads();
+    sur_lds& L = lds[threadIdx.x >> 6];
+    const int lane = threadIdx.x & 63;
+    const uint64_t lt_mask = (lane == 0) ? 0ull : (~0ull >> (64 - lane));
+    int local_flags = 0;
+    unsigned long long fb = 0, tot = 0;
+    for (int64_t base = ((int64_t)blockIdx.x * blockDim.x + (threadIdx.x & ~63)); base < count;
+         base += (int64_t)gridDim.x * blockDim.x) {
+        const int64_t mine = base + lane;
+        mlp_hist m;
+        bool use, in_range;
+        int cell;
+        sur_wave_setup(g, h, start + mine, mine < count, tab, K, nf, range, cells, true, L, m, use, in_range, cell);
+        int mS = 0;
+        uint32_t first = 0;
+        int64_t off = 0, cap = 0;
+        sur_hist sh{};
+        if (mine < count) {
+            mS = m.S;
+            first = h.first_voxel[start + mine];
+            off = offsets[mine];
+            cap = offsets[mine + 1] - off;
+            if (chord_out) chord_out[mine] = m.chord;
+            if (mS >= K) atomicOr(status, PCT_STATUS_TABLE);
+            sh = sur_setup(m);
+        }
+        const uint64_t umask = __ballot(use);
+        const int nh = (int)((count - base) < 64 ? (count - base) : 64);
+        tot += nh;
+        fb += nh - __popcll(umask);
+        for (int j = 0; j < nh; ++j) {
+            int S = bcast_i(mS, j);
+            if (S > K) S = K;
+            uint32_t carry = (uint32_t)bcast_i((int)first, j);
+            int64_t hoff = bcast_l(off, j);
+            int64_t hcap = bcast_l(cap, j);
+            int64_t n;
+            uint32_t hsh;
+            if ((umask >> j) & 1) {
+                sur_hist b = sur_bcast(sh, j);
+                if (lane < 40) L.Q[lane] = sur_coef(lane, b, L.out + 8 * j);
+                wave_lds_sync();
+                walk_list([&](int k) { return sur_step(g, gf, b, L.Q, u1s, k); }, S, carry, hoff, hcap, voxels, lane,
+                          lt_mask, n, hsh);
+                wave_lds_sync();
+            } else {
+                exact_walk ew{tab, K};
+                mlp_hist b = ew.bcast(m, j);
+                walk_list([&](int k) { return mlp_step(g, b, tab, K, k); }, S, carry, hoff, hcap, voxels, lane,
+                          lt_mask, n, hsh);
+            }
+            if (n > hcap) local_flags |= PCT_STATUS_CAPACITY;
+            if (n > g.max_intersections) local_flags |= PCT_STATUS_OVERFLOW;
+            if (lane == 0) lengths[base + j] = (uint32_t)(n < hcap ? n : hcap);
+            if (hash_out || hash_expect) {
+#pragma unroll
+                for (int d = 32; d >= 1; d >>= 1) hsh += (uint32_t)__shfl_xor((int)hsh, d, 64);
+                hsh += (uint32_t)n * 0x27d4eb2fu;
+                if (lane == 0) {
+                    if (hash_out) hash_out[base + j] = hsh;
+                    if (hash_expect && hash_expect[base + j] != hsh) local_flags |= PCT_STATUS_PLAN;
+                }
+            }
+        }
+    }
+    if (lane == 0) {
+        if (local_flags) atomicOr(status, local_flags);
+        if (tot) {
+            atomicAdd(&counts[0], fb);
+            atomicAdd(&counts[1], tot);
+        }
+    }
+}
+
+/* Calibration check: sampled history q (index start + q stride) -- largest
+ * |t| / |v| deviation of the surrogate path from the exact MLP over its steps,
+ * whether the two voxel lists are equal, its cell and whether its inputs are
+ * inside the trained range. */
+__global__ void __launch_bounds__(256)
+k_net_calib(pct_geo g, pct_hist_soa h, int64_t start, int64_t stride, int64_t nsamp, const double* __restrict__ tab,
+            int K, const float* __restrict__ u1f, grid_f gf, const pct_net_frag* __restrict__ nf,
+            const float* __restrict__ range, float* __restrict__ dev_out, int32_t* __restrict__ cell_out,
+            uint8_t* __restrict__ mismatch_out, uint8_t* __restrict__ in_range_out) {
+    __shared__ __align__(16) sur_lds lds[4];
+    extern __shared__ float u1s[];
+    for (int k = threadIdx.x; k < K; k += blockDim.x) u1s[k] = u1f[k];
+    __syncthreads();
+    sur_lds& L = lds[threadIdx.x >> 6];
+    const int lane = threadIdx.x & 63;
+    const uint64_t lt_mask = (lane == 0) ? 0ull : (~0ull >> (64 - lane));
+    for (int64_t base = ((int64_t)blockIdx.x * blockDim.x + (threadIdx.x & ~63)); base < nsamp;
+         base += (int64_t)gridDim.x * blockDim.x) {
+        const int64_t q = base + lane;
+        mlp_hist m;
+        bool use, in_range;
+        int cell;
+        sur_wave_setup(g, h, start + q * stride, q < nsamp, tab, K, nf, range, nullptr, false, L, m, use, in_range,
+                       cell);
+        int mS = 0;
+        uint32_t first = 0;
+        sur_hist sh{};
+        if (q < nsamp) {
+            mS = m.S < K ? m.S : K;
+            first = h.first_voxel[start + q * stride];
+            sh = sur_setup(m);
+        }
+        const int nh = (int)((nsamp - base) < 64 ? (nsamp - base) : 64);
+        for (int j = 0; j < nh; ++j) {
+            const int S = bcast_i(mS, j);
+            const uint32_t carry = (uint32_t)bcast_i((int)first, j);
+            sur_hist b = sur_bcast(sh, j);
+            exact_walk ew{tab, K};
+            mlp_hist e = ew.bcast(m, j);
+            if (lane < 40) L.Q[lane] = sur_coef(lane, b, L.out + 8 * j);
+            wave_lds_sync();
+            double dev = 0.0;
+            for (int c = 0; c < S; c += 64) {
+                const int k = c + lane;
+                if (k < S) {
+                    double u, te, ve;
+                    mlp_tv(g, e, tab, K, k, u, te, ve);
+                    float ts, vs;
+                    sur_pos(b, L.Q, u1s[k], ts, vs);
+                    double d1 = pct_fabs((double)ts - te), d2 = pct_fabs((double)vs - ve);
+                    d1 = d1 > d2 ? d1 : d2;
+                    dev = d1 > dev ? d1 : dev;
+                }
+            }
+            for (int d = 32; d >= 1; d >>= 1) {
+                double o = __shfl_xor(dev, d, 64);
+                dev = o > dev ? o : dev;
+            }
+            int64_t n1, n2;
+            uint32_t h1, h2;
+            walk_list([&](int k) { return sur_step(g, gf, b, L.Q, u1s, k); }, S, carry, 0, 0, nullptr, lane, lt_mask,
+                      n1, h1);
+            walk_list([&](int k) { return mlp_step(g, e, tab, K, k); }, S, carry, 0, 0, nullptr, lane, lt_mask, n2, h2);
+            uint32_t hd = h1 - h2;
+            for (int d = 32; d >= 1; d >>= 1) hd += (uint32_t)__shfl_xor((int)hd, d, 64);
+            const int ir = bcast_i((int)in_range, j), cl = bcast_i(cell, j);
+            if (lane == 0) {
+                dev_out[base + j] = (float)dev;
+                cell_out[base + j] = cl;
+                mismatch_out[base + j] = (n1 != n2 || hd != 0) ? 1 : 0;
+                in_range_out[base + j] = (uint8_t)ir;
+            }
+            wave_lds_sync();
+        }
+    }
+}
+
+/* training data: network inputs and the exact FP64 MLP offsets at the knots */
+__global__ void __launch_bounds__(256)
+k_net_features(pct_geo g, pct_hist_soa h, int64_t start, int64_t count, const double* __restrict__ tab, int K,
+               float* __restrict__ features, float* __restrict__ targets) {
+    const int64_t q = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (q >= count) return;
+    const int64_t i = start + q;
+    mlp_hist m;
+    mlp_setup(g, h, i, tab + MLP_TAB_U1 * K, K, m);
+    float f[PCT_NET_IN];
+    net_features(m, h.xz_entry_angle[i], h.xz_exit_angle[i], h.wepl[i], f);
+#pragma unroll
+    for (int k = 0; k < PCT_NET_IN; ++k) features[q * PCT_NET_IN + k] = f[k];
+    if (!targets) return;
+    for (int j = 1; j <= 4; ++j) {
+        double s = j / 5.0;
+        double u_1 = m.u_2 * j / 5.0;
+        double tt, vv;
+        mlp_point_at(g, m, u_1, tt, vv);
+        double h01 = 3 * s * s - 2 * s * s * s, h11 = s * s * s - s * s;
+        double tH = m.t_in + (m.t_out - m.t_in) * h01 + m.u_2 * m.dT * h11;
+        double vH = m.v_in + (m.v_out - m.v_in) * h01 + m.u_2 * m.dV * h11;
+        targets[q * PCT_NET_OUT + (j - 1)] = (float)((tt - tH) / PCT_NET_OUT_SCALE);
+        targets[q * PCT_NET_OUT + 4 + (j - 1)] = (float)((vv - vH) / PCT_NET_OUT_SCALE);
+    }
 }
 
 }  // namespace
@@ -586,6 +975,15 @@ static int ensure_mlp_table(pct_ctx* ctx) {
     if (e != hipSuccess) { (void)hipFree(d); return pct_fail(ctx, PCT_E_HIP, "MLP table copy: %s", hipGetErrorString(e)); }
     ctx->d_mlp_tab = d;
     ctx->mlp_tab_k = K;
+    /* the u_1 column rounded to float, for the FP32 surrogate path */
+    std::vector<float> u1f(K);
+    for (int k = 0; k < K; ++k) u1f[k] = (float)tab[MLP_TAB_U1 * K + k];
+    if (hipMalloc(&ctx->d_u1f, sizeof(float) * K) != hipSuccess) {
+        ctx->d_u1f = nullptr;
+        return pct_fail(ctx, PCT_E_NOMEM, "u_1 table alloc");
+    }
+    e = hipMemcpy(ctx->d_u1f, u1f.data(), sizeof(float) * K, hipMemcpyHostToDevice);
+    if (e != hipSuccess) return pct_fail(ctx, PCT_E_HIP, "u_1 table copy: %s", hipGetErrorString(e));
     return PCT_OK;
 }
 
@@ -612,6 +1010,25 @@ static int ensure_scat_table(pct_ctx* ctx) {
     if (e != hipSuccess) { (void)hipFree(d); return pct_fail(ctx, PCT_E_HIP, "depth table copy: %s", hipGetErrorString(e)); }
     ctx->d_scat_tab = d;
     ctx->scat_q = Q;
+    return PCT_OK;
+}
+
+static grid_f make_grid_f(const pct_geo& g) {
+    return grid_f{(float)g.x_zero, (float)g.y_zero, (float)g.z_zero, (float)(1.0 / g.voxel_width),
+                  (float)(1.0 / g.voxel_height), (float)(1.0 / g.voxel_thickness)};
+}
+
+/* resident workgroups per CU of each walk kernel (the grid-stride launch size) */
+static int mlp_occupancy(pct_ctx* ctx) {
+    if (ctx->cus) return PCT_OK;
+    PCT_HIP_CHECK(ctx, hipDeviceGetAttribute(&ctx->cus, hipDeviceAttributeMultiprocessorCount, ctx->device));
+    PCT_HIP_CHECK(ctx, hipOccupancyMaxActiveBlocksPerMultiprocessor(&ctx->mlp_occ[PCT_MLP_FP64],
+                                                                    k_mlp_paths<exact_walk>, 256, 0));
+    PCT_HIP_CHECK(ctx, hipOccupancyMaxActiveBlocksPerMultiprocessor(&ctx->mlp_occ[PCT_MLP_TABLE],
+                                                                    k_mlp_paths<table_walk>, 256, 0));
+    PCT_HIP_CHECK(ctx, hipOccupancyMaxActiveBlocksPerMultiprocessor(&ctx->mlp_occ[PCT_MLP_SURROGATE], k_mlp_paths_sur,
+                                                                    256, sizeof(float) * ctx->mlp_tab_k));
+    ctx->cus = std::max(1, ctx->cus);
     return PCT_OK;
 }
 
@@ -660,30 +1077,30 @@ int pct_mlp_paths_on(pct_ctx* ctx, hipStream_t stream, const pct_hist_soa* hist,
                      const uint32_t* hash_expect, int mode) {
     int rc = ensure_mlp_table(ctx);
     if (rc) return rc;
-    if (mode != PCT_MLP_FP64 && mode != PCT_MLP_TABLE)
-        return pct_fail(ctx, PCT_E_INVALID, "unknown MLP mode %d", mode);
+    if (!pct_mlp_mode_ok(ctx, mode))
+        return pct_fail(ctx, PCT_E_INVALID, mode == PCT_MLP_SURROGATE ? "surrogate MLP mode without a network (pct_net_set)"
+                                                                      : "unknown MLP mode %d", mode);
     if (mode == PCT_MLP_TABLE && (rc = ensure_scat_table(ctx))) return rc;
     if (count == 0) return PCT_OK;
-    if (!ctx->cus) {
-        /* resident workgroups of each walk: the grid-stride launch size */
-        PCT_HIP_CHECK(ctx, hipDeviceGetAttribute(&ctx->cus, hipDeviceAttributeMultiprocessorCount, ctx->device));
-        PCT_HIP_CHECK(ctx, hipOccupancyMaxActiveBlocksPerMultiprocessor(&ctx->mlp_occ[0], k_mlp_paths<exact_walk>, 256, 0));
-        PCT_HIP_CHECK(ctx, hipOccupancyMaxActiveBlocksPerMultiprocessor(&ctx->mlp_occ[1], k_mlp_paths<table_walk>, 256, 0));
-        ctx->cus = std::max(1, ctx->cus);
-    }
+    if ((rc = mlp_occupancy(ctx))) return rc;
     /* a workgroup of 256 is one wavefront on each of the CU's 4 SIMDs */
-    int per_cu = std::max(1, ctx->mlp_occ[mode == PCT_MLP_FP64 ? 0 : 1]);
+    int per_cu = std::max(1, ctx->mlp_occ[mode]);
     if (ctx->tune_mlp_waves > 0 && ctx->tune_mlp_waves < per_cu) per_cu = ctx->tune_mlp_waves;
     const unsigned grid = std::min(pct_grid(count, 256), (unsigned)(ctx->cus * per_cu));
-    if (mode == PCT_MLP_FP64) {
+    if (mode == PCT_MLP_SURROGATE) {
+        hipLaunchKernelGGL(k_mlp_paths_sur, dim3(grid), dim3(256), sizeof(float) * ctx->mlp_tab_k, stream, ctx->g, *hist,
+                           start, count,
+                           (const double*)ctx->d_mlp_tab, ctx->mlp_tab_k, (const float*)ctx->d_u1f, make_grid_f(ctx->g),
+                           (const pct_net_frag*)ctx->d_net, (const float*)ctx->d_net_range,
+                           (const uint8_t*)ctx->d_net_cells, offsets, voxels, lengths, chord, ctx->d_status, hash_out,
+                           hash_expect, ctx->d_net_counts);
+    } else if (mode == PCT_MLP_FP64) {
         exact_walk w{(const double*)ctx->d_mlp_tab, ctx->mlp_tab_k};
         hipLaunchKernelGGL(k_mlp_paths<exact_walk>, dim3(grid), dim3(256), 0, stream, ctx->g, *hist,
                            start, count, w, offsets, voxels, lengths, chord, ctx->d_status, hash_out, hash_expect);
     } else {
-        const pct_geo& g = ctx->g;
-        grid_f gf{(float)g.x_zero, (float)g.y_zero, (float)g.z_zero, (float)(1.0 / g.voxel_width),
-                  (float)(1.0 / g.voxel_height), (float)(1.0 / g.voxel_thickness)};
-        table_walk w{(const double*)ctx->d_mlp_tab, ctx->mlp_tab_k, (const double*)ctx->d_scat_tab, ctx->scat_q, gf};
+        table_walk w{(const double*)ctx->d_mlp_tab, ctx->mlp_tab_k, (const double*)ctx->d_scat_tab, ctx->scat_q,
+                     make_grid_f(ctx->g)};
         hipLaunchKernelGGL(k_mlp_paths<table_walk>, dim3(grid), dim3(256), 0, stream, ctx->g, *hist,
                            start, count, w, offsets, voxels, lengths, chord, ctx->d_status, hash_out, hash_expect);
     }
@@ -720,8 +1137,7 @@ extern "C" int pct_mlp_paths(pct_ctx* ctx, const pct_hist_soa* hist, int64_t sta
                              uint32_t* voxels, uint32_t* lengths, int mode) {
     int rc = check_hist(ctx, hist, start, count, "pct_mlp_paths");
     if (rc) return rc;
-    if (mode != PCT_MLP_FP64 && mode != PCT_MLP_TABLE)
-        return pct_fail(ctx, PCT_E_INVALID, "pct_mlp_paths: unknown mode %d", mode);
+    if (!pct_mlp_mode_ok(ctx, mode)) return pct_fail(ctx, PCT_E_INVALID, "pct_mlp_paths: mode %d not available", mode);
     if (!offsets || !voxels || !lengths) return pct_fail(ctx, PCT_E_INVALID, "pct_mlp_paths: null output");
     rc = pct_mlp_paths_impl(ctx, hist, start, count, offsets, voxels, lengths, nullptr, nullptr, nullptr, mode);
     if (rc) return rc;
@@ -729,4 +1145,104 @@ extern "C" int pct_mlp_paths(pct_ctx* ctx, const pct_hist_soa* hist, int64_t sta
     rc = pct_take_status(ctx, &flags);
     if (rc) return rc;
     return pct_status_to_rc(ctx, flags);
+}
+
+/* ------------------------------------------------------------------------- */
+/* learned-path form: training data and the calibration check                 */
+/* ------------------------------------------------------------------------- */
+extern "C" int pct_net_features(pct_ctx* ctx, const pct_hist_soa* hist, int64_t start, int64_t count, float* features,
+                                float* targets) {
+    int rc = check_hist(ctx, hist, start, count, "pct_net_features");
+    if (rc) return rc;
+    if (!features || !hist->wepl) return pct_fail(ctx, PCT_E_INVALID, "pct_net_features: null features/wepl");
+    if ((rc = ensure_mlp_table(ctx))) return rc;
+    if (count == 0) return PCT_OK;
+    hipLaunchKernelGGL(k_net_features, dim3(pct_grid(count, 256)), dim3(256), 0, ctx->stream, ctx->g, *hist, start,
+                       count, (const double*)ctx->d_mlp_tab, ctx->mlp_tab_k, features, targets);
+    PCT_HIP_CHECK(ctx, hipGetLastError());
+    return PCT_OK;
+}
+
+extern "C" int pct_net_calibrate(pct_ctx* ctx, const pct_hist_soa* hist, int64_t start, int64_t count, int64_t stride,
+                                 float tol, int32_t min_samples, pct_net_calib* out) {
+    int rc = check_hist(ctx, hist, start, count, "pct_net_calibrate");
+    if (rc) return rc;
+    if (!out || stride <= 0 || !hist->wepl) return pct_fail(ctx, PCT_E_INVALID, "pct_net_calibrate: bad argument");
+    if (!ctx->net_version) return pct_fail(ctx, PCT_E_INVALID, "pct_net_calibrate: no network installed");
+    if ((rc = ensure_mlp_table(ctx)) || (rc = mlp_occupancy(ctx))) return rc;
+    const int64_t nsamp = (count + stride - 1) / stride;
+    *out = pct_net_calib{};
+    if (nsamp == 0) return PCT_OK;
+    const size_t per = sizeof(float) + sizeof(int32_t) + 2;
+    if ((rc = pct_ensure(ctx, ctx->net_calib, per * (size_t)nsamp + 64))) return rc;
+    char* base = (char*)ctx->net_calib.ptr;
+    float* dev = (float*)base;
+    int32_t* cell = (int32_t*)(base + sizeof(float) * nsamp);
+    uint8_t* mis = (uint8_t*)(cell + nsamp);
+    uint8_t* inr = mis + nsamp;
+    const unsigned grid = std::min(pct_grid(nsamp, 256), (unsigned)(ctx->cus * std::max(1, ctx->mlp_occ[PCT_MLP_SURROGATE])));
+    hipLaunchKernelGGL(k_net_calib, dim3(grid), dim3(256), sizeof(float) * ctx->mlp_tab_k, ctx->stream, ctx->g, *hist,
+                       start, stride, nsamp,
+                       (const double*)ctx->d_mlp_tab, ctx->mlp_tab_k, (const float*)ctx->d_u1f, make_grid_f(ctx->g),
+                       (const pct_net_frag*)ctx->d_net, (const float*)ctx->d_net_range, dev, cell, mis, inr);
+    PCT_HIP_CHECK(ctx, hipGetLastError());
+    std::vector<float> hdev(nsamp);
+    std::vector<int32_t> hcell(nsamp);
+    std::vector<uint8_t> hmis(nsamp), hinr(nsamp);
+    PCT_HIP_CHECK(ctx, hipMemcpyAsync(hdev.data(), dev, sizeof(float) * nsamp, hipMemcpyDeviceToHost, ctx->stream));
+    PCT_HIP_CHECK(ctx, hipMemcpyAsync(hcell.data(), cell, sizeof(int32_t) * nsamp, hipMemcpyDeviceToHost, ctx->stream));
+    PCT_HIP_CHECK(ctx, hipMemcpyAsync(hmis.data(), mis, nsamp, hipMemcpyDeviceToHost, ctx->stream));
+    PCT_HIP_CHECK(ctx, hipMemcpyAsync(hinr.data(), inr, nsamp, hipMemcpyDeviceToHost, ctx->stream));
+    PCT_HIP_CHECK(ctx, hipStreamSynchronize(ctx->stream));
+    /* statistics over the samples whose inputs are inside the trained range
+     * (the others take the exact walk whatever their cell) */
+    std::vector<std::vector<float>> per_cell(PCT_NET_CELLS);
+    std::vector<float> all;
+    out->samples = nsamp;
+    for (int64_t q = 0; q < nsamp; ++q) {
+        out->list_mismatch += hmis[q];
+        if (!hinr[q]) continue;
+        out->in_range++;
+        all.push_back(hdev[q]);
+        int c = hcell[q] < 0 ? 0 : (hcell[q] >= PCT_NET_CELLS ? PCT_NET_CELLS - 1 : hcell[q]);
+        per_cell[c].push_back(hdev[q]);
+    }
+    auto quant = [](std::vector<float>& v, double p) -> double {
+        if (v.empty()) return 0.0;
+        size_t k = (size_t)std::min<double>((double)v.size() - 1, std::floor(p * (double)(v.size() - 1) + 0.5));
+        std::nth_element(v.begin(), v.begin() + k, v.end());
+        return v[k];
+    };
+    out->dev_median = quant(all, 0.5);
+    out->dev_p99 = quant(all, 0.99);
+    out->dev_max = all.empty() ? 0.0 : *std::max_element(all.begin(), all.end());
+    uint8_t flags[PCT_NET_CELLS];
+    for (int c = 0; c < PCT_NET_CELLS; ++c) {
+        out->cell_samples[c] = (int32_t)per_cell[c].size();
+        out->cell_p99[c] = (float)quant(per_cell[c], 0.99);
+        bool fb = (int32_t)per_cell[c].size() < min_samples || out->cell_p99[c] > tol;
+        out->cell_fallback[c] = fb ? 1 : 0;
+        flags[c] = out->cell_fallback[c];
+        out->cells_fallback += fb ? 1 : 0;
+    }
+    if (tol > 0.0f) {
+        PCT_HIP_CHECK(ctx, hipMemcpy(ctx->d_net_cells, flags, sizeof flags, hipMemcpyHostToDevice));
+        static int calib_version = 1 << 20;
+        ctx->net_version = ++calib_version;     /* plans built before the calibration are stale */
+    } else {
+        /* measurement only: report the flags already in force */
+        PCT_HIP_CHECK(ctx, hipMemcpy(flags, ctx->d_net_cells, sizeof flags, hipMemcpyDeviceToHost));
+        out->cells_fallback = 0;
+        for (int c = 0; c < PCT_NET_CELLS; ++c) {
+            out->cell_fallback[c] = flags[c];
+            out->cells_fallback += flags[c] ? 1 : 0;
+        }
+    }
+    for (int64_t q = 0; q < nsamp; ++q) {
+        int c = hcell[q] < 0 ? 0 : (hcell[q] >= PCT_NET_CELLS ? PCT_NET_CELLS - 1 : hcell[q]);
+        if (!hinr[q] || flags[c]) continue;
+        out->surrogate++;
+        out->surrogate_mismatch += hmis[q];
+    }
+    return PCT_OK;
 }

@@ -16,6 +16,7 @@ struct pct_plan {
     int64_t E = 0;                 /* path entries (sum of list lengths)    */
     int64_t S = 0;                 /* segments = sum over blocks of |V_B|   */
     int mlp_mode = 0;              /* MLP mode the lists were built with    */
+    int net_version = 0;           /* network + calibration of a surrogate plan */
     /* path lists, block-strided: history k of block b at
      * list[list_base[b] + k * stride[b]], unused slots hold the sentinel */
     pct_buffer list;               /* u32                                   */

@@ -100,9 +100,11 @@ __global__ void k_hist_offsets(const int64_t* __restrict__ list_base, const int3
     hist_off[h] = list_base[b] + (h - b * B) * stride[b];
 }
 
+/* grid-stride: a dispatch holds at most 2^32 - 1 work-items, and the list
+ * store of a large scan has more entries than that */
 __global__ void k_fill_u32(uint32_t* __restrict__ p, int64_t n, uint32_t v) {
-    int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
-    if (i < n) p[i] = v;
+    for (int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; i < n; i += (int64_t)gridDim.x * blockDim.x)
+        p[i] = v;
 }
 
 /* one wavefront per history: copy its list from the capacity-strided build
@@ -110,13 +112,13 @@ __global__ void k_fill_u32(uint32_t* __restrict__ p, int64_t n, uint32_t v) {
 __global__ void __launch_bounds__(256)
 k_copy_lists(const uint32_t* __restrict__ src, const int64_t* __restrict__ cap_off, const uint32_t* __restrict__ len,
              const int64_t* __restrict__ hist_off, int64_t n, uint32_t* __restrict__ dst) {
-    int64_t h = (int64_t)blockIdx.x * 4 + (threadIdx.x >> 6);
-    int lane = threadIdx.x & 63;
-    if (h >= n) return;
-    int m = (int)len[h];
-    const uint32_t* s = src + cap_off[h];
-    uint32_t* d = dst + hist_off[h];
-    for (int j = lane; j < m; j += 64) d[j] = s[j];
+    const int lane = threadIdx.x & 63;
+    for (int64_t h = (int64_t)blockIdx.x * 4 + (threadIdx.x >> 6); h < n; h += (int64_t)gridDim.x * 4) {
+        int m = (int)len[h];
+        const uint32_t* s = src + cap_off[h];
+        uint32_t* d = dst + hist_off[h];
+        for (int j = lane; j < m; j += 64) d[j] = s[j];
+    }
 }
 
 /* one wavefront per history of the chunk: keys = (local block << vbits) | voxel,
@@ -320,9 +322,9 @@ static int plan_build(pct_ctx* ctx, const pct_hist_soa* hist, int B, int mlp_mod
     hipLaunchKernelGGL(k_hist_offsets, dim3(pct_grid(n + 1, 256)), dim3(256), 0, ctx->stream,
                        (const int64_t*)tmp_base.ptr, (const int32_t*)tmp_stride.ptr, n, (int64_t)B, nb,
                        (int64_t*)pl->hist_off.ptr);
-    hipLaunchKernelGGL(k_fill_u32, dim3(pct_grid(list_size, 256)), dim3(256), 0, ctx->stream, (uint32_t*)pl->list.ptr,
-                       list_size, LIST_SENTINEL);
-    hipLaunchKernelGGL(k_copy_lists, dim3((unsigned)((n + 3) / 4)), dim3(256), 0, ctx->stream,
+    hipLaunchKernelGGL(k_fill_u32, dim3(std::min(pct_grid(list_size, 256), PCT_GRID_CAP)), dim3(256), 0, ctx->stream,
+                       (uint32_t*)pl->list.ptr, list_size, LIST_SENTINEL);
+    hipLaunchKernelGGL(k_copy_lists, dim3(std::min(pct_grid(n, 4), PCT_GRID_CAP)), dim3(256), 0, ctx->stream,
                        (const uint32_t*)ctx->path_store.ptr, cap_off, (const uint32_t*)len,
                        (const int64_t*)pl->hist_off.ptr, n, (uint32_t*)pl->list.ptr);
     PCT_HIP_CHECK(ctx, hipGetLastError());
@@ -452,6 +454,7 @@ static int plan_build(pct_ctx* ctx, const pct_hist_soa* hist, int B, int mlp_mod
     pl->S = S_total;
     pl->key = *hist;
     pl->mlp_mode = mlp_mode;
+    pl->net_version = ctx->net_version;
     pl->chunk_b.clear();
     pl->valid = true;
     return PCT_OK;
@@ -463,12 +466,19 @@ static bool same_hist(const pct_hist_soa& a, const pct_hist_soa& b) {
 }
 
 /* internal entry points for the batched form (pct_batch.hip) */
+/* the plan holds the lists of these histories, this block size and MLP mode
+ * (and, for the surrogate, the network and calibration it was built with) */
+static bool plan_matches(const pct_ctx* ctx, const pct_plan* pl, const pct_hist_soa* hist, int B, int mlp) {
+    return pl && pl->valid && pl->B == B && pl->mlp_mode == mlp && same_hist(pl->key, *hist) &&
+           (mlp != PCT_MLP_SURROGATE || pl->net_version == ctx->net_version);
+}
+
 int pct_plan_build(pct_ctx* ctx, const pct_hist_soa* hist, int B, int mlp_mode) {
     return plan_build(ctx, hist, B, mlp_mode);
 }
 bool pct_plan_ready(const pct_ctx* ctx, const pct_hist_soa* hist, int B, int mlp_mode) {
     const pct_plan* pl = ctx->plan;
-    return pl && pl->valid && pl->B == B && pl->mlp_mode == mlp_mode && same_hist(pl->key, *hist);
+    return plan_matches(ctx, pl, hist, B, mlp_mode);
 }
 
 /* the block loop over blocks [b_begin, b_end) on the plan */
@@ -622,8 +632,7 @@ extern "C" int pct_drop_prepare(pct_ctx* ctx, const pct_hist_soa* hist, int bloc
     if (!ctx || !hist || !hist->wepl || !hist->first_voxel || !hist->x_entry)
         return pct_fail(ctx, PCT_E_INVALID, "pct_drop_prepare: null argument");
     const int mlp = mode & PCT_MLP_MODE_MASK;
-    if (mlp != PCT_MLP_FP64 && mlp != PCT_MLP_TABLE)
-        return pct_fail(ctx, PCT_E_INVALID, "pct_drop_prepare: unknown MLP mode %d", mlp);
+    if (!pct_mlp_mode_ok(ctx, mlp)) return pct_fail(ctx, PCT_E_INVALID, "pct_drop_prepare: MLP mode %d not available", mlp);
     if (hist->n <= 0) return pct_fail(ctx, PCT_E_INVALID, "pct_drop_prepare: no histories");
     return plan_build(ctx, hist, block_size, mlp);
 }
@@ -632,15 +641,15 @@ extern "C" int pct_drop_iteration(pct_ctx* ctx, const pct_hist_soa* hist, float*
     if (!ctx || !hist || !hist->wepl || !hist->first_voxel || !hist->x_entry)
         return pct_fail(ctx, PCT_E_INVALID, "pct_drop_iteration: null argument");
     const int mlp = mode & PCT_MLP_MODE_MASK;
-    if (mlp != PCT_MLP_FP64 && mlp != PCT_MLP_TABLE)
-        return pct_fail(ctx, PCT_E_INVALID, "pct_drop_iteration: unknown MLP mode %d", mlp);
+    if (!pct_mlp_mode_ok(ctx, mlp))
+        return pct_fail(ctx, PCT_E_INVALID, "pct_drop_iteration: MLP mode %d not available", mlp);
     if (!x || block_size <= 0) return pct_fail(ctx, PCT_E_INVALID, "pct_drop_iteration: bad x or block size");
     if (hist->n <= 0) return PCT_OK;
     if (mode & PCT_DROP_ATOMIC) return pct_drop_iteration_atomic(ctx, hist, x, block_size, lambda, mlp);
     int rc;
     pct_plan* pl = ctx->plan;
     bool fresh = false;
-    if (!pl || !pl->valid || pl->B != block_size || pl->mlp_mode != mlp || !same_hist(pl->key, *hist)) {
+    if (!plan_matches(ctx, pl, hist, block_size, mlp)) {
         if ((rc = plan_build(ctx, hist, block_size, mlp))) return rc;
         pl = ctx->plan;
         fresh = true;
