@@ -279,9 +279,10 @@ def main():
     ap.add_argument("--no-large", action="store_true", help="skip the 256x256x48 / 50 M learned-form case")
     ap.add_argument("--quick", action="store_true",
                     help="only the timed iterations (for counter runs, where every dispatch is slow)")
-    ap.add_argument("--form", default="default", choices=["default", "cached", "sequential", "precision"],
+    ap.add_argument("--form", default="default", choices=["default", "cached", "sequential", "precision", "learned"],
                     help="with --quick: the form of the timed iterations (default: MLP beside the block loop; "
-                         "cached: the block loop alone; sequential: MLP then block loop; precision)")
+                         "cached: the block loop alone; sequential: MLP then block loop; precision; learned: "
+                         "network trained first on 10 M exact-MLP paths)")
     ap.add_argument("--copy-calibration", action="store_true",
                     help="5 device copies of 256 MiB and nothing else: the reference kernel for the HBM counters")
     args = ap.parse_args()
@@ -365,6 +366,16 @@ def main():
     elif args.quick and args.form == "precision":
         form_mode = pct_amd.PCT_PRECISION
         ctx.drop_prepare(hist, B, pct_amd.PCT_MLP_TABLE)
+    elif args.quick and args.form == "learned":
+        from pct_amd import learned
+        scans = learned.training_scans(configs.LEARNED_TRAIN_SCANS, configs.LEARNED_TRAIN_HISTORIES_PER_SCAN)
+        F, Y = learned.training_set(configs.LEARNED_TRAIN_PATHS, scans)
+        net = learned.train_network(ctx, F, Y, epochs=20, batch=8192, lr=3e-3, seed=3)
+        del F, Y
+        net.install(ctx)
+        learned.calibrate(ctx, hist, stride=64, tol=0.03 * 2 * p.recon_cyl_radius / p.columns)
+        form_mode = pct_amd.PCT_LEARNED
+        ctx.drop_prepare(hist, B, pct_amd.PCT_MLP_SURROGATE)
     for _ in range(args.warmup):
         ctx.drop_iteration(hist, x, B, p.lambda_, form_mode)
     ctx.sync()

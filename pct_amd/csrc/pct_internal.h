@@ -80,7 +80,7 @@ struct pct_ctx {
     unsigned long long* d_net_counts = nullptr;
     float* d_u1f = nullptr;
     int net_version = 0;
-    pct_buffer net_partial, net_state, net_loss, net_calib;
+    pct_buffer net_partial, net_state, net_loss, net_calib, net_stage_buf;
     /* execution knobs (pct_set_tuning; 0 = default) */
     int tune_mlp_waves = 0, tune_chunk_first = 0, tune_chunk_max = 0, tune_overlap = 0;
     /* workspaces, grown on demand */

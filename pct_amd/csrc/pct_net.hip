@@ -12,8 +12,9 @@
  * "samples are rows": Z1 = X W1, Z2 = A1 W2, Z3 = A2 W3, and the backward
  * products dW = A^T dZ reduce over the 64 samples as the MFMA K dimension.
  * Each workgroup writes its partial gradient (and squared error) to its own
- * row; k_net_reduce sums the rows in row order, so the gradient does not
- * depend on scheduling.
+ * row; k_net_reduce sums the rows in a fixed order, so the gradient does not
+ * depend on scheduling. The weights enter every step as a bfloat16 image in
+ * the kernel's LDS layout, written by the previous optimizer step.
  */
 #include <algorithm>
 #include <cmath>
@@ -35,34 +36,70 @@ __device__ inline net_u32x4 gather8(const uint16_t* base, int stride) {
     return r;
 }
 
+/* bfloat16 image of the weights in the training kernel's LDS layout (W1 with
+ * rows 12..15 zero, W3 with columns 8..15 zero) and the FP32 biases; kept up
+ * to date by the optimizer step, copied to LDS with 16-byte loads */
+struct net_stage {
+    uint16_t W1s[16 * 64], W2s[64 * 64], W3s[64 * 16];
+    float bias[136];                     /* b1[64], b2[64], b3[8] */
+};
+static_assert(sizeof(net_stage) % 16 == 0, "stage copied in 16-byte units");
+
+/* position of parameter p in the stage: bf16 index (>= 0) or -(1 + f32 index) */
+__host__ __device__ inline int stage_pos(int p) {
+    if (p < PCT_NET_OFF_B1) return p;                                              /* W1[f][h] -> f*64 + h */
+    if (p < PCT_NET_OFF_W2) return -(1 + (p - PCT_NET_OFF_B1));                    /* b1 */
+    if (p < PCT_NET_OFF_B2) return 16 * 64 + (p - PCT_NET_OFF_W2);                 /* W2 */
+    if (p < PCT_NET_OFF_W3) return -(1 + 64 + (p - PCT_NET_OFF_B2));              /* b2 */
+    if (p < PCT_NET_OFF_B3) {                                                      /* W3[h][o] -> h*16 + o */
+        int q = p - PCT_NET_OFF_W3;
+        return 16 * 64 + 64 * 64 + (q >> 3) * 16 + (q & 7);
+    }
+    return -(1 + 128 + (p - PCT_NET_OFF_B3));                                      /* b3 */
+}
+
+__device__ inline void stage_write(net_stage* st, int p, float v) {
+    int q = stage_pos(p);
+    if (q >= 0) ((uint16_t*)st)[q] = (uint16_t)pct_f2bf(v);
+    else st->bias[-q - 1] = v;
+}
+
+/* the whole stage from the parameters (padding zeroed) */
+__global__ void __launch_bounds__(256) k_net_stage(const float* __restrict__ params, net_stage* __restrict__ st) {
+    for (int i = threadIdx.x; i < 16 * 64; i += 256) st->W1s[i] = 0;
+    for (int i = threadIdx.x; i < 64 * 16; i += 256) st->W3s[i] = 0;
+    __syncthreads();
+    for (int p = threadIdx.x; p < NP; p += 256) stage_write(st, p, params[p]);
+}
+
 /* Forward + backward of 64 samples. Samples are positions j0 + 64 blockIdx.x
  * + s (s < 64) of the pass, position j < j0 + nsamp being sample
  * (pa j + pb) mod n (pa = 0: sample j). scale = 2 / (8 N) for a batch of N. */
 __global__ void __launch_bounds__(64)
-k_net_train(const float* __restrict__ params, const float* __restrict__ X, const float* __restrict__ Y, int64_t n,
+k_net_train(const net_stage* __restrict__ stage, const float* __restrict__ X, const float* __restrict__ Y, int64_t n,
             int64_t j0, int64_t nsamp, uint64_t pa, uint64_t pb, float scale, float* __restrict__ partial) {
-    __shared__ __align__(16) uint16_t W1s[16 * 64], W2s[64 * 64], W3s[64 * 16];
+    __shared__ __align__(16) net_stage S;
     __shared__ __align__(16) uint16_t Xs[64 * 16], A1s[64 * 64], A2s[64 * 64], D3s[64 * 16];
-    __shared__ float b1s[64], b2s[64], b3s[8];
     __shared__ int64_t idx_s[64];
     __shared__ int valid_s[64];
+    uint16_t* W1s = S.W1s;
+    uint16_t* W2s = S.W2s;
+    uint16_t* W3s = S.W3s;
+    const float* b1s = S.bias;
+    const float* b2s = S.bias + 64;
+    const float* b3s = S.bias + 128;
     uint16_t* D2s = A2s;   /* dZ2 replaces A2 once dW3 is done */
     uint16_t* D1s = A1s;   /* dZ1 replaces A1 once dW2 is done */
     const int lane = threadIdx.x, g = lane >> 4, c16 = lane & 15;
     float* out = partial + (int64_t)blockIdx.x * PS;
 
-    for (int e = lane; e < 16 * 64; e += 64) {
-        int f = e >> 6, h = e & 63;
-        W1s[e] = f < PCT_NET_IN ? (uint16_t)pct_f2bf(params[PCT_NET_OFF_W1 + f * 64 + h]) : (uint16_t)0;
+    {
+        const net_u32x4* src = (const net_u32x4*)stage;
+        net_u32x4* dst = (net_u32x4*)&S;
+        constexpr int NV = (int)(sizeof(net_stage) / 16);
+#pragma unroll
+        for (int i = lane; i < NV; i += 64) dst[i] = src[i];
     }
-    for (int e = lane; e < 64 * 64; e += 64) W2s[e] = (uint16_t)pct_f2bf(params[PCT_NET_OFF_W2 + e]);
-    for (int e = lane; e < 64 * 16; e += 64) {
-        int h = e >> 4, o = e & 15;
-        W3s[e] = o < PCT_NET_OUT ? (uint16_t)pct_f2bf(params[PCT_NET_OFF_W3 + h * 8 + o]) : (uint16_t)0;
-    }
-    b1s[lane] = params[PCT_NET_OFF_B1 + lane];
-    b2s[lane] = params[PCT_NET_OFF_B2 + lane];
-    if (lane < 8) b3s[lane] = params[PCT_NET_OFF_B3 + lane];
     {
         const int64_t j = j0 + (int64_t)blockIdx.x * 64 + lane;
         const bool valid = j < j0 + nsamp && j < n;
@@ -247,17 +284,38 @@ k_net_train(const float* __restrict__ params, const float* __restrict__ X, const
     if (lane == 0) out[NP] = loss;
 }
 
-/* sum of the partial rows in row order; then either the gradient is written
- * (grad != null) or an Adam step is taken (params != null). The squared-error
- * sum goes to loss_out[0]. */
+/* Sum of the partial rows; then either the gradient is written (grad !=
+ * null) or an Adam step is taken (params != null) and the bfloat16 stage
+ * updated. A workgroup takes 32 parameters; its 8 row groups (rows q, q + 8,
+ * ...) are summed in row order and then combined in group order, so the
+ * result does not depend on scheduling. The squared-error sum goes to
+ * loss_out[0]. */
 __global__ void __launch_bounds__(256)
 k_net_reduce(const float* __restrict__ partial, int nrows, float* __restrict__ grad, float* __restrict__ params,
              float* __restrict__ m, float* __restrict__ v, float lr, float beta1, float beta2, float eps, float c1,
-             float c2, float* __restrict__ loss_out) {
-    const int p = blockIdx.x * blockDim.x + threadIdx.x;
-    if (p > NP) return;
+             float c2, float* __restrict__ loss_out, net_stage* __restrict__ stage) {
+    __shared__ float part[8][32];
+    const int pl = threadIdx.x & 31, q = threadIdx.x >> 5;
+    const int p = blockIdx.x * 32 + pl;
     float s = 0.0f;
-    for (int w = 0; w < nrows; ++w) s += partial[(int64_t)w * PS + p];
+    if (p <= NP) {
+        int w = q;
+        for (; w + 24 < nrows; w += 32) {
+            float a0 = partial[(int64_t)w * PS + p], a1 = partial[(int64_t)(w + 8) * PS + p];
+            float a2 = partial[(int64_t)(w + 16) * PS + p], a3 = partial[(int64_t)(w + 24) * PS + p];
+            s += a0;
+            s += a1;
+            s += a2;
+            s += a3;
+        }
+        for (; w < nrows; w += 8) s += partial[(int64_t)w * PS + p];
+    }
+    part[q][pl] = s;
+    __syncthreads();
+    if (q != 0 || p > NP) return;
+    s = part[0][pl];
+#pragma unroll
+    for (int k = 1; k < 8; ++k) s += part[k][pl];
     if (p == NP) {
         loss_out[0] = s;
         return;
@@ -268,7 +326,9 @@ k_net_reduce(const float* __restrict__ partial, int nrows, float* __restrict__ g
         float vv = beta2 * v[p] + (1.0f - beta2) * s * s;
         m[p] = mm;
         v[p] = vv;
-        params[p] = params[p] - lr * (mm * c1) / (__builtin_sqrtf(vv * c2) + eps);
+        float np = params[p] - lr * (mm * c1) / (__builtin_sqrtf(vv * c2) + eps);
+        params[p] = np;
+        stage_write(stage, p, np);
     }
 }
 
@@ -390,7 +450,7 @@ int pct_net_release(pct_ctx* ctx) {
     void* ptrs[] = {ctx->d_net, ctx->d_net_range, ctx->d_net_cells, ctx->d_net_counts, ctx->d_u1f};
     for (void* p : ptrs)
         if (p) (void)hipFree(p);
-    pct_buffer* bufs[] = {&ctx->net_partial, &ctx->net_state, &ctx->net_loss, &ctx->net_calib};
+    pct_buffer* bufs[] = {&ctx->net_partial, &ctx->net_state, &ctx->net_loss, &ctx->net_calib, &ctx->net_stage_buf};
     for (pct_buffer* b : bufs)
         if (b->ptr) (void)hipFree(b->ptr);
     ctx->d_net = nullptr;
@@ -409,14 +469,18 @@ extern "C" int pct_net_gradient(pct_ctx* ctx, const float* params, const float* 
     if (rows > (1 << 24)) return pct_fail(ctx, PCT_E_INVALID, "pct_net_gradient: n too large");
     int rc = pct_ensure(ctx, ctx->net_partial, sizeof(float) * rows * PS);
     if (!rc) rc = pct_ensure(ctx, ctx->net_loss, sizeof(float) * 8);
+    if (!rc) rc = pct_ensure(ctx, ctx->net_stage_buf, sizeof(net_stage));
     if (rc) return rc;
     float* partial = (float*)ctx->net_partial.ptr;
     float* dloss = (float*)ctx->net_loss.ptr;
-    hipLaunchKernelGGL(k_net_train, dim3((unsigned)rows), dim3(64), 0, ctx->stream, params, features, targets, n,
-                       (int64_t)0, n, (uint64_t)0, (uint64_t)0, (float)(2.0 / (8.0 * (double)n)), partial);
+    net_stage* stage = (net_stage*)ctx->net_stage_buf.ptr;
+    hipLaunchKernelGGL(k_net_stage, dim3(1), dim3(256), 0, ctx->stream, params, stage);
+    hipLaunchKernelGGL(k_net_train, dim3((unsigned)rows), dim3(64), 0, ctx->stream, (const net_stage*)stage, features,
+                       targets, n, (int64_t)0, n, (uint64_t)0, (uint64_t)0, (float)(2.0 / (8.0 * (double)n)), partial);
     PCT_HIP_CHECK(ctx, hipGetLastError());
-    hipLaunchKernelGGL(k_net_reduce, dim3(pct_grid(PS, 256)), dim3(256), 0, ctx->stream, partial, (int)rows, grad,
-                       (float*)nullptr, (float*)nullptr, (float*)nullptr, 0.0f, 0.0f, 0.0f, 0.0f, 0.0f, 0.0f, dloss);
+    hipLaunchKernelGGL(k_net_reduce, dim3(pct_grid(PS, 32)), dim3(256), 0, ctx->stream, partial, (int)rows, grad,
+                       (float*)nullptr, (float*)nullptr, (float*)nullptr, 0.0f, 0.0f, 0.0f, 0.0f, 0.0f, 0.0f, dloss,
+                       (net_stage*)nullptr);
     PCT_HIP_CHECK(ctx, hipGetLastError());
     if (loss) {
         float l = 0.0f;
@@ -437,14 +501,16 @@ extern "C" int pct_net_train(pct_ctx* ctx, float* params, const float* features,
     const int64_t steps = n / cfg->batch;
     const int64_t total = steps * cfg->epochs;
     int rc = pct_ensure(ctx, ctx->net_partial, sizeof(float) * (size_t)rows * PS);
-    if (!rc) rc = pct_ensure(ctx, ctx->net_state, sizeof(float) * 2 * NP);
+    if (!rc) rc = pct_ensure(ctx, ctx->net_state, sizeof(float) * 2 * NP + sizeof(net_stage));
     if (!rc) rc = pct_ensure(ctx, ctx->net_loss, sizeof(float) * (size_t)total);
     if (rc) return rc;
     float* partial = (float*)ctx->net_partial.ptr;
     float* m = (float*)ctx->net_state.ptr;
     float* v = m + NP;
+    net_stage* stage = (net_stage*)(v + NP);
     float* dloss = (float*)ctx->net_loss.ptr;
     PCT_HIP_CHECK(ctx, hipMemsetAsync(m, 0, sizeof(float) * 2 * NP, ctx->stream));
+    hipLaunchKernelGGL(k_net_stage, dim3(1), dim3(256), 0, ctx->stream, (const float*)params, stage);
     uint64_t seed = cfg->seed;
     int64_t t = 0;
     const float scale = (float)(2.0 / (8.0 * (double)cfg->batch));
@@ -456,11 +522,11 @@ extern "C" int pct_net_train(pct_ctx* ctx, float* params, const float* features,
             const double lr = cfg->lr_min + 0.5 * (cfg->lr - cfg->lr_min) * (1.0 + std::cos(M_PI * (double)t / (double)total));
             const double c1 = 1.0 / (1.0 - std::pow((double)cfg->beta1, (double)(t + 1)));
             const double c2 = 1.0 / (1.0 - std::pow((double)cfg->beta2, (double)(t + 1)));
-            hipLaunchKernelGGL(k_net_train, dim3(rows), dim3(64), 0, ctx->stream, (const float*)params, features, targets,
-                               n, s * cfg->batch, (int64_t)cfg->batch, pa, pb, scale, partial);
-            hipLaunchKernelGGL(k_net_reduce, dim3(pct_grid(PS, 256)), dim3(256), 0, ctx->stream, (const float*)partial,
+            hipLaunchKernelGGL(k_net_train, dim3(rows), dim3(64), 0, ctx->stream, (const net_stage*)stage, features,
+                               targets, n, s * cfg->batch, (int64_t)cfg->batch, pa, pb, scale, partial);
+            hipLaunchKernelGGL(k_net_reduce, dim3(pct_grid(PS, 32)), dim3(256), 0, ctx->stream, (const float*)partial,
                                rows, (float*)nullptr, params, m, v, (float)lr, cfg->beta1, cfg->beta2, cfg->eps,
-                               (float)c1, (float)c2, dloss + t);
+                               (float)c1, (float)c2, dloss + t, stage);
         }
         PCT_HIP_CHECK(ctx, hipGetLastError());
     }

@@ -209,3 +209,38 @@ def train_network(ctx, F, Y, epochs=12, batch=8192, lr=3e-3, seed=0, range_margi
     losses = train(ctx, params, F, Y, epochs=epochs, batch=batch, lr=lr, seed=seed + 1)
     lo, hi = input_range(F, range_margin)
     return PathNetwork(params, lo, hi, losses)
+
+
+def main(argv=None):
+    """Train a path network on exact-MLP paths of random-ellipsoid phantoms and
+    save it:  python -m pct_amd.learned --paths 10000000 --epochs 20 --save net.npz"""
+    import argparse
+    import time
+    import torch
+    from . import Context
+    ap = argparse.ArgumentParser(description=main.__doc__)
+    ap.add_argument("--paths", type=int, default=configs.LEARNED_TRAIN_PATHS)
+    ap.add_argument("--scans", type=int, default=configs.LEARNED_TRAIN_SCANS)
+    ap.add_argument("--epochs", type=int, default=20)
+    ap.add_argument("--batch", type=int, default=8192)
+    ap.add_argument("--lr", type=float, default=3e-3)
+    ap.add_argument("--seed", type=int, default=3)
+    ap.add_argument("--save", default=None)
+    a = ap.parse_args(argv)
+    per_scan = int(a.paths / a.scans * 1.25) + 1
+    t0 = time.time()
+    F, Y = training_set(a.paths, training_scans(a.scans, per_scan), log=print)
+    torch.cuda.synchronize()
+    t1 = time.time()
+    ctx = Context(configs.params("cfg2"))
+    net = train_network(ctx, F, Y, epochs=a.epochs, batch=a.batch, lr=a.lr, seed=a.seed)
+    torch.cuda.synchronize()
+    t2 = time.time()
+    print(f"training set {t1 - t0:.2f} s, training {t2 - t1:.2f} s, loss per epoch {net.epoch_loss}", flush=True)
+    if a.save:
+        net.save(a.save)
+    ctx.close()
+
+
+if __name__ == "__main__":
+    main()

@@ -193,7 +193,7 @@ def test_network_gains_over_hermite_path(cfg2, trained):
     pop = np.array(cal_net["cell_samples"])
     kept_net = pop[np.array(cal_net["cell_fallback"]) == 0].sum() / pop.sum()
     kept_h = pop[np.array(cal_h["cell_fallback"]) == 0].sum() / pop.sum()
-    print("share of histories in cells kept at 10 um: network", kept_net, "Hermite", kept_h)
+    print("share of histories in cells kept at 30 um: network", kept_net, "Hermite", kept_h)
     assert kept_net > 0.9 and kept_h < 0.2, (kept_net, kept_h)
 
 
