@@ -23,7 +23,7 @@
 
 int pct_mlp_paths_on(pct_ctx* ctx, hipStream_t stream, const pct_hist_soa* hist, int64_t start, int64_t count,
                      const int64_t* offsets, uint32_t* voxels, uint32_t* lengths, double* chord, uint32_t* hash_out,
-                     const uint32_t* hash_expect, int mode);
+                     const uint32_t* hash_expect, int mode, bool beside);
 int pct_status_to_rc(pct_ctx* ctx, int flags);
 
 namespace {
@@ -307,7 +307,7 @@ extern "C" int pct_batch_iteration(pct_batch* b, const pct_hist_soa* hists, floa
                 if (h1 <= h0) continue;
                 rc = pct_mlp_paths_on(cs, b->aux, &hists[s], h0, h1 - h0, (const int64_t*)pl->hist_off.ptr + h0,
                                       (uint32_t*)pl->list.ptr, (uint32_t*)pl->len.ptr + h0, (double*)pl->chord.ptr + h0,
-                                      nullptr, (const uint32_t*)pl->hash.ptr + h0, mlp);
+                                      nullptr, (const uint32_t*)pl->hash.ptr + h0, mlp, false);
                 if (rc) {
                     pct_fail(c0, rc, "scan %d: %s", s, pct_last_error(cs));
                     return rc;

@@ -1074,7 +1074,7 @@ int pct_mlp_capacity_impl(pct_ctx* ctx, const pct_hist_soa* hist, int64_t start,
  * history start) */
 int pct_mlp_paths_on(pct_ctx* ctx, hipStream_t stream, const pct_hist_soa* hist, int64_t start, int64_t count,
                      const int64_t* offsets, uint32_t* voxels, uint32_t* lengths, double* chord, uint32_t* hash_out,
-                     const uint32_t* hash_expect, int mode) {
+                     const uint32_t* hash_expect, int mode, bool beside) {
     int rc = ensure_mlp_table(ctx);
     if (rc) return rc;
     if (!pct_mlp_mode_ok(ctx, mode))
@@ -1083,9 +1083,15 @@ int pct_mlp_paths_on(pct_ctx* ctx, hipStream_t stream, const pct_hist_soa* hist,
     if (mode == PCT_MLP_TABLE && (rc = ensure_scat_table(ctx))) return rc;
     if (count == 0) return PCT_OK;
     if ((rc = mlp_occupancy(ctx))) return rc;
-    /* a workgroup of 256 is one wavefront on each of the CU's 4 SIMDs */
+    /* a workgroup of 256 is one wavefront on each of the CU's 4 SIMDs. Beside
+     * a single scan's block loop the cheaper walks leave room for the
+     * latency-bound block kernels (measured at 200x200x32: exact walk best at
+     * its occupancy, the precision form at 2, the surrogate at 1 wavefront per
+     * SIMD; the batched form's steps do better with the full occupancy) */
+    static const int beside_default[3] = {0, 2, 1};
     int per_cu = std::max(1, ctx->mlp_occ[mode]);
-    if (ctx->tune_mlp_waves > 0 && ctx->tune_mlp_waves < per_cu) per_cu = ctx->tune_mlp_waves;
+    int lim = ctx->tune_mlp_waves > 0 ? ctx->tune_mlp_waves : (beside ? beside_default[mode] : 0);
+    if (lim > 0 && lim < per_cu) per_cu = lim;
     const unsigned grid = std::min(pct_grid(count, 256), (unsigned)(ctx->cus * per_cu));
     if (mode == PCT_MLP_SURROGATE) {
         hipLaunchKernelGGL(k_mlp_paths_sur, dim3(grid), dim3(256), sizeof(float) * ctx->mlp_tab_k, stream, ctx->g, *hist,
@@ -1112,7 +1118,7 @@ int pct_mlp_paths_impl(pct_ctx* ctx, const pct_hist_soa* hist, int64_t start, in
                        uint32_t* voxels, uint32_t* lengths, double* chord, uint32_t* hash_out,
                        const uint32_t* hash_expect, int mode) {
     return pct_mlp_paths_on(ctx, ctx->stream, hist, start, count, offsets, voxels, lengths, chord, hash_out,
-                            hash_expect, mode);
+                            hash_expect, mode, false);
 }
 
 int pct_status_to_rc(pct_ctx* ctx, int flags) {

@@ -59,7 +59,7 @@ int pct_drop_iteration_atomic(pct_ctx* ctx, const pct_hist_soa* hist, float* x, 
                               int mode);
 int pct_mlp_paths_on(pct_ctx* ctx, hipStream_t stream, const pct_hist_soa* hist, int64_t start, int64_t count,
                      const int64_t* offsets, uint32_t* voxels, uint32_t* lengths, double* chord, uint32_t* hash_out,
-                     const uint32_t* hash_expect, int mode);
+                     const uint32_t* hash_expect, int mode, bool beside);
 
 
 
@@ -585,7 +585,7 @@ static int plan_iterate(pct_ctx* ctx, pct_plan* pl, const pct_hist_soa* hist, fl
         /* sequential: all lists first, then the block loop */
         rc = pct_mlp_paths_on(ctx, ctx->stream, hist, 0, pl->n, (const int64_t*)pl->hist_off.ptr,
                               (uint32_t*)pl->list.ptr, (uint32_t*)pl->len.ptr, (double*)pl->chord.ptr, nullptr,
-                              (const uint32_t*)pl->hash.ptr, mlp);
+                              (const uint32_t*)pl->hash.ptr, mlp, false);
         if (rc) return rc;
         recompute = false;
         if (ctx->profile) {
@@ -602,7 +602,7 @@ static int plan_iterate(pct_ctx* ctx, pct_plan* pl, const pct_hist_soa* hist, fl
             int64_t h1 = std::min(pl->n, pl->chunk_b[c + 1] * pl->B);
             rc = pct_mlp_paths_on(ctx, aux, hist, h0, h1 - h0, (const int64_t*)pl->hist_off.ptr + h0,
                                   (uint32_t*)pl->list.ptr, (uint32_t*)pl->len.ptr + h0, (double*)pl->chord.ptr + h0,
-                                  nullptr, (const uint32_t*)pl->hash.ptr + h0, mlp);
+                                  nullptr, (const uint32_t*)pl->hash.ptr + h0, mlp, true);
             if (rc) return rc;
             PCT_HIP_CHECK(ctx, hipEventRecord(pl->ev_chunk[c], aux));
             if (ctx->profile && c == 0) PCT_HIP_CHECK(ctx, hipEventRecord(ctx->ev[1], aux));
