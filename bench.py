@@ -326,6 +326,18 @@ def main():
     pre = pipeline.preprocess(ctx, raw)
     torch.cuda.synchronize()
     t_pre = time.time() - t_pre
+    # again, each stage alone between two events: time, algorithmic bytes and
+    # HBM GB/s per stage (the same results; this run's are the ones used)
+    del pre
+    torch.cuda.empty_cache()
+    timings = []
+    pre = pipeline.preprocess(ctx, raw, timings=timings)
+    stages = {}
+    for name, ms, nbytes in timings:
+        gbs = nbytes / (ms * 1e-3) / 1e9
+        stages[name] = {"ms": round(ms, 4), "bytes": int(nbytes), "gb_s": round(gbs, 1),
+                        "frac": round(gbs / HBM_PEAK_GBS, 4)}
+    log("preprocessing stages: " + ", ".join(f"{k} {v['ms']:.3f} ms {v['gb_s']:.0f} GB/s" for k, v in stages.items()))
     del raw
     hist = pre.hist
     n_r = hist.n
@@ -564,6 +576,9 @@ def main():
                       "note": "the MLP pass and the block loop overlap chunk by chunk; alone they take "
                               f"{mlp_alone_ms:.2f} and {blk_alone_ms:.2f} ms",
                       "plan_setup_once": round(t_plan * 1e3, 1), "preprocess_once": round(t_pre * 1e3, 1)},
+        "preprocess_stages": {"note": "each stage alone between two HIP events; bytes = arrays read and written "
+                                      "once at their stored width (pct_amd/pipeline.py STAGE_BYTES); frac of "
+                                      f"{HBM_PEAK_GBS:g} GB/s", **stages},
         "precision_form": precision,
         "batched_form": batched,
         "learned_form": learned_res,

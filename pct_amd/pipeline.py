@@ -30,33 +30,105 @@ class Preprocessed:
     counts: dict = field(default_factory=dict)
 
 
-def preprocess(ctx: Context, raw: RawScan, hull_kind: str = "msc") -> Preprocessed:
-    """Raw tracker data -> reconstruction histories, hull and x0."""
+# Algorithmic HBM bytes of each stage: every array the stage has to read or
+# write, counted once at its stored width. n = histories in, k = histories
+# kept by the stage, V = voxels, B = sinogram bins. History rows are 11 f32
+# + bin_num + first_voxel = 52 B; raw rows 14 x 4 B. Gathers of small tables
+# (hull bits, bin statistics) and voxel atomics that stay in L2 are not
+# counted, so a stage dominated by them shows a low fraction of HBM peak.
+STAGE_BYTES = {
+    # raw row in; 11 f32 + missed out
+    "recon_volume_intersections": lambda n, k, V, B: 56 * n + 45 * n,
+    # 6 f32 + missed in; bin_num, in_range, sort key and value out; one
+    # key/value pass of the sort (8 B in, 8 B out); sorted keys read for the
+    # bin ranges; first/last per bin written
+    "binning": lambda n, k, V, B: 25 * n + 13 * n + 16 * n + 4 * n + 8 * B,
+    # cylinder counts written, then missed, wepl and 6 f32 per history read
+    # and the counts read and written once
+    "msc_counts": lambda n, k, V, B: 4 * V + 29 * n + 8 * V,
+    "space_carve": lambda n, k, V, B: 29 * n + V / 4,
+    # sorted index, in_range, wepl and 4 angles per history; ranges in, 7
+    # statistics out per bin
+    "bin_statistics": lambda n, k, V, B: 25 * n + 36 * B,
+    # missed copied to failed, sorted index + wepl + 4 angles read, failed
+    # written; ranges and 6 statistics in, count and sinogram out per bin
+    "statistical_cuts": lambda n, k, V, B: 2 * n + 24 * n + n + 40 * B,
+    # keep flag per history; each kept row read and written
+    "compact_post_cut": lambda n, k, V, B: n + 104 * k,
+    "fbp_filter": lambda n, k, V, B: 8 * B,
+    "backproject": lambda n, k, V, B: 4 * B + 4 * V,
+    "msc_edge": lambda n, k, V, B: 8 * V,
+    "hull_select": lambda n, k, V, B: 4 * V + V / 8,
+    "fbp_threshold": lambda n, k, V, B: 4 * V + V / 8,
+    "initial_iterate": lambda n, k, V, B: 8 * V + V / 8,
+    # 10 f32 in; 6 f32, first_voxel and the intersected flag out
+    "mlp_endpoints": lambda n, k, V, B: 40 * n + 29 * n,
+    "compact_reconstruction": lambda n, k, V, B: n + 104 * k,
+}
+
+
+class _StageTimer:
+    """HIP events on the context's stream around each stage, when the caller
+    asks for timings: appends (stage, ms, algorithmic bytes) per stage.
+
+    A spin kernel of about a millisecond runs on the stream ahead of the
+    first event, so the host has queued the stage's launches by the time the
+    event is stamped and the interval holds the stage's device work, not the
+    host's launch latency (a stage that reads a count back to the host still
+    pays for that round trip)."""
+
+    GATE_CYCLES = 3_000_000
+
+    def __init__(self, ctx, out):
+        self.ctx, self.out = ctx, out
+
+    def __call__(self, name, n, fn, *args, kept=None):
+        if self.out is None:
+            return fn(*args)
+        torch = self.ctx.torch
+        e0, e1 = torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)
+        self.ctx.sync()
+        with torch.cuda.stream(self.ctx.stream):
+            torch.cuda._sleep(self.GATE_CYCLES)
+        e0.record(self.ctx.stream)
+        r = fn(*args)
+        e1.record(self.ctx.stream)
+        torch.cuda.synchronize()
+        k = kept(r) if kept is not None else 0
+        self.out.append((name, e0.elapsed_time(e1), float(STAGE_BYTES[name](n, k, self.ctx.V, self.ctx.n_bins))))
+        return r
+
+
+def preprocess(ctx: Context, raw: RawScan, hull_kind: str = "msc", timings: list = None) -> Preprocessed:
+    """Raw tracker data -> reconstruction histories, hull and x0. With a list
+    as `timings`, each stage runs alone between two events and (stage, ms,
+    algorithmic bytes) is appended per stage."""
     torch = ctx.torch
     n_raw = raw.n
-    hist, missed = ctx.recon_volume_intersections(raw)
-    in_range = ctx.binning(hist, missed)
+    T = _StageTimer(ctx, timings)
+    hist, missed = T("recon_volume_intersections", n_raw, ctx.recon_volume_intersections, raw)
+    in_range = T("binning", n_raw, ctx.binning, hist, missed)
     if hull_kind == "msc":
-        counts = ctx.msc_counts(hist, missed)
+        counts = T("msc_counts", n_raw, ctx.msc_counts, hist, missed)
     elif hull_kind == "sc":
-        sc_bits = ctx.space_carve(hist, missed, ctx.hull_init())
-    st = ctx.bin_statistics(hist, missed, in_range)
-    failed, sinogram = ctx.statistical_cuts(hist, missed, st)
-    post_cut = ctx.compact(hist, (failed == 0).to(torch.uint8))
-    fbp = ctx.backproject(ctx.fbp_filter(sinogram))
+        sc_bits = T("space_carve", n_raw, ctx.space_carve, hist, missed, ctx.hull_init())
+    st = T("bin_statistics", n_raw, ctx.bin_statistics, hist, missed, in_range)
+    failed, sinogram = T("statistical_cuts", n_raw, ctx.statistical_cuts, hist, missed, st)
+    post_cut = T("compact_post_cut", n_raw, ctx.compact, hist, (failed == 0).to(torch.uint8), kept=lambda h: h.n)
+    fbp = T("backproject", 0, ctx.backproject, T("fbp_filter", 0, ctx.fbp_filter, sinogram))
     if hull_kind == "msc":
-        hull = ctx.hull_select(ctx.msc_edge(counts))
+        hull = T("hull_select", 0, ctx.hull_select, T("msc_edge", 0, ctx.msc_edge, counts))
     elif hull_kind == "sc":
         hull = sc_bits
     elif hull_kind == "fbp":
-        hull = ctx.fbp_threshold(fbp)
+        hull = T("fbp_threshold", 0, ctx.fbp_threshold, fbp)
     elif hull_kind == "cylinder":
         hull = ctx.hull_init()
     else:
         raise ValueError(f"unknown hull kind {hull_kind}")
-    x0 = ctx.initial_iterate(fbp, hull)
-    inter = ctx.mlp_endpoints(post_cut, hull)
-    recon = ctx.compact(post_cut, inter)
+    x0 = T("initial_iterate", 0, ctx.initial_iterate, fbp, hull)
+    inter = T("mlp_endpoints", post_cut.n, ctx.mlp_endpoints, post_cut, hull)
+    recon = T("compact_reconstruction", post_cut.n, ctx.compact, post_cut, inter, kept=lambda h: h.n)
     ctx.sync()
     return Preprocessed(hist=recon, x0=x0, hull=hull, fbp=fbp, sinogram=sinogram,
                         counts={"histories": n_raw, "post_cut": post_cut.n, "reconstruction": recon.n})

@@ -128,3 +128,29 @@ def test_full_pipeline_matches_oracle():
     assert d.max() <= 1e-5
     assert np.mean(d == 0) > 0.99
     ctx.close()
+
+
+def test_timed_pipeline_gives_the_same_results():
+    """the per-stage timing path of pipeline.preprocess runs the same stages:
+    results bit-identical to the untimed run, one positive time and byte
+    count per stage"""
+    import pct_amd
+    from pct_amd import configs, pipeline
+    p = configs.params("cfg1")
+    ctx = pct_amd.Context(p)
+    raw = ctx.generate(configs.synth_cfg("cfg1", n_histories=200_000))
+    a = pipeline.preprocess(ctx, raw)
+    timings = []
+    b = pipeline.preprocess(ctx, raw, timings=timings)
+    names = [t[0] for t in timings]
+    assert names == ["recon_volume_intersections", "binning", "msc_counts", "bin_statistics", "statistical_cuts",
+                     "compact_post_cut", "fbp_filter", "backproject", "msc_edge", "hull_select", "initial_iterate",
+                     "mlp_endpoints", "compact_reconstruction"]
+    assert all(ms > 0 and nbytes > 0 for _, ms, nbytes in timings)
+    assert a.hist.n == b.hist.n and a.counts == b.counts
+    assert np.array_equal(a.x0.cpu().numpy().view(np.uint32), b.x0.cpu().numpy().view(np.uint32))
+    assert np.array_equal(a.hull.cpu().numpy(), b.hull.cpu().numpy())
+    ga, gb = a.hist.numpy(), b.hist.numpy()
+    for f in ga:
+        assert np.array_equal(np.asarray(ga[f]).view(np.uint32), np.asarray(gb[f]).view(np.uint32)), f
+    ctx.close()
