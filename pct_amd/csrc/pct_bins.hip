@@ -6,11 +6,18 @@
  * (.cu:2718), statistical_cuts_GPU (.cu:2835) and construct_sinogram_GPU
  * (.cu:2912). The reference forms every per-bin sum with float atomics, so
  * its statistics -- and therefore which histories the cuts keep -- depend on
- * thread scheduling. Here the sums are formed in storage order: the histories
- * are stably sorted by bin once (rocPRIM radix sort on the bin number), and one
- * thread per bin walks its members in storage order, accumulating in float as
- * the reference's arrays do. The result is deterministic and bit-identical to a
- * sequential pass; the decision paths (means, deviations, cut masks) are exact.
+ * thread scheduling. Here the sums are formed in storage order: the binning
+ * kernel writes each history's bin key and a 16-byte record of what the
+ * statistics read (WEPL, the two exit-minus-entry angle differences as the
+ * float subtractions give them, the in-range flag), the records are stably
+ * sorted by bin once (rocPRIM radix sort), and one thread per bin walks its
+ * bin's contiguous records in storage order, accumulating in float as the
+ * reference's arrays do. The cut flags are then evaluated per history in
+ * storage order against its bin's statistics (coalesced, no scatter). The
+ * result is deterministic and bit-identical to a sequential pass; the
+ * decision paths (means, deviations, cut masks) are exact. The records are a
+ * snapshot taken by pct_binning: the statistics and cuts that follow use the
+ * WEPL and angles the histories had then.
  *
  * Reference quirks kept: an out-of-range history keeps bin 0 and is kept
  * (it enters bin 0's squared deviations and cuts, not its mean); a bin with
@@ -34,11 +41,19 @@ __device__ __forceinline__ double wrap_pi(double a) {
     return a;
 }
 
+/* what the statistics and the cuts read of a history, sorted by bin */
+struct __align__(16) bin_rec {
+    float wepl;
+    float d_ut;          /* xy_exit_angle - xy_entry_angle (float subtraction) */
+    float d_uv;          /* xz_exit_angle - xz_entry_angle                     */
+    uint32_t in_range;
+};
+
 /* bin number of every history; sort key = bin for kept (not missed)
  * histories, `bins` (sorts last) for missed ones */
 __global__ void __launch_bounds__(256)
 k_binning(bin_geo b, pct_hist_soa h, const uint8_t* __restrict__ missed, uint8_t* __restrict__ in_range,
-          uint32_t* __restrict__ keys, uint32_t* __restrict__ vals) {
+          uint32_t* __restrict__ keys, bin_rec* __restrict__ recs) {
     int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
     if (i >= h.n) return;
     const double RADIANS_TO_ANGLE = 180.0 / PCT_PI, ANGLE_TO_RADIANS = PCT_PI / 180.0;
@@ -61,7 +76,12 @@ k_binning(bin_geo b, pct_hist_soa h, const uint8_t* __restrict__ missed, uint8_t
     h.bin_num[i] = bin;
     in_range[i] = ok;
     keys[i] = missed[i] ? (uint32_t)b.bins : (uint32_t)bin;
-    vals[i] = (uint32_t)i;
+    bin_rec r;
+    r.wepl = h.wepl[i];
+    r.d_ut = h.xy_exit_angle[i] - h.xy_entry_angle[i];
+    r.d_uv = h.xz_exit_angle[i] - h.xz_entry_angle[i];
+    r.in_range = ok;
+    recs[i] = r;
 }
 
 /* [first, last) member range of every bin in the sorted order */
@@ -83,20 +103,19 @@ __global__ void k_fill_u32(uint32_t* __restrict__ a, int64_t n, uint32_t v) {
 /* A2 sums + A3 means + A4 squared deviations + A5 deviations, one thread per bin */
 __global__ void __launch_bounds__(256)
 k_bin_stats(int64_t bins, const uint32_t* __restrict__ first, const uint32_t* __restrict__ last,
-            const uint32_t* __restrict__ order, pct_hist_soa h, const uint8_t* __restrict__ in_range,
-            pct_bin_stats st) {
+            const bin_rec* __restrict__ recs, pct_bin_stats st) {
     int64_t bin = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
     if (bin >= bins) return;
     uint32_t a = first[bin], e = last[bin];
     int32_t count = 0;
     float sw = 0.0f, su = 0.0f, sv = 0.0f;
     for (uint32_t k = a; k < e; ++k) {
-        uint32_t i = order[k];
-        if (!in_range[i]) continue;
-        double rel_ut = wrap_pi(h.xy_exit_angle[i] - h.xy_entry_angle[i]);
-        double rel_uv = wrap_pi(h.xz_exit_angle[i] - h.xz_entry_angle[i]);
+        bin_rec r = recs[k];
+        if (!r.in_range) continue;
+        double rel_ut = wrap_pi(r.d_ut);
+        double rel_uv = wrap_pi(r.d_uv);
         count += 1;
-        sw += h.wepl[i];
+        sw += r.wepl;
         su += (float)rel_ut;
         sv += (float)rel_uv;
     }
@@ -107,10 +126,10 @@ k_bin_stats(int64_t bins, const uint32_t* __restrict__ first, const uint32_t* __
     }
     float qw = 0.0f, qu = 0.0f, qv = 0.0f;
     for (uint32_t k = a; k < e; ++k) {
-        uint32_t i = order[k];
-        double rel_ut = wrap_pi(h.xy_exit_angle[i] - h.xy_entry_angle[i]);
-        double rel_uv = wrap_pi(h.xz_exit_angle[i] - h.xz_entry_angle[i]);
-        double dw = h.wepl[i] - sw;
+        bin_rec r = recs[k];
+        double rel_ut = wrap_pi(r.d_ut);
+        double rel_uv = wrap_pi(r.d_uv);
+        double dw = r.wepl - sw;
         double du = rel_ut - su;
         double dv = rel_uv - sv;
         qw += (float)(dw * dw);
@@ -127,16 +146,39 @@ k_bin_stats(int64_t bins, const uint32_t* __restrict__ first, const uint32_t* __
     st.stddev_wepl[bin] = qw; st.stddev_rel_ut[bin] = qu; st.stddev_rel_uv[bin] = qv;
 }
 
-__global__ void k_copy_u8(const uint8_t* __restrict__ a, uint8_t* __restrict__ b, int64_t n) {
-    int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
-    if (i < n) b[i] = a[i];
+/* A6: does a history fail its bin's 3-sigma cuts (strict angle cuts,
+ * inclusive WEPL cut)? */
+__device__ __forceinline__ bool cut_fails(float w, float d_ut, float d_uv, float mw, float mu, float mv, float cw,
+                                          float cu, float cv) {
+    double rel_ut = wrap_pi(d_ut);
+    double rel_uv = wrap_pi(d_uv);
+    bool pu = pct_fabs(rel_ut - mu) < (double)cu;
+    bool pv = pct_fabs(rel_uv - mv) < (double)cv;
+    bool pw = pct_fabsf(mw - w) <= cw;
+    return !pu || !pv || !pw;
 }
 
-/* A6 + A7, one thread per bin */
+/* A6 per history in storage order: failed = missed, or outside its bin's
+ * cuts (an out-of-range history is judged in bin 0, as it is keyed) */
 __global__ void __launch_bounds__(256)
-k_bin_cuts(int64_t bins, const uint32_t* __restrict__ first, const uint32_t* __restrict__ last,
-           const uint32_t* __restrict__ order, pct_hist_soa h, pct_bin_stats st, float sig,
-           uint8_t* __restrict__ failed, float* __restrict__ sinogram) {
+k_cut_flags(pct_hist_soa h, const uint8_t* __restrict__ missed, pct_bin_stats st, float sig,
+            uint8_t* __restrict__ failed) {
+    int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= h.n) return;
+    bool f = true;
+    if (!missed[i]) {
+        int32_t bin = h.bin_num[i];
+        f = cut_fails(h.wepl[i], h.xy_exit_angle[i] - h.xy_entry_angle[i], h.xz_exit_angle[i] - h.xz_entry_angle[i],
+                      st.mean_wepl[bin], st.mean_rel_ut[bin], st.mean_rel_uv[bin], sig * st.stddev_wepl[bin],
+                      sig * st.stddev_rel_ut[bin], sig * st.stddev_rel_uv[bin]);
+    }
+    failed[i] = f;
+}
+
+/* A7 + the survivors' count, one thread per bin over its sorted records */
+__global__ void __launch_bounds__(256)
+k_bin_sinogram(int64_t bins, const uint32_t* __restrict__ first, const uint32_t* __restrict__ last,
+               const bin_rec* __restrict__ recs, pct_bin_stats st, float sig, float* __restrict__ sinogram) {
     int64_t bin = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
     if (bin >= bins) return;
     uint32_t a = first[bin], e = last[bin];
@@ -145,18 +187,10 @@ k_bin_cuts(int64_t bins, const uint32_t* __restrict__ first, const uint32_t* __r
     int32_t count = 0;
     float sum = 0.0f;
     for (uint32_t k = a; k < e; ++k) {
-        uint32_t i = order[k];
-        double rel_ut = wrap_pi(h.xy_exit_angle[i] - h.xy_entry_angle[i]);
-        double rel_uv = wrap_pi(h.xz_exit_angle[i] - h.xz_entry_angle[i]);
-        float w = h.wepl[i];
-        bool pu = pct_fabs(rel_ut - mu) < (double)cu;
-        bool pv = pct_fabs(rel_uv - mv) < (double)cv;
-        bool pw = pct_fabsf(mw - w) <= cw;
-        bool f = !pu || !pv || !pw;
-        failed[i] = f;
-        if (!f) {
+        bin_rec r = recs[k];
+        if (!cut_fails(r.wepl, r.d_ut, r.d_uv, mw, mu, mv, cw, cu, cv)) {
             count += 1;
-            sum += w;
+            sum += r.wepl;
         }
     }
     if (count > 0) sum /= count;
@@ -170,7 +204,7 @@ k_bin_cuts(int64_t bins, const uint32_t* __restrict__ first, const uint32_t* __r
 struct pct_bin_order {
     pct_hist_soa key{};
     int64_t n = 0;
-    pct_buffer keys_in, keys_out, vals_in, vals_out, first, last, tmp;
+    pct_buffer keys_in, keys_out, recs_in, recs_out, first, last, tmp;
 };
 
 static bin_geo make_bin_geo(const pct_params& p) {
@@ -202,12 +236,12 @@ static pct_bin_order* order_of(pct_ctx* ctx) {
 
 void pct_bin_order_destroy(pct_bin_order* o) {
     if (!o) return;
-    for (pct_buffer* b : {&o->keys_in, &o->keys_out, &o->vals_in, &o->vals_out, &o->first, &o->last, &o->tmp})
+    for (pct_buffer* b : {&o->keys_in, &o->keys_out, &o->recs_in, &o->recs_out, &o->first, &o->last, &o->tmp})
         if (b->ptr) (void)hipFree(b->ptr);
     delete o;
 }
 
-/* binning + stable sort by bin */
+/* binning + stable sort of the statistics records by bin */
 static int bin_and_sort(pct_ctx* ctx, const pct_hist_soa* h, const uint8_t* missed, uint8_t* in_range) {
     bin_geo b = make_bin_geo(ctx->p);
     pct_bin_order* o = order_of(ctx);
@@ -215,21 +249,21 @@ static int bin_and_sort(pct_ctx* ctx, const pct_hist_soa* h, const uint8_t* miss
     int rc;
     size_t nn = (size_t)(n > 0 ? n : 1);
     if ((rc = pct_ensure(ctx, o->keys_in, 4 * nn)) || (rc = pct_ensure(ctx, o->keys_out, 4 * nn)) ||
-        (rc = pct_ensure(ctx, o->vals_in, 4 * nn)) || (rc = pct_ensure(ctx, o->vals_out, 4 * nn)) ||
+        (rc = pct_ensure(ctx, o->recs_in, sizeof(bin_rec) * nn)) || (rc = pct_ensure(ctx, o->recs_out, sizeof(bin_rec) * nn)) ||
         (rc = pct_ensure(ctx, o->first, 4 * (size_t)b.bins)) || (rc = pct_ensure(ctx, o->last, 4 * (size_t)b.bins)))
         return rc;
     if (n > 0)
         hipLaunchKernelGGL(k_binning, dim3(pct_grid(n, 256)), dim3(256), 0, ctx->stream, b, *h, missed, in_range,
-                           (uint32_t*)o->keys_in.ptr, (uint32_t*)o->vals_in.ptr);
+                           (uint32_t*)o->keys_in.ptr, (bin_rec*)o->recs_in.ptr);
     size_t tb = 0;
     int endbit = bins_bits(b.bins);
-    PCT_HIP_CHECK(ctx, rocprim::radix_sort_pairs(nullptr, tb, (uint32_t*)nullptr, (uint32_t*)nullptr, (uint32_t*)nullptr,
-                                                 (uint32_t*)nullptr, nn, 0, endbit, ctx->stream));
+    PCT_HIP_CHECK(ctx, rocprim::radix_sort_pairs(nullptr, tb, (uint32_t*)nullptr, (uint32_t*)nullptr, (bin_rec*)nullptr,
+                                                 (bin_rec*)nullptr, nn, 0, endbit, ctx->stream));
     if ((rc = pct_ensure(ctx, o->tmp, tb))) return rc;
     tb = o->tmp.bytes;
     if (n > 0)
         PCT_HIP_CHECK(ctx, rocprim::radix_sort_pairs(o->tmp.ptr, tb, (uint32_t*)o->keys_in.ptr, (uint32_t*)o->keys_out.ptr,
-                                                     (uint32_t*)o->vals_in.ptr, (uint32_t*)o->vals_out.ptr, (size_t)n, 0,
+                                                     (bin_rec*)o->recs_in.ptr, (bin_rec*)o->recs_out.ptr, (size_t)n, 0,
                                                      endbit, ctx->stream));
     hipLaunchKernelGGL(k_fill_u32, dim3(pct_grid(b.bins, 256)), dim3(256), 0, ctx->stream, (uint32_t*)o->first.ptr,
                        b.bins, 0u);
@@ -267,8 +301,8 @@ extern "C" int pct_bin_statistics(pct_ctx* ctx, const pct_hist_soa* hist, const 
         return pct_fail(ctx, PCT_E_INVALID, "pct_bin_statistics: call pct_binning on these histories first");
     bin_geo b = make_bin_geo(ctx->p);
     hipLaunchKernelGGL(k_bin_stats, dim3(pct_grid(b.bins, 256)), dim3(256), 0, ctx->stream, b.bins,
-                       (const uint32_t*)o->first.ptr, (const uint32_t*)o->last.ptr, (const uint32_t*)o->vals_out.ptr,
-                       *hist, in_range, *stats);
+                       (const uint32_t*)o->first.ptr, (const uint32_t*)o->last.ptr, (const bin_rec*)o->recs_out.ptr,
+                       *stats);
     PCT_HIP_CHECK(ctx, hipGetLastError());
     return PCT_OK;
 }
@@ -282,11 +316,14 @@ extern "C" int pct_statistical_cuts(pct_ctx* ctx, const pct_hist_soa* hist, cons
     if (!same_arrays(o->key, *hist))
         return pct_fail(ctx, PCT_E_INVALID, "pct_statistical_cuts: call pct_binning on these histories first");
     bin_geo b = make_bin_geo(ctx->p);
+    const float sig = (float)ctx->p.sigmas_to_keep;
     if (hist->n > 0)
-        hipLaunchKernelGGL(k_copy_u8, dim3(pct_grid(hist->n, 256)), dim3(256), 0, ctx->stream, missed, failed, hist->n);
-    hipLaunchKernelGGL(k_bin_cuts, dim3(pct_grid(b.bins, 256)), dim3(256), 0, ctx->stream, b.bins,
-                       (const uint32_t*)o->first.ptr, (const uint32_t*)o->last.ptr, (const uint32_t*)o->vals_out.ptr,
-                       *hist, *stats, (float)ctx->p.sigmas_to_keep, failed, sinogram);
+        hipLaunchKernelGGL(k_cut_flags, dim3(pct_grid(hist->n, 256)), dim3(256), 0, ctx->stream, *hist, missed, *stats,
+                           sig, failed);
+    /* after the flags: this kernel overwrites the counts with the survivors' */
+    hipLaunchKernelGGL(k_bin_sinogram, dim3(pct_grid(b.bins, 256)), dim3(256), 0, ctx->stream, b.bins,
+                       (const uint32_t*)o->first.ptr, (const uint32_t*)o->last.ptr, (const bin_rec*)o->recs_out.ptr,
+                       *stats, sig, sinogram);
     PCT_HIP_CHECK(ctx, hipGetLastError());
     return PCT_OK;
 }

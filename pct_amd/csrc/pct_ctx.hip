@@ -62,6 +62,9 @@ pct_geo pct_make_geo(const pct_params& p) {
     g.voxel_width = diameter / p.columns;
     g.voxel_height = diameter / p.rows;
     g.voxel_thickness = (p.slices * p.slice_thickness) / p.slices;
+    g.inv_width = 1.0 / g.voxel_width;
+    g.inv_height = 1.0 / g.voxel_height;
+    g.inv_thickness = 1.0 / g.voxel_thickness;
     g.x_zero = -p.recon_cyl_radius;
     g.y_zero = p.recon_cyl_radius;
     g.z_zero = p.recon_cyl_height / 2;

@@ -19,6 +19,7 @@ struct pct_geo {
     int C, R, S;
     int64_t V;
     double voxel_width, voxel_height, voxel_thickness;   /* data.h:268-270 */
+    double inv_width, inv_height, inv_thickness;         /* 1 / size, rounded once */
     double x_zero, y_zero, z_zero;                       /* data.h:272-274 */
     double mlp_u_step;                                   /* data.h:300     */
     double radius, height;

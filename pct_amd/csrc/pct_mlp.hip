@@ -35,6 +35,19 @@ __device__ __forceinline__ int calc_voxel(double zero, double pos, double size) 
     double q = pct_fabs(pos - zero) / size;
     return q < 1.0e9 ? (int)q : -1;
 }
+/* calc_voxel with the division replaced by a product with the rounded
+ * reciprocal. q = |pos - zero| * inv carries two roundings (of inv and of the
+ * product), the division's quotient one, so the two differ by less than
+ * 2^-51 q and have the same integer part unless an integer lies that close to
+ * q: then `near` is set and the caller evaluates calc_voxel itself (also for
+ * q = 0). Branch-free, so that two steps in flight stay one straight-line
+ * block. */
+__device__ __forceinline__ int calc_voxel_r(double zero, double pos, double inv, bool& near) {
+    double a = pct_fabs(pos - zero);
+    double q = a * inv;
+    near = near || (pct_fabs(q - __builtin_rint(q)) <= q * 0x1p-49);
+    return q < 1.0e9 ? (int)q : -1;
+}
 __device__ __forceinline__ double edge_coord(double zero, int voxel_entered, double size, int inc, int step) {
     int on_edge = (step == inc) ? voxel_entered : voxel_entered + 1;
     return zero + (inc * on_edge * size);
@@ -238,7 +251,6 @@ __device__ __forceinline__ void mlp_setup(const pct_geo& g, const pct_hist_soa& 
 __device__ __forceinline__ void mlp_tv_terms(const pct_geo& g, const mlp_hist& m, double u_1, double S1I0, double S1I1,
                                              double S1I3, double cst1, double cst2, double cst3, double& t_out,
                                              double& v_out) {
-    double S1I2 = S1I1;
     double d = m.u_2 - u_1;                       /* R_1[1] = R_1T[2] */
     double l2 = g.e_0 * (1 + 0.038 * pct_log((m.u_2 - u_1) / g.x_0));
     double c2 = l2 * l2 / g.x_0;
@@ -247,35 +259,33 @@ __device__ __forceinline__ void mlp_tv_terms(const pct_geo& g, const mlp_hist& m
     double sigma_theta2 = c2 * (m.pre3 - cst1);
     double det2 = sigma_t2 * sigma_theta2 - sigma_t2_theta2 * sigma_t2_theta2;
     double S2I0 = sigma_theta2 / det2;
-    double S2I1 = -sigma_t2_theta2 / det2;
-    double S2I2 = S2I1;
+    double S2I1 = -sigma_t2_theta2 / det2;          /* = Sigma_2^-1[2] */
     double S2I3 = sigma_t2 / det2;
-    /* R_1 = {1, d, 0, 1}, R_1T = {1, 0, d, 1} */
-    double f13_1 = S2I0 * 1.0 + S2I1 * 0.0;
-    double f13_2 = S2I2 * 1.0 + S2I3 * 0.0;
-    double f24_1 = S2I0 * d + S2I1 * 1.0;
-    double f24_2 = S2I2 * d + S2I3 * 1.0;
-    double F0 = S1I0 + 1.0 * f13_1 + 0.0 * f13_2;
-    double F1 = S1I1 + 1.0 * f24_1 + 0.0 * f24_2;
-    double F2 = S1I2 + d * f13_1 + 1.0 * f13_2;
-    double F3 = S1I3 + d * f24_1 + 1.0 * f24_2;
+    /* The reference forms these products with R_1 = {1, d, 0, 1}, R_1T =
+     * {1, 0, d, 1}, R_0 = {1, u_1, 0, 1} and T_0 = {t_in, 0}. A product with
+     * an exact 1 is the operand itself, and a product with an exact 0 adds a
+     * signed zero, which leaves every nonzero sum as it is; both are left out
+     * and the remaining terms are summed in the reference's order, so t_1 and
+     * v_1 are the reference's values (up to the sign of an exactly zero
+     * result). */
+    double f24_1 = S2I0 * d + S2I1;
+    double f24_2 = S2I1 * d + S2I3;
+    double F0 = S1I0 + S2I0;
+    double F1 = S1I1 + f24_1;
+    double F2 = (S1I1 + d * S2I0) + S2I1;
+    double F3 = (S1I3 + d * f24_1) + f24_2;
     double detF = F0 * F3 - F1 * F2;
     double I0 = F3 / detF;
     double I1 = -F1 / detF;
-    /* R_0 = {1, u_1, 0, 1}, T_0 = {t_in, 0} */
-    double a1 = 1.0 * m.t_in + u_1 * 0.0;
-    double a2 = 0.0 * m.t_in + 1.0 * 0.0;
     double a3 = S2I0 * m.t_out + S2I1 * m.dT;
-    double a4 = S2I2 * m.t_out + S2I3 * m.dT;
-    double s0 = S1I0 * a1 + S1I1 * a2 + 1.0 * a3 + 0.0 * a4;
-    double s1 = S1I2 * a1 + S1I3 * a2 + d * a3 + 1.0 * a4;
+    double a4 = S2I1 * m.t_out + S2I3 * m.dT;
+    double s0 = S1I0 * m.t_in + a3;
+    double s1 = (S1I1 * m.t_in + d * a3) + a4;
     double t_1 = I0 * s0 + I1 * s1;
-    double b1 = 1.0 * m.v_in + u_1 * 0.0;
-    double b2 = 0.0 * m.v_in + 1.0 * 0.0;
     double b3 = S2I0 * m.v_out + S2I1 * m.dV;
-    double b4 = S2I2 * m.v_out + S2I3 * m.dV;
-    double r0 = S1I0 * b1 + S1I1 * b2 + 1.0 * b3 + 0.0 * b4;
-    double r1 = S1I2 * b1 + S1I3 * b2 + d * b3 + 1.0 * b4;
+    double b4 = S2I1 * m.v_out + S2I3 * m.dV;
+    double r0 = S1I0 * m.v_in + b3;
+    double r1 = (S1I1 * m.v_in + d * b3) + b4;
     double v_1 = I0 * r0 + I1 * r1;
     t_out = t_1;
     v_out = v_1;
@@ -320,6 +330,24 @@ __device__ __forceinline__ int64_t mlp_step(const pct_geo& g, const mlp_hist& m,
     return voxel;
 }
 
+/* mlp_step with the voxel index from the reciprocal products; returns -2
+ * where one of them is too close to a voxel face to decide, and mlp_step
+ * then gives the step's voxel */
+__device__ __forceinline__ int64_t mlp_step_r(const pct_geo& g, const mlp_hist& m, const double* __restrict__ tab, int K,
+                                              int k) {
+    double u_1, t_1, v_1;
+    mlp_tv(g, m, tab, K, k, u_1, t_1, v_1);
+    double x_1 = (m.ca * (m.u_in + u_1)) - (m.sa * t_1);
+    double y_1 = (m.sa * (m.u_in + u_1)) + (m.ca * t_1);
+    bool near = false;
+    int vx = calc_voxel_r(g.x_zero, x_1, g.inv_width, near);
+    int vy = calc_voxel_r(g.y_zero, y_1, g.inv_height, near);
+    int vz = calc_voxel_r(g.z_zero, v_1, g.inv_thickness, near);
+    int64_t voxel = (int64_t)vx + (int64_t)vy * g.C + (int64_t)vz * g.C * g.R;
+    if (vx < 0 || vy < 0 || vz < 0 || voxel >= g.V) voxel = -1;
+    return near ? -2 : voxel;
+}
+
 /* position-aware mix of one list element; a list's hash is the sum over its
  * elements plus a length term, so it can be formed by a wave reduction */
 __device__ __forceinline__ uint32_t pct_list_mix(uint32_t v, uint32_t pos) {
@@ -346,9 +374,10 @@ __device__ __forceinline__ int64_t bcast_l(int64_t v, int lane) {
  * dedup of .cu:5478-5485: a ballot and a prefix popcount across the wave), the
  * list is written from voxels[hoff] on (the first voxel `carry` included)
  * while it fits hcap, and its position-aware hash is accumulated. Returns the
- * list length n (uniform) and this lane's part of the hash. */
-template <class StepF>
-__device__ __forceinline__ void walk_list(StepF step, int S, uint32_t carry, int64_t hoff, int64_t hcap,
+ * list length n (uniform) and this lane's part of the hash. A step that
+ * returns -2 is evaluated again by fix(k). */
+template <class StepF, class FixF>
+__device__ __forceinline__ void walk_list(StepF step, FixF fix, int S, uint32_t carry, int64_t hoff, int64_t hcap,
                                           uint32_t* __restrict__ voxels, int lane, uint64_t lt_mask, int64_t& n_out,
                                           uint32_t& hsh_out) {
     if (lane == 0 && hcap > 0) voxels[hoff] = carry;
@@ -357,7 +386,10 @@ __device__ __forceinline__ void walk_list(StepF step, int S, uint32_t carry, int
     for (int c = 0; c < S; c += 64) {
         int k = c + lane;
         int64_t vox = -1;
-        if (k < S) vox = step(k);
+        if (k < S) {
+            vox = step(k);
+            if (vox == -2) vox = fix(k);
+        }
         bool valid = vox >= 0;
         uint64_t vmask = __ballot(valid);
         uint64_t below = vmask & lt_mask;
@@ -506,7 +538,8 @@ struct exact_walk {
         b.pre3 = bcast_d(m.pre3, j);
         return b;
     }
-    __device__ int64_t step(const pct_geo& g, const H& b, int k) const { return mlp_step(g, b, tab, K, k); }
+    __device__ int64_t step(const pct_geo& g, const H& b, int k) const { return mlp_step_r(g, b, tab, K, k); }
+    __device__ int64_t fix(const pct_geo& g, const H& b, int k) const { return mlp_step(g, b, tab, K, k); }
 };
 
 struct table_walk {
@@ -536,6 +569,7 @@ struct table_walk {
     __device__ int64_t step(const pct_geo& g, const H& b, int k) const {
         return mlp_step_table(g, gf, b, tab, K, scat, Q, k);
     }
+    __device__ int64_t fix(const pct_geo& g, const H& b, int k) const { return step(g, b, k); }
 };
 
 /* MLP paths. Each wavefront owns 64 consecutive histories: lane j computes the
@@ -579,7 +613,8 @@ k_mlp_paths(pct_geo g, pct_hist_soa h, int64_t start, int64_t count, W walk,
         int64_t hcap = bcast_l(cap, j);
         int64_t n;
         uint32_t hsh;
-        walk_list([&](int k) { return walk.step(g, b, k); }, S, carry, hoff, hcap, voxels, lane, lt_mask, n, hsh);
+        walk_list([&](int k) { return walk.step(g, b, k); }, [&](int k) { return walk.fix(g, b, k); }, S, carry, hoff,
+                  hcap, voxels, lane, lt_mask, n, hsh);
         if (n > hcap) local_flags |= PCT_STATUS_CAPACITY;
         if (n > g.max_intersections) local_flags |= PCT_STATUS_OVERFLOW;
         if (lane == 0) lengths[base + j] = (uint32_t)(n < hcap ? n : hcap);
@@ -801,14 +836,14 @@ k_mlp_paths_sur(pct_geo g, pct_hist_soa h, int64_t start, int64_t count, const d
                 sur_hist b = sur_bcast(sh, j);
                 if (lane < 40) L.Q[lane] = sur_coef(lane, b, L.out + 8 * j);
                 wave_lds_sync();
-                walk_list([&](int k) { return sur_step(g, gf, b, L.Q, u1s, k); }, S, carry, hoff, hcap, voxels, lane,
-                          lt_mask, n, hsh);
+                auto st = [&](int k) { return sur_step(g, gf, b, L.Q, u1s, k); };
+                walk_list(st, st, S, carry, hoff, hcap, voxels, lane, lt_mask, n, hsh);
                 wave_lds_sync();
             } else {
                 exact_walk ew{tab, K};
                 mlp_hist b = ew.bcast(m, j);
-                walk_list([&](int k) { return mlp_step(g, b, tab, K, k); }, S, carry, hoff, hcap, voxels, lane,
-                          lt_mask, n, hsh);
+                auto st = [&](int k) { return mlp_step(g, b, tab, K, k); };
+                walk_list(st, st, S, carry, hoff, hcap, voxels, lane, lt_mask, n, hsh);
             }
             if (n > hcap) local_flags |= PCT_STATUS_CAPACITY;
             if (n > g.max_intersections) local_flags |= PCT_STATUS_OVERFLOW;
@@ -893,9 +928,10 @@ k_net_calib(pct_geo g, pct_hist_soa h, int64_t start, int64_t stride, int64_t ns
             }
             int64_t n1, n2;
             uint32_t h1, h2;
-            walk_list([&](int k) { return sur_step(g, gf, b, L.Q, u1s, k); }, S, carry, 0, 0, nullptr, lane, lt_mask,
-                      n1, h1);
-            walk_list([&](int k) { return mlp_step(g, e, tab, K, k); }, S, carry, 0, 0, nullptr, lane, lt_mask, n2, h2);
+            auto st1 = [&](int k) { return sur_step(g, gf, b, L.Q, u1s, k); };
+            auto st2 = [&](int k) { return mlp_step(g, e, tab, K, k); };
+            walk_list(st1, st1, S, carry, 0, 0, nullptr, lane, lt_mask, n1, h1);
+            walk_list(st2, st2, S, carry, 0, 0, nullptr, lane, lt_mask, n2, h2);
             uint32_t hd = h1 - h2;
             for (int d = 32; d >= 1; d >>= 1) hd += (uint32_t)__shfl_xor((int)hd, d, 64);
             const int ir = bcast_i((int)in_range, j), cl = bcast_i(cell, j);
@@ -1084,11 +1120,12 @@ int pct_mlp_paths_on(pct_ctx* ctx, hipStream_t stream, const pct_hist_soa* hist,
     if (count == 0) return PCT_OK;
     if ((rc = mlp_occupancy(ctx))) return rc;
     /* a workgroup of 256 is one wavefront on each of the CU's 4 SIMDs. Beside
-     * a single scan's block loop the cheaper walks leave room for the
-     * latency-bound block kernels (measured at 200x200x32: exact walk best at
-     * its occupancy, the precision form at 2, the surrogate at 1 wavefront per
-     * SIMD; the batched form's steps do better with the full occupancy) */
-    static const int beside_default[3] = {0, 2, 1};
+     * a single scan's block loop the walks leave room (wave slots and
+     * registers) for the latency-bound block kernels (measured at 200x200x32:
+     * the exact walk best at 3 wavefronts per SIMD, the precision form at 2,
+     * the surrogate at 1; the batched form's steps do better with the full
+     * occupancy) */
+    static const int beside_default[3] = {3, 2, 1};
     int per_cu = std::max(1, ctx->mlp_occ[mode]);
     int lim = ctx->tune_mlp_waves > 0 ? ctx->tune_mlp_waves : (beside ? beside_default[mode] : 0);
     if (lim > 0 && lim < per_cu) per_cu = lim;

@@ -39,20 +39,19 @@ class Preprocessed:
 STAGE_BYTES = {
     # raw row in; 11 f32 + missed out
     "recon_volume_intersections": lambda n, k, V, B: 56 * n + 45 * n,
-    # 6 f32 + missed in; bin_num, in_range, sort key and value out; one
-    # key/value pass of the sort (8 B in, 8 B out); sorted keys read for the
-    # bin ranges; first/last per bin written
-    "binning": lambda n, k, V, B: 25 * n + 13 * n + 16 * n + 4 * n + 8 * B,
-    # cylinder counts written, then missed, wepl and 6 f32 per history read
-    # and the counts read and written once
-    "msc_counts": lambda n, k, V, B: 4 * V + 29 * n + 8 * V,
-    "space_carve": lambda n, k, V, B: 29 * n + V / 4,
-    # sorted index, in_range, wepl and 4 angles per history; ranges in, 7
-    # statistics out per bin
-    "bin_statistics": lambda n, k, V, B: 25 * n + 36 * B,
-    # missed copied to failed, sorted index + wepl + 4 angles read, failed
-    # written; ranges and 6 statistics in, count and sinogram out per bin
-    "statistical_cuts": lambda n, k, V, B: 2 * n + 24 * n + n + 40 * B,
+    # 11 f32 + missed in; bin_num, in_range, sort key and 16-byte statistics
+    # record out; one key + record pass of the sort (20 B in, 20 B out);
+    # sorted keys read for the bin ranges; first/last per bin written
+    "binning": lambda n, k, V, B: 45 * n + 25 * n + 40 * n + 4 * n + 8 * B,
+    # cylinder counts written; missed and wepl of every history, the 6 f32
+    # positions of the k histories that walk; counts read and written once
+    "msc_counts": lambda n, k, V, B: 4 * V + 5 * n + 24 * k + 8 * V,
+    "space_carve": lambda n, k, V, B: 5 * n + 24 * k + V / 4,
+    # sorted records once; ranges in, 7 statistics out per bin
+    "bin_statistics": lambda n, k, V, B: 16 * n + 36 * B,
+    # per history: missed, bin_num, wepl, 4 angles in, failed out; per bin:
+    # its sorted records, ranges and 6 statistics in, count and sinogram out
+    "statistical_cuts": lambda n, k, V, B: 26 * n + 16 * n + 40 * B,
     # keep flag per history; each kept row read and written
     "compact_post_cut": lambda n, k, V, B: n + 104 * k,
     "fbp_filter": lambda n, k, V, B: 8 * B,
@@ -108,10 +107,13 @@ def preprocess(ctx: Context, raw: RawScan, hull_kind: str = "msc", timings: list
     T = _StageTimer(ctx, timings)
     hist, missed = T("recon_volume_intersections", n_raw, ctx.recon_volume_intersections, raw)
     in_range = T("binning", n_raw, ctx.binning, hist, missed)
+    def walkers(thr):                     # histories whose straight path is carved
+        return lambda _: int(((missed == 0) & (hist.wepl <= thr)).sum())
     if hull_kind == "msc":
-        counts = T("msc_counts", n_raw, ctx.msc_counts, hist, missed)
+        counts = T("msc_counts", n_raw, ctx.msc_counts, hist, missed, kept=walkers(ctx.params.msc_threshold))
     elif hull_kind == "sc":
-        sc_bits = T("space_carve", n_raw, ctx.space_carve, hist, missed, ctx.hull_init())
+        sc_bits = T("space_carve", n_raw, ctx.space_carve, hist, missed, ctx.hull_init(),
+                    kept=walkers(ctx.params.sc_threshold))
     st = T("bin_statistics", n_raw, ctx.bin_statistics, hist, missed, in_range)
     failed, sinogram = T("statistical_cuts", n_raw, ctx.statistical_cuts, hist, missed, st)
     post_cut = T("compact_post_cut", n_raw, ctx.compact, hist, (failed == 0).to(torch.uint8), kept=lambda h: h.n)

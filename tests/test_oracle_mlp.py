@@ -223,3 +223,28 @@ def test_overflow_is_reported():
     h = _single(p, -8.0, 0.3, 0.1, 8.0, 0.3, 0.1)
     with pytest.raises(OverflowError):
         oracle.mlp_path(p, h, 0)
+
+
+def test_reciprocal_voxel_index_equals_division():
+    """The MLP step's voxel index (calc_voxel_q in pct_mlp.hip) takes
+    |pos - zero| * RN(1/size) and falls back to the division when the product
+    lies within 2^-49 q of an integer. Restated here in IEEE float64 (numpy),
+    it must give the integer part of the division's quotient for every input,
+    including positions on and one ulp either side of every voxel face."""
+    rng = np.random.default_rng(11)
+
+    def fast(a, size):
+        q = a * (1.0 / size)
+        near = np.abs(q - np.rint(q)) <= q * 2.0 ** -49
+        q = np.where(near, a / size, q)
+        return np.floor(q), near
+
+    for size in (0.1, 0.2, 0.25, 1.0 / 3.0, 0.05, 2.5 / 3.0):
+        k = np.arange(0, 400, dtype=np.float64)
+        faces = k * size
+        a = np.concatenate([faces, np.nextafter(faces, np.inf), np.nextafter(faces, -np.inf).clip(0),
+                            rng.uniform(0, 400 * size, 2_000_000), (k + 0.5) * size])
+        got, near = fast(a, size)
+        np.testing.assert_array_equal(got, np.floor(a / size))
+        # the fallback is taken on the faces and (almost) nowhere else
+        assert near[len(k):].mean() < 0.02
