@@ -69,13 +69,18 @@ class RawScan:
 class Histories:
     """Per-history SoA (pct_hist_soa, data.h:577-590) in torch tensors."""
 
-    def __init__(self, n, device):
+    def __init__(self, n, device, zero=True):
+        """zero=False leaves the 11 float fields uninitialised, for producers
+        that write every one of them; bin_num and first_voxel start at 0
+        unless `zero` is None (a compaction writes every field)"""
         import torch
         self.n = n
+        alloc = torch.zeros if zero else torch.empty
         for f in _lib.HIST_FLOAT_FIELDS:
-            setattr(self, f, torch.zeros(n, dtype=torch.float32, device=device))
-        self.bin_num = torch.zeros(n, dtype=torch.int32, device=device)
-        self.first_voxel = torch.zeros(n, dtype=torch.int32, device=device)   # u32 bit pattern
+            setattr(self, f, alloc(n, dtype=torch.float32, device=device))
+        alloc = torch.empty if zero is None else torch.zeros
+        self.bin_num = alloc(n, dtype=torch.int32, device=device)
+        self.first_voxel = alloc(n, dtype=torch.int32, device=device)   # u32 bit pattern
 
     def soa(self):
         s = PctHistSoa()
@@ -186,7 +191,7 @@ class Context:
     # -- (a) recon-volume intersection (.cu:2066-2168 / 2169-2387) ------------
     def recon_volume_intersections(self, raw: RawScan):
         torch = self.torch
-        hist = Histories(raw.n, self.device)
+        hist = Histories(raw.n, self.device, zero=False)      # the kernel writes the 11 float fields
         missed = torch.empty(raw.n, dtype=torch.uint8, device=self.device)
         rs, hs = raw.soa(), hist.soa()
         self._check(self.L.pct_recon_volume_intersections(self._ctx, ctypes.byref(rs), ctypes.byref(hs), _ptr(missed)),
@@ -299,7 +304,7 @@ class Context:
         return inter
 
     def compact(self, hist: Histories, keep) -> Histories:
-        out = Histories(hist.n, self.device)
+        out = Histories(hist.n, self.device, zero=None)       # every field of every kept row is written
         n_kept = ctypes.c_int64(0)
         s, d = hist.soa(), out.soa()
         self._check(self.L.pct_compact(self._ctx, ctypes.byref(s), _ptr(keep), ctypes.byref(d), ctypes.byref(n_kept)),

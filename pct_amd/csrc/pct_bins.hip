@@ -221,11 +221,14 @@ static int bins_bits(int64_t bins) {
     return b;
 }
 
+/* per-history arrays may be null when there is no history */
 static int check_bin_args(pct_ctx* ctx, const pct_hist_soa* h, const uint8_t* missed, const char* who) {
-    if (!ctx || !h || !missed) return pct_fail(ctx, PCT_E_INVALID, "%s: null argument", who);
+    if (!ctx || !h) return pct_fail(ctx, PCT_E_INVALID, "%s: null argument", who);
+    if (h->n < 0 || h->n >= ((int64_t)1 << 32)) return pct_fail(ctx, PCT_E_INVALID, "%s: bad history count", who);
+    if (h->n == 0) return PCT_OK;
+    if (!missed) return pct_fail(ctx, PCT_E_INVALID, "%s: null missed", who);
     if (!h->bin_num || !h->wepl || !h->x_entry || !h->xy_entry_angle || !h->xz_entry_angle)
         return pct_fail(ctx, PCT_E_INVALID, "%s: missing history field", who);
-    if (h->n < 0 || h->n >= ((int64_t)1 << 32)) return pct_fail(ctx, PCT_E_INVALID, "%s: bad history count", who);
     return PCT_OK;
 }
 
@@ -281,7 +284,7 @@ static int bin_and_sort(pct_ctx* ctx, const pct_hist_soa* h, const uint8_t* miss
 extern "C" int pct_binning(pct_ctx* ctx, const pct_hist_soa* hist, const uint8_t* missed, uint8_t* in_range) {
     int rc = check_bin_args(ctx, hist, missed, "pct_binning");
     if (rc) return rc;
-    if (!in_range) return pct_fail(ctx, PCT_E_INVALID, "pct_binning: null in_range");
+    if (!in_range && hist->n > 0) return pct_fail(ctx, PCT_E_INVALID, "pct_binning: null in_range");
     return bin_and_sort(ctx, hist, missed, in_range);
 }
 
@@ -293,7 +296,7 @@ extern "C" int pct_bin_statistics(pct_ctx* ctx, const pct_hist_soa* hist, const 
                                   const pct_bin_stats* stats) {
     int rc = check_bin_args(ctx, hist, missed, "pct_bin_statistics");
     if (rc) return rc;
-    if (!in_range || !stats || !stats->counts || !stats->mean_wepl || !stats->mean_rel_ut || !stats->mean_rel_uv ||
+    if ((!in_range && hist->n > 0) || !stats || !stats->counts || !stats->mean_wepl || !stats->mean_rel_ut || !stats->mean_rel_uv ||
         !stats->stddev_wepl || !stats->stddev_rel_ut || !stats->stddev_rel_uv)
         return pct_fail(ctx, PCT_E_INVALID, "pct_bin_statistics: null output");
     pct_bin_order* o = order_of(ctx);
@@ -311,7 +314,8 @@ extern "C" int pct_statistical_cuts(pct_ctx* ctx, const pct_hist_soa* hist, cons
                                     const pct_bin_stats* stats, uint8_t* failed, float* sinogram) {
     int rc = check_bin_args(ctx, hist, missed, "pct_statistical_cuts");
     if (rc) return rc;
-    if (!stats || !failed || !sinogram) return pct_fail(ctx, PCT_E_INVALID, "pct_statistical_cuts: null argument");
+    if (!stats || (!failed && hist->n > 0) || !sinogram)
+        return pct_fail(ctx, PCT_E_INVALID, "pct_statistical_cuts: null argument");
     pct_bin_order* o = order_of(ctx);
     if (!same_arrays(o->key, *hist))
         return pct_fail(ctx, PCT_E_INVALID, "pct_statistical_cuts: call pct_binning on these histories first");

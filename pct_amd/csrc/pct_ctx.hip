@@ -322,10 +322,11 @@ static hist_ptrs to_ptrs(const pct_hist_soa* h) {
 
 extern "C" int pct_compact(pct_ctx* ctx, const pct_hist_soa* src, const uint8_t* keep,
                            const pct_hist_soa* dst, int64_t* n_kept) {
-    if (!ctx || !src || !dst || !keep || !n_kept) return pct_fail(ctx, PCT_E_INVALID, "pct_compact: null argument");
+    if (!ctx || !src || !dst || !n_kept) return pct_fail(ctx, PCT_E_INVALID, "pct_compact: null argument");
     int64_t n = src->n;
     if (n < 0) return pct_fail(ctx, PCT_E_INVALID, "pct_compact: negative n");
     if (n == 0) { *n_kept = 0; return PCT_OK; }
+    if (!keep) return pct_fail(ctx, PCT_E_INVALID, "pct_compact: null keep");
     hist_ptrs s = to_ptrs(src), d = to_ptrs(dst);
     for (int k = 0; k < 11; ++k)
         if ((s.f[k] == nullptr) != (d.f[k] == nullptr))

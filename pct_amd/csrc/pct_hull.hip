@@ -197,7 +197,10 @@ k_hull_select(pct_geo g, int radius, double threshold, const int32_t* __restrict
 }  // namespace
 
 static int check_carve(pct_ctx* ctx, const pct_hist_soa* h, const uint8_t* missed, const void* out, const char* who) {
-    if (!ctx || !h || !missed || !out) return pct_fail(ctx, PCT_E_INVALID, "%s: null argument", who);
+    if (!ctx || !h || !out) return pct_fail(ctx, PCT_E_INVALID, "%s: null argument", who);
+    if (h->n < 0) return pct_fail(ctx, PCT_E_INVALID, "%s: negative history count", who);
+    if (h->n == 0) return PCT_OK;                   /* no history: the arrays are not read */
+    if (!missed) return pct_fail(ctx, PCT_E_INVALID, "%s: null missed", who);
     if (!h->x_entry || !h->x_exit || !h->wepl) return pct_fail(ctx, PCT_E_INVALID, "%s: missing history field", who);
     return PCT_OK;
 }

@@ -152,11 +152,12 @@ k_synth(pct_synth_cfg cfg, int64_t start, int64_t count, pct_synth_soa out) {
 
 extern "C" int pct_recon_volume_intersections(pct_ctx* ctx, const pct_raw_soa* raw, const pct_hist_soa* out,
                                               uint8_t* missed) {
-    if (!ctx || !raw || !out || !missed) return pct_fail(ctx, PCT_E_INVALID, "pct_recon_volume_intersections: null argument");
+    if (!ctx || !raw || !out) return pct_fail(ctx, PCT_E_INVALID, "pct_recon_volume_intersections: null argument");
     if (raw->n != out->n || raw->n < 0)
         return pct_fail(ctx, PCT_E_INVALID, "pct_recon_volume_intersections: raw->n (%lld) != out->n (%lld)",
                         (long long)raw->n, (long long)out->n);
-    if (raw->n == 0) return PCT_OK;
+    if (raw->n == 0) return PCT_OK;                 /* no history: the arrays are not read */
+    if (!missed) return pct_fail(ctx, PCT_E_INVALID, "pct_recon_volume_intersections: null missed");
     hipLaunchKernelGGL(k_recon_volume_intersections, dim3(pct_grid(raw->n, 256)), dim3(256), 0, ctx->stream,
                        ctx->g, *raw, *out, missed);
     PCT_HIP_CHECK(ctx, hipGetLastError());

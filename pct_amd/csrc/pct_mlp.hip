@@ -1073,6 +1073,7 @@ static int check_hist(pct_ctx* ctx, const pct_hist_soa* h, int64_t start, int64_
     if (start < 0 || count < 0 || start + count > h->n)
         return pct_fail(ctx, PCT_E_INVALID, "%s: range [%lld, %lld) outside [0, %lld)", who, (long long)start,
                         (long long)(start + count), (long long)h->n);
+    if (count == 0) return PCT_OK;                  /* no history: the arrays are not read */
     if (!h->x_entry || !h->y_entry || !h->z_entry || !h->x_exit || !h->y_exit || !h->z_exit || !h->xy_entry_angle ||
         !h->xz_entry_angle || !h->xy_exit_angle || !h->xz_exit_angle || !h->first_voxel)
         return pct_fail(ctx, PCT_E_INVALID, "%s: missing history field", who);
@@ -1082,8 +1083,8 @@ static int check_hist(pct_ctx* ctx, const pct_hist_soa* h, int64_t start, int64_
 extern "C" int pct_mlp_endpoints(pct_ctx* ctx, const pct_hist_soa* hist, const uint32_t* hull_bits, uint8_t* intersected) {
     int rc = check_hist(ctx, hist, 0, hist ? hist->n : 0, "pct_mlp_endpoints");
     if (rc) return rc;
-    if (!hull_bits || !intersected) return pct_fail(ctx, PCT_E_INVALID, "pct_mlp_endpoints: null hull/intersected");
     if (hist->n == 0) return PCT_OK;
+    if (!hull_bits || !intersected) return pct_fail(ctx, PCT_E_INVALID, "pct_mlp_endpoints: null hull/intersected");
     hipLaunchKernelGGL(k_mlp_endpoints, dim3(pct_grid(hist->n, 256)), dim3(256), 0, ctx->stream, ctx->g, *hist,
                        hull_bits, intersected);
     PCT_HIP_CHECK(ctx, hipGetLastError());
@@ -1181,6 +1182,7 @@ extern "C" int pct_mlp_paths(pct_ctx* ctx, const pct_hist_soa* hist, int64_t sta
     int rc = check_hist(ctx, hist, start, count, "pct_mlp_paths");
     if (rc) return rc;
     if (!pct_mlp_mode_ok(ctx, mode)) return pct_fail(ctx, PCT_E_INVALID, "pct_mlp_paths: mode %d not available", mode);
+    if (count == 0) return PCT_OK;
     if (!offsets || !voxels || !lengths) return pct_fail(ctx, PCT_E_INVALID, "pct_mlp_paths: null output");
     rc = pct_mlp_paths_impl(ctx, hist, start, count, offsets, voxels, lengths, nullptr, nullptr, nullptr, mode);
     if (rc) return rc;
@@ -1197,9 +1199,9 @@ extern "C" int pct_net_features(pct_ctx* ctx, const pct_hist_soa* hist, int64_t 
                                 float* targets) {
     int rc = check_hist(ctx, hist, start, count, "pct_net_features");
     if (rc) return rc;
+    if (count == 0) return PCT_OK;
     if (!features || !hist->wepl) return pct_fail(ctx, PCT_E_INVALID, "pct_net_features: null features/wepl");
     if ((rc = ensure_mlp_table(ctx))) return rc;
-    if (count == 0) return PCT_OK;
     hipLaunchKernelGGL(k_net_features, dim3(pct_grid(count, 256)), dim3(256), 0, ctx->stream, ctx->g, *hist, start,
                        count, (const double*)ctx->d_mlp_tab, ctx->mlp_tab_k, features, targets);
     PCT_HIP_CHECK(ctx, hipGetLastError());

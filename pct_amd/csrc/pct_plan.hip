@@ -638,7 +638,9 @@ extern "C" int pct_drop_prepare(pct_ctx* ctx, const pct_hist_soa* hist, int bloc
 }
 
 extern "C" int pct_drop_iteration(pct_ctx* ctx, const pct_hist_soa* hist, float* x, int block_size, float lambda, int mode) {
-    if (!ctx || !hist || !hist->wepl || !hist->first_voxel || !hist->x_entry)
+    if (!ctx || !hist) return pct_fail(ctx, PCT_E_INVALID, "pct_drop_iteration: null argument");
+    if (hist->n == 0 && x && block_size > 0) return PCT_OK;   /* no history: x is left as it is */
+    if (!hist->wepl || !hist->first_voxel || !hist->x_entry)
         return pct_fail(ctx, PCT_E_INVALID, "pct_drop_iteration: null argument");
     const int mlp = mode & PCT_MLP_MODE_MASK;
     if (!pct_mlp_mode_ok(ctx, mlp))
