@@ -101,6 +101,22 @@ __global__ void k_fill_u32(uint32_t* __restrict__ a, int64_t n, uint32_t v) {
 }
 
 /* A2 sums + A3 means + A4 squared deviations + A5 deviations, one thread per bin */
+/* f(record) for the records [a, e) in order. A thread walks its own bin, so
+ * the records are read in batches of 8 independent loads before any is used:
+ * one memory round trip per 8 records instead of one per record. */
+template <class F>
+__device__ __forceinline__ void for_each_rec(const bin_rec* __restrict__ recs, uint32_t a, uint32_t e, F f) {
+    uint32_t k = a;
+    for (; k + 8 <= e; k += 8) {
+        bin_rec r[8];
+#pragma unroll
+        for (int u = 0; u < 8; ++u) r[u] = recs[k + u];
+#pragma unroll
+        for (int u = 0; u < 8; ++u) f(r[u]);
+    }
+    for (; k < e; ++k) f(recs[k]);
+}
+
 __global__ void __launch_bounds__(256)
 k_bin_stats(int64_t bins, const uint32_t* __restrict__ first, const uint32_t* __restrict__ last,
             const bin_rec* __restrict__ recs, pct_bin_stats st) {
@@ -109,24 +125,22 @@ k_bin_stats(int64_t bins, const uint32_t* __restrict__ first, const uint32_t* __
     uint32_t a = first[bin], e = last[bin];
     int32_t count = 0;
     float sw = 0.0f, su = 0.0f, sv = 0.0f;
-    for (uint32_t k = a; k < e; ++k) {
-        bin_rec r = recs[k];
-        if (!r.in_range) continue;
+    for_each_rec(recs, a, e, [&](const bin_rec& r) {
+        if (!r.in_range) return;
         double rel_ut = wrap_pi(r.d_ut);
         double rel_uv = wrap_pi(r.d_uv);
         count += 1;
         sw += r.wepl;
         su += (float)rel_ut;
         sv += (float)rel_uv;
-    }
+    });
     if (count > 0) {
         sw /= count;
         su /= count;
         sv /= count;
     }
     float qw = 0.0f, qu = 0.0f, qv = 0.0f;
-    for (uint32_t k = a; k < e; ++k) {
-        bin_rec r = recs[k];
+    for_each_rec(recs, a, e, [&](const bin_rec& r) {
         double rel_ut = wrap_pi(r.d_ut);
         double rel_uv = wrap_pi(r.d_uv);
         double dw = r.wepl - sw;
@@ -135,7 +149,7 @@ k_bin_stats(int64_t bins, const uint32_t* __restrict__ first, const uint32_t* __
         qw += (float)(dw * dw);
         qu += (float)(du * du);
         qv += (float)(dv * dv);
-    }
+    });
     if (count > 1) {
         qw = pct_sqrtf(qw / (count - 1));
         qu = pct_sqrtf(qu / (count - 1));
@@ -158,19 +172,35 @@ __device__ __forceinline__ bool cut_fails(float w, float d_ut, float d_uv, float
     return !pu || !pv || !pw;
 }
 
+/* a bin's cut window: means and sigmas_to_keep x deviations, one 32-byte
+ * record per bin so that a history's check gathers one cache line */
+struct __align__(32) bin_cut {
+    float mw, mu, mv, cw, cu, cv, pad0, pad1;
+};
+
+__global__ void __launch_bounds__(256)
+k_pack_cuts(int64_t bins, pct_bin_stats st, float sig, bin_cut* __restrict__ cut) {
+    int64_t bin = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (bin >= bins) return;
+    bin_cut c;
+    c.mw = st.mean_wepl[bin]; c.mu = st.mean_rel_ut[bin]; c.mv = st.mean_rel_uv[bin];
+    c.cw = sig * st.stddev_wepl[bin]; c.cu = sig * st.stddev_rel_ut[bin]; c.cv = sig * st.stddev_rel_uv[bin];
+    c.pad0 = c.pad1 = 0.0f;
+    cut[bin] = c;
+}
+
 /* A6 per history in storage order: failed = missed, or outside its bin's
  * cuts (an out-of-range history is judged in bin 0, as it is keyed) */
 __global__ void __launch_bounds__(256)
-k_cut_flags(pct_hist_soa h, const uint8_t* __restrict__ missed, pct_bin_stats st, float sig,
+k_cut_flags(pct_hist_soa h, const uint8_t* __restrict__ missed, const bin_cut* __restrict__ cut,
             uint8_t* __restrict__ failed) {
     int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
     if (i >= h.n) return;
     bool f = true;
     if (!missed[i]) {
-        int32_t bin = h.bin_num[i];
+        bin_cut c = cut[h.bin_num[i]];
         f = cut_fails(h.wepl[i], h.xy_exit_angle[i] - h.xy_entry_angle[i], h.xz_exit_angle[i] - h.xz_entry_angle[i],
-                      st.mean_wepl[bin], st.mean_rel_ut[bin], st.mean_rel_uv[bin], sig * st.stddev_wepl[bin],
-                      sig * st.stddev_rel_ut[bin], sig * st.stddev_rel_uv[bin]);
+                      c.mw, c.mu, c.mv, c.cw, c.cu, c.cv);
     }
     failed[i] = f;
 }
@@ -186,13 +216,12 @@ k_bin_sinogram(int64_t bins, const uint32_t* __restrict__ first, const uint32_t*
     float cw = sig * st.stddev_wepl[bin], cu = sig * st.stddev_rel_ut[bin], cv = sig * st.stddev_rel_uv[bin];
     int32_t count = 0;
     float sum = 0.0f;
-    for (uint32_t k = a; k < e; ++k) {
-        bin_rec r = recs[k];
+    for_each_rec(recs, a, e, [&](const bin_rec& r) {
         if (!cut_fails(r.wepl, r.d_ut, r.d_uv, mw, mu, mv, cw, cu, cv)) {
             count += 1;
             sum += r.wepl;
         }
-    }
+    });
     if (count > 0) sum /= count;
     st.counts[bin] = count;
     sinogram[bin] = sum;
@@ -204,7 +233,7 @@ k_bin_sinogram(int64_t bins, const uint32_t* __restrict__ first, const uint32_t*
 struct pct_bin_order {
     pct_hist_soa key{};
     int64_t n = 0;
-    pct_buffer keys_in, keys_out, recs_in, recs_out, first, last, tmp;
+    pct_buffer keys_in, keys_out, recs_in, recs_out, first, last, tmp, cut;
 };
 
 static bin_geo make_bin_geo(const pct_params& p) {
@@ -239,7 +268,7 @@ static pct_bin_order* order_of(pct_ctx* ctx) {
 
 void pct_bin_order_destroy(pct_bin_order* o) {
     if (!o) return;
-    for (pct_buffer* b : {&o->keys_in, &o->keys_out, &o->recs_in, &o->recs_out, &o->first, &o->last, &o->tmp})
+    for (pct_buffer* b : {&o->keys_in, &o->keys_out, &o->recs_in, &o->recs_out, &o->first, &o->last, &o->tmp, &o->cut})
         if (b->ptr) (void)hipFree(b->ptr);
     delete o;
 }
@@ -321,9 +350,13 @@ extern "C" int pct_statistical_cuts(pct_ctx* ctx, const pct_hist_soa* hist, cons
         return pct_fail(ctx, PCT_E_INVALID, "pct_statistical_cuts: call pct_binning on these histories first");
     bin_geo b = make_bin_geo(ctx->p);
     const float sig = (float)ctx->p.sigmas_to_keep;
-    if (hist->n > 0)
-        hipLaunchKernelGGL(k_cut_flags, dim3(pct_grid(hist->n, 256)), dim3(256), 0, ctx->stream, *hist, missed, *stats,
-                           sig, failed);
+    if (hist->n > 0) {
+        if ((rc = pct_ensure(ctx, o->cut, sizeof(bin_cut) * (size_t)b.bins))) return rc;
+        hipLaunchKernelGGL(k_pack_cuts, dim3(pct_grid(b.bins, 256)), dim3(256), 0, ctx->stream, b.bins, *stats, sig,
+                           (bin_cut*)o->cut.ptr);
+        hipLaunchKernelGGL(k_cut_flags, dim3(pct_grid(hist->n, 256)), dim3(256), 0, ctx->stream, *hist, missed,
+                           (const bin_cut*)o->cut.ptr, failed);
+    }
     /* after the flags: this kernel overwrites the counts with the survivors' */
     hipLaunchKernelGGL(k_bin_sinogram, dim3(pct_grid(b.bins, 256)), dim3(256), 0, ctx->stream, b.bins,
                        (const uint32_t*)o->first.ptr, (const uint32_t*)o->last.ptr, (const bin_rec*)o->recs_out.ptr,

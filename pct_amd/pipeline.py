@@ -49,9 +49,10 @@ STAGE_BYTES = {
     "space_carve": lambda n, k, V, B: 5 * n + 24 * k + V / 4,
     # sorted records once; ranges in, 7 statistics out per bin
     "bin_statistics": lambda n, k, V, B: 16 * n + 36 * B,
-    # per history: missed, bin_num, wepl, 4 angles in, failed out; per bin:
-    # its sorted records, ranges and 6 statistics in, count and sinogram out
-    "statistical_cuts": lambda n, k, V, B: 26 * n + 16 * n + 40 * B,
+    # per history: missed, bin_num, wepl, 4 angles in, failed out, and its
+    # sorted record; per bin: ranges and 6 statistics in, the 32-byte cut
+    # window written, count and sinogram out
+    "statistical_cuts": lambda n, k, V, B: 26 * n + 16 * n + 72 * B,
     # keep flag per history; each kept row read and written
     "compact_post_cut": lambda n, k, V, B: n + 104 * k,
     "fbp_filter": lambda n, k, V, B: 8 * B,
