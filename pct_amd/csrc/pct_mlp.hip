@@ -35,6 +35,9 @@ __device__ __forceinline__ int calc_voxel(double zero, double pos, double size) 
     double q = pct_fabs(pos - zero) / size;
     return q < 1.0e9 ? (int)q : -1;
 }
+#define PCT_LDS_PER_CU (160u * 1024u)        /* LDS of one CDNA4 compute unit */
+#define PCT_MLP_LDS_TAB_MAX (64u * 1024u)   /* largest per-depth table staged in LDS */
+
 /* calc_voxel with the division replaced by a product with the rounded
  * reciprocal. q = |pos - zero| * inv carries two roundings (of inv and of the
  * product), the division's quotient one, so the two differ by less than
@@ -574,13 +577,24 @@ struct table_walk {
 
 /* MLP paths. Each wavefront owns 64 consecutive histories: lane j computes the
  * per-history constants of history j, then the wave walks the histories one
- * by one with the 64 lanes on 64 consecutive steps. */
-template <class W>
+ * by one with the 64 lanes on 64 consecutive steps. With LDS_TAB the
+ * per-depth table (7 FP64 fields per step, read by every step of every
+ * history) is staged in LDS once per workgroup, so that the steps' table
+ * reads stay off the vector memory path that the block kernels running
+ * beside the walk depend on. */
+template <class W, bool LDS_TAB>
 __global__ void __launch_bounds__(256)
 k_mlp_paths(pct_geo g, pct_hist_soa h, int64_t start, int64_t count, W walk,
             const int64_t* __restrict__ offsets, uint32_t* __restrict__ voxels, uint32_t* __restrict__ lengths,
             double* __restrict__ chord_out, int* __restrict__ status, uint32_t* __restrict__ hash_out,
             const uint32_t* __restrict__ hash_expect) {
+    extern __shared__ double s_mlp_tab[];
+    if (LDS_TAB) {
+        const int n = MLP_TAB_FIELDS * walk.K;
+        for (int i = threadIdx.x; i < n; i += blockDim.x) s_mlp_tab[i] = walk.tab[i];
+        __syncthreads();
+        walk.tab = s_mlp_tab;
+    }
     const int K = walk.K;
     const int lane = threadIdx.x & 63;
     const uint64_t lt_mask = (lane == 0) ? 0ull : (~0ull >> (64 - lane));
@@ -1059,9 +1073,9 @@ static int mlp_occupancy(pct_ctx* ctx) {
     if (ctx->cus) return PCT_OK;
     PCT_HIP_CHECK(ctx, hipDeviceGetAttribute(&ctx->cus, hipDeviceAttributeMultiprocessorCount, ctx->device));
     PCT_HIP_CHECK(ctx, hipOccupancyMaxActiveBlocksPerMultiprocessor(&ctx->mlp_occ[PCT_MLP_FP64],
-                                                                    k_mlp_paths<exact_walk>, 256, 0));
+                                                                    k_mlp_paths<exact_walk, false>, 256, 0));
     PCT_HIP_CHECK(ctx, hipOccupancyMaxActiveBlocksPerMultiprocessor(&ctx->mlp_occ[PCT_MLP_TABLE],
-                                                                    k_mlp_paths<table_walk>, 256, 0));
+                                                                    k_mlp_paths<table_walk, false>, 256, 0));
     PCT_HIP_CHECK(ctx, hipOccupancyMaxActiveBlocksPerMultiprocessor(&ctx->mlp_occ[PCT_MLP_SURROGATE], k_mlp_paths_sur,
                                                                     256, sizeof(float) * ctx->mlp_tab_k));
     ctx->cus = std::max(1, ctx->cus);
@@ -1138,15 +1152,24 @@ int pct_mlp_paths_on(pct_ctx* ctx, hipStream_t stream, const pct_hist_soa* hist,
                            (const pct_net_frag*)ctx->d_net, (const float*)ctx->d_net_range,
                            (const uint8_t*)ctx->d_net_cells, offsets, voxels, lengths, chord, ctx->d_status, hash_out,
                            hash_expect, ctx->d_net_counts);
-    } else if (mode == PCT_MLP_FP64) {
-        exact_walk w{(const double*)ctx->d_mlp_tab, ctx->mlp_tab_k};
-        hipLaunchKernelGGL(k_mlp_paths<exact_walk>, dim3(grid), dim3(256), 0, stream, ctx->g, *hist,
-                           start, count, w, offsets, voxels, lengths, chord, ctx->d_status, hash_out, hash_expect);
     } else {
-        table_walk w{(const double*)ctx->d_mlp_tab, ctx->mlp_tab_k, (const double*)ctx->d_scat_tab, ctx->scat_q,
-                     make_grid_f(ctx->g)};
-        hipLaunchKernelGGL(k_mlp_paths<table_walk>, dim3(grid), dim3(256), 0, stream, ctx->g, *hist,
-                           start, count, w, offsets, voxels, lengths, chord, ctx->d_status, hash_out, hash_expect);
+        /* the per-depth table in LDS when the workgroups this launch keeps on
+         * a CU fit in its 160 KB beside their registers' limit */
+        const size_t tab_bytes = sizeof(double) * MLP_TAB_FIELDS * (size_t)ctx->mlp_tab_k;
+        const bool lds = tab_bytes <= PCT_MLP_LDS_TAB_MAX && tab_bytes * (size_t)per_cu <= PCT_LDS_PER_CU;
+        const size_t shm = lds ? tab_bytes : 0;
+        if (mode == PCT_MLP_FP64) {
+            exact_walk w{(const double*)ctx->d_mlp_tab, ctx->mlp_tab_k};
+            hipLaunchKernelGGL((lds ? k_mlp_paths<exact_walk, true> : k_mlp_paths<exact_walk, false>), dim3(grid),
+                               dim3(256), shm, stream, ctx->g, *hist, start, count, w, offsets, voxels, lengths, chord,
+                               ctx->d_status, hash_out, hash_expect);
+        } else {
+            table_walk w{(const double*)ctx->d_mlp_tab, ctx->mlp_tab_k, (const double*)ctx->d_scat_tab, ctx->scat_q,
+                         make_grid_f(ctx->g)};
+            hipLaunchKernelGGL((lds ? k_mlp_paths<table_walk, true> : k_mlp_paths<table_walk, false>), dim3(grid),
+                               dim3(256), shm, stream, ctx->g, *hist, start, count, w, offsets, voxels, lengths, chord,
+                               ctx->d_status, hash_out, hash_expect);
+        }
     }
     PCT_HIP_CHECK(ctx, hipGetLastError());
     return PCT_OK;
