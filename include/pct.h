@@ -133,7 +133,11 @@ typedef struct pct_bin_stats {
 /* (a) Binning: replaces binning_GPU (.cu:2507-2578). Writes hist->bin_num
  * (0 for a history whose bin is out of range, as the reference's calloc'd
  * array leaves it) and in_range[i]; orders the non-missed histories by bin
- * (stably) for the two calls below. */
+ * (stably) for the two calls below, keeping a snapshot of what they read
+ * (WEPL, exit-minus-entry angles, in_range): the histories' WEPL and angles
+ * must not change before those calls. With hist->n == 0 the per-history
+ * arrays (and missed, in_range) may be null; this holds for every entry
+ * point of this header whose history count or range is empty. */
 int pct_binning(pct_ctx* ctx, const pct_hist_soa* hist, const uint8_t* missed, uint8_t* in_range);
 /* (a) Per-bin count and means of WEPL and relative angles over in-range
  * non-missed histories, squared deviations over all non-missed ones, standard
