@@ -136,8 +136,10 @@ typedef struct pct_bin_stats {
  * (stably) for the two calls below, keeping a snapshot of what they read
  * (WEPL, exit-minus-entry angles, in_range): the histories' WEPL and angles
  * must not change before those calls. With hist->n == 0 the per-history
- * arrays (and missed, in_range) may be null; this holds for every entry
- * point of this header whose history count or range is empty. */
+ * arrays (and missed, in_range) may be null. The same holds for every stage
+ * entry point of this header whose history count or range is empty, and
+ * pct_drop_iteration returns at once for an empty set; pct_drop_prepare,
+ * pct_drop_stats and the batched form need at least one history. */
 int pct_binning(pct_ctx* ctx, const pct_hist_soa* hist, const uint8_t* missed, uint8_t* in_range);
 /* (a) Per-bin count and means of WEPL and relative angles over in-range
  * non-missed histories, squared deviations over all non-missed ones, standard
