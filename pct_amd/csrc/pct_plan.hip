@@ -265,6 +265,7 @@ static int vbits_for(int64_t V) {
 /* Build (or rebuild) the plan of `hist` with blocks of B histories. */
 static int plan_build(pct_ctx* ctx, const pct_hist_soa* hist, int B, int mlp_mode) {
     const int64_t n = hist->n;
+    if (n <= 0) return pct_fail(ctx, PCT_E_INVALID, "DROP plan: no histories");
     if (B <= 0 || B > SEG_MAX_B)
         return pct_fail(ctx, PCT_E_INVALID, "DROP block size %d outside [1, %lld]", B, (long long)SEG_MAX_B);
     const int vbits = vbits_for(ctx->g.V);
