@@ -100,7 +100,6 @@ __global__ void k_fill_u32(uint32_t* __restrict__ a, int64_t n, uint32_t v) {
     if (i < n) a[i] = v;
 }
 
-/* A2 sums + A3 means + A4 squared deviations + A5 deviations, one thread per bin */
 /* f(record) for the records [a, e) in order. A thread walks its own bin, so
  * the records are read in batches of 8 independent loads before any is used:
  * one memory round trip per 8 records instead of one per record. */
@@ -117,6 +116,7 @@ __device__ __forceinline__ void for_each_rec(const bin_rec* __restrict__ recs, u
     for (; k < e; ++k) f(recs[k]);
 }
 
+/* A2 sums + A3 means + A4 squared deviations + A5 deviations, one thread per bin */
 __global__ void __launch_bounds__(256)
 k_bin_stats(int64_t bins, const uint32_t* __restrict__ first, const uint32_t* __restrict__ last,
             const bin_rec* __restrict__ recs, pct_bin_stats st) {
