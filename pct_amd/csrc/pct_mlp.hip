@@ -1144,8 +1144,8 @@ int pct_mlp_paths_on(pct_ctx* ctx, hipStream_t stream, const pct_hist_soa* hist,
     int per_cu = std::max(1, ctx->mlp_occ[mode]);
     int lim = ctx->tune_mlp_waves > 0 ? ctx->tune_mlp_waves : (beside ? beside_default[mode] : 0);
     if (lim > 0 && lim < per_cu) per_cu = lim;
-    const unsigned grid = std::min(pct_grid(count, 256), (unsigned)(ctx->cus * per_cu));
     if (mode == PCT_MLP_SURROGATE) {
+        const unsigned grid = std::min(pct_grid(count, 256), (unsigned)(ctx->cus * per_cu));
         hipLaunchKernelGGL(k_mlp_paths_sur, dim3(grid), dim3(256), sizeof(float) * ctx->mlp_tab_k, stream, ctx->g, *hist,
                            start, count,
                            (const double*)ctx->d_mlp_tab, ctx->mlp_tab_k, (const float*)ctx->d_u1f, make_grid_f(ctx->g),
@@ -1156,17 +1156,23 @@ int pct_mlp_paths_on(pct_ctx* ctx, hipStream_t stream, const pct_hist_soa* hist,
         /* the per-depth table in LDS when the workgroups this launch keeps on
          * a CU fit in its 160 KB beside their registers' limit */
         const size_t tab_bytes = sizeof(double) * MLP_TAB_FIELDS * (size_t)ctx->mlp_tab_k;
-        const bool lds = tab_bytes <= PCT_MLP_LDS_TAB_MAX && tab_bytes * (size_t)per_cu <= PCT_LDS_PER_CU;
+        bool lds = tab_bytes <= PCT_MLP_LDS_TAB_MAX && tab_bytes * (size_t)per_cu <= PCT_LDS_PER_CU;
+        if (!lds && tab_bytes <= PCT_MLP_LDS_TAB_MAX && PCT_LDS_PER_CU / tab_bytes >= 3) {
+            /* a larger grid's table: one wavefront per SIMD less, table in LDS */
+            per_cu = (int)(PCT_LDS_PER_CU / tab_bytes);
+            lds = true;
+        }
+        const unsigned tgrid = std::min(pct_grid(count, 256), (unsigned)(ctx->cus * per_cu));
         const size_t shm = lds ? tab_bytes : 0;
         if (mode == PCT_MLP_FP64) {
             exact_walk w{(const double*)ctx->d_mlp_tab, ctx->mlp_tab_k};
-            hipLaunchKernelGGL((lds ? k_mlp_paths<exact_walk, true> : k_mlp_paths<exact_walk, false>), dim3(grid),
+            hipLaunchKernelGGL((lds ? k_mlp_paths<exact_walk, true> : k_mlp_paths<exact_walk, false>), dim3(tgrid),
                                dim3(256), shm, stream, ctx->g, *hist, start, count, w, offsets, voxels, lengths, chord,
                                ctx->d_status, hash_out, hash_expect);
         } else {
             table_walk w{(const double*)ctx->d_mlp_tab, ctx->mlp_tab_k, (const double*)ctx->d_scat_tab, ctx->scat_q,
                          make_grid_f(ctx->g)};
-            hipLaunchKernelGGL((lds ? k_mlp_paths<table_walk, true> : k_mlp_paths<table_walk, false>), dim3(grid),
+            hipLaunchKernelGGL((lds ? k_mlp_paths<table_walk, true> : k_mlp_paths<table_walk, false>), dim3(tgrid),
                                dim3(256), shm, stream, ctx->g, *hist, start, count, w, offsets, voxels, lengths, chord,
                                ctx->d_status, hash_out, hash_expect);
         }
