@@ -196,3 +196,29 @@ def test_synth_ellipsoid_line_integrals():
         n_hit += w > 0
         assert abs(float(raw["wepl"][i]) - w) < 2e-3 * max(w, 1.0) + 2e-3, (i, raw["wepl"][i], w)
     assert n_hit > 30
+
+
+def test_tuner_candidates():
+    """the batched form's block-size candidates: one uniform assignment per
+    allowed size first, then step-balanced ones in which every scan takes the
+    smallest size that keeps it within the step target; no duplicates"""
+    import math
+    from pct_amd.tuner import candidate_assignments
+    sizes = [800, 1600, 3200, 6400]
+    hist = [540_629, 6_193_135, 1_218_385, 11_938_713, 3_969_720]
+    cands = candidate_assignments(hist, sizes)
+    assert cands[:4] == [tuple([b] * len(hist)) for b in sizes]
+    assert len(cands) == len(set(cands))
+    for c in cands:
+        assert all(b in sizes for b in c)
+    for c in cands[4:]:
+        # balanced: within the assignment's own step count, no scan could
+        # take a smaller block
+        steps = max(math.ceil(h / b) for h, b in zip(hist, c))
+        for h, b in zip(hist, c):
+            assert all(math.ceil(h / s) > steps for s in sizes if s < b)
+    # the balanced assignment for the largest scan's minimum step count gives
+    # the smaller scans smaller blocks than the largest one
+    t = math.ceil(max(hist) / sizes[-1])
+    bal = tuple(next((b for b in sizes if math.ceil(h / b) <= t), sizes[-1]) for h in hist)
+    assert bal in cands and min(bal) < max(bal)
