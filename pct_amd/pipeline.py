@@ -88,8 +88,9 @@ class _StageTimer:
         torch = self.ctx.torch
         e0, e1 = torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)
         self.ctx.sync()
-        with torch.cuda.stream(self.ctx.stream):
-            torch.cuda._sleep(self.GATE_CYCLES)
+        if hasattr(torch.cuda, "_sleep"):
+            with torch.cuda.stream(self.ctx.stream):
+                torch.cuda._sleep(self.GATE_CYCLES)
         e0.record(self.ctx.stream)
         r = fn(*args)
         e1.record(self.ctx.stream)
