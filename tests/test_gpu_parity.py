@@ -236,3 +236,41 @@ def test_tuning_knobs_do_not_change_results(ctx_mini, mini_host):
     pct_amd.set_tuning(ctx_mini, overlap=0, chunk_first=0, chunk_max=0, mlp_waves=0)
     for o in out[1:]:
         assert np.array_equal(out[0], o)
+
+
+@pytest.mark.parametrize("columns, launch", [(256, "sequential"), (256, "beside"), (64, "sequential"),
+                                             (512, "sequential")])
+def test_mlp_lists_bit_exact_on_other_grids(columns, launch):
+    """MLP voxel lists against the oracle on grids whose per-depth tables take
+    the different launch shapes: 256 columns (a 40 KB table: three
+    workgroups per CU with the table in LDS, at the full occupancy and beside
+    the block loop), 64 columns (a 10 KB table, four workgroups) and 512
+    columns (an 81 KB table, read from global memory)"""
+    import oracle
+    import pct_amd
+    from pct_amd import configs
+    slices = 8 if columns > 256 else 16
+    p = configs.params("cfg2", columns=columns, rows=columns, slices=slices)
+    p.recon_cyl_height = slices * p.slice_thickness
+    p.max_intersections = max(p.max_intersections, 4 * columns)   # paths of ~2 columns of voxels
+    cfg = configs.synth_cfg("cfg2", n_histories=60_000, seed=configs.SEED_BASE + 40 + columns)
+    raw = pct_amd.generate_host(cfg).numpy()
+    h_all, missed = oracle.recon_volume_intersections(p, dict(raw))
+    h = h_all.select(missed == 0)
+    inter = oracle.mlp_endpoints(p, h, oracle.hull_init(p))
+    hr = h.select(inter == 1).select(np.arange(int(inter.sum())) < 6000)
+    ctx = pct_amd.Context(p)
+    d = _to_dev(hr)
+    if launch == "beside":
+        # the DROP plan's lists come from the single-pass MLP; a recomputing
+        # iteration walks them again beside the block loop and checks them
+        # against the plan's hash (an error on any difference)
+        x = ctx.torch.zeros(ctx.V, dtype=ctx.torch.float32, device=ctx.device)
+        ctx.drop_iteration(d, x, 1000, 1e-3)
+        ctx.drop_iteration(d, x, 1000, 1e-3)
+    offs, vox, lens = ctx.mlp_paths(d)
+    offs, vox, lens = offs.cpu().numpy(), vox.cpu().numpy().view(np.uint32), lens.cpu().numpy()
+    exp = oracle.mlp_paths(p, hr)
+    bad = sum(not np.array_equal(vox[offs[k]:offs[k] + lens[k]], e) for k, e in enumerate(exp))
+    assert bad == 0, f"{bad} of {len(exp)} voxel lists differ"
+    ctx.close()
