@@ -268,9 +268,11 @@ def test_mlp_lists_bit_exact_on_other_grids(columns, launch):
         x = ctx.torch.zeros(ctx.V, dtype=ctx.torch.float32, device=ctx.device)
         ctx.drop_iteration(d, x, 1000, 1e-3)
         ctx.drop_iteration(d, x, 1000, 1e-3)
-    offs, vox, lens = ctx.mlp_paths(d)
-    offs, vox, lens = offs.cpu().numpy(), vox.cpu().numpy().view(np.uint32), lens.cpu().numpy()
-    exp = oracle.mlp_paths(p, hr)
-    bad = sum(not np.array_equal(vox[offs[k]:offs[k] + lens[k]], e) for k, e in enumerate(exp))
-    assert bad == 0, f"{bad} of {len(exp)} voxel lists differ"
+    for table in (False, True):            # exact walk and precision form
+        mode = pct_amd.PCT_MLP_TABLE if table else pct_amd.PCT_MLP_FP64
+        offs, vox, lens = ctx.mlp_paths(d, mode=mode)
+        offs, vox, lens = offs.cpu().numpy(), vox.cpu().numpy().view(np.uint32), lens.cpu().numpy()
+        exp = oracle.mlp_paths(p, hr, table=table)
+        bad = sum(not np.array_equal(vox[offs[k]:offs[k] + lens[k]], e) for k, e in enumerate(exp))
+        assert bad == 0, f"{bad} of {len(exp)} voxel lists differ (table={table})"
     ctx.close()
