@@ -96,8 +96,20 @@ class PctNetCalib(ctypes.Structure):
                 ("cell_fallback", ctypes.c_uint8 * NET_CELLS)]
 
 
+PCTD_STR = 256
+
+
+class PctPctdHeader(ctypes.Structure):
+    _fields_ = [("version", c_i32), ("n_histories", c_i32), ("projection_angle", c_f32), ("beam_energy", c_f32),
+                ("generation_date", c_i32), ("preprocess_date", c_i32),
+                ("phantom_name", ctypes.c_char * PCTD_STR), ("data_source", ctypes.c_char * PCTD_STR),
+                ("prepared_by", ctypes.c_char * PCTD_STR), ("data_offset", c_i64)]
+
+
+PCT_TXT_F32, PCT_TXT_F64, PCT_TXT_I32, PCT_TXT_U32, PCT_TXT_BOOL = 0, 1, 2, 3, 4
+
 # name -> (restype, argtypes) of every exported entry point of include/pct.h,
-# include/pct_net.h and include/pct_synth_api.h
+# include/pct_net.h, include/pct_synth_api.h and include/pct_io.h
 P = ctypes.POINTER
 SIGNATURES = {
     "pct_create": (c_vp, [P(PctParams), c_vp]),
@@ -146,6 +158,12 @@ SIGNATURES = {
     "pct_net_fallback_stats": (ctypes.c_int, [c_vp, P(c_i64), P(c_i64)]),
     "pct_synth_generate": (ctypes.c_int, [c_vp, P(PctSynthCfg), c_i64, c_i64, P(PctSynthSoa)]),
     "pct_synth_generate_host": (ctypes.c_int, [P(PctSynthCfg), c_i64, c_i64, P(PctSynthSoa)]),
+    "pct_pctd_read_header": (ctypes.c_int, [ctypes.c_char_p, P(PctPctdHeader)]),
+    "pct_pctd_read": (ctypes.c_int, [ctypes.c_char_p, ctypes.c_int, c_i64, P(PctSynthSoa), P(c_i64)]),
+    "pct_pctd_write": (ctypes.c_int, [ctypes.c_char_p, P(PctPctdHeader), P(PctRawSoa)]),
+    "pct_array_to_disk": (ctypes.c_int, [ctypes.c_char_p, ctypes.c_char_p, ctypes.c_char_p, c_vp, ctypes.c_int,
+                                         ctypes.c_int, ctypes.c_int, ctypes.c_int, c_i64, ctypes.c_int]),
+    "pct_io_last_error": (ctypes.c_char_p, []),
 }
 
 PCT_MLP_FP64 = 0

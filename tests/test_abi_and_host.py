@@ -49,22 +49,26 @@ def test_struct_layout_matches_c(tmp_path):
 #include "{ROOT}/include/pct.h"
 #include "{ROOT}/include/pct_synth_api.h"
 #include "{ROOT}/include/pct_net.h"
+#include "{ROOT}/include/pct_io.h"
 long v[] = {{ sizeof(pct_params), offsetof(pct_params, a), offsetof(pct_params, max_intersections),
              offsetof(pct_params, lambda), sizeof(pct_hist_soa), sizeof(pct_raw_soa), sizeof(pct_synth_cfg),
              offsetof(pct_synth_cfg, cyl_rsp), sizeof(pct_synth_soa), sizeof(pct_net_train_cfg),
              offsetof(pct_net_train_cfg, seed), sizeof(pct_net_calib), offsetof(pct_net_calib, cell_samples),
              offsetof(pct_net_calib, cell_p99), offsetof(pct_net_calib, cell_fallback),
-             PCT_NET_PARAMS, PCT_NET_CELLS, PCT_MLP_SURROGATE }};
+             PCT_NET_PARAMS, PCT_NET_CELLS, PCT_MLP_SURROGATE, sizeof(pct_pctd_header),
+             offsetof(pct_pctd_header, prepared_by), offsetof(pct_pctd_header, data_offset), PCT_TXT_BOOL, PCT_E_IO }};
 """)
     so = tmp_path / "probe.so"
     subprocess.run(["gcc", "-shared", "-fPIC", "-o", str(so), str(src)], check=True)
-    v = (ctypes.c_long * 18).in_dll(ctypes.CDLL(str(so)), "v")
+    v = (ctypes.c_long * 23).in_dll(ctypes.CDLL(str(so)), "v")
     exp = [ctypes.sizeof(_lib.PctParams), _lib.PctParams.a.offset, _lib.PctParams.max_intersections.offset,
            _lib.PctParams.lambda_.offset, ctypes.sizeof(_lib.PctHistSoa), ctypes.sizeof(_lib.PctRawSoa),
            ctypes.sizeof(_lib.PctSynthCfg), _lib.PctSynthCfg.cyl_rsp.offset, ctypes.sizeof(_lib.PctSynthSoa),
            ctypes.sizeof(_lib.PctNetTrainCfg), _lib.PctNetTrainCfg.seed.offset, ctypes.sizeof(_lib.PctNetCalib),
            _lib.PctNetCalib.cell_samples.offset, _lib.PctNetCalib.cell_p99.offset,
-           _lib.PctNetCalib.cell_fallback.offset, _lib.NET_PARAMS, _lib.NET_CELLS, _lib.PCT_MLP_SURROGATE]
+           _lib.PctNetCalib.cell_fallback.offset, _lib.NET_PARAMS, _lib.NET_CELLS, _lib.PCT_MLP_SURROGATE,
+           ctypes.sizeof(_lib.PctPctdHeader), _lib.PctPctdHeader.prepared_by.offset,
+           _lib.PctPctdHeader.data_offset.offset, _lib.PCT_TXT_BOOL, -5]
     assert list(v) == exp
 
 

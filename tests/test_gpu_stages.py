@@ -154,3 +154,31 @@ def test_timed_pipeline_gives_the_same_results():
     for f in ga:
         assert np.array_equal(np.asarray(ga[f]).view(np.uint32), np.asarray(gb[f]).view(np.uint32)), f
     ctx.close()
+
+
+def test_pipeline_from_pctd_files(tmp_path):
+    """a scan stored as PCTD projection files (mm, one file per gantry angle)
+    and read back the reference's way, then preprocessed on the GPU: equal to
+    the oracle's preprocessing of the same arrays; the x0 text dump equals the
+    dump of the oracle's x0"""
+    import oracle
+    import pct_amd
+    from pct_amd import configs, pctd, pipeline
+    p = configs.params("cfg1")
+    cfg = configs.synth_cfg("cfg1", n_histories=200_000)
+    pctd.write_scan(pct_amd.generate_host(cfg).numpy(), str(tmp_path), "/in", phantom_name="cfg1")
+    host = pctd.read_scan(str(tmp_path), "/in")
+    recon, x0, hull, fbp = oracle.preprocess(p, host)
+    ctx = pct_amd.Context(p)
+    raw = pctd.read_scan(str(tmp_path), "/in", device=ctx.device)
+    pre = pipeline.preprocess(ctx, raw)
+    assert pre.hist.n == recon.n > 0
+    assert np.array_equal(pre.x0.cpu().numpy().view(np.uint32), x0.view(np.uint32))
+    got = pre.hist.numpy()
+    for f in ("x_entry", "z_exit", "wepl", "first_voxel", "bin_num"):
+        assert np.array_equal(np.asarray(got[f]).view(np.uint32), np.asarray(recon.arrays()[f]).view(np.uint32)), f
+    d = str(tmp_path) + "/"
+    pctd.array_to_disk(pre.x0, "x0_gpu", d, "out", p.columns, p.rows, p.slices)
+    pctd.array_to_disk(x0, "x0_oracle", d, "out", p.columns, p.rows, p.slices)
+    assert open(d + "out/x0_gpu.txt").read() == open(d + "out/x0_oracle.txt").read()
+    ctx.close()
