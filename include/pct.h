@@ -52,6 +52,9 @@ typedef struct pct_params {
     int32_t drop_block_size;                 /* DROP_BLOCK_SIZE / BLOCK_SIZE_RECON (data.h:377-378) */
     float   lambda;                          /* LAMBDA (data.h:374)                             */
     int32_t iterations;                      /* ITERATIONS (cfg.h:5)                            */
+    double  eta;                             /* ETA of the robust DROP variants (data.h:369,
+                                                command line .cu:329-335)                      */
+    int32_t psi_sign;                        /* PSI_SIGN (data.h:371)                           */
 } pct_params;
 
 /* Raw tracker data of one read chunk (data.h:497-508): 4 tracker planes in the
@@ -100,6 +103,15 @@ typedef struct pct_hist_soa {
 #define PCT_DROP_F32       0x800 /* precision form of the update: a_i.x, the
                                     residual and the per-voxel sums in FP32,
                                     sums Kahan-compensated                      */
+/* The reference's research variants of the FP64 update (CPU functions there),
+ * with psi_v = ETA and PSI_SIGN applied to (1 - x_v) at the block's start: */
+#define PCT_DROP_ROBUST    0x1000 /* DROP_blocks_red + DROP_update (.cu:8005-8055):
+                                    every path entry adds
+                                    L lambda r_i / (a_i.a_i + psi_v)          */
+#define PCT_DROP_ROBUST2   0x2000 /* DROP_blocks_robust2 + DROP_update_robust2
+                                    (.cu:7871-7901): the plain update scaled per
+                                    voxel by N_v / (N_v + psi_v), N_v the sum of
+                                    the block's L_i^2 through v                */
 
 typedef struct pct_ctx pct_ctx;
 

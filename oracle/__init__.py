@@ -43,6 +43,8 @@ def lib():
                                   ctypes.c_int64]
         L.oracle_drop_mt.argtypes = [P(abi.PctParams), P(abi.PctHistSoa), vp, ctypes.c_int, ctypes.c_float,
                                      ctypes.c_int, ctypes.c_int64, ctypes.c_int]
+        L.oracle_drop_variant_mt.argtypes = [P(abi.PctParams), P(abi.PctHistSoa), vp, ctypes.c_int, ctypes.c_float,
+                                     ctypes.c_int, ctypes.c_int64, ctypes.c_int, ctypes.c_int]
         L.oracle_mlp_trace.argtypes = [P(abi.PctParams), P(abi.PctHistSoa), ctypes.c_int64, vp, vp, vp, vp, vp,
                                        ctypes.c_int, P(ctypes.c_int), ctypes.c_int]
         L.oracle_drop_lists.argtypes = [ctypes.c_int64, ctypes.c_int64, vp, vp, vp, vp, vp, ctypes.c_int,
@@ -315,16 +317,18 @@ def drop_lists(V, lists, chord, b, x0, block_size, lam, iterations=1):
 
 
 def drop(p, h: HostHistories, x0: np.ndarray, iterations=1, block_size=None, lam=None, max_histories=-1,
-         threads=1):
-    """Plain DROP (E6 semantics) from x0; returns the new float32 image. The
-    result does not depend on `threads` (paths in parallel, accumulation in
-    storage order)."""
+         threads=1, variant=0):
+    """DROP (E6 semantics) from x0; returns the new float32 image. variant 1
+    is DROP_blocks_red + DROP_update, 2 DROP_blocks_robust2 +
+    DROP_update_robust2 (ETA = p.eta, PSI_SIGN = p.psi_sign). The result does
+    not depend on `threads` (paths in parallel, accumulation in storage
+    order)."""
     x = np.ascontiguousarray(x0, dtype=np.float32).copy()
     block_size = p.drop_block_size if block_size is None else block_size
     lam = p.lambda_ if lam is None else lam
     hs = h.soa()
-    rc = lib().oracle_drop_mt(ctypes.byref(p), ctypes.byref(hs), _p(x), int(block_size), float(lam), int(iterations),
-                              int(max_histories), int(threads))
+    rc = lib().oracle_drop_variant_mt(ctypes.byref(p), ctypes.byref(hs), _p(x), int(block_size), float(lam),
+                                      int(iterations), int(max_histories), int(threads), int(variant))
     if rc != 0:
         raise OverflowError("MLP path overflow in the oracle DROP")
     return x

@@ -760,9 +760,12 @@ int oracle_drop_lists(int64_t V, int64_t n_lists, const int64_t* offsets, const 
 
 /* E6 + E4 semantics: `iterations` DROP iterations over all histories in
  * storage order with blocks of `block_size`; x is updated in place.
- * x_update is double (data.h:558) and counts unsigned (DROP_blocks). */
-int oracle_drop_mt(const pct_params* p, const pct_hist_soa* h, float* x, int block_size, float lambda,
-                   int iterations, int64_t max_histories, int threads) {
+ * x_update is double (data.h:558) and counts unsigned (DROP_blocks).
+ * variant 1: DROP_blocks_red + DROP_update (.cu:8005-8055), variant 2:
+ * DROP_blocks_robust2 + DROP_update_robust2 (.cu:7871-7901), with
+ * ETA = p->eta and PSI_SIGN = p->psi_sign. */
+int oracle_drop_variant_mt(const pct_params* p, const pct_hist_soa* h, float* x, int block_size, float lambda,
+                           int iterations, int64_t max_histories, int threads, int variant) {
     /* Per block the histories' paths and update values are independent of
      * each other (they read x as it was at the start of the block), so they
      * may be computed by several threads; the accumulation into x_update and
@@ -771,12 +774,16 @@ int oracle_drop_mt(const pct_params* p, const pct_hist_soa* h, float* x, int blo
      * depend on the thread count. */
     Geo g = make_geo(*p);
     if (threads < 1) threads = 1;
+    const double ETA = p->eta;
+    const int PSI_SIGN = p->psi_sign;
+    const double LAMBDA = lambda;
     std::vector<double> x_update(g.V, 0.0);
     std::vector<uint32_t> counts(g.V, 0);
+    std::vector<double> norm_Ai(variant == 2 ? g.V : 0, 0.0);
     const int stride = g.max_intersections + 1;
     std::vector<uint32_t> paths((size_t)block_size * stride);
     std::vector<int> lens(block_size);
-    std::vector<double> upd(block_size);
+    std::vector<double> upd(block_size), res(block_size), aa(block_size), chord(block_size);
     std::vector<uint32_t> touched;
     int64_t n_hist = h->n;
     if (max_histories >= 0 && max_histories < n_hist) n_hist = max_histories;
@@ -798,10 +805,21 @@ int oracle_drop_mt(const pct_params* p, const pct_hist_soa* h, float* x, int blo
                                                   ((double)h->xz_entry_angle[i] + (double)h->xz_exit_angle[i]) / 2.0);
                 double sum = 0.0;
                 for (int j = 0; j < n; ++j) sum += x[path[j]];
-                double a_i_dot_x_k = sum * L;
-                double a_i_dot_a_i = (L * L) * n;
                 double b_i = h->wepl[i];
-                upd[k] = L * ((b_i - a_i_dot_x_k) / a_i_dot_a_i) * lambda;
+                if (variant == 0) {
+                    double a_i_dot_x_k = sum * L;
+                    double a_i_dot_a_i = (L * L) * n;
+                    upd[k] = L * ((b_i - a_i_dot_x_k) / a_i_dot_a_i) * lambda;
+                } else {
+                    /* scalar_dot_product2 (.cu:7686): scalar * sum; pow(L, 2.0)
+                     * taken as the square rounded once, L * L */
+                    double a_i_dot_x_k = L * sum;
+                    double a_i_dot_a_i = (L * L) * n;
+                    chord[k] = L;
+                    res[k] = b_i - a_i_dot_x_k;
+                    aa[k] = a_i_dot_a_i;
+                    upd[k] = L * LAMBDA * (res[k] / a_i_dot_a_i);   /* robust2's update_value */
+                }
             }
             touched.clear();
             for (int64_t k = 0; k < cnt; ++k) {
@@ -811,19 +829,36 @@ int oracle_drop_mt(const pct_params* p, const pct_hist_soa* h, float* x, int blo
                 for (int j = 0; j < n; ++j) {
                     uint32_t v = path[j];
                     if (counts[v] == 0) touched.push_back(v);
-                    x_update[v] += upd[k];
+                    if (variant == 1) {
+                        double psi_i = ((1.0 - x[v]) * ETA) * PSI_SIGN;
+                        x_update[v] += chord[k] * LAMBDA * (res[k] / (aa[k] + psi_i));
+                    } else {
+                        x_update[v] += upd[k];
+                    }
+                    if (variant == 2) norm_Ai[v] += chord[k] * chord[k];
                     counts[v] += 1;
                 }
             }
             /* DROP_update restricted to touched voxels (equal to the full scan) */
             for (uint32_t v : touched) {
-                x[v] = (float)(x[v] + x_update[v] / counts[v]);
+                if (variant == 2) {
+                    double psi_i = PSI_SIGN * (1 - x[v]) * ETA;
+                    x[v] += (norm_Ai[v] / (norm_Ai[v] + psi_i)) * x_update[v] / counts[v];
+                    norm_Ai[v] = 0.0;
+                } else {
+                    x[v] = (float)(x[v] + x_update[v] / counts[v]);
+                }
                 x_update[v] = 0.0;
                 counts[v] = 0;
             }
         }
     }
     return rc;
+}
+
+int oracle_drop_mt(const pct_params* p, const pct_hist_soa* h, float* x, int block_size, float lambda,
+                   int iterations, int64_t max_histories, int threads) {
+    return oracle_drop_variant_mt(p, h, x, block_size, lambda, iterations, max_histories, threads, 0);
 }
 
 int oracle_drop(const pct_params* p, const pct_hist_soa* h, float* x, int block_size, float lambda,

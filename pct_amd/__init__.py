@@ -10,12 +10,12 @@ import ctypes
 
 from . import _lib
 from . import configs
-from ._lib import (PCT_DROP_ATOMIC, PCT_DROP_CACHED, PCT_DROP_F32, PCT_DROP_NO_GRAPH, PCT_LEARNED, PCT_MLP_FP64,
-                   PCT_MLP_SURROGATE, PCT_MLP_TABLE, PCT_PRECISION, PctHistSoa, PctParams, PctRawSoa, PctSynthCfg,
-                   PctSynthSoa)
+from ._lib import (PCT_DROP_ATOMIC, PCT_DROP_CACHED, PCT_DROP_F32, PCT_DROP_NO_GRAPH, PCT_DROP_ROBUST,
+                   PCT_DROP_ROBUST2, PCT_LEARNED, PCT_MLP_FP64, PCT_MLP_SURROGATE, PCT_MLP_TABLE, PCT_PRECISION,
+                   PctHistSoa, PctParams, PctRawSoa, PctSynthCfg, PctSynthSoa)
 
 __all__ = ["Context", "Batch", "Histories", "RawScan", "configs", "PCT_MLP_FP64", "PCT_MLP_TABLE", "PCT_MLP_SURROGATE",
-           "PCT_PRECISION", "PCT_LEARNED", "generate_host"]
+           "PCT_PRECISION", "PCT_LEARNED", "PCT_DROP_ROBUST", "PCT_DROP_ROBUST2", "generate_host"]
 
 
 class PctError(RuntimeError):

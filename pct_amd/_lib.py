@@ -26,6 +26,7 @@ class PctParams(ctypes.Structure):
         ("hull_filter_threshold", c_f64), ("hull_filter_radius", c_i32), ("msc_diff_thresh", c_i32),
         ("e_0", c_f64), ("x_0", c_f64), ("a", c_f64 * 6),
         ("max_intersections", c_i32), ("drop_block_size", c_i32), ("lambda_", c_f32), ("iterations", c_i32),
+        ("eta", c_f64), ("psi_sign", c_i32),
     ]
 
     def copy(self):
@@ -174,6 +175,8 @@ PCT_DROP_CACHED = 0x100
 PCT_DROP_ATOMIC = 0x200
 PCT_DROP_NO_GRAPH = 0x400
 PCT_DROP_F32 = 0x800
+PCT_DROP_ROBUST = 0x1000
+PCT_DROP_ROBUST2 = 0x2000
 # pct_set_tuning keys
 PCT_TUNE_MLP_WAVES = 1
 PCT_TUNE_CHUNK_FIRST = 2

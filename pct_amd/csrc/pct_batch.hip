@@ -146,6 +146,8 @@ static int check_batch_args(pct_batch* b, const pct_hist_soa* hists, const int* 
     for (pct_ctx* c : b->scans)
         if (!pct_mlp_mode_ok(c, mlp)) return pct_fail(c0, PCT_E_INVALID, "%s: MLP mode %d not available", who, mlp);
     if (mode & PCT_DROP_ATOMIC) return pct_fail(c0, PCT_E_INVALID, "%s: no atomic form in a batch", who);
+    if (mode & (PCT_DROP_ROBUST | PCT_DROP_ROBUST2))
+        return pct_fail(c0, PCT_E_INVALID, "%s: the robust updates run per scan (pct_drop_iteration)", who);
     for (size_t s = 0; s < b->scans.size(); ++s) {
         const pct_hist_soa& h = hists[s];
         if (!h.wepl || !h.first_voxel || !h.x_entry || h.n <= 0)

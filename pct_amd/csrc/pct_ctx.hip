@@ -51,6 +51,8 @@ extern "C" void pct_default_params(pct_params* p) {
     p->drop_block_size = 3200;
     p->lambda = 0.001f;
     p->iterations = 12;
+    p->eta = 2.5;
+    p->psi_sign = 1;
 }
 
 pct_geo pct_make_geo(const pct_params& p) {

@@ -47,6 +47,9 @@ struct pct_plan {
     const float* graph_wepl = nullptr;
     double graph_lambda = 0.0;
     bool graph_f32 = false;
+    int graph_variant = 0;
+    double graph_eta = 0.0;
+    int graph_psi = 0;
 };
 
 void pct_plan_destroy(pct_plan* pl);

@@ -221,6 +221,29 @@ k_block_apply_f32(const uint64_t* __restrict__ segs, int64_t s0, int64_t ns, con
     pct_blk::apply_f32(segs[s0 + k], csc, delta, x);
 }
 
+/* the research variants of the FP64 update (pct_plan_kernels.h) */
+template <int V>
+__global__ void __launch_bounds__(64 * DELTA_WAVES)
+k_block_delta_var(const uint32_t* __restrict__ list, int stride, const uint32_t* __restrict__ len,
+                  const double* __restrict__ chord, const float* __restrict__ wepl, int64_t h0, int64_t count,
+                  const float* __restrict__ x, double lambda, double* __restrict__ out, int64_t B) {
+    __builtin_amdgcn_s_setprio(3);
+    const int64_t k = (int64_t)blockIdx.x * DELTA_WAVES + (threadIdx.x >> 6);
+    if (k >= count) return;
+    pct_blk::delta_variant(V, list, stride, len, chord, wepl, h0, k, x, lambda, out, B, threadIdx.x & 63);
+}
+
+template <int V>
+__global__ void __launch_bounds__(256)
+k_block_apply_var(const uint64_t* __restrict__ segs, int64_t s0, int64_t ns, const uint16_t* __restrict__ csc,
+                  const double* __restrict__ out, int64_t B, const double* __restrict__ chord_b, double lambda,
+                  double eta, int psi_sign, float* __restrict__ x) {
+    __builtin_amdgcn_s_setprio(3);
+    int64_t k = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (k >= ns) return;
+    pct_blk::apply_variant<V>(segs[s0 + k], csc, out, B, chord_b, lambda, eta, psi_sign, x);
+}
+
 }  // namespace
 
 /* ------------------------------------------------------------------------- */
@@ -447,7 +470,7 @@ static int plan_build(pct_ctx* ctx, const pct_hist_soa* hist, int B, int mlp_mod
     PCT_HIP_CHECK(ctx, hipMemcpyAsync(pl->blk_seg_h.data(), pl->blk_seg.ptr, sizeof(int64_t) * (nb + 1),
                                       hipMemcpyDeviceToHost, ctx->stream));
     PCT_HIP_CHECK(ctx, hipStreamSynchronize(ctx->stream));
-    if ((rc = pct_ensure(ctx, pl->delta, sizeof(double) * B))) return rc;
+    if ((rc = pct_ensure(ctx, pl->delta, sizeof(double) * 2 * (size_t)B))) return rc;   /* 2 per history: ROBUST */
     pl->n = n;
     pl->B = B;
     pl->nb = nb;
@@ -483,8 +506,14 @@ bool pct_plan_ready(const pct_ctx* ctx, const pct_hist_soa* hist, int B, int mlp
 }
 
 /* the block loop over blocks [b_begin, b_end) on the plan */
+/* update variant of a pct_drop_iteration mode: 0 plain, 1 PCT_DROP_ROBUST,
+ * 2 PCT_DROP_ROBUST2 */
+static int update_variant(int mode) {
+    return (mode & PCT_DROP_ROBUST) ? 1 : (mode & PCT_DROP_ROBUST2) ? 2 : 0;
+}
+
 static int plan_block_loop(pct_ctx* ctx, pct_plan* pl, const pct_hist_soa* hist, float* x, double lambda, bool f32,
-                           int64_t b_begin, int64_t b_end) {
+                           int variant, int64_t b_begin, int64_t b_end) {
     const int64_t B = pl->B;
     const uint32_t* list = (const uint32_t*)pl->list.ptr;
     const uint32_t* len = (const uint32_t*)pl->len.ptr;
@@ -497,6 +526,26 @@ static int plan_block_loop(pct_ctx* ctx, pct_plan* pl, const pct_hist_soa* hist,
         const uint32_t* lb = list + pl->list_base_h[b];
         int stride = pl->stride_h[b];
         dim3 dgrid((unsigned)((cnt + DELTA_WAVES - 1) / DELTA_WAVES));
+        if (variant != 0) {
+            const double eta = ctx->p.eta;
+            const int psi = ctx->p.psi_sign;
+            double* out = (double*)delta;
+            if (variant == 1)
+                hipLaunchKernelGGL(k_block_delta_var<1>, dgrid, dim3(64 * DELTA_WAVES), 0, ctx->stream, lb, stride, len,
+                                   chord, (const float*)hist->wepl, h0, cnt, (const float*)x, lambda, out, B);
+            else
+                hipLaunchKernelGGL(k_block_delta_var<2>, dgrid, dim3(64 * DELTA_WAVES), 0, ctx->stream, lb, stride, len,
+                                   chord, (const float*)hist->wepl, h0, cnt, (const float*)x, lambda, out, B);
+            const int64_t s0 = pl->blk_seg_h[b], ns = pl->blk_seg_h[b + 1] - s0;
+            const uint16_t* cb = csc + pl->blk_entry_h[b];
+            if (ns > 0 && variant == 1)
+                hipLaunchKernelGGL(k_block_apply_var<1>, dim3(pct_grid(ns, 256)), dim3(256), 0, ctx->stream, segs, s0, ns,
+                                   cb, (const double*)out, B, chord + h0, lambda, eta, psi, x);
+            else if (ns > 0)
+                hipLaunchKernelGGL(k_block_apply_var<2>, dim3(pct_grid(ns, 256)), dim3(256), 0, ctx->stream, segs, s0, ns,
+                                   cb, (const double*)out, B, chord + h0, lambda, eta, psi, x);
+            continue;
+        }
         if (f32)
             hipLaunchKernelGGL(k_block_delta_f32, dgrid, dim3(64 * DELTA_WAVES), 0, ctx->stream, lb, stride, len, chord,
                                (const float*)hist->wepl, h0, cnt, (const float*)x, (float)lambda, (float*)delta);
@@ -531,17 +580,18 @@ static void plan_chunks(pct_ctx* ctx, pct_plan* pl) {
 }
 
 static int plan_ensure_graphs(pct_ctx* ctx, pct_plan* pl, const pct_hist_soa* hist, float* x, double lambda,
-                              bool f32) {
+                              bool f32, int variant) {
     const int64_t nchunks = (int64_t)pl->chunk_b.size() - 1;
     if ((int64_t)pl->graphs.size() == nchunks && pl->graph_x == x && pl->graph_lambda == lambda &&
-        pl->graph_f32 == f32 && pl->graph_wepl == hist->wepl)
+        pl->graph_f32 == f32 && pl->graph_wepl == hist->wepl && pl->graph_variant == variant &&
+        (variant == 0 || (pl->graph_eta == ctx->p.eta && pl->graph_psi == ctx->p.psi_sign)))
         return PCT_OK;
     plan_release_graph(pl);
     for (int64_t c = 0; c < nchunks; ++c) {
         int64_t b0 = pl->chunk_b[c], b1 = pl->chunk_b[c + 1];
         hipGraph_t g = nullptr;
         PCT_HIP_CHECK(ctx, hipStreamBeginCapture(ctx->stream, hipStreamCaptureModeThreadLocal));
-        int rc = plan_block_loop(ctx, pl, hist, x, lambda, f32, b0, b1);
+        int rc = plan_block_loop(ctx, pl, hist, x, lambda, f32, variant, b0, b1);
         hipError_t e = hipStreamEndCapture(ctx->stream, &g);
         if (rc) { if (g) (void)hipGraphDestroy(g); plan_release_graph(pl); return rc; }
         if (e != hipSuccess) { plan_release_graph(pl); return pct_fail(ctx, PCT_E_HIP, "graph capture: %s", hipGetErrorString(e)); }
@@ -555,6 +605,9 @@ static int plan_ensure_graphs(pct_ctx* ctx, pct_plan* pl, const pct_hist_soa* hi
     pl->graph_lambda = lambda;
     pl->graph_f32 = f32;
     pl->graph_wepl = hist->wepl;
+    pl->graph_variant = variant;
+    pl->graph_eta = ctx->p.eta;
+    pl->graph_psi = ctx->p.psi_sign;
     return PCT_OK;
 }
 
@@ -575,12 +628,12 @@ static int plan_ensure_events(pct_ctx* ctx, pct_plan* pl, int64_t nchunks) {
  * loop, which takes chunk c as soon as its lists are done: the FP64-bound MLP
  * of later chunks runs beside the latency-bound block loop of earlier ones. */
 static int plan_iterate(pct_ctx* ctx, pct_plan* pl, const pct_hist_soa* hist, float* x, double lambda, bool f32,
-                        int mlp, bool recompute, bool eager) {
+                        int variant, int mlp, bool recompute, bool eager) {
     const bool sequential = recompute && ctx->tune_overlap == 2;
     int rc;
     plan_chunks(ctx, pl);
     const int64_t nchunks = (int64_t)pl->chunk_b.size() - 1;
-    if (!eager && (rc = plan_ensure_graphs(ctx, pl, hist, x, lambda, f32))) return rc;
+    if (!eager && (rc = plan_ensure_graphs(ctx, pl, hist, x, lambda, f32, variant))) return rc;
     if (ctx->profile) PCT_HIP_CHECK(ctx, hipEventRecord(ctx->ev[0], ctx->stream));
     if (recompute && ctx->tune_overlap == 2) {
         /* sequential: all lists first, then the block loop */
@@ -617,7 +670,7 @@ static int plan_iterate(pct_ctx* ctx, pct_plan* pl, const pct_hist_soa* hist, fl
         if (recompute) PCT_HIP_CHECK(ctx, hipStreamWaitEvent(ctx->stream, pl->ev_chunk[c], 0));
         if (eager) {
             int64_t b0 = pl->chunk_b[c], b1 = pl->chunk_b[c + 1];
-            if ((rc = plan_block_loop(ctx, pl, hist, x, lambda, f32, b0, b1))) return rc;
+            if ((rc = plan_block_loop(ctx, pl, hist, x, lambda, f32, variant, b0, b1))) return rc;
         } else {
             PCT_HIP_CHECK(ctx, hipGraphLaunch(pl->graphs[c], ctx->stream));
         }
@@ -648,6 +701,11 @@ extern "C" int pct_drop_iteration(pct_ctx* ctx, const pct_hist_soa* hist, float*
         return pct_fail(ctx, PCT_E_INVALID, "pct_drop_iteration: MLP mode %d not available", mlp);
     if (!x || block_size <= 0) return pct_fail(ctx, PCT_E_INVALID, "pct_drop_iteration: bad x or block size");
     if (hist->n <= 0) return PCT_OK;
+    const int variant = update_variant(mode);
+    if (variant != 0 && (mode & (PCT_DROP_F32 | PCT_DROP_ATOMIC | (variant == 1 ? PCT_DROP_ROBUST2 : 0))))
+        return pct_fail(ctx, PCT_E_INVALID,
+                        "pct_drop_iteration: the robust updates are FP64 plan forms (no PCT_DROP_F32 or ATOMIC, one "
+                        "variant at a time)");
     if (mode & PCT_DROP_ATOMIC) return pct_drop_iteration_atomic(ctx, hist, x, block_size, lambda, mlp);
     int rc;
     pct_plan* pl = ctx->plan;
@@ -661,7 +719,7 @@ extern "C" int pct_drop_iteration(pct_ctx* ctx, const pct_hist_soa* hist, float*
      * default mode, into the plan's list store (a list's capacity is its
      * block's stride), checking every list against the plan's hash */
     const bool recompute = !(mode & PCT_DROP_CACHED) && !fresh;
-    rc = plan_iterate(ctx, pl, hist, x, (double)lambda, (mode & PCT_DROP_F32) != 0, mlp, recompute,
+    rc = plan_iterate(ctx, pl, hist, x, (double)lambda, (mode & PCT_DROP_F32) != 0, variant, mlp, recompute,
                       (mode & PCT_DROP_NO_GRAPH) != 0);
     if (rc) return rc;
     int flags = 0;

@@ -115,6 +115,97 @@ __device__ __forceinline__ void delta_f32(const uint32_t* __restrict__ list, int
     }
 }
 
+/* The research variants of the FP64 update (the reference's CPU functions,
+ * with its expressions and operation order):
+ *   1 DROP_blocks_red + DROP_update (.cu:8005-8055): history k writes its
+ *     residual r = b - L sum x to out[k] and a.a = L^2 n to out[B + k];
+ *     voxel v adds, over the block's histories through it in history order,
+ *     (L lambda) (r / (a.a + psi_v)), psi_v = ((1 - x_v) ETA) PSI_SIGN, and
+ *     x_v += sum / count;
+ *   2 DROP_blocks_robust2 + DROP_update_robust2 (.cu:7871-7901): history k
+ *     writes (L lambda) (r / a.a) to out[k]; voxel v sums those and
+ *     N_v = sum L^2, and x_v += (N_v / (N_v + psi_v)) sum / count with
+ *     psi_v = PSI_SIGN (1 - x_v) ETA, the first product in float.
+ * x_v is the value at the block's start in both. */
+__device__ __forceinline__ void delta_variant(int variant, const uint32_t* __restrict__ list, int stride,
+                                              const uint32_t* __restrict__ len, const double* __restrict__ chord,
+                                              const float* __restrict__ wepl, int64_t h0, int64_t k,
+                                              const float* __restrict__ x, double lambda, double* __restrict__ out,
+                                              int64_t B, int lane) {
+    const int64_t h = h0 + k;
+    const uint32_t n = len[h];
+    const double L = chord[h];
+    const float b_i = wepl[h];
+    const uint32_t* p = list + k * stride;
+    uint32_t v[DELTA_SLOTS];
+#pragma unroll
+    for (int r = 0; r < DELTA_SLOTS; ++r) {
+        int j = lane + 64 * r;
+        v[r] = j < stride ? p[j] : LIST_SENTINEL;
+    }
+    double s = 0.0;
+#pragma unroll
+    for (int r = 0; r < DELTA_SLOTS; ++r)
+        if (v[r] != LIST_SENTINEL) s += (double)x[v[r]];
+    for (int j = lane + 64 * DELTA_SLOTS; j < stride; j += 64) {
+        uint32_t w = p[j];
+        if (w != LIST_SENTINEL) s += (double)x[w];
+    }
+    s = wave_sum_d(s);
+    if (lane == 0) {
+        const double a_i_dot_x_k = L * s;
+        const double a_i_dot_a_i = (L * L) * n;
+        const double residual = b_i - a_i_dot_x_k;
+        if (variant == 1) {
+            out[k] = residual;
+            out[B + k] = a_i_dot_a_i;
+        } else {
+            out[k] = (L * lambda) * (residual / a_i_dot_a_i);
+        }
+    }
+}
+
+/* the block-local history indices of segment record r, in history order */
+template <class F>
+__device__ __forceinline__ uint32_t for_each_history(uint64_t r, const uint16_t* __restrict__ csc, F f) {
+    const uint32_t c = (uint32_t)(r >> SEG_VOX_BITS) & 3u;
+    if (c != 0u) {
+        f((uint32_t)((r >> 24) & 0x1fffu));
+        if (c > 1u) f((uint32_t)((r >> 37) & 0x1fffu));
+        if (c > 2u) f((uint32_t)((r >> 50) & 0x1fffu));
+        return c;
+    }
+    const uint32_t cnt = (uint32_t)(r >> 51);
+    const uint16_t* q = csc + ((r >> 24) & 0x7ffffffu);
+    for (uint32_t i = 0; i < cnt; ++i) f((uint32_t)q[i]);
+    return cnt;
+}
+
+template <int VARIANT>
+__device__ __forceinline__ void apply_variant(uint64_t r, const uint16_t* __restrict__ csc,
+                                              const double* __restrict__ out, int64_t B,
+                                              const double* __restrict__ chord_b, double lambda, double eta,
+                                              int psi_sign, float* __restrict__ x) {
+    const uint32_t v = (uint32_t)(r & (SEG_MAX_V - 1));
+    const float xv = x[v];
+    double upd = 0.0;
+    if (VARIANT == 1) {
+        const double psi = ((1.0 - (double)xv) * eta) * (double)psi_sign;
+        const uint32_t cnt = for_each_history(r, csc, [&](uint32_t i) {
+            upd += (chord_b[i] * lambda) * (out[i] / (out[B + i] + psi));
+        });
+        x[v] = (float)((double)xv + upd / (double)cnt);
+    } else {
+        double norm = 0.0;
+        const uint32_t cnt = for_each_history(r, csc, [&](uint32_t i) {
+            upd += out[i];
+            norm += chord_b[i] * chord_b[i];
+        });
+        const double psi = (double)((float)psi_sign * (1.0f - xv)) * eta;
+        x[v] = (float)((double)xv + ((norm / (norm + psi)) * upd) / (double)cnt);
+    }
+}
+
 /* segment record r of a block; csc points at the block's first entry */
 __device__ __forceinline__ void apply_f64(uint64_t r, const uint16_t* __restrict__ csc,
                                           const double* __restrict__ delta, float* __restrict__ x) {

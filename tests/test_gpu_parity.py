@@ -276,3 +276,36 @@ def test_mlp_lists_bit_exact_on_other_grids(columns, launch):
         bad = sum(not np.array_equal(vox[offs[k]:offs[k] + lens[k]], e) for k, e in enumerate(exp))
         assert bad == 0, f"{bad} of {len(exp)} voxel lists differ (table={table})"
     ctx.close()
+
+
+@pytest.mark.parametrize("variant", [1, 2])
+def test_robust_drop_variants_match_oracle(ctx_mini, mini_host, variant):
+    """the research variants of the update (PCT_DROP_ROBUST: DROP_blocks_red,
+    PCT_DROP_ROBUST2: DROP_blocks_robust2 + DROP_update_robust2) against the
+    oracle's restatement, on an image that is not 0/1 so that psi_v varies;
+    invalid combinations are refused"""
+    import oracle
+    import pct_amd
+    torch = ctx_mini.torch
+    p = mini_host["p"]
+    assert (p.eta, p.psi_sign) == (2.5, 1)          # the reference's defaults (data.h:369-371)
+    hr = mini_host["hr"]
+    k = np.arange(mini_host["hull"].size)
+    x0 = (mini_host["hull"] * (1.0 + 0.3 * np.sin(0.37 * k))).astype(np.float32)
+    flag = pct_amd.PCT_DROP_ROBUST if variant == 1 else pct_amd.PCT_DROP_ROBUST2
+    lam = 1e-2
+    exp = oracle.drop(p, hr, x0, iterations=2, lam=lam, variant=variant)
+    plain = oracle.drop(p, hr, x0, iterations=2, lam=lam)
+    d = _to_dev(hr)
+    x = torch.from_numpy(x0.copy()).cuda()
+    ctx_mini.drop_iteration(d, x, lam=lam, mode=flag)
+    ctx_mini.drop_iteration(d, x, lam=lam, mode=flag)
+    got = x.cpu().numpy()
+    assert np.abs(exp - plain).max() > 100 * np.spacing(np.float32(1.0))   # the variant changes the image
+    assert np.max(np.abs(got - exp)) <= 8 * np.spacing(np.float32(2.0))
+    assert np.mean(got == exp) > 0.98
+    assert np.array_equal(got != x0, exp != x0)
+    for bad in (flag | pct_amd.PCT_DROP_F32, flag | pct_amd.PCT_DROP_ATOMIC,
+                pct_amd.PCT_DROP_ROBUST | pct_amd.PCT_DROP_ROBUST2):
+        with pytest.raises(pct_amd.PctError):
+            ctx_mini.drop_iteration(d, x, lam=lam, mode=bad)
