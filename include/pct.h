@@ -55,6 +55,8 @@ typedef struct pct_params {
     double  eta;                             /* ETA of the robust DROP variants (data.h:369,
                                                 command line .cu:329-335)                      */
     int32_t psi_sign;                        /* PSI_SIGN (data.h:371)                           */
+    double  sm_lower_threshold;              /* SM_LOWER_THRESHOLD [cm] (data.h:284)            */
+    double  sm_scale_threshold;              /* SM_SCALE_THRESHOLD (data.h:286)                 */
 } pct_params;
 
 /* Raw tracker data of one read chunk (data.h:497-508): 4 tracker planes in the
@@ -186,6 +188,21 @@ int pct_msc_count(pct_ctx* ctx, const pct_hist_soa* hist, const uint8_t* missed,
  * out of place (out != counts): out[v] = 0 where the in-slice 3x3 count drop
  * exceeds MSC_DIFF_THRESH or the voxel is on the cylinder rim, else 1. */
 int pct_msc_edge(pct_ctx* ctx, const int32_t* counts, int32_t* out);
+/* (c) Space modelling: replaces SM_GPU (.cu:3598-3710): counts start as
+ * pct_msc_init's cylinder (initialize_hull on the SM counts, .cu:3301) and are
+ * incremented for every walk voxel of each non-missed history with WEPL >=
+ * SM_LOWER_THRESHOLD (the protons that crossed the object). */
+int pct_sm_count(pct_ctx* ctx, const pct_hist_soa* hist, const uint8_t* missed, int32_t* counts);
+/* (c) SM edge detection: replaces SM_edge_detection (.cu:3711-3773) with
+ * carve_differences (.cu:3274-3292) and SM_edge_detection_GPU (.cu:3774-3789),
+ * out of place (out != counts). Per slice, the difference of a pixel is its
+ * largest in-slice 3x3 count drop (0 on the image border); the slice's
+ * threshold is the count of the first pixel, in pixel order, with the
+ * largest difference (0 when no difference is positive). out[v] = 1 where
+ * counts[v] > SM_SCALE_THRESHOLD x threshold and the voxel is inside the
+ * cylinder rim, else 0. thresholds (slices values; may be null) receives the
+ * per-slice thresholds. */
+int pct_sm_edge(pct_ctx* ctx, const int32_t* counts, int32_t* out, int32_t* thresholds);
 /* (c) Hull selection + averaging filter: replaces hull_selection (.cu:3954)
  * and averaging_filter_GPU (.cu:4007-4029): hull = counts != 0, then a voxel
  * is kept when its in-slice (2r+1)^2 neighbourhood sum exceeds

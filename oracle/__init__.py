@@ -53,9 +53,9 @@ def lib():
         L.oracle_bin_statistics.argtypes = [P(abi.PctParams), P(abi.PctHistSoa), vp, vp, vp, vp, vp, vp, vp, vp]
         L.oracle_statistical_cuts.argtypes = [P(abi.PctParams), P(abi.PctHistSoa), vp, vp, vp, vp, vp, vp, vp, vp,
                                               vp, vp]
-        for name, n in (("oracle_space_carve", 2), ("oracle_msc_count", 2)):
+        for name, n in (("oracle_space_carve", 2), ("oracle_msc_count", 2), ("oracle_sm_count", 2)):
             getattr(L, name).argtypes = [P(abi.PctParams), P(abi.PctHistSoa)] + [vp] * n
-        for name, n in (("oracle_msc_edge", 2), ("oracle_hull_select", 2), ("oracle_fbp_filter", 2),
+        for name, n in (("oracle_msc_edge", 2), ("oracle_sm_edge", 3), ("oracle_hull_select", 2), ("oracle_fbp_filter", 2),
                         ("oracle_backprojection", 2), ("oracle_fbp_hull", 2), ("oracle_initial_iterate", 3)):
             getattr(L, name).argtypes = [P(abi.PctParams)] + [vp] * n
         L.oracle_math.argtypes = [ctypes.c_int, vp, vp, vp, ctypes.c_int64]
@@ -197,6 +197,21 @@ def msc_edge(p, counts):
     return out
 
 
+def sm_counts(p, h: HostHistories, missed):
+    counts = hull_init(p).astype(np.int32)
+    hs = h.soa()
+    lib().oracle_sm_count(ctypes.byref(p), ctypes.byref(hs), _p(np.ascontiguousarray(missed, np.uint8)), _p(counts))
+    return counts
+
+
+def sm_edge(p, counts):
+    """(SM hull, per-slice thresholds)"""
+    out = np.zeros_like(counts, dtype=np.int32)
+    thresholds = np.zeros(p.slices, np.int32)
+    lib().oracle_sm_edge(ctypes.byref(p), _p(np.ascontiguousarray(counts, np.int32)), _p(out), _p(thresholds))
+    return out, thresholds
+
+
 def hull_select(p, counts):
     hull = np.zeros(counts.size, np.uint8)
     lib().oracle_hull_select(ctypes.byref(p), _p(np.ascontiguousarray(counts, np.int32)), _p(hull))
@@ -235,6 +250,8 @@ def preprocess(p, raw: dict, hull_kind="msc"):
     r = bin_stats_and_cuts(p, h, missed)
     if hull_kind == "msc":
         hull = hull_select(p, msc_edge(p, msc_counts(p, h, missed)))
+    elif hull_kind == "sm":
+        hull = hull_select(p, sm_edge(p, sm_counts(p, h, missed))[0])
     elif hull_kind == "sc":
         hull = space_carve(p, h, missed, hull_init(p))
     elif hull_kind == "cylinder":

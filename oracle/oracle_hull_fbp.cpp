@@ -160,6 +160,64 @@ int oracle_msc_count(const pct_params* p, const pct_hist_soa* h, const uint8_t* 
     return 0;
 }
 
+/* C3 (SM): SM_GPU over all histories (.cu:3608-3710): the same walk for the
+ * histories with WEPL >= SM_LOWER_THRESHOLD (.cu:3615) */
+int oracle_sm_count(const pct_params* p, const pct_hist_soa* h, const uint8_t* missed, int32_t* counts) {
+    Geo g = make_geo(*p);
+    for (int64_t i = 0; i < h->n; ++i) {
+        if (missed[i] || !(h->wepl[i] >= p->sm_lower_threshold)) continue;
+        carve_walk(g, h->x_entry[i], h->y_entry[i], h->z_entry[i], h->x_exit[i], h->y_exit[i], h->z_exit[i],
+                   [&](int64_t v) { if (v >= 0 && v < g.V) counts[v] += 1; });
+    }
+    return 0;
+}
+
+/* C4 (SM): SM_edge_detection (.cu:3711-3773): carve_differences (.cu:3274-3292)
+ * into a zeroed array, the per-slice threshold from an in-order scan with a
+ * strict > on the counts as they were before edge detection (the host copy
+ * written for WRITE_SM_COUNTS, .cu:3933-3937), then SM_edge_detection_GPU
+ * (.cu:3774-3789) out of place */
+int oracle_sm_edge(const pct_params* p, const int32_t* counts, int32_t* out, int32_t* thresholds) {
+    Geo g = make_geo(*p);
+    const int C = g.C, R = g.R;
+    std::vector<int32_t> differences(g.V, 0);
+    for (int slice = 0; slice < g.S; ++slice)
+        for (int row = 1; row < R - 1; ++row)
+            for (int column = 1; column < C - 1; ++column) {
+                int64_t voxel = column + (int64_t)row * C + (int64_t)slice * C * R;
+                int max_difference = 0;
+                for (int cr = row - 1; cr <= row + 1; ++cr)
+                    for (int cc = column - 1; cc <= column + 1; ++cc) {
+                        int d = counts[voxel] - counts[cc + (int64_t)cr * C + (int64_t)slice * C * R];
+                        if (d > max_difference) max_difference = d;
+                    }
+                differences[voxel] = max_difference;
+            }
+    double rr = g.radius - (g.voxel_width > g.voxel_height ? g.voxel_width : g.voxel_height) / 2;
+    for (int slice = 0; slice < g.S; ++slice) {
+        int threshold = 0, max_difference = 0;
+        for (int64_t pixel = 0; pixel < (int64_t)C * R; ++pixel) {
+            int64_t voxel = pixel + (int64_t)slice * C * R;
+            if (differences[voxel] > max_difference) {
+                max_difference = differences[voxel];
+                threshold = counts[voxel];
+            }
+        }
+        if (thresholds) thresholds[slice] = threshold;
+        for (int row = 0; row < R; ++row)
+            for (int column = 0; column < C; ++column) {
+                int64_t voxel = column + (int64_t)row * C + (int64_t)slice * C * R;
+                float x = (float)((column - C / 2 + 0.5) * g.voxel_width);
+                float y = (float)((R / 2 - row - 0.5) * g.voxel_height);
+                int v = counts[voxel] > p->sm_scale_threshold * threshold ? 1 : 0;
+                float s2 = x * x + y * y;
+                if ((double)s2 >= rr * rr) v = 0;
+                out[voxel] = v;
+            }
+    }
+    return 0;
+}
+
 /* C4: MSC_edge_detection_GPU, out of place */
 int oracle_msc_edge(const pct_params* p, const int32_t* counts, int32_t* out) {
     Geo g = make_geo(*p);

@@ -268,6 +268,22 @@ class Context:
         self._check(self.L.pct_msc_edge(self._ctx, _ptr(counts), _ptr(out)), "pct_msc_edge")
         return out
 
+    def sm_counts(self, hist, missed, counts=None):
+        """SM counts (initialized to the cylinder when counts is None)."""
+        if counts is None:
+            counts = self.torch.empty(self.V, dtype=self.torch.int32, device=self.device)
+            self._check(self.L.pct_msc_init(self._ctx, _ptr(counts)), "pct_msc_init")
+        hs = hist.soa()
+        self._check(self.L.pct_sm_count(self._ctx, ctypes.byref(hs), _ptr(missed), _ptr(counts)), "pct_sm_count")
+        return counts
+
+    def sm_edge(self, counts, thresholds=None):
+        """SM edge detection; with a tensor as `thresholds` (slices int32)
+        the per-slice thresholds are written into it."""
+        out = self.torch.empty_like(counts)
+        self._check(self.L.pct_sm_edge(self._ctx, _ptr(counts), _ptr(out), _ptr(thresholds)), "pct_sm_edge")
+        return out
+
     def hull_select(self, counts):
         bits = self.torch.empty((self.V + 31) // 32, dtype=self.torch.int32, device=self.device)
         self._check(self.L.pct_hull_select(self._ctx, _ptr(counts), _ptr(bits)), "pct_hull_select")

@@ -26,7 +26,7 @@ class PctParams(ctypes.Structure):
         ("hull_filter_threshold", c_f64), ("hull_filter_radius", c_i32), ("msc_diff_thresh", c_i32),
         ("e_0", c_f64), ("x_0", c_f64), ("a", c_f64 * 6),
         ("max_intersections", c_i32), ("drop_block_size", c_i32), ("lambda_", c_f32), ("iterations", c_i32),
-        ("eta", c_f64), ("psi_sign", c_i32),
+        ("eta", c_f64), ("psi_sign", c_i32), ("sm_lower_threshold", c_f64), ("sm_scale_threshold", c_f64),
     ]
 
     def copy(self):
@@ -124,6 +124,8 @@ SIGNATURES = {
     "pct_msc_init": (ctypes.c_int, [c_vp, c_vp]),
     "pct_msc_count": (ctypes.c_int, [c_vp, P(PctHistSoa), c_vp, c_vp]),
     "pct_msc_edge": (ctypes.c_int, [c_vp, c_vp, c_vp]),
+    "pct_sm_count": (ctypes.c_int, [c_vp, P(PctHistSoa), c_vp, c_vp]),
+    "pct_sm_edge": (ctypes.c_int, [c_vp, c_vp, c_vp, c_vp]),
     "pct_hull_select": (ctypes.c_int, [c_vp, c_vp, c_vp]),
     "pct_fbp_filter": (ctypes.c_int, [c_vp, c_vp, c_vp]),
     "pct_backproject": (ctypes.c_int, [c_vp, c_vp, c_vp]),

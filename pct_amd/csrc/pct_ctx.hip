@@ -53,6 +53,8 @@ extern "C" void pct_default_params(pct_params* p) {
     p->iterations = 12;
     p->eta = 2.5;
     p->psi_sign = 1;
+    p->sm_lower_threshold = 6.0;
+    p->sm_scale_threshold = 1.0;
 }
 
 pct_geo pct_make_geo(const pct_params& p) {
@@ -151,7 +153,7 @@ extern "C" pct_ctx* pct_create(const pct_params* params, void* stream) {
 extern "C" void pct_destroy(pct_ctx* ctx) {
     if (!ctx) return;
     if (ctx->stream) (void)hipStreamSynchronize(ctx->stream);
-    pct_buffer* bufs[] = {&ctx->scan_tmp, &ctx->steps, &ctx->offsets, &ctx->path_store, &ctx->path_len,
+    pct_buffer* bufs[] = {&ctx->hull_tmp, &ctx->scan_tmp, &ctx->steps, &ctx->offsets, &ctx->path_store, &ctx->path_len,
                           &ctx->chord, &ctx->xupd, &ctx->counts, &ctx->blk_offsets, &ctx->blk_voxels,
                           &ctx->blk_lengths, &ctx->blk_chord};
     for (pct_buffer* b : bufs)

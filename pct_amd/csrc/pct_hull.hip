@@ -5,7 +5,8 @@
  * Replaces SC_GPU / MSC_GPU (.cu:3338-3558, straight voxel walk
  * take_3D_step_GPU / take_2D_step_GPU .cu:8754-8867), MSC_edge_detection_GPU
  * (.cu:3569-3596), hull_selection (.cu:3954-3990) and averaging_filter_GPU
- * (.cu:4007-4029). Hulls are bit-packed (1 bit per voxel): SC clears bits with
+ * (.cu:4007-4029), and the space-modelling hull SM_GPU / SM_edge_detection
+ * (.cu:3598-3789). Hulls are bit-packed (1 bit per voxel): SC clears bits with
  * atomicAnd, MSC counts with integer atomics (both order-independent, so the
  * result is deterministic), the filtered hull is packed with a wavefront
  * ballot. The reference's edge detection rewrites the counts in place while
@@ -125,11 +126,16 @@ k_space_carve(pct_geo g, pct_hist_soa h, const uint8_t* __restrict__ missed, dou
     });
 }
 
+/* MSC (SM = false): histories with WEPL <= threshold; SM (SM = true):
+ * histories with WEPL >= threshold (.cu:3615) */
+template <bool SM>
 __global__ void __launch_bounds__(256)
-k_msc_count(pct_geo g, pct_hist_soa h, const uint8_t* __restrict__ missed, double threshold,
-            int32_t* __restrict__ counts) {
+k_walk_count(pct_geo g, pct_hist_soa h, const uint8_t* __restrict__ missed, double threshold,
+             int32_t* __restrict__ counts) {
     int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
-    if (i >= h.n || missed[i] || !(h.wepl[i] <= threshold)) return;
+    if (i >= h.n || missed[i]) return;
+    const float w = h.wepl[i];
+    if (SM ? !(w >= threshold) : !(w <= threshold)) return;
     carve_walk(g, h.x_entry[i], h.y_entry[i], h.z_entry[i], h.x_exit[i], h.y_exit[i], h.z_exit[i], [&](int64_t v) {
         if (v >= 0 && v < g.V) atomicAdd(&counts[v], 1);
     });
@@ -167,6 +173,80 @@ __global__ void k_msc_edge(pct_geo g, int diff_thresh, const int32_t* __restrict
     float s2 = x * x + y * y;
     if ((double)s2 >= rr * rr) v = 0;
     out[voxel] = v;
+}
+
+/* SM edge detection, step 1: per slice the largest carve_differences value
+ * (.cu:3274-3292) and the first pixel holding it, as one 64-bit key
+ * (difference << 32 | ~pixel): the largest key is the largest difference at
+ * the smallest pixel, which is what the reference's in-order scan with a
+ * strict > keeps (.cu:3736-3747). Differences of 0 leave the key at 0. A
+ * workgroup covers SM_TILE pixels of one slice (blockIdx.y) and adds one
+ * atomic max. */
+constexpr int SM_TILE = 1024;
+__global__ void __launch_bounds__(256)
+k_sm_diff_max(pct_geo g, const int32_t* __restrict__ counts, unsigned long long* __restrict__ keys) {
+    const int C = g.C, R = g.R;
+    const int64_t plane = (int64_t)C * R;
+    const int64_t slice_base = (int64_t)blockIdx.y * plane;
+    unsigned long long best = 0;
+    for (int k = 0; k < SM_TILE / 256; ++k) {
+        int64_t pixel = (int64_t)blockIdx.x * SM_TILE + k * 256 + threadIdx.x;
+        if (pixel >= plane) break;
+        int column = (int)(pixel % C), row = (int)(pixel / C);
+        if (row == 0 || row == R - 1 || column == 0 || column == C - 1) continue;
+        const int32_t* c = counts + slice_base;
+        int c0 = c[pixel];
+        int max_difference = 0;
+        for (int cr = row - 1; cr <= row + 1; ++cr)
+            for (int cc = column - 1; cc <= column + 1; ++cc) {
+                int d = c0 - c[cc + (int64_t)cr * C];
+                if (d > max_difference) max_difference = d;
+            }
+        if (max_difference > 0) {
+            unsigned long long key = ((unsigned long long)(uint32_t)max_difference << 32) | (uint32_t)~(uint32_t)pixel;
+            best = key > best ? key : best;
+        }
+    }
+    for (int off = 32; off > 0; off >>= 1) {
+        unsigned long long o = __shfl_xor(best, off, 64);
+        best = o > best ? o : best;
+    }
+    __shared__ unsigned long long s_best[4];
+    if ((threadIdx.x & 63) == 0) s_best[threadIdx.x >> 6] = best;
+    __syncthreads();
+    if (threadIdx.x == 0) {
+        for (int w = 1; w < 4; ++w) best = s_best[w] > best ? s_best[w] : best;
+        if (best) atomicMax(&keys[blockIdx.y], best);
+    }
+}
+
+__global__ void k_sm_keys_clear(unsigned long long* keys, int n) {
+    int i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i < n) keys[i] = 0;
+}
+
+/* SM edge detection, step 2: SM_edge_detection_GPU (.cu:3774-3789) with the
+ * slice's threshold = the counts at the key's pixel */
+__global__ void __launch_bounds__(256)
+k_sm_edge(pct_geo g, double scale, const int32_t* __restrict__ counts, const unsigned long long* __restrict__ keys,
+          int32_t* __restrict__ out, int32_t* __restrict__ thresholds) {
+    int64_t voxel = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (voxel >= g.V) return;
+    const int C = g.C, R = g.R;
+    const int64_t plane = (int64_t)C * R;
+    const int slice = (int)(voxel / plane);
+    const int64_t pixel = voxel - slice * plane;
+    const int column = (int)(pixel % C), row = (int)(pixel / C);
+    const unsigned long long key = keys[slice];
+    const int threshold = key ? counts[slice * plane + (uint32_t)~(uint32_t)key] : 0;
+    float x = (float)((column - C / 2 + 0.5) * g.voxel_width);
+    float y = (float)((R / 2 - row - 0.5) * g.voxel_height);
+    int v = counts[voxel] > scale * threshold ? 1 : 0;
+    double rr = g.radius - (g.voxel_width > g.voxel_height ? g.voxel_width : g.voxel_height) / 2;
+    float s2 = x * x + y * y;
+    if ((double)s2 >= rr * rr) v = 0;
+    out[voxel] = v;
+    if (pixel == 0 && thresholds) thresholds[slice] = threshold;
 }
 
 /* hull = (counts != 0) then the averaging filter with threshold; one voxel per
@@ -226,7 +306,7 @@ extern "C" int pct_msc_count(pct_ctx* ctx, const pct_hist_soa* hist, const uint8
     int rc = check_carve(ctx, hist, missed, counts, "pct_msc_count");
     if (rc) return rc;
     if (hist->n > 0)
-        hipLaunchKernelGGL(k_msc_count, dim3(pct_grid(hist->n, 256)), dim3(256), 0, ctx->stream, ctx->g, *hist, missed,
+        hipLaunchKernelGGL(k_walk_count<false>, dim3(pct_grid(hist->n, 256)), dim3(256), 0, ctx->stream, ctx->g, *hist, missed,
                            ctx->p.msc_threshold, counts);
     PCT_HIP_CHECK(ctx, hipGetLastError());
     return PCT_OK;
@@ -236,6 +316,33 @@ extern "C" int pct_msc_edge(pct_ctx* ctx, const int32_t* counts, int32_t* out) {
     if (!ctx || !counts || !out || counts == out) return pct_fail(ctx, PCT_E_INVALID, "pct_msc_edge: bad arguments");
     hipLaunchKernelGGL(k_msc_edge, dim3(pct_grid(ctx->g.V, 256)), dim3(256), 0, ctx->stream, ctx->g,
                        ctx->p.msc_diff_thresh, counts, out);
+    PCT_HIP_CHECK(ctx, hipGetLastError());
+    return PCT_OK;
+}
+
+extern "C" int pct_sm_count(pct_ctx* ctx, const pct_hist_soa* hist, const uint8_t* missed, int32_t* counts) {
+    int rc = check_carve(ctx, hist, missed, counts, "pct_sm_count");
+    if (rc) return rc;
+    if (hist->n > 0)
+        hipLaunchKernelGGL(k_walk_count<true>, dim3(pct_grid(hist->n, 256)), dim3(256), 0, ctx->stream, ctx->g, *hist,
+                           missed, ctx->p.sm_lower_threshold, counts);
+    PCT_HIP_CHECK(ctx, hipGetLastError());
+    return PCT_OK;
+}
+
+extern "C" int pct_sm_edge(pct_ctx* ctx, const int32_t* counts, int32_t* out, int32_t* thresholds) {
+    if (!ctx || !counts || !out || counts == out) return pct_fail(ctx, PCT_E_INVALID, "pct_sm_edge: bad arguments");
+    const int64_t plane = (int64_t)ctx->g.C * ctx->g.R;
+    if (plane > INT32_MAX) return pct_fail(ctx, PCT_E_INVALID, "pct_sm_edge: slice of more than 2^31 pixels");
+    const int S = ctx->g.S;
+    int rc = pct_ensure(ctx, ctx->hull_tmp, sizeof(unsigned long long) * S);
+    if (rc) return rc;
+    unsigned long long* keys = (unsigned long long*)ctx->hull_tmp.ptr;
+    hipLaunchKernelGGL(k_sm_keys_clear, dim3((S + 255) / 256), dim3(256), 0, ctx->stream, keys, S);
+    hipLaunchKernelGGL(k_sm_diff_max, dim3((unsigned)((plane + SM_TILE - 1) / SM_TILE), S), dim3(256), 0, ctx->stream,
+                       ctx->g, counts, keys);
+    hipLaunchKernelGGL(k_sm_edge, dim3(pct_grid(ctx->g.V, 256)), dim3(256), 0, ctx->stream, ctx->g,
+                       ctx->p.sm_scale_threshold, counts, keys, out, thresholds);
     PCT_HIP_CHECK(ctx, hipGetLastError());
     return PCT_OK;
 }

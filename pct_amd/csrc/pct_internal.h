@@ -85,7 +85,7 @@ struct pct_ctx {
     /* execution knobs (pct_set_tuning; 0 = default) */
     int tune_mlp_waves = 0, tune_chunk_first = 0, tune_chunk_max = 0, tune_overlap = 0;
     /* workspaces, grown on demand */
-    pct_buffer scan_tmp, steps, offsets, path_store, path_len, chord, xupd, counts;
+    pct_buffer hull_tmp, scan_tmp, steps, offsets, path_store, path_len, chord, xupd, counts;
     pct_buffer blk_offsets, blk_voxels, blk_lengths, blk_chord;
     /* phase timing of pct_drop_iteration (HIP events on `stream`) */
     bool profile = false;

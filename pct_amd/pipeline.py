@@ -46,6 +46,7 @@ STAGE_BYTES = {
     # cylinder counts written; missed and wepl of every history, the 6 f32
     # positions of the k histories that walk; counts read and written once
     "msc_counts": lambda n, k, V, B: 4 * V + 5 * n + 24 * k + 8 * V,
+    "sm_counts": lambda n, k, V, B: 4 * V + 5 * n + 24 * k + 8 * V,
     "space_carve": lambda n, k, V, B: 5 * n + 24 * k + V / 4,
     # sorted records once; ranges in, 7 statistics out per bin
     "bin_statistics": lambda n, k, V, B: 16 * n + 36 * B,
@@ -58,6 +59,8 @@ STAGE_BYTES = {
     "fbp_filter": lambda n, k, V, B: 8 * B,
     "backproject": lambda n, k, V, B: 4 * B + 4 * V,
     "msc_edge": lambda n, k, V, B: 8 * V,
+    # counts read for the differences and again for the threshold test; hull out
+    "sm_edge": lambda n, k, V, B: 12 * V,
     "hull_select": lambda n, k, V, B: 4 * V + V / 8,
     "fbp_threshold": lambda n, k, V, B: 4 * V + V / 8,
     "initial_iterate": lambda n, k, V, B: 8 * V + V / 8,
@@ -111,8 +114,12 @@ def preprocess(ctx: Context, raw: RawScan, hull_kind: str = "msc", timings: list
     in_range = T("binning", n_raw, ctx.binning, hist, missed)
     def walkers(thr):                     # histories whose straight path is carved
         return lambda _: int(((missed == 0) & (hist.wepl <= thr)).sum())
+    def crossers(thr):                    # histories whose path is counted by SM
+        return lambda _: int(((missed == 0) & (hist.wepl >= thr)).sum())
     if hull_kind == "msc":
         counts = T("msc_counts", n_raw, ctx.msc_counts, hist, missed, kept=walkers(ctx.params.msc_threshold))
+    elif hull_kind == "sm":
+        counts = T("sm_counts", n_raw, ctx.sm_counts, hist, missed, kept=crossers(ctx.params.sm_lower_threshold))
     elif hull_kind == "sc":
         sc_bits = T("space_carve", n_raw, ctx.space_carve, hist, missed, ctx.hull_init(),
                     kept=walkers(ctx.params.sc_threshold))
@@ -122,6 +129,8 @@ def preprocess(ctx: Context, raw: RawScan, hull_kind: str = "msc", timings: list
     fbp = T("backproject", 0, ctx.backproject, T("fbp_filter", 0, ctx.fbp_filter, sinogram))
     if hull_kind == "msc":
         hull = T("hull_select", 0, ctx.hull_select, T("msc_edge", 0, ctx.msc_edge, counts))
+    elif hull_kind == "sm":
+        hull = T("hull_select", 0, ctx.hull_select, T("sm_edge", 0, ctx.sm_edge, counts))
     elif hull_kind == "sc":
         hull = sc_bits
     elif hull_kind == "fbp":

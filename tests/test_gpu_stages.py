@@ -62,8 +62,11 @@ def hull_sample():
     fbp = oracle.backprojection(p, filt)
     fh = oracle.fbp_hull(p, fbp)
     x0 = oracle.initial_iterate(p, fbp, hull)
+    sm = oracle.sm_counts(p, h, missed)
+    sm_edge, sm_thr = oracle.sm_edge(p, sm)
     return dict(p=p, cfg=cfg, missed=missed, counts=counts, edge=edge, hull=hull, sc=sc, sino=r["sinogram"],
-                filt=filt, fbp=fbp, fh=fh, x0=x0)
+                filt=filt, fbp=fbp, fh=fh, x0=x0, sm=sm, sm_edge=sm_edge, sm_thr=sm_thr,
+                sm_hull=oracle.hull_select(p, sm_edge))
 
 
 def _unpack(bits, V):
@@ -98,6 +101,74 @@ def test_hull_and_fbp_bit_exact(hull_sample):
     assert np.array_equal(_unpack(ctx.fbp_threshold(fbp), V), s["fh"])
     x0 = ctx.initial_iterate(fbp, hull)
     assert np.array_equal(x0.cpu().numpy().view(np.uint32), s["x0"].view(np.uint32))
+    ctx.close()
+
+
+def test_sm_hull_bit_exact(hull_sample):
+    """space modelling: counts, per-slice thresholds, edge and selected hull
+    equal the oracle's; then counts with ties of the largest difference in
+    several workgroups' tiles of a slice (the first pixel in order wins) and a
+    flat slice (threshold 0)"""
+    import oracle
+    import pct_amd
+    s = hull_sample
+    p = s["p"]
+    C, R, S = p.columns, p.rows, p.slices
+    V = C * R * S
+    ctx = pct_amd.Context(p)
+    torch = ctx.torch
+    hist, missed = ctx.recon_volume_intersections(ctx.generate(s["cfg"]))
+    sm = ctx.sm_counts(hist, missed)
+    assert (s["sm"] > 1).sum() > 1000 and np.all(s["sm_thr"] > 0)
+    assert np.array_equal(sm.cpu().numpy(), s["sm"])
+    thr = torch.full((S,), -1, dtype=torch.int32, device="cuda")
+    edge = ctx.sm_edge(sm, thr)
+    assert np.array_equal(thr.cpu().numpy(), s["sm_thr"])
+    assert np.array_equal(edge.cpu().numpy(), s["sm_edge"])
+    assert np.array_equal(_unpack(ctx.hull_select(edge), V), s["sm_hull"])
+    rng = np.random.default_rng(11)
+    counts = rng.integers(0, 6, V).astype(np.int32)
+    c3 = counts.reshape(S, R, C)
+    c3[0] = 3
+    c3[1] = 10
+    for (r, c, top) in ((R - 3, C - 3, 25), (R // 2, C // 2, 21), (2, 2, 18)):   # last tile first: same drop 8
+        c3[1, r - 1:r + 2, c - 1:c + 2] = top - 8
+        c3[1, r, c] = top
+    for scale in (1.0, 0.5):
+        q = p.copy()
+        q.sm_scale_threshold = scale
+        exp, exp_thr = oracle.sm_edge(q, counts)
+        assert exp_thr[0] == 0 and exp_thr[1] == 18
+        cq = pct_amd.Context(q)
+        thr.fill_(-1)
+        got = cq.sm_edge(torch.from_numpy(counts).cuda(), thr)
+        assert np.array_equal(thr.cpu().numpy(), exp_thr)
+        assert np.array_equal(got.cpu().numpy(), exp)
+        cq.close()
+    # in place is refused (other workgroups read the counts being thresholded)
+    assert ctx.L.pct_sm_edge(ctx._ctx, pct_amd._ptr(sm), pct_amd._ptr(sm), None) < 0
+    ctx.close()
+
+
+def test_sm_pipeline_matches_oracle():
+    """raw scan -> SM hull -> x0 -> reconstruction histories on the GPU
+    equal the oracle's preprocessing with the SM hull"""
+    import oracle
+    import pct_amd
+    from pct_amd import configs, pipeline
+    p = configs.params("cfg1")
+    cfg = configs.synth_cfg("cfg1", n_histories=300_000)
+    cfg.cyl_r[0] = 8.0
+    raw = pct_amd.generate_host(cfg).numpy()
+    recon, x0, hull, _ = oracle.preprocess(p, raw, hull_kind="sm")
+    ctx = pct_amd.Context(p)
+    pre = pipeline.preprocess(ctx, ctx.generate(cfg), hull_kind="sm")
+    V = p.columns * p.rows * p.slices
+    assert hull.sum() > 1000
+    assert np.array_equal(_unpack(pre.hull, V), hull)
+    assert np.array_equal(pre.x0.cpu().numpy().view(np.uint32), x0.view(np.uint32))
+    assert pre.hist.n == recon.n
+    assert np.array_equal(pre.hist.numpy()["first_voxel"], recon.arrays()["first_voxel"])
     ctx.close()
 
 

@@ -8,7 +8,13 @@
   - SC/MSC voxel walk (.cu:3338-3558): the walk's voxels equal the voxels a
     dense sampling of the entry->exit segment passes through;
   - MSC edge detection / hull selection / averaging filter / FBP threshold /
-    x0 = FBP * hull against numpy.
+    x0 = FBP * hull against numpy;
+  - SM (.cu:3598-3789): only non-missed histories with WEPL >= 6 cm are
+    counted, each by the walk above; the edge detection against a vectorised
+    numpy restatement (3x3 differences, np.argmax's first maximum per slice,
+    scaled threshold, rim), on counts with many ties, a flat slice and a
+    non-unit scale; taking the last maximum instead of the first would change
+    the thresholds of that data.
 """
 import math
 
@@ -197,3 +203,86 @@ def test_edge_select_threshold_x0_numpy():
     np.testing.assert_array_equal(fh, ((img.reshape(S, R, C) > p.fbp_threshold) & cyl).astype(np.uint8))
     x0 = oracle.initial_iterate(p, img, hull.reshape(-1))
     np.testing.assert_array_equal(x0, img * hull.reshape(-1).astype(np.float32))
+
+
+def test_sm_counts_only_crossing_histories():
+    """SM_GPU (.cu:3615): non-missed histories with WEPL >= SM_LOWER_THRESHOLD;
+    NaN and 5.9 cm are not counted, 6.0 is"""
+    p = configs.params("mini")
+    assert p.sm_lower_threshold == 6.0 and p.sm_scale_threshold == 1.0
+    pts = [((-9.0, 0.3, 0.1), (9.0, 0.4, 0.1)), ((0.2, -9.0, 0.5), (0.3, 9.0, -0.4)),
+           ((-8.0, -6.0, 1.0), (7.0, 6.5, -1.0)), ((-9.0, 2.0, 0.0), (9.0, -2.0, 0.0)),
+           ((-6.0, 7.0, 0.2), (6.0, -7.0, 0.3))]
+    h = _walk_hist(pts)
+    h.wepl[:] = [6.0, 5.9, 20.0, float("nan"), 8.0]
+    missed = np.array([0, 0, 0, 0, 1], np.uint8)
+    got = oracle.sm_counts(p, h, missed) - oracle.hull_init(p)
+    # the same walks counted by the MSC code with a threshold that admits all
+    q = configs.params("mini")
+    q.msc_threshold = 1e30
+    sel = np.array([True, False, True, False, False])
+    exp = oracle.msc_counts(q, h.select(sel), np.zeros(2, np.uint8)) - oracle.hull_init(q)
+    np.testing.assert_array_equal(got, exp)
+    assert got.sum() > 30
+
+
+def _sm_edge_numpy(p, counts, last=False):
+    C, R, S = p.columns, p.rows, p.slices
+    cnt = counts.reshape(S, R, C).astype(np.int64)
+    diff = np.zeros_like(cnt)
+    core = cnt[:, 1:-1, 1:-1]
+    for dr in (-1, 0, 1):
+        for dc in (-1, 0, 1):
+            nb = cnt[:, 1 + dr:R - 1 + dr, 1 + dc:C - 1 + dc]
+            diff[:, 1:-1, 1:-1] = np.maximum(diff[:, 1:-1, 1:-1], core - nb)
+    flat = diff.reshape(S, -1)
+    if last:
+        arg = flat.shape[1] - 1 - np.argmax(flat[:, ::-1], axis=1)
+    else:
+        arg = np.argmax(flat, axis=1)
+    thr = np.where(flat.max(axis=1) > 0, cnt.reshape(S, -1)[np.arange(S), arg], 0)
+    out = (cnt > p.sm_scale_threshold * thr[:, None, None]).astype(np.int32)
+    w = 2 * p.recon_cyl_radius / C
+    x = ((np.arange(C) - C // 2 + 0.5) * w).astype(np.float32)
+    y = ((R // 2 - np.arange(R) - 0.5) * w).astype(np.float32)
+    s2 = (x[None, :] * x[None, :] + y[:, None] * y[:, None]).astype(np.float32)
+    rim = s2.astype(np.float64) >= (p.recon_cyl_radius - w / 2) ** 2
+    out[:, rim] = 0
+    return out.reshape(-1), thr.astype(np.int32)
+
+
+@pytest.mark.parametrize("scale", [1.0, 0.5])
+def test_sm_edge_numpy(scale):
+    p = _p_small(24, 4)
+    p.sm_scale_threshold = scale
+    C, R, S = p.columns, p.rows, p.slices
+    rng = np.random.default_rng(7)
+    counts = rng.integers(0, 6, C * R * S).astype(np.int32)
+    counts.reshape(S, R, C)[2] = 3                        # a flat slice: no positive difference
+    tie = counts.reshape(S, R, C)[1]                      # two pixels with the same largest drop (8)
+    tie[:] = 10                                           # but different counts: 18 first, 25 later
+    tie[3, 3] = 18
+    tie[14:17, 14:17] = 17
+    tie[15, 15] = 25
+    counts.reshape(S, R, C)[3, 5:9, 5:9] += 40           # one clear step
+    out, thr = oracle.sm_edge(p, counts)
+    exp, exp_thr = _sm_edge_numpy(p, counts)
+    np.testing.assert_array_equal(thr, exp_thr)
+    np.testing.assert_array_equal(out, exp)
+    assert thr[2] == 0 and out.reshape(S, R, C)[2].sum() > 0 and thr[1] == 18
+    # the ties make the first maximum matter
+    _, last_thr = _sm_edge_numpy(p, counts, last=True)
+    assert not np.array_equal(last_thr, exp_thr)
+
+
+def test_sm_hull_of_a_scan():
+    """on the synthetic scan the SM hull covers the object: the pipeline's
+    SM hull overlaps the MSC hull in most voxels"""
+    from pct_amd import configs as cf
+    import pct_amd
+    p = cf.params("mini")
+    raw = pct_amd.generate_host(cf.synth_cfg("mini", n_histories=60_000)).numpy()
+    _, _, sm, _ = oracle.preprocess(p, raw, hull_kind="sm")
+    _, _, msc, _ = oracle.preprocess(p, raw, hull_kind="msc")
+    assert sm.sum() > 0.3 * msc.sum()
+    assert (sm & msc).sum() > 0.8 * min(sm.sum(), msc.sum())
