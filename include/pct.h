@@ -256,6 +256,43 @@ int pct_drop_block_update(pct_ctx* ctx, const pct_hist_soa* hist, int64_t start,
  * x[v] += x_update[v] / counts[v] where counts[v] > 0, then clears both. */
 int pct_drop_image_update(pct_ctx* ctx, float* x, float* x_update, uint32_t* counts);
 
+/* (e) Effective chord length of every history: replaces
+ * EffectiveChordLength_GPU (.cu:4974-4995) with the GPU driver's double
+ * average of the float angles (.cu:5315) when float_average == 0, and
+ * EffectiveChordLength (.cu:7380-7400) with the CPU research loop's float
+ * average (.cu:7539) when float_average != 0. out: hist->n doubles. */
+int pct_chord_lengths(pct_ctx* ctx, const pct_hist_soa* hist, int float_average, double* out);
+
+/* Update rules of pct_drop_lists_iteration (each followed by DROP_update,
+ * .cu:8044-8055, after every full block) */
+#define PCT_LISTS_DROP          0   /* DROP_blocks (.cu:8023-8043)                       */
+#define PCT_LISTS_DROP_RED      1   /* DROP_blocks_red (.cu:7998-8022): psi = (1-x_v) ETA PSI_SIGN */
+#define PCT_LISTS_ERROR_RED     2   /* DROP_blocks_error_red (.cu:7816-7843): chord L + e ETA, psi */
+#define PCT_LISTS_ERROR         3   /* DROP_blocks_error (.cu:7844-7870): chord L + e ETA L        */
+#define PCT_LISTS_ERROR_RED_SP  4   /* DROP_blocks_error_red_sp (.cu:7768-7815): chord 1.8 cm at
+                                       every 5th intersection, psi                                 */
+
+/* (e) One iteration of the CPU research reconstruction loop over voxel
+ * lists given by the caller: replaces the iteration body of
+ * image_reconstruction (.cu:7513-7565), which reads the lists from its
+ * MLP_paths file and the per-intersection path errors from its
+ * MLP_path_error file. List k is voxels[offsets[k] .. offsets[k+1]) (device
+ * arrays; offsets[0] = 0), with WEPL wepl[k], chord length chord[k] and, for
+ * the error variants, errors[offsets[k] + j] for its j-th intersection
+ * (may be null otherwise). After every full block of block_size lists
+ * ((n+1) % BLOCK_SIZE_RECON == 0, .cu:7554) the update runs; x_update
+ * (double) and counts (uint32), V values each, are the caller's and persist
+ * between calls like the reference's x_update_h / block_counts_h: zero them
+ * before the first iteration; a partial last block leaves its sums there for
+ * the next call. ETA and PSI_SIGN come from the parameters. Results equal
+ * the reference's loop bit for bit, including its error variants' weighting
+ * of x by list position (x_k[i]). Voxel indices, or for the error variants
+ * list positions, outside the image are skipped and reported as
+ * PCT_E_INVALID. */
+int pct_drop_lists_iteration(pct_ctx* ctx, int64_t n, const int64_t* offsets, const uint32_t* voxels,
+                             const float* errors, const double* chord, const float* wepl, float* x,
+                             double* x_update, uint32_t* counts, int block_size, float lambda, int variant);
+
 /* (e) Per-scan DROP plan: computes every MLP voxel list of `hist` and the
  * per-block voxel->history transpose used by pct_drop_iteration. Replaces the
  * set-up part of image_reconstruction_GPU (.cu:6764-6782). Called implicitly

@@ -79,6 +79,27 @@ int pct_pctd_write(const char* path, const pct_pctd_header* header, const pct_ra
 int pct_array_to_disk(const char* filename_base, const char* directory, const char* folder, const void* data,
                       int type, int x_max, int y_max, int z_max, int64_t elements, int single_file);
 
+/* MLP path files of the research loop (.cu:7479-7493, 4746-4761): the
+ * number of paths as text ("%u\n"), then per path its intersection count
+ * (uint32) and that many voxel indices (uint32). Path-error files hold one
+ * float per intersection, path after path, with no header (.cu:4758-4761).
+ * The header is read up to its one newline; the reference's fscanf("%u\n")
+ * would also skip a first count whose low byte is a whitespace code. */
+/* Number of paths the header announces, paths present, and their total
+ * intersection count. */
+int pct_mlp_paths_file_info(const char* path, int64_t* n_header, int64_t* n_paths, int64_t* total);
+/* Up to max_paths paths (the reference reads min(header count,
+ * reconstruction histories), .cu:7522) into host arrays: offsets (n+1,
+ * offsets[0] = 0) and voxels (at least `capacity` values). */
+int pct_mlp_paths_file_read(const char* path, int64_t max_paths, int64_t capacity, int64_t* offsets, uint32_t* voxels,
+                            int64_t* n_read);
+/* write_MLP_path over n lists, with the header (.cu:7480-7491) */
+int pct_mlp_paths_file_write(const char* path, int64_t n, const int64_t* offsets, const uint32_t* voxels);
+/* read_MLP_path_error (.cu:4758) for `count` intersections, and its writer
+ * (no counterpart in the reference: its error files come from elsewhere) */
+int pct_path_errors_read(const char* path, int64_t count, float* errors);
+int pct_path_errors_write(const char* path, int64_t count, const float* errors);
+
 const char* pct_io_last_error(void);
 
 #ifdef __cplusplus

@@ -33,8 +33,13 @@ def lib():
         L.oracle_recon_volume_intersections.argtypes = [P(abi.PctParams), P(abi.PctRawSoa), P(abi.PctHistSoa), vp]
         L.oracle_hull_init.argtypes = [P(abi.PctParams), vp]
         L.oracle_mlp_endpoints.argtypes = [P(abi.PctParams), P(abi.PctHistSoa), vp, vp]
-        L.oracle_chord_length.argtypes = [P(abi.PctParams), ctypes.c_float, ctypes.c_float, ctypes.c_float, ctypes.c_float]
-        L.oracle_chord_length.restype = ctypes.c_double
+        for name in ("oracle_chord_length", "oracle_chord_length_float_avg"):
+            getattr(L, name).argtypes = [P(abi.PctParams), ctypes.c_float, ctypes.c_float, ctypes.c_float,
+                                         ctypes.c_float]
+            getattr(L, name).restype = ctypes.c_double
+        L.oracle_drop_lists_research.argtypes = [ctypes.c_int64, ctypes.c_int64, vp, vp, vp, vp, vp, vp, vp, vp,
+                                                 ctypes.c_int, ctypes.c_float, ctypes.c_double, ctypes.c_int,
+                                                 ctypes.c_int]
         L.oracle_mlp_path.argtypes = [P(abi.PctParams), P(abi.PctHistSoa), ctypes.c_int64, vp, P(ctypes.c_int)]
         L.oracle_mlp_paths.argtypes = [P(abi.PctParams), P(abi.PctHistSoa), ctypes.c_int64, ctypes.c_int64, vp,
                                        ctypes.c_int64, vp]
@@ -331,6 +336,31 @@ def drop_lists(V, lists, chord, b, x0, block_size, lam, iterations=1):
     lib().oracle_drop_lists(V, len(lists), _p(offsets), _p(vox), _p(np.ascontiguousarray(chord, np.float64)),
                             _p(np.ascontiguousarray(b, np.float32)), _p(x), int(block_size), float(lam), int(iterations))
     return x
+
+
+def chord_length_float_avg(p, xy_in, xy_out, xz_in, xz_out):
+    """EffectiveChordLength of the float averages (the CPU research call)"""
+    return lib().oracle_chord_length_float_avg(ctypes.byref(p), xy_in, xy_out, xz_in, xz_out)
+
+
+def lists_csr(lists):
+    offsets = np.zeros(len(lists) + 1, np.int64)
+    offsets[1:] = np.cumsum([len(a) for a in lists])
+    vox = np.concatenate([np.asarray(a, np.uint32) for a in lists]) if lists else np.zeros(0, np.uint32)
+    return offsets, vox
+
+
+def drop_lists_research(V, offsets, voxels, errors, chord, b, x, x_update, counts, block_size, lam, eta=2.5,
+                        psi_sign=1, variant=2):
+    """One iteration of the research loop (oracle_drop_lists_research); x,
+    x_update (float64) and counts (uint32) are updated in place."""
+    assert x.dtype == np.float32 and x_update.dtype == np.float64 and counts.dtype == np.uint32
+    assert x.flags.c_contiguous and x_update.flags.c_contiguous and counts.flags.c_contiguous
+    keep = [np.ascontiguousarray(offsets, np.int64), np.ascontiguousarray(voxels, np.uint32),
+            None if errors is None else np.ascontiguousarray(errors, np.float32),
+            np.ascontiguousarray(chord, np.float64), np.ascontiguousarray(b, np.float32)]
+    lib().oracle_drop_lists_research(int(V), len(offsets) - 1, *(_p(a) for a in keep), _p(x), _p(x_update),
+                                     _p(counts), int(block_size), float(lam), float(eta), int(psi_sign), int(variant))
 
 
 def drop(p, h: HostHistories, x0: np.ndarray, iterations=1, block_size=None, lam=None, max_histories=-1,

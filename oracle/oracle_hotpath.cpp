@@ -652,6 +652,15 @@ int oracle_mlp_endpoints(const pct_params* p, const pct_hist_soa* h, const uint8
 }
 
 /* D3 */
+/* EffectiveChordLength called as the CPU research loop calls it (.cu:7539):
+ * the float average of the float angles */
+double oracle_chord_length_float_avg(const pct_params* p, float xy_in, float xy_out, float xz_in, float xz_out) {
+    Geo g = make_geo(*p);
+    float t = (xy_in + xy_out) / 2;
+    float v = (xz_in + xz_out) / 2;
+    return effective_chord_length(g, t, v);
+}
+
 double oracle_chord_length(const pct_params* p, float xy_in, float xy_out, float xz_in, float xz_out) {
     Geo g = make_geo(*p);
     return effective_chord_length(g, ((double)xy_in + (double)xy_out) / 2.0, ((double)xz_in + (double)xz_out) / 2.0);
@@ -754,6 +763,91 @@ int oracle_drop_lists(int64_t V, int64_t n_lists, const int64_t* offsets, const 
                 counts[v] = 0;
             }
         }
+    }
+    return 0;
+}
+
+/* One iteration of the CPU research loop over given lists (image_reconstruction
+ * .cu:7513-7565): list k (history WEPL b[k], chord L = chord[k], path errors
+ * errors[offsets[k] + i]) goes through
+ *   variant 0 DROP_blocks (.cu:8023-8043), 1 DROP_blocks_red (.cu:7998-8022),
+ *   2 DROP_blocks_error_red (.cu:7816-7843), 3 DROP_blocks_error (.cu:7844-7870),
+ *   4 DROP_blocks_error_red_sp (.cu:7768-7815),
+ * and DROP_update (.cu:8044-8055) runs when (k+1) % block_size == 0
+ * (.cu:7554). x_update and counts persist between calls (the reference's
+ * globals). The error variants form a.x with x_k[i], the list position, as
+ * written; pow(c, 2.0) is c*c; LAMBDA is the float lambda. */
+int oracle_drop_lists_research(int64_t V, int64_t n_lists, const int64_t* offsets, const uint32_t* voxels,
+                               const float* errors, const double* chord, const float* b, float* x, double* x_update,
+                               uint32_t* counts, int block_size, float lambda, double eta, int psi_sign, int variant) {
+    const double LAMBDA = lambda, ETA = eta;
+    const int PSI_SIGN = psi_sign;
+    for (int64_t k = 0; k < n_lists; ++k) {
+        const uint32_t* a_i = voxels + offsets[k];
+        const float* row_error = errors ? errors + offsets[k] : nullptr;
+        const unsigned int num_intersections = (unsigned int)(offsets[k + 1] - offsets[k]);
+        const double mean_chord_length = chord[k];
+        const double bi = b[k];
+        if (variant == 0 || variant == 1) {
+            double sum = 0;
+            for (unsigned int i = 0; i < num_intersections; i++) sum += x[a_i[i]];
+            double a_i_dot_x_k = mean_chord_length * sum;
+            double a_i_dot_a_i = (mean_chord_length * mean_chord_length) * num_intersections;
+            if (variant == 0) {
+                double scaled_residual = (bi - a_i_dot_x_k) / a_i_dot_a_i;
+                double update_value = mean_chord_length * LAMBDA * scaled_residual;
+                for (unsigned int j = 0; j < num_intersections; j++) {
+                    x_update[a_i[j]] += update_value;
+                    counts[a_i[j]] += 1;
+                }
+            } else {
+                double residual = bi - a_i_dot_x_k;
+                for (unsigned int j = 0; j < num_intersections; j++) {
+                    double psi_i = ((1.0 - x[a_i[j]]) * ETA) * PSI_SIGN;
+                    x_update[a_i[j]] += mean_chord_length * LAMBDA * (residual / (a_i_dot_a_i + psi_i));
+                    counts[a_i[j]] += 1;
+                }
+            }
+        } else {
+            auto weight = [&](unsigned int i) -> double {
+                if (variant == 2) {
+                    double error = row_error[i];
+                    return mean_chord_length + (error * ETA);
+                }
+                if (variant == 3) {
+                    float error = row_error[i];
+                    return mean_chord_length + error * ETA * mean_chord_length;
+                }
+                return (i % 5 == 0) ? 1.8 : mean_chord_length;
+            };
+            double a_i_dot_x_k = 0.0, a_i_dot_a_i = 0.0;
+            for (unsigned int i = 0; i < num_intersections; i++) {
+                double c = weight(i);
+                a_i_dot_x_k += (c * x[i]);
+                a_i_dot_a_i += c * c;
+            }
+            if (variant == 3) {
+                double scaled_residual = (bi - a_i_dot_x_k) / a_i_dot_a_i;
+                for (unsigned int j = 0; j < num_intersections; j++) {
+                    x_update[a_i[j]] += weight(j) * LAMBDA * scaled_residual;
+                    counts[a_i[j]] += 1;
+                }
+            } else {
+                double residual = bi - a_i_dot_x_k;
+                for (unsigned int j = 0; j < num_intersections; j++) {
+                    double psi_i = ((1.0 - x[a_i[j]]) * ETA) * PSI_SIGN;
+                    x_update[a_i[j]] += weight(j) * LAMBDA * (residual / (a_i_dot_a_i + psi_i));
+                    counts[a_i[j]] += 1;
+                }
+            }
+        }
+        if ((k + 1) % block_size == 0)
+            for (int64_t v = 0; v < V; v++)
+                if (counts[v] > 0) {
+                    x[v] += x_update[v] / counts[v];
+                    x_update[v] = 0;
+                    counts[v] = 0;
+                }
     }
     return 0;
 }

@@ -10,7 +10,8 @@ import ctypes
 
 from . import _lib
 from . import configs
-from ._lib import (PCT_DROP_ATOMIC, PCT_DROP_CACHED, PCT_DROP_F32, PCT_DROP_NO_GRAPH, PCT_DROP_ROBUST,
+from ._lib import (PCT_LISTS_DROP, PCT_LISTS_DROP_RED, PCT_LISTS_ERROR, PCT_LISTS_ERROR_RED,
+                   PCT_LISTS_ERROR_RED_SP, PCT_DROP_ATOMIC, PCT_DROP_CACHED, PCT_DROP_F32, PCT_DROP_NO_GRAPH, PCT_DROP_ROBUST,
                    PCT_DROP_ROBUST2, PCT_LEARNED, PCT_MLP_FP64, PCT_MLP_SURROGATE, PCT_MLP_TABLE, PCT_PRECISION,
                    PctHistSoa, PctParams, PctRawSoa, PctSynthCfg, PctSynthSoa)
 
@@ -369,6 +370,33 @@ class Context:
         hs = hist.soa()
         self._check(self.L.pct_drop_iteration(self._ctx, ctypes.byref(hs), _ptr(x), int(block_size), float(lam), mode),
                     "pct_drop_iteration")
+
+
+    def chord_lengths(self, hist, float_average=False):
+        """effective chord length per history (float64; float_average: the
+        CPU research loop's float average of the angles)"""
+        out = self.torch.empty(max(hist.n, 1), dtype=self.torch.float64, device=self.device)
+        hs = hist.soa()
+        self._check(self.L.pct_chord_lengths(self._ctx, ctypes.byref(hs), int(bool(float_average)), _ptr(out)),
+                    "pct_chord_lengths")
+        return out[:hist.n]
+
+    def drop_lists_iteration(self, offsets, voxels, errors, chord, wepl, x, x_update, counts, block_size=None,
+                             lam=None, variant=PCT_LISTS_ERROR_RED):
+        """One iteration of the research loop over given lists (device
+        tensors: offsets int64 n+1, voxels int32/uint32, errors float32 or
+        None, chord float64, wepl float32; x float32, x_update float64 and
+        counts int32 of V values, kept between calls)."""
+        block_size = self.params.drop_block_size if block_size is None else block_size
+        lam = self.params.lambda_ if lam is None else lam
+        n = offsets.numel() - 1
+        assert x.numel() == self.V and x_update.numel() == self.V and counts.numel() == self.V
+        assert x_update.dtype == self.torch.float64 and counts.dtype in (self.torch.int32, self.torch.uint32)
+        assert chord.numel() >= n and wepl.numel() >= n and (errors is None or errors.numel() >= voxels.numel())
+        self._check(self.L.pct_drop_lists_iteration(self._ctx, n, _ptr(offsets), _ptr(voxels), _ptr(errors),
+                                                    _ptr(chord), _ptr(wepl), _ptr(x), _ptr(x_update), _ptr(counts),
+                                                    int(block_size), float(lam), int(variant)),
+                    "pct_drop_lists_iteration")
 
 
 def set_profiling(ctx: "Context", on: bool):

@@ -167,6 +167,14 @@ SIGNATURES = {
     "pct_array_to_disk": (ctypes.c_int, [ctypes.c_char_p, ctypes.c_char_p, ctypes.c_char_p, c_vp, ctypes.c_int,
                                          ctypes.c_int, ctypes.c_int, ctypes.c_int, c_i64, ctypes.c_int]),
     "pct_io_last_error": (ctypes.c_char_p, []),
+    "pct_mlp_paths_file_info": (ctypes.c_int, [ctypes.c_char_p, P(c_i64), P(c_i64), P(c_i64)]),
+    "pct_mlp_paths_file_read": (ctypes.c_int, [ctypes.c_char_p, c_i64, c_i64, c_vp, c_vp, P(c_i64)]),
+    "pct_mlp_paths_file_write": (ctypes.c_int, [ctypes.c_char_p, c_i64, c_vp, c_vp]),
+    "pct_path_errors_read": (ctypes.c_int, [ctypes.c_char_p, c_i64, c_vp]),
+    "pct_path_errors_write": (ctypes.c_int, [ctypes.c_char_p, c_i64, c_vp]),
+    "pct_chord_lengths": (ctypes.c_int, [c_vp, P(PctHistSoa), ctypes.c_int, c_vp]),
+    "pct_drop_lists_iteration": (ctypes.c_int, [c_vp, c_i64, c_vp, c_vp, c_vp, c_vp, c_vp, c_vp, c_vp, c_vp,
+                                                ctypes.c_int, c_f32, ctypes.c_int]),
 }
 
 PCT_MLP_FP64 = 0
@@ -178,6 +186,7 @@ PCT_DROP_ATOMIC = 0x200
 PCT_DROP_NO_GRAPH = 0x400
 PCT_DROP_F32 = 0x800
 PCT_DROP_ROBUST = 0x1000
+PCT_LISTS_DROP, PCT_LISTS_DROP_RED, PCT_LISTS_ERROR_RED, PCT_LISTS_ERROR, PCT_LISTS_ERROR_RED_SP = 0, 1, 2, 3, 4
 PCT_DROP_ROBUST2 = 0x2000
 # pct_set_tuning keys
 PCT_TUNE_MLP_WAVES = 1

@@ -85,6 +85,9 @@ struct pct_ctx {
     /* execution knobs (pct_set_tuning; 0 = default) */
     int tune_mlp_waves = 0, tune_chunk_first = 0, tune_chunk_max = 0, tune_overlap = 0;
     /* workspaces, grown on demand */
+    /* research list iteration: sort keys, entries, lists, per-list factors,
+     * block offsets, sort storage (pct_research.hip) */
+    pct_buffer research[8];
     pct_buffer hull_tmp, scan_tmp, steps, offsets, path_store, path_len, chord, xupd, counts;
     pct_buffer blk_offsets, blk_voxels, blk_lengths, blk_chord;
     /* phase timing of pct_drop_iteration (HIP events on `stream`) */

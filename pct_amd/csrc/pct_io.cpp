@@ -13,6 +13,8 @@
  * Writer: the same layout, for synthetic scans.
  * Text dumps: array_2_disk (.cu:8092-8134), same file names, loop bounds and
  * stream formatting.
+ * MLP path and path-error files of the CPU research loop (write_MLP_path,
+ * read_MLP_path, read_MLP_path_error, .cu:4746-4761, 7479-7525).
  */
 #include <stdio.h>
 #include <string.h>
@@ -197,5 +199,109 @@ extern "C" int pct_array_to_disk(const char* filename_base, const char* director
         z_start += 1;
         z_end += 1;
     }
+    return PCT_OK;
+}
+
+namespace {
+
+/* the "%u\n" header: decimal digits, then one newline */
+bool read_paths_header(FILE* f, int64_t* n) {
+    int64_t v = 0;
+    int c, digits = 0;
+    while ((c = fgetc(f)) >= '0' && c <= '9') {
+        v = v * 10 + (c - '0');
+        if (++digits > 10) return false;
+    }
+    if (c != '\n' || digits == 0 || v > UINT32_MAX) return false;
+    *n = v;
+    return true;
+}
+
+}  // namespace
+
+extern "C" int pct_mlp_paths_file_info(const char* path, int64_t* n_header, int64_t* n_paths, int64_t* total) {
+    if (!path || !n_header || !n_paths || !total) return io_fail(PCT_E_INVALID, "pct_mlp_paths_file_info: null argument");
+    file_closer fc{fopen(path, "rb")};
+    if (!fc.f) return io_fail(PCT_E_IO, std::string(path) + ": cannot open");
+    if (!read_paths_header(fc.f, n_header)) return io_fail(PCT_E_IO, std::string(path) + ": no path-count line");
+    int64_t n = 0, t = 0;
+    uint32_t count;
+    while (fread(&count, 4, 1, fc.f) == 1) {
+        if (fseek(fc.f, 4L * count, SEEK_CUR) != 0) return io_fail(PCT_E_IO, std::string(path) + ": short path");
+        n += 1;
+        t += count;
+    }
+    /* fseek past the end succeeds: check the size */
+    long end = ftell(fc.f);
+    if (fseek(fc.f, 0, SEEK_END) != 0 || ftell(fc.f) != end) return io_fail(PCT_E_IO, std::string(path) + ": short path");
+    *n_paths = n;
+    *total = t;
+    return PCT_OK;
+}
+
+extern "C" int pct_mlp_paths_file_read(const char* path, int64_t max_paths, int64_t capacity, int64_t* offsets,
+                                       uint32_t* voxels, int64_t* n_read) {
+    if (!path || !offsets || !n_read || (capacity > 0 && !voxels))
+        return io_fail(PCT_E_INVALID, "pct_mlp_paths_file_read: null argument");
+    file_closer fc{fopen(path, "rb")};
+    if (!fc.f) return io_fail(PCT_E_IO, std::string(path) + ": cannot open");
+    int64_t header;
+    if (!read_paths_header(fc.f, &header)) return io_fail(PCT_E_IO, std::string(path) + ": no path-count line");
+    const int64_t want = header < max_paths ? header : max_paths;
+    offsets[0] = 0;
+    int64_t k = 0;
+    for (; k < want; ++k) {
+        uint32_t count;
+        if (fread(&count, 4, 1, fc.f) != 1)
+            return io_fail(PCT_E_IO, std::string(path) + ": " + std::to_string(k) + " paths, header says " +
+                                         std::to_string(header));
+        if (offsets[k] + (int64_t)count > capacity)
+            return io_fail(PCT_E_INVALID, std::string(path) + ": intersections exceed the capacity");
+        if (!read_exact(fc.f, voxels + offsets[k], 4 * (size_t)count))
+            return io_fail(PCT_E_IO, std::string(path) + ": short path");
+        offsets[k + 1] = offsets[k] + count;
+    }
+    *n_read = k;
+    return PCT_OK;
+}
+
+extern "C" int pct_mlp_paths_file_write(const char* path, int64_t n, const int64_t* offsets, const uint32_t* voxels) {
+    if (!path || n < 0 || n > UINT32_MAX || (n > 0 && !offsets))
+        return io_fail(PCT_E_INVALID, "pct_mlp_paths_file_write: bad arguments");
+    for (int64_t k = 0; k < n; ++k) {
+        int64_t c = offsets[k + 1] - offsets[k];
+        if (c < 0 || c > UINT32_MAX || (c > 0 && !voxels))
+            return io_fail(PCT_E_INVALID, "pct_mlp_paths_file_write: bad offsets");
+    }
+    file_closer fc{fopen(path, "wb")};
+    if (!fc.f) return io_fail(PCT_E_IO, std::string(path) + ": cannot create");
+    bool ok = fprintf(fc.f, "%u\n", (unsigned)n) > 0;
+    for (int64_t k = 0; ok && k < n; ++k) {
+        const uint32_t count = (uint32_t)(offsets[k + 1] - offsets[k]);
+        ok = fwrite(&count, 4, 1, fc.f) == 1 && (count == 0 || fwrite(voxels + offsets[k], 4, count, fc.f) == count);
+    }
+    int closed = fclose(fc.f);
+    fc.f = nullptr;
+    if (!ok || closed != 0) return io_fail(PCT_E_IO, std::string(path) + ": write failed");
+    return PCT_OK;
+}
+
+extern "C" int pct_path_errors_read(const char* path, int64_t count, float* errors) {
+    if (!path || count < 0 || (count > 0 && !errors)) return io_fail(PCT_E_INVALID, "pct_path_errors_read: bad arguments");
+    file_closer fc{fopen(path, "rb")};
+    if (!fc.f) return io_fail(PCT_E_IO, std::string(path) + ": cannot open");
+    if (!read_exact(fc.f, errors, sizeof(float) * (size_t)count))
+        return io_fail(PCT_E_IO, std::string(path) + ": fewer than " + std::to_string(count) + " path errors");
+    return PCT_OK;
+}
+
+extern "C" int pct_path_errors_write(const char* path, int64_t count, const float* errors) {
+    if (!path || count < 0 || (count > 0 && !errors)) return io_fail(PCT_E_INVALID, "pct_path_errors_write: bad arguments");
+    file_closer fc{fopen(path, "wb")};
+    if (!fc.f) return io_fail(PCT_E_IO, std::string(path) + ": cannot create");
+    bool ok = count == 0 || fwrite(errors, sizeof(float), (size_t)count, fc.f) == (size_t)count;
+    int closed = fclose(fc.f);
+    fc.f = nullptr;
+    if (!ok || closed != 0) return io_fail(PCT_E_IO, std::string(path) + ": write failed");
     return PCT_OK;
 }

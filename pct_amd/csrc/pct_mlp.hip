@@ -198,6 +198,25 @@ __device__ __forceinline__ double chord_length(const pct_geo& g, double abs_angl
     return g.voxel_width * chord_2d * chord_3d;
 }
 
+/* chord length per history: the double average of the float angles (the
+ * GPU call, .cu:5315) or their float average (the CPU research call, .cu:7539) */
+__global__ void __launch_bounds__(256)
+k_chord_lengths(pct_geo g, pct_hist_soa h, int float_average, double* __restrict__ out) {
+    int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= h.n) return;
+    double t, v;
+    if (float_average) {
+        float st = h.xy_entry_angle[i] + h.xy_exit_angle[i];
+        float sv = h.xz_entry_angle[i] + h.xz_exit_angle[i];
+        t = st / 2;
+        v = sv / 2;
+    } else {
+        t = ((double)h.xy_entry_angle[i] + (double)h.xy_exit_angle[i]) / 2.0;
+        v = ((double)h.xz_entry_angle[i] + (double)h.xz_exit_angle[i]) / 2.0;
+    }
+    out[i] = chord_length(g, t, v);
+}
+
 /* per-history MLP constants (.cu:5306-5374) */
 struct mlp_hist {
     double ca, sa, u_in, t_in, t_out, dT, v_in, v_out, dV, u_2, pre1, pre2, pre3;
@@ -1318,5 +1337,17 @@ extern "C" int pct_net_calibrate(pct_ctx* ctx, const pct_hist_soa* hist, int64_t
         out->surrogate++;
         out->surrogate_mismatch += hmis[q];
     }
+    return PCT_OK;
+}
+
+extern "C" int pct_chord_lengths(pct_ctx* ctx, const pct_hist_soa* hist, int float_average, double* out) {
+    if (!ctx || !hist) return pct_fail(ctx, PCT_E_INVALID, "pct_chord_lengths: null argument");
+    if (hist->n < 0) return pct_fail(ctx, PCT_E_INVALID, "pct_chord_lengths: negative history count");
+    if (hist->n == 0) return PCT_OK;
+    if (!out || !hist->xy_entry_angle || !hist->xy_exit_angle || !hist->xz_entry_angle || !hist->xz_exit_angle)
+        return pct_fail(ctx, PCT_E_INVALID, "pct_chord_lengths: missing angle or output array");
+    hipLaunchKernelGGL(k_chord_lengths, dim3(pct_grid(hist->n, 256)), dim3(256), 0, ctx->stream, ctx->g, *hist,
+                       float_average, out);
+    PCT_HIP_CHECK(ctx, hipGetLastError());
     return PCT_OK;
 }
