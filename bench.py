@@ -265,6 +265,61 @@ def learned_large(net, steps, log):
     return res
 
 
+def research_forms(ctx, hist, x0, B, p, steps, timed, log, m_lists=2_000_000):
+    """The research update rules on this scan: the robust variants on the
+    block loop (FP64, MLP lists recomputed as in the default form), and the
+    research loop over given lists (pct_drop_lists_iteration, the live
+    DROP_blocks_error_red rule with random path errors) on the MLP lists of
+    the first m_lists reconstruction histories; per call the loop also sorts
+    the lists' entries by (block, voxel)."""
+    import torch
+    import pct_amd
+    res = {}
+    for name, flag in (("robust_drop_blocks_red", pct_amd.PCT_DROP_ROBUST),
+                       ("robust2_drop_blocks_robust2", pct_amd.PCT_DROP_ROBUST2)):
+        xv = x0.clone()
+        ctx.drop_iteration(hist, xv, B, p.lambda_, pct_amd.PCT_MLP_FP64 | flag)
+        ms, _ = timed(pct_amd.PCT_MLP_FP64 | flag, steps, xv)
+        ms /= steps
+        res[name] = {"ms_per_iteration": round(ms, 3), "histories_per_s": round(hist.n / (ms / 1e3), 1),
+                     "eta": p.eta, "psi_sign": p.psi_sign}
+    m = min(m_lists, hist.n)
+    sub = hist.head(m)
+    offs, vox, lens = ctx.mlp_paths(sub)
+    lens = lens.to(torch.int64)
+    E = int(lens.sum())
+    starts = torch.repeat_interleave(offs[:-1], lens)
+    first = torch.cumsum(lens, 0) - lens
+    idx = starts + torch.arange(E, device=lens.device) - torch.repeat_interleave(first, lens)
+    voxels = vox[idx].contiguous()
+    del starts, idx, vox
+    offsets = torch.zeros(m + 1, dtype=torch.int64, device=lens.device)
+    offsets[1:] = torch.cumsum(lens, 0)
+    g = torch.Generator(device=lens.device)
+    g.manual_seed(7)
+    errors = (0.5 * torch.randn(E, generator=g, device=lens.device)).to(torch.float32)
+    chord = ctx.chord_lengths(sub, float_average=True)
+    xv = x0.clone()
+    xu = torch.zeros(ctx.V, dtype=torch.float64, device=lens.device)
+    cnt = torch.zeros(ctx.V, dtype=torch.int32, device=lens.device)
+    ctx.drop_lists_iteration(offsets, voxels, errors, chord, sub.wepl, xv, xu, cnt, B, p.lambda_)
+    torch.cuda.synchronize()
+    ev0, ev1 = torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)
+    ev0.record(ctx.stream)
+    for _ in range(steps):
+        ctx.drop_lists_iteration(offsets, voxels, errors, chord, sub.wepl, xv, xu, cnt, B, p.lambda_)
+    ev1.record(ctx.stream)
+    torch.cuda.synchronize()
+    ms = ev0.elapsed_time(ev1) / steps
+    res["research_lists_drop_blocks_error_red"] = {
+        "ms_per_iteration": round(ms, 3), "lists": m, "intersections": E,
+        "lists_per_s": round(m / (ms / 1e3), 1),
+        "note": "one thread per list forms a.x in list order and entries are sorted by (block, voxel) each call, "
+                "so the result equals the reference loop bit for bit; not the production path"}
+    log("research forms: " + ", ".join(f"{k} {v['ms_per_iteration']:.2f} ms" for k, v in res.items()))
+    return res
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -277,6 +332,7 @@ def main():
     ap.add_argument("--no-batched", action="store_true", help="skip the batched 12-scan form")
     ap.add_argument("--no-learned", action="store_true", help="skip the learned-path form")
     ap.add_argument("--no-large", action="store_true", help="skip the 256x256x48 / 50 M learned-form case")
+    ap.add_argument("--no-research", action="store_true", help="skip the research update rules and the SM hull")
     ap.add_argument("--quick", action="store_true",
                     help="only the timed iterations (for counter runs, where every dispatch is slow)")
     ap.add_argument("--form", default="default", choices=["default", "cached", "sequential", "precision", "learned"],
@@ -338,6 +394,20 @@ def main():
         stages[name] = {"ms": round(ms, 4), "bytes": int(nbytes), "gb_s": round(gbs, 1),
                         "frac": round(gbs / HBM_PEAK_GBS, 4)}
     log("preprocessing stages: " + ", ".join(f"{k} {v['ms']:.3f} ms {v['gb_s']:.0f} GB/s" for k, v in stages.items()))
+    sm_stages = None
+    if not args.no_research:
+        # the space-modelling hull in place of MSC (same scan, stages alone)
+        t_sm = []
+        pre_sm = pipeline.preprocess(ctx, raw, hull_kind="sm", timings=t_sm)
+        sm_stages = {}
+        for name, ms, nbytes in t_sm:
+            if name in ("sm_counts", "sm_edge", "hull_select"):
+                gbs = nbytes / (ms * 1e-3) / 1e9
+                sm_stages[name] = {"ms": round(ms, 4), "bytes": int(nbytes), "gb_s": round(gbs, 1),
+                                   "frac": round(gbs / HBM_PEAK_GBS, 4)}
+        sm_stages["hull_voxels"] = int(np.unpackbits(pre_sm.hull.cpu().numpy().view(np.uint8)).sum())
+        del pre_sm
+        log("SM hull stages: " + ", ".join(f"{k} {v['ms']:.3f} ms" for k, v in sm_stages.items() if isinstance(v, dict)))
     del raw
     hist = pre.hist
     n_r = hist.n
@@ -456,6 +526,17 @@ def main():
                  "speedup_vs_fp64_form": round(ms_per_step / ms_prec, 3),
                  "error_vs_fp64": prec_err,
                  "error_region": "voxels of the hull (x0 != 0); reference image: exact FP64 form"}
+
+    # ---- research update rules (robust variants, list loop) ----
+    research = None
+    if not args.no_research:
+        try:
+            ctx.drop_prepare(hist, B)
+            research = research_forms(ctx, hist, pre.x0, B, p, args.steps, timed, log)
+            research["sm_hull_stages"] = sm_stages
+        except Exception as e:
+            research = {"error": f"{type(e).__name__}: {e}"}
+            log(f"research forms failed: {e}")
 
     # ---- learned-path form: network trained here on 10 M exact-MLP paths ----
     learned_res, net = None, None
@@ -582,6 +663,7 @@ def main():
         "precision_form": precision,
         "batched_form": batched,
         "learned_form": learned_res,
+        "research_forms": research,
         "other_forms": {"sequential_mlp_then_blocks_ms": round(ms_seq, 3),
                         "cached_lists_histories_per_s": round(n_r / (ms_cached / 1e3), 1),
                         "cached_lists_ms": round(ms_cached, 3),

@@ -160,6 +160,8 @@ extern "C" void pct_destroy(pct_ctx* ctx) {
         if (b->ptr) (void)hipFree(b->ptr);
     for (pct_buffer& b : ctx->research)
         if (b.ptr) (void)hipFree(b.ptr);
+    for (pct_buffer& b : ctx->walk_order)
+        if (b.ptr) (void)hipFree(b.ptr);
     if (ctx->d_status) (void)hipFree(ctx->d_status);
     if (ctx->d_mlp_tab) (void)hipFree(ctx->d_mlp_tab);
     if (ctx->d_scat_tab) (void)hipFree(ctx->d_scat_tab);

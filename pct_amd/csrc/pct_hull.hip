@@ -15,6 +15,8 @@
  */
 #include "pct_internal.h"
 
+#include <rocprim/device/device_radix_sort.hpp>
+
 namespace {
 
 __device__ __forceinline__ int calc_voxel(double zero, double pos, double size) {
@@ -136,6 +138,49 @@ k_walk_count(pct_geo g, pct_hist_soa h, const uint8_t* __restrict__ missed, doub
     if (i >= h.n || missed[i]) return;
     const float w = h.wepl[i];
     if (SM ? !(w >= threshold) : !(w <= threshold)) return;
+    carve_walk(g, h.x_entry[i], h.y_entry[i], h.z_entry[i], h.x_exit[i], h.y_exit[i], h.z_exit[i], [&](int64_t v) {
+        if (v >= 0 && v < g.V) atomicAdd(&counts[v], 1);
+    });
+}
+
+/* The SM walk visits most histories, and the per-voxel atomics of a
+ * wavefront cost one cache line each: walked in storage order (angle-major,
+ * random lateral position) 64 lanes touch ~64 lines per step. Ordered by
+ * (height, beam direction, lateral offset) the lanes of a wave trace adjacent
+ * parallel paths and touch a few neighbouring lines. The order only changes
+ * the order of integer increments, so the counts are the same. Key: half-slice
+ * row of the entry height, 1-degree direction, 0.5 mm lateral offset;
+ * histories that do not walk sort last. */
+__global__ void __launch_bounds__(256)
+k_walk_keys(pct_geo g, pct_hist_soa h, const uint8_t* __restrict__ missed, double threshold, int vbits,
+            uint32_t* __restrict__ keys, uint32_t* __restrict__ order) {
+    int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= h.n) return;
+    order[i] = (uint32_t)i;
+    const float w = h.wepl[i];
+    if (missed[i] || !(w >= threshold)) {
+        keys[i] = 0xFFFFFFFFu;
+        return;
+    }
+    const float xe = h.x_entry[i], ye = h.y_entry[i];
+    const float a = atan2f(h.y_exit[i] - ye, h.x_exit[i] - xe);
+    const float t = -sinf(a) * xe + cosf(a) * ye;
+    int vq = (int)(fabsf(h.z_entry[i] - (float)g.z_zero) / (float)(0.5 * g.voxel_thickness));
+    int aq = (int)((a + 3.14159265f) * (360.0f / 6.2831853f));
+    int tq = (int)((t + (float)g.radius) * 20.0f);
+    vq = vq < 0 ? 0 : (vq > (1 << vbits) - 1 ? (1 << vbits) - 1 : vq);
+    aq = aq < 0 ? 0 : (aq > 359 ? 359 : aq);
+    tq = tq < 0 ? 0 : (tq > 4095 ? 4095 : tq);
+    keys[i] = ((uint32_t)vq << 21) | ((uint32_t)aq << 12) | (uint32_t)tq;
+}
+
+/* the SM walk over the histories in `order`; the first n_walk of them walk */
+__global__ void __launch_bounds__(256)
+k_walk_count_ordered(pct_geo g, pct_hist_soa h, const uint32_t* __restrict__ keys, const uint32_t* __restrict__ order,
+                     int32_t* __restrict__ counts) {
+    int64_t j = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (j >= h.n || keys[j] == 0xFFFFFFFFu) return;
+    const int64_t i = order[j];
     carve_walk(g, h.x_entry[i], h.y_entry[i], h.z_entry[i], h.x_exit[i], h.y_exit[i], h.z_exit[i], [&](int64_t v) {
         if (v >= 0 && v < g.V) atomicAdd(&counts[v], 1);
     });
@@ -323,9 +368,32 @@ extern "C" int pct_msc_edge(pct_ctx* ctx, const int32_t* counts, int32_t* out) {
 extern "C" int pct_sm_count(pct_ctx* ctx, const pct_hist_soa* hist, const uint8_t* missed, int32_t* counts) {
     int rc = check_carve(ctx, hist, missed, counts, "pct_sm_count");
     if (rc) return rc;
-    if (hist->n > 0)
-        hipLaunchKernelGGL(k_walk_count<true>, dim3(pct_grid(hist->n, 256)), dim3(256), 0, ctx->stream, ctx->g, *hist,
-                           missed, ctx->p.sm_lower_threshold, counts);
+    const int64_t n = hist->n;
+    if (n == 0) return PCT_OK;
+    if (n >= (int64_t)UINT32_MAX) {                 /* too many to order with 32-bit indices: storage order */
+        hipLaunchKernelGGL(k_walk_count<true>, dim3(pct_grid(n, 256)), dim3(256), 0, ctx->stream, ctx->g, *hist, missed,
+                           ctx->p.sm_lower_threshold, counts);
+        PCT_HIP_CHECK(ctx, hipGetLastError());
+        return PCT_OK;
+    }
+    int vbits = 1;
+    while ((1 << vbits) < 2 * ctx->g.S && vbits < 11) ++vbits;
+    pct_buffer* w = ctx->walk_order;
+    const size_t nn = (size_t)n;
+    if ((rc = pct_ensure(ctx, w[0], 4 * nn)) || (rc = pct_ensure(ctx, w[1], 4 * nn)) ||
+        (rc = pct_ensure(ctx, w[2], 4 * nn)) || (rc = pct_ensure(ctx, w[3], 4 * nn)))
+        return rc;
+    uint32_t *k_in = (uint32_t*)w[0].ptr, *k_out = (uint32_t*)w[1].ptr;
+    uint32_t *o_in = (uint32_t*)w[2].ptr, *o_out = (uint32_t*)w[3].ptr;
+    hipLaunchKernelGGL(k_walk_keys, dim3(pct_grid(n, 256)), dim3(256), 0, ctx->stream, ctx->g, *hist, missed,
+                       ctx->p.sm_lower_threshold, vbits, k_in, o_in);
+    size_t tb = 0;
+    PCT_HIP_CHECK(ctx, rocprim::radix_sort_pairs(nullptr, tb, k_in, k_out, o_in, o_out, nn, 0, 32, ctx->stream));
+    if ((rc = pct_ensure(ctx, w[4], tb))) return rc;
+    tb = w[4].bytes;
+    PCT_HIP_CHECK(ctx, rocprim::radix_sort_pairs(w[4].ptr, tb, k_in, k_out, o_in, o_out, nn, 0, 32, ctx->stream));
+    hipLaunchKernelGGL(k_walk_count_ordered, dim3(pct_grid(n, 256)), dim3(256), 0, ctx->stream, ctx->g, *hist, k_out,
+                       o_out, counts);
     PCT_HIP_CHECK(ctx, hipGetLastError());
     return PCT_OK;
 }

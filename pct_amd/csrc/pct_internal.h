@@ -88,6 +88,8 @@ struct pct_ctx {
     /* research list iteration: sort keys, entries, lists, per-list factors,
      * block offsets, sort storage (pct_research.hip) */
     pct_buffer research[8];
+    /* walk order of the SM counts: keys, indices (in/out), sort storage */
+    pct_buffer walk_order[5];
     pct_buffer hull_tmp, scan_tmp, steps, offsets, path_store, path_len, chord, xupd, counts;
     pct_buffer blk_offsets, blk_voxels, blk_lengths, blk_chord;
     /* phase timing of pct_drop_iteration (HIP events on `stream`) */
