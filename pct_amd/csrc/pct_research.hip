@@ -16,7 +16,7 @@
  * block leaves its sums there and the next call's first update applies them.
  *
  * Results equal the reference's sequential loop bit for bit: per list one
- * thread forms a.x and |a|^2 in the list's order; the block's entries are
+ * wavefront forms the terms and one lane adds them in the list's order; the block's entries are
  * stably sorted by (block, voxel) once per call, so each voxel's run adds
  * its contributions onto x_update[v] in list order, as the loop does.
  * Arithmetic as written in the reference: the error variants weight x by
@@ -56,39 +56,60 @@ __device__ __forceinline__ double entry_chord(const lists_args& a, int64_t k, in
     }
 }
 
-/* one thread per list of the block: r[k] = the per-list factor of the
- * update (the scaled residual, or the residual and |a|^2 for the psi forms) */
-__global__ void __launch_bounds__(256)
+/* one wavefront per list of the block: the lanes form the list's terms
+ * (x_v, or c_i x_i and c_i^2) for up to 256 entries at a time into LDS, then
+ * lane 0 adds them in list order -- the reference's sequential sums, bit for
+ * bit. r[t] = the per-list factor of the update (the scaled residual, or the
+ * residual and |a|^2 for the psi forms). */
+constexpr int DOT_CHUNK = 256;
+__global__ void __launch_bounds__(64)
 k_lists_dot(lists_args a, int64_t k0, int64_t count, const float* __restrict__ x, double* __restrict__ r,
             double* __restrict__ aa_out, int* __restrict__ status) {
-    int64_t t = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    __shared__ double s_dot[DOT_CHUNK], s_aa[DOT_CHUNK];
+    const int64_t t = blockIdx.x;
     if (t >= count) return;
+    const int lane = threadIdx.x;
     const int64_t k = k0 + t;
     const int64_t e0 = a.offsets[k], n = a.offsets[k + 1] - e0;
     const double L = a.chord[k];
-    const double bi = a.wepl[k];
-    double dot, aa;
+    const bool by_voxel = a.variant == PCT_LISTS_DROP || a.variant == PCT_LISTS_DROP_RED;
     bool bad = false;
-    if (a.variant == PCT_LISTS_DROP || a.variant == PCT_LISTS_DROP_RED) {
-        double sum = 0.0;                               /* scalar_dot_product2 (.cu:7692-7698) */
-        for (int64_t i = 0; i < n; ++i) {
-            uint32_t v = a.voxels[e0 + i];
-            if (v < a.V) sum += x[v]; else bad = true;
+    double sum = 0.0, dot = 0.0, aa = 0.0;
+    for (int64_t c0 = 0; c0 < n; c0 += DOT_CHUNK) {
+        const int m = (int)(n - c0 < DOT_CHUNK ? n - c0 : DOT_CHUNK);
+        for (int j = lane; j < m; j += 64) {
+            const int64_t i = c0 + j;
+            if (by_voxel) {
+                const uint32_t v = a.voxels[e0 + i];
+                s_dot[j] = v < a.V ? (double)x[v] : 0.0;  /* scalar_dot_product2 (.cu:7692-7698) */
+                bad |= v >= a.V;
+            } else {
+                const double c = entry_chord(a, k, i, e0 + i);
+                const double xi = i < a.V ? (double)x[i] : 0.0;
+                bad |= i >= a.V;
+                s_dot[j] = c * xi;
+                s_aa[j] = c * c;
+            }
         }
-        dot = L * sum;
-        aa = (L * L) * (double)(uint32_t)n;
-    } else {
-        dot = 0.0;
-        aa = 0.0;
-        for (int64_t i = 0; i < n; ++i) {
-            double c = entry_chord(a, k, i, e0 + i);
-            double xi = 0.0;
-            if (i < a.V) xi = x[i]; else bad = true;
-            dot += (c * xi);
-            aa += c * c;
+        __syncthreads();
+        if (lane == 0) {
+            if (by_voxel)
+                for (int j = 0; j < m; ++j) sum += s_dot[j];
+            else
+                for (int j = 0; j < m; ++j) {
+                    dot += s_dot[j];
+                    aa += s_aa[j];
+                }
         }
+        __syncthreads();
     }
     if (bad) atomicOr(status, PCT_STATUS_OVERFLOW);
+    if (lane != 0) return;
+    if (by_voxel) {
+        dot = L * sum;
+        aa = (L * L) * (double)(uint32_t)n;
+    }
+    const double bi = a.wepl[k];
     const bool scaled = a.variant == PCT_LISTS_DROP || a.variant == PCT_LISTS_ERROR;
     r[t] = scaled ? (bi - dot) / aa : bi - dot;
     aa_out[t] = aa;
@@ -264,7 +285,7 @@ extern "C" int pct_drop_lists_iteration(pct_ctx* ctx, int64_t n, const int64_t* 
         const int64_t k0 = b * block_size;
         const int64_t count = (k0 + block_size <= n) ? block_size : n - k0;
         const int64_t e0 = blk_off[b], e1 = blk_off[b + 1];
-        hipLaunchKernelGGL(k_lists_dot, dim3(pct_grid(count, 256)), dim3(256), 0, ctx->stream, a, k0, count, x, r, aa,
+        hipLaunchKernelGGL(k_lists_dot, dim3((unsigned)count), dim3(64), 0, ctx->stream, a, k0, count, x, r, aa,
                            ctx->d_status);
         if (e1 > e0)
             hipLaunchKernelGGL(k_lists_accumulate, dim3(pct_grid(e1 - e0, 256)), dim3(256), 0, ctx->stream, a, k0, e0, e1,
