@@ -314,8 +314,9 @@ def research_forms(ctx, hist, x0, B, p, steps, timed, log, m_lists=2_000_000):
     res["research_lists_drop_blocks_error_red"] = {
         "ms_per_iteration": round(ms, 3), "lists": m, "intersections": E,
         "lists_per_s": round(m / (ms / 1e3), 1),
-        "note": "one thread per list forms a.x in list order and entries are sorted by (block, voxel) each call, "
-                "so the result equals the reference loop bit for bit; not the production path"}
+        "note": "one wavefront per list forms the terms and one lane adds them in list order, and the entries "
+                "are sorted by (block, voxel) each call, so the result equals the reference loop bit for bit; "
+                "not the production path"}
     log("research forms: " + ", ".join(f"{k} {v['ms_per_iteration']:.2f} ms" for k, v in res.items()))
     return res
 
