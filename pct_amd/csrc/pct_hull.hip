@@ -186,6 +186,68 @@ k_walk_count_ordered(pct_geo g, pct_hist_soa h, const uint32_t* __restrict__ key
     });
 }
 
+/* The same walk with the increments of a slab of `wslices` slices kept in
+ * LDS: a workgroup takes SMW_CHUNK consecutive walkers of the order above
+ * (they enter at the same height), counts the voxels of its window in 16-bit
+ * LDS halves (two per word; a chunk has fewer than 65536 histories and adds
+ * at most 1 per history to a half, so a half cannot overflow), sends the other
+ * voxels to global atomics, and adds the window to the counts with one
+ * coalesced pass. Window: the entry slice and its neighbour on the side of
+ * the entry's half slice. */
+constexpr int SMW_THREADS = 1024;
+constexpr int SMW_PER_THREAD = 16;
+constexpr int64_t SMW_CHUNK = (int64_t)SMW_THREADS * SMW_PER_THREAD;
+static_assert(SMW_CHUNK < 65536, "16-bit window counters");
+
+__global__ void __launch_bounds__(SMW_THREADS)
+k_walk_count_window(pct_geo g, pct_hist_soa h, const uint32_t* __restrict__ keys, const uint32_t* __restrict__ order,
+                    int wslices, int32_t* __restrict__ counts) {
+    extern __shared__ uint32_t s_win[];
+    const int64_t base = (int64_t)blockIdx.x * SMW_CHUNK;
+    const uint32_t key0 = keys[base];
+    if (key0 == 0xFFFFFFFFu) return;                /* no walker in this chunk (uniform) */
+    const int64_t plane = (int64_t)g.C * g.R;
+    const int64_t wvox = wslices * plane;
+    const int64_t wwords = (wvox + 1) / 2;
+    const int vq = (int)(key0 >> 21);
+    int wb = vq >> 1;
+    if (wslices == 2 && (vq & 1) == 0) wb -= 1;
+    if (wb > g.S - wslices) wb = g.S - wslices;
+    if (wb < 0) wb = 0;
+    const int64_t lo = wb * plane, hi = lo + wvox;
+    for (int64_t w = threadIdx.x; w < wwords; w += SMW_THREADS) s_win[w] = 0u;
+    __syncthreads();
+    for (int j = 0; j < SMW_PER_THREAD; ++j) {
+        const int64_t q = base + (int64_t)j * SMW_THREADS + threadIdx.x;
+        if (q >= h.n || keys[q] == 0xFFFFFFFFu) break;
+        const int64_t i = order[q];
+        int64_t prev = -1;
+        carve_walk(g, h.x_entry[i], h.y_entry[i], h.z_entry[i], h.x_exit[i], h.y_exit[i], h.z_exit[i], [&](int64_t v) {
+            if (v < 0 || v >= g.V) return;
+            /* a walk's column, row and slice indices are monotone (the slice
+             * index stops at 0), so a voxel recurs only as the previous one;
+             * recurrences go to global memory, keeping each history's LDS
+             * increment per voxel at most 1 */
+            const bool again = v == prev;
+            prev = v;
+            if (!again && v >= lo && v < hi) {
+                const int64_t o = v - lo;
+                atomicAdd(&s_win[o >> 1], (o & 1) ? 0x10000u : 1u);
+            } else {
+                atomicAdd(&counts[v], 1);
+            }
+        });
+    }
+    __syncthreads();
+    for (int64_t w = threadIdx.x; w < wwords; w += SMW_THREADS) {
+        const uint32_t c = s_win[w];
+        if (!c) continue;
+        const int64_t o = 2 * w;
+        if (c & 0xFFFFu) atomicAdd(&counts[lo + o], (int)(c & 0xFFFFu));
+        if ((c >> 16) && o + 1 < wvox) atomicAdd(&counts[lo + o + 1], (int)(c >> 16));
+    }
+}
+
 /* initialize_hull_GPU<int> (.cu:3316) for the MSC counts */
 __global__ void k_counts_init(pct_geo g, int32_t* __restrict__ counts) {
     int64_t v = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
@@ -392,8 +454,35 @@ extern "C" int pct_sm_count(pct_ctx* ctx, const pct_hist_soa* hist, const uint8_
     if ((rc = pct_ensure(ctx, w[4], tb))) return rc;
     tb = w[4].bytes;
     PCT_HIP_CHECK(ctx, rocprim::radix_sort_pairs(w[4].ptr, tb, k_in, k_out, o_in, o_out, nn, 0, 32, ctx->stream));
-    hipLaunchKernelGGL(k_walk_count_ordered, dim3(pct_grid(n, 256)), dim3(256), 0, ctx->stream, ctx->g, *hist, k_out,
-                       o_out, counts);
+    /* the largest slab window (2 or 1 slices of 16-bit counters) the
+     * workgroup's LDS holds, decided once per context */
+    if (ctx->sm_window < 0) {
+        int lim = 0, optin = 0;
+        (void)hipDeviceGetAttribute(&lim, hipDeviceAttributeMaxSharedMemoryPerBlock, ctx->device);
+        (void)hipDeviceGetAttribute(&optin, hipDeviceAttributeSharedMemPerBlockOptin, ctx->device);
+        const int64_t avail = optin > lim ? optin : lim;
+        const int64_t plane = (int64_t)ctx->g.C * ctx->g.R;
+        ctx->sm_window = 0;
+        for (int ws = 2; ws >= 1 && ctx->sm_window == 0; --ws) {
+            const int64_t bytes = 4 * ((ws * plane + 1) / 2);
+            if (ws > ctx->g.S || bytes > avail) continue;
+            if (bytes > lim && hipFuncSetAttribute((const void*)k_walk_count_window,
+                                                   hipFuncAttributeMaxDynamicSharedMemorySize, (int)bytes) != hipSuccess) {
+                (void)hipGetLastError();
+                continue;
+            }
+            ctx->sm_window = ws;
+        }
+    }
+    if (ctx->sm_window > 0) {
+        const int ws = ctx->sm_window;
+        const size_t bytes = 4 * (size_t)((ws * (int64_t)ctx->g.C * ctx->g.R + 1) / 2);
+        hipLaunchKernelGGL(k_walk_count_window, dim3((unsigned)((n + SMW_CHUNK - 1) / SMW_CHUNK)), dim3(SMW_THREADS), bytes,
+                           ctx->stream, ctx->g, *hist, k_out, o_out, ws, counts);
+    } else {
+        hipLaunchKernelGGL(k_walk_count_ordered, dim3(pct_grid(n, 256)), dim3(256), 0, ctx->stream, ctx->g, *hist, k_out,
+                           o_out, counts);
+    }
     PCT_HIP_CHECK(ctx, hipGetLastError());
     return PCT_OK;
 }

@@ -90,6 +90,8 @@ struct pct_ctx {
     pct_buffer research[8];
     /* walk order of the SM counts: keys, indices (in/out), sort storage */
     pct_buffer walk_order[5];
+    /* slices of the SM count's LDS window (-1 = not decided, 0 = none) */
+    int sm_window = -1;
     pct_buffer hull_tmp, scan_tmp, steps, offsets, path_store, path_len, chord, xupd, counts;
     pct_buffer blk_offsets, blk_voxels, blk_lengths, blk_chord;
     /* phase timing of pct_drop_iteration (HIP events on `stream`) */

@@ -150,6 +150,45 @@ def test_sm_hull_bit_exact(hull_sample):
     ctx.close()
 
 
+@pytest.mark.parametrize("config", ["mini", "cfg2"])
+def test_sm_counts_random_segments(config):
+    """SM counts of random entry/exit segments inside the volume, a quarter of
+    them steep (the slice index of a walk going up stops at 0 and a voxel
+    recurs), a few missed or below the WEPL threshold, equal the oracle's;
+    on the cfg2 grid the walks use the two-slice LDS window of 156 KB"""
+    import oracle
+    import pct_amd
+    import torch
+    from pct_amd import configs
+    p = configs.params(config)
+    rng = np.random.default_rng(31)
+    n = 40_000
+    r, hz = p.recon_cyl_radius, p.recon_cyl_height / 2
+    h = oracle.HostHistories(n)
+    ang = rng.uniform(0, 2 * np.pi, (2, n))
+    rad = r * np.sqrt(rng.uniform(0, 0.98, (2, n)))
+    h.x_entry[:], h.y_entry[:] = rad[0] * np.cos(ang[0]), rad[0] * np.sin(ang[0])
+    h.x_exit[:], h.y_exit[:] = rad[1] * np.cos(ang[1]), rad[1] * np.sin(ang[1])
+    h.z_entry[:] = rng.uniform(-hz, hz, n) * 0.9
+    steep = rng.uniform(size=n) < 0.25
+    h.z_exit[:] = np.where(steep, rng.choice([-1.3, 1.3], n) * hz, h.z_entry + rng.normal(0, 0.05, n))
+    short = steep & (rng.uniform(size=n) < 0.5)             # steep and short in x/y
+    h.x_exit[short] = h.x_entry[short] + rng.normal(0, 0.05, short.sum())
+    h.y_exit[short] = h.y_entry[short] + rng.normal(0, 0.05, short.sum())
+    h.wepl[:] = rng.uniform(0, 30, n).astype(np.float32)
+    missed = (rng.uniform(size=n) < 0.02).astype(np.uint8)
+    exp = oracle.sm_counts(p, h, missed)
+    assert (exp - oracle.hull_init(p)).max() > 1
+    ctx = pct_amd.Context(p)
+    d = pct_amd.Histories(n, "cuda")
+    for f, a in h.arrays().items():
+        a = np.ascontiguousarray(a)
+        getattr(d, f).copy_(torch.from_numpy(a.view(np.int32) if a.dtype == np.uint32 else a).cuda())
+    got = ctx.sm_counts(d, torch.from_numpy(missed).cuda())
+    assert np.array_equal(got.cpu().numpy(), exp)
+    ctx.close()
+
+
 def test_sm_pipeline_matches_oracle():
     """raw scan -> SM hull -> x0 -> reconstruction histories on the GPU
     equal the oracle's preprocessing with the SM hull"""
