@@ -220,10 +220,7 @@ extern "C" int pct_drop_lists_iteration(pct_ctx* ctx, int64_t n, const int64_t* 
         return pct_fail(ctx, PCT_E_INVALID, "pct_drop_lists_iteration: unknown variant %d", variant);
     if (!x || !x_update || !counts) return pct_fail(ctx, PCT_E_INVALID, "pct_drop_lists_iteration: null image or state");
     if (n == 0) return PCT_OK;
-    if (!offsets || !voxels || !chord || !wepl)
-        return pct_fail(ctx, PCT_E_INVALID, "pct_drop_lists_iteration: null list array");
-    if ((variant == PCT_LISTS_ERROR_RED || variant == PCT_LISTS_ERROR) && !errors)
-        return pct_fail(ctx, PCT_E_INVALID, "pct_drop_lists_iteration: this variant needs path errors");
+    if (!offsets || !chord || !wepl) return pct_fail(ctx, PCT_E_INVALID, "pct_drop_lists_iteration: null list array");
     const int64_t V = ctx->g.V;
     int64_t ends[2];
     PCT_HIP_CHECK(ctx, hipMemcpyAsync(&ends[0], offsets, sizeof(int64_t), hipMemcpyDeviceToHost, ctx->stream));
@@ -233,6 +230,10 @@ extern "C" int pct_drop_lists_iteration(pct_ctx* ctx, int64_t n, const int64_t* 
     const int64_t E = ends[1];
     if (E >= (int64_t)UINT32_MAX || n >= (int64_t)UINT32_MAX)
         return pct_fail(ctx, PCT_E_INVALID, "pct_drop_lists_iteration: more than 2^32 intersections or lists");
+    /* the voxel and error arrays are read only when there are intersections */
+    if (E > 0 && !voxels) return pct_fail(ctx, PCT_E_INVALID, "pct_drop_lists_iteration: null voxel array");
+    if (E > 0 && (variant == PCT_LISTS_ERROR_RED || variant == PCT_LISTS_ERROR) && !errors)
+        return pct_fail(ctx, PCT_E_INVALID, "pct_drop_lists_iteration: this variant needs path errors");
     int flags = 0;
     int rc = pct_take_status(ctx, &flags);
     if (rc) return rc;

@@ -134,3 +134,62 @@ def test_every_history_misses():
     assert pre.hist.n == 0 and pre.counts["post_cut"] == 0
     assert np.array_equal(pre.x0.cpu().numpy().view(np.uint32), x0_o.view(np.uint32))
     ctx.close()
+
+
+def test_sm_research_chord_on_empty_and_degenerate_input(mini_host):
+    """the SM hull, the chord lengths and the research list loop on no
+    histories, on lists without intersections, and on single lists with
+    blocks of one -- as the oracle"""
+    import oracle
+    import pct_amd
+    from pct_amd import configs, pipeline
+    p = configs.params("mini")
+    V = p.columns * p.rows * p.slices
+    ctx = pct_amd.Context(p)
+    torch = ctx.torch
+    cfg = configs.synth_cfg("mini", n_histories=0)
+    raw = ctx.generate(cfg)
+    hist, missed = ctx.recon_volume_intersections(raw)
+    counts = ctx.sm_counts(hist, missed)
+    init = oracle.hull_init(p).astype(np.int32)
+    assert np.array_equal(counts.cpu().numpy(), init)
+    thr = torch.full((p.slices,), -1, dtype=torch.int32, device="cuda")
+    edge = ctx.sm_edge(counts, thr)
+    exp_edge, exp_thr = oracle.sm_edge(p, init)
+    assert np.array_equal(edge.cpu().numpy(), exp_edge) and np.array_equal(thr.cpu().numpy(), exp_thr)
+    assert ctx.chord_lengths(hist).numel() == 0
+    pre = pipeline.preprocess(ctx, raw, hull_kind="sm")
+    _, x0_o, hull_o, _ = oracle.preprocess(p, pct_amd.generate_host(cfg).numpy(), hull_kind="sm")
+    assert np.array_equal(pre.x0.cpu().numpy().view(np.uint32), x0_o.view(np.uint32))
+    # lists: none; five without intersections; one list per block
+    x0 = (1.0 + 0.1 * np.sin(np.arange(V))).astype(np.float32)
+    x = torch.from_numpy(x0.copy()).cuda()
+    xu = torch.zeros(V, dtype=torch.float64, device="cuda")
+    cnt = torch.zeros(V, dtype=torch.int32, device="cuda")
+    one = torch.zeros(1, dtype=torch.float32, device="cuda")
+    empty_offs = torch.zeros(1, dtype=torch.int64, device="cuda")
+    none = torch.zeros(0, dtype=torch.int32, device="cuda")
+    ctx.drop_lists_iteration(empty_offs, none, one, one.double(), one, x, xu, cnt, 2, 0.01, pct_amd.PCT_LISTS_ERROR_RED)
+    five = torch.zeros(6, dtype=torch.int64, device="cuda")
+    ctx.drop_lists_iteration(five, none, one, torch.full((5,), 0.3, dtype=torch.float64, device="cuda"),
+                             torch.ones(5, device="cuda"), x, xu, cnt, 2, 0.01, pct_amd.PCT_LISTS_ERROR_RED)
+    assert torch.equal(x.cpu(), torch.from_numpy(x0)) and int(cnt.sum()) == 0
+    hr = mini_host["hr"].select(np.arange(mini_host["hr"].n) < 3)
+    lists = oracle.mlp_paths(p, hr)
+    offsets, vox = oracle.lists_csr(lists)
+    rng = np.random.default_rng(2)
+    err = rng.normal(0, 1, vox.size).astype(np.float32)
+    chord = np.array([0.31, 0.29, 0.33])
+    for variant in range(5):
+        xo, xuo, cno = x0.copy(), np.zeros(V), np.zeros(V, np.uint32)
+        oracle.drop_lists_research(V, offsets, vox, err, chord, hr.wepl, xo, xuo, cno, 1, 0.05, p.eta, p.psi_sign,
+                                   variant)
+        xd = torch.from_numpy(x0.copy()).cuda()
+        xu.zero_()
+        cnt.zero_()
+        ctx.drop_lists_iteration(torch.from_numpy(offsets).cuda(), torch.from_numpy(vox.view(np.int32)).cuda(),
+                                 torch.from_numpy(err).cuda(), torch.from_numpy(chord).cuda(),
+                                 torch.from_numpy(np.ascontiguousarray(hr.wepl)).cuda(), xd, xu, cnt, 1, 0.05, variant)
+        assert np.array_equal(xd.cpu().numpy().view(np.uint32), xo.view(np.uint32)), variant
+        assert int(cnt.sum()) == 0 and np.sum(xo != x0) > 0
+    ctx.close()
