@@ -150,20 +150,24 @@ def test_sm_hull_bit_exact(hull_sample):
     ctx.close()
 
 
-@pytest.mark.parametrize("config", ["mini", "cfg2", "mini300"])
+@pytest.mark.parametrize("config", ["mini", "cfg2", "mini300", "mini1"])
 def test_sm_counts_random_segments(config):
     """SM counts of random entry/exit segments inside the volume, a quarter of
     them steep (the slice index of a walk going up stops at 0 and a voxel
     recurs), a few missed or below the WEPL threshold, equal the oracle's;
     on the cfg2 grid the walks use the two-slice LDS window of 156 KB, on a
-    300x300 grid (one slice of 16-bit counters is 180 KB) no window"""
+    300x300 grid (one slice of 16-bit counters is 180 KB) no window, on a
+    single-slice grid a one-slice window"""
     import oracle
     import pct_amd
     import torch
     from pct_amd import configs
-    p = configs.params("mini" if config == "mini300" else config)
+    p = configs.params("mini" if config in ("mini300", "mini1") else config)
     if config == "mini300":
         p.columns = p.rows = 300
+    if config == "mini1":
+        p.slices = 1
+        p.recon_cyl_height = p.slice_thickness
     rng = np.random.default_rng(31)
     n = 40_000
     r, hz = p.recon_cyl_radius, p.recon_cyl_height / 2
