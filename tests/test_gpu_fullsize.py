@@ -1,0 +1,61 @@
+"""GPU checks at the full size of the headline workload (200x200x32 voxels,
+20 M histories -> ~17 M reconstruction histories, 5325 DROP blocks): the
+oracle cannot run there in test time, so size-independent properties:
+
+  - the schedules of an iteration -- MLP lists recomputed beside the block
+    loop in chunks (default), cached lists, MLP first then the block loop,
+    other chunkings -- give bit-identical images over two iterations;
+  - lambda = 0 leaves the image unchanged;
+  - the plan's recomputed lists match the cached ones (no plan flag);
+  - FBP x0 is zero outside the hull and the reconstruction histories all
+    start on a hull voxel (first_voxel inside the image).
+The bench compares one iteration over the first 276 blocks of this scan
+with the oracle (rmse_vs_oracle).
+"""
+import numpy as np
+import pytest
+
+pytestmark = pytest.mark.gpu
+
+
+def test_full_size_schedules_agree():
+    import pct_amd
+    from pct_amd import configs, pipeline
+    p = configs.params("cfg2")
+    ctx = pct_amd.Context(p)
+    torch = ctx.torch
+    raw = ctx.generate(configs.synth_cfg("cfg2"))
+    pre = pipeline.preprocess(ctx, raw)
+    del raw
+    hist, B, V = pre.hist, p.drop_block_size, ctx.V
+    assert hist.n > 15_000_000
+    fv = hist.first_voxel.to(torch.int64)
+    assert int(fv.min()) >= 0 and int(fv.max()) < V
+    bits = pre.hull.cpu().numpy().view(np.uint32)
+    idx = np.arange(V)
+    in_hull = ((bits[idx >> 5] >> (idx & 31)) & 1).astype(bool)
+    x0 = pre.x0.cpu().numpy()
+    assert in_hull.sum() > 100_000 and np.all(x0[~in_hull] == 0) and np.any(x0[in_hull] != 0)
+    assert np.all(in_hull[fv.cpu().numpy()])
+    print(f"cfg2 full size: {hist.n} reconstruction histories", flush=True)
+    images = {}
+    for name, mode, knobs in (("default", pct_amd.PCT_MLP_FP64, {}),
+                              ("cached", pct_amd.PCT_MLP_FP64 | pct_amd.PCT_DROP_CACHED, {}),
+                              ("sequential", pct_amd.PCT_MLP_FP64, {"overlap": 2}),
+                              ("chunks", pct_amd.PCT_MLP_FP64, {"chunk_first": 5, "chunk_max": 40})):
+        pct_amd.set_tuning(ctx, overlap=0, chunk_first=0, chunk_max=0, mlp_waves=0)
+        pct_amd.set_tuning(ctx, **knobs)
+        x = pre.x0.clone()
+        for _ in range(2):
+            ctx.drop_iteration(hist, x, B, p.lambda_, mode)
+        images[name] = x.cpu().numpy()
+        print(f"  {name} done", flush=True)
+    pct_amd.set_tuning(ctx, overlap=0, chunk_first=0, chunk_max=0, mlp_waves=0)
+    ref = images["default"]
+    assert np.sum(ref != pre.x0.cpu().numpy()) > 100_000
+    for name, img in images.items():
+        assert np.array_equal(img.view(np.uint32), ref.view(np.uint32)), name
+    x = pre.x0.clone()
+    ctx.drop_iteration(hist, x, B, 0.0, pct_amd.PCT_MLP_FP64)
+    assert torch.equal(x, pre.x0)
+    ctx.close()
