@@ -4,7 +4,8 @@ oracle cannot run there in test time, so size-independent properties:
 
   - the schedules of an iteration -- MLP lists recomputed beside the block
     loop in chunks (default), cached lists, MLP first then the block loop,
-    other chunkings -- give bit-identical images over two iterations;
+    other chunkings -- give bit-identical images over two iterations, also
+    for the research variants and the precision form;
   - lambda = 0 leaves the image unchanged;
   - the plan's recomputed lists match the cached ones (no plan flag);
   - FBP x0 is zero outside the hull and the reconstruction histories all
@@ -18,15 +19,23 @@ import pytest
 pytestmark = pytest.mark.gpu
 
 
-def test_full_size_schedules_agree():
+@pytest.fixture(scope="module")
+def full():
     import pct_amd
     from pct_amd import configs, pipeline
     p = configs.params("cfg2")
     ctx = pct_amd.Context(p)
-    torch = ctx.torch
     raw = ctx.generate(configs.synth_cfg("cfg2"))
     pre = pipeline.preprocess(ctx, raw)
     del raw
+    yield p, ctx, pre
+    ctx.close()
+
+
+def test_full_size_schedules_agree(full):
+    import pct_amd
+    p, ctx, pre = full
+    torch = ctx.torch
     hist, B, V = pre.hist, p.drop_block_size, ctx.V
     assert hist.n > 15_000_000
     fv = hist.first_voxel.to(torch.int64)
@@ -58,4 +67,22 @@ def test_full_size_schedules_agree():
     x = pre.x0.clone()
     ctx.drop_iteration(hist, x, B, 0.0, pct_amd.PCT_MLP_FP64)
     assert torch.equal(x, pre.x0)
-    ctx.close()
+
+
+@pytest.mark.parametrize("form", ["robust", "robust2", "precision"])
+def test_full_size_forms_schedules_agree(full, form):
+    """the research variants and the precision form: lists recomputed beside
+    the block loop and cached lists give the same image"""
+    import pct_amd
+    p, ctx, pre = full
+    mode = {"robust": pct_amd.PCT_MLP_FP64 | pct_amd.PCT_DROP_ROBUST,
+            "robust2": pct_amd.PCT_MLP_FP64 | pct_amd.PCT_DROP_ROBUST2,
+            "precision": pct_amd.PCT_PRECISION}[form]
+    out = []
+    for extra in (0, pct_amd.PCT_DROP_CACHED):
+        x = pre.x0.clone()
+        for _ in range(2):
+            ctx.drop_iteration(pre.hist, x, p.drop_block_size, p.lambda_, mode | extra)
+        out.append(x.cpu().numpy())
+    assert np.sum(out[0] != pre.x0.cpu().numpy()) > 100_000
+    assert np.array_equal(out[0].view(np.uint32), out[1].view(np.uint32))
