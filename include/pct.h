@@ -280,7 +280,8 @@ int pct_chord_lengths(pct_ctx* ctx, const pct_hist_soa* hist, int float_average,
  * arrays; offsets[0] = 0), with WEPL wepl[k], chord length chord[k] and, for
  * the error variants, errors[offsets[k] + j] for its j-th intersection
  * (may be null otherwise). With no intersection at all (offsets[n] = 0)
- * voxels and errors may be null. After every full block of block_size lists
+ * voxels and errors may be null. Lists of up to 65535 voxels (the MLP's
+ * lists are at most MAX_INTERSECTIONS long), blocks of 1..65535 lists. After every full block of block_size lists
  * ((n+1) % BLOCK_SIZE_RECON == 0, .cu:7554) the update runs; x_update
  * (double) and counts (uint32), V values each, are the caller's and persist
  * between calls like the reference's x_update_h / block_counts_h: zero them

@@ -119,8 +119,8 @@ k_lists_dot(lists_args a, int64_t k0, int64_t count, const float* __restrict__ x
  * the run's contributions, in list order, onto x_update[v] and counts[v] */
 __global__ void __launch_bounds__(256)
 k_lists_accumulate(lists_args a, int64_t k0, int64_t e_begin, int64_t e_end, int vbits,
-                   const uint64_t* __restrict__ keys, const uint32_t* __restrict__ entries,
-                   const uint32_t* __restrict__ entry_list, const float* __restrict__ x, const double* __restrict__ r,
+                   const uint64_t* __restrict__ keys, const uint64_t* __restrict__ entries,
+                   const float* __restrict__ x, const double* __restrict__ r,
                    const double* __restrict__ aa, double* __restrict__ x_update, uint32_t* __restrict__ counts) {
     int64_t s = e_begin + (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
     if (s >= e_end) return;
@@ -134,10 +134,13 @@ k_lists_accumulate(lists_args a, int64_t k0, int64_t e_begin, int64_t e_end, int
     double acc = x_update[v];
     uint32_t cnt = counts[v];
     for (int64_t q = s; q < e_end && keys[q] == key; ++q) {
-        const uint32_t e = entries[q];
-        const int64_t k = entry_list[e];
-        const int64_t t = k - k0;
-        const double c = entry_chord(a, k, (int64_t)e - a.offsets[k], e);
+        /* entry e, its list's index t in the block and its position i */
+        const uint64_t pl = entries[q];
+        const uint32_t e = (uint32_t)pl;
+        const int64_t t = (int64_t)((pl >> 32) & 0xFFFFu);
+        const int64_t i = (int64_t)(pl >> 48);
+        const int64_t k = k0 + t;
+        const double c = entry_chord(a, k, i, e);
         acc += psi_form ? c * a.lambda * (r[t] / (aa[t] + psi_i)) : c * a.lambda * r[t];
         cnt += 1;
     }
@@ -171,21 +174,25 @@ k_lists_update_dense(int64_t V, float* __restrict__ x, double* __restrict__ x_up
     counts[v] = 0;
 }
 
-/* sort keys (block << vbits | voxel, out-of-image voxels as V), entry ids
- * and the list of every entry; one wavefront per list */
+/* sort keys (block << vbits | voxel, out-of-image voxels as V) and the
+ * payload of every entry: e | (list index in its block) << 32 | (position in
+ * its list) << 48; one wavefront per list. Lists longer than 65535 are
+ * flagged (the position has 16 bits). */
 __global__ void __launch_bounds__(256)
 k_lists_keys(const int64_t* __restrict__ offsets, const uint32_t* __restrict__ voxels, int64_t n, int64_t V,
-             int block_size, int vbits, uint64_t* __restrict__ keys, uint32_t* __restrict__ entries,
-             uint32_t* __restrict__ entry_list) {
+             int block_size, int vbits, uint64_t* __restrict__ keys, uint64_t* __restrict__ entries,
+             int* __restrict__ status) {
     int64_t k = ((int64_t)blockIdx.x * blockDim.x + threadIdx.x) >> 6;
     int lane = threadIdx.x & 63;
     if (k >= n) return;
+    const int64_t e0 = offsets[k], e1 = offsets[k + 1];
+    if (lane == 0 && e1 - e0 > 0xFFFF) atomicOr(status, PCT_STATUS_CAPACITY);
     const uint64_t blk = (uint64_t)(k / block_size) << vbits;
-    for (int64_t e = offsets[k] + lane; e < offsets[k + 1]; e += 64) {
+    const uint64_t t = (uint64_t)(k % block_size) << 32;
+    for (int64_t e = e0 + lane; e < e1; e += 64) {
         uint32_t v = voxels[e];
         keys[e] = blk | (uint64_t)(v < V ? v : V);
-        entries[e] = (uint32_t)e;
-        entry_list[e] = (uint32_t)k;
+        entries[e] = (uint64_t)(uint32_t)e | t | ((uint64_t)((e - e0) & 0xFFFF) << 48);
     }
 }
 
@@ -215,7 +222,8 @@ extern "C" int pct_drop_lists_iteration(pct_ctx* ctx, int64_t n, const int64_t* 
                                         double* x_update, uint32_t* counts, int block_size, float lambda,
                                         int variant) {
     if (!ctx) return PCT_E_INVALID;
-    if (n < 0 || block_size <= 0) return pct_fail(ctx, PCT_E_INVALID, "pct_drop_lists_iteration: bad n or block size");
+    if (n < 0 || block_size <= 0 || block_size > 0xFFFF)
+        return pct_fail(ctx, PCT_E_INVALID, "pct_drop_lists_iteration: bad n, or block size outside 1..65535");
     if (variant < PCT_LISTS_DROP || variant > PCT_LISTS_ERROR_RED_SP)
         return pct_fail(ctx, PCT_E_INVALID, "pct_drop_lists_iteration: unknown variant %d", variant);
     if (!x || !x_update || !counts) return pct_fail(ctx, PCT_E_INVALID, "pct_drop_lists_iteration: null image or state");
@@ -248,25 +256,25 @@ extern "C" int pct_drop_lists_iteration(pct_ctx* ctx, int64_t n, const int64_t* 
     const size_t EE = (size_t)(E > 0 ? E : 1);
     pct_buffer* rb = ctx->research;
     if ((rc = pct_ensure(ctx, rb[0], 8 * EE)) || (rc = pct_ensure(ctx, rb[1], 8 * EE)) ||
-        (rc = pct_ensure(ctx, rb[2], 4 * EE)) || (rc = pct_ensure(ctx, rb[3], 4 * EE)) ||
-        (rc = pct_ensure(ctx, rb[4], 4 * EE)) || (rc = pct_ensure(ctx, rb[5], 16 * (size_t)block_size)) ||
-        (rc = pct_ensure(ctx, rb[6], 8 * (size_t)(nblocks + 1))))
+        (rc = pct_ensure(ctx, rb[2], 8 * EE)) || (rc = pct_ensure(ctx, rb[3], 8 * EE)) ||
+        (rc = pct_ensure(ctx, rb[5], 16 * (size_t)block_size)) || (rc = pct_ensure(ctx, rb[6], 8 * (size_t)(nblocks + 1))))
         return rc;
     uint64_t* keys_in = (uint64_t*)rb[0].ptr;
     uint64_t* keys = (uint64_t*)rb[1].ptr;
-    uint32_t* ent_in = (uint32_t*)rb[2].ptr;
-    uint32_t* entries = (uint32_t*)rb[3].ptr;
-    uint32_t* entry_list = (uint32_t*)rb[4].ptr;
+    uint64_t* ent_in = (uint64_t*)rb[2].ptr;
+    uint64_t* entries = (uint64_t*)rb[3].ptr;
     double* r = (double*)rb[5].ptr;
     double* aa = r + block_size;
     int64_t* blk_off_d = (int64_t*)rb[6].ptr;
 
     if (E > 0) {
         hipLaunchKernelGGL(k_lists_keys, dim3((unsigned)((n + 3) / 4)), dim3(256), 0, ctx->stream, offsets, voxels, n, V,
-                           block_size, vbits, keys_in, ent_in, entry_list);
+                           block_size, vbits, keys_in, ent_in, ctx->d_status);
+        if ((rc = pct_take_status(ctx, &flags))) return rc;
+        if (flags) return pct_fail(ctx, PCT_E_INVALID, "pct_drop_lists_iteration: a list of more than 65535 voxels");
         size_t tb = 0;
         PCT_HIP_CHECK(ctx, rocprim::radix_sort_pairs(nullptr, tb, (uint64_t*)nullptr, (uint64_t*)nullptr,
-                                                     (uint32_t*)nullptr, (uint32_t*)nullptr, (size_t)E, 0, end_bit,
+                                                     (uint64_t*)nullptr, (uint64_t*)nullptr, (size_t)E, 0, end_bit,
                                                      ctx->stream));
         if ((rc = pct_ensure(ctx, rb[7], tb))) return rc;
         tb = rb[7].bytes;
@@ -290,7 +298,7 @@ extern "C" int pct_drop_lists_iteration(pct_ctx* ctx, int64_t n, const int64_t* 
                            ctx->d_status);
         if (e1 > e0)
             hipLaunchKernelGGL(k_lists_accumulate, dim3(pct_grid(e1 - e0, 256)), dim3(256), 0, ctx->stream, a, k0, e0, e1,
-                               vbits, keys, entries, entry_list, x, r, aa, x_update, counts);
+                               vbits, keys, entries, x, r, aa, x_update, counts);
         if (count == block_size) {                      /* (n + 1) % BLOCK_SIZE_RECON == 0 (.cu:7554) */
             if (first_update)
                 hipLaunchKernelGGL(k_lists_update_dense, dim3(pct_grid(V, 256)), dim3(256), 0, ctx->stream, V, x,
