@@ -302,21 +302,27 @@ def research_forms(ctx, hist, x0, B, p, steps, timed, log, m_lists=2_000_000):
     xv = x0.clone()
     xu = torch.zeros(ctx.V, dtype=torch.float64, device=lens.device)
     cnt = torch.zeros(ctx.V, dtype=torch.int32, device=lens.device)
-    ctx.drop_lists_iteration(offsets, voxels, errors, chord, sub.wepl, xv, xu, cnt, B, p.lambda_)
-    torch.cuda.synchronize()
-    ev0, ev1 = torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)
-    ev0.record(ctx.stream)
-    for _ in range(steps):
-        ctx.drop_lists_iteration(offsets, voxels, errors, chord, sub.wepl, xv, xu, cnt, B, p.lambda_)
-    ev1.record(ctx.stream)
-    torch.cuda.synchronize()
-    ms = ev0.elapsed_time(ev1) / steps
+    rule = pct_amd.PCT_LISTS_ERROR_RED
+
+    def timed_lists(flags, k):
+        torch.cuda.synchronize()
+        ev0, ev1 = torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)
+        ev0.record(ctx.stream)
+        for _ in range(k):
+            ctx.drop_lists_iteration(offsets, voxels, errors, chord, sub.wepl, xv, xu, cnt, B, p.lambda_, rule | flags)
+        ev1.record(ctx.stream)
+        torch.cuda.synchronize()
+        return ev0.elapsed_time(ev1) / k
+
+    timed_lists(0, 1)                                   # warm-up
+    ms_sort = timed_lists(0, 1)                         # a call on new lists: their sort included
+    ms = timed_lists(pct_amd.PCT_LISTS_REUSE, steps)    # later iterations over the same lists
     res["research_lists_drop_blocks_error_red"] = {
-        "ms_per_iteration": round(ms, 3), "lists": m, "intersections": E,
-        "lists_per_s": round(m / (ms / 1e3), 1),
+        "ms_per_iteration": round(ms, 3), "ms_first_iteration_with_sort": round(ms_sort, 3), "lists": m,
+        "intersections": E, "lists_per_s": round(m / (ms / 1e3), 1),
         "note": "one wavefront per list forms the terms and one lane adds them in list order, and the entries "
-                "are sorted by (block, voxel) each call, so the result equals the reference loop bit for bit; "
-                "not the production path"}
+                "are sorted by (block, voxel) once per list set, so the result equals the reference loop bit for "
+                "bit; not the production path"}
     log("research forms: " + ", ".join(f"{k} {v['ms_per_iteration']:.2f} ms" for k, v in res.items()))
     return res
 

@@ -271,6 +271,10 @@ int pct_chord_lengths(pct_ctx* ctx, const pct_hist_soa* hist, int float_average,
 #define PCT_LISTS_ERROR         3   /* DROP_blocks_error (.cu:7844-7870): chord L + e ETA L        */
 #define PCT_LISTS_ERROR_RED_SP  4   /* DROP_blocks_error_red_sp (.cu:7768-7815): chord 1.8 cm at
                                        every 5th intersection, psi                                 */
+#define PCT_LISTS_REUSE    0x100    /* or-ed to the rule: the lists (offsets, voxels, their
+                                       contents, block_size) are those of the previous call on
+                                       this context, so their sorted order is reused; with other
+                                       arrays or sizes the call sorts as usual                     */
 
 /* (e) One iteration of the CPU research reconstruction loop over voxel
  * lists given by the caller: replaces the iteration body of

@@ -11,7 +11,7 @@ import ctypes
 from . import _lib
 from . import configs
 from ._lib import (PCT_LISTS_DROP, PCT_LISTS_DROP_RED, PCT_LISTS_ERROR, PCT_LISTS_ERROR_RED,
-                   PCT_LISTS_ERROR_RED_SP, PCT_DROP_ATOMIC, PCT_DROP_CACHED, PCT_DROP_F32, PCT_DROP_NO_GRAPH, PCT_DROP_ROBUST,
+                   PCT_LISTS_ERROR_RED_SP, PCT_LISTS_REUSE, PCT_DROP_ATOMIC, PCT_DROP_CACHED, PCT_DROP_F32, PCT_DROP_NO_GRAPH, PCT_DROP_ROBUST,
                    PCT_DROP_ROBUST2, PCT_LEARNED, PCT_MLP_FP64, PCT_MLP_SURROGATE, PCT_MLP_TABLE, PCT_PRECISION,
                    PctHistSoa, PctParams, PctRawSoa, PctSynthCfg, PctSynthSoa)
 

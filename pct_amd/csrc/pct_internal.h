@@ -88,6 +88,13 @@ struct pct_ctx {
     /* research list iteration: sort keys, entries, lists, per-list factors,
      * block offsets, sort storage (pct_research.hip) */
     pct_buffer research[8];
+    /* the lists whose sorted order research[1], research[3] hold (reused by
+     * PCT_LISTS_REUSE calls on the same lists) */
+    const void* rs_offsets = nullptr;
+    const void* rs_voxels = nullptr;
+    int64_t rs_n = -1, rs_E = -1;
+    int rs_block = 0;
+    std::vector<int64_t> rs_blk_off;
     /* walk order of the SM counts: keys, indices (in/out), sort storage */
     pct_buffer walk_order[5];
     /* slices of the SM count's LDS window (-1 = not decided, 0 = none) */

@@ -222,6 +222,8 @@ extern "C" int pct_drop_lists_iteration(pct_ctx* ctx, int64_t n, const int64_t* 
                                         double* x_update, uint32_t* counts, int block_size, float lambda,
                                         int variant) {
     if (!ctx) return PCT_E_INVALID;
+    const bool reuse = (variant & PCT_LISTS_REUSE) != 0;
+    variant &= ~PCT_LISTS_REUSE;
     if (n < 0 || block_size <= 0 || block_size > 0xFFFF)
         return pct_fail(ctx, PCT_E_INVALID, "pct_drop_lists_iteration: bad n, or block size outside 1..65535");
     if (variant < PCT_LISTS_DROP || variant > PCT_LISTS_ERROR_RED_SP)
@@ -245,9 +247,13 @@ extern "C" int pct_drop_lists_iteration(pct_ctx* ctx, int64_t n, const int64_t* 
     int flags = 0;
     int rc = pct_take_status(ctx, &flags);
     if (rc) return rc;
-    hipLaunchKernelGGL(k_offsets_ok, dim3(pct_grid(n, 256)), dim3(256), 0, ctx->stream, offsets, n, ctx->d_status);
-    if ((rc = pct_take_status(ctx, &flags))) return rc;
-    if (flags) return pct_fail(ctx, PCT_E_INVALID, "pct_drop_lists_iteration: offsets decrease");
+    const bool cached = reuse && ctx->rs_offsets == (const void*)offsets && ctx->rs_voxels == (const void*)voxels &&
+                        ctx->rs_n == n && ctx->rs_E == E && ctx->rs_block == block_size;
+    if (!cached) {
+        hipLaunchKernelGGL(k_offsets_ok, dim3(pct_grid(n, 256)), dim3(256), 0, ctx->stream, offsets, n, ctx->d_status);
+        if ((rc = pct_take_status(ctx, &flags))) return rc;
+        if (flags) return pct_fail(ctx, PCT_E_INVALID, "pct_drop_lists_iteration: offsets decrease");
+    }
 
     const int64_t nblocks = (n + block_size - 1) / block_size;
     const int vbits = bits_for((uint64_t)V);              /* V itself (out-of-image marker) fits */
@@ -267,7 +273,8 @@ extern "C" int pct_drop_lists_iteration(pct_ctx* ctx, int64_t n, const int64_t* 
     double* aa = r + block_size;
     int64_t* blk_off_d = (int64_t*)rb[6].ptr;
 
-    if (E > 0) {
+    if (!cached) ctx->rs_offsets = nullptr;         /* the sort below replaces the cached order */
+    if (E > 0 && !cached) {
         hipLaunchKernelGGL(k_lists_keys, dim3((unsigned)((n + 3) / 4)), dim3(256), 0, ctx->stream, offsets, voxels, n, V,
                            block_size, vbits, keys_in, ent_in, ctx->d_status);
         if ((rc = pct_take_status(ctx, &flags))) return rc;
@@ -281,12 +288,20 @@ extern "C" int pct_drop_lists_iteration(pct_ctx* ctx, int64_t n, const int64_t* 
         PCT_HIP_CHECK(ctx, rocprim::radix_sort_pairs(rb[7].ptr, tb, keys_in, keys, ent_in, entries, (size_t)E, 0, end_bit,
                                                      ctx->stream));
     }
-    hipLaunchKernelGGL(k_block_offsets, dim3(pct_grid(nblocks + 1, 256)), dim3(256), 0, ctx->stream, offsets, n,
-                       block_size, nblocks, blk_off_d);
-    std::vector<int64_t> blk_off((size_t)nblocks + 1);
-    PCT_HIP_CHECK(ctx, hipMemcpyAsync(blk_off.data(), blk_off_d, sizeof(int64_t) * (nblocks + 1), hipMemcpyDeviceToHost,
-                                      ctx->stream));
-    PCT_HIP_CHECK(ctx, hipStreamSynchronize(ctx->stream));
+    if (!cached) {
+        hipLaunchKernelGGL(k_block_offsets, dim3(pct_grid(nblocks + 1, 256)), dim3(256), 0, ctx->stream, offsets, n,
+                           block_size, nblocks, blk_off_d);
+        ctx->rs_blk_off.assign((size_t)nblocks + 1, 0);
+        PCT_HIP_CHECK(ctx, hipMemcpyAsync(ctx->rs_blk_off.data(), blk_off_d, sizeof(int64_t) * (nblocks + 1),
+                                          hipMemcpyDeviceToHost, ctx->stream));
+        PCT_HIP_CHECK(ctx, hipStreamSynchronize(ctx->stream));
+        ctx->rs_offsets = offsets;
+        ctx->rs_voxels = voxels;
+        ctx->rs_n = n;
+        ctx->rs_E = E;
+        ctx->rs_block = block_size;
+    }
+    const std::vector<int64_t>& blk_off = ctx->rs_blk_off;
 
     lists_args a{offsets, voxels, errors, chord, wepl, V, (double)lambda, ctx->p.eta, (double)ctx->p.psi_sign, variant};
     bool first_update = true;

@@ -17,7 +17,7 @@ import ctypes
 import numpy as np
 
 from . import _lib
-from ._lib import PCT_LISTS_ERROR_RED
+from ._lib import PCT_LISTS_ERROR_RED, PCT_LISTS_REUSE
 
 PRIME_OFFSET = 26410633
 
@@ -107,8 +107,14 @@ class ResearchLoop:
         self.chord = torch.as_tensor(chord).to(device=dev, dtype=torch.float64)
         self.x_update = torch.zeros(ctx.V, dtype=torch.float64, device=dev)
         self.counts = torch.zeros(ctx.V, dtype=torch.int32, device=dev)
+        self._sorted_for = None
 
     def iterate(self, x, block_size=None, lam=None):
+        """one iteration; the lists never change, so from the second call on
+        with the same block size their sorted order is reused"""
+        block_size = self.ctx.params.drop_block_size if block_size is None else block_size
+        rule = self.variant | (PCT_LISTS_REUSE if self._sorted_for == block_size else 0)
         self.ctx.drop_lists_iteration(self.offsets, self.voxels, self.errors, self.chord, self.wepl, x,
-                                      self.x_update, self.counts, block_size, lam, self.variant)
+                                      self.x_update, self.counts, block_size, lam, rule)
+        self._sorted_for = block_size
         return x

@@ -83,8 +83,9 @@ def test_lists_iteration_bit_exact(lists_case, variant, psi_sign):
     for it in range(2):
         oracle.drop_lists_research(V, c["offsets"], c["vox"], c["err"], chord, hr.wepl, x, xu, cnt, block, lam,
                                    2.5, psi_sign, variant)
+        rule = variant | (pct_amd.PCT_LISTS_REUSE if it > 0 else 0)   # the sorted order of call 1 reused
         ctx.drop_lists_iteration(dev["offsets"], dev["voxels"], dev["errors"], dev["chord"], dev["wepl"], xd, xud, cntd,
-                                 block, lam, variant)
+                                 block, lam, rule)
         assert np.array_equal(xd.cpu().numpy().view(np.uint32), x.view(np.uint32)), f"iteration {it}"
         assert np.array_equal(xud.cpu().numpy().view(np.uint64), xu.view(np.uint64))
         assert np.array_equal(cntd.cpu().numpy().view(np.uint32), cnt)
