@@ -203,7 +203,7 @@ k_block_delta_f32(const uint32_t* __restrict__ list, int stride, const uint32_t*
     pct_blk::delta_f32(list, stride, len, chord, wepl, h0, k, x, lambda, delta, threadIdx.x & 63);
 }
 
-__global__ void __launch_bounds__(256)
+__global__ void __launch_bounds__(APPLY_THREADS)
 k_block_apply(const uint64_t* __restrict__ segs, int64_t s0, int64_t ns, const uint16_t* __restrict__ csc,
               const double* __restrict__ delta, float* __restrict__ x) {
     __builtin_amdgcn_s_setprio(3);
@@ -212,7 +212,7 @@ k_block_apply(const uint64_t* __restrict__ segs, int64_t s0, int64_t ns, const u
     pct_blk::apply_f64(segs[s0 + k], csc, delta, x);
 }
 
-__global__ void __launch_bounds__(256)
+__global__ void __launch_bounds__(APPLY_THREADS)
 k_block_apply_f32(const uint64_t* __restrict__ segs, int64_t s0, int64_t ns, const uint16_t* __restrict__ csc,
                   const float* __restrict__ delta, float* __restrict__ x) {
     __builtin_amdgcn_s_setprio(3);
@@ -234,7 +234,7 @@ k_block_delta_var(const uint32_t* __restrict__ list, int stride, const uint32_t*
 }
 
 template <int V>
-__global__ void __launch_bounds__(256)
+__global__ void __launch_bounds__(APPLY_THREADS)
 k_block_apply_var(const uint64_t* __restrict__ segs, int64_t s0, int64_t ns, const uint16_t* __restrict__ csc,
                   const double* __restrict__ out, int64_t B, const double* __restrict__ chord_b, double lambda,
                   double eta, int psi_sign, float* __restrict__ x) {
@@ -539,10 +539,10 @@ static int plan_block_loop(pct_ctx* ctx, pct_plan* pl, const pct_hist_soa* hist,
             const int64_t s0 = pl->blk_seg_h[b], ns = pl->blk_seg_h[b + 1] - s0;
             const uint16_t* cb = csc + pl->blk_entry_h[b];
             if (ns > 0 && variant == 1)
-                hipLaunchKernelGGL(k_block_apply_var<1>, dim3(pct_grid(ns, 256)), dim3(256), 0, ctx->stream, segs, s0, ns,
+                hipLaunchKernelGGL(k_block_apply_var<1>, dim3(pct_grid(ns, APPLY_THREADS)), dim3(APPLY_THREADS), 0, ctx->stream, segs, s0, ns,
                                    cb, (const double*)out, B, chord + h0, lambda, eta, psi, x);
             else if (ns > 0)
-                hipLaunchKernelGGL(k_block_apply_var<2>, dim3(pct_grid(ns, 256)), dim3(256), 0, ctx->stream, segs, s0, ns,
+                hipLaunchKernelGGL(k_block_apply_var<2>, dim3(pct_grid(ns, APPLY_THREADS)), dim3(APPLY_THREADS), 0, ctx->stream, segs, s0, ns,
                                    cb, (const double*)out, B, chord + h0, lambda, eta, psi, x);
             continue;
         }
@@ -555,10 +555,10 @@ static int plan_block_loop(pct_ctx* ctx, pct_plan* pl, const pct_hist_soa* hist,
         int64_t s0 = pl->blk_seg_h[b], ns = pl->blk_seg_h[b + 1] - s0;
         const uint16_t* cb = csc + pl->blk_entry_h[b];
         if (ns > 0 && f32)
-            hipLaunchKernelGGL(k_block_apply_f32, dim3(pct_grid(ns, 256)), dim3(256), 0, ctx->stream, segs, s0, ns, cb,
+            hipLaunchKernelGGL(k_block_apply_f32, dim3(pct_grid(ns, APPLY_THREADS)), dim3(APPLY_THREADS), 0, ctx->stream, segs, s0, ns, cb,
                                (const float*)delta, x);
         else if (ns > 0)
-            hipLaunchKernelGGL(k_block_apply, dim3(pct_grid(ns, 256)), dim3(256), 0, ctx->stream, segs, s0, ns, cb,
+            hipLaunchKernelGGL(k_block_apply, dim3(pct_grid(ns, APPLY_THREADS)), dim3(APPLY_THREADS), 0, ctx->stream, segs, s0, ns, cb,
                                (const double*)delta, x);
     }
     PCT_HIP_CHECK(ctx, hipGetLastError());

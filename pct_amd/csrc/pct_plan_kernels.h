@@ -33,6 +33,10 @@
 #ifndef DELTA_WAVES
 #define DELTA_WAVES 2
 #endif
+/* threads per workgroup of the apply kernels (one segment each) */
+#ifndef APPLY_THREADS
+#define APPLY_THREADS 256
+#endif
 #define DELTA_SLOTS 8
 
 namespace pct_blk {
