@@ -310,9 +310,21 @@ int pct_drop_prepare(pct_ctx* ctx, const pct_hist_soa* hist, int block_size, int
  * image_reconstruction_GPU (.cu:6795-6857). Equivalent to the sequence of
  * pct_drop_block_update + pct_drop_image_update per block. By default the
  * MLP lists are recomputed (and checked against the plan) in every iteration,
- * as the reference does; see the PCT_DROP_* flags. */
+ * as the reference does; see the PCT_DROP_* flags.
+ * The plan is keyed on the history arrays' addresses, the count, the block
+ * size and the MLP mode, not on their contents. By default a recomputed list
+ * that differs from the plan's (the histories changed in place) is detected
+ * by a 32-bit hash per list after the block loop ran: the call then restores
+ * x to its value before the call, marks the plan stale and returns
+ * PCT_E_INVALID; the next call rebuilds the plan. With PCT_DROP_CACHED the
+ * lists are not recomputed, so the caller must call pct_drop_invalidate (or
+ * pct_drop_prepare) after changing histories in place. */
 int pct_drop_iteration(pct_ctx* ctx, const pct_hist_soa* hist, float* x,
                        int block_size, float lambda, int mode);
+/* Marks the context's DROP plan stale (no counterpart in the reference, which
+ * recomputes every list in every block): the next pct_drop_iteration rebuilds
+ * it. */
+int pct_drop_invalidate(pct_ctx* ctx);
 
 /* Measurement (counterpart of the reference's per-iteration clock() and
  * cudaEvent timing, .cu:6773-6849): when profiling is on, pct_drop_iteration
@@ -340,6 +352,10 @@ int pct_drop_stats(pct_ctx* ctx, const pct_hist_soa* hist, int block_size, int64
 #define PCT_TUNE_CHUNK_MAX     3  /* largest chunk in blocks (default 256)     */
 #define PCT_TUNE_OVERLAP       4  /* 1: MLP beside the block loop (default),
                                      2: MLP first, then the block loop         */
+#define PCT_TUNE_BLOCK_FORM    5  /* 1: one launch per block, the voxel updates
+                                     of block b-1 beside the deltas of block b;
+                                     2: two launches per block (default; the
+                                     research variants always)                 */
 int pct_set_tuning(pct_ctx* ctx, int key, int value);
 
 /* (e) Batched form: the DROP iterations of several independent scans (each
@@ -362,7 +378,10 @@ void       pct_batch_destroy(pct_batch* b);
 int        pct_batch_prepare(pct_batch* b, const pct_hist_soa* hists, const int* block_sizes, int mode);
 /* one DROP iteration of every scan: xs[s] is scan s's image, lambdas[s] its
  * relaxation; mode as pct_drop_iteration (MLP mode | PCT_DROP_F32 |
- * PCT_DROP_CACHED | PCT_DROP_NO_GRAPH) */
+ * PCT_DROP_CACHED | PCT_DROP_NO_GRAPH), with the same contract for lists that
+ * differ from a plan (every such scan's image is restored). A context may
+ * appear once in a batch (pct_batch_create returns NULL otherwise); a plan
+ * rebuilt through its own context is picked up by the next batch call. */
 int        pct_batch_iteration(pct_batch* b, const pct_hist_soa* hists, float* const* xs, const int* block_sizes,
                                const float* lambdas, int mode);
 

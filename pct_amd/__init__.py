@@ -371,6 +371,11 @@ class Context:
         self._check(self.L.pct_drop_iteration(self._ctx, ctypes.byref(hs), _ptr(x), int(block_size), float(lam), mode),
                     "pct_drop_iteration")
 
+    def drop_invalidate(self):
+        """Marks the DROP plan stale: the next drop_iteration rebuilds it (after
+        changing histories in place when iterating with PCT_DROP_CACHED)."""
+        self._check(self.L.pct_drop_invalidate(self._ctx), "pct_drop_invalidate")
+
 
     def chord_lengths(self, hist, float_average=False):
         """effective chord length per history (float64; float_average: the
@@ -474,12 +479,14 @@ class Batch:
 
 
 TUNING_KEYS = {"mlp_waves": _lib.PCT_TUNE_MLP_WAVES, "chunk_first": _lib.PCT_TUNE_CHUNK_FIRST,
-               "chunk_max": _lib.PCT_TUNE_CHUNK_MAX, "overlap": _lib.PCT_TUNE_OVERLAP}
+               "chunk_max": _lib.PCT_TUNE_CHUNK_MAX, "overlap": _lib.PCT_TUNE_OVERLAP,
+               "block_form": _lib.PCT_TUNE_BLOCK_FORM}
 
 
 def set_tuning(ctx: "Context", **knobs):
     """Execution knobs of the DROP iteration (pct_set_tuning): mlp_waves,
-    chunk_first, chunk_max, overlap (1 beside, 2 sequential); 0 = default."""
+    chunk_first, chunk_max, overlap (1 beside, 2 sequential), block_form (1 one
+    launch per block, 2 two launches per block, the default); 0 = default."""
     for k, v in knobs.items():
         if k not in TUNING_KEYS:
             raise PctError(f"unknown tuning knob {k}")

@@ -31,7 +31,7 @@ namespace {
 struct batch_delta_args {
     int n;
     uint32_t wg0[PCT_BATCH_MAX + 1];
-    const uint32_t* list[PCT_BATCH_MAX];
+    const uint64_t* list[PCT_BATCH_MAX];
     int stride[PCT_BATCH_MAX];
     const uint32_t* len[PCT_BATCH_MAX];
     const double* chord[PCT_BATCH_MAX];
@@ -103,6 +103,10 @@ struct pct_batch {
     std::vector<float*> g_x;
     std::vector<const float*> g_wepl;
     std::vector<double> g_lambda;
+    /* generation of every scan's plan when the graphs were captured: a plan
+     * rebuilt by its own context (pct_drop_iteration, pct_drop_prepare) may
+     * hold other buffers and offsets */
+    std::vector<uint64_t> g_gen;
     bool g_f32 = false;
 };
 
@@ -114,8 +118,11 @@ static void batch_release_graphs(pct_batch* b) {
 
 extern "C" pct_batch* pct_batch_create(pct_ctx* const* scans, int n_scans) {
     if (!scans || n_scans <= 0 || n_scans > PCT_BATCH_MAX) return nullptr;
-    for (int s = 0; s < n_scans; ++s)
+    for (int s = 0; s < n_scans; ++s) {
         if (!scans[s] || scans[s]->device != scans[0]->device) return nullptr;
+        for (int r = 0; r < s; ++r)
+            if (scans[r] == scans[s]) return nullptr;     /* one plan per context: a scan at most once */
+    }
     pct_batch* b = new pct_batch();
     b->scans.assign(scans, scans + n_scans);
     if (hipStreamCreateWithFlags(&b->aux, hipStreamNonBlocking) != hipSuccess ||
@@ -172,6 +179,8 @@ static int batch_plans(pct_batch* b, const pct_hist_soa* hists, const int* block
     }
     int64_t steps = 0;
     for (pct_ctx* c : b->scans) steps = std::max(steps, c->plan->nb);
+    for (size_t s = 0; !*rebuilt && s < b->scans.size() && s < b->g_gen.size(); ++s)
+        if (b->g_gen[s] != b->scans[s]->plan->generation) batch_release_graphs(b);
     if (*rebuilt || steps != b->steps || b->chunk.empty()) {
         batch_release_graphs(b);
         b->steps = steps;
@@ -205,7 +214,7 @@ static int batch_steps(pct_batch* b, float* const* xs, const pct_hist_soa* hists
             const int64_t s0 = pl->blk_seg_h[j], ns = pl->blk_seg_h[j + 1] - s0;
             int d = da.n++;
             da.wg0[d] = dwg;
-            da.list[d] = (const uint32_t*)pl->list.ptr + pl->list_base_h[j];
+            da.list[d] = (const uint64_t*)pl->list.ptr + pl->list_base_h[j];
             da.stride[d] = pl->stride_h[j];
             da.len[d] = (const uint32_t*)pl->len.ptr;
             da.chord[d] = (const double*)pl->chord.ptr;
@@ -245,7 +254,8 @@ static int batch_graphs(pct_batch* b, float* const* xs, const pct_hist_soa* hist
     const size_t n = b->scans.size();
     bool same = b->graphs.size() + 1 == b->chunk.size() && b->g_f32 == f32 && b->g_x.size() == n;
     for (size_t s = 0; same && s < n; ++s)
-        same = b->g_x[s] == xs[s] && b->g_wepl[s] == hists[s].wepl && b->g_lambda[s] == (double)lambdas[s];
+        same = b->g_x[s] == xs[s] && b->g_wepl[s] == hists[s].wepl && b->g_lambda[s] == (double)lambdas[s] &&
+               b->g_gen.size() == n && b->g_gen[s] == b->scans[s]->plan->generation;
     if (same) return PCT_OK;
     batch_release_graphs(b);
     pct_ctx* c0 = b->scans[0];
@@ -263,6 +273,8 @@ static int batch_graphs(pct_batch* b, float* const* xs, const pct_hist_soa* hist
         b->graphs.push_back(ge);
     }
     b->g_x.assign(xs, xs + n);
+    b->g_gen.resize(n);
+    for (size_t s = 0; s < n; ++s) b->g_gen[s] = b->scans[s]->plan->generation;
     b->g_wepl.resize(n);
     b->g_lambda.resize(n);
     for (size_t s = 0; s < n; ++s) {
@@ -294,6 +306,19 @@ extern "C" int pct_batch_iteration(pct_batch* b, const pct_hist_soa* hists, floa
         PCT_HIP_CHECK(c0, hipEventCreateWithFlags(&e, hipEventDisableTiming));
         b->ev_chunk.push_back(e);
     }
+    if (recompute) {
+        /* lists that differ from a plan are only known after the block steps
+         * ran on them: keep every image to restore it then */
+        for (int s = 0; s < n; ++s) {
+            pct_ctx* cs = b->scans[s];
+            if ((rc = pct_ensure(cs, cs->x_saved, sizeof(float) * (size_t)cs->g.V))) {
+                pct_fail(c0, rc, "scan %d: %s", s, pct_last_error(cs));
+                return rc;
+            }
+            PCT_HIP_CHECK(c0, hipMemcpyAsync(cs->x_saved.ptr, xs[s], sizeof(float) * (size_t)cs->g.V,
+                                             hipMemcpyDeviceToDevice, c0->stream));
+        }
+    }
     if (c0->profile) PCT_HIP_CHECK(c0, hipEventRecord(c0->ev[0], c0->stream));
     if (recompute) {
         /* every scan's lists for the histories of each chunk of steps, on the
@@ -308,8 +333,8 @@ extern "C" int pct_batch_iteration(pct_batch* b, const pct_hist_soa* hists, floa
                 const int64_t h1 = std::min(pl->n, b->chunk[c + 1] * pl->B);
                 if (h1 <= h0) continue;
                 rc = pct_mlp_paths_on(cs, b->aux, &hists[s], h0, h1 - h0, (const int64_t*)pl->hist_off.ptr + h0,
-                                      (uint32_t*)pl->list.ptr, (uint32_t*)pl->len.ptr + h0, (double*)pl->chord.ptr + h0,
-                                      nullptr, (const uint32_t*)pl->hash.ptr + h0, mlp, false);
+                                      nullptr, nullptr, nullptr, nullptr, (const uint32_t*)pl->hash.ptr + h0, mlp,
+                                      false);
                 if (rc) {
                     pct_fail(c0, rc, "scan %d: %s", s, pct_last_error(cs));
                     return rc;
@@ -342,7 +367,14 @@ extern "C" int pct_batch_iteration(pct_batch* b, const pct_hist_soa* hists, floa
         pct_ctx* cs = b->scans[s];
         int flags = 0;
         if ((rc = pct_take_status(cs, &flags))) return rc;
-        if (flags) cs->plan->valid = false;
+        if (flags) {
+            cs->plan->valid = false;
+            if (recompute) {
+                PCT_HIP_CHECK(c0, hipMemcpyAsync(xs[s], cs->x_saved.ptr, sizeof(float) * (size_t)cs->g.V,
+                                                 hipMemcpyDeviceToDevice, c0->stream));
+                PCT_HIP_CHECK(c0, hipStreamSynchronize(c0->stream));
+            }
+        }
         if ((rc = pct_status_to_rc(cs, flags)) && first_rc == PCT_OK) {
             pct_fail(c0, rc, "scan %d: %s", s, pct_last_error(cs));
             first_rc = rc;

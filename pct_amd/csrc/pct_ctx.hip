@@ -155,7 +155,7 @@ extern "C" void pct_destroy(pct_ctx* ctx) {
     if (ctx->stream) (void)hipStreamSynchronize(ctx->stream);
     pct_buffer* bufs[] = {&ctx->hull_tmp, &ctx->scan_tmp, &ctx->steps, &ctx->offsets, &ctx->path_store, &ctx->path_len,
                           &ctx->chord, &ctx->xupd, &ctx->counts, &ctx->blk_offsets, &ctx->blk_voxels,
-                          &ctx->blk_lengths, &ctx->blk_chord};
+                          &ctx->blk_lengths, &ctx->blk_chord, &ctx->x_saved};
     for (pct_buffer* b : bufs)
         if (b->ptr) (void)hipFree(b->ptr);
     for (pct_buffer& b : ctx->research)

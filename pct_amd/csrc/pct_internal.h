@@ -83,7 +83,7 @@ struct pct_ctx {
     int net_version = 0;
     pct_buffer net_partial, net_state, net_loss, net_calib, net_stage_buf;
     /* execution knobs (pct_set_tuning; 0 = default) */
-    int tune_mlp_waves = 0, tune_chunk_first = 0, tune_chunk_max = 0, tune_overlap = 0;
+    int tune_mlp_waves = 0, tune_chunk_first = 0, tune_chunk_max = 0, tune_overlap = 0, tune_block_form = 0;
     /* workspaces, grown on demand */
     /* research list iteration: sort keys, entries, lists, per-list factors,
      * block offsets, sort storage (pct_research.hip) */
@@ -101,6 +101,9 @@ struct pct_ctx {
     int sm_window = -1;
     pct_buffer hull_tmp, scan_tmp, steps, offsets, path_store, path_len, chord, xupd, counts;
     pct_buffer blk_offsets, blk_voxels, blk_lengths, blk_chord;
+    /* the image before a recomputing DROP iteration, restored when the
+     * recomputed lists turn out to differ from the plan */
+    pct_buffer x_saved;
     /* phase timing of pct_drop_iteration (HIP events on `stream`) */
     bool profile = false;
     hipEvent_t ev[4] = {nullptr, nullptr, nullptr, nullptr};

@@ -402,7 +402,7 @@ template <class StepF, class FixF>
 __device__ __forceinline__ void walk_list(StepF step, FixF fix, int S, uint32_t carry, int64_t hoff, int64_t hcap,
                                           uint32_t* __restrict__ voxels, int lane, uint64_t lt_mask, int64_t& n_out,
                                           uint32_t& hsh_out) {
-    if (lane == 0 && hcap > 0) voxels[hoff] = carry;
+    if (lane == 0 && hcap > 0 && voxels) voxels[hoff] = carry;
     int64_t n = 1;
     uint32_t hsh = lane == 0 ? pct_list_mix(carry, 0) : 0u;
     for (int c = 0; c < S; c += 64) {
@@ -422,7 +422,7 @@ __device__ __forceinline__ void walk_list(StepF step, FixF fix, int S, uint32_t 
         uint64_t nmask = __ballot(is_new);
         if (is_new) {
             int64_t pos = n + __popcll(nmask & lt_mask);
-            if (pos < hcap) voxels[hoff + pos] = (uint32_t)vox;
+            if (pos < hcap && voxels) voxels[hoff + pos] = (uint32_t)vox;
             hsh += pct_list_mix((uint32_t)vox, (uint32_t)pos);
         }
         n += __popcll(nmask);
@@ -430,6 +430,13 @@ __device__ __forceinline__ void walk_list(StepF step, FixF fix, int S, uint32_t 
     }
     n_out = n;
     hsh_out = hsh;
+}
+
+/* the list length (capped at the capacity); verification passes against a
+ * DROP plan's hashes store no lists and no lengths */
+__device__ __forceinline__ void store_length(uint32_t* __restrict__ lengths, int64_t at, int64_t n, int64_t hcap,
+                                             int lane) {
+    if (lengths && lane == 0) lengths[at] = (uint32_t)(n < hcap ? n : hcap);
 }
 
 /* capacity pass: steps[k] = S + 1 (upper bound of the path length) */
@@ -650,7 +657,7 @@ k_mlp_paths(pct_geo g, pct_hist_soa h, int64_t start, int64_t count, W walk,
                   hcap, voxels, lane, lt_mask, n, hsh);
         if (n > hcap) local_flags |= PCT_STATUS_CAPACITY;
         if (n > g.max_intersections) local_flags |= PCT_STATUS_OVERFLOW;
-        if (lane == 0) lengths[base + j] = (uint32_t)(n < hcap ? n : hcap);
+        store_length(lengths, base + j, n, hcap, lane);
         if (hash_out || hash_expect) {
 #pragma unroll
             for (int d = 32; d >= 1; d >>= 1) hsh += (uint32_t)__shfl_xor((int)hsh, d, 64);
@@ -880,7 +887,7 @@ k_mlp_paths_sur(pct_geo g, pct_hist_soa h, int64_t start, int64_t count, const d
             }
             if (n > hcap) local_flags |= PCT_STATUS_CAPACITY;
             if (n > g.max_intersections) local_flags |= PCT_STATUS_OVERFLOW;
-            if (lane == 0) lengths[base + j] = (uint32_t)(n < hcap ? n : hcap);
+            store_length(lengths, base + j, n, hcap, lane);
             if (hash_out || hash_expect) {
 #pragma unroll
                 for (int d = 32; d >= 1; d >>= 1) hsh += (uint32_t)__shfl_xor((int)hsh, d, 64);

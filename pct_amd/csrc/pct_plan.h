@@ -9,6 +9,9 @@
 
 struct pct_plan {
     bool valid = false;
+    /* bumped by every build: graphs captured elsewhere (the batched form)
+     * that hold this plan's pointers and offsets compare it */
+    uint64_t generation = 0;
     pct_hist_soa key{};            /* the histories the plan was built from */
     int64_t n = 0;                 /* histories                             */
     int B = 0;                     /* DROP block size                       */
@@ -17,9 +20,11 @@ struct pct_plan {
     int64_t S = 0;                 /* segments = sum over blocks of |V_B|   */
     int mlp_mode = 0;              /* MLP mode the lists were built with    */
     int net_version = 0;           /* network + calibration of a surrogate plan */
-    /* path lists, block-strided: history k of block b at
-     * list[list_base[b] + k * stride[b]], unused slots hold the sentinel */
-    pct_buffer list;               /* u32                                   */
+    /* path entries, block-strided: history k of block b at
+     * list[list_base[b] + k * stride[b]]; each the voxel, or the previous
+     * block's segment record of it when that block touches the voxel; unused
+     * slots hold the sentinel */
+    pct_buffer list;               /* u64                                   */
     pct_buffer hist_off;           /* int64 per history+1: list offsets     */
     pct_buffer len;                /* u32 per history: list length          */
     pct_buffer chord;              /* f64 per history: effective chord      */
@@ -32,7 +37,9 @@ struct pct_plan {
     pct_buffer csc;                /* u16 per entry                         */
     pct_buffer blk_seg;            /* int64 per block+1: first segment      */
     pct_buffer blk_entry;          /* int64 per block+1: first csc entry    */
-    pct_buffer delta;              /* f64 per history of one block          */
+    pct_buffer delta;              /* 2 f64 per history of two blocks       */
+    pct_buffer xw;                 /* u64 per voxel: {x, stamp} words of the
+                                      one-launch-per-block form             */
     std::vector<int64_t> blk_entry_h, blk_seg_h, list_base_h;
     std::vector<int32_t> stride_h;
     /* the block loop in chunks of blocks [chunk_b[c], chunk_b[c+1]), one
@@ -48,6 +55,7 @@ struct pct_plan {
     double graph_lambda = 0.0;
     bool graph_f32 = false;
     int graph_variant = 0;
+    int graph_form = 0;
     double graph_eta = 0.0;
     int graph_psi = 0;
 };

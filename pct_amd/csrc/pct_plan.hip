@@ -102,22 +102,51 @@ __global__ void k_hist_offsets(const int64_t* __restrict__ list_base, const int3
 
 /* grid-stride: a dispatch holds at most 2^32 - 1 work-items, and the list
  * store of a large scan has more entries than that */
-__global__ void k_fill_u32(uint32_t* __restrict__ p, int64_t n, uint32_t v) {
+__global__ void k_fill_u64(uint64_t* __restrict__ p, int64_t n, uint64_t v) {
     for (int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; i < n; i += (int64_t)gridDim.x * blockDim.x)
         p[i] = v;
 }
 
-/* one wavefront per history: copy its list from the capacity-strided build
- * store into the block-strided list store */
+/* voxel -> segment maps of blocks [b0, b1) (one map per block, slot b % M):
+ * map[slot][v] = (b << 32) | (segment index - blk_seg[b]); grid.y = block */
 __global__ void __launch_bounds__(256)
-k_copy_lists(const uint32_t* __restrict__ src, const int64_t* __restrict__ cap_off, const uint32_t* __restrict__ len,
-             const int64_t* __restrict__ hist_off, int64_t n, uint32_t* __restrict__ dst) {
+k_seg_map(const uint64_t* __restrict__ segs, const int64_t* __restrict__ blk_seg, int64_t b0, int64_t M, int64_t V,
+          uint64_t* __restrict__ map) {
+    const int64_t b = b0 + blockIdx.y;
+    const int64_t s0 = blk_seg[b], ns = blk_seg[b + 1] - s0;
+    uint64_t* m = map + (b % M) * V;
+    for (int64_t k = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; k < ns; k += (int64_t)gridDim.x * blockDim.x) {
+        const uint32_t v = (uint32_t)(segs[s0 + k] & (SEG_MAX_V - 1));
+        m[v] = ((uint64_t)b << 32) | (uint64_t)k;
+    }
+}
+
+/* one wavefront per history of [h0, h1): its path entries in the
+ * block-strided list store -- the voxel, or the previous block's segment
+ * record of the voxel when that block touches it (looked up in the previous
+ * block's map) */
+__global__ void __launch_bounds__(256)
+k_build_entries(const uint32_t* __restrict__ path, const int64_t* __restrict__ cap_off,
+                const uint32_t* __restrict__ len, const int64_t* __restrict__ hist_off, int64_t h0, int64_t h1,
+                int64_t B, const uint64_t* __restrict__ segs, const int64_t* __restrict__ blk_seg, int64_t M, int64_t V,
+                const uint64_t* __restrict__ map, uint64_t* __restrict__ ent) {
+    const int64_t h = h0 + (int64_t)blockIdx.x * 4 + (threadIdx.x >> 6);
     const int lane = threadIdx.x & 63;
-    for (int64_t h = (int64_t)blockIdx.x * 4 + (threadIdx.x >> 6); h < n; h += (int64_t)gridDim.x * 4) {
-        int m = (int)len[h];
-        const uint32_t* s = src + cap_off[h];
-        uint32_t* d = dst + hist_off[h];
-        for (int j = lane; j < m; j += 64) d[j] = s[j];
+    if (h >= h1) return;
+    const int64_t b = h / B;
+    const int n = (int)len[h];
+    const uint32_t* src = path + cap_off[h];
+    uint64_t* dst = ent + hist_off[h];
+    const uint64_t* m = b > 0 ? map + ((b - 1) % M) * V : nullptr;
+    const int64_t sp = b > 0 ? blk_seg[b - 1] : 0;
+    for (int j = lane; j < n; j += 64) {
+        const uint32_t v = src[j];
+        uint64_t e = v;
+        if (b > 0) {
+            const uint64_t w = m[v];
+            if ((w >> 32) == (uint64_t)(b - 1)) e = segs[sp + (int64_t)(uint32_t)w];
+        }
+        dst[j] = e;
     }
 }
 
@@ -184,7 +213,7 @@ __global__ void k_write_segments(const uint32_t* __restrict__ keys, const uint32
  * wave priority: they are latency-bound and share the CUs with the
  * long-running MLP waves of the next chunks. */
 __global__ void __launch_bounds__(64 * DELTA_WAVES)
-k_block_delta(const uint32_t* __restrict__ list, int stride, const uint32_t* __restrict__ len,
+k_block_delta(const uint64_t* __restrict__ list, int stride, const uint32_t* __restrict__ len,
               const double* __restrict__ chord, const float* __restrict__ wepl, int64_t h0, int64_t count,
               const float* __restrict__ x, double lambda, double* __restrict__ delta) {
     __builtin_amdgcn_s_setprio(3);
@@ -194,7 +223,7 @@ k_block_delta(const uint32_t* __restrict__ list, int stride, const uint32_t* __r
 }
 
 __global__ void __launch_bounds__(64 * DELTA_WAVES)
-k_block_delta_f32(const uint32_t* __restrict__ list, int stride, const uint32_t* __restrict__ len,
+k_block_delta_f32(const uint64_t* __restrict__ list, int stride, const uint32_t* __restrict__ len,
                   const double* __restrict__ chord, const float* __restrict__ wepl, int64_t h0, int64_t count,
                   const float* __restrict__ x, float lambda, float* __restrict__ delta) {
     __builtin_amdgcn_s_setprio(3);
@@ -224,7 +253,7 @@ k_block_apply_f32(const uint64_t* __restrict__ segs, int64_t s0, int64_t ns, con
 /* the research variants of the FP64 update (pct_plan_kernels.h) */
 template <int V>
 __global__ void __launch_bounds__(64 * DELTA_WAVES)
-k_block_delta_var(const uint32_t* __restrict__ list, int stride, const uint32_t* __restrict__ len,
+k_block_delta_var(const uint64_t* __restrict__ list, int stride, const uint32_t* __restrict__ len,
                   const double* __restrict__ chord, const float* __restrict__ wepl, int64_t h0, int64_t count,
                   const float* __restrict__ x, double lambda, double* __restrict__ out, int64_t B) {
     __builtin_amdgcn_s_setprio(3);
@@ -242,6 +271,41 @@ k_block_apply_var(const uint64_t* __restrict__ segs, int64_t s0, int64_t ns, con
     int64_t k = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
     if (k >= ns) return;
     pct_blk::apply_variant<V>(segs[s0 + k], csc, out, B, chord_b, lambda, eta, psi_sign, x);
+}
+
+/* one launch per block b: the voxel updates of block b-1 (workgroups [0, na),
+ * one thread per segment) and the deltas of block b (the rest, one wavefront
+ * per history); bodies in pct_plan_kernels.h */
+#define FUSED_THREADS 256
+#define FUSED_DELTA_WAVES (FUSED_THREADS / 64)
+template <bool F32>
+__global__ void __launch_bounds__(FUSED_THREADS)
+k_block_fused(const uint64_t* __restrict__ list, int stride, const uint32_t* __restrict__ len,
+              const double* __restrict__ chord, const float* __restrict__ wepl, int64_t h0, int64_t count,
+              double lambda, void* __restrict__ delta_out, const uint64_t* __restrict__ segs_p, int64_t ns_p,
+              const uint16_t* __restrict__ csc_p, const void* __restrict__ delta_p, uint32_t stamp_p, unsigned na,
+              uint64_t* __restrict__ xw) {
+    __builtin_amdgcn_s_setprio(3);
+    const pct_blk::fused_prev<F32> pv{csc_p, delta_p, stamp_p};
+    if (blockIdx.x < na) {
+        const int64_t k = (int64_t)blockIdx.x * FUSED_THREADS + threadIdx.x;
+        if (k < ns_p) pct_blk::fused_apply<F32>(segs_p[k], xw, pv);
+        return;
+    }
+    const int64_t k = (int64_t)(blockIdx.x - na) * FUSED_DELTA_WAVES + (threadIdx.x >> 6);
+    if (k >= count) return;
+    pct_blk::fused_delta<F32>(list, stride, len, chord, wepl, h0, k, xw, pv, lambda, delta_out, threadIdx.x & 63);
+}
+
+/* the image as {value, stamp} words (stamp: no block yet) and back */
+__global__ void k_xw_pack(const float* __restrict__ x, int64_t V, uint64_t* __restrict__ xw) {
+    const int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (i < V) xw[i] = pct_blk::xw_pack(x[i], 0xFFFFFFFFu);
+}
+
+__global__ void k_xw_unpack(const uint64_t* __restrict__ xw, int64_t V, float* __restrict__ x) {
+    const int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (i < V) x[i] = pct_blk::xw_value(xw[i]);
 }
 
 }  // namespace
@@ -271,7 +335,7 @@ void pct_plan_destroy(pct_plan* pl) {
     if (!pl) return;
     plan_release_graph(pl);
     pct_buffer* bufs[] = {&pl->list, &pl->hist_off, &pl->len, &pl->chord, &pl->hash, &pl->segs,
-                          &pl->csc, &pl->blk_seg, &pl->blk_entry, &pl->delta};
+                          &pl->csc, &pl->blk_seg, &pl->blk_entry, &pl->delta, &pl->xw};
     for (pct_buffer* b : bufs) release(*b);
     for (hipEvent_t e : pl->ev_chunk)
         if (e) (void)hipEventDestroy(e);
@@ -299,6 +363,7 @@ static int plan_build(pct_ctx* ctx, const pct_hist_soa* hist, int B, int mlp_mod
     pct_plan* pl = ctx->plan;
     plan_release_graph(pl);
     pl->valid = false;
+    ++pl->generation;
     int rc;
     const int64_t nb = (n + B - 1) / B;
     /* 1. the MLP lists into a capacity-strided build store */
@@ -317,7 +382,8 @@ static int plan_build(pct_ctx* ctx, const pct_hist_soa* hist, int B, int mlp_mod
     int flags = 0;
     if ((rc = pct_take_status(ctx, &flags))) return rc;
     if ((rc = pct_status_to_rc(ctx, flags))) return rc;
-    /* 2. block strides = longest list per block; block-strided list store */
+    /* 2. block strides = longest list per block; slot offsets of the
+     * block-strided list store */
     if ((rc = pct_ensure(ctx, ctx->steps, sizeof(int32_t) * (n > nb ? n : nb)))) return rc;
     int32_t* bmax = (int32_t*)ctx->steps.ptr;
     hipLaunchKernelGGL(k_block_max_len, dim3((unsigned)nb), dim3(256), 0, ctx->stream, (const uint32_t*)len, n,
@@ -331,32 +397,29 @@ static int plan_build(pct_ctx* ctx, const pct_hist_soa* hist, int B, int mlp_mod
         pl->list_base_h[b + 1] = pl->list_base_h[b] + cnt * pl->stride_h[b];
     }
     const int64_t list_size = pl->list_base_h[nb];
-    if ((rc = pct_ensure(ctx, pl->list, sizeof(uint32_t) * (list_size > 0 ? list_size : 1)))) return rc;
     if ((rc = pct_ensure(ctx, pl->hist_off, sizeof(int64_t) * (n + 1)))) return rc;
-    pct_buffer tmp_base, tmp_stride;
-    auto free_tmp = [&]() { release(tmp_base); release(tmp_stride); };
-    if ((rc = pct_ensure(ctx, tmp_base, sizeof(int64_t) * (nb + 1))) || (rc = pct_ensure(ctx, tmp_stride, sizeof(int32_t) * nb))) {
+    {
+        pct_buffer tmp_base, tmp_stride;
+        auto free_tmp = [&]() { release(tmp_base); release(tmp_stride); };
+        if ((rc = pct_ensure(ctx, tmp_base, sizeof(int64_t) * (nb + 1))) ||
+            (rc = pct_ensure(ctx, tmp_stride, sizeof(int32_t) * nb))) {
+            free_tmp();
+            return rc;
+        }
+        PCT_HIP_CHECK(ctx, hipMemcpyAsync(tmp_base.ptr, pl->list_base_h.data(), sizeof(int64_t) * (nb + 1),
+                                          hipMemcpyHostToDevice, ctx->stream));
+        PCT_HIP_CHECK(ctx, hipMemcpyAsync(tmp_stride.ptr, pl->stride_h.data(), sizeof(int32_t) * nb,
+                                          hipMemcpyHostToDevice, ctx->stream));
+        hipLaunchKernelGGL(k_hist_offsets, dim3(pct_grid(n + 1, 256)), dim3(256), 0, ctx->stream,
+                           (const int64_t*)tmp_base.ptr, (const int32_t*)tmp_stride.ptr, n, (int64_t)B, nb,
+                           (int64_t*)pl->hist_off.ptr);
+        PCT_HIP_CHECK(ctx, hipGetLastError());
+        PCT_HIP_CHECK(ctx, hipStreamSynchronize(ctx->stream));
         free_tmp();
-        return rc;
     }
-    PCT_HIP_CHECK(ctx, hipMemcpyAsync(tmp_base.ptr, pl->list_base_h.data(), sizeof(int64_t) * (nb + 1),
-                                      hipMemcpyHostToDevice, ctx->stream));
-    PCT_HIP_CHECK(ctx, hipMemcpyAsync(tmp_stride.ptr, pl->stride_h.data(), sizeof(int32_t) * nb, hipMemcpyHostToDevice,
-                                      ctx->stream));
-    hipLaunchKernelGGL(k_hist_offsets, dim3(pct_grid(n + 1, 256)), dim3(256), 0, ctx->stream,
-                       (const int64_t*)tmp_base.ptr, (const int32_t*)tmp_stride.ptr, n, (int64_t)B, nb,
-                       (int64_t*)pl->hist_off.ptr);
-    hipLaunchKernelGGL(k_fill_u32, dim3(std::min(pct_grid(list_size, 256), PCT_GRID_CAP)), dim3(256), 0, ctx->stream,
-                       (uint32_t*)pl->list.ptr, list_size, LIST_SENTINEL);
-    hipLaunchKernelGGL(k_copy_lists, dim3(std::min(pct_grid(n, 4), PCT_GRID_CAP)), dim3(256), 0, ctx->stream,
-                       (const uint32_t*)ctx->path_store.ptr, cap_off, (const uint32_t*)len,
-                       (const int64_t*)pl->hist_off.ptr, n, (uint32_t*)pl->list.ptr);
-    PCT_HIP_CHECK(ctx, hipGetLastError());
-    PCT_HIP_CHECK(ctx, hipStreamSynchronize(ctx->stream));
-    free_tmp();
-    release(ctx->path_store);          /* the build store is not needed any more */
     const int64_t* hist_off = (const int64_t*)pl->hist_off.ptr;
-    const uint32_t* list = (const uint32_t*)pl->list.ptr;
+    /* the build store holds every list at its capacity offset */
+    const uint32_t* list = (const uint32_t*)ctx->path_store.ptr;
     /* 3. exact entry offsets (prefix sum of the lengths) */
     pct_buffer ex_off;
     if ((rc = pct_ensure(ctx, ex_off, sizeof(int64_t) * (n + 1)))) return rc;
@@ -427,7 +490,7 @@ static int plan_build(pct_ctx* ctx, const pct_hist_soa* hist, int B, int mlp_mod
             int endbit = vbits + vbits_for(b1 - b0 + 1);
             if (endbit > 32) endbit = 32;
             hipLaunchKernelGGL(k_emit_keys, dim3((unsigned)((h1 - h0 + 3) / 4)), dim3(256), 0, ctx->stream, list,
-                               hist_off, (const uint32_t*)len, (const int64_t*)ex_off.ptr, h0, h1, (int64_t)B, b0, vbits,
+                               cap_off, (const uint32_t*)len, (const int64_t*)ex_off.ptr, h0, h1, (int64_t)B, b0, vbits,
                                entry0, (uint32_t*)kin.ptr, (uint32_t*)vin.ptr);
             size_t tb = tmp.bytes;
             PCT_HIP_CHECK(ctx, rocprim::radix_sort_pairs(tmp.ptr, tb, (uint32_t*)kin.ptr, (uint32_t*)kout.ptr,
@@ -470,7 +533,48 @@ static int plan_build(pct_ctx* ctx, const pct_hist_soa* hist, int B, int mlp_mod
     PCT_HIP_CHECK(ctx, hipMemcpyAsync(pl->blk_seg_h.data(), pl->blk_seg.ptr, sizeof(int64_t) * (nb + 1),
                                       hipMemcpyDeviceToHost, ctx->stream));
     PCT_HIP_CHECK(ctx, hipStreamSynchronize(ctx->stream));
-    if ((rc = pct_ensure(ctx, pl->delta, sizeof(double) * 2 * (size_t)B))) return rc;   /* 2 per history: ROBUST */
+    /* 5. the list store: one 8-byte entry per slot, the voxel or the previous
+     * block's segment record of it, unused slots the sentinel. Maps from voxel
+     * to segment for M blocks at a time (tagged with the block, so never
+     * cleared); the build store is freed afterwards. */
+    if ((rc = pct_ensure(ctx, pl->list, sizeof(uint64_t) * (size_t)(list_size > 0 ? list_size : 1)))) return rc;
+    hipLaunchKernelGGL(k_fill_u64, dim3(std::min(pct_grid(list_size, 256), PCT_GRID_CAP)), dim3(256), 0, ctx->stream,
+                       (uint64_t*)pl->list.ptr, list_size, ENT_SENTINEL);
+    {
+        const int64_t V = ctx->g.V;
+        int64_t M = std::max<int64_t>(2, std::min<int64_t>(256, ((int64_t)1 << 31) / (8 * V)));
+        M = std::min<int64_t>(M, nb + 1);
+        pct_buffer map;
+        if ((rc = pct_ensure(ctx, map, sizeof(uint64_t) * (size_t)(M * V)))) return rc;
+        hipLaunchKernelGGL(k_fill_u64, dim3(std::min(pct_grid(M * V, 256), PCT_GRID_CAP)), dim3(256), 0, ctx->stream,
+                           (uint64_t*)map.ptr, M * V, ~0ull);
+        /* entries of blocks [c0, c1) read the maps of blocks [c0 - 1, c1 - 1) */
+        const int64_t step = M - 1;
+        for (int64_t c0 = 0; c0 < nb; c0 += step) {
+            const int64_t c1 = std::min(nb, c0 + step);
+            const int64_t m0 = c0 > 0 ? c0 - 1 : 0, m1 = c1 - 1;
+            if (m1 > m0) {
+                int64_t maxs = 0;
+                for (int64_t b = m0; b < m1; ++b) maxs = std::max(maxs, pl->blk_seg_h[b + 1] - pl->blk_seg_h[b]);
+                if (maxs > 0)
+                    hipLaunchKernelGGL(k_seg_map, dim3(std::min<unsigned>(pct_grid(maxs, 256), 1024u), (unsigned)(m1 - m0)),
+                                       dim3(256), 0, ctx->stream, (const uint64_t*)pl->segs.ptr,
+                                       (const int64_t*)pl->blk_seg.ptr, m0, M, V, (uint64_t*)map.ptr);
+            }
+            const int64_t h0 = c0 * B, h1 = std::min(n, c1 * B);
+            hipLaunchKernelGGL(k_build_entries, dim3((unsigned)((h1 - h0 + 3) / 4)), dim3(256), 0, ctx->stream,
+                               (const uint32_t*)ctx->path_store.ptr, cap_off, (const uint32_t*)len, hist_off, h0, h1,
+                               (int64_t)B, (const uint64_t*)pl->segs.ptr, (const int64_t*)pl->blk_seg.ptr, M, V,
+                               (const uint64_t*)map.ptr, (uint64_t*)pl->list.ptr);
+        }
+        PCT_HIP_CHECK(ctx, hipGetLastError());
+        PCT_HIP_CHECK(ctx, hipStreamSynchronize(ctx->stream));
+        release(map);
+    }
+    release(ctx->path_store);
+    /* deltas of two blocks (the one-launch form reads block b-1's while writing
+     * block b's), two doubles per history (ROBUST) */
+    if ((rc = pct_ensure(ctx, pl->delta, sizeof(double) * 4 * (size_t)B))) return rc;
     pl->n = n;
     pl->B = B;
     pl->nb = nb;
@@ -512,18 +616,60 @@ static int update_variant(int mode) {
     return (mode & PCT_DROP_ROBUST) ? 1 : (mode & PCT_DROP_ROBUST2) ? 2 : 0;
 }
 
+/* block-loop form: 1 = one launch per block on {value, stamp} image words,
+ * 2 = two launches per block on the float image (default: measured 13.7
+ * against 14.0 us per block at 200x200x32; the research variants always) */
+static int block_form(const pct_ctx* ctx, int variant) {
+    return (variant != 0 || ctx->tune_block_form != 1) ? 2 : 1;
+}
+
 static int plan_block_loop(pct_ctx* ctx, pct_plan* pl, const pct_hist_soa* hist, float* x, double lambda, bool f32,
                            int variant, int64_t b_begin, int64_t b_end) {
     const int64_t B = pl->B;
-    const uint32_t* list = (const uint32_t*)pl->list.ptr;
+    const uint64_t* list = (const uint64_t*)pl->list.ptr;
     const uint32_t* len = (const uint32_t*)pl->len.ptr;
     const double* chord = (const double*)pl->chord.ptr;
     const uint64_t* segs = (const uint64_t*)pl->segs.ptr;
     const uint16_t* csc = (const uint16_t*)pl->csc.ptr;
+    if (block_form(ctx, variant) == 1) {
+        /* launch b: deltas of block b and the voxel updates of block b-1; the
+         * last chunk adds launch nb (updates of the last block only) */
+        uint64_t* xw = (uint64_t*)pl->xw.ptr;
+        const int64_t b_last = b_end == pl->nb ? pl->nb : b_end - 1;
+        for (int64_t b = b_begin; b <= b_last; ++b) {
+            const int64_t h0 = b * B, cnt = b < pl->nb ? std::min<int64_t>(B, pl->n - h0) : 0;
+            const uint64_t* lb = b < pl->nb ? list + pl->list_base_h[b] : nullptr;
+            const int stride = b < pl->nb ? pl->stride_h[b] : 0;
+            void* dout = (char*)pl->delta.ptr + (size_t)(b & 1) * 2 * B * sizeof(double);
+            const void* dprev = (const char*)pl->delta.ptr + (size_t)((b + 1) & 1) * 2 * B * sizeof(double);
+            const uint64_t* sp = nullptr;
+            const uint16_t* cp = nullptr;
+            int64_t nsp = 0;
+            if (b > 0) {
+                sp = segs + pl->blk_seg_h[b - 1];
+                nsp = pl->blk_seg_h[b] - pl->blk_seg_h[b - 1];
+                cp = csc + pl->blk_entry_h[b - 1];
+            }
+            const unsigned na = pct_grid(nsp, FUSED_THREADS) * (nsp > 0);
+            const unsigned nd = (unsigned)((cnt + FUSED_DELTA_WAVES - 1) / FUSED_DELTA_WAVES);
+            if (na + nd == 0) continue;
+            const uint32_t stamp = (uint32_t)(b - 1);
+            if (f32)
+                hipLaunchKernelGGL(k_block_fused<true>, dim3(na + nd), dim3(FUSED_THREADS), 0, ctx->stream, lb, stride,
+                                   len, chord, (const float*)hist->wepl, h0, cnt, lambda, dout, sp, nsp, cp, dprev,
+                                   stamp, na, xw);
+            else
+                hipLaunchKernelGGL(k_block_fused<false>, dim3(na + nd), dim3(FUSED_THREADS), 0, ctx->stream, lb, stride,
+                                   len, chord, (const float*)hist->wepl, h0, cnt, lambda, dout, sp, nsp, cp, dprev,
+                                   stamp, na, xw);
+        }
+        PCT_HIP_CHECK(ctx, hipGetLastError());
+        return PCT_OK;
+    }
     void* delta = pl->delta.ptr;
     for (int64_t b = b_begin; b < b_end; ++b) {
         int64_t h0 = b * B, cnt = (pl->n - h0) < B ? (pl->n - h0) : B;
-        const uint32_t* lb = list + pl->list_base_h[b];
+        const uint64_t* lb = list + pl->list_base_h[b];
         int stride = pl->stride_h[b];
         dim3 dgrid((unsigned)((cnt + DELTA_WAVES - 1) / DELTA_WAVES));
         if (variant != 0) {
@@ -584,6 +730,7 @@ static int plan_ensure_graphs(pct_ctx* ctx, pct_plan* pl, const pct_hist_soa* hi
     const int64_t nchunks = (int64_t)pl->chunk_b.size() - 1;
     if ((int64_t)pl->graphs.size() == nchunks && pl->graph_x == x && pl->graph_lambda == lambda &&
         pl->graph_f32 == f32 && pl->graph_wepl == hist->wepl && pl->graph_variant == variant &&
+        pl->graph_form == block_form(ctx, variant) &&
         (variant == 0 || (pl->graph_eta == ctx->p.eta && pl->graph_psi == ctx->p.psi_sign)))
         return PCT_OK;
     plan_release_graph(pl);
@@ -606,6 +753,7 @@ static int plan_ensure_graphs(pct_ctx* ctx, pct_plan* pl, const pct_hist_soa* hi
     pl->graph_f32 = f32;
     pl->graph_wepl = hist->wepl;
     pl->graph_variant = variant;
+    pl->graph_form = block_form(ctx, variant);
     pl->graph_eta = ctx->p.eta;
     pl->graph_psi = ctx->p.psi_sign;
     return PCT_OK;
@@ -630,16 +778,20 @@ static int plan_ensure_events(pct_ctx* ctx, pct_plan* pl, int64_t nchunks) {
 static int plan_iterate(pct_ctx* ctx, pct_plan* pl, const pct_hist_soa* hist, float* x, double lambda, bool f32,
                         int variant, int mlp, bool recompute, bool eager) {
     const bool sequential = recompute && ctx->tune_overlap == 2;
+    const bool words = block_form(ctx, variant) == 1;
     int rc;
     plan_chunks(ctx, pl);
     const int64_t nchunks = (int64_t)pl->chunk_b.size() - 1;
+    if (words && (rc = pct_ensure(ctx, pl->xw, sizeof(uint64_t) * (size_t)ctx->g.V))) return rc;
     if (!eager && (rc = plan_ensure_graphs(ctx, pl, hist, x, lambda, f32, variant))) return rc;
+    if (words)
+        hipLaunchKernelGGL(k_xw_pack, dim3(pct_grid(ctx->g.V, 256)), dim3(256), 0, ctx->stream, (const float*)x,
+                           ctx->g.V, (uint64_t*)pl->xw.ptr);
     if (ctx->profile) PCT_HIP_CHECK(ctx, hipEventRecord(ctx->ev[0], ctx->stream));
     if (recompute && ctx->tune_overlap == 2) {
         /* sequential: all lists first, then the block loop */
-        rc = pct_mlp_paths_on(ctx, ctx->stream, hist, 0, pl->n, (const int64_t*)pl->hist_off.ptr,
-                              (uint32_t*)pl->list.ptr, (uint32_t*)pl->len.ptr, (double*)pl->chord.ptr, nullptr,
-                              (const uint32_t*)pl->hash.ptr, mlp, false);
+        rc = pct_mlp_paths_on(ctx, ctx->stream, hist, 0, pl->n, (const int64_t*)pl->hist_off.ptr, nullptr, nullptr,
+                              nullptr, nullptr, (const uint32_t*)pl->hash.ptr, mlp, false);
         if (rc) return rc;
         recompute = false;
         if (ctx->profile) {
@@ -654,9 +806,8 @@ static int plan_iterate(pct_ctx* ctx, pct_plan* pl, const pct_hist_soa* hist, fl
         for (int64_t c = 0; c < nchunks; ++c) {
             int64_t h0 = pl->chunk_b[c] * pl->B;
             int64_t h1 = std::min(pl->n, pl->chunk_b[c + 1] * pl->B);
-            rc = pct_mlp_paths_on(ctx, aux, hist, h0, h1 - h0, (const int64_t*)pl->hist_off.ptr + h0,
-                                  (uint32_t*)pl->list.ptr, (uint32_t*)pl->len.ptr + h0, (double*)pl->chord.ptr + h0,
-                                  nullptr, (const uint32_t*)pl->hash.ptr + h0, mlp, true);
+            rc = pct_mlp_paths_on(ctx, aux, hist, h0, h1 - h0, (const int64_t*)pl->hist_off.ptr + h0, nullptr, nullptr,
+                                  nullptr, nullptr, (const uint32_t*)pl->hash.ptr + h0, mlp, true);
             if (rc) return rc;
             PCT_HIP_CHECK(ctx, hipEventRecord(pl->ev_chunk[c], aux));
             if (ctx->profile && c == 0) PCT_HIP_CHECK(ctx, hipEventRecord(ctx->ev[1], aux));
@@ -675,6 +826,9 @@ static int plan_iterate(pct_ctx* ctx, pct_plan* pl, const pct_hist_soa* hist, fl
             PCT_HIP_CHECK(ctx, hipGraphLaunch(pl->graphs[c], ctx->stream));
         }
     }
+    if (words)
+        hipLaunchKernelGGL(k_xw_unpack, dim3(pct_grid(ctx->g.V, 256)), dim3(256), 0, ctx->stream,
+                           (const uint64_t*)pl->xw.ptr, ctx->g.V, x);
     if (ctx->profile) {
         PCT_HIP_CHECK(ctx, hipEventRecord(ctx->ev[3], ctx->stream));
         ctx->ev_recorded = true;
@@ -719,15 +873,40 @@ extern "C" int pct_drop_iteration(pct_ctx* ctx, const pct_hist_soa* hist, float*
      * default mode, into the plan's list store (a list's capacity is its
      * block's stride), checking every list against the plan's hash */
     const bool recompute = !(mode & PCT_DROP_CACHED) && !fresh;
+    if (recompute) {
+        /* a recomputed list that differs from the plan is only known once the
+         * block loop has run on it: keep x to restore it then */
+        if ((rc = pct_ensure(ctx, ctx->x_saved, sizeof(float) * (size_t)ctx->g.V))) return rc;
+        PCT_HIP_CHECK(ctx, hipMemcpyAsync(ctx->x_saved.ptr, x, sizeof(float) * (size_t)ctx->g.V,
+                                          hipMemcpyDeviceToDevice, ctx->stream));
+    }
     rc = plan_iterate(ctx, pl, hist, x, (double)lambda, (mode & PCT_DROP_F32) != 0, variant, mlp, recompute,
                       (mode & PCT_DROP_NO_GRAPH) != 0);
     if (rc) return rc;
     int flags = 0;
     if ((rc = pct_take_status(ctx, &flags))) return rc;
     /* recomputed lists that differ from the plan (or overflow a stride) have
-     * overwritten the list store: the plan is rebuilt on the next call */
-    if (flags) pl->valid = false;
+     * overwritten the list store: x goes back to its value before the call,
+     * and the plan is rebuilt on the next call */
+    if (flags) {
+        pl->valid = false;
+        if (recompute && (flags & (PCT_STATUS_PLAN | PCT_STATUS_CAPACITY | PCT_STATUS_TABLE | PCT_STATUS_OVERFLOW))) {
+            PCT_HIP_CHECK(ctx, hipMemcpyAsync(x, ctx->x_saved.ptr, sizeof(float) * (size_t)ctx->g.V,
+                                              hipMemcpyDeviceToDevice, ctx->stream));
+            PCT_HIP_CHECK(ctx, hipStreamSynchronize(ctx->stream));
+        }
+    }
     return pct_status_to_rc(ctx, flags);
+}
+
+extern "C" int pct_drop_invalidate(pct_ctx* ctx) {
+    if (!ctx) return PCT_E_INVALID;
+    if (ctx->plan) {
+        plan_release_graph(ctx->plan);
+        ctx->plan->valid = false;
+        ++ctx->plan->generation;
+    }
+    return PCT_OK;
 }
 
 extern "C" int pct_drop_stats(pct_ctx* ctx, const pct_hist_soa* hist, int block_size, int64_t* sum_path,
@@ -751,6 +930,10 @@ extern "C" int pct_set_tuning(pct_ctx* ctx, int key, int value) {
     case PCT_TUNE_MLP_WAVES: ctx->tune_mlp_waves = value; break;
     case PCT_TUNE_CHUNK_FIRST: ctx->tune_chunk_first = value; break;
     case PCT_TUNE_CHUNK_MAX: ctx->tune_chunk_max = value; break;
+    case PCT_TUNE_BLOCK_FORM:
+        if (value > 2) return pct_fail(ctx, PCT_E_INVALID, "pct_set_tuning: block form %d not in {1, 2}", value);
+        ctx->tune_block_form = value;
+        break;
     case PCT_TUNE_OVERLAP:
         if (value > 2) return pct_fail(ctx, PCT_E_INVALID, "pct_set_tuning: overlap %d not in {1, 2}", value);
         ctx->tune_overlap = value;

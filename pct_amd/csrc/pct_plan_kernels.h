@@ -17,6 +17,10 @@
 #include <hip/hip_runtime.h>
 
 #define LIST_SENTINEL 0xFFFFFFFFu
+/* the plan's list store holds one 8-byte entry per path slot: the voxel in
+ * bits 0-21, and for a voxel the previous DROP block also touches, that
+ * block's segment record of the voxel (the layout below); all ones = unused */
+#define ENT_SENTINEL 0xFFFFFFFFFFFFFFFFull
 /* 8-byte segment record: bits 0-21 voxel, 22-23 inline count (1-3; 0 = the
  * history indices are in csc), then either up to three 13-bit block-local
  * history indices (bits 24-36, 37-49, 50-62) or the block-relative first csc
@@ -38,6 +42,8 @@
 #define APPLY_THREADS 256
 #endif
 #define DELTA_SLOTS 8
+/* history indices of a long segment loaded per memory round */
+#define CSC_BATCH 8
 
 namespace pct_blk {
 
@@ -56,7 +62,9 @@ __device__ __forceinline__ void kahan_add(float& s, float& c, float v) {
 
 /* history k of the block (one wavefront); list points at the block's first
  * list, history k's list is at k * stride; h0 = first history of the block */
-__device__ __forceinline__ void delta_f64(const uint32_t* __restrict__ list, int stride, const uint32_t* __restrict__ len,
+__device__ __forceinline__ uint32_t ent_voxel(uint64_t e) { return (uint32_t)e & (uint32_t)(SEG_MAX_V - 1); }
+
+__device__ __forceinline__ void delta_f64(const uint64_t* __restrict__ list, int stride, const uint32_t* __restrict__ len,
                                           const double* __restrict__ chord, const float* __restrict__ wepl, int64_t h0,
                                           int64_t k, const float* __restrict__ x, double lambda,
                                           double* __restrict__ delta, int lane) {
@@ -64,20 +72,21 @@ __device__ __forceinline__ void delta_f64(const uint32_t* __restrict__ list, int
     const uint32_t n = len[h];
     const double L = chord[h];
     const float b_i = wepl[h];
-    const uint32_t* p = list + k * stride;
+    const uint64_t* p = list + k * stride;
     uint32_t v[DELTA_SLOTS];
 #pragma unroll
     for (int r = 0; r < DELTA_SLOTS; ++r) {
         int j = lane + 64 * r;
-        v[r] = j < stride ? p[j] : LIST_SENTINEL;
+        const uint64_t e = j < stride ? p[j] : ENT_SENTINEL;
+        v[r] = e == ENT_SENTINEL ? LIST_SENTINEL : ent_voxel(e);
     }
     double s = 0.0;
 #pragma unroll
     for (int r = 0; r < DELTA_SLOTS; ++r)
         if (v[r] != LIST_SENTINEL) s += (double)x[v[r]];
     for (int j = lane + 64 * DELTA_SLOTS; j < stride; j += 64) {
-        uint32_t w = p[j];
-        if (w != LIST_SENTINEL) s += (double)x[w];
+        const uint64_t e = p[j];
+        if (e != ENT_SENTINEL) s += (double)x[ent_voxel(e)];
     }
     s = wave_sum_d(s);
     if (lane == 0) {
@@ -87,7 +96,7 @@ __device__ __forceinline__ void delta_f64(const uint32_t* __restrict__ list, int
     }
 }
 
-__device__ __forceinline__ void delta_f32(const uint32_t* __restrict__ list, int stride, const uint32_t* __restrict__ len,
+__device__ __forceinline__ void delta_f32(const uint64_t* __restrict__ list, int stride, const uint32_t* __restrict__ len,
                                           const double* __restrict__ chord, const float* __restrict__ wepl, int64_t h0,
                                           int64_t k, const float* __restrict__ x, float lambda,
                                           float* __restrict__ delta, int lane) {
@@ -95,20 +104,21 @@ __device__ __forceinline__ void delta_f32(const uint32_t* __restrict__ list, int
     const uint32_t n = len[h];
     const float L = (float)chord[h];
     const float b_i = wepl[h];
-    const uint32_t* p = list + k * stride;
+    const uint64_t* p = list + k * stride;
     uint32_t v[DELTA_SLOTS];
 #pragma unroll
     for (int r = 0; r < DELTA_SLOTS; ++r) {
         int j = lane + 64 * r;
-        v[r] = j < stride ? p[j] : LIST_SENTINEL;
+        const uint64_t e = j < stride ? p[j] : ENT_SENTINEL;
+        v[r] = e == ENT_SENTINEL ? LIST_SENTINEL : ent_voxel(e);
     }
     float s = 0.0f, c = 0.0f;
 #pragma unroll
     for (int r = 0; r < DELTA_SLOTS; ++r)
         if (v[r] != LIST_SENTINEL) kahan_add(s, c, x[v[r]]);
     for (int j = lane + 64 * DELTA_SLOTS; j < stride; j += 64) {
-        uint32_t w = p[j];
-        if (w != LIST_SENTINEL) kahan_add(s, c, x[w]);
+        const uint64_t e = p[j];
+        if (e != ENT_SENTINEL) kahan_add(s, c, x[ent_voxel(e)]);
     }
 #pragma unroll
     for (int d = 32; d >= 1; d >>= 1) {
@@ -135,7 +145,7 @@ __device__ __forceinline__ void delta_f32(const uint32_t* __restrict__ list, int
  *     N_v = sum L^2, and x_v += (N_v / (N_v + psi_v)) sum / count with
  *     psi_v = PSI_SIGN (1 - x_v) ETA, the first product in float.
  * x_v is the value at the block's start in both. */
-__device__ __forceinline__ void delta_variant(int variant, const uint32_t* __restrict__ list, int stride,
+__device__ __forceinline__ void delta_variant(int variant, const uint64_t* __restrict__ list, int stride,
                                               const uint32_t* __restrict__ len, const double* __restrict__ chord,
                                               const float* __restrict__ wepl, int64_t h0, int64_t k,
                                               const float* __restrict__ x, double lambda, double* __restrict__ out,
@@ -144,20 +154,21 @@ __device__ __forceinline__ void delta_variant(int variant, const uint32_t* __res
     const uint32_t n = len[h];
     const double L = chord[h];
     const float b_i = wepl[h];
-    const uint32_t* p = list + k * stride;
+    const uint64_t* p = list + k * stride;
     uint32_t v[DELTA_SLOTS];
 #pragma unroll
     for (int r = 0; r < DELTA_SLOTS; ++r) {
         int j = lane + 64 * r;
-        v[r] = j < stride ? p[j] : LIST_SENTINEL;
+        const uint64_t e = j < stride ? p[j] : ENT_SENTINEL;
+        v[r] = e == ENT_SENTINEL ? LIST_SENTINEL : ent_voxel(e);
     }
     double s = 0.0;
 #pragma unroll
     for (int r = 0; r < DELTA_SLOTS; ++r)
         if (v[r] != LIST_SENTINEL) s += (double)x[v[r]];
     for (int j = lane + 64 * DELTA_SLOTS; j < stride; j += 64) {
-        uint32_t w = p[j];
-        if (w != LIST_SENTINEL) s += (double)x[w];
+        const uint64_t e = p[j];
+        if (e != ENT_SENTINEL) s += (double)x[ent_voxel(e)];
     }
     s = wave_sum_d(s);
     if (lane == 0) {
@@ -185,7 +196,14 @@ __device__ __forceinline__ uint32_t for_each_history(uint64_t r, const uint16_t*
     }
     const uint32_t cnt = (uint32_t)(r >> 51);
     const uint16_t* q = csc + ((r >> 24) & 0x7ffffffu);
-    for (uint32_t i = 0; i < cnt; ++i) f((uint32_t)q[i]);
+    for (uint32_t i = 0; i < cnt; i += CSC_BATCH) {
+        uint32_t idx[CSC_BATCH];
+#pragma unroll
+        for (int t = 0; t < CSC_BATCH; ++t) idx[t] = i + t < cnt ? q[i + t] : 0u;
+#pragma unroll
+        for (int t = 0; t < CSC_BATCH; ++t)
+            if (i + t < cnt) f(idx[t]);
+    }
     return cnt;
 }
 
@@ -214,12 +232,12 @@ __device__ __forceinline__ void apply_variant(uint64_t r, const uint16_t* __rest
     }
 }
 
-/* segment record r of a block; csc points at the block's first entry */
-__device__ __forceinline__ void apply_f64(uint64_t r, const uint16_t* __restrict__ csc,
-                                          const double* __restrict__ delta, float* __restrict__ x) {
-    const uint32_t v = (uint32_t)(r & (SEG_MAX_V - 1));
+/* the new value of a voxel of a block from its value before the block and its
+ * segment record r (csc points at the block's first entry): the deltas of the
+ * block's histories through it summed in history order, x + sum / count */
+__device__ __forceinline__ float applied_f64(float xv, uint64_t r, const uint16_t* __restrict__ csc,
+                                             const double* __restrict__ delta) {
     const uint32_t c = (uint32_t)(r >> SEG_VOX_BITS) & 3u;
-    const float xv = x[v];
     double upd = 0.0;
     uint32_t cnt;
     if (c != 0u) {
@@ -233,16 +251,26 @@ __device__ __forceinline__ void apply_f64(uint64_t r, const uint16_t* __restrict
     } else {
         cnt = (uint32_t)(r >> 51);
         const uint16_t* q = csc + ((r >> 24) & 0x7ffffffu);
-        for (uint32_t i = 0; i < cnt; ++i) upd += delta[q[i]];
+        /* CSC_BATCH indices, then their deltas, are loaded together before
+         * they are added in history order: two memory rounds per batch */
+        for (uint32_t i = 0; i < cnt; i += CSC_BATCH) {
+            uint32_t idx[CSC_BATCH];
+            double d[CSC_BATCH];
+#pragma unroll
+            for (int t = 0; t < CSC_BATCH; ++t) idx[t] = i + t < cnt ? q[i + t] : 0u;
+#pragma unroll
+            for (int t = 0; t < CSC_BATCH; ++t) d[t] = i + t < cnt ? delta[idx[t]] : 0.0;
+#pragma unroll
+            for (int t = 0; t < CSC_BATCH; ++t)
+                if (i + t < cnt) upd += d[t];
+        }
     }
-    x[v] = (float)((double)xv + upd / (double)cnt);
+    return (float)((double)xv + upd / (double)cnt);
 }
 
-__device__ __forceinline__ void apply_f32(uint64_t r, const uint16_t* __restrict__ csc,
-                                          const float* __restrict__ delta, float* __restrict__ x) {
-    const uint32_t v = (uint32_t)(r & (SEG_MAX_V - 1));
+__device__ __forceinline__ float applied_f32(float xv, uint64_t r, const uint16_t* __restrict__ csc,
+                                             const float* __restrict__ delta) {
     const uint32_t c = (uint32_t)(r >> SEG_VOX_BITS) & 3u;
-    const float xv = x[v];
     float upd = 0.0f, comp = 0.0f;
     uint32_t cnt;
     if (c != 0u) {
@@ -256,9 +284,144 @@ __device__ __forceinline__ void apply_f32(uint64_t r, const uint16_t* __restrict
     } else {
         cnt = (uint32_t)(r >> 51);
         const uint16_t* q = csc + ((r >> 24) & 0x7ffffffu);
-        for (uint32_t i = 0; i < cnt; ++i) kahan_add(upd, comp, delta[q[i]]);
+        for (uint32_t i = 0; i < cnt; i += CSC_BATCH) {
+            uint32_t idx[CSC_BATCH];
+            float d[CSC_BATCH];
+#pragma unroll
+            for (int t = 0; t < CSC_BATCH; ++t) idx[t] = i + t < cnt ? q[i + t] : 0u;
+#pragma unroll
+            for (int t = 0; t < CSC_BATCH; ++t) d[t] = i + t < cnt ? delta[idx[t]] : 0.0f;
+#pragma unroll
+            for (int t = 0; t < CSC_BATCH; ++t)
+                if (i + t < cnt) kahan_add(upd, comp, d[t]);
+        }
     }
-    x[v] = xv + upd / (float)cnt;
+    return xv + upd / (float)cnt;
+}
+
+/* segment record r of a block; csc points at the block's first entry */
+__device__ __forceinline__ void apply_f64(uint64_t r, const uint16_t* __restrict__ csc,
+                                          const double* __restrict__ delta, float* __restrict__ x) {
+    const uint32_t v = (uint32_t)(r & (SEG_MAX_V - 1));
+    x[v] = applied_f64(x[v], r, csc, delta);
+}
+
+__device__ __forceinline__ void apply_f32(uint64_t r, const uint16_t* __restrict__ csc,
+                                          const float* __restrict__ delta, float* __restrict__ x) {
+    const uint32_t v = (uint32_t)(r & (SEG_MAX_V - 1));
+    x[v] = applied_f32(x[v], r, csc, delta);
+}
+
+/* ------------------------------------------------------------------------- */
+/* One launch per block: the voxel updates of block b-1 and the deltas of     */
+/* block b together. The image is held as 8-byte words {value, stamp}; the    */
+/* update of block b-1 writes its voxels' words with stamp b-1. A history of  */
+/* block b reads each path voxel's word: a voxel block b-1 does not touch has */
+/* its final value already; one it touches either carries stamp b-1 (written  */
+/* by this launch) or still its value before block b-1, and then the history  */
+/* applies block b-1's update to it itself, from the segment record its path  */
+/* entry carries -- the same expression as the voxel update, so the value is  */
+/* the same bit for bit. Nothing waits on anything: the launch boundary       */
+/* orders block b-1's deltas before both halves.                              */
+/* ------------------------------------------------------------------------- */
+__device__ __forceinline__ uint64_t xw_pack(float v, uint32_t stamp) {
+    return ((uint64_t)stamp << 32) | (uint64_t)__float_as_uint(v);
+}
+__device__ __forceinline__ float xw_value(uint64_t w) { return __uint_as_float((uint32_t)w); }
+__device__ __forceinline__ uint32_t xw_stamp(uint64_t w) { return (uint32_t)(w >> 32); }
+
+/* the path entry's voxel touched by the previous block? */
+__device__ __forceinline__ bool ent_has_prev(uint64_t e) {
+    return ((e >> SEG_VOX_BITS) & 3u) != 0u || (e >> 51) != 0u;
+}
+
+__device__ __forceinline__ uint64_t xw_load(const uint64_t* p) {
+    return __hip_atomic_load(p, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+}
+__device__ __forceinline__ void xw_store(uint64_t* p, uint64_t w) {
+    __hip_atomic_store(p, w, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+}
+
+template <bool F32>
+struct fused_prev {
+    const uint16_t* csc;          /* block b-1's first csc entry */
+    const void* delta;            /* block b-1's deltas          */
+    uint32_t stamp;               /* b - 1                        */
+};
+
+/* x value of block b's path entry e after block b-1 */
+template <bool F32>
+__device__ __forceinline__ float fused_x(uint64_t e, const uint64_t* __restrict__ xw, const fused_prev<F32>& pv) {
+    const uint64_t w = xw_load(xw + ent_voxel(e));
+    if (!ent_has_prev(e) || xw_stamp(w) == pv.stamp) return xw_value(w);
+    if (F32) return applied_f32(xw_value(w), e, pv.csc, (const float*)pv.delta);
+    return applied_f64(xw_value(w), e, pv.csc, (const double*)pv.delta);
+}
+
+template <bool F32>
+__device__ __forceinline__ void fused_delta(const uint64_t* __restrict__ list, int stride,
+                                            const uint32_t* __restrict__ len, const double* __restrict__ chord,
+                                            const float* __restrict__ wepl, int64_t h0, int64_t k,
+                                            const uint64_t* __restrict__ xw, const fused_prev<F32>& pv,
+                                            double lambda, void* __restrict__ delta, int lane) {
+    const int64_t h = h0 + k;
+    const uint32_t n = len[h];
+    const double Ld = chord[h];
+    const float b_i = wepl[h];
+    const uint64_t* p = list + k * stride;
+    uint64_t e[DELTA_SLOTS];
+#pragma unroll
+    for (int r = 0; r < DELTA_SLOTS; ++r) {
+        int j = lane + 64 * r;
+        e[r] = j < stride ? p[j] : ENT_SENTINEL;
+    }
+    if (F32) {
+        float s = 0.0f, c = 0.0f;
+#pragma unroll
+        for (int r = 0; r < DELTA_SLOTS; ++r)
+            if (e[r] != ENT_SENTINEL) kahan_add(s, c, fused_x<F32>(e[r], xw, pv));
+        for (int j = lane + 64 * DELTA_SLOTS; j < stride; j += 64) {
+            const uint64_t q = p[j];
+            if (q != ENT_SENTINEL) kahan_add(s, c, fused_x<F32>(q, xw, pv));
+        }
+#pragma unroll
+        for (int d = 32; d >= 1; d >>= 1) {
+            float os = __shfl_xor(s, d, 64), oc = __shfl_xor(c, d, 64);
+            c += oc;
+            kahan_add(s, c, os);
+        }
+        if (lane == 0) {
+            const float L = (float)Ld;
+            const float a_i_dot_x_k = s * L;
+            const float a_i_dot_a_i = (L * L) * (float)n;
+            ((float*)delta)[k] = L * ((b_i - a_i_dot_x_k) / a_i_dot_a_i) * (float)lambda;
+        }
+    } else {
+        double s = 0.0;
+#pragma unroll
+        for (int r = 0; r < DELTA_SLOTS; ++r)
+            if (e[r] != ENT_SENTINEL) s += (double)fused_x<F32>(e[r], xw, pv);
+        for (int j = lane + 64 * DELTA_SLOTS; j < stride; j += 64) {
+            const uint64_t q = p[j];
+            if (q != ENT_SENTINEL) s += (double)fused_x<F32>(q, xw, pv);
+        }
+        s = wave_sum_d(s);
+        if (lane == 0) {
+            const double a_i_dot_x_k = s * Ld;
+            const double a_i_dot_a_i = (Ld * Ld) * n;
+            ((double*)delta)[k] = Ld * ((b_i - a_i_dot_x_k) / a_i_dot_a_i) * lambda;
+        }
+    }
+}
+
+/* the voxel update of block b-1's segment record r on the image words */
+template <bool F32>
+__device__ __forceinline__ void fused_apply(uint64_t r, uint64_t* __restrict__ xw, const fused_prev<F32>& pv) {
+    const uint32_t v = (uint32_t)(r & (SEG_MAX_V - 1));
+    const float xv = xw_value(xw[v]);
+    const float nv = F32 ? applied_f32(xv, r, pv.csc, (const float*)pv.delta)
+                         : applied_f64(xv, r, pv.csc, (const double*)pv.delta);
+    xw_store(xw + v, xw_pack(nv, pv.stamp));
 }
 
 }  // namespace pct_blk

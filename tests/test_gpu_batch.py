@@ -75,3 +75,35 @@ def test_batch_replays_and_changed_block_sizes(scans):
                 ctx.drop_iteration(hist, r, B, None, 0)
             assert np.array_equal(x.cpu().numpy(), r.cpu().numpy())
     batch.close()
+
+
+def test_batch_follows_plans_rebuilt_by_their_own_context(scans):
+    """a scan whose histories change in place and whose plan its own context
+    rebuilds (pct_drop_iteration) is picked up by the next batch call: the
+    batch's captured graphs are dropped and every scan still equals its own
+    iteration; a context listed twice is refused"""
+    import pct_amd
+    with pytest.raises(pct_amd.PctError):
+        pct_amd.Batch([scans[0][0], scans[0][0]])
+    ctxs = [c for c, _, _ in scans[:3]]
+    hists = [h for _, h, _ in scans[:3]]
+    sizes = [3200, 1600, 800]
+    batch = pct_amd.Batch(ctxs)
+    xs = [x0.clone() for _, _, x0 in scans[:3]]
+    batch.iteration(hists, xs, sizes, None, pct_amd.PCT_DROP_CACHED)
+    # scan 1: histories changed in place, plan rebuilt through its own context
+    h1 = hists[1]
+    saved = h1.x_exit[:200].clone()
+    h1.x_exit[:200] += 0.3
+    ctxs[1].drop_invalidate()
+    x_own = scans[1][2].clone()
+    ctxs[1].drop_iteration(h1, x_own, sizes[1], None, pct_amd.PCT_DROP_CACHED)
+    xs = [x0.clone() for _, _, x0 in scans[:3]]
+    batch.iteration(hists, xs, sizes, None, pct_amd.PCT_DROP_CACHED)
+    for (ctx, hist, x0), B, x in zip(scans[:3], sizes, xs):
+        r = x0.clone()
+        ctx.drop_iteration(hist, r, B, None, pct_amd.PCT_DROP_CACHED)
+        assert np.array_equal(x.cpu().numpy(), r.cpu().numpy())
+    batch.close()
+    h1.x_exit[:200].copy_(saved)
+    ctxs[1].drop_invalidate()

@@ -4,8 +4,9 @@ oracle cannot run there in test time, so size-independent properties:
 
   - the schedules of an iteration -- MLP lists recomputed beside the block
     loop in chunks (default), cached lists, MLP first then the block loop,
-    other chunkings -- give bit-identical images over two iterations, also
-    for the research variants and the precision form;
+    other chunkings, one or two launches per block -- give bit-identical
+    images over two iterations, also for the research variants and the
+    precision form (one and two launches per block);
   - lambda = 0 leaves the image unchanged;
   - the plan's recomputed lists match the cached ones (no plan flag);
   - FBP x0 is zero outside the hull and the reconstruction histories all
@@ -51,15 +52,17 @@ def test_full_size_schedules_agree(full):
     for name, mode, knobs in (("default", pct_amd.PCT_MLP_FP64, {}),
                               ("cached", pct_amd.PCT_MLP_FP64 | pct_amd.PCT_DROP_CACHED, {}),
                               ("sequential", pct_amd.PCT_MLP_FP64, {"overlap": 2}),
-                              ("chunks", pct_amd.PCT_MLP_FP64, {"chunk_first": 5, "chunk_max": 40})):
-        pct_amd.set_tuning(ctx, overlap=0, chunk_first=0, chunk_max=0, mlp_waves=0)
+                              ("chunks", pct_amd.PCT_MLP_FP64, {"chunk_first": 5, "chunk_max": 40}),
+                              ("one launch per block", pct_amd.PCT_MLP_FP64 | pct_amd.PCT_DROP_CACHED,
+                               {"block_form": 1})):
+        pct_amd.set_tuning(ctx, overlap=0, chunk_first=0, chunk_max=0, mlp_waves=0, block_form=0)
         pct_amd.set_tuning(ctx, **knobs)
         x = pre.x0.clone()
         for _ in range(2):
             ctx.drop_iteration(hist, x, B, p.lambda_, mode)
         images[name] = x.cpu().numpy()
         print(f"  {name} done", flush=True)
-    pct_amd.set_tuning(ctx, overlap=0, chunk_first=0, chunk_max=0, mlp_waves=0)
+    pct_amd.set_tuning(ctx, overlap=0, chunk_first=0, chunk_max=0, mlp_waves=0, block_form=0)
     ref = images["default"]
     assert np.sum(ref != pre.x0.cpu().numpy()) > 100_000
     for name, img in images.items():
@@ -79,10 +82,13 @@ def test_full_size_forms_schedules_agree(full, form):
             "robust2": pct_amd.PCT_MLP_FP64 | pct_amd.PCT_DROP_ROBUST2,
             "precision": pct_amd.PCT_PRECISION}[form]
     out = []
-    for extra in (0, pct_amd.PCT_DROP_CACHED):
+    for extra, form in ((0, 0), (pct_amd.PCT_DROP_CACHED, 0), (pct_amd.PCT_DROP_CACHED, 1)):
+        pct_amd.set_tuning(ctx, block_form=form)
         x = pre.x0.clone()
         for _ in range(2):
             ctx.drop_iteration(pre.hist, x, p.drop_block_size, p.lambda_, mode | extra)
         out.append(x.cpu().numpy())
+    pct_amd.set_tuning(ctx, block_form=0)
     assert np.sum(out[0] != pre.x0.cpu().numpy()) > 100_000
-    assert np.array_equal(out[0].view(np.uint32), out[1].view(np.uint32))
+    for o in out[1:]:
+        assert np.array_equal(out[0].view(np.uint32), o.view(np.uint32))

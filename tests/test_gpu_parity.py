@@ -155,14 +155,20 @@ def test_drop_modes_agree(ctx_mini, mini_host):
     x0 = mini_host["hull"].astype(np.float32)
     lam = 1e-2
     out = {}
-    for name, mode in [("default", 0), ("cached", pct_amd.PCT_DROP_CACHED), ("eager", pct_amd.PCT_DROP_NO_GRAPH),
-                       ("atomic", pct_amd.PCT_DROP_ATOMIC)]:
+    for name, mode, form in [("default", 0, 0), ("cached", pct_amd.PCT_DROP_CACHED, 0),
+                             ("eager", pct_amd.PCT_DROP_NO_GRAPH, 0), ("one-launch", 0, 1),
+                             ("one-launch eager", pct_amd.PCT_DROP_NO_GRAPH, 1),
+                             ("f32", pct_amd.PCT_DROP_F32, 0), ("f32 one-launch", pct_amd.PCT_DROP_F32, 1),
+                             ("atomic", pct_amd.PCT_DROP_ATOMIC, 0)]:
+        pct_amd.set_tuning(ctx_mini, block_form=form)
         x = torch.from_numpy(x0.copy()).cuda()
         ctx_mini.drop_iteration(d, x, lam=lam, mode=mode)
         ctx_mini.drop_iteration(d, x, lam=lam, mode=mode)
         out[name] = x.cpu().numpy()
-    assert np.array_equal(out["default"], out["cached"])
-    assert np.array_equal(out["default"], out["eager"])
+    pct_amd.set_tuning(ctx_mini, block_form=0)
+    for name in ("cached", "eager", "one-launch", "one-launch eager"):
+        assert np.array_equal(out["default"].view(np.uint32), out[name].view(np.uint32)), name
+    assert np.array_equal(out["f32"].view(np.uint32), out["f32 one-launch"].view(np.uint32))
     assert np.max(np.abs(out["default"] - out["atomic"])) < 1e-5
 
 
@@ -191,9 +197,22 @@ def test_plan_detects_changed_inputs(mini_host):
     d = _to_dev(mini_host["hr"])
     x = torch.from_numpy(mini_host["hull"].astype(np.float32)).cuda()
     ctx.drop_iteration(d, x, lam=1e-2)
+    before = x.clone()
     d.x_exit[:100] += 0.5           # same arrays, different content
     with pytest.raises(pct_amd.PctError):
         ctx.drop_iteration(d, x, lam=1e-2)
+    assert torch.equal(x, before)   # x restored to its value before the call
+    ctx.drop_iteration(d, x, lam=1e-2)          # the plan is rebuilt for the new content
+    assert not torch.equal(x, before)
+    # cached lists after an in-place change: the caller invalidates the plan
+    x2 = before.clone()
+    ctx.drop_invalidate()
+    ctx.drop_iteration(d, x2, lam=1e-2, mode=pct_amd.PCT_DROP_CACHED)
+    x3 = before.clone()
+    ctx2 = pct_amd.Context(mini_host["p"])
+    ctx2.drop_iteration(d, x3, lam=1e-2, mode=pct_amd.PCT_DROP_CACHED)
+    assert torch.equal(x2, x3)
+    ctx2.close()
     ctx.close()
 
 
