@@ -252,6 +252,23 @@ int pct_drop_block_update(pct_ctx* ctx, const pct_hist_soa* hist, int64_t start,
                           int64_t count, const float* x, float* x_update,
                           uint32_t* counts, float lambda, int mode);
 
+/* Allocation-free form of pct_drop_block_update (no counterpart in the
+ * reference): same arguments, same kernels and results, for callers that
+ * cannot allocate device memory -- code under HIP graph capture, tight
+ * serving loops, harnesses that forbid hipMalloc. The block's MLP lists live
+ * in a fixed device array of the library (about 17 MB per GPU, held until the
+ * process ends) instead of context workspaces grown on demand, so a call
+ * takes at most PCT_NOALLOC_MAX_HISTORIES histories and needs the context's
+ * max_intersections <= PCT_NOALLOC_MAX_SLOTS (PCT_E_INVALID otherwise). The
+ * call allocates nothing (the context's tables and its pinned status word
+ * are made by pct_create). Every call on one GPU shares that array: calls
+ * that overlap on one GPU (other streams or threads) race. */
+#define PCT_NOALLOC_MAX_HISTORIES 8192
+#define PCT_NOALLOC_MAX_SLOTS 512
+int pct_drop_block_update_noalloc(pct_ctx* ctx, const pct_hist_soa* hist, int64_t start,
+                                  int64_t count, const float* x, float* x_update,
+                                  uint32_t* counts, float lambda, int mode);
+
 /* (e) Image update: replaces image_update_GPU (.cu:5900-5938):
  * x[v] += x_update[v] / counts[v] where counts[v] > 0, then clears both. */
 int pct_drop_image_update(pct_ctx* ctx, float* x, float* x_update, uint32_t* counts);

@@ -354,6 +354,14 @@ class Context:
         self._check(self.L.pct_drop_block_update(self._ctx, ctypes.byref(hs), start, count, _ptr(x), _ptr(x_update),
                                                  _ptr(counts), float(lam), mode), "pct_drop_block_update")
 
+    def drop_block_update_noalloc(self, hist, start, count, x, x_update, counts, lam, mode=PCT_MLP_FP64):
+        """pct_drop_block_update_noalloc: the same update with the lists in the
+        library's fixed device array (no allocation; at most 8192 histories)."""
+        hs = hist.soa()
+        self._check(self.L.pct_drop_block_update_noalloc(self._ctx, ctypes.byref(hs), start, count, _ptr(x),
+                                                         _ptr(x_update), _ptr(counts), float(lam), mode),
+                    "pct_drop_block_update_noalloc")
+
     def drop_image_update(self, x, x_update, counts):
         self._check(self.L.pct_drop_image_update(self._ctx, _ptr(x), _ptr(x_update), _ptr(counts)),
                     "pct_drop_image_update")

@@ -139,6 +139,8 @@ SIGNATURES = {
     "pct_mlp_capacity": (ctypes.c_int, [c_vp, P(PctHistSoa), c_i64, c_i64, c_vp, P(c_i64)]),
     "pct_mlp_paths": (ctypes.c_int, [c_vp, P(PctHistSoa), c_i64, c_i64, c_vp, c_vp, c_vp, ctypes.c_int]),
     "pct_drop_block_update": (ctypes.c_int, [c_vp, P(PctHistSoa), c_i64, c_i64, c_vp, c_vp, c_vp, c_f32, ctypes.c_int]),
+    "pct_drop_block_update_noalloc": (ctypes.c_int, [c_vp, P(PctHistSoa), c_i64, c_i64, c_vp, c_vp, c_vp, c_f32,
+                                                     ctypes.c_int]),
     "pct_drop_image_update": (ctypes.c_int, [c_vp, c_vp, c_vp, c_vp]),
     "pct_drop_iteration": (ctypes.c_int, [c_vp, P(PctHistSoa), c_vp, ctypes.c_int, c_f32, ctypes.c_int]),
     "pct_drop_invalidate": (ctypes.c_int, [c_vp]),
@@ -178,6 +180,8 @@ SIGNATURES = {
                                                 ctypes.c_int, c_f32, ctypes.c_int]),
 }
 
+PCT_NOALLOC_MAX_HISTORIES = 8192
+PCT_NOALLOC_MAX_SLOTS = 512
 PCT_MLP_FP64 = 0
 PCT_MLP_TABLE = 1
 PCT_MLP_SURROGATE = 2

@@ -15,7 +15,8 @@ CSRC = os.path.join(HERE, "csrc")
 OBJ = os.path.join(HERE, "_build")
 OUT = os.path.join(HERE, "libpct.so")
 SOURCES = ["pct_ctx.hip", "pct_intersect.hip", "pct_mlp.hip", "pct_drop.hip", "pct_plan.hip", "pct_batch.hip",
-           "pct_bins.hip", "pct_hull.hip", "pct_fbp.hip", "pct_net.hip", "pct_research.hip", "pct_io.cpp"]
+           "pct_bins.hip", "pct_hull.hip", "pct_fbp.hip", "pct_net.hip", "pct_research.hip", "pct_noalloc.hip",
+           "pct_io.cpp"]
 
 HIPCC_FLAGS = [
     "-O3", "-std=c++17", "--offload-arch=gfx950", "-fPIC",

@@ -120,13 +120,16 @@ int pct_ensure(pct_ctx* ctx, pct_buffer& b, size_t bytes) {
 }
 
 int pct_take_status(pct_ctx* ctx, int* flags) {
-    int h = 0;
-    PCT_HIP_CHECK(ctx, hipMemcpyAsync(&h, ctx->d_status, sizeof(int), hipMemcpyDeviceToHost, ctx->stream));
+    PCT_HIP_CHECK(ctx, hipMemcpyAsync(ctx->h_status, ctx->d_status, sizeof(int), hipMemcpyDeviceToHost, ctx->stream));
     PCT_HIP_CHECK(ctx, hipStreamSynchronize(ctx->stream));
+    const int h = *ctx->h_status;
     if (h) PCT_HIP_CHECK(ctx, hipMemsetAsync(ctx->d_status, 0, sizeof(int), ctx->stream));
     *flags = h;
     return PCT_OK;
 }
+
+int pct_mlp_tables(pct_ctx* ctx);
+int pct_noalloc_init(int device);
 
 extern "C" pct_ctx* pct_create(const pct_params* params, void* stream) {
     if (!params) return nullptr;
@@ -143,10 +146,18 @@ extern "C" pct_ctx* pct_create(const pct_params* params, void* stream) {
         ctx->own_stream = true;
     }
     if (hipMalloc(&ctx->d_status, sizeof(int)) != hipSuccess ||
+        hipHostMalloc((void**)&ctx->h_status, sizeof(int), hipHostMallocDefault) != hipSuccess ||
         hipMemsetAsync(ctx->d_status, 0, sizeof(int), ctx->stream) != hipSuccess) {
         pct_destroy(ctx);
         return nullptr;
     }
+    /* the MLP tables now, and the device address of the allocation-free
+     * entry point's scratch array (once per device, this device current) */
+    if (pct_mlp_tables(ctx) != PCT_OK) {
+        pct_destroy(ctx);
+        return nullptr;
+    }
+    (void)pct_noalloc_init(ctx->device);
     return ctx;
 }
 
@@ -163,6 +174,7 @@ extern "C" void pct_destroy(pct_ctx* ctx) {
     for (pct_buffer& b : ctx->walk_order)
         if (b.ptr) (void)hipFree(b.ptr);
     if (ctx->d_status) (void)hipFree(ctx->d_status);
+    if (ctx->h_status) (void)hipHostFree(ctx->h_status);
     if (ctx->d_mlp_tab) (void)hipFree(ctx->d_mlp_tab);
     if (ctx->d_scat_tab) (void)hipFree(ctx->d_scat_tab);
     if (ctx->d_stats) (void)hipFree(ctx->d_stats);

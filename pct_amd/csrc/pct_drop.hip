@@ -159,6 +159,14 @@ static int launch_scatter(pct_ctx* ctx, const uint32_t* voxels, const int64_t* o
     return PCT_OK;
 }
 
+/* the scatter of pct_drop_block_update for lists already made (shared with
+ * its allocation-free form, pct_noalloc.hip) */
+int pct_drop_launch_scatter(pct_ctx* ctx, const uint32_t* voxels, const int64_t* offsets, const uint32_t* lengths,
+                            const double* chord, const float* wepl, int64_t count, const float* x, float* x_update,
+                            uint32_t* counts, float lambda) {
+    return launch_scatter(ctx, voxels, offsets, lengths, chord, wepl, count, x, x_update, counts, lambda);
+}
+
 static int launch_apply(pct_ctx* ctx, float* x, float* x_update, uint32_t* counts, bool stats = false) {
     int64_t V = ctx->g.V;
     bool vec = ((uintptr_t)counts & 15) == 0;

@@ -60,6 +60,8 @@ struct pct_ctx {
     std::string err;
     /* device-side status word written by kernels (overflow flags etc.) */
     int* d_status = nullptr;
+    /* pinned host copy the status word is read back into */
+    int* h_status = nullptr;
     /* MLP per-depth table */
     double* d_mlp_tab = nullptr;
     int mlp_tab_k = 0;

@@ -1108,6 +1108,14 @@ static int mlp_occupancy(pct_ctx* ctx) {
     return PCT_OK;
 }
 
+/* the per-depth tables of both MLP forms and the walk kernels' occupancy,
+ * made when the context is created (so that no later MLP call allocates) */
+int pct_mlp_tables(pct_ctx* ctx) {
+    int rc;
+    if ((rc = ensure_mlp_table(ctx)) || (rc = ensure_scat_table(ctx))) return rc;
+    return mlp_occupancy(ctx);
+}
+
 static int check_hist(pct_ctx* ctx, const pct_hist_soa* h, int64_t start, int64_t count, const char* who) {
     if (!ctx || !h) return pct_fail(ctx, PCT_E_INVALID, "%s: null argument", who);
     if (start < 0 || count < 0 || start + count > h->n)
