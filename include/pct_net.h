@@ -109,10 +109,17 @@ int pct_net_forward(pct_ctx* ctx, const float* features, int64_t n, float* out);
 /* Calibration check on every `stride`-th history of [start, start+count):
  * both walks run; per sampled history the largest |t| and |v| deviation of the
  * surrogate path from the exact MLP over all steps and whether the two voxel
- * lists are equal. A cell whose 99th-percentile deviation exceeds `tol` cm, or
- * that has fewer than `min_samples` samples, is sent to the exact walk from
- * then on. `tol` <= 0 only measures (against the cell flags in force). Results
- * are in *out (host). */
+ * lists are equal. Cells are 8 bins of u2 x 8 bins of the bending magnitude.
+ * With rule PCT_NET_RULE_DEVIATION a cell whose 99th-percentile deviation
+ * exceeds `threshold` cm goes to the exact walk from then on; with
+ * PCT_NET_RULE_LISTS (the north star's rule: fall back wherever the voxel
+ * list disagrees) a cell whose share of sampled histories with a voxel list
+ * different from the exact walk's exceeds `threshold` (0: any disagreement)
+ * does. Either way a cell with fewer than `min_samples` samples falls back.
+ * threshold < 0 only measures (against the cell flags in force). Results are
+ * in *out (host). */
+#define PCT_NET_RULE_DEVIATION 0
+#define PCT_NET_RULE_LISTS     1
 typedef struct pct_net_calib {
     int64_t samples;             /* histories checked                          */
     int64_t list_mismatch;       /* of them, voxel list != exact voxel list     */
@@ -125,7 +132,12 @@ typedef struct pct_net_calib {
     int32_t cell_samples[PCT_NET_CELLS];
     float   cell_p99[PCT_NET_CELLS];
     uint8_t cell_fallback[PCT_NET_CELLS];
+    float   cell_list_mismatch[PCT_NET_CELLS];  /* share of the cell's samples
+                                                   whose lists disagree     */
 } pct_net_calib;
+int pct_net_calibrate_rule(pct_ctx* ctx, const pct_hist_soa* hist, int64_t start, int64_t count, int64_t stride,
+                           int rule, float threshold, int32_t min_samples, pct_net_calib* out);
+/* the deviation rule with tol as its threshold (tol <= 0: measure only) */
 int pct_net_calibrate(pct_ctx* ctx, const pct_hist_soa* hist, int64_t start, int64_t count, int64_t stride,
                       float tol, int32_t min_samples, pct_net_calib* out);
 

@@ -94,7 +94,7 @@ class PctNetCalib(ctypes.Structure):
                 ("surrogate_mismatch", c_i64), ("dev_median", c_f64),
                 ("dev_p99", c_f64), ("dev_max", c_f64), ("cells_fallback", c_i32),
                 ("cell_samples", c_i32 * NET_CELLS), ("cell_p99", c_f32 * NET_CELLS),
-                ("cell_fallback", ctypes.c_uint8 * NET_CELLS)]
+                ("cell_fallback", ctypes.c_uint8 * NET_CELLS), ("cell_list_mismatch", c_f32 * NET_CELLS)]
 
 
 PCTD_STR = 256
@@ -161,6 +161,8 @@ SIGNATURES = {
     "pct_net_set_hermite": (ctypes.c_int, [c_vp, c_vp, c_vp]),
     "pct_net_forward": (ctypes.c_int, [c_vp, c_vp, c_i64, c_vp]),
     "pct_net_calibrate": (ctypes.c_int, [c_vp, P(PctHistSoa), c_i64, c_i64, c_i64, c_f32, c_i32, P(PctNetCalib)]),
+    "pct_net_calibrate_rule": (ctypes.c_int, [c_vp, P(PctHistSoa), c_i64, c_i64, c_i64, ctypes.c_int, c_f32, c_i32,
+                                              P(PctNetCalib)]),
     "pct_net_fallback_stats": (ctypes.c_int, [c_vp, P(c_i64), P(c_i64)]),
     "pct_synth_generate": (ctypes.c_int, [c_vp, P(PctSynthCfg), c_i64, c_i64, P(PctSynthSoa)]),
     "pct_synth_generate_host": (ctypes.c_int, [P(PctSynthCfg), c_i64, c_i64, P(PctSynthSoa)]),
@@ -181,6 +183,8 @@ SIGNATURES = {
 }
 
 PCT_NOALLOC_MAX_HISTORIES = 8192
+PCT_NET_RULE_DEVIATION = 0
+PCT_NET_RULE_LISTS = 1
 PCT_NOALLOC_MAX_SLOTS = 512
 PCT_MLP_FP64 = 0
 PCT_MLP_TABLE = 1

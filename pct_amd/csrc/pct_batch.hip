@@ -31,7 +31,7 @@ namespace {
 struct batch_delta_args {
     int n;
     uint32_t wg0[PCT_BATCH_MAX + 1];
-    const uint64_t* list[PCT_BATCH_MAX];
+    const uint32_t* list[PCT_BATCH_MAX];
     int stride[PCT_BATCH_MAX];
     const uint32_t* len[PCT_BATCH_MAX];
     const double* chord[PCT_BATCH_MAX];
@@ -214,7 +214,7 @@ static int batch_steps(pct_batch* b, float* const* xs, const pct_hist_soa* hists
             const int64_t s0 = pl->blk_seg_h[j], ns = pl->blk_seg_h[j + 1] - s0;
             int d = da.n++;
             da.wg0[d] = dwg;
-            da.list[d] = (const uint64_t*)pl->list.ptr + pl->list_base_h[j];
+            da.list[d] = (const uint32_t*)pl->list.ptr + pl->list_base_h[j];
             da.stride[d] = pl->stride_h[j];
             da.len[d] = (const uint32_t*)pl->len.ptr;
             da.chord[d] = (const double*)pl->chord.ptr;

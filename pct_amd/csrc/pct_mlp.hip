@@ -1273,9 +1273,17 @@ extern "C" int pct_net_features(pct_ctx* ctx, const pct_hist_soa* hist, int64_t 
 
 extern "C" int pct_net_calibrate(pct_ctx* ctx, const pct_hist_soa* hist, int64_t start, int64_t count, int64_t stride,
                                  float tol, int32_t min_samples, pct_net_calib* out) {
+    return pct_net_calibrate_rule(ctx, hist, start, count, stride, PCT_NET_RULE_DEVIATION, tol > 0.0f ? tol : -1.0f,
+                                  min_samples, out);
+}
+
+extern "C" int pct_net_calibrate_rule(pct_ctx* ctx, const pct_hist_soa* hist, int64_t start, int64_t count,
+                                      int64_t stride, int rule, float threshold, int32_t min_samples,
+                                      pct_net_calib* out) {
     int rc = check_hist(ctx, hist, start, count, "pct_net_calibrate");
     if (rc) return rc;
-    if (!out || stride <= 0 || !hist->wepl) return pct_fail(ctx, PCT_E_INVALID, "pct_net_calibrate: bad argument");
+    if (!out || stride <= 0 || !hist->wepl || (rule != PCT_NET_RULE_DEVIATION && rule != PCT_NET_RULE_LISTS))
+        return pct_fail(ctx, PCT_E_INVALID, "pct_net_calibrate: bad argument");
     if (!ctx->net_version) return pct_fail(ctx, PCT_E_INVALID, "pct_net_calibrate: no network installed");
     if ((rc = ensure_mlp_table(ctx)) || (rc = mlp_occupancy(ctx))) return rc;
     const int64_t nsamp = (count + stride - 1) / stride;
@@ -1305,6 +1313,7 @@ extern "C" int pct_net_calibrate(pct_ctx* ctx, const pct_hist_soa* hist, int64_t
     /* statistics over the samples whose inputs are inside the trained range
      * (the others take the exact walk whatever their cell) */
     std::vector<std::vector<float>> per_cell(PCT_NET_CELLS);
+    std::vector<int64_t> cell_mis(PCT_NET_CELLS, 0);
     std::vector<float> all;
     out->samples = nsamp;
     for (int64_t q = 0; q < nsamp; ++q) {
@@ -1314,6 +1323,7 @@ extern "C" int pct_net_calibrate(pct_ctx* ctx, const pct_hist_soa* hist, int64_t
         all.push_back(hdev[q]);
         int c = hcell[q] < 0 ? 0 : (hcell[q] >= PCT_NET_CELLS ? PCT_NET_CELLS - 1 : hcell[q]);
         per_cell[c].push_back(hdev[q]);
+        cell_mis[c] += hmis[q];
     }
     auto quant = [](std::vector<float>& v, double p) -> double {
         if (v.empty()) return 0.0;
@@ -1326,14 +1336,17 @@ extern "C" int pct_net_calibrate(pct_ctx* ctx, const pct_hist_soa* hist, int64_t
     out->dev_max = all.empty() ? 0.0 : *std::max_element(all.begin(), all.end());
     uint8_t flags[PCT_NET_CELLS];
     for (int c = 0; c < PCT_NET_CELLS; ++c) {
-        out->cell_samples[c] = (int32_t)per_cell[c].size();
+        const int64_t ns = (int64_t)per_cell[c].size();
+        out->cell_samples[c] = (int32_t)ns;
         out->cell_p99[c] = (float)quant(per_cell[c], 0.99);
-        bool fb = (int32_t)per_cell[c].size() < min_samples || out->cell_p99[c] > tol;
+        out->cell_list_mismatch[c] = ns > 0 ? (float)((double)cell_mis[c] / (double)ns) : 0.0f;
+        const float score = rule == PCT_NET_RULE_LISTS ? out->cell_list_mismatch[c] : out->cell_p99[c];
+        bool fb = ns < min_samples || score > threshold;
         out->cell_fallback[c] = fb ? 1 : 0;
         flags[c] = out->cell_fallback[c];
         out->cells_fallback += fb ? 1 : 0;
     }
-    if (tol > 0.0f) {
+    if (threshold >= 0.0f) {
         PCT_HIP_CHECK(ctx, hipMemcpy(ctx->d_net_cells, flags, sizeof flags, hipMemcpyHostToDevice));
         static int calib_version = 1 << 20;
         ctx->net_version = ++calib_version;     /* plans built before the calibration are stale */

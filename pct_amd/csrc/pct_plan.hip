@@ -102,6 +102,11 @@ __global__ void k_hist_offsets(const int64_t* __restrict__ list_base, const int3
 
 /* grid-stride: a dispatch holds at most 2^32 - 1 work-items, and the list
  * store of a large scan has more entries than that */
+__global__ void k_fill_u32(uint32_t* __restrict__ p, int64_t n, uint32_t v) {
+    for (int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; i < n; i += (int64_t)gridDim.x * blockDim.x)
+        p[i] = v;
+}
+
 __global__ void k_fill_u64(uint64_t* __restrict__ p, int64_t n, uint64_t v) {
     for (int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; i < n; i += (int64_t)gridDim.x * blockDim.x)
         p[i] = v;
@@ -121,32 +126,35 @@ k_seg_map(const uint64_t* __restrict__ segs, const int64_t* __restrict__ blk_seg
     }
 }
 
-/* one wavefront per history of [h0, h1): its path entries in the
- * block-strided list store -- the voxel, or the previous block's segment
- * record of the voxel when that block touches it (looked up in the previous
- * block's map) */
+/* one wavefront per history of [h0, h1): its voxel list into the
+ * block-strided list store and, when prev is given, slot for slot the
+ * previous block's segment record of each voxel (0 when that block does not
+ * touch it; looked up in the previous block's map) */
 __global__ void __launch_bounds__(256)
 k_build_entries(const uint32_t* __restrict__ path, const int64_t* __restrict__ cap_off,
                 const uint32_t* __restrict__ len, const int64_t* __restrict__ hist_off, int64_t h0, int64_t h1,
                 int64_t B, const uint64_t* __restrict__ segs, const int64_t* __restrict__ blk_seg, int64_t M, int64_t V,
-                const uint64_t* __restrict__ map, uint64_t* __restrict__ ent) {
+                const uint64_t* __restrict__ map, uint32_t* __restrict__ list, uint64_t* __restrict__ prev) {
     const int64_t h = h0 + (int64_t)blockIdx.x * 4 + (threadIdx.x >> 6);
     const int lane = threadIdx.x & 63;
     if (h >= h1) return;
     const int64_t b = h / B;
     const int n = (int)len[h];
     const uint32_t* src = path + cap_off[h];
-    uint64_t* dst = ent + hist_off[h];
-    const uint64_t* m = b > 0 ? map + ((b - 1) % M) * V : nullptr;
+    const int64_t dst = hist_off[h];
+    const uint64_t* m = (prev && b > 0) ? map + ((b - 1) % M) * V : nullptr;
     const int64_t sp = b > 0 ? blk_seg[b - 1] : 0;
     for (int j = lane; j < n; j += 64) {
         const uint32_t v = src[j];
-        uint64_t e = v;
-        if (b > 0) {
-            const uint64_t w = m[v];
-            if ((w >> 32) == (uint64_t)(b - 1)) e = segs[sp + (int64_t)(uint32_t)w];
+        list[dst + j] = v;
+        if (prev) {
+            uint64_t e = 0;
+            if (m) {
+                const uint64_t w = m[v];
+                if ((w >> 32) == (uint64_t)(b - 1)) e = segs[sp + (int64_t)(uint32_t)w];
+            }
+            prev[dst + j] = e;
         }
-        dst[j] = e;
     }
 }
 
@@ -213,7 +221,7 @@ __global__ void k_write_segments(const uint32_t* __restrict__ keys, const uint32
  * wave priority: they are latency-bound and share the CUs with the
  * long-running MLP waves of the next chunks. */
 __global__ void __launch_bounds__(64 * DELTA_WAVES)
-k_block_delta(const uint64_t* __restrict__ list, int stride, const uint32_t* __restrict__ len,
+k_block_delta(const uint32_t* __restrict__ list, int stride, const uint32_t* __restrict__ len,
               const double* __restrict__ chord, const float* __restrict__ wepl, int64_t h0, int64_t count,
               const float* __restrict__ x, double lambda, double* __restrict__ delta) {
     __builtin_amdgcn_s_setprio(3);
@@ -223,7 +231,7 @@ k_block_delta(const uint64_t* __restrict__ list, int stride, const uint32_t* __r
 }
 
 __global__ void __launch_bounds__(64 * DELTA_WAVES)
-k_block_delta_f32(const uint64_t* __restrict__ list, int stride, const uint32_t* __restrict__ len,
+k_block_delta_f32(const uint32_t* __restrict__ list, int stride, const uint32_t* __restrict__ len,
                   const double* __restrict__ chord, const float* __restrict__ wepl, int64_t h0, int64_t count,
                   const float* __restrict__ x, float lambda, float* __restrict__ delta) {
     __builtin_amdgcn_s_setprio(3);
@@ -253,7 +261,7 @@ k_block_apply_f32(const uint64_t* __restrict__ segs, int64_t s0, int64_t ns, con
 /* the research variants of the FP64 update (pct_plan_kernels.h) */
 template <int V>
 __global__ void __launch_bounds__(64 * DELTA_WAVES)
-k_block_delta_var(const uint64_t* __restrict__ list, int stride, const uint32_t* __restrict__ len,
+k_block_delta_var(const uint32_t* __restrict__ list, int stride, const uint32_t* __restrict__ len,
                   const double* __restrict__ chord, const float* __restrict__ wepl, int64_t h0, int64_t count,
                   const float* __restrict__ x, double lambda, double* __restrict__ out, int64_t B) {
     __builtin_amdgcn_s_setprio(3);
@@ -280,7 +288,8 @@ k_block_apply_var(const uint64_t* __restrict__ segs, int64_t s0, int64_t ns, con
 #define FUSED_DELTA_WAVES (FUSED_THREADS / 64)
 template <bool F32>
 __global__ void __launch_bounds__(FUSED_THREADS)
-k_block_fused(const uint64_t* __restrict__ list, int stride, const uint32_t* __restrict__ len,
+k_block_fused(const uint32_t* __restrict__ list, const uint64_t* __restrict__ prev, int stride,
+              const uint32_t* __restrict__ len,
               const double* __restrict__ chord, const float* __restrict__ wepl, int64_t h0, int64_t count,
               double lambda, void* __restrict__ delta_out, const uint64_t* __restrict__ segs_p, int64_t ns_p,
               const uint16_t* __restrict__ csc_p, const void* __restrict__ delta_p, uint32_t stamp_p, unsigned na,
@@ -294,7 +303,8 @@ k_block_fused(const uint64_t* __restrict__ list, int stride, const uint32_t* __r
     }
     const int64_t k = (int64_t)(blockIdx.x - na) * FUSED_DELTA_WAVES + (threadIdx.x >> 6);
     if (k >= count) return;
-    pct_blk::fused_delta<F32>(list, stride, len, chord, wepl, h0, k, xw, pv, lambda, delta_out, threadIdx.x & 63);
+    pct_blk::fused_delta<F32>(list, prev, stride, len, chord, wepl, h0, k, xw, pv, lambda, delta_out,
+                              threadIdx.x & 63);
 }
 
 /* the image as {value, stamp} words (stamp: no block yet) and back */
@@ -334,7 +344,7 @@ static void release(pct_buffer& b) {
 void pct_plan_destroy(pct_plan* pl) {
     if (!pl) return;
     plan_release_graph(pl);
-    pct_buffer* bufs[] = {&pl->list, &pl->hist_off, &pl->len, &pl->chord, &pl->hash, &pl->segs,
+    pct_buffer* bufs[] = {&pl->list, &pl->prev, &pl->hist_off, &pl->len, &pl->chord, &pl->hash, &pl->segs,
                           &pl->csc, &pl->blk_seg, &pl->blk_entry, &pl->delta, &pl->xw};
     for (pct_buffer* b : bufs) release(*b);
     for (hipEvent_t e : pl->ev_chunk)
@@ -533,27 +543,36 @@ static int plan_build(pct_ctx* ctx, const pct_hist_soa* hist, int B, int mlp_mod
     PCT_HIP_CHECK(ctx, hipMemcpyAsync(pl->blk_seg_h.data(), pl->blk_seg.ptr, sizeof(int64_t) * (nb + 1),
                                       hipMemcpyDeviceToHost, ctx->stream));
     PCT_HIP_CHECK(ctx, hipStreamSynchronize(ctx->stream));
-    /* 5. the list store: one 8-byte entry per slot, the voxel or the previous
-     * block's segment record of it, unused slots the sentinel. Maps from voxel
-     * to segment for M blocks at a time (tagged with the block, so never
-     * cleared); the build store is freed afterwards. */
-    if ((rc = pct_ensure(ctx, pl->list, sizeof(uint64_t) * (size_t)(list_size > 0 ? list_size : 1)))) return rc;
-    hipLaunchKernelGGL(k_fill_u64, dim3(std::min(pct_grid(list_size, 256), PCT_GRID_CAP)), dim3(256), 0, ctx->stream,
-                       (uint64_t*)pl->list.ptr, list_size, ENT_SENTINEL);
+    /* 5. the list store (unused slots the sentinel) and, for the
+     * one-launch-per-block form, slot for slot the previous block's segment
+     * record of each voxel. Maps from voxel to segment for M blocks at a time
+     * (tagged with the block, so never cleared); the build store is freed
+     * afterwards. */
+    const bool with_prev = ctx->tune_block_form == 1;
+    if ((rc = pct_ensure(ctx, pl->list, sizeof(uint32_t) * (size_t)(list_size > 0 ? list_size : 1)))) return rc;
+    hipLaunchKernelGGL(k_fill_u32, dim3(std::min(pct_grid(list_size, 256), PCT_GRID_CAP)), dim3(256), 0, ctx->stream,
+                       (uint32_t*)pl->list.ptr, list_size, LIST_SENTINEL);
+    if (with_prev) {
+        if ((rc = pct_ensure(ctx, pl->prev, sizeof(uint64_t) * (size_t)(list_size > 0 ? list_size : 1)))) return rc;
+    } else {
+        release(pl->prev);
+    }
     {
         const int64_t V = ctx->g.V;
-        int64_t M = std::max<int64_t>(2, std::min<int64_t>(256, ((int64_t)1 << 31) / (8 * V)));
+        int64_t M = with_prev ? std::max<int64_t>(2, std::min<int64_t>(256, ((int64_t)1 << 31) / (8 * V))) : 256;
         M = std::min<int64_t>(M, nb + 1);
         pct_buffer map;
-        if ((rc = pct_ensure(ctx, map, sizeof(uint64_t) * (size_t)(M * V)))) return rc;
-        hipLaunchKernelGGL(k_fill_u64, dim3(std::min(pct_grid(M * V, 256), PCT_GRID_CAP)), dim3(256), 0, ctx->stream,
-                           (uint64_t*)map.ptr, M * V, ~0ull);
+        if (with_prev) {
+            if ((rc = pct_ensure(ctx, map, sizeof(uint64_t) * (size_t)(M * V)))) return rc;
+            hipLaunchKernelGGL(k_fill_u64, dim3(std::min(pct_grid(M * V, 256), PCT_GRID_CAP)), dim3(256), 0,
+                               ctx->stream, (uint64_t*)map.ptr, M * V, ~0ull);
+        }
         /* entries of blocks [c0, c1) read the maps of blocks [c0 - 1, c1 - 1) */
         const int64_t step = M - 1;
         for (int64_t c0 = 0; c0 < nb; c0 += step) {
             const int64_t c1 = std::min(nb, c0 + step);
             const int64_t m0 = c0 > 0 ? c0 - 1 : 0, m1 = c1 - 1;
-            if (m1 > m0) {
+            if (with_prev && m1 > m0) {
                 int64_t maxs = 0;
                 for (int64_t b = m0; b < m1; ++b) maxs = std::max(maxs, pl->blk_seg_h[b + 1] - pl->blk_seg_h[b]);
                 if (maxs > 0)
@@ -565,12 +584,14 @@ static int plan_build(pct_ctx* ctx, const pct_hist_soa* hist, int B, int mlp_mod
             hipLaunchKernelGGL(k_build_entries, dim3((unsigned)((h1 - h0 + 3) / 4)), dim3(256), 0, ctx->stream,
                                (const uint32_t*)ctx->path_store.ptr, cap_off, (const uint32_t*)len, hist_off, h0, h1,
                                (int64_t)B, (const uint64_t*)pl->segs.ptr, (const int64_t*)pl->blk_seg.ptr, M, V,
-                               (const uint64_t*)map.ptr, (uint64_t*)pl->list.ptr);
+                               (const uint64_t*)map.ptr, (uint32_t*)pl->list.ptr,
+                               with_prev ? (uint64_t*)pl->prev.ptr : nullptr);
         }
         PCT_HIP_CHECK(ctx, hipGetLastError());
         PCT_HIP_CHECK(ctx, hipStreamSynchronize(ctx->stream));
         release(map);
     }
+    pl->has_prev = with_prev;
     release(ctx->path_store);
     /* deltas of two blocks (the one-launch form reads block b-1's while writing
      * block b's), two doubles per history (ROBUST) */
@@ -598,7 +619,8 @@ static bool same_hist(const pct_hist_soa& a, const pct_hist_soa& b) {
  * (and, for the surrogate, the network and calibration it was built with) */
 static bool plan_matches(const pct_ctx* ctx, const pct_plan* pl, const pct_hist_soa* hist, int B, int mlp) {
     return pl && pl->valid && pl->B == B && pl->mlp_mode == mlp && same_hist(pl->key, *hist) &&
-           (mlp != PCT_MLP_SURROGATE || pl->net_version == ctx->net_version);
+           (mlp != PCT_MLP_SURROGATE || pl->net_version == ctx->net_version) &&
+           (ctx->tune_block_form != 1 || pl->has_prev);
 }
 
 int pct_plan_build(pct_ctx* ctx, const pct_hist_soa* hist, int B, int mlp_mode) {
@@ -626,7 +648,8 @@ static int block_form(const pct_ctx* ctx, int variant) {
 static int plan_block_loop(pct_ctx* ctx, pct_plan* pl, const pct_hist_soa* hist, float* x, double lambda, bool f32,
                            int variant, int64_t b_begin, int64_t b_end) {
     const int64_t B = pl->B;
-    const uint64_t* list = (const uint64_t*)pl->list.ptr;
+    const uint32_t* list = (const uint32_t*)pl->list.ptr;
+    const uint64_t* prev = (const uint64_t*)pl->prev.ptr;
     const uint32_t* len = (const uint32_t*)pl->len.ptr;
     const double* chord = (const double*)pl->chord.ptr;
     const uint64_t* segs = (const uint64_t*)pl->segs.ptr;
@@ -638,7 +661,8 @@ static int plan_block_loop(pct_ctx* ctx, pct_plan* pl, const pct_hist_soa* hist,
         const int64_t b_last = b_end == pl->nb ? pl->nb : b_end - 1;
         for (int64_t b = b_begin; b <= b_last; ++b) {
             const int64_t h0 = b * B, cnt = b < pl->nb ? std::min<int64_t>(B, pl->n - h0) : 0;
-            const uint64_t* lb = b < pl->nb ? list + pl->list_base_h[b] : nullptr;
+            const uint32_t* lb = b < pl->nb ? list + pl->list_base_h[b] : nullptr;
+            const uint64_t* pb = b < pl->nb ? prev + pl->list_base_h[b] : nullptr;
             const int stride = b < pl->nb ? pl->stride_h[b] : 0;
             void* dout = (char*)pl->delta.ptr + (size_t)(b & 1) * 2 * B * sizeof(double);
             const void* dprev = (const char*)pl->delta.ptr + (size_t)((b + 1) & 1) * 2 * B * sizeof(double);
@@ -655,11 +679,11 @@ static int plan_block_loop(pct_ctx* ctx, pct_plan* pl, const pct_hist_soa* hist,
             if (na + nd == 0) continue;
             const uint32_t stamp = (uint32_t)(b - 1);
             if (f32)
-                hipLaunchKernelGGL(k_block_fused<true>, dim3(na + nd), dim3(FUSED_THREADS), 0, ctx->stream, lb, stride,
+                hipLaunchKernelGGL(k_block_fused<true>, dim3(na + nd), dim3(FUSED_THREADS), 0, ctx->stream, lb, pb, stride,
                                    len, chord, (const float*)hist->wepl, h0, cnt, lambda, dout, sp, nsp, cp, dprev,
                                    stamp, na, xw);
             else
-                hipLaunchKernelGGL(k_block_fused<false>, dim3(na + nd), dim3(FUSED_THREADS), 0, ctx->stream, lb, stride,
+                hipLaunchKernelGGL(k_block_fused<false>, dim3(na + nd), dim3(FUSED_THREADS), 0, ctx->stream, lb, pb, stride,
                                    len, chord, (const float*)hist->wepl, h0, cnt, lambda, dout, sp, nsp, cp, dprev,
                                    stamp, na, xw);
         }
@@ -669,7 +693,7 @@ static int plan_block_loop(pct_ctx* ctx, pct_plan* pl, const pct_hist_soa* hist,
     void* delta = pl->delta.ptr;
     for (int64_t b = b_begin; b < b_end; ++b) {
         int64_t h0 = b * B, cnt = (pl->n - h0) < B ? (pl->n - h0) : B;
-        const uint64_t* lb = list + pl->list_base_h[b];
+        const uint32_t* lb = list + pl->list_base_h[b];
         int stride = pl->stride_h[b];
         dim3 dgrid((unsigned)((cnt + DELTA_WAVES - 1) / DELTA_WAVES));
         if (variant != 0) {

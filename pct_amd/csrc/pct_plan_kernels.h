@@ -17,10 +17,6 @@
 #include <hip/hip_runtime.h>
 
 #define LIST_SENTINEL 0xFFFFFFFFu
-/* the plan's list store holds one 8-byte entry per path slot: the voxel in
- * bits 0-21, and for a voxel the previous DROP block also touches, that
- * block's segment record of the voxel (the layout below); all ones = unused */
-#define ENT_SENTINEL 0xFFFFFFFFFFFFFFFFull
 /* 8-byte segment record: bits 0-21 voxel, 22-23 inline count (1-3; 0 = the
  * history indices are in csc), then either up to three 13-bit block-local
  * history indices (bits 24-36, 37-49, 50-62) or the block-relative first csc
@@ -62,9 +58,7 @@ __device__ __forceinline__ void kahan_add(float& s, float& c, float v) {
 
 /* history k of the block (one wavefront); list points at the block's first
  * list, history k's list is at k * stride; h0 = first history of the block */
-__device__ __forceinline__ uint32_t ent_voxel(uint64_t e) { return (uint32_t)e & (uint32_t)(SEG_MAX_V - 1); }
-
-__device__ __forceinline__ void delta_f64(const uint64_t* __restrict__ list, int stride, const uint32_t* __restrict__ len,
+__device__ __forceinline__ void delta_f64(const uint32_t* __restrict__ list, int stride, const uint32_t* __restrict__ len,
                                           const double* __restrict__ chord, const float* __restrict__ wepl, int64_t h0,
                                           int64_t k, const float* __restrict__ x, double lambda,
                                           double* __restrict__ delta, int lane) {
@@ -72,21 +66,20 @@ __device__ __forceinline__ void delta_f64(const uint64_t* __restrict__ list, int
     const uint32_t n = len[h];
     const double L = chord[h];
     const float b_i = wepl[h];
-    const uint64_t* p = list + k * stride;
+    const uint32_t* p = list + k * stride;
     uint32_t v[DELTA_SLOTS];
 #pragma unroll
     for (int r = 0; r < DELTA_SLOTS; ++r) {
         int j = lane + 64 * r;
-        const uint64_t e = j < stride ? p[j] : ENT_SENTINEL;
-        v[r] = e == ENT_SENTINEL ? LIST_SENTINEL : ent_voxel(e);
+        v[r] = j < stride ? p[j] : LIST_SENTINEL;
     }
     double s = 0.0;
 #pragma unroll
     for (int r = 0; r < DELTA_SLOTS; ++r)
         if (v[r] != LIST_SENTINEL) s += (double)x[v[r]];
     for (int j = lane + 64 * DELTA_SLOTS; j < stride; j += 64) {
-        const uint64_t e = p[j];
-        if (e != ENT_SENTINEL) s += (double)x[ent_voxel(e)];
+        uint32_t w = p[j];
+        if (w != LIST_SENTINEL) s += (double)x[w];
     }
     s = wave_sum_d(s);
     if (lane == 0) {
@@ -96,7 +89,7 @@ __device__ __forceinline__ void delta_f64(const uint64_t* __restrict__ list, int
     }
 }
 
-__device__ __forceinline__ void delta_f32(const uint64_t* __restrict__ list, int stride, const uint32_t* __restrict__ len,
+__device__ __forceinline__ void delta_f32(const uint32_t* __restrict__ list, int stride, const uint32_t* __restrict__ len,
                                           const double* __restrict__ chord, const float* __restrict__ wepl, int64_t h0,
                                           int64_t k, const float* __restrict__ x, float lambda,
                                           float* __restrict__ delta, int lane) {
@@ -104,21 +97,20 @@ __device__ __forceinline__ void delta_f32(const uint64_t* __restrict__ list, int
     const uint32_t n = len[h];
     const float L = (float)chord[h];
     const float b_i = wepl[h];
-    const uint64_t* p = list + k * stride;
+    const uint32_t* p = list + k * stride;
     uint32_t v[DELTA_SLOTS];
 #pragma unroll
     for (int r = 0; r < DELTA_SLOTS; ++r) {
         int j = lane + 64 * r;
-        const uint64_t e = j < stride ? p[j] : ENT_SENTINEL;
-        v[r] = e == ENT_SENTINEL ? LIST_SENTINEL : ent_voxel(e);
+        v[r] = j < stride ? p[j] : LIST_SENTINEL;
     }
     float s = 0.0f, c = 0.0f;
 #pragma unroll
     for (int r = 0; r < DELTA_SLOTS; ++r)
         if (v[r] != LIST_SENTINEL) kahan_add(s, c, x[v[r]]);
     for (int j = lane + 64 * DELTA_SLOTS; j < stride; j += 64) {
-        const uint64_t e = p[j];
-        if (e != ENT_SENTINEL) kahan_add(s, c, x[ent_voxel(e)]);
+        uint32_t w = p[j];
+        if (w != LIST_SENTINEL) kahan_add(s, c, x[w]);
     }
 #pragma unroll
     for (int d = 32; d >= 1; d >>= 1) {
@@ -145,7 +137,7 @@ __device__ __forceinline__ void delta_f32(const uint64_t* __restrict__ list, int
  *     N_v = sum L^2, and x_v += (N_v / (N_v + psi_v)) sum / count with
  *     psi_v = PSI_SIGN (1 - x_v) ETA, the first product in float.
  * x_v is the value at the block's start in both. */
-__device__ __forceinline__ void delta_variant(int variant, const uint64_t* __restrict__ list, int stride,
+__device__ __forceinline__ void delta_variant(int variant, const uint32_t* __restrict__ list, int stride,
                                               const uint32_t* __restrict__ len, const double* __restrict__ chord,
                                               const float* __restrict__ wepl, int64_t h0, int64_t k,
                                               const float* __restrict__ x, double lambda, double* __restrict__ out,
@@ -154,21 +146,20 @@ __device__ __forceinline__ void delta_variant(int variant, const uint64_t* __res
     const uint32_t n = len[h];
     const double L = chord[h];
     const float b_i = wepl[h];
-    const uint64_t* p = list + k * stride;
+    const uint32_t* p = list + k * stride;
     uint32_t v[DELTA_SLOTS];
 #pragma unroll
     for (int r = 0; r < DELTA_SLOTS; ++r) {
         int j = lane + 64 * r;
-        const uint64_t e = j < stride ? p[j] : ENT_SENTINEL;
-        v[r] = e == ENT_SENTINEL ? LIST_SENTINEL : ent_voxel(e);
+        v[r] = j < stride ? p[j] : LIST_SENTINEL;
     }
     double s = 0.0;
 #pragma unroll
     for (int r = 0; r < DELTA_SLOTS; ++r)
         if (v[r] != LIST_SENTINEL) s += (double)x[v[r]];
     for (int j = lane + 64 * DELTA_SLOTS; j < stride; j += 64) {
-        const uint64_t e = p[j];
-        if (e != ENT_SENTINEL) s += (double)x[ent_voxel(e)];
+        uint32_t w = p[j];
+        if (w != LIST_SENTINEL) s += (double)x[w];
     }
     s = wave_sum_d(s);
     if (lane == 0) {
@@ -319,21 +310,16 @@ __device__ __forceinline__ void apply_f32(uint64_t r, const uint16_t* __restrict
 /* block b reads each path voxel's word: a voxel block b-1 does not touch has */
 /* its final value already; one it touches either carries stamp b-1 (written  */
 /* by this launch) or still its value before block b-1, and then the history  */
-/* applies block b-1's update to it itself, from the segment record its path  */
-/* entry carries -- the same expression as the voxel update, so the value is  */
-/* the same bit for bit. Nothing waits on anything: the launch boundary       */
-/* orders block b-1's deltas before both halves.                              */
+/* applies block b-1's update to it itself, from the segment record stored    */
+/* beside its path slot -- the same expression as the voxel update, so the    */
+/* value is the same bit for bit. Nothing waits on anything: the launch       */
+/* boundary orders block b-1's deltas before both halves.                     */
 /* ------------------------------------------------------------------------- */
 __device__ __forceinline__ uint64_t xw_pack(float v, uint32_t stamp) {
     return ((uint64_t)stamp << 32) | (uint64_t)__float_as_uint(v);
 }
 __device__ __forceinline__ float xw_value(uint64_t w) { return __uint_as_float((uint32_t)w); }
 __device__ __forceinline__ uint32_t xw_stamp(uint64_t w) { return (uint32_t)(w >> 32); }
-
-/* the path entry's voxel touched by the previous block? */
-__device__ __forceinline__ bool ent_has_prev(uint64_t e) {
-    return ((e >> SEG_VOX_BITS) & 3u) != 0u || (e >> 51) != 0u;
-}
 
 __device__ __forceinline__ uint64_t xw_load(const uint64_t* p) {
     return __hip_atomic_load(p, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
@@ -349,40 +335,48 @@ struct fused_prev {
     uint32_t stamp;               /* b - 1                        */
 };
 
-/* x value of block b's path entry e after block b-1 */
+/* x value of path voxel v of block b after block b-1; rec = block b-1's
+ * segment record of v, 0 when block b-1 does not touch v */
 template <bool F32>
-__device__ __forceinline__ float fused_x(uint64_t e, const uint64_t* __restrict__ xw, const fused_prev<F32>& pv) {
-    const uint64_t w = xw_load(xw + ent_voxel(e));
-    if (!ent_has_prev(e) || xw_stamp(w) == pv.stamp) return xw_value(w);
-    if (F32) return applied_f32(xw_value(w), e, pv.csc, (const float*)pv.delta);
-    return applied_f64(xw_value(w), e, pv.csc, (const double*)pv.delta);
+__device__ __forceinline__ float fused_x(uint32_t v, uint64_t rec, const uint64_t* __restrict__ xw,
+                                         const fused_prev<F32>& pv) {
+    const uint64_t w = xw_load(xw + v);
+    if (rec == 0 || xw_stamp(w) == pv.stamp) return xw_value(w);
+    if (F32) return applied_f32(xw_value(w), rec, pv.csc, (const float*)pv.delta);
+    return applied_f64(xw_value(w), rec, pv.csc, (const double*)pv.delta);
 }
 
+/* list / prev: the block's voxel lists and, slot for slot, the previous
+ * block's segment records of those voxels */
 template <bool F32>
-__device__ __forceinline__ void fused_delta(const uint64_t* __restrict__ list, int stride,
-                                            const uint32_t* __restrict__ len, const double* __restrict__ chord,
-                                            const float* __restrict__ wepl, int64_t h0, int64_t k,
-                                            const uint64_t* __restrict__ xw, const fused_prev<F32>& pv,
-                                            double lambda, void* __restrict__ delta, int lane) {
+__device__ __forceinline__ void fused_delta(const uint32_t* __restrict__ list, const uint64_t* __restrict__ prev,
+                                            int stride, const uint32_t* __restrict__ len,
+                                            const double* __restrict__ chord, const float* __restrict__ wepl,
+                                            int64_t h0, int64_t k, const uint64_t* __restrict__ xw,
+                                            const fused_prev<F32>& pv, double lambda, void* __restrict__ delta,
+                                            int lane) {
     const int64_t h = h0 + k;
     const uint32_t n = len[h];
     const double Ld = chord[h];
     const float b_i = wepl[h];
-    const uint64_t* p = list + k * stride;
+    const uint32_t* p = list + k * stride;
+    const uint64_t* pr = prev + k * stride;
+    uint32_t v[DELTA_SLOTS];
     uint64_t e[DELTA_SLOTS];
 #pragma unroll
     for (int r = 0; r < DELTA_SLOTS; ++r) {
         int j = lane + 64 * r;
-        e[r] = j < stride ? p[j] : ENT_SENTINEL;
+        v[r] = j < stride ? p[j] : LIST_SENTINEL;
+        e[r] = j < stride ? pr[j] : 0ull;
     }
     if (F32) {
         float s = 0.0f, c = 0.0f;
 #pragma unroll
         for (int r = 0; r < DELTA_SLOTS; ++r)
-            if (e[r] != ENT_SENTINEL) kahan_add(s, c, fused_x<F32>(e[r], xw, pv));
+            if (v[r] != LIST_SENTINEL) kahan_add(s, c, fused_x<F32>(v[r], e[r], xw, pv));
         for (int j = lane + 64 * DELTA_SLOTS; j < stride; j += 64) {
-            const uint64_t q = p[j];
-            if (q != ENT_SENTINEL) kahan_add(s, c, fused_x<F32>(q, xw, pv));
+            const uint32_t q = p[j];
+            if (q != LIST_SENTINEL) kahan_add(s, c, fused_x<F32>(q, pr[j], xw, pv));
         }
 #pragma unroll
         for (int d = 32; d >= 1; d >>= 1) {
@@ -400,10 +394,10 @@ __device__ __forceinline__ void fused_delta(const uint64_t* __restrict__ list, i
         double s = 0.0;
 #pragma unroll
         for (int r = 0; r < DELTA_SLOTS; ++r)
-            if (e[r] != ENT_SENTINEL) s += (double)fused_x<F32>(e[r], xw, pv);
+            if (v[r] != LIST_SENTINEL) s += (double)fused_x<F32>(v[r], e[r], xw, pv);
         for (int j = lane + 64 * DELTA_SLOTS; j < stride; j += 64) {
-            const uint64_t q = p[j];
-            if (q != ENT_SENTINEL) s += (double)fused_x<F32>(q, xw, pv);
+            const uint32_t q = p[j];
+            if (q != LIST_SENTINEL) s += (double)fused_x<F32>(q, pr[j], xw, pv);
         }
         s = wave_sum_d(s);
         if (lane == 0) {

@@ -106,21 +106,32 @@ def forward(ctx, F):
     return out[:F.shape[0]]
 
 
-def calibrate(ctx, hist, stride=64, tol=0.0, min_samples=8, start=0, count=None):
-    """calibration check on every stride-th history (tol <= 0: measure only);
-    returns a dict of the results"""
+def calibrate(ctx, hist, stride=64, tol=0.0, min_samples=8, start=0, count=None, rule="deviation",
+              threshold=None):
+    """calibration check on every stride-th history; rule "deviation": a cell
+    falls back when its 99th-percentile path deviation exceeds tol cm (tol <= 0:
+    measure only); rule "lists": when the share of its sampled histories whose
+    voxel list differs from the exact walk's exceeds `threshold` (0 = any
+    disagreement; None or < 0: measure only). Returns a dict of the results."""
     count = hist.n - start if count is None else count
     c = PctNetCalib()
     hs = hist.soa()
-    ctx._check(ctx.L.pct_net_calibrate(ctx._ctx, ctypes.byref(hs), start, count, stride, float(tol), int(min_samples),
-                                       ctypes.byref(c)), "pct_net_calibrate")
+    if rule == "deviation":
+        code, thr = _lib.PCT_NET_RULE_DEVIATION, (tol if tol > 0 else -1.0)
+    elif rule == "lists":
+        code, thr = _lib.PCT_NET_RULE_LISTS, (-1.0 if threshold is None else threshold)
+    else:
+        raise ValueError(rule)
+    ctx._check(ctx.L.pct_net_calibrate_rule(ctx._ctx, ctypes.byref(hs), start, count, stride, code, float(thr),
+                                            int(min_samples), ctypes.byref(c)), "pct_net_calibrate_rule")
     return {"samples": c.samples, "list_mismatch": c.list_mismatch, "in_range": c.in_range,
             "list_mismatch_rate": c.list_mismatch / max(c.samples, 1),
             "surrogate": c.surrogate, "surrogate_mismatch": c.surrogate_mismatch,
             "fallback_rate": 1.0 - c.surrogate / max(c.samples, 1),
             "dev_median_cm": c.dev_median, "dev_p99_cm": c.dev_p99, "dev_max_cm": c.dev_max,
             "cells_fallback": c.cells_fallback, "cell_samples": list(c.cell_samples),
-            "cell_p99_cm": [float(v) for v in c.cell_p99], "cell_fallback": list(c.cell_fallback)}
+            "cell_p99_cm": [float(v) for v in c.cell_p99], "cell_fallback": list(c.cell_fallback),
+            "cell_list_mismatch": [float(v) for v in c.cell_list_mismatch]}
 
 
 def fallback_stats(ctx):

@@ -223,3 +223,38 @@ def test_learned_drop_iterations(cfg2, trained):
     assert tot == 4 * hist.n and fb < 0.1 * tot
     assert 0 < rel < 2e-3, rel
     assert np.sqrt(np.mean((r - x) ** 2)) > 10 * np.sqrt(np.mean((g - r) ** 2))
+
+
+def test_list_rule_cells_fall_back_to_exact_lists(cfg2, trained):
+    """the north star's fallback rule: with PCT_NET_RULE_LISTS every cell in
+    which a sampled history's surrogate voxel list disagrees with the exact
+    walk's (threshold 0) goes to the exact walk; the histories of those cells
+    then get the exact walk's lists bit for bit (oracle.mlp_paths), the
+    others the surrogate's; the per-cell disagreement shares are reported"""
+    import oracle
+    import pct_amd
+    from pct_amd import configs, learned
+    ctx, hist = cfg2
+    net, F, _ = trained
+    lo, hi = learned.input_range(F, 0.02)
+    learned.install(ctx, net.params, lo, hi)
+    n = 3000
+    cal = learned.calibrate(ctx, hist, stride=1, start=0, count=n, rule="lists", threshold=0.0, min_samples=1)
+    mis = np.array(cal["cell_list_mismatch"])
+    fbc = np.array(cal["cell_fallback"]).astype(bool)
+    pop = np.array(cal["cell_samples"])
+    print("cells with samples", int((pop > 0).sum()), "falling back", int(fbc.sum()),
+          "list disagreement per cell", np.round(mis[pop > 0], 3))
+    assert np.all(fbc[(pop > 0) & (mis > 0)])            # any disagreement sends the cell back
+    assert not np.any(fbc[(pop > 0) & (mis == 0)])       # ... and only that (min_samples 1)
+    assert cal["surrogate_mismatch"] == 0                 # every kept sample's list is the exact one
+    offs, vox, lens = ctx.mlp_paths(hist, 0, n, mode=pct_amd.PCT_MLP_SURROGATE)
+    fb, tot = learned.fallback_stats(ctx)
+    assert tot == n and fb == cal["samples"] - cal["surrogate"]
+    offs, vox, lens = offs.cpu().numpy(), vox.cpu().numpy().view(np.uint32), lens.cpu().numpy()
+    exact = oracle.mlp_paths(configs.params("cfg2"), _host(hist), 0, n)
+    bad = sum(not np.array_equal(vox[offs[i]:offs[i] + lens[i]], exact[i]) for i in range(n))
+    assert bad == 0, f"{bad} of {n} lists differ from the exact walk's"
+    # the deviation rule keeps these cells (the contrast the bench reports)
+    cal_dev = learned.calibrate(ctx, hist, stride=1, start=0, count=n, tol=3e-3, min_samples=1)
+    assert cal_dev["cells_fallback"] < cal["cells_fallback"]
