@@ -106,6 +106,14 @@ class Histories:
             setattr(h, f, getattr(self, f)[:n])
         return h
 
+    def slice(self, start, n):
+        """histories [start, start + n) as views of these arrays"""
+        h = Histories.__new__(Histories)
+        h.n = n
+        for f in _lib.HIST_FLOAT_FIELDS + ("bin_num", "first_voxel"):
+            setattr(h, f, getattr(self, f)[start:start + n])
+        return h
+
     def numpy(self):
         d = {f: getattr(self, f).cpu().numpy() for f in _lib.HIST_FLOAT_FIELDS}
         d["bin_num"] = self.bin_num.cpu().numpy()
