@@ -33,8 +33,7 @@ struct batch_delta_args {
     uint32_t wg0[PCT_BATCH_MAX + 1];
     const uint32_t* list[PCT_BATCH_MAX];
     int stride[PCT_BATCH_MAX];
-    const uint32_t* len[PCT_BATCH_MAX];
-    const double* chord[PCT_BATCH_MAX];
+    pct_blk::blk_pos pos[PCT_BATCH_MAX];
     const float* wepl[PCT_BATCH_MAX];
     int64_t h0[PCT_BATCH_MAX];
     int64_t count[PCT_BATCH_MAX];
@@ -67,10 +66,10 @@ k_batch_delta(batch_delta_args a) {
     const int64_t k = (int64_t)(blockIdx.x - a.wg0[s]) * DELTA_WAVES + (threadIdx.x >> 6);
     if (k >= a.count[s]) return;
     if (F32)
-        pct_blk::delta_f32(a.list[s], a.stride[s], a.len[s], a.chord[s], a.wepl[s], a.h0[s], k, a.x[s],
-                           (float)a.lambda[s], (float*)a.delta[s], threadIdx.x & 63);
+        pct_blk::delta_f32(a.list[s], a.stride[s], a.pos[s], a.wepl[s], a.h0[s], k, a.x[s], (float)a.lambda[s],
+                           (float*)a.delta[s], threadIdx.x & 63);
     else
-        pct_blk::delta_f64(a.list[s], a.stride[s], a.len[s], a.chord[s], a.wepl[s], a.h0[s], k, a.x[s], a.lambda[s],
+        pct_blk::delta_f64(a.list[s], a.stride[s], a.pos[s], a.wepl[s], a.h0[s], k, a.x[s], a.lambda[s],
                            (double*)a.delta[s], threadIdx.x & 63);
 }
 
@@ -216,8 +215,8 @@ static int batch_steps(pct_batch* b, float* const* xs, const pct_hist_soa* hists
             da.wg0[d] = dwg;
             da.list[d] = (const uint32_t*)pl->list.ptr + pl->list_base_h[j];
             da.stride[d] = pl->stride_h[j];
-            da.len[d] = (const uint32_t*)pl->len.ptr;
-            da.chord[d] = (const double*)pl->chord.ptr;
+            da.pos[d] = pct_blk::blk_pos{(const uint16_t*)pl->pos_k.ptr + h0, (const uint32_t*)pl->pos_len.ptr + h0,
+                                         (const double*)pl->pos_chord.ptr + h0};
             da.wepl[d] = hists[s].wepl;
             da.h0[d] = h0;
             da.count[d] = cnt;

@@ -638,8 +638,9 @@ k_mlp_paths(pct_geo g, pct_hist_soa h, int64_t start, int64_t count, W walk,
         double chord;
         walk.setup(g, h, start + mine, m, chord, mS);
         first = h.first_voxel[start + mine];
-        off = offsets[mine];
-        cap = offsets[mine + 1] - off;
+        /* a verification pass (no list store) needs no capacity */
+        off = voxels ? offsets[mine] : 0;
+        cap = voxels ? offsets[mine + 1] - off : ((int64_t)1 << 40);
         if (chord_out) chord_out[mine] = chord;
         if (mS >= K) atomicOr(status, PCT_STATUS_TABLE);
     }
@@ -854,8 +855,8 @@ k_mlp_paths_sur(pct_geo g, pct_hist_soa h, int64_t start, int64_t count, const d
         if (mine < count) {
             mS = m.S;
             first = h.first_voxel[start + mine];
-            off = offsets[mine];
-            cap = offsets[mine + 1] - off;
+            off = voxels ? offsets[mine] : 0;
+            cap = voxels ? offsets[mine + 1] - off : ((int64_t)1 << 40);
             if (chord_out) chord_out[mine] = m.chord;
             if (mS >= K) atomicOr(status, PCT_STATUS_TABLE);
             sh = sur_setup(m);

@@ -20,14 +20,19 @@ struct pct_plan {
     int64_t S = 0;                 /* segments = sum over blocks of |V_B|   */
     int mlp_mode = 0;              /* MLP mode the lists were built with    */
     int net_version = 0;           /* network + calibration of a surrogate plan */
-    /* path lists, block-strided: history k of block b at
-     * list[list_base[b] + k * stride[b]], unused slots hold the sentinel */
+    /* path lists, block-strided: the history at position q of block b at
+     * list[list_base[b] + q * stride[b]], unused slots hold the sentinel */
     pct_buffer list;               /* u32                                   */
     /* for the one-launch-per-block form only: slot for slot, the previous
      * block's segment record of the voxel (0 if it does not touch it) */
     pct_buffer prev;               /* u64                                   */
     bool has_prev = false;
-    pct_buffer hist_off;           /* int64 per history+1: list offsets     */
+    pct_buffer hist_off;           /* int64 per history+1: row offsets      */
+    /* rows of a block in placement order (lateral bands per XCD): per
+     * position the block-local history index, list length, chord length */
+    pct_buffer pos_k;              /* u16 per history                       */
+    pct_buffer pos_len;            /* u32 per history                       */
+    pct_buffer pos_chord;          /* f64 per history                       */
     pct_buffer len;                /* u32 per history: list length          */
     pct_buffer chord;              /* f64 per history: effective chord      */
     pct_buffer hash;               /* u32 per history: hash of its list     */

@@ -90,14 +90,57 @@ k_block_max_len(const uint32_t* __restrict__ len, int64_t n, int64_t B, int32_t*
     if (threadIdx.x == 0) out[b] = (int32_t)sm[0];
 }
 
-/* hist_off[h] = list_base[b] + (h - b B) stride[b]; hist_off[n] = list size */
-__global__ void k_hist_offsets(const int64_t* __restrict__ list_base, const int32_t* __restrict__ stride, int64_t n,
-                               int64_t B, int64_t nb, int64_t* __restrict__ hist_off) {
-    int64_t h = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
-    if (h > n) return;
-    if (h == n) { hist_off[n] = list_base[nb]; return; }
-    int64_t b = h / B;
-    hist_off[h] = list_base[b] + (h - b * B) * stride[b];
+/* placement key of every history: (block << 32) | the order-preserving bits
+ * of its lateral coordinate t = (m x d) / |d|, m the chord midpoint and d the
+ * chord direction in the xy plane (the distance of the path from the axis,
+ * signed); the placement never changes results */
+__global__ void k_lateral_keys(pct_hist_soa h, int64_t n, int64_t B, uint64_t* __restrict__ keys,
+                               uint32_t* __restrict__ vals) {
+    const int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= n) return;
+    const float dx = h.x_exit[i] - h.x_entry[i], dy = h.y_exit[i] - h.y_entry[i];
+    const float xm = 0.5f * (h.x_entry[i] + h.x_exit[i]), ym = 0.5f * (h.y_entry[i] + h.y_exit[i]);
+    const float d = sqrtf(dx * dx + dy * dy);
+    const float t = d > 0.0f ? (ym * dx - xm * dy) / d : 0.0f;
+    uint32_t u = __float_as_uint(t);
+    u = (u & 0x80000000u) ? ~u : (u | 0x80000000u);
+    keys[i] = ((uint64_t)(i / B) << 32) | u;
+    vals[i] = (uint32_t)i;
+}
+
+/* placement: the histories of each block sorted by (block, t) are split into
+ * 8 contiguous lateral bands whose sizes are the numbers of positions q of the
+ * block with (q / DELTA_WAVES) % 8 == g -- the positions the delta workgroups
+ * of XCD g take (workgroup j runs on XCD j % 8) -- and band g fills those
+ * positions in order. Writes per position the block-local index, the list
+ * length and the chord length, and per history the offset of its row. */
+__global__ void k_positions(const uint32_t* __restrict__ sorted, int64_t n, int64_t B, int64_t nb,
+                            const int64_t* __restrict__ list_base, const int32_t* __restrict__ stride,
+                            const uint32_t* __restrict__ len, const double* __restrict__ chord,
+                            uint16_t* __restrict__ pos_k, uint32_t* __restrict__ pos_len, double* __restrict__ pos_chord,
+                            int64_t* __restrict__ hist_off) {
+    const int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= n) return;
+    const int64_t h = sorted[i];
+    const int64_t b = i / B;
+    const int r = (int)(i - b * B);
+    const int cnt = (int)((n - b * B) < B ? (n - b * B) : B);
+    constexpr int W = DELTA_WAVES, CYC = 8 * DELTA_WAVES;
+    const int full = cnt / CYC, rem = cnt % CYC;
+    int g = 0, c0 = 0;
+    for (; g < 8; ++g) {
+        const int cg = W * full + min(W, max(0, rem - W * g));
+        if (r < c0 + cg) break;
+        c0 += cg;
+    }
+    const int m = r - c0;
+    const int q = CYC * (m / W) + W * g + (m % W);
+    const int64_t at = b * B + q;
+    pos_k[at] = (uint16_t)(h - b * B);
+    pos_len[at] = len[h];
+    pos_chord[at] = chord[h];
+    hist_off[h] = list_base[b] + (int64_t)q * stride[b];
+    if (i == n - 1) hist_off[n] = list_base[nb];
 }
 
 /* grid-stride: a dispatch holds at most 2^32 - 1 work-items, and the list
@@ -221,23 +264,22 @@ __global__ void k_write_segments(const uint32_t* __restrict__ keys, const uint32
  * wave priority: they are latency-bound and share the CUs with the
  * long-running MLP waves of the next chunks. */
 __global__ void __launch_bounds__(64 * DELTA_WAVES)
-k_block_delta(const uint32_t* __restrict__ list, int stride, const uint32_t* __restrict__ len,
-              const double* __restrict__ chord, const float* __restrict__ wepl, int64_t h0, int64_t count,
-              const float* __restrict__ x, double lambda, double* __restrict__ delta) {
+k_block_delta(const uint32_t* __restrict__ list, int stride, pct_blk::blk_pos pos, const float* __restrict__ wepl,
+              int64_t h0, int64_t count, const float* __restrict__ x, double lambda, double* __restrict__ delta) {
     __builtin_amdgcn_s_setprio(3);
-    const int64_t k = (int64_t)blockIdx.x * DELTA_WAVES + (threadIdx.x >> 6);
-    if (k >= count) return;
-    pct_blk::delta_f64(list, stride, len, chord, wepl, h0, k, x, lambda, delta, threadIdx.x & 63);
+    const int64_t q = (int64_t)blockIdx.x * DELTA_WAVES + (threadIdx.x >> 6);
+    if (q >= count) return;
+    pct_blk::delta_f64(list, stride, pos, wepl, h0, q, x, lambda, delta, threadIdx.x & 63);
 }
 
 __global__ void __launch_bounds__(64 * DELTA_WAVES)
-k_block_delta_f32(const uint32_t* __restrict__ list, int stride, const uint32_t* __restrict__ len,
-                  const double* __restrict__ chord, const float* __restrict__ wepl, int64_t h0, int64_t count,
-                  const float* __restrict__ x, float lambda, float* __restrict__ delta) {
+k_block_delta_f32(const uint32_t* __restrict__ list, int stride, pct_blk::blk_pos pos,
+                  const float* __restrict__ wepl, int64_t h0, int64_t count, const float* __restrict__ x,
+                  float lambda, float* __restrict__ delta) {
     __builtin_amdgcn_s_setprio(3);
-    const int64_t k = (int64_t)blockIdx.x * DELTA_WAVES + (threadIdx.x >> 6);
-    if (k >= count) return;
-    pct_blk::delta_f32(list, stride, len, chord, wepl, h0, k, x, lambda, delta, threadIdx.x & 63);
+    const int64_t q = (int64_t)blockIdx.x * DELTA_WAVES + (threadIdx.x >> 6);
+    if (q >= count) return;
+    pct_blk::delta_f32(list, stride, pos, wepl, h0, q, x, lambda, delta, threadIdx.x & 63);
 }
 
 __global__ void __launch_bounds__(APPLY_THREADS)
@@ -261,13 +303,13 @@ k_block_apply_f32(const uint64_t* __restrict__ segs, int64_t s0, int64_t ns, con
 /* the research variants of the FP64 update (pct_plan_kernels.h) */
 template <int V>
 __global__ void __launch_bounds__(64 * DELTA_WAVES)
-k_block_delta_var(const uint32_t* __restrict__ list, int stride, const uint32_t* __restrict__ len,
-                  const double* __restrict__ chord, const float* __restrict__ wepl, int64_t h0, int64_t count,
-                  const float* __restrict__ x, double lambda, double* __restrict__ out, int64_t B) {
+k_block_delta_var(const uint32_t* __restrict__ list, int stride, pct_blk::blk_pos pos,
+                  const float* __restrict__ wepl, int64_t h0, int64_t count, const float* __restrict__ x,
+                  double lambda, double* __restrict__ out, int64_t B) {
     __builtin_amdgcn_s_setprio(3);
-    const int64_t k = (int64_t)blockIdx.x * DELTA_WAVES + (threadIdx.x >> 6);
-    if (k >= count) return;
-    pct_blk::delta_variant(V, list, stride, len, chord, wepl, h0, k, x, lambda, out, B, threadIdx.x & 63);
+    const int64_t q = (int64_t)blockIdx.x * DELTA_WAVES + (threadIdx.x >> 6);
+    if (q >= count) return;
+    pct_blk::delta_variant(V, list, stride, pos, wepl, h0, q, x, lambda, out, B, threadIdx.x & 63);
 }
 
 template <int V>
@@ -288,9 +330,8 @@ k_block_apply_var(const uint64_t* __restrict__ segs, int64_t s0, int64_t ns, con
 #define FUSED_DELTA_WAVES (FUSED_THREADS / 64)
 template <bool F32>
 __global__ void __launch_bounds__(FUSED_THREADS)
-k_block_fused(const uint32_t* __restrict__ list, const uint64_t* __restrict__ prev, int stride,
-              const uint32_t* __restrict__ len,
-              const double* __restrict__ chord, const float* __restrict__ wepl, int64_t h0, int64_t count,
+k_block_fused(const uint32_t* __restrict__ list, const uint64_t* __restrict__ prev, int stride, pct_blk::blk_pos pos,
+              const float* __restrict__ wepl, int64_t h0, int64_t count,
               double lambda, void* __restrict__ delta_out, const uint64_t* __restrict__ segs_p, int64_t ns_p,
               const uint16_t* __restrict__ csc_p, const void* __restrict__ delta_p, uint32_t stamp_p, unsigned na,
               uint64_t* __restrict__ xw) {
@@ -301,10 +342,9 @@ k_block_fused(const uint32_t* __restrict__ list, const uint64_t* __restrict__ pr
         if (k < ns_p) pct_blk::fused_apply<F32>(segs_p[k], xw, pv);
         return;
     }
-    const int64_t k = (int64_t)(blockIdx.x - na) * FUSED_DELTA_WAVES + (threadIdx.x >> 6);
-    if (k >= count) return;
-    pct_blk::fused_delta<F32>(list, prev, stride, len, chord, wepl, h0, k, xw, pv, lambda, delta_out,
-                              threadIdx.x & 63);
+    const int64_t q = (int64_t)(blockIdx.x - na) * FUSED_DELTA_WAVES + (threadIdx.x >> 6);
+    if (q >= count) return;
+    pct_blk::fused_delta<F32>(list, prev, stride, pos, wepl, h0, q, xw, pv, lambda, delta_out, threadIdx.x & 63);
 }
 
 /* the image as {value, stamp} words (stamp: no block yet) and back */
@@ -345,7 +385,8 @@ void pct_plan_destroy(pct_plan* pl) {
     if (!pl) return;
     plan_release_graph(pl);
     pct_buffer* bufs[] = {&pl->list, &pl->prev, &pl->hist_off, &pl->len, &pl->chord, &pl->hash, &pl->segs,
-                          &pl->csc, &pl->blk_seg, &pl->blk_entry, &pl->delta, &pl->xw};
+                          &pl->csc, &pl->blk_seg, &pl->blk_entry, &pl->delta, &pl->xw, &pl->pos_k, &pl->pos_len,
+                          &pl->pos_chord};
     for (pct_buffer* b : bufs) release(*b);
     for (hipEvent_t e : pl->ev_chunk)
         if (e) (void)hipEventDestroy(e);
@@ -420,11 +461,41 @@ static int plan_build(pct_ctx* ctx, const pct_hist_soa* hist, int B, int mlp_mod
                                           hipMemcpyHostToDevice, ctx->stream));
         PCT_HIP_CHECK(ctx, hipMemcpyAsync(tmp_stride.ptr, pl->stride_h.data(), sizeof(int32_t) * nb,
                                           hipMemcpyHostToDevice, ctx->stream));
-        hipLaunchKernelGGL(k_hist_offsets, dim3(pct_grid(n + 1, 256)), dim3(256), 0, ctx->stream,
-                           (const int64_t*)tmp_base.ptr, (const int32_t*)tmp_stride.ptr, n, (int64_t)B, nb,
-                           (int64_t*)pl->hist_off.ptr);
+        /* placement of each block's rows: lateral bands per XCD */
+        if ((rc = pct_ensure(ctx, pl->pos_k, sizeof(uint16_t) * (size_t)n)) ||
+            (rc = pct_ensure(ctx, pl->pos_len, sizeof(uint32_t) * (size_t)n)) ||
+            (rc = pct_ensure(ctx, pl->pos_chord, sizeof(double) * (size_t)n))) {
+            free_tmp();
+            return rc;
+        }
+        pct_buffer kin, kout, vin, vout, stmp;
+        auto free_sort = [&]() { for (pct_buffer* b : {&kin, &kout, &vin, &vout, &stmp}) release(*b); };
+        size_t sort_bytes = 0;
+        PCT_HIP_CHECK(ctx, rocprim::radix_sort_pairs(nullptr, sort_bytes, (uint64_t*)nullptr, (uint64_t*)nullptr,
+                                                     (uint32_t*)nullptr, (uint32_t*)nullptr, (size_t)n, 0, 64,
+                                                     ctx->stream));
+        if ((rc = pct_ensure(ctx, kin, 8 * (size_t)n)) || (rc = pct_ensure(ctx, kout, 8 * (size_t)n)) ||
+            (rc = pct_ensure(ctx, vin, 4 * (size_t)n)) || (rc = pct_ensure(ctx, vout, 4 * (size_t)n)) ||
+            (rc = pct_ensure(ctx, stmp, sort_bytes))) {
+            free_sort();
+            free_tmp();
+            return rc;
+        }
+        hipLaunchKernelGGL(k_lateral_keys, dim3(pct_grid(n, 256)), dim3(256), 0, ctx->stream, *hist, n, (int64_t)B,
+                           (uint64_t*)kin.ptr, (uint32_t*)vin.ptr);
+        size_t sb = stmp.bytes;
+        int endbit = 32 + vbits_for(nb + 1);
+        if (endbit > 64) endbit = 64;
+        PCT_HIP_CHECK(ctx, rocprim::radix_sort_pairs(stmp.ptr, sb, (uint64_t*)kin.ptr, (uint64_t*)kout.ptr,
+                                                     (uint32_t*)vin.ptr, (uint32_t*)vout.ptr, (size_t)n, 0, endbit,
+                                                     ctx->stream));
+        hipLaunchKernelGGL(k_positions, dim3(pct_grid(n, 256)), dim3(256), 0, ctx->stream, (const uint32_t*)vout.ptr,
+                           n, (int64_t)B, nb, (const int64_t*)tmp_base.ptr, (const int32_t*)tmp_stride.ptr,
+                           (const uint32_t*)len, (const double*)pl->chord.ptr, (uint16_t*)pl->pos_k.ptr,
+                           (uint32_t*)pl->pos_len.ptr, (double*)pl->pos_chord.ptr, (int64_t*)pl->hist_off.ptr);
         PCT_HIP_CHECK(ctx, hipGetLastError());
         PCT_HIP_CHECK(ctx, hipStreamSynchronize(ctx->stream));
+        free_sort();
         free_tmp();
     }
     const int64_t* hist_off = (const int64_t*)pl->hist_off.ptr;
@@ -650,8 +721,11 @@ static int plan_block_loop(pct_ctx* ctx, pct_plan* pl, const pct_hist_soa* hist,
     const int64_t B = pl->B;
     const uint32_t* list = (const uint32_t*)pl->list.ptr;
     const uint64_t* prev = (const uint64_t*)pl->prev.ptr;
-    const uint32_t* len = (const uint32_t*)pl->len.ptr;
     const double* chord = (const double*)pl->chord.ptr;
+    auto pos_of = [&](int64_t b) {
+        return pct_blk::blk_pos{(const uint16_t*)pl->pos_k.ptr + b * B, (const uint32_t*)pl->pos_len.ptr + b * B,
+                                (const double*)pl->pos_chord.ptr + b * B};
+    };
     const uint64_t* segs = (const uint64_t*)pl->segs.ptr;
     const uint16_t* csc = (const uint16_t*)pl->csc.ptr;
     if (block_form(ctx, variant) == 1) {
@@ -680,11 +754,11 @@ static int plan_block_loop(pct_ctx* ctx, pct_plan* pl, const pct_hist_soa* hist,
             const uint32_t stamp = (uint32_t)(b - 1);
             if (f32)
                 hipLaunchKernelGGL(k_block_fused<true>, dim3(na + nd), dim3(FUSED_THREADS), 0, ctx->stream, lb, pb, stride,
-                                   len, chord, (const float*)hist->wepl, h0, cnt, lambda, dout, sp, nsp, cp, dprev,
+                                   pos_of(b), (const float*)hist->wepl, h0, cnt, lambda, dout, sp, nsp, cp, dprev,
                                    stamp, na, xw);
             else
                 hipLaunchKernelGGL(k_block_fused<false>, dim3(na + nd), dim3(FUSED_THREADS), 0, ctx->stream, lb, pb, stride,
-                                   len, chord, (const float*)hist->wepl, h0, cnt, lambda, dout, sp, nsp, cp, dprev,
+                                   pos_of(b), (const float*)hist->wepl, h0, cnt, lambda, dout, sp, nsp, cp, dprev,
                                    stamp, na, xw);
         }
         PCT_HIP_CHECK(ctx, hipGetLastError());
@@ -701,11 +775,11 @@ static int plan_block_loop(pct_ctx* ctx, pct_plan* pl, const pct_hist_soa* hist,
             const int psi = ctx->p.psi_sign;
             double* out = (double*)delta;
             if (variant == 1)
-                hipLaunchKernelGGL(k_block_delta_var<1>, dgrid, dim3(64 * DELTA_WAVES), 0, ctx->stream, lb, stride, len,
-                                   chord, (const float*)hist->wepl, h0, cnt, (const float*)x, lambda, out, B);
+                hipLaunchKernelGGL(k_block_delta_var<1>, dgrid, dim3(64 * DELTA_WAVES), 0, ctx->stream, lb, stride,
+                                   pos_of(b), (const float*)hist->wepl, h0, cnt, (const float*)x, lambda, out, B);
             else
-                hipLaunchKernelGGL(k_block_delta_var<2>, dgrid, dim3(64 * DELTA_WAVES), 0, ctx->stream, lb, stride, len,
-                                   chord, (const float*)hist->wepl, h0, cnt, (const float*)x, lambda, out, B);
+                hipLaunchKernelGGL(k_block_delta_var<2>, dgrid, dim3(64 * DELTA_WAVES), 0, ctx->stream, lb, stride,
+                                   pos_of(b), (const float*)hist->wepl, h0, cnt, (const float*)x, lambda, out, B);
             const int64_t s0 = pl->blk_seg_h[b], ns = pl->blk_seg_h[b + 1] - s0;
             const uint16_t* cb = csc + pl->blk_entry_h[b];
             if (ns > 0 && variant == 1)
@@ -717,10 +791,10 @@ static int plan_block_loop(pct_ctx* ctx, pct_plan* pl, const pct_hist_soa* hist,
             continue;
         }
         if (f32)
-            hipLaunchKernelGGL(k_block_delta_f32, dgrid, dim3(64 * DELTA_WAVES), 0, ctx->stream, lb, stride, len, chord,
+            hipLaunchKernelGGL(k_block_delta_f32, dgrid, dim3(64 * DELTA_WAVES), 0, ctx->stream, lb, stride, pos_of(b),
                                (const float*)hist->wepl, h0, cnt, (const float*)x, (float)lambda, (float*)delta);
         else
-            hipLaunchKernelGGL(k_block_delta, dgrid, dim3(64 * DELTA_WAVES), 0, ctx->stream, lb, stride, len, chord,
+            hipLaunchKernelGGL(k_block_delta, dgrid, dim3(64 * DELTA_WAVES), 0, ctx->stream, lb, stride, pos_of(b),
                                (const float*)hist->wepl, h0, cnt, (const float*)x, lambda, (double*)delta);
         int64_t s0 = pl->blk_seg_h[b], ns = pl->blk_seg_h[b + 1] - s0;
         const uint16_t* cb = csc + pl->blk_entry_h[b];

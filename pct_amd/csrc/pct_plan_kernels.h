@@ -56,17 +56,27 @@ __device__ __forceinline__ void kahan_add(float& s, float& c, float v) {
     s = t;
 }
 
-/* history k of the block (one wavefront); list points at the block's first
- * list, history k's list is at k * stride; h0 = first history of the block */
-__device__ __forceinline__ void delta_f64(const uint32_t* __restrict__ list, int stride, const uint32_t* __restrict__ len,
-                                          const double* __restrict__ chord, const float* __restrict__ wepl, int64_t h0,
-                                          int64_t k, const float* __restrict__ x, double lambda,
-                                          double* __restrict__ delta, int lane) {
-    const int64_t h = h0 + k;
-    const uint32_t n = len[h];
-    const double L = chord[h];
-    const float b_i = wepl[h];
-    const uint32_t* p = list + k * stride;
+/* The block's histories in placement order: position q's list is row q of the
+ * block's rows (list + q * stride), and pos.k[q], pos.len[q], pos.chord[q]
+ * are that history's block-local index, list length and chord length; the
+ * block's first history is h0. Rows are placed so that the workgroups on one
+ * XCD get one lateral band of the block's paths (the plan build); results do
+ * not depend on the placement, every history's delta goes to delta[k]. */
+struct blk_pos {
+    const uint16_t* k;
+    const uint32_t* len;
+    const double* chord;
+};
+
+/* the history at position q of the block (one wavefront) */
+__device__ __forceinline__ void delta_f64(const uint32_t* __restrict__ list, int stride, blk_pos pos,
+                                          const float* __restrict__ wepl, int64_t h0, int64_t q,
+                                          const float* __restrict__ x, double lambda, double* __restrict__ delta,
+                                          int lane) {
+    const uint32_t k = pos.k[q];
+    const uint32_t n = pos.len[q];
+    const double L = pos.chord[q];
+    const uint32_t* p = list + q * stride;
     uint32_t v[DELTA_SLOTS];
 #pragma unroll
     for (int r = 0; r < DELTA_SLOTS; ++r) {
@@ -81,6 +91,7 @@ __device__ __forceinline__ void delta_f64(const uint32_t* __restrict__ list, int
         uint32_t w = p[j];
         if (w != LIST_SENTINEL) s += (double)x[w];
     }
+    const float b_i = wepl[h0 + k];
     s = wave_sum_d(s);
     if (lane == 0) {
         const double a_i_dot_x_k = s * L;
@@ -89,15 +100,14 @@ __device__ __forceinline__ void delta_f64(const uint32_t* __restrict__ list, int
     }
 }
 
-__device__ __forceinline__ void delta_f32(const uint32_t* __restrict__ list, int stride, const uint32_t* __restrict__ len,
-                                          const double* __restrict__ chord, const float* __restrict__ wepl, int64_t h0,
-                                          int64_t k, const float* __restrict__ x, float lambda,
-                                          float* __restrict__ delta, int lane) {
-    const int64_t h = h0 + k;
-    const uint32_t n = len[h];
-    const float L = (float)chord[h];
-    const float b_i = wepl[h];
-    const uint32_t* p = list + k * stride;
+__device__ __forceinline__ void delta_f32(const uint32_t* __restrict__ list, int stride, blk_pos pos,
+                                          const float* __restrict__ wepl, int64_t h0, int64_t q,
+                                          const float* __restrict__ x, float lambda, float* __restrict__ delta,
+                                          int lane) {
+    const uint32_t k = pos.k[q];
+    const uint32_t n = pos.len[q];
+    const float L = (float)pos.chord[q];
+    const uint32_t* p = list + q * stride;
     uint32_t v[DELTA_SLOTS];
 #pragma unroll
     for (int r = 0; r < DELTA_SLOTS; ++r) {
@@ -118,6 +128,7 @@ __device__ __forceinline__ void delta_f32(const uint32_t* __restrict__ list, int
         c += oc;
         kahan_add(s, c, os);
     }
+    const float b_i = wepl[h0 + k];
     if (lane == 0) {
         const float a_i_dot_x_k = s * L;
         const float a_i_dot_a_i = (L * L) * (float)n;
@@ -138,15 +149,13 @@ __device__ __forceinline__ void delta_f32(const uint32_t* __restrict__ list, int
  *     psi_v = PSI_SIGN (1 - x_v) ETA, the first product in float.
  * x_v is the value at the block's start in both. */
 __device__ __forceinline__ void delta_variant(int variant, const uint32_t* __restrict__ list, int stride,
-                                              const uint32_t* __restrict__ len, const double* __restrict__ chord,
-                                              const float* __restrict__ wepl, int64_t h0, int64_t k,
+                                              blk_pos pos, const float* __restrict__ wepl, int64_t h0, int64_t q,
                                               const float* __restrict__ x, double lambda, double* __restrict__ out,
                                               int64_t B, int lane) {
-    const int64_t h = h0 + k;
-    const uint32_t n = len[h];
-    const double L = chord[h];
-    const float b_i = wepl[h];
-    const uint32_t* p = list + k * stride;
+    const uint32_t k = pos.k[q];
+    const uint32_t n = pos.len[q];
+    const double L = pos.chord[q];
+    const uint32_t* p = list + q * stride;
     uint32_t v[DELTA_SLOTS];
 #pragma unroll
     for (int r = 0; r < DELTA_SLOTS; ++r) {
@@ -161,6 +170,7 @@ __device__ __forceinline__ void delta_variant(int variant, const uint32_t* __res
         uint32_t w = p[j];
         if (w != LIST_SENTINEL) s += (double)x[w];
     }
+    const float b_i = wepl[h0 + k];
     s = wave_sum_d(s);
     if (lane == 0) {
         const double a_i_dot_x_k = L * s;
@@ -350,17 +360,14 @@ __device__ __forceinline__ float fused_x(uint32_t v, uint64_t rec, const uint64_
  * block's segment records of those voxels */
 template <bool F32>
 __device__ __forceinline__ void fused_delta(const uint32_t* __restrict__ list, const uint64_t* __restrict__ prev,
-                                            int stride, const uint32_t* __restrict__ len,
-                                            const double* __restrict__ chord, const float* __restrict__ wepl,
-                                            int64_t h0, int64_t k, const uint64_t* __restrict__ xw,
-                                            const fused_prev<F32>& pv, double lambda, void* __restrict__ delta,
-                                            int lane) {
-    const int64_t h = h0 + k;
-    const uint32_t n = len[h];
-    const double Ld = chord[h];
-    const float b_i = wepl[h];
-    const uint32_t* p = list + k * stride;
-    const uint64_t* pr = prev + k * stride;
+                                            int stride, blk_pos pos, const float* __restrict__ wepl, int64_t h0,
+                                            int64_t q, const uint64_t* __restrict__ xw, const fused_prev<F32>& pv,
+                                            double lambda, void* __restrict__ delta, int lane) {
+    const uint32_t k = pos.k[q];
+    const uint32_t n = pos.len[q];
+    const double Ld = pos.chord[q];
+    const uint32_t* p = list + q * stride;
+    const uint64_t* pr = prev + q * stride;
     uint32_t v[DELTA_SLOTS];
     uint64_t e[DELTA_SLOTS];
 #pragma unroll
@@ -384,6 +391,7 @@ __device__ __forceinline__ void fused_delta(const uint32_t* __restrict__ list, c
             c += oc;
             kahan_add(s, c, os);
         }
+        const float b_i = wepl[h0 + k];
         if (lane == 0) {
             const float L = (float)Ld;
             const float a_i_dot_x_k = s * L;
@@ -399,6 +407,7 @@ __device__ __forceinline__ void fused_delta(const uint32_t* __restrict__ list, c
             const uint32_t q = p[j];
             if (q != LIST_SENTINEL) s += (double)fused_x<F32>(q, pr[j], xw, pv);
         }
+        const float b_i = wepl[h0 + k];
         s = wave_sum_d(s);
         if (lane == 0) {
             const double a_i_dot_x_k = s * Ld;
