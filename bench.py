@@ -123,7 +123,7 @@ def _rel_errors(x, refs, x0_np):
     return out
 
 
-def learned_form(ctx, hist, x0, ref, B, p, ms_fp64, args, timed, log):
+def learned_form(ctx, hist, x0, ref, B, p, ms_fp64, args, timed, log, oracle_imgs=None):
     """The learned-path form on this scan: training set (10 M exact-MLP paths
     of 8 random-ellipsoid phantom scans of other grids), training with the
     library's kernels, calibration check on 1 in 64 histories (cells whose
@@ -162,59 +162,154 @@ def learned_form(ctx, hist, x0, ref, B, p, ms_fp64, args, timed, log):
     del F, Y
     log(f"learned: training set {t_data:.1f}s, training {t_train:.1f}s, loss {net.epoch_loss[0]:.3e} -> "
         f"{net.epoch_loss[-1]:.3e}")
-    res.update(evaluate_learned(ctx, hist, x0, ref, B, p, net, timed, args.steps, log))
+    res.update(evaluate_learned(ctx, hist, x0, ref, B, p, net, timed, args.steps, log, oracle_imgs))
     res["speedup_vs_fp64_form"] = round(ms_fp64 / res["ms_per_iteration"], 3)
+    res["lists_rule"]["speedup_vs_fp64_form"] = round(ms_fp64 / res["lists_rule"]["ms_per_iteration"], 3)
+    if oracle_imgs and "error_vs_cpu_oracle" in res:
+        vs_oracle = res.pop("error_vs_cpu_oracle")
+        res["error_vs_cpu_oracle"] = vs_oracle
     return res, net
 
 
-def evaluate_learned(ctx, hist, x0, ref, B, p, net, timed, steps, log):
+def cfg1_pipeline(log, threads):
+    """The plain reference pipeline of the 100x100x16 / 2 M-history
+    configuration as the reference runs it on the CPU -- the oracle's
+    preprocessing (intersections, binning, 3-sigma cuts, FBP, MSC hull,
+    endpoints) and 12 DROP iterations (MLP + DROP_blocks/DROP_update) on
+    `threads` host cores -- timed, with the same pipeline on the GPU beside it
+    and the two images compared."""
     import torch
+    import oracle
+    import pct_amd
+    from pct_amd import configs, pipeline
+    p = configs.params("cfg1")
+    cfg = configs.synth_cfg("cfg1")
+    iters = p.iterations
+    t0 = time.perf_counter()
+    raw = pct_amd.generate_host(cfg).numpy()
+    t_gen = time.perf_counter() - t0
+    t0 = time.perf_counter()
+    h, x0, _, _ = oracle.preprocess(p, raw)
+    t_pre = time.perf_counter() - t0
+    del raw
+    t0 = time.perf_counter()
+    x_cpu = oracle.drop(p, h, x0, iterations=iters, threads=threads)
+    t_drop = time.perf_counter() - t0
+    log(f"cfg1 on the CPU ({threads} threads): preprocess {t_pre:.1f} s, {iters} iterations {t_drop:.1f} s")
+    ctx = pct_amd.Context(p)
+    torch.cuda.synchronize()
+    t0 = time.perf_counter()
+    pre = pipeline.preprocess(ctx, ctx.generate(cfg))
+    torch.cuda.synchronize()
+    g_pre = time.perf_counter() - t0
+    B = p.drop_block_size
+    x = pre.x0.clone()
+    t0 = time.perf_counter()
+    ctx.drop_prepare(pre.hist, B)
+    ctx.sync()
+    g_plan = time.perf_counter() - t0
+    e0, e1 = torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)
+    e0.record(ctx.stream)
+    for _ in range(iters):
+        ctx.drop_iteration(pre.hist, x, B, p.lambda_)
+    e1.record(ctx.stream)
+    torch.cuda.synchronize()
+    g_ms = e0.elapsed_time(e1)
+    xg = x.cpu().numpy()
+    inside = x0 != 0
+    d = xg.astype(np.float64) - x_cpu.astype(np.float64)
+    res = {"workload": f"cfg1: {p.columns}x{p.rows}x{p.slices} voxels, {cfg.n_histories} histories, "
+                       f"{iters} DROP iterations, block {B}, lambda {p.lambda_:g}",
+           "reconstruction_histories": int(h.n), "gpu_reconstruction_histories": int(pre.hist.n),
+           "cpu": {"threads": threads, "host_nproc": os.cpu_count(), "cpu_model": _cpu_model(),
+                   "generate_s": round(t_gen, 2), "preprocess_s": round(t_pre, 2), "drop_s": round(t_drop, 2),
+                   "histories_per_s_per_iteration": round(h.n * iters / t_drop, 1)},
+           "gpu": {"preprocess_s": round(g_pre, 3), "plan_s": round(g_plan, 3),
+                   "drop_ms_per_iteration": round(g_ms / iters, 3),
+                   "histories_per_s_per_iteration": round(pre.hist.n * iters / (g_ms / 1e3), 1)},
+           "image_gpu_vs_cpu": {"rmse": float(np.sqrt(np.mean(d[inside] ** 2))),
+                                "rmse_rel": float(np.sqrt(np.mean(d[inside] ** 2)) / np.mean(np.abs(x_cpu[inside]))),
+                                "max_abs": float(np.max(np.abs(d))),
+                                "identical_voxels": float(np.mean(xg == x_cpu))}}
+    log(f"cfg1: GPU {g_ms / iters:.2f} ms/iteration; image vs CPU {res['image_gpu_vs_cpu']}")
+    ctx.close()
+    return res
+
+
+def evaluate_learned(ctx, hist, x0, ref, B, p, net, timed, steps, log, oracle_imgs=None):
+    """The installed network under both fallback rules of the calibration check
+    (1 history in 64): "deviation" -- a cell whose 99th-percentile path
+    deviation from the exact MLP exceeds 3 % of a voxel width takes the exact
+    walk; "lists" (the north star's rule) -- a cell in which any sampled
+    history's voxel list differs from the exact walk's does. Per rule: the
+    agreement with the exact walk over every history, timed iterations and
+    the RSP error against the exact FP64 form."""
     import pct_amd
     from pct_amd import learned
     res = {}
-    net.install(ctx)
     vw = 2 * p.recon_cyl_radius / p.columns
     tol = 0.03 * vw
-    cal = learned.calibrate(ctx, hist, stride=64, tol=tol)
-    full = learned.calibrate(ctx, hist, stride=1)
-    res["calibration"] = {"tol_cm": tol, "samples": cal["samples"], "cells_fallback": cal["cells_fallback"],
-                          "fallback_rate": round(cal["fallback_rate"], 5)}
-    res["vs_exact_walk_all_histories"] = {
-        "histories": full["samples"], "in_trained_range": full["in_range"],
-        "surrogate_histories": full["surrogate"], "fallback_rate": round(full["fallback_rate"], 5),
-        "list_mismatch_of_surrogate_histories": round(full["surrogate_mismatch"] / max(full["surrogate"], 1), 4),
-        "path_deviation_um": {"median": round(full["dev_median_cm"] * 1e4, 3),
-                              "p99": round(full["dev_p99_cm"] * 1e4, 3), "max": round(full["dev_max_cm"] * 1e4, 3)}}
-    log(f"learned: {res['vs_exact_walk_all_histories']}")
-    t0 = time.time()
-    ctx.drop_prepare(hist, B, pct_amd.PCT_MLP_SURROGATE)
-    ctx.sync()
-    res["plan_setup_once_ms"] = round((time.time() - t0) * 1e3, 1)
-    xs = x0.clone()
-    ctx.drop_iteration(hist, xs, B, p.lambda_, pct_amd.PCT_LEARNED)
-    learned.fallback_stats(ctx)
-    ms, ph = timed(pct_amd.PCT_LEARNED, steps, xs)
-    ms /= steps
-    fb, tot = learned.fallback_stats(ctx)
-    _, mlp_ms, blk_ms = ph.mean(axis=0)
-    pct_amd.set_tuning(ctx, overlap=2)
-    _, ph_seq = timed(pct_amd.PCT_LEARNED, 1, xs)
-    pct_amd.set_tuning(ctx, overlap=0)
-    n = hist.n
-    res.update({"histories_per_s": round(n / (ms / 1e3), 1), "ms_per_iteration": round(ms, 3),
-                "phases_ms": {"mlp_span": round(float(mlp_ms), 3), "block_loop_span": round(float(blk_ms), 3),
-                              "surrogate_mlp_alone": round(float(ph_seq[0][0]), 3)},
-                "fallback_rate_timed_iterations": round(fb / max(tot, 1), 5)})
-    if ref:
-        xl = x0.clone()
-        snaps = {}
-        for it in range(1, max(ref) + 1):
-            ctx.drop_iteration(hist, xl, B, p.lambda_, pct_amd.PCT_LEARNED)
-            if it in ref:
-                snaps[it] = xl.cpu().numpy()
-        res["error_vs_fp64"] = _rel_errors(snaps, ref, x0.cpu().numpy())
-        res["error_region"] = "voxels of the hull (x0 != 0); reference image: exact FP64 form"
-    log(f"learned form: {ms:.2f} ms/iteration, fallback {fb / max(tot, 1):.4f}, errors {res.get('error_vs_fp64')}")
+    for rule in ("deviation", "lists"):
+        r = {}
+        net.install(ctx)
+        if rule == "deviation":
+            cal = learned.calibrate(ctx, hist, stride=64, tol=tol)
+            r["calibration"] = {"rule": f"cell p99 path deviation > {tol:.4g} cm", "samples": cal["samples"]}
+        else:
+            cal = learned.calibrate(ctx, hist, stride=64, rule="lists", threshold=0.0)
+            r["calibration"] = {"rule": "any sampled voxel list of the cell differs from the exact walk's",
+                                "samples": cal["samples"],
+                                "cell_list_disagreement": [round(v, 4) for v, c in
+                                                           zip(cal["cell_list_mismatch"], cal["cell_samples"]) if c]}
+        r["calibration"].update({"cells_fallback": cal["cells_fallback"],
+                                 "fallback_rate": round(cal["fallback_rate"], 5)})
+        full = learned.calibrate(ctx, hist, stride=1)
+        r["vs_exact_walk_all_histories"] = {
+            "histories": full["samples"], "in_trained_range": full["in_range"],
+            "surrogate_histories": full["surrogate"], "fallback_rate": round(full["fallback_rate"], 5),
+            "list_mismatch_of_surrogate_histories": round(full["surrogate_mismatch"] / max(full["surrogate"], 1), 4),
+            "list_mismatch_of_all_histories": round(full["surrogate_mismatch"] / max(full["samples"], 1), 4),
+            "path_deviation_um": {"median": round(full["dev_median_cm"] * 1e4, 3),
+                                  "p99": round(full["dev_p99_cm"] * 1e4, 3), "max": round(full["dev_max_cm"] * 1e4, 3)}}
+        log(f"learned ({rule} rule): {r['vs_exact_walk_all_histories']}")
+        t0 = time.time()
+        ctx.drop_prepare(hist, B, pct_amd.PCT_MLP_SURROGATE)
+        ctx.sync()
+        r["plan_setup_once_ms"] = round((time.time() - t0) * 1e3, 1)
+        xs = x0.clone()
+        ctx.drop_iteration(hist, xs, B, p.lambda_, pct_amd.PCT_LEARNED)
+        learned.fallback_stats(ctx)
+        ms, ph = timed(pct_amd.PCT_LEARNED, steps, xs)
+        ms /= steps
+        fb, tot = learned.fallback_stats(ctx)
+        _, mlp_ms, blk_ms = ph.mean(axis=0)
+        pct_amd.set_tuning(ctx, overlap=2)
+        _, ph_seq = timed(pct_amd.PCT_LEARNED, 1, xs)
+        pct_amd.set_tuning(ctx, overlap=0)
+        n = hist.n
+        r.update({"histories_per_s": round(n / (ms / 1e3), 1), "ms_per_iteration": round(ms, 3),
+                  "phases_ms": {"mlp_span": round(float(mlp_ms), 3), "block_loop_span": round(float(blk_ms), 3),
+                                "surrogate_mlp_alone": round(float(ph_seq[0][0]), 3)},
+                  "fallback_rate_timed_iterations": round(fb / max(tot, 1), 5)})
+        if ref:
+            xl = x0.clone()
+            snaps = {}
+            for it in range(1, max(ref) + 1):
+                ctx.drop_iteration(hist, xl, B, p.lambda_, pct_amd.PCT_LEARNED)
+                if it in ref:
+                    snaps[it] = xl.cpu().numpy()
+            r["error_vs_fp64"] = _rel_errors(snaps, ref, x0.cpu().numpy())
+            if oracle_imgs:
+                r["error_vs_cpu_oracle"] = _rel_errors(snaps, oracle_imgs, x0.cpu().numpy())
+        log(f"learned form ({rule} rule): {ms:.2f} ms/iteration, fallback {fb / max(tot, 1):.4f}, "
+            f"errors {r.get('error_vs_fp64')}")
+        if rule == "deviation":
+            res.update(r)        # the form as timed by default
+        else:
+            res["lists_rule"] = r
+    res["error_region"] = "voxels of the hull (x0 != 0); reference image: exact FP64 form"
+    res["rules"] = ("top level: the deviation rule; lists_rule: the same network with the list rule "
+                    "(fall back wherever a sampled voxel list disagrees with the exact walk)")
     return res
 
 
@@ -342,10 +437,19 @@ def main():
     ap.add_argument("--no-research", action="store_true", help="skip the research update rules and the SM hull")
     ap.add_argument("--quick", action="store_true",
                     help="only the timed iterations (for counter runs, where every dispatch is slow)")
-    ap.add_argument("--form", default="default", choices=["default", "cached", "sequential", "precision", "learned"],
+    ap.add_argument("--form", default="default",
+                    choices=["default", "cached", "sequential", "precision", "learned", "walks"],
                     help="with --quick: the form of the timed iterations (default: MLP beside the block loop; "
                          "cached: the block loop alone; sequential: MLP then block loop; precision; learned: "
-                         "network trained first on 10 M exact-MLP paths)")
+                         "network trained first on 10 M exact-MLP paths; walks: no iteration, the exact, "
+                         "precision and surrogate MLP walks once over all histories, one launch each, the "
+                         "surrogate with an untrained network over the whole input range)")
+    ap.add_argument("--block-form", type=int, default=0, choices=[0, 1, 2],
+                    help="block loop: 1 one launch per block, 2 two launches per block (0: the default, 2)")
+    ap.add_argument("--no-cfg1", action="store_true", help="skip the 100x100x16 / 2 M CPU-and-GPU pipeline")
+    ap.add_argument("--oracle-images", default=None,
+                    help="directory of the CPU oracle's images of this scan after iterations 1, 6, 12 "
+                         "(python -m oracle.full_image): every form's RSP error against them")
     ap.add_argument("--copy-calibration", action="store_true",
                     help="5 device copies of 256 MiB and nothing else: the reference kernel for the HBM counters")
     args = ap.parse_args()
@@ -421,6 +525,24 @@ def main():
     x = pre.x0.clone()
     x0_np = pre.x0.cpu().numpy()
     B = p.drop_block_size
+    if args.block_form:
+        pct_amd.set_tuning(ctx, block_form=args.block_form)
+    if args.quick and args.form == "walks":
+        from pct_amd import learned
+        learned.install(ctx, learned.init_params(1, dev), [-float("inf")] * 12, [float("inf")] * 12)
+        res = {}
+        for name, mode in (("exact", pct_amd.PCT_MLP_FP64), ("precision", pct_amd.PCT_MLP_TABLE),
+                           ("surrogate", pct_amd.PCT_MLP_SURROGATE)):
+            torch.cuda.synchronize()
+            e0, e1 = torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)
+            e0.record(ctx.stream)
+            offs, vox, lens = ctx.mlp_paths(hist, mode=mode)
+            e1.record(ctx.stream)
+            torch.cuda.synchronize()
+            res[name] = {"ms": round(e0.elapsed_time(e1), 3), "voxels": int(lens.to(torch.int64).sum())}
+            del offs, vox, lens
+        print(json.dumps({"quick": True, "form": "walks", "histories": n_r, "walks": res}), flush=True)
+        return
     t1 = time.time()
     ctx.drop_prepare(hist, B)
     ctx.sync()
@@ -496,6 +618,12 @@ def main():
     log(f"block loop alone {ms_cached:.2f} ms, MLP alone {mlp_alone_ms:.2f} ms, sequential iteration {ms_seq:.2f} ms, "
         f"reference-structure (atomic) iteration {ms_atomic:.2f} ms")
 
+    oracle_imgs = None
+    if args.oracle_images:
+        oracle_imgs = {it: np.load(os.path.join(args.oracle_images, f"x{it}.npy")) for it in (1, 6, 12)}
+        assert np.array_equal(np.load(os.path.join(args.oracle_images, "x0.npy")), x0_np), "oracle x0 differs"
+    vs_oracle = {}
+
     # ---- precision form: tabulated MLP + FP32 stepping + FP32/Kahan update ----
     # RSP error vs the exact FP64 form after iterations 1, 6 and 12 from x0 (the
     # exact form equals the CPU oracle bit for bit, see rmse_vs_oracle below)
@@ -517,15 +645,21 @@ def main():
     ms_prec /= args.steps
     _, mlp_ms_p, blk_ms_p = ph_p.mean(axis=0)
     in_hull = x0_np != 0
+    if oracle_imgs:
+        vs_oracle["fp64"] = _rel_errors(ref, oracle_imgs, x0_np)
     prec_err = {}
+    prec_snaps = {}
     xp = pre.x0.clone()
     for it in range(1, snaps[-1] + 1):
         ctx.drop_iteration(hist, xp, B, p.lambda_, pct_amd.PCT_PRECISION)
         if it in snaps:
+            prec_snaps[it] = xp.cpu().numpy()
             d = xp.cpu().numpy().astype(np.float64) - ref[it].astype(np.float64)
             r = float(np.sqrt(np.mean(d[in_hull] ** 2)))
             prec_err[f"iteration_{it}"] = {"rmse": r, "rmse_rel": r / float(np.mean(np.abs(ref[it][in_hull]))),
                                            "max_abs": float(np.max(np.abs(d)))}
+    if oracle_imgs:
+        vs_oracle["precision"] = _rel_errors(prec_snaps, oracle_imgs, x0_np)
     log(f"precision form: {ms_prec:.2f} ms/iter (MLP {mlp_ms_p:.2f} ms, blocks {blk_ms_p:.2f} ms), error {prec_err}")
     precision = {"histories_per_s": round(n_r / (ms_prec / 1e3), 1), "ms_per_iteration": round(ms_prec, 3),
                  "phases_ms": {"mlp": round(float(mlp_ms_p), 3), "blocks": round(float(blk_ms_p), 3),
@@ -549,14 +683,17 @@ def main():
     learned_res, net = None, None
     if not args.no_learned:
         try:
-            learned_res, net = learned_form(ctx, hist, pre.x0, ref, B, p, ms_per_step, args, timed, log)
+            learned_res, net = learned_form(ctx, hist, pre.x0, ref, B, p, ms_per_step, args, timed, log, oracle_imgs)
+            if oracle_imgs and "error_vs_cpu_oracle" in learned_res:
+                vs_oracle["learned"] = learned_res["error_vs_cpu_oracle"]
         except Exception as e:
             learned_res = {"error": f"{type(e).__name__}: {e}"}
             log(f"learned form failed: {e}")
 
-    # algorithmic work per iteration: 48 B per history record, 4 B gather + 4 B
-    # scatter per path entry, 24 B per voxel touched by a block
-    bytes_update = 48.0 * n_r + 8.0 * sum_n + 24.0 * sum_vb
+    # algorithmic work per iteration: 48 B per history record, per path entry
+    # the 4 B list index streamed, a 4 B x gather and a 4 B scatter, 24 B per
+    # voxel touched by a block
+    bytes_update = 48.0 * n_r + 12.0 * sum_n + 24.0 * sum_vb
     nb = (n_r + B - 1) // B
     upd_gbs = bytes_update / (blk_alone_ms / 1e3) / 1e9
     mlp_tflops = MLP_FLOPS_PER_STEP * float(sum_steps) / (mlp_alone_ms / 1e3) / 1e12
@@ -565,7 +702,7 @@ def main():
                    "kernel": "k_block_delta+k_block_apply, one pair per DROP block (hipGraph replay)",
                    "us_per_block": round(blk_alone_ms * 1e3 / nb, 3), "bytes_alg_per_block": round(bytes_update / nb),
                    "ms_per_iteration": round(float(blk_alone_ms), 3), "bytes_alg": bytes_update}
-    roof_mlp = {"bound": "mfma", "achieved": round(mlp_tflops, 3), "peak": FP64_PEAK_TFLOPS, "unit": "TFLOP/s",
+    roof_mlp = {"bound": "fp64-valu", "achieved": round(mlp_tflops, 3), "peak": FP64_PEAK_TFLOPS, "unit": "TFLOP/s",
                 "frac": round(mlp_tflops / FP64_PEAK_TFLOPS, 4), "traffic": None,
                 "kernel": "k_mlp_paths<exact_walk> (FP64 VALU; peak = MI355X FP64 78.6 TFLOP/s)",
                 "ms_per_iteration": round(mlp_alone_ms, 3), "flops_alg": MLP_FLOPS_PER_STEP * float(sum_steps)}
@@ -602,12 +739,15 @@ def main():
                "sample": f"first {m} reconstruction histories ({m // B} DROP blocks of {B}), one iteration from the "
                          f"same x0, FP64 oracle (MLP + DROP_blocks/DROP_update), single thread, {cpu_t:.1f} s",
                "cpu_model": _cpu_model()}
-        # the same sample with the per-block MLP work spread over the host cores
+        # the same sample with the per-block MLP work spread over the host
+        # cores this GPU's share of the host allows (16 per GPU on the MI355X
+        # boxes, whatever nproc shows)
         threads = min(16, os.cpu_count() or 1)
         t = time.perf_counter()
         x_mt = oracle.drop(p, hs, x0_np, iterations=1, threads=threads)
         mt_t = time.perf_counter() - t
-        cpu["multithreaded"] = {"value": round(m / mt_t, 1), "cores": threads, "seconds": round(mt_t, 2),
+        cpu["multithreaded"] = {"value": round(m / mt_t, 1), "cores": threads, "host_nproc": os.cpu_count(),
+                                "seconds": round(mt_t, 2),
                                 "identical_to_single_thread": bool(np.array_equal(x_mt, x_cpu))}
         # the GPU on the same sample from the same x0
         x_g = torch.from_numpy(x0_np).to(dev)
@@ -639,6 +779,14 @@ def main():
         except Exception as e:
             learned_res["large"] = {"error": f"{type(e).__name__}: {e}"}
             log(f"learned large case failed: {e}")
+
+    cfg1 = None
+    if not args.no_cfg1 and args.config == "cfg2":
+        try:
+            cfg1 = cfg1_pipeline(log, min(16, os.cpu_count() or 1))
+        except Exception as e:
+            cfg1 = {"error": f"{type(e).__name__}: {e}"}
+            log(f"cfg1 pipeline failed: {e}")
 
     out = {
         "metric": "histories/s per DROP iteration (MLP+update), RSP RMSE vs FP64 oracle, 1 GPU",
@@ -680,6 +828,8 @@ def main():
         "roofline_update": roof_update,
         "roofline_mlp": roof_mlp,
         "rmse_vs_oracle": rmse,
+        "rmse_vs_full_cpu_oracle": ({"source": args.oracle_images, **vs_oracle} if oracle_imgs else None),
+        "cfg1_cpu_pipeline": cfg1,
         "cpu_baseline": cpu,
     }
     print(json.dumps(out), flush=True)
