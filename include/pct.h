@@ -100,7 +100,9 @@ typedef struct pct_hist_soa {
 #define PCT_DROP_CACHED    0x100 /* reuse the plan's voxel lists instead of
                                     recomputing the MLP in this iteration      */
 #define PCT_DROP_ATOMIC    0x200 /* reference structure: atomic scatter and a
-                                    volume scan per block, no plan             */
+                                    volume scan per block, no plan (also what
+                                    the default runs for a scan whose plan
+                                    does not fit the device)                   */
 #define PCT_DROP_NO_GRAPH  0x400 /* launch the block loop eagerly (no hipGraph) */
 #define PCT_DROP_F32       0x800 /* precision form of the update: a_i.x, the
                                     residual and the per-voxel sums in FP32,
@@ -373,7 +375,27 @@ int pct_drop_stats(pct_ctx* ctx, const pct_hist_soa* hist, int block_size, int64
                                      of block b-1 beside the deltas of block b;
                                      2: two launches per block (default; the
                                      research variants always)                 */
+#define PCT_TUNE_PLAN_LIMIT_MB 6  /* cap, in MiB, on the device memory a DROP
+                                     plan may take (default: free memory); a
+                                     scan whose plan exceeds it runs in chunks
+                                     of blocks (see pct_drop_plan_choice)     */
 int pct_set_tuning(pct_ctx* ctx, int key, int value);
+
+/* Capacity (no counterpart in the reference, which walks every path inside
+ * its per-block kernel, .cu:5774-5841): the peak device bytes of building a
+ * DROP plan for n histories with `slots` MLP capacity slots (the sum of the
+ * step counts + 1, pct_mlp_capacity's total), `list_slots` slots of its
+ * block-strided list store and `entries` path entries (0, 0 before the MLP
+ * pass: what the build needs to start), with the one-launch form's records or
+ * not; and the choice pct_drop_iteration makes before and after its MLP pass
+ * -- 1: build the plan; 0: run the reference's structure (per block an atomic
+ * scatter and a volume scan) over chunks of whole blocks whose MLP lists take
+ * at most half the budget -- when the plan would exceed `budget` bytes (free
+ * device memory, or PCT_TUNE_PLAN_LIMIT_MB), V > 2^22 or the block size >
+ * 8191. *need receives the bytes. Pure functions (no device). */
+int64_t pct_drop_plan_bytes(int64_t n, int64_t slots, int64_t list_slots, int64_t entries, int with_prev);
+int     pct_drop_plan_choice(int64_t n, int64_t slots, int64_t list_slots, int64_t entries, int64_t V,
+                             int block_size, int64_t budget, int with_prev, int64_t* need);
 
 /* (e) Batched form: the DROP iterations of several independent scans (each
  * with its own context, i.e. its own grid and parameters) run together. The

@@ -496,7 +496,7 @@ class Batch:
 
 TUNING_KEYS = {"mlp_waves": _lib.PCT_TUNE_MLP_WAVES, "chunk_first": _lib.PCT_TUNE_CHUNK_FIRST,
                "chunk_max": _lib.PCT_TUNE_CHUNK_MAX, "overlap": _lib.PCT_TUNE_OVERLAP,
-               "block_form": _lib.PCT_TUNE_BLOCK_FORM}
+               "block_form": _lib.PCT_TUNE_BLOCK_FORM, "plan_limit_mb": _lib.PCT_TUNE_PLAN_LIMIT_MB}
 
 
 def set_tuning(ctx: "Context", **knobs):

@@ -144,6 +144,9 @@ SIGNATURES = {
     "pct_drop_image_update": (ctypes.c_int, [c_vp, c_vp, c_vp, c_vp]),
     "pct_drop_iteration": (ctypes.c_int, [c_vp, P(PctHistSoa), c_vp, ctypes.c_int, c_f32, ctypes.c_int]),
     "pct_drop_invalidate": (ctypes.c_int, [c_vp]),
+    "pct_drop_plan_bytes": (c_i64, [c_i64, c_i64, c_i64, c_i64, ctypes.c_int]),
+    "pct_drop_plan_choice": (ctypes.c_int, [c_i64, c_i64, c_i64, c_i64, c_i64, ctypes.c_int, c_i64, ctypes.c_int,
+                                            ctypes.POINTER(c_i64)]),
     "pct_drop_prepare": (ctypes.c_int, [c_vp, P(PctHistSoa), ctypes.c_int, ctypes.c_int]),
     "pct_set_profiling": (ctypes.c_int, [c_vp, ctypes.c_int]),
     "pct_last_timings": (ctypes.c_int, [c_vp, c_vp]),
@@ -204,6 +207,7 @@ PCT_TUNE_CHUNK_FIRST = 2
 PCT_TUNE_CHUNK_MAX = 3
 PCT_TUNE_OVERLAP = 4
 PCT_TUNE_BLOCK_FORM = 5
+PCT_TUNE_PLAN_LIMIT_MB = 6
 # the precision form: tabulated MLP with FP32 stepping + FP32/Kahan update
 PCT_PRECISION = PCT_MLP_TABLE | PCT_DROP_F32
 # the learned-path form: surrogate MLP (exact fallback) + FP32/Kahan update

@@ -20,6 +20,8 @@
  *      16-byte loads of the counts. Blocks stay in storage order and are never
  *      merged, so the algorithm is the reference's.
  */
+#include <algorithm>
+#include <vector>
 #include "pct_internal.h"
 
 int pct_mlp_capacity_impl(pct_ctx* ctx, const pct_hist_soa* hist, int64_t start, int64_t count, int64_t* offsets,
@@ -236,47 +238,75 @@ extern "C" int pct_drop_image_update(pct_ctx* ctx, float* x, float* x_update, ui
     return launch_apply(ctx, x, x_update, counts);
 }
 
-/* phase 1 of an iteration: capacity offsets + path store of all histories */
-static int build_path_store(pct_ctx* ctx, const pct_hist_soa* hist, int mode) {
-    const int64_t n = hist->n;
-    int rc;
-    if ((rc = ensure_accumulators(ctx))) return rc;
-    if ((rc = pct_ensure(ctx, ctx->offsets, sizeof(int64_t) * (n + 1)))) return rc;
-    int64_t total = 0;
-    if (ctx->profile) PCT_HIP_CHECK(ctx, hipEventRecord(ctx->ev[0], ctx->stream));
-    if ((rc = pct_mlp_capacity_impl(ctx, hist, 0, n, (int64_t*)ctx->offsets.ptr, &total))) return rc;
-    if ((rc = pct_ensure(ctx, ctx->path_store, sizeof(uint32_t) * (total > 0 ? total : 1)))) return rc;
-    if ((rc = pct_ensure(ctx, ctx->path_len, sizeof(uint32_t) * n))) return rc;
-    if ((rc = pct_ensure(ctx, ctx->chord, sizeof(double) * n))) return rc;
-    rc = pct_mlp_paths_impl(ctx, hist, 0, n, (const int64_t*)ctx->offsets.ptr, (uint32_t*)ctx->path_store.ptr,
-                            (uint32_t*)ctx->path_len.ptr, (double*)ctx->chord.ptr, nullptr, nullptr, mode);
-    if (rc) return rc;
-    /* sequential phases: the block loop starts when all lists are done */
-    if (ctx->profile) PCT_HIP_CHECK(ctx, hipEventRecord(ctx->ev[1], ctx->stream));
-    if (ctx->profile) PCT_HIP_CHECK(ctx, hipEventRecord(ctx->ev[2], ctx->stream));
-    return PCT_OK;
-}
-
-/* the reference's structure on the GPU: all MLP lists, then per block an
- * atomic scatter (block_update_GPU) and a volume scan (image_update_GPU) */
-int pct_drop_iteration_atomic(pct_ctx* ctx, const pct_hist_soa* hist, float* x, int block_size, float lambda,
-                              int mode) {
+/* Reference structure of the iteration (.cu:6795-6857), chunk by chunk: the
+ * MLP lists of a chunk of whole blocks into a path store of at most
+ * `store_limit` bytes (capacity offsets from the step counts), then per block
+ * the atomic scatter and the volume scan. Chunks never split a block, so the
+ * blocks run in storage order as in the reference; a chunk holds at least one
+ * block whatever the limit. Used for PCT_DROP_ATOMIC and, automatically, for
+ * scans whose DROP plan would not fit the device (pct_plan.hip). */
+int pct_drop_iteration_chunked(pct_ctx* ctx, const pct_hist_soa* hist, float* x, int block_size, float lambda,
+                               int mode, int64_t store_limit) {
     int rc = check_drop_hist(ctx, hist, "pct_drop_iteration");
     if (rc) return rc;
     const int64_t n = hist->n;
     if (n <= 0) return PCT_OK;
-    if ((rc = build_path_store(ctx, hist, mode))) return rc;
-    const int64_t* offs = (const int64_t*)ctx->offsets.ptr;
+    if ((rc = ensure_accumulators(ctx))) return rc;
+    float* xu = (float*)ctx->xupd.ptr;
+    uint32_t* cn = (uint32_t*)ctx->counts.ptr;
+    /* step counts of every history, then chunks of blocks whose capacity
+     * fits the limit */
+    if ((rc = pct_ensure(ctx, ctx->offsets, sizeof(int64_t) * (n + 1)))) return rc;
+    int64_t total = 0;
+    if (ctx->profile) PCT_HIP_CHECK(ctx, hipEventRecord(ctx->ev[0], ctx->stream));
+    if ((rc = pct_mlp_capacity_impl(ctx, hist, 0, n, (int64_t*)ctx->offsets.ptr, &total))) return rc;
+    std::vector<int64_t> cap((size_t)((n + block_size - 1) / block_size) + 1);
+    {
+        std::vector<int64_t> all((size_t)n + 1);
+        PCT_HIP_CHECK(ctx, hipMemcpyAsync(all.data(), ctx->offsets.ptr, sizeof(int64_t) * (n + 1),
+                                          hipMemcpyDeviceToHost, ctx->stream));
+        PCT_HIP_CHECK(ctx, hipStreamSynchronize(ctx->stream));
+        for (size_t b = 0; b + 1 < cap.size(); ++b) cap[b] = all[b * (size_t)block_size];
+        cap.back() = all[(size_t)n];
+    }
+    const int64_t nb = (int64_t)cap.size() - 1;
+    const int64_t slots_limit = std::max<int64_t>(1, store_limit / (int64_t)sizeof(uint32_t));
+    int64_t max_slots = 0, max_hist = 0;
+    std::vector<int64_t> chunk_b(1, 0);
+    while (chunk_b.back() < nb) {
+        int64_t b0 = chunk_b.back(), b1 = b0 + 1;
+        while (b1 < nb && cap[b1 + 1] - cap[b0] <= slots_limit) ++b1;
+        chunk_b.push_back(b1);
+        max_slots = std::max(max_slots, cap[b1] - cap[b0]);
+        max_hist = std::max(max_hist, std::min(n, b1 * block_size) - b0 * block_size);
+    }
+    if ((rc = pct_ensure(ctx, ctx->path_store, sizeof(uint32_t) * (size_t)(max_slots > 0 ? max_slots : 1))) ||
+        (rc = pct_ensure(ctx, ctx->blk_offsets, sizeof(int64_t) * (size_t)(max_hist + 1))) ||
+        (rc = pct_ensure(ctx, ctx->path_len, sizeof(uint32_t) * (size_t)max_hist)) ||
+        (rc = pct_ensure(ctx, ctx->chord, sizeof(double) * (size_t)max_hist)))
+        return rc;
     const uint32_t* vox = (const uint32_t*)ctx->path_store.ptr;
     const uint32_t* len = (const uint32_t*)ctx->path_len.ptr;
     const double* chord = (const double*)ctx->chord.ptr;
-    /* phase 2: the sequential DROP blocks */
-    float* xu = (float*)ctx->xupd.ptr;
-    uint32_t* cn = (uint32_t*)ctx->counts.ptr;
-    for (int64_t s = 0; s < n; s += block_size) {
-        int64_t cnt = (n - s) < block_size ? (n - s) : block_size;
-        if ((rc = launch_scatter(ctx, vox, offs + s, len + s, chord + s, hist->wepl + s, cnt, x, xu, cn, lambda))) return rc;
-        if ((rc = launch_apply(ctx, x, xu, cn))) return rc;
+    int64_t* offs = (int64_t*)ctx->blk_offsets.ptr;
+    for (size_t c = 0; c + 1 < chunk_b.size(); ++c) {
+        const int64_t h0 = chunk_b[c] * block_size, h1 = std::min(n, chunk_b[c + 1] * block_size);
+        /* capacity offsets of the chunk, then its lists */
+        if ((rc = pct_mlp_capacity_impl(ctx, hist, h0, h1 - h0, offs, &total))) return rc;
+        if ((rc = pct_mlp_paths_impl(ctx, hist, h0, h1 - h0, offs, (uint32_t*)vox, (uint32_t*)len, (double*)chord,
+                                     nullptr, nullptr, mode)))
+            return rc;
+        if (c == 0 && ctx->profile) {
+            PCT_HIP_CHECK(ctx, hipEventRecord(ctx->ev[1], ctx->stream));
+            PCT_HIP_CHECK(ctx, hipEventRecord(ctx->ev[2], ctx->stream));
+        }
+        for (int64_t s0 = h0; s0 < h1; s0 += block_size) {
+            const int64_t cnt = (h1 - s0) < block_size ? (h1 - s0) : block_size;
+            const int64_t l = s0 - h0;
+            if ((rc = launch_scatter(ctx, vox, offs + l, len + l, chord + l, hist->wepl + s0, cnt, x, xu, cn, lambda)))
+                return rc;
+            if ((rc = launch_apply(ctx, x, xu, cn))) return rc;
+        }
     }
     if (ctx->profile) {
         PCT_HIP_CHECK(ctx, hipEventRecord(ctx->ev[3], ctx->stream));
@@ -285,6 +315,12 @@ int pct_drop_iteration_atomic(pct_ctx* ctx, const pct_hist_soa* hist, float* x, 
     int flags = 0;
     if ((rc = pct_take_status(ctx, &flags))) return rc;
     return pct_status_to_rc(ctx, flags);
+}
+
+int pct_drop_iteration_atomic(pct_ctx* ctx, const pct_hist_soa* hist, float* x, int block_size, float lambda,
+                              int mode) {
+    /* the whole scan in one chunk when its lists take at most 8 GiB */
+    return pct_drop_iteration_chunked(ctx, hist, x, block_size, lambda, mode, (int64_t)8 << 30);
 }
 
 extern "C" int pct_set_profiling(pct_ctx* ctx, int on) {

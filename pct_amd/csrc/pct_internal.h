@@ -86,6 +86,7 @@ struct pct_ctx {
     pct_buffer net_partial, net_state, net_loss, net_calib, net_stage_buf;
     /* execution knobs (pct_set_tuning; 0 = default) */
     int tune_mlp_waves = 0, tune_chunk_first = 0, tune_chunk_max = 0, tune_overlap = 0, tune_block_form = 0;
+    int tune_plan_limit_mb = 0;
     /* workspaces, grown on demand */
     /* research list iteration: sort keys, entries, lists, per-list factors,
      * block offsets, sort storage (pct_research.hip) */

@@ -57,6 +57,8 @@ int pct_mlp_paths_impl(pct_ctx* ctx, const pct_hist_soa* hist, int64_t start, in
 int pct_status_to_rc(pct_ctx* ctx, int flags);
 int pct_drop_iteration_atomic(pct_ctx* ctx, const pct_hist_soa* hist, float* x, int block_size, float lambda,
                               int mode);
+int pct_drop_iteration_chunked(pct_ctx* ctx, const pct_hist_soa* hist, float* x, int block_size, float lambda,
+                               int mode, int64_t store_limit);
 int pct_mlp_paths_on(pct_ctx* ctx, hipStream_t stream, const pct_hist_soa* hist, int64_t start, int64_t count,
                      const int64_t* offsets, uint32_t* voxels, uint32_t* lengths, double* chord, uint32_t* hash_out,
                      const uint32_t* hash_expect, int mode, bool beside);
@@ -401,15 +403,26 @@ static int vbits_for(int64_t V) {
 }
 
 /* Build (or rebuild) the plan of `hist` with blocks of B histories. */
-static int plan_build(pct_ctx* ctx, const pct_hist_soa* hist, int B, int mlp_mode) {
+static int64_t plan_budget(pct_ctx* ctx);
+static void plan_free_buffers(pct_plan* pl);
+
+/* *capacity is set when the plan cannot be built for this scan on this
+ * device (grid or block beyond the records' index fields, or not enough
+ * memory); the context then holds no plan buffers */
+static int plan_build(pct_ctx* ctx, const pct_hist_soa* hist, int B, int mlp_mode, bool* capacity = nullptr) {
     const int64_t n = hist->n;
+    bool cap_dummy = false;
+    bool& cap_fail = capacity ? *capacity : cap_dummy;
+    cap_fail = false;
     if (n <= 0) return pct_fail(ctx, PCT_E_INVALID, "DROP plan: no histories");
-    if (B <= 0 || B > SEG_MAX_B)
-        return pct_fail(ctx, PCT_E_INVALID, "DROP block size %d outside [1, %lld]", B, (long long)SEG_MAX_B);
+    if (B <= 0 || B > SEG_MAX_B || ctx->g.V > SEG_MAX_V) {
+        cap_fail = true;
+        if (ctx->plan) plan_free_buffers(ctx->plan);
+        return pct_fail(ctx, PCT_E_INVALID, "no DROP plan for %lld voxels (at most %lld) in blocks of %d (at most %lld)",
+                        (long long)ctx->g.V, (long long)SEG_MAX_V, B, (long long)SEG_MAX_B);
+    }
     const int vbits = vbits_for(ctx->g.V);
-    if (ctx->g.V > SEG_MAX_V)
-        return pct_fail(ctx, PCT_E_INVALID, "image too large for the plan (%lld voxels > %lld)", (long long)ctx->g.V,
-                        (long long)SEG_MAX_V);
+    const bool with_prev = ctx->tune_block_form == 1;
     if (!ctx->plan) ctx->plan = new pct_plan();
     pct_plan* pl = ctx->plan;
     plan_release_graph(pl);
@@ -421,6 +434,16 @@ static int plan_build(pct_ctx* ctx, const pct_hist_soa* hist, int B, int mlp_mod
     if ((rc = pct_ensure(ctx, ctx->offsets, sizeof(int64_t) * (n + 1)))) return rc;
     int64_t total = 0;
     if ((rc = pct_mlp_capacity_impl(ctx, hist, 0, n, (int64_t*)ctx->offsets.ptr, &total))) return rc;
+    {
+        const int64_t budget = plan_budget(ctx);
+        int64_t need = 0;
+        if (!pct_drop_plan_choice(n, total, 0, 0, ctx->g.V, B, budget, with_prev, &need)) {
+            cap_fail = true;
+            plan_free_buffers(pl);
+            return pct_fail(ctx, PCT_E_NOMEM, "the DROP plan's build needs %.1f GB, %.1f GB available", need / 1e9,
+                            budget / 1e9);
+        }
+    }
     if ((rc = pct_ensure(ctx, ctx->path_store, sizeof(uint32_t) * (total > 0 ? total : 1)))) return rc;
     if ((rc = pct_ensure(ctx, pl->len, sizeof(uint32_t) * n))) return rc;
     if ((rc = pct_ensure(ctx, pl->chord, sizeof(double) * n))) return rc;
@@ -515,6 +538,18 @@ static int plan_build(pct_ctx* ctx, const pct_hist_soa* hist, int B, int mlp_mod
     PCT_HIP_CHECK(ctx, hipMemcpyAsync(ex_h.data(), ex_off.ptr, sizeof(int64_t) * (n + 1), hipMemcpyDeviceToHost,
                                       ctx->stream));
     PCT_HIP_CHECK(ctx, hipStreamSynchronize(ctx->stream));
+    {
+        const int64_t budget = plan_budget(ctx);
+        int64_t need = 0;
+        if (!pct_drop_plan_choice(n, total, list_size, E, ctx->g.V, B, budget, with_prev, &need)) {
+            release(ex_off);
+            release(ctx->path_store);
+            cap_fail = true;
+            plan_free_buffers(pl);
+            return pct_fail(ctx, PCT_E_NOMEM, "the DROP plan needs %.1f GB, %.1f GB available", need / 1e9,
+                            budget / 1e9);
+        }
+    }
     std::vector<int64_t> blk_entry_h(nb + 1);
     for (int64_t b = 0; b <= nb; ++b) blk_entry_h[b] = ex_h[b * B < n ? b * B : n];
     if ((rc = pct_ensure(ctx, pl->blk_entry, sizeof(int64_t) * (nb + 1))) ||
@@ -619,7 +654,6 @@ static int plan_build(pct_ctx* ctx, const pct_hist_soa* hist, int B, int mlp_mod
      * record of each voxel. Maps from voxel to segment for M blocks at a time
      * (tagged with the block, so never cleared); the build store is freed
      * afterwards. */
-    const bool with_prev = ctx->tune_block_form == 1;
     if ((rc = pct_ensure(ctx, pl->list, sizeof(uint32_t) * (size_t)(list_size > 0 ? list_size : 1)))) return rc;
     hipLaunchKernelGGL(k_fill_u32, dim3(std::min(pct_grid(list_size, 256), PCT_GRID_CAP)), dim3(256), 0, ctx->stream,
                        (uint32_t*)pl->list.ptr, list_size, LIST_SENTINEL);
@@ -934,13 +968,73 @@ static int plan_iterate(pct_ctx* ctx, pct_plan* pl, const pct_hist_soa* hist, fl
     return PCT_OK;
 }
 
+/* ------------------------------------------------------------------------- */
+/* capacity: does the plan of a scan fit the device?                          */
+/* ------------------------------------------------------------------------- */
+/* Bytes a plan takes at its build's peak: n histories, `slots` MLP capacity
+ * slots (the sum over histories of the step count + 1, an upper bound of the
+ * path lengths: the capacity-strided build store holds 4 B per slot),
+ * `list_slots` slots of the block-strided list store (4 B each, 12 B with the
+ * one-launch form's previous-block records), `entries` path entries (at most
+ * one 8 B segment record and one 2 B csc index each), 38 B per history kept
+ * (length, chord, hash, row offset, placement) and 32 B per history plus up
+ * to 4 GiB of workspace while building (offsets, placement sort, the radix
+ * sort of a chunk of blocks, the voxel maps). Before the MLP pass only n and
+ * slots are known: list_slots = entries = 0 gives what the build needs to
+ * start. */
+extern "C" int64_t pct_drop_plan_bytes(int64_t n, int64_t slots, int64_t list_slots, int64_t entries,
+                                       int with_prev) {
+    return 4 * slots + (with_prev ? 12 : 4) * list_slots + 10 * entries + 70 * n + ((int64_t)4 << 30);
+}
+
+/* 1: build a plan; 0: run the chunked reference structure instead -- when
+ * the plan's segment records cannot index the grid (V > 2^22) or the block
+ * (more than 8191 histories), or its peak bytes exceed `budget` */
+extern "C" int pct_drop_plan_choice(int64_t n, int64_t slots, int64_t list_slots, int64_t entries, int64_t V,
+                                    int block_size, int64_t budget, int with_prev, int64_t* need) {
+    const int64_t b = pct_drop_plan_bytes(n, slots, list_slots, entries, with_prev);
+    if (need) *need = b;
+    if (V > SEG_MAX_V || block_size <= 0 || block_size > SEG_MAX_B) return 0;
+    return b <= budget ? 1 : 0;
+}
+
+/* device bytes a plan may take now: free memory plus what the context's plan
+ * holds (a rebuild reuses or frees it), less 1 GiB, capped by the knob */
+static int64_t plan_budget(pct_ctx* ctx) {
+    size_t fr = 0, tot = 0;
+    int64_t budget = hipMemGetInfo(&fr, &tot) == hipSuccess ? (int64_t)fr : 0;
+    if (ctx->plan) {
+        pct_plan* pl = ctx->plan;
+        for (pct_buffer* b : {&pl->list, &pl->prev, &pl->segs, &pl->csc, &pl->hist_off, &pl->pos_k, &pl->pos_len,
+                              &pl->pos_chord, &pl->len, &pl->chord, &pl->hash})
+            budget += (int64_t)b->bytes;
+    }
+    budget -= (int64_t)1 << 30;
+    if (ctx->tune_plan_limit_mb > 0) budget = std::min<int64_t>(budget, (int64_t)ctx->tune_plan_limit_mb << 20);
+    return budget;
+}
+
+static void plan_free_buffers(pct_plan* pl) {
+    plan_release_graph(pl);
+    for (pct_buffer* b : {&pl->list, &pl->prev, &pl->hist_off, &pl->len, &pl->chord, &pl->hash, &pl->segs, &pl->csc,
+                          &pl->blk_seg, &pl->blk_entry, &pl->delta, &pl->xw, &pl->pos_k, &pl->pos_len, &pl->pos_chord})
+        release(*b);
+    pl->valid = false;
+    ++pl->generation;
+}
+
 extern "C" int pct_drop_prepare(pct_ctx* ctx, const pct_hist_soa* hist, int block_size, int mode) {
     if (!ctx || !hist || !hist->wepl || !hist->first_voxel || !hist->x_entry)
         return pct_fail(ctx, PCT_E_INVALID, "pct_drop_prepare: null argument");
     const int mlp = mode & PCT_MLP_MODE_MASK;
     if (!pct_mlp_mode_ok(ctx, mlp)) return pct_fail(ctx, PCT_E_INVALID, "pct_drop_prepare: MLP mode %d not available", mlp);
     if (hist->n <= 0) return pct_fail(ctx, PCT_E_INVALID, "pct_drop_prepare: no histories");
-    return plan_build(ctx, hist, block_size, mlp);
+    bool capacity = false;
+    int rc = plan_build(ctx, hist, block_size, mlp, &capacity);
+    if (rc && capacity)
+        return pct_fail(ctx, PCT_E_NOMEM, "pct_drop_prepare: %s; pct_drop_iteration runs this scan in chunks of "
+                                          "blocks instead", ctx->err.c_str());
+    return rc;
 }
 
 extern "C" int pct_drop_iteration(pct_ctx* ctx, const pct_hist_soa* hist, float* x, int block_size, float lambda, int mode) {
@@ -963,7 +1057,18 @@ extern "C" int pct_drop_iteration(pct_ctx* ctx, const pct_hist_soa* hist, float*
     pct_plan* pl = ctx->plan;
     bool fresh = false;
     if (!plan_matches(ctx, pl, hist, block_size, mlp)) {
-        if ((rc = plan_build(ctx, hist, block_size, mlp))) return rc;
+        /* a scan whose plan does not fit the device (or whose grid or block
+         * size the plan's records cannot index) runs in chunks of blocks, the
+         * reference's structure; the research variants need the plan */
+        bool capacity = false;
+        if ((rc = plan_build(ctx, hist, block_size, mlp, &capacity))) {
+            if (!capacity) return rc;
+            if (variant != 0)
+                return pct_fail(ctx, PCT_E_NOMEM, "pct_drop_iteration: the research variants need a DROP plan (%s)",
+                                ctx->err.c_str());
+            const int64_t store = std::max<int64_t>(1, std::min<int64_t>((int64_t)8 << 30, plan_budget(ctx) / 2));
+            return pct_drop_iteration_chunked(ctx, hist, x, block_size, lambda, mlp, store);
+        }
         pl = ctx->plan;
         fresh = true;
     }
@@ -1028,6 +1133,7 @@ extern "C" int pct_set_tuning(pct_ctx* ctx, int key, int value) {
     case PCT_TUNE_MLP_WAVES: ctx->tune_mlp_waves = value; break;
     case PCT_TUNE_CHUNK_FIRST: ctx->tune_chunk_first = value; break;
     case PCT_TUNE_CHUNK_MAX: ctx->tune_chunk_max = value; break;
+    case PCT_TUNE_PLAN_LIMIT_MB: ctx->tune_plan_limit_mb = value; break;
     case PCT_TUNE_BLOCK_FORM:
         if (value > 2) return pct_fail(ctx, PCT_E_INVALID, "pct_set_tuning: block form %d not in {1, 2}", value);
         ctx->tune_block_form = value;
