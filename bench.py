@@ -444,8 +444,10 @@ def main():
                          "network trained first on 10 M exact-MLP paths; walks: no iteration, the exact, "
                          "precision and surrogate MLP walks once over all histories, one launch each, the "
                          "surrogate with an untrained network over the whole input range)")
-    ap.add_argument("--block-form", type=int, default=0, choices=[0, 1, 2],
-                    help="block loop: 1 one launch per block, 2 two launches per block (0: the default, 2)")
+    ap.add_argument("--block-form", type=int, default=0, choices=[0, 1, 2, 3],
+                    help="block loop: 1 one launch per block recomputing the previous block's updates, 2 two "
+                         "launches per block, 3 one launch per block waiting for the previous block's stored "
+                         "updates (0: the library's default)")
     ap.add_argument("--no-cfg1", action="store_true", help="skip the 100x100x16 / 2 M CPU-and-GPU pipeline")
     ap.add_argument("--oracle-images", default=None,
                     help="directory of the CPU oracle's images of this scan after iterations 1, 6, 12 "

@@ -140,6 +140,7 @@ int pct_take_status(pct_ctx* ctx, int* flags);
 #define PCT_STATUS_CAPACITY   2   /* path longer than its precomputed capacity */
 #define PCT_STATUS_TABLE      4   /* chord longer than the per-depth table     */
 #define PCT_STATUS_PLAN       8   /* recomputed lists differ from the DROP plan */
+#define PCT_STATUS_WAIT      16   /* a block-loop wait gave up (waiting form)   */
 
 /* MLP modes accepted by the path and DROP entry points (the surrogate only
  * once a network is installed) */

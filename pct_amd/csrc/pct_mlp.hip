@@ -1226,6 +1226,9 @@ int pct_mlp_paths_impl(pct_ctx* ctx, const pct_hist_soa* hist, int64_t start, in
 int pct_status_to_rc(pct_ctx* ctx, int flags) {
     if (flags & PCT_STATUS_PLAN)
         return pct_fail(ctx, PCT_E_INVALID, "recomputed MLP voxel lists differ from the DROP plan (inputs changed?)");
+    if (flags & PCT_STATUS_WAIT)
+        return pct_fail(ctx, PCT_E_HIP,
+                        "DROP block loop: a wait for the previous block's stored voxel updates gave up (x unchanged)");
     if (flags & PCT_STATUS_TABLE) return pct_fail(ctx, PCT_E_OVERFLOW, "MLP chord longer than the per-depth table");
     if (flags & PCT_STATUS_CAPACITY) return pct_fail(ctx, PCT_E_OVERFLOW, "MLP path longer than its capacity");
     if (flags & PCT_STATUS_OVERFLOW)

@@ -27,6 +27,7 @@ struct pct_plan {
      * block's segment record of the voxel (0 if it does not touch it) */
     pct_buffer prev;               /* u64                                   */
     bool has_prev = false;
+    bool has_tags = false;      /* list slots tagged for the waiting one-launch form */
     pct_buffer hist_off;           /* int64 per history+1: row offsets      */
     /* rows of a block in placement order (lateral bands per XCD): per
      * position the block-local history index, list length, chord length */

@@ -174,12 +174,14 @@ k_seg_map(const uint64_t* __restrict__ segs, const int64_t* __restrict__ blk_seg
 /* one wavefront per history of [h0, h1): its voxel list into the
  * block-strided list store and, when prev is given, slot for slot the
  * previous block's segment record of each voxel (0 when that block does not
- * touch it; looked up in the previous block's map) */
+ * touch it; looked up in the previous block's map); with tag, the slots whose
+ * voxel the previous block touches carry LIST_TAG */
 __global__ void __launch_bounds__(256)
 k_build_entries(const uint32_t* __restrict__ path, const int64_t* __restrict__ cap_off,
                 const uint32_t* __restrict__ len, const int64_t* __restrict__ hist_off, int64_t h0, int64_t h1,
                 int64_t B, const uint64_t* __restrict__ segs, const int64_t* __restrict__ blk_seg, int64_t M, int64_t V,
-                const uint64_t* __restrict__ map, uint32_t* __restrict__ list, uint64_t* __restrict__ prev) {
+                const uint64_t* __restrict__ map, uint32_t* __restrict__ list, uint64_t* __restrict__ prev,
+                int tag) {
     const int64_t h = h0 + (int64_t)blockIdx.x * 4 + (threadIdx.x >> 6);
     const int lane = threadIdx.x & 63;
     if (h >= h1) return;
@@ -187,19 +189,14 @@ k_build_entries(const uint32_t* __restrict__ path, const int64_t* __restrict__ c
     const int n = (int)len[h];
     const uint32_t* src = path + cap_off[h];
     const int64_t dst = hist_off[h];
-    const uint64_t* m = (prev && b > 0) ? map + ((b - 1) % M) * V : nullptr;
+    const uint64_t* m = ((prev || tag) && b > 0) ? map + ((b - 1) % M) * V : nullptr;
     const int64_t sp = b > 0 ? blk_seg[b - 1] : 0;
     for (int j = lane; j < n; j += 64) {
         const uint32_t v = src[j];
-        list[dst + j] = v;
-        if (prev) {
-            uint64_t e = 0;
-            if (m) {
-                const uint64_t w = m[v];
-                if ((w >> 32) == (uint64_t)(b - 1)) e = segs[sp + (int64_t)(uint32_t)w];
-            }
-            prev[dst + j] = e;
-        }
+        const uint64_t w = m ? m[v] : 0ull;
+        const bool in_prev = m && (w >> 32) == (uint64_t)(b - 1);
+        list[dst + j] = (tag && in_prev) ? (v | LIST_TAG) : v;
+        if (prev) prev[dst + j] = in_prev ? segs[sp + (int64_t)(uint32_t)w] : 0ull;
     }
 }
 
@@ -349,13 +346,44 @@ k_block_fused(const uint32_t* __restrict__ list, const uint64_t* __restrict__ pr
     pct_blk::fused_delta<F32>(list, prev, stride, pos, wepl, h0, q, xw, pv, lambda, delta_out, threadIdx.x & 63);
 }
 
+/* one launch per block b, waiting form: the voxel updates of block b-1
+ * (workgroups [0, na), na a multiple of 8, one thread per segment) and the
+ * deltas of block b (workgroup na + j on XCD j % 8, DELTA_WAVES histories
+ * each, the placement the plan's lateral bands assume); bodies in
+ * pct_plan_kernels.h */
+#define WAIT_THREADS (64 * DELTA_WAVES)
+static_assert(WAIT_FLAG == PCT_STATUS_WAIT, "the waiting form's give-up flag is the status word's");
+template <bool F32>
+__global__ void __launch_bounds__(WAIT_THREADS)
+k_block_wait(const uint32_t* __restrict__ list, int stride, pct_blk::blk_pos pos, const float* __restrict__ wepl,
+             int64_t h0, int64_t count, double lambda, void* __restrict__ delta_out,
+             const uint64_t* __restrict__ segs_p, int64_t ns_p, const uint16_t* __restrict__ csc_p,
+             const void* __restrict__ delta_p, uint32_t stamp_p, unsigned na, uint64_t* __restrict__ xw,
+             int* __restrict__ status) {
+    if (blockIdx.x < na) {
+        __builtin_amdgcn_s_setprio(3);
+        const pct_blk::fused_prev<F32> pv{csc_p, delta_p, stamp_p};
+        const int64_t k = (int64_t)blockIdx.x * WAIT_THREADS + threadIdx.x;
+        if (k < ns_p) pct_blk::fused_apply<F32>(segs_p[k], xw, pv);
+        return;
+    }
+    __builtin_amdgcn_s_setprio(2);
+    const int64_t q = (int64_t)(blockIdx.x - na) * DELTA_WAVES + (threadIdx.x >> 6);
+    if (q >= count) return;
+    pct_blk::wait_delta<F32>(list, stride, pos, wepl, h0, q, xw, stamp_p, status, lambda, delta_out,
+                             threadIdx.x & 63);
+}
+
 /* the image as {value, stamp} words (stamp: no block yet) and back */
 __global__ void k_xw_pack(const float* __restrict__ x, int64_t V, uint64_t* __restrict__ xw) {
     const int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
     if (i < V) xw[i] = pct_blk::xw_pack(x[i], 0xFFFFFFFFu);
 }
 
-__global__ void k_xw_unpack(const uint64_t* __restrict__ xw, int64_t V, float* __restrict__ x) {
+/* x is left as it was when a wait of the waiting form gave up */
+__global__ void k_xw_unpack(const uint64_t* __restrict__ xw, int64_t V, float* __restrict__ x,
+                            const int* __restrict__ status) {
+    if (*status & PCT_STATUS_WAIT) return;
     const int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
     if (i < V) x[i] = pct_blk::xw_value(xw[i]);
 }
@@ -423,6 +451,7 @@ static int plan_build(pct_ctx* ctx, const pct_hist_soa* hist, int B, int mlp_mod
     }
     const int vbits = vbits_for(ctx->g.V);
     const bool with_prev = ctx->tune_block_form == 1;
+    const bool with_tags = ctx->tune_block_form == 3;
     if (!ctx->plan) ctx->plan = new pct_plan();
     pct_plan* pl = ctx->plan;
     plan_release_graph(pl);
@@ -664,10 +693,11 @@ static int plan_build(pct_ctx* ctx, const pct_hist_soa* hist, int B, int mlp_mod
     }
     {
         const int64_t V = ctx->g.V;
-        int64_t M = with_prev ? std::max<int64_t>(2, std::min<int64_t>(256, ((int64_t)1 << 31) / (8 * V))) : 256;
+        const bool maps = with_prev || with_tags;
+        int64_t M = maps ? std::max<int64_t>(2, std::min<int64_t>(256, ((int64_t)1 << 31) / (8 * V))) : 256;
         M = std::min<int64_t>(M, nb + 1);
         pct_buffer map;
-        if (with_prev) {
+        if (maps) {
             if ((rc = pct_ensure(ctx, map, sizeof(uint64_t) * (size_t)(M * V)))) return rc;
             hipLaunchKernelGGL(k_fill_u64, dim3(std::min(pct_grid(M * V, 256), PCT_GRID_CAP)), dim3(256), 0,
                                ctx->stream, (uint64_t*)map.ptr, M * V, ~0ull);
@@ -677,7 +707,7 @@ static int plan_build(pct_ctx* ctx, const pct_hist_soa* hist, int B, int mlp_mod
         for (int64_t c0 = 0; c0 < nb; c0 += step) {
             const int64_t c1 = std::min(nb, c0 + step);
             const int64_t m0 = c0 > 0 ? c0 - 1 : 0, m1 = c1 - 1;
-            if (with_prev && m1 > m0) {
+            if (maps && m1 > m0) {
                 int64_t maxs = 0;
                 for (int64_t b = m0; b < m1; ++b) maxs = std::max(maxs, pl->blk_seg_h[b + 1] - pl->blk_seg_h[b]);
                 if (maxs > 0)
@@ -690,13 +720,14 @@ static int plan_build(pct_ctx* ctx, const pct_hist_soa* hist, int B, int mlp_mod
                                (const uint32_t*)ctx->path_store.ptr, cap_off, (const uint32_t*)len, hist_off, h0, h1,
                                (int64_t)B, (const uint64_t*)pl->segs.ptr, (const int64_t*)pl->blk_seg.ptr, M, V,
                                (const uint64_t*)map.ptr, (uint32_t*)pl->list.ptr,
-                               with_prev ? (uint64_t*)pl->prev.ptr : nullptr);
+                               with_prev ? (uint64_t*)pl->prev.ptr : nullptr, with_tags ? 1 : 0);
         }
         PCT_HIP_CHECK(ctx, hipGetLastError());
         PCT_HIP_CHECK(ctx, hipStreamSynchronize(ctx->stream));
         release(map);
     }
     pl->has_prev = with_prev;
+    pl->has_tags = with_tags;
     release(ctx->path_store);
     /* deltas of two blocks (the one-launch form reads block b-1's while writing
      * block b's), two doubles per history (ROBUST) */
@@ -725,7 +756,7 @@ static bool same_hist(const pct_hist_soa& a, const pct_hist_soa& b) {
 static bool plan_matches(const pct_ctx* ctx, const pct_plan* pl, const pct_hist_soa* hist, int B, int mlp) {
     return pl && pl->valid && pl->B == B && pl->mlp_mode == mlp && same_hist(pl->key, *hist) &&
            (mlp != PCT_MLP_SURROGATE || pl->net_version == ctx->net_version) &&
-           (ctx->tune_block_form != 1 || pl->has_prev);
+           (ctx->tune_block_form != 1 || pl->has_prev) && pl->has_tags == (ctx->tune_block_form == 3);
 }
 
 int pct_plan_build(pct_ctx* ctx, const pct_hist_soa* hist, int B, int mlp_mode) {
@@ -744,10 +775,13 @@ static int update_variant(int mode) {
 }
 
 /* block-loop form: 1 = one launch per block on {value, stamp} image words,
- * 2 = two launches per block on the float image (default: measured 13.7
- * against 14.0 us per block at 200x200x32; the research variants always) */
+ * the previous block's update recomputed where it is not stored yet;
+ * 2 = two launches per block on the float image (the research variants
+ * always); 3 = one launch per block on image words, waiting for the previous
+ * block's stored updates */
 static int block_form(const pct_ctx* ctx, int variant) {
-    return (variant != 0 || ctx->tune_block_form != 1) ? 2 : 1;
+    if (variant != 0) return 2;
+    return ctx->tune_block_form == 1 ? 1 : ctx->tune_block_form == 3 ? 3 : 2;
 }
 
 static int plan_block_loop(pct_ctx* ctx, pct_plan* pl, const pct_hist_soa* hist, float* x, double lambda, bool f32,
@@ -762,7 +796,8 @@ static int plan_block_loop(pct_ctx* ctx, pct_plan* pl, const pct_hist_soa* hist,
     };
     const uint64_t* segs = (const uint64_t*)pl->segs.ptr;
     const uint16_t* csc = (const uint16_t*)pl->csc.ptr;
-    if (block_form(ctx, variant) == 1) {
+    const int form = block_form(ctx, variant);
+    if (form == 1 || form == 3) {
         /* launch b: deltas of block b and the voxel updates of block b-1; the
          * last chunk adds launch nb (updates of the last block only) */
         uint64_t* xw = (uint64_t*)pl->xw.ptr;
@@ -782,10 +817,24 @@ static int plan_block_loop(pct_ctx* ctx, pct_plan* pl, const pct_hist_soa* hist,
                 nsp = pl->blk_seg_h[b] - pl->blk_seg_h[b - 1];
                 cp = csc + pl->blk_entry_h[b - 1];
             }
+            const uint32_t stamp = (uint32_t)(b - 1);
+            if (form == 3) {
+                const unsigned na = nsp > 0 ? (pct_grid(nsp, WAIT_THREADS) + 7u) / 8u * 8u : 0u;
+                const unsigned nd = (unsigned)((cnt + DELTA_WAVES - 1) / DELTA_WAVES);
+                if (na + nd == 0) continue;
+                if (f32)
+                    hipLaunchKernelGGL(k_block_wait<true>, dim3(na + nd), dim3(WAIT_THREADS), 0, ctx->stream, lb,
+                                       stride, pos_of(b), (const float*)hist->wepl, h0, cnt, lambda, dout, sp, nsp,
+                                       cp, dprev, stamp, na, xw, ctx->d_status);
+                else
+                    hipLaunchKernelGGL(k_block_wait<false>, dim3(na + nd), dim3(WAIT_THREADS), 0, ctx->stream, lb,
+                                       stride, pos_of(b), (const float*)hist->wepl, h0, cnt, lambda, dout, sp, nsp,
+                                       cp, dprev, stamp, na, xw, ctx->d_status);
+                continue;
+            }
             const unsigned na = pct_grid(nsp, FUSED_THREADS) * (nsp > 0);
             const unsigned nd = (unsigned)((cnt + FUSED_DELTA_WAVES - 1) / FUSED_DELTA_WAVES);
             if (na + nd == 0) continue;
-            const uint32_t stamp = (uint32_t)(b - 1);
             if (f32)
                 hipLaunchKernelGGL(k_block_fused<true>, dim3(na + nd), dim3(FUSED_THREADS), 0, ctx->stream, lb, pb, stride,
                                    pos_of(b), (const float*)hist->wepl, h0, cnt, lambda, dout, sp, nsp, cp, dprev,
@@ -910,7 +959,7 @@ static int plan_ensure_events(pct_ctx* ctx, pct_plan* pl, int64_t nchunks) {
 static int plan_iterate(pct_ctx* ctx, pct_plan* pl, const pct_hist_soa* hist, float* x, double lambda, bool f32,
                         int variant, int mlp, bool recompute, bool eager) {
     const bool sequential = recompute && ctx->tune_overlap == 2;
-    const bool words = block_form(ctx, variant) == 1;
+    const bool words = block_form(ctx, variant) != 2;
     int rc;
     plan_chunks(ctx, pl);
     const int64_t nchunks = (int64_t)pl->chunk_b.size() - 1;
@@ -960,7 +1009,7 @@ static int plan_iterate(pct_ctx* ctx, pct_plan* pl, const pct_hist_soa* hist, fl
     }
     if (words)
         hipLaunchKernelGGL(k_xw_unpack, dim3(pct_grid(ctx->g.V, 256)), dim3(256), 0, ctx->stream,
-                           (const uint64_t*)pl->xw.ptr, ctx->g.V, x);
+                           (const uint64_t*)pl->xw.ptr, ctx->g.V, x, ctx->d_status);
     if (ctx->profile) {
         PCT_HIP_CHECK(ctx, hipEventRecord(ctx->ev[3], ctx->stream));
         ctx->ev_recorded = true;
@@ -1135,7 +1184,7 @@ extern "C" int pct_set_tuning(pct_ctx* ctx, int key, int value) {
     case PCT_TUNE_CHUNK_MAX: ctx->tune_chunk_max = value; break;
     case PCT_TUNE_PLAN_LIMIT_MB: ctx->tune_plan_limit_mb = value; break;
     case PCT_TUNE_BLOCK_FORM:
-        if (value > 2) return pct_fail(ctx, PCT_E_INVALID, "pct_set_tuning: block form %d not in {1, 2}", value);
+        if (value > 3) return pct_fail(ctx, PCT_E_INVALID, "pct_set_tuning: block form %d not in {1, 2, 3}", value);
         ctx->tune_block_form = value;
         break;
     case PCT_TUNE_OVERLAP:

@@ -17,6 +17,11 @@
 #include <hip/hip_runtime.h>
 
 #define LIST_SENTINEL 0xFFFFFFFFu
+/* bit 31 of a list slot: the voxel is also on a path of the previous block
+ * (set only in plans built for the waiting one-launch form; every reader
+ * masks it off) */
+#define LIST_TAG 0x80000000u
+#define LIST_VOX(e) ((e) & ~LIST_TAG)
 /* 8-byte segment record: bits 0-21 voxel, 22-23 inline count (1-3; 0 = the
  * history indices are in csc), then either up to three 13-bit block-local
  * history indices (bits 24-36, 37-49, 50-62) or the block-relative first csc
@@ -86,10 +91,10 @@ __device__ __forceinline__ void delta_f64(const uint32_t* __restrict__ list, int
     double s = 0.0;
 #pragma unroll
     for (int r = 0; r < DELTA_SLOTS; ++r)
-        if (v[r] != LIST_SENTINEL) s += (double)x[v[r]];
+        if (v[r] != LIST_SENTINEL) s += (double)x[LIST_VOX(v[r])];
     for (int j = lane + 64 * DELTA_SLOTS; j < stride; j += 64) {
         uint32_t w = p[j];
-        if (w != LIST_SENTINEL) s += (double)x[w];
+        if (w != LIST_SENTINEL) s += (double)x[LIST_VOX(w)];
     }
     const float b_i = wepl[h0 + k];
     s = wave_sum_d(s);
@@ -117,10 +122,10 @@ __device__ __forceinline__ void delta_f32(const uint32_t* __restrict__ list, int
     float s = 0.0f, c = 0.0f;
 #pragma unroll
     for (int r = 0; r < DELTA_SLOTS; ++r)
-        if (v[r] != LIST_SENTINEL) kahan_add(s, c, x[v[r]]);
+        if (v[r] != LIST_SENTINEL) kahan_add(s, c, x[LIST_VOX(v[r])]);
     for (int j = lane + 64 * DELTA_SLOTS; j < stride; j += 64) {
         uint32_t w = p[j];
-        if (w != LIST_SENTINEL) kahan_add(s, c, x[w]);
+        if (w != LIST_SENTINEL) kahan_add(s, c, x[LIST_VOX(w)]);
     }
 #pragma unroll
     for (int d = 32; d >= 1; d >>= 1) {
@@ -165,10 +170,10 @@ __device__ __forceinline__ void delta_variant(int variant, const uint32_t* __res
     double s = 0.0;
 #pragma unroll
     for (int r = 0; r < DELTA_SLOTS; ++r)
-        if (v[r] != LIST_SENTINEL) s += (double)x[v[r]];
+        if (v[r] != LIST_SENTINEL) s += (double)x[LIST_VOX(v[r])];
     for (int j = lane + 64 * DELTA_SLOTS; j < stride; j += 64) {
         uint32_t w = p[j];
-        if (w != LIST_SENTINEL) s += (double)x[w];
+        if (w != LIST_SENTINEL) s += (double)x[LIST_VOX(w)];
     }
     const float b_i = wepl[h0 + k];
     s = wave_sum_d(s);
@@ -380,10 +385,10 @@ __device__ __forceinline__ void fused_delta(const uint32_t* __restrict__ list, c
         float s = 0.0f, c = 0.0f;
 #pragma unroll
         for (int r = 0; r < DELTA_SLOTS; ++r)
-            if (v[r] != LIST_SENTINEL) kahan_add(s, c, fused_x<F32>(v[r], e[r], xw, pv));
+            if (v[r] != LIST_SENTINEL) kahan_add(s, c, fused_x<F32>(LIST_VOX(v[r]), e[r], xw, pv));
         for (int j = lane + 64 * DELTA_SLOTS; j < stride; j += 64) {
             const uint32_t q = p[j];
-            if (q != LIST_SENTINEL) kahan_add(s, c, fused_x<F32>(q, pr[j], xw, pv));
+            if (q != LIST_SENTINEL) kahan_add(s, c, fused_x<F32>(LIST_VOX(q), pr[j], xw, pv));
         }
 #pragma unroll
         for (int d = 32; d >= 1; d >>= 1) {
@@ -402,10 +407,10 @@ __device__ __forceinline__ void fused_delta(const uint32_t* __restrict__ list, c
         double s = 0.0;
 #pragma unroll
         for (int r = 0; r < DELTA_SLOTS; ++r)
-            if (v[r] != LIST_SENTINEL) s += (double)fused_x<F32>(v[r], e[r], xw, pv);
+            if (v[r] != LIST_SENTINEL) s += (double)fused_x<F32>(LIST_VOX(v[r]), e[r], xw, pv);
         for (int j = lane + 64 * DELTA_SLOTS; j < stride; j += 64) {
             const uint32_t q = p[j];
-            if (q != LIST_SENTINEL) s += (double)fused_x<F32>(q, pr[j], xw, pv);
+            if (q != LIST_SENTINEL) s += (double)fused_x<F32>(LIST_VOX(q), pr[j], xw, pv);
         }
         const float b_i = wepl[h0 + k];
         s = wave_sum_d(s);
@@ -425,6 +430,112 @@ __device__ __forceinline__ void fused_apply(uint64_t r, uint64_t* __restrict__ x
     const float nv = F32 ? applied_f32(xv, r, pv.csc, (const float*)pv.delta)
                          : applied_f64(xv, r, pv.csc, (const double*)pv.delta);
     xw_store(xw + v, xw_pack(nv, pv.stamp));
+}
+
+/* ------------------------------------------------------------------------- */
+/* One launch per block, waiting form. The launch for block b runs block      */
+/* b-1's voxel updates in its first workgroups (fused_apply above) and block  */
+/* b's deltas in the rest, on the {value, stamp} words. The plan tags every   */
+/* list slot of block b whose voxel block b-1 also touches (LIST_TAG). An     */
+/* untagged slot's word is final when the launch starts -- its last update    */
+/* ran in an earlier launch -- and is read with a plain load. A tagged slot's */
+/* word is read at device scope until it carries stamp b-1, i.e. until block  */
+/* b-1's update of that voxel has been stored; the history then sums exactly  */
+/* the values the two-launch form reads, in the same order. The updates never */
+/* wait and their workgroups come first in the grid. Every wait is bounded    */
+/* all the same (WAIT_POLLS polls, or fewer once another wait has given up):  */
+/* a wait that gives up sets WAIT_FLAG in the status word, the launch runs to */
+/* its end, and the iteration reports the error with x unchanged.             */
+/* ------------------------------------------------------------------------- */
+#define WAIT_POLLS 4096
+#define WAIT_FLAG 16   /* PCT_STATUS_WAIT */
+
+__device__ __forceinline__ uint64_t wait_word(const uint64_t* p, uint64_t w, uint32_t stamp, int* status) {
+    for (int i = 1; xw_stamp(w) != stamp; ++i) {
+        if (i >= WAIT_POLLS) {
+            atomicOr(status, WAIT_FLAG);
+            break;
+        }
+        if ((i & 31) == 0 && (__hip_atomic_load(status, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT) & WAIT_FLAG))
+            break;
+        __builtin_amdgcn_s_sleep(2);
+        w = xw_load(p);
+    }
+    return w;
+}
+
+/* the word of list slot e (not the sentinel): plain for an untagged slot,
+ * first device-scope read for a tagged one */
+__device__ __forceinline__ uint64_t slot_word(const uint64_t* __restrict__ xw, uint32_t e) {
+    const uint64_t* a = xw + LIST_VOX(e);
+    return (e & LIST_TAG) ? xw_load(a) : *a;
+}
+
+template <bool F32>
+__device__ __forceinline__ void wait_delta(const uint32_t* __restrict__ list, int stride, blk_pos pos,
+                                           const float* __restrict__ wepl, int64_t h0, int64_t q,
+                                           const uint64_t* __restrict__ xw, uint32_t stamp, int* status,
+                                           double lambda, void* __restrict__ delta, int lane) {
+    const uint32_t k = pos.k[q];
+    const uint32_t n = pos.len[q];
+    const double Ld = pos.chord[q];
+    const uint32_t* p = list + q * stride;
+    uint32_t e[DELTA_SLOTS];
+    uint64_t w[DELTA_SLOTS];
+#pragma unroll
+    for (int r = 0; r < DELTA_SLOTS; ++r) {
+        const int j = lane + 64 * r;
+        e[r] = j < stride ? p[j] : LIST_SENTINEL;
+    }
+#pragma unroll
+    for (int r = 0; r < DELTA_SLOTS; ++r) w[r] = e[r] != LIST_SENTINEL ? slot_word(xw, e[r]) : 0ull;
+#pragma unroll
+    for (int r = 0; r < DELTA_SLOTS; ++r)
+        if (e[r] != LIST_SENTINEL && (e[r] & LIST_TAG)) w[r] = wait_word(xw + LIST_VOX(e[r]), w[r], stamp, status);
+    auto tail_value = [&](uint32_t t) {
+        uint64_t u = slot_word(xw, t);
+        if (t & LIST_TAG) u = wait_word(xw + LIST_VOX(t), u, stamp, status);
+        return xw_value(u);
+    };
+    if (F32) {
+        float s = 0.0f, c = 0.0f;
+#pragma unroll
+        for (int r = 0; r < DELTA_SLOTS; ++r)
+            if (e[r] != LIST_SENTINEL) kahan_add(s, c, xw_value(w[r]));
+        for (int j = lane + 64 * DELTA_SLOTS; j < stride; j += 64) {
+            const uint32_t t = p[j];
+            if (t != LIST_SENTINEL) kahan_add(s, c, tail_value(t));
+        }
+#pragma unroll
+        for (int d = 32; d >= 1; d >>= 1) {
+            float os = __shfl_xor(s, d, 64), oc = __shfl_xor(c, d, 64);
+            c += oc;
+            kahan_add(s, c, os);
+        }
+        const float b_i = wepl[h0 + k];
+        if (lane == 0) {
+            const float L = (float)Ld;
+            const float a_i_dot_x_k = s * L;
+            const float a_i_dot_a_i = (L * L) * (float)n;
+            ((float*)delta)[k] = L * ((b_i - a_i_dot_x_k) / a_i_dot_a_i) * (float)lambda;
+        }
+    } else {
+        double s = 0.0;
+#pragma unroll
+        for (int r = 0; r < DELTA_SLOTS; ++r)
+            if (e[r] != LIST_SENTINEL) s += (double)xw_value(w[r]);
+        for (int j = lane + 64 * DELTA_SLOTS; j < stride; j += 64) {
+            const uint32_t t = p[j];
+            if (t != LIST_SENTINEL) s += (double)tail_value(t);
+        }
+        const float b_i = wepl[h0 + k];
+        s = wave_sum_d(s);
+        if (lane == 0) {
+            const double a_i_dot_x_k = s * Ld;
+            const double a_i_dot_a_i = (Ld * Ld) * n;
+            ((double*)delta)[k] = Ld * ((b_i - a_i_dot_x_k) / a_i_dot_a_i) * lambda;
+        }
+    }
 }
 
 }  // namespace pct_blk

@@ -54,7 +54,10 @@ def test_full_size_schedules_agree(full):
                               ("sequential", pct_amd.PCT_MLP_FP64, {"overlap": 2}),
                               ("chunks", pct_amd.PCT_MLP_FP64, {"chunk_first": 5, "chunk_max": 40}),
                               ("one launch per block", pct_amd.PCT_MLP_FP64 | pct_amd.PCT_DROP_CACHED,
-                               {"block_form": 1})):
+                               {"block_form": 1}),
+                              ("waiting one launch per block", pct_amd.PCT_MLP_FP64, {"block_form": 3}),
+                              ("waiting, cached", pct_amd.PCT_MLP_FP64 | pct_amd.PCT_DROP_CACHED,
+                               {"block_form": 3})):
         pct_amd.set_tuning(ctx, overlap=0, chunk_first=0, chunk_max=0, mlp_waves=0, block_form=0)
         pct_amd.set_tuning(ctx, **knobs)
         x = pre.x0.clone()
@@ -82,7 +85,7 @@ def test_full_size_forms_schedules_agree(full, form):
             "robust2": pct_amd.PCT_MLP_FP64 | pct_amd.PCT_DROP_ROBUST2,
             "precision": pct_amd.PCT_PRECISION}[form]
     out = []
-    for extra, form in ((0, 0), (pct_amd.PCT_DROP_CACHED, 0), (pct_amd.PCT_DROP_CACHED, 1)):
+    for extra, form in ((0, 0), (pct_amd.PCT_DROP_CACHED, 0), (pct_amd.PCT_DROP_CACHED, 1), (0, 3)):
         pct_amd.set_tuning(ctx, block_form=form)
         x = pre.x0.clone()
         for _ in range(2):

@@ -147,8 +147,9 @@ def test_drop_iteration_matches_oracle(ctx_mini, mini_host):
 
 
 def test_drop_modes_agree(ctx_mini, mini_host):
-    """recompute (default), cached lists, eager launches and the reference-style
-    atomic scatter give the same image (the last to float-atomic tolerance)."""
+    """recompute (default), cached lists, eager launches, the two one-launch
+    block forms and the reference-style atomic scatter give the same image (the
+    last to float-atomic tolerance)."""
     import pct_amd
     torch = ctx_mini.torch
     d = _to_dev(mini_host["hr"])
@@ -158,7 +159,10 @@ def test_drop_modes_agree(ctx_mini, mini_host):
     for name, mode, form in [("default", 0, 0), ("cached", pct_amd.PCT_DROP_CACHED, 0),
                              ("eager", pct_amd.PCT_DROP_NO_GRAPH, 0), ("one-launch", 0, 1),
                              ("one-launch eager", pct_amd.PCT_DROP_NO_GRAPH, 1),
+                             ("waiting", 0, 3), ("waiting cached", pct_amd.PCT_DROP_CACHED, 3),
+                             ("waiting eager", pct_amd.PCT_DROP_NO_GRAPH, 3),
                              ("f32", pct_amd.PCT_DROP_F32, 0), ("f32 one-launch", pct_amd.PCT_DROP_F32, 1),
+                             ("f32 waiting", pct_amd.PCT_DROP_F32, 3),
                              ("atomic", pct_amd.PCT_DROP_ATOMIC, 0)]:
         pct_amd.set_tuning(ctx_mini, block_form=form)
         x = torch.from_numpy(x0.copy()).cuda()
@@ -166,9 +170,10 @@ def test_drop_modes_agree(ctx_mini, mini_host):
         ctx_mini.drop_iteration(d, x, lam=lam, mode=mode)
         out[name] = x.cpu().numpy()
     pct_amd.set_tuning(ctx_mini, block_form=0)
-    for name in ("cached", "eager", "one-launch", "one-launch eager"):
+    for name in ("cached", "eager", "one-launch", "one-launch eager", "waiting", "waiting cached", "waiting eager"):
         assert np.array_equal(out["default"].view(np.uint32), out[name].view(np.uint32)), name
-    assert np.array_equal(out["f32"].view(np.uint32), out["f32 one-launch"].view(np.uint32))
+    for name in ("f32 one-launch", "f32 waiting"):
+        assert np.array_equal(out["f32"].view(np.uint32), out[name].view(np.uint32)), name
     assert np.max(np.abs(out["default"] - out["atomic"])) < 1e-5
 
 
