@@ -372,9 +372,16 @@ int pct_drop_stats(pct_ctx* ctx, const pct_hist_soa* hist, int block_size, int64
 #define PCT_TUNE_OVERLAP       4  /* 1: MLP beside the block loop (default),
                                      2: MLP first, then the block loop         */
 #define PCT_TUNE_BLOCK_FORM    5  /* 1: one launch per block, the voxel updates
-                                     of block b-1 beside the deltas of block b;
+                                     of block b-1 beside the deltas of block b,
+                                     which recompute the updates not stored yet;
                                      2: two launches per block (default; the
-                                     research variants always)                 */
+                                     research variants always);
+                                     3: one launch per block, the deltas of
+                                     block b waiting (bounded) for block b-1's
+                                     stored updates of their voxels; a wait
+                                     that gives up returns PCT_E_HIP with x
+                                     unchanged. All three give the same image
+                                     bit for bit.                              */
 #define PCT_TUNE_PLAN_LIMIT_MB 6  /* cap, in MiB, on the device memory a DROP
                                      plan may take (default: free memory); a
                                      scan whose plan exceeds it runs in chunks
