@@ -501,11 +501,13 @@ def main():
     torch.cuda.empty_cache()
     timings = []
     pre = pipeline.preprocess(ctx, raw, timings=timings)
-    stages = {}
-    for name, ms, nbytes in timings:
+    def stage_entry(ms, nbytes, traffic):
         gbs = nbytes / (ms * 1e-3) / 1e9
-        stages[name] = {"ms": round(ms, 4), "bytes": int(nbytes), "gb_s": round(gbs, 1),
-                        "frac": round(gbs / HBM_PEAK_GBS, 4)}
+        return {"ms": round(ms, 4), "bytes": int(nbytes), "gb_s": round(gbs, 1),
+                "frac": round(gbs / HBM_PEAK_GBS, 4), "traffic": int(traffic),
+                "traffic_gb_s": round(traffic / (ms * 1e-3) / 1e9, 1)}
+
+    stages = {name: stage_entry(ms, nbytes, traffic) for name, ms, nbytes, traffic in timings}
     log("preprocessing stages: " + ", ".join(f"{k} {v['ms']:.3f} ms {v['gb_s']:.0f} GB/s" for k, v in stages.items()))
     sm_stages = None
     if not args.no_research:
@@ -513,11 +515,9 @@ def main():
         t_sm = []
         pre_sm = pipeline.preprocess(ctx, raw, hull_kind="sm", timings=t_sm)
         sm_stages = {}
-        for name, ms, nbytes in t_sm:
+        for name, ms, nbytes, traffic in t_sm:
             if name in ("sm_counts", "sm_edge", "hull_select"):
-                gbs = nbytes / (ms * 1e-3) / 1e9
-                sm_stages[name] = {"ms": round(ms, 4), "bytes": int(nbytes), "gb_s": round(gbs, 1),
-                                   "frac": round(gbs / HBM_PEAK_GBS, 4)}
+                sm_stages[name] = stage_entry(ms, nbytes, traffic)
         sm_stages["hull_voxels"] = int(np.unpackbits(pre_sm.hull.cpu().numpy().view(np.uint8)).sum())
         del pre_sm
         log("SM hull stages: " + ", ".join(f"{k} {v['ms']:.3f} ms" for k, v in sm_stages.items() if isinstance(v, dict)))
@@ -814,9 +814,12 @@ def main():
                       "note": "the MLP pass and the block loop overlap chunk by chunk; alone they take "
                               f"{mlp_alone_ms:.2f} and {blk_alone_ms:.2f} ms",
                       "plan_setup_once": round(t_plan * 1e3, 1), "preprocess_once": round(t_pre * 1e3, 1)},
-        "preprocess_stages": {"note": "each stage alone between two HIP events; bytes = arrays read and written "
-                                      "once at their stored width (pct_amd/pipeline.py STAGE_BYTES); frac of "
-                                      f"{HBM_PEAK_GBS:g} GB/s", **stages},
+        "preprocess_stages": {"note": "each stage alone between two HIP events; bytes = the arrays the "
+                                      "reference's kernels for the stage read and write, once at their stored "
+                                      "width (pct_amd/pipeline.py STAGE_BYTES), gb_s and frac of "
+                                      f"{HBM_PEAK_GBS:g} GB/s from them; traffic = the bytes this implementation's "
+                                      "own arrays move (sort passes, records, cut windows; STAGE_TRAFFIC)",
+                              **stages},
         "precision_form": precision,
         "batched_form": batched,
         "learned_form": learned_res,

@@ -285,7 +285,7 @@ def test_timed_pipeline_gives_the_same_results():
     assert names == ["recon_volume_intersections", "binning", "msc_counts", "bin_statistics", "statistical_cuts",
                      "compact_post_cut", "fbp_filter", "backproject", "msc_edge", "hull_select", "initial_iterate",
                      "mlp_endpoints", "compact_reconstruction"]
-    assert all(ms > 0 and nbytes > 0 for _, ms, nbytes in timings)
+    assert all(ms > 0 and nbytes > 0 and traffic > 0 for _, ms, nbytes, traffic in timings)
     assert a.hist.n == b.hist.n and a.counts == b.counts
     assert np.array_equal(a.x0.cpu().numpy().view(np.uint32), b.x0.cpu().numpy().view(np.uint32))
     assert np.array_equal(a.hull.cpu().numpy(), b.hull.cpu().numpy())
