@@ -25,6 +25,7 @@ int pct_mlp_paths_on(pct_ctx* ctx, hipStream_t stream, const pct_hist_soa* hist,
                      const int64_t* offsets, uint32_t* voxels, uint32_t* lengths, double* chord, uint32_t* hash_out,
                      const uint32_t* hash_expect, int mode, bool beside);
 int pct_status_to_rc(pct_ctx* ctx, int flags);
+int pct_mlp_fallback_reserve(pct_ctx* ctx, int64_t n);
 
 namespace {
 
@@ -316,6 +317,12 @@ extern "C" int pct_batch_iteration(pct_batch* b, const pct_hist_soa* hists, floa
             }
             PCT_HIP_CHECK(c0, hipMemcpyAsync(cs->x_saved.ptr, xs[s], sizeof(float) * (size_t)cs->g.V,
                                              hipMemcpyDeviceToDevice, c0->stream));
+            /* the surrogate's fallback list sized for the whole scan before
+             * any chunk is queued (growing it later would wait for the device) */
+            if (mlp == PCT_MLP_SURROGATE && (rc = pct_mlp_fallback_reserve(cs, cs->plan->n))) {
+                pct_fail(c0, rc, "scan %d: %s", s, pct_last_error(cs));
+                return rc;
+            }
         }
     }
     if (c0->profile) PCT_HIP_CHECK(c0, hipEventRecord(c0->ev[0], c0->stream));

@@ -84,6 +84,8 @@ struct pct_ctx {
     float* d_u1f = nullptr;
     int net_version = 0;
     pct_buffer net_partial, net_state, net_loss, net_calib, net_stage_buf;
+    /* the surrogate walk's fallback list: u64 count, indices from byte 256 */
+    pct_buffer net_fallback;
     /* execution knobs (pct_set_tuning; 0 = default) */
     int tune_mlp_waves = 0, tune_chunk_first = 0, tune_chunk_max = 0, tune_overlap = 0, tune_block_form = 0;
     int tune_plan_limit_mb = 0;

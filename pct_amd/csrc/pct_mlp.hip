@@ -607,14 +607,30 @@ struct table_walk {
  * per-depth table (7 FP64 fields per step, read by every step of every
  * history) is staged in LDS once per workgroup, so that the steps' table
  * reads stay off the vector memory path that the block kernels running
- * beside the walk depend on. */
-template <class W, bool LDS_TAB>
+ * beside the walk depend on. With IDX the launch walks histories
+ * idx[0 .. *idx_count) of the range instead (the surrogate form's fallbacks),
+ * each written where the direct launch would write it, in groups of G <= 64
+ * per wavefront chosen from the list's length so that a short list spreads
+ * over the waves instead of queueing in a few. */
+template <class W, bool LDS_TAB, bool IDX = false>
 __global__ void __launch_bounds__(256)
 k_mlp_paths(pct_geo g, pct_hist_soa h, int64_t start, int64_t count, W walk,
             const int64_t* __restrict__ offsets, uint32_t* __restrict__ voxels, uint32_t* __restrict__ lengths,
             double* __restrict__ chord_out, int* __restrict__ status, uint32_t* __restrict__ hash_out,
-            const uint32_t* __restrict__ hash_expect) {
+            const uint32_t* __restrict__ hash_expect, const uint32_t* __restrict__ idx,
+            const unsigned long long* __restrict__ idx_count) {
     extern __shared__ double s_mlp_tab[];
+    const int64_t waves = (int64_t)gridDim.x * (blockDim.x >> 6);
+    int G = 64;
+    if (IDX) {
+        /* a list launch is sized for the whole range: the group size follows
+         * the list's length, and workgroups past its end leave before
+         * staging the table */
+        count = (int64_t)*idx_count;
+        const int64_t gsz = (count + waves - 1) / waves;
+        G = gsz < 1 ? 1 : (gsz > 64 ? 64 : (int)gsz);
+        if ((int64_t)blockIdx.x * (blockDim.x >> 6) * G >= count) return;
+    }
     if (LDS_TAB) {
         const int n = MLP_TAB_FIELDS * walk.K;
         for (int i = threadIdx.x; i < n; i += blockDim.x) s_mlp_tab[i] = walk.tab[i];
@@ -625,16 +641,18 @@ k_mlp_paths(pct_geo g, pct_hist_soa h, int64_t start, int64_t count, W walk,
     const int lane = threadIdx.x & 63;
     const uint64_t lt_mask = (lane == 0) ? 0ull : (~0ull >> (64 - lane));
     int local_flags = 0;
-    /* grid-stride over groups of 64 histories: the grid is sized to the
+    /* grid-stride over groups of G histories: the grid is sized to the
      * resident waves, so a launch has no partial last round */
-    for (int64_t base = ((int64_t)blockIdx.x * blockDim.x + (threadIdx.x & ~63)); base < count;
-         base += (int64_t)gridDim.x * blockDim.x) {
-    const int64_t mine = base + lane;
+    for (int64_t base = ((int64_t)blockIdx.x * (blockDim.x >> 6) + (threadIdx.x >> 6)) * G; base < count;
+         base += waves * G) {
+    /* list position base + lane; its history (relative to start) is mine */
+    const bool have = lane < G && base + lane < count;
+    const int64_t mine = (IDX && have) ? (int64_t)idx[base + lane] : base + lane;
     typename W::H m;
     int mS = 0;
     uint32_t first = 0;
     int64_t off = 0, cap = 0;
-    if (mine < count) {
+    if (have) {
         double chord;
         walk.setup(g, h, start + mine, m, chord, mS);
         first = h.first_voxel[start + mine];
@@ -644,9 +662,10 @@ k_mlp_paths(pct_geo g, pct_hist_soa h, int64_t start, int64_t count, W walk,
         if (chord_out) chord_out[mine] = chord;
         if (mS >= K) atomicOr(status, PCT_STATUS_TABLE);
     }
-    const int nh = (int)((count - base) < 64 ? (count - base) : 64);
+    const int nh = (int)((count - base) < G ? (count - base) : G);
     for (int j = 0; j < nh; ++j) {
         typename W::H b = walk.bcast(m, j);
+        const int64_t hj = IDX ? bcast_l(mine, j) : base + j;
         int S = bcast_i(mS, j);
         if (S > K) S = K;
         uint32_t carry = (uint32_t)bcast_i((int)first, j);
@@ -658,14 +677,14 @@ k_mlp_paths(pct_geo g, pct_hist_soa h, int64_t start, int64_t count, W walk,
                   hcap, voxels, lane, lt_mask, n, hsh);
         if (n > hcap) local_flags |= PCT_STATUS_CAPACITY;
         if (n > g.max_intersections) local_flags |= PCT_STATUS_OVERFLOW;
-        store_length(lengths, base + j, n, hcap, lane);
+        store_length(lengths, hj, n, hcap, lane);
         if (hash_out || hash_expect) {
 #pragma unroll
             for (int d = 32; d >= 1; d >>= 1) hsh += (uint32_t)__shfl_xor((int)hsh, d, 64);
             hsh += (uint32_t)n * 0x27d4eb2fu;
             if (lane == 0) {
-                if (hash_out) hash_out[base + j] = hsh;
-                if (hash_expect && hash_expect[base + j] != hsh) local_flags |= PCT_STATUS_PLAN;
+                if (hash_out) hash_out[hj] = hsh;
+                if (hash_expect && hash_expect[hj] != hsh) local_flags |= PCT_STATUS_PLAN;
             }
         }
     }
@@ -824,14 +843,18 @@ __device__ __forceinline__ void sur_wave_setup(const pct_geo& g, const pct_hist_
 
 /* Surrogate MLP paths: like k_mlp_paths, one wavefront per 64 histories; a
  * history the guard accepts is walked on its FP32 surrogate path, any other
- * with the exact FP64 steps. counts[0] += fallbacks, counts[1] += histories. */
+ * is appended to the fallback list fb_idx (*fb_count entries, one atomic per
+ * wavefront) for the exact walk's launch that follows (k_mlp_paths with the
+ * list), so that this kernel holds no FP64 walk state. counts[0] +=
+ * fallbacks, counts[1] += histories. */
 __global__ void __launch_bounds__(256)
 k_mlp_paths_sur(pct_geo g, pct_hist_soa h, int64_t start, int64_t count, const double* __restrict__ tab, int K,
                 const float* __restrict__ u1f, grid_f gf, const pct_net_frag* __restrict__ nf,
                 const float* __restrict__ range, const uint8_t* __restrict__ cells, const int64_t* __restrict__ offsets,
                 uint32_t* __restrict__ voxels, uint32_t* __restrict__ lengths, double* __restrict__ chord_out,
                 int* __restrict__ status, uint32_t* __restrict__ hash_out, const uint32_t* __restrict__ hash_expect,
-                unsigned long long* __restrict__ counts) {
+                unsigned long long* __restrict__ counts, uint32_t* __restrict__ fb_idx,
+                unsigned long long* __restrict__ fb_count) {
     __shared__ __align__(16) sur_lds lds[4];
     extern __shared__ float u1s[];          /* the K step depths, staged once per workgroup */
     for (int k = threadIdx.x; k < K; k += blockDim.x) u1s[k] = u1f[k];
@@ -864,8 +887,16 @@ k_mlp_paths_sur(pct_geo g, pct_hist_soa h, int64_t start, int64_t count, const d
         const uint64_t umask = __ballot(use);
         const int nh = (int)((count - base) < 64 ? (count - base) : 64);
         tot += nh;
-        fb += nh - __popcll(umask);
-        for (int j = 0; j < nh; ++j) {
+        const uint64_t fmask = __ballot(mine < count && !use);
+        if (fmask) {
+            unsigned long long at = 0;
+            if (lane == 0) at = atomicAdd(fb_count, (unsigned long long)__popcll(fmask));
+            at = (unsigned long long)bcast_l((int64_t)at, 0);
+            if ((fmask >> lane) & 1) fb_idx[at + __popcll(fmask & lt_mask)] = (uint32_t)mine;
+            fb += __popcll(fmask);
+        }
+        for (uint64_t rest = umask; rest; rest &= rest - 1) {
+            const int j = __ffsll((unsigned long long)rest) - 1;
             int S = bcast_i(mS, j);
             if (S > K) S = K;
             uint32_t carry = (uint32_t)bcast_i((int)first, j);
@@ -873,19 +904,12 @@ k_mlp_paths_sur(pct_geo g, pct_hist_soa h, int64_t start, int64_t count, const d
             int64_t hcap = bcast_l(cap, j);
             int64_t n;
             uint32_t hsh;
-            if ((umask >> j) & 1) {
-                sur_hist b = sur_bcast(sh, j);
-                if (lane < 40) L.Q[lane] = sur_coef(lane, b, L.out + 8 * j);
-                wave_lds_sync();
-                auto st = [&](int k) { return sur_step(g, gf, b, L.Q, u1s, k); };
-                walk_list(st, st, S, carry, hoff, hcap, voxels, lane, lt_mask, n, hsh);
-                wave_lds_sync();
-            } else {
-                exact_walk ew{tab, K};
-                mlp_hist b = ew.bcast(m, j);
-                auto st = [&](int k) { return mlp_step(g, b, tab, K, k); };
-                walk_list(st, st, S, carry, hoff, hcap, voxels, lane, lt_mask, n, hsh);
-            }
+            sur_hist b = sur_bcast(sh, j);
+            if (lane < 40) L.Q[lane] = sur_coef(lane, b, L.out + 8 * j);
+            wave_lds_sync();
+            auto st = [&](int k) { return sur_step(g, gf, b, L.Q, u1s, k); };
+            walk_list(st, st, S, carry, hoff, hcap, voxels, lane, lt_mask, n, hsh);
+            wave_lds_sync();
             if (n > hcap) local_flags |= PCT_STATUS_CAPACITY;
             if (n > g.max_intersections) local_flags |= PCT_STATUS_OVERFLOW;
             store_length(lengths, base + j, n, hcap, lane);
@@ -1155,12 +1179,19 @@ int pct_mlp_capacity_impl(pct_ctx* ctx, const pct_hist_soa* hist, int64_t start,
     return pct_exclusive_scan_i32(ctx, steps, count, offsets, total);
 }
 
+__global__ void k_zero_u64(unsigned long long* __restrict__ p) {
+    if (threadIdx.x == 0) *p = 0ull;
+}
+
 /* internal: the MLP lists of histories [start, start+count) on `stream`;
  * offsets/lengths/chord/hash arrays are indexed from `start` (entry 0 is
- * history start) */
-int pct_mlp_paths_on(pct_ctx* ctx, hipStream_t stream, const pct_hist_soa* hist, int64_t start, int64_t count,
+ * history start). The surrogate form lists its fallback histories in fb_idx
+ * (room for count entries) and their number in *fb_count, then walks them
+ * with the exact walk's kernel. */
+int pct_mlp_paths_fb(pct_ctx* ctx, hipStream_t stream, const pct_hist_soa* hist, int64_t start, int64_t count,
                      const int64_t* offsets, uint32_t* voxels, uint32_t* lengths, double* chord, uint32_t* hash_out,
-                     const uint32_t* hash_expect, int mode, bool beside) {
+                     const uint32_t* hash_expect, int mode, bool beside, uint32_t* fb_idx,
+                     unsigned long long* fb_count) {
     int rc = ensure_mlp_table(ctx);
     if (rc) return rc;
     if (!pct_mlp_mode_ok(ctx, mode))
@@ -1176,44 +1207,94 @@ int pct_mlp_paths_on(pct_ctx* ctx, hipStream_t stream, const pct_hist_soa* hist,
      * the surrogate at 1; the batched form's steps do better with the full
      * occupancy) */
     static const int beside_default[3] = {3, 2, 1};
-    int per_cu = std::max(1, ctx->mlp_occ[mode]);
-    int lim = ctx->tune_mlp_waves > 0 ? ctx->tune_mlp_waves : (beside ? beside_default[mode] : 0);
-    if (lim > 0 && lim < per_cu) per_cu = lim;
-    if (mode == PCT_MLP_SURROGATE) {
-        const unsigned grid = std::min(pct_grid(count, 256), (unsigned)(ctx->cus * per_cu));
-        hipLaunchKernelGGL(k_mlp_paths_sur, dim3(grid), dim3(256), sizeof(float) * ctx->mlp_tab_k, stream, ctx->g, *hist,
-                           start, count,
-                           (const double*)ctx->d_mlp_tab, ctx->mlp_tab_k, (const float*)ctx->d_u1f, make_grid_f(ctx->g),
-                           (const pct_net_frag*)ctx->d_net, (const float*)ctx->d_net_range,
-                           (const uint8_t*)ctx->d_net_cells, offsets, voxels, lengths, chord, ctx->d_status, hash_out,
-                           hash_expect, ctx->d_net_counts);
-    } else {
-        /* the per-depth table in LDS when the workgroups this launch keeps on
-         * a CU fit in its 160 KB beside their registers' limit */
+    /* workgroups per CU of a walk launch: the walk's occupancy, capped beside
+     * the block loop or by the knob; for the table walks the per-depth table
+     * goes to LDS when the workgroups the launch keeps on a CU fit in its
+     * 160 KB beside their registers' limit */
+    auto walk_shape = [&](int walk_mode, int& per_cu, bool& lds, size_t& shm) {
+        per_cu = std::max(1, ctx->mlp_occ[walk_mode]);
+        const int lim = ctx->tune_mlp_waves > 0 ? ctx->tune_mlp_waves : (beside ? beside_default[walk_mode] : 0);
+        if (lim > 0 && lim < per_cu) per_cu = lim;
+        lds = false;
+        shm = 0;
+        if (walk_mode == PCT_MLP_SURROGATE) return;
         const size_t tab_bytes = sizeof(double) * MLP_TAB_FIELDS * (size_t)ctx->mlp_tab_k;
-        bool lds = tab_bytes <= PCT_MLP_LDS_TAB_MAX && tab_bytes * (size_t)per_cu <= PCT_LDS_PER_CU;
+        lds = tab_bytes <= PCT_MLP_LDS_TAB_MAX && tab_bytes * (size_t)per_cu <= PCT_LDS_PER_CU;
         if (!lds && tab_bytes <= PCT_MLP_LDS_TAB_MAX && PCT_LDS_PER_CU / tab_bytes >= 3) {
             /* a larger grid's table: one wavefront per SIMD less, table in LDS */
             per_cu = (int)(PCT_LDS_PER_CU / tab_bytes);
             lds = true;
         }
+        shm = lds ? tab_bytes : 0;
+    };
+    auto exact_launch = [&](const uint32_t* idx, const unsigned long long* idx_count) {
+        int per_cu;
+        bool lds;
+        size_t shm;
+        walk_shape(PCT_MLP_FP64, per_cu, lds, shm);
         const unsigned tgrid = std::min(pct_grid(count, 256), (unsigned)(ctx->cus * per_cu));
-        const size_t shm = lds ? tab_bytes : 0;
-        if (mode == PCT_MLP_FP64) {
-            exact_walk w{(const double*)ctx->d_mlp_tab, ctx->mlp_tab_k};
+        exact_walk w{(const double*)ctx->d_mlp_tab, ctx->mlp_tab_k};
+        if (idx)
+            hipLaunchKernelGGL((lds ? k_mlp_paths<exact_walk, true, true> : k_mlp_paths<exact_walk, false, true>),
+                               dim3(tgrid), dim3(256), shm, stream, ctx->g, *hist, start, count, w, offsets, voxels,
+                               lengths, chord, ctx->d_status, hash_out, hash_expect, idx, idx_count);
+        else
             hipLaunchKernelGGL((lds ? k_mlp_paths<exact_walk, true> : k_mlp_paths<exact_walk, false>), dim3(tgrid),
                                dim3(256), shm, stream, ctx->g, *hist, start, count, w, offsets, voxels, lengths, chord,
-                               ctx->d_status, hash_out, hash_expect);
-        } else {
-            table_walk w{(const double*)ctx->d_mlp_tab, ctx->mlp_tab_k, (const double*)ctx->d_scat_tab, ctx->scat_q,
-                         make_grid_f(ctx->g)};
-            hipLaunchKernelGGL((lds ? k_mlp_paths<table_walk, true> : k_mlp_paths<table_walk, false>), dim3(tgrid),
-                               dim3(256), shm, stream, ctx->g, *hist, start, count, w, offsets, voxels, lengths, chord,
-                               ctx->d_status, hash_out, hash_expect);
-        }
+                               ctx->d_status, hash_out, hash_expect, nullptr, nullptr);
+    };
+    if (mode == PCT_MLP_SURROGATE) {
+        if (!fb_idx || !fb_count) return pct_fail(ctx, PCT_E_INVALID, "surrogate MLP paths without a fallback list");
+        int per_cu;
+        bool lds;
+        size_t shm;
+        walk_shape(PCT_MLP_SURROGATE, per_cu, lds, shm);
+        const unsigned grid = std::min(pct_grid(count, 256), (unsigned)(ctx->cus * per_cu));
+        hipLaunchKernelGGL(k_zero_u64, dim3(1), dim3(64), 0, stream, fb_count);
+        hipLaunchKernelGGL(k_mlp_paths_sur, dim3(grid), dim3(256), sizeof(float) * ctx->mlp_tab_k, stream, ctx->g, *hist,
+                           start, count,
+                           (const double*)ctx->d_mlp_tab, ctx->mlp_tab_k, (const float*)ctx->d_u1f, make_grid_f(ctx->g),
+                           (const pct_net_frag*)ctx->d_net, (const float*)ctx->d_net_range,
+                           (const uint8_t*)ctx->d_net_cells, offsets, voxels, lengths, chord, ctx->d_status, hash_out,
+                           hash_expect, ctx->d_net_counts, fb_idx, fb_count);
+        exact_launch(fb_idx, fb_count);
+    } else if (mode == PCT_MLP_FP64) {
+        exact_launch(nullptr, nullptr);
+    } else {
+        int per_cu;
+        bool lds;
+        size_t shm;
+        walk_shape(PCT_MLP_TABLE, per_cu, lds, shm);
+        const unsigned tgrid = std::min(pct_grid(count, 256), (unsigned)(ctx->cus * per_cu));
+        table_walk w{(const double*)ctx->d_mlp_tab, ctx->mlp_tab_k, (const double*)ctx->d_scat_tab, ctx->scat_q,
+                     make_grid_f(ctx->g)};
+        hipLaunchKernelGGL((lds ? k_mlp_paths<table_walk, true> : k_mlp_paths<table_walk, false>), dim3(tgrid),
+                           dim3(256), shm, stream, ctx->g, *hist, start, count, w, offsets, voxels, lengths, chord,
+                           ctx->d_status, hash_out, hash_expect, nullptr, nullptr);
     }
     PCT_HIP_CHECK(ctx, hipGetLastError());
     return PCT_OK;
+}
+
+/* room for the surrogate form's fallback list of n histories in the
+ * context (its count in the first 8 bytes, the indices from byte 256 on) */
+int pct_mlp_fallback_reserve(pct_ctx* ctx, int64_t n) {
+    return pct_ensure(ctx, ctx->net_fallback, 256 + sizeof(uint32_t) * (size_t)(n > 0 ? n : 1));
+}
+
+int pct_mlp_paths_on(pct_ctx* ctx, hipStream_t stream, const pct_hist_soa* hist, int64_t start, int64_t count,
+                     const int64_t* offsets, uint32_t* voxels, uint32_t* lengths, double* chord, uint32_t* hash_out,
+                     const uint32_t* hash_expect, int mode, bool beside) {
+    uint32_t* fb_idx = nullptr;
+    unsigned long long* fb_count = nullptr;
+    if (mode == PCT_MLP_SURROGATE && count > 0) {
+        int rc = pct_mlp_fallback_reserve(ctx, count);
+        if (rc) return rc;
+        fb_count = (unsigned long long*)ctx->net_fallback.ptr;
+        fb_idx = (uint32_t*)((char*)ctx->net_fallback.ptr + 256);
+    }
+    return pct_mlp_paths_fb(ctx, stream, hist, start, count, offsets, voxels, lengths, chord, hash_out, hash_expect,
+                            mode, beside, fb_idx, fb_count);
 }
 
 int pct_mlp_paths_impl(pct_ctx* ctx, const pct_hist_soa* hist, int64_t start, int64_t count, const int64_t* offsets,

@@ -450,7 +450,8 @@ int pct_net_release(pct_ctx* ctx) {
     void* ptrs[] = {ctx->d_net, ctx->d_net_range, ctx->d_net_cells, ctx->d_net_counts, ctx->d_u1f};
     for (void* p : ptrs)
         if (p) (void)hipFree(p);
-    pct_buffer* bufs[] = {&ctx->net_partial, &ctx->net_state, &ctx->net_loss, &ctx->net_calib, &ctx->net_stage_buf};
+    pct_buffer* bufs[] = {&ctx->net_partial, &ctx->net_state, &ctx->net_loss, &ctx->net_calib, &ctx->net_stage_buf,
+                          &ctx->net_fallback};
     for (pct_buffer* b : bufs)
         if (b->ptr) (void)hipFree(b->ptr);
     ctx->d_net = nullptr;

@@ -13,7 +13,8 @@
  * slot range of the context's max_intersections entries (a longer path is an
  * overflow error in both entry points), so no prefix sum and no sort storage
  * is needed either. The per-depth MLP tables and the pinned status word the
- * call reads back are made by pct_create.
+ * call reads back are made by pct_create. The surrogate form's fallback list
+ * (the histories its guard sends to the exact walk) is in the same array.
  *
  * The array's address on a device comes from hipGetSymbolAddress, called once
  * per device with that device current (pct_create, through
@@ -24,9 +25,10 @@
 #include <mutex>
 #include "pct_internal.h"
 
-int pct_mlp_paths_impl(pct_ctx* ctx, const pct_hist_soa* hist, int64_t start, int64_t count, const int64_t* offsets,
-                       uint32_t* voxels, uint32_t* lengths, double* chord, uint32_t* hash_out,
-                       const uint32_t* hash_expect, int mode);
+int pct_mlp_paths_fb(pct_ctx* ctx, hipStream_t stream, const pct_hist_soa* hist, int64_t start, int64_t count,
+                     const int64_t* offsets, uint32_t* voxels, uint32_t* lengths, double* chord, uint32_t* hash_out,
+                     const uint32_t* hash_expect, int mode, bool beside, uint32_t* fb_idx,
+                     unsigned long long* fb_count);
 int pct_drop_launch_scatter(pct_ctx* ctx, const uint32_t* voxels, const int64_t* offsets, const uint32_t* lengths,
                             const double* chord, const float* wepl, int64_t count, const float* x, float* x_update,
                             uint32_t* counts, float lambda);
@@ -36,7 +38,9 @@ int pct_status_to_rc(pct_ctx* ctx, int flags);
 #define NA_OFFSETS 0                                                     /* int64 [MAX_H + 1] */
 #define NA_CHORD (((8 * (PCT_NOALLOC_MAX_HISTORIES + 1)) + 255) & ~255)   /* f64 [MAX_H]       */
 #define NA_LENGTHS (NA_CHORD + 8 * PCT_NOALLOC_MAX_HISTORIES)            /* u32 [MAX_H]       */
-#define NA_VOXELS (NA_LENGTHS + 4 * PCT_NOALLOC_MAX_HISTORIES)           /* u32 [MAX_H x MAX_SLOTS] */
+#define NA_FB_COUNT (NA_LENGTHS + 4 * PCT_NOALLOC_MAX_HISTORIES)         /* u64: surrogate fallbacks */
+#define NA_FB_IDX (NA_FB_COUNT + 256)                                    /* u32 [MAX_H]       */
+#define NA_VOXELS (NA_FB_IDX + 4 * PCT_NOALLOC_MAX_HISTORIES)            /* u32 [MAX_H x MAX_SLOTS] */
 #define NA_BYTES (NA_VOXELS + 4 * PCT_NOALLOC_MAX_HISTORIES * PCT_NOALLOC_MAX_SLOTS)
 
 static __device__ __align__(256) unsigned char scratch[NA_BYTES];
@@ -104,10 +108,13 @@ extern "C" int pct_drop_block_update_noalloc(pct_ctx* ctx, const pct_hist_soa* h
     double* chord = (double*)(base + NA_CHORD);
     uint32_t* lengths = (uint32_t*)(base + NA_LENGTHS);
     uint32_t* voxels = (uint32_t*)(base + NA_VOXELS);
+    unsigned long long* fb_count = (unsigned long long*)(base + NA_FB_COUNT);
+    uint32_t* fb_idx = (uint32_t*)(base + NA_FB_IDX);
     hipLaunchKernelGGL(k_noalloc_offsets, dim3(pct_grid(count + 1, 256)), dim3(256), 0, ctx->stream, offsets, count,
                        stride);
     PCT_HIP_CHECK(ctx, hipGetLastError());
-    int rc = pct_mlp_paths_impl(ctx, hist, start, count, offsets, voxels, lengths, chord, nullptr, nullptr, mode);
+    int rc = pct_mlp_paths_fb(ctx, ctx->stream, hist, start, count, offsets, voxels, lengths, chord, nullptr, nullptr,
+                              mode, false, fb_idx, fb_count);
     if (rc) return rc;
     rc = pct_drop_launch_scatter(ctx, voxels, offsets, lengths, chord, hist->wepl + start, count, x, x_update, counts,
                                  lambda);

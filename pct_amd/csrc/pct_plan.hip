@@ -55,6 +55,7 @@ int pct_mlp_paths_impl(pct_ctx* ctx, const pct_hist_soa* hist, int64_t start, in
                        uint32_t* voxels, uint32_t* lengths, double* chord, uint32_t* hash_out,
                        const uint32_t* hash_expect, int mode);
 int pct_status_to_rc(pct_ctx* ctx, int flags);
+int pct_mlp_fallback_reserve(pct_ctx* ctx, int64_t n);
 int pct_drop_iteration_atomic(pct_ctx* ctx, const pct_hist_soa* hist, float* x, int block_size, float lambda,
                               int mode);
 int pct_drop_iteration_chunked(pct_ctx* ctx, const pct_hist_soa* hist, float* x, int block_size, float lambda,
@@ -981,6 +982,9 @@ static int plan_iterate(pct_ctx* ctx, pct_plan* pl, const pct_hist_soa* hist, fl
         }
     } else if (recompute) {
         if ((rc = plan_ensure_events(ctx, pl, nchunks))) return rc;
+        /* the surrogate's fallback list sized for the whole scan before any
+         * chunk is queued (growing it later would wait for the device) */
+        if (mlp == PCT_MLP_SURROGATE && (rc = pct_mlp_fallback_reserve(ctx, pl->n))) return rc;
         hipStream_t aux = ctx->aux_stream;
         PCT_HIP_CHECK(ctx, hipEventRecord(pl->ev_start, ctx->stream));
         PCT_HIP_CHECK(ctx, hipStreamWaitEvent(aux, pl->ev_start, 0));
