@@ -35,9 +35,16 @@
 
 #if defined(__HIPCC__)
 #define PCT_HD __host__ __device__ inline
+#define PCT_HD_MEMBER __host__ __device__
 #else
 #define PCT_HD static inline
+#define PCT_HD_MEMBER
 #endif
+
+/* the IEEE quotient a / b: the division pct_log_div takes in pct_log */
+struct pct_div_ieee {
+    PCT_HD_MEMBER double operator()(double a, double b) const { return a / b; }
+};
 
 PCT_HD uint64_t pct_bits(double x) { return __builtin_bit_cast(uint64_t, x); }
 PCT_HD double pct_from_bits(uint64_t u) { return __builtin_bit_cast(double, u); }
@@ -66,7 +73,10 @@ PCT_HD double pct_round_int(double x) {
 #define PCT_LN2_LO  1.9082149292705877e-10
 
 /* natural logarithm, x > 0 finite (x <= 0 returns -inf / NaN like libm) */
-PCT_HD double pct_log(double x) {
+/* natural logarithm; div(a, b) must return the IEEE quotient a / b for
+ * a = f in [-0.293, 0.415] (or +0) and b = 2 + f, the one division */
+template <class Div>
+PCT_HD double pct_log_div(double x, Div div) {
     if (!(x > 0.0)) return x == 0.0 ? -__builtin_inf() : __builtin_nan("");
     if (pct_bits(x) >= 0x7ff0000000000000ULL) return x; /* +inf */
     int k = 0;
@@ -77,13 +87,14 @@ PCT_HD double pct_log(double x) {
     if (m > 1.4142135623730951) { m = m * 0.5; k += 1; }
     double f = m - 1.0;
     double h = 0.5 * f * f;
-    double s = f / (2.0 + f);
+    double s = div(f, 2.0 + f);
     double z = s * s;
     double R = z * (2.0 / 3.0 + z * (2.0 / 5.0 + z * (2.0 / 7.0 + z * (2.0 / 9.0 + z * (2.0 / 11.0
              + z * (2.0 / 13.0 + z * (2.0 / 15.0 + z * (2.0 / 17.0 + z * (2.0 / 19.0 + z * (2.0 / 21.0))))))))));
     double dk = (double)k;
     return dk * PCT_LN2_HI - ((h - (s * (h + R) + dk * PCT_LN2_LO)) - f);
 }
+PCT_HD double pct_log(double x) { return pct_log_div(x, pct_div_ieee{}); }
 
 /* sin and cos of r, |r| <= ~pi/4 (Taylor to degree 17 / 18) */
 PCT_HD double pct_sin_kernel(double r) {

@@ -268,21 +268,83 @@ __device__ __forceinline__ void mlp_setup(const pct_geo& g, const pct_hist_soa& 
     m.S = steps_below(u1tab, K, u_2 - g.mlp_u_step);
 }
 
+/* FP64 quotients with one reciprocal per denominator. The compiler's
+ * correctly rounded num / den is: v_div_scale of both operands (a power-of-two
+ * scaling when their exponents are extreme), r = v_rcp(den) refined by two
+ * Newton steps, q = num r, e = fma(-den, q, num), v_div_fmas (fma(e, r, q)
+ * when nothing was scaled), then v_div_fixup (zeros, infinities, NaNs,
+ * overflow). For operands whose magnitudes lie in [2^-300, 2^300] (div_ok)
+ * the scaling, the fmas' rescale and the fixup change nothing, so the same
+ * arithmetic with r formed once per denominator gives the compiler's quotient
+ * bit for bit; with a zero numerator and a positive denominator both give +0. */
+struct rdiv {
+    double den, r;
+};
+
+__device__ __forceinline__ rdiv rdiv_make(double den) {
+    double r = __builtin_amdgcn_rcp(den);
+    double e = __builtin_fma(-den, r, 1.0);
+    r = __builtin_fma(r, e, r);
+    e = __builtin_fma(-den, r, 1.0);
+    r = __builtin_fma(r, e, r);
+    return rdiv{den, r};
+}
+
+__device__ __forceinline__ double rdiv_q(const rdiv& d, double num) {
+    const double q = num * d.r;
+    const double e = __builtin_fma(-d.den, q, num);
+    return __builtin_fma(e, d.r, q);
+}
+
+__device__ __forceinline__ bool div_ok(double v) {
+    const double a = __builtin_fabs(v);
+    return (a >= 0x1p-300) & (a <= 0x1p300);
+}
+
+/* the logarithm's one division: f / (2 + f) with f in [-0.293, 0.415] or +0,
+ * always inside the range above (or the zero case) */
+struct div_rcp {
+    __device__ double operator()(double a, double b) const { return rdiv_q(rdiv_make(b), a); }
+};
+
 /* the u_2-dependent part of one MLP step (.cu:5402-5472): (t_1, v_1) from the
- * u_1-only terms S1I* (Sigma_1^-1) and cst* (the common sigma_2 polynomials) */
-__device__ __forceinline__ void mlp_tv_terms(const pct_geo& g, const mlp_hist& m, double u_1, double S1I0, double S1I1,
+ * u_1-only terms S1I* (Sigma_1^-1) and cst* (the common sigma_2 polynomials).
+ * With FAST the seven quotients share their reciprocals (rdiv) and the return
+ * value says whether every operand was inside div_ok's range -- when it is
+ * not, the caller evaluates the step again without FAST. */
+template <bool FAST = false>
+__device__ __forceinline__ bool mlp_tv_terms(const pct_geo& g, const mlp_hist& m, double u_1, double S1I0, double S1I1,
                                              double S1I3, double cst1, double cst2, double cst3, double& t_out,
                                              double& v_out) {
     double d = m.u_2 - u_1;                       /* R_1[1] = R_1T[2] */
-    double l2 = g.e_0 * (1 + 0.038 * pct_log((m.u_2 - u_1) / g.x_0));
-    double c2 = l2 * l2 / g.x_0;
+    bool ok = true;
+    double l2, c2;
+    if constexpr (FAST) {
+        const rdiv x0 = rdiv_make(g.x_0);
+        l2 = g.e_0 * (1 + 0.038 * pct_log_div(rdiv_q(x0, m.u_2 - u_1), div_rcp{}));
+        const double l2sq = l2 * l2;
+        c2 = rdiv_q(x0, l2sq);
+        ok = div_ok(g.x_0) & div_ok(d) & div_ok(l2sq);
+    } else {
+        l2 = g.e_0 * (1 + 0.038 * pct_log((m.u_2 - u_1) / g.x_0));
+        c2 = l2 * l2 / g.x_0;
+    }
     double sigma_t2 = c2 * (m.pre1 - m.u_2 * m.u_2 * cst1 + 2 * m.u_2 * cst2 - cst3);
     double sigma_t2_theta2 = c2 * (m.pre2 - m.u_2 * cst1 + cst2);
     double sigma_theta2 = c2 * (m.pre3 - cst1);
     double det2 = sigma_t2 * sigma_theta2 - sigma_t2_theta2 * sigma_t2_theta2;
-    double S2I0 = sigma_theta2 / det2;
-    double S2I1 = -sigma_t2_theta2 / det2;          /* = Sigma_2^-1[2] */
-    double S2I3 = sigma_t2 / det2;
+    double S2I0, S2I1, S2I3;
+    if constexpr (FAST) {
+        const rdiv r2 = rdiv_make(det2);
+        S2I0 = rdiv_q(r2, sigma_theta2);
+        S2I1 = rdiv_q(r2, -sigma_t2_theta2);
+        S2I3 = rdiv_q(r2, sigma_t2);
+        ok = ok & div_ok(det2) & div_ok(sigma_theta2) & div_ok(sigma_t2_theta2) & div_ok(sigma_t2);
+    } else {
+        S2I0 = sigma_theta2 / det2;
+        S2I1 = -sigma_t2_theta2 / det2;          /* = Sigma_2^-1[2] */
+        S2I3 = sigma_t2 / det2;
+    }
     /* The reference forms these products with R_1 = {1, d, 0, 1}, R_1T =
      * {1, 0, d, 1}, R_0 = {1, u_1, 0, 1} and T_0 = {t_in, 0}. A product with
      * an exact 1 is the operand itself, and a product with an exact 0 adds a
@@ -297,8 +359,16 @@ __device__ __forceinline__ void mlp_tv_terms(const pct_geo& g, const mlp_hist& m
     double F2 = (S1I1 + d * S2I0) + S2I1;
     double F3 = (S1I3 + d * f24_1) + f24_2;
     double detF = F0 * F3 - F1 * F2;
-    double I0 = F3 / detF;
-    double I1 = -F1 / detF;
+    double I0, I1;
+    if constexpr (FAST) {
+        const rdiv rF = rdiv_make(detF);
+        I0 = rdiv_q(rF, F3);
+        I1 = rdiv_q(rF, -F1);
+        ok = ok & div_ok(detF) & div_ok(F3) & div_ok(F1);
+    } else {
+        I0 = F3 / detF;
+        I1 = -F1 / detF;
+    }
     double a3 = S2I0 * m.t_out + S2I1 * m.dT;
     double a4 = S2I1 * m.t_out + S2I3 * m.dT;
     double s0 = S1I0 * m.t_in + a3;
@@ -311,14 +381,17 @@ __device__ __forceinline__ void mlp_tv_terms(const pct_geo& g, const mlp_hist& m
     double v_1 = I0 * r0 + I1 * r1;
     t_out = t_1;
     v_out = v_1;
+    return ok;
 }
 
-/* (t_1, v_1) of the MLP at table index k */
-__device__ __forceinline__ void mlp_tv(const pct_geo& g, const mlp_hist& m, const double* __restrict__ tab, int K, int k,
-                                       double& u_1, double& t_1, double& v_1) {
+/* (t_1, v_1) of the MLP at table index k (FAST: see mlp_tv_terms) */
+template <bool FAST = false>
+__device__ __forceinline__ bool mlp_tv(const pct_geo& g, const mlp_hist& m, const double* __restrict__ tab, int K,
+                                       int k, double& u_1, double& t_1, double& v_1) {
     u_1 = tab[MLP_TAB_U1 * K + k];
-    mlp_tv_terms(g, m, u_1, tab[MLP_TAB_S1I0 * K + k], tab[MLP_TAB_S1I1 * K + k], tab[MLP_TAB_S1I3 * K + k],
-                 tab[MLP_TAB_C1 * K + k], tab[MLP_TAB_C2 * K + k], tab[MLP_TAB_C3 * K + k], t_1, v_1);
+    return mlp_tv_terms<FAST>(g, m, u_1, tab[MLP_TAB_S1I0 * K + k], tab[MLP_TAB_S1I1 * K + k],
+                              tab[MLP_TAB_S1I3 * K + k], tab[MLP_TAB_C1 * K + k], tab[MLP_TAB_C2 * K + k],
+                              tab[MLP_TAB_C3 * K + k], t_1, v_1);
 }
 
 /* (t_1, v_1) of the MLP at an arbitrary depth u_1: the u_1-only terms
@@ -352,13 +425,14 @@ __device__ __forceinline__ int64_t mlp_step(const pct_geo& g, const mlp_hist& m,
     return voxel;
 }
 
-/* mlp_step with the voxel index from the reciprocal products; returns -2
- * where one of them is too close to a voxel face to decide, and mlp_step
- * then gives the step's voxel */
+/* mlp_step with the shared-reciprocal quotients and the voxel index from the
+ * reciprocal products; returns -2 where an operand is outside the quotients'
+ * range or a product too close to a voxel face to decide, and mlp_step then
+ * gives the step's voxel */
 __device__ __forceinline__ int64_t mlp_step_r(const pct_geo& g, const mlp_hist& m, const double* __restrict__ tab, int K,
                                               int k) {
     double u_1, t_1, v_1;
-    mlp_tv(g, m, tab, K, k, u_1, t_1, v_1);
+    if (!mlp_tv<true>(g, m, tab, K, k, u_1, t_1, v_1)) return -2;
     double x_1 = (m.ca * (m.u_in + u_1)) - (m.sa * t_1);
     double y_1 = (m.sa * (m.u_in + u_1)) + (m.ca * t_1);
     bool near = false;
