@@ -74,7 +74,7 @@ def lib():
 
 
 MATH_FUNCS = {"log": 0, "sin": 1, "cos": 2, "tan": 3, "atan": 4, "atan2": 5, "sinf": 6, "cosf": 7, "atan2f": 8,
-              "sqrt": 9, "div": 10}
+              "sqrt": 9, "div": 10, "div_shared_rcp": 11, "log_rcp_div": 12}
 
 
 def math_eval(func, a, b=None):

@@ -21,6 +21,8 @@ extern "C" void oracle_math(int func, const double* in, const double* in2, doubl
             case 8: r = (double)pct_atan2f((float)a, (float)b); break;
             case 9: r = pct_sqrt(a); break;
             case 10: r = a / b; break;
+            case 11: r = a / b; break;      /* the device's shared-reciprocal quotient */
+            case 12: r = pct_log(a); break; /* the device's log with that division */
             default: r = 0.0; break;
         }
         out[i] = r;

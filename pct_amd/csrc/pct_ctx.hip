@@ -14,6 +14,7 @@
 #include <string.h>
 #include "pct_internal.h"
 #include "pct_plan.h"
+#include "pct_rdiv.h"
 
 /* ------------------------------------------------------------------------- */
 /* parameters                                                                 */
@@ -414,15 +415,17 @@ __global__ void k_selftest_math(int func, const double* __restrict__ in, const d
         case 8: r = (double)pct_atan2f((float)a, (float)b); break;
         case 9: r = pct_sqrt(a); break;
         case 10: r = a / b; break;
+        case 11: r = rdiv_q(rdiv_make(b), a); break;          /* = a / b where div_ok(a), div_ok(b) */
+        case 12: r = pct_log_div(a, div_rcp{}); break;        /* = pct_log(a) */
         default: r = 0.0; break;
     }
     out[i] = r;
 }
 
 extern "C" int pct_selftest_math(pct_ctx* ctx, int func, const double* in, const double* in2, double* out, int64_t n) {
-    if (!ctx || !in || !out || n < 0 || func < 0 || func > 10)
+    if (!ctx || !in || !out || n < 0 || func < 0 || func > 12)
         return pct_fail(ctx, PCT_E_INVALID, "pct_selftest_math: bad argument");
-    if ((func == 5 || func == 8 || func == 10) && !in2)
+    if ((func == 5 || func == 8 || func == 10 || func == 11) && !in2)
         return pct_fail(ctx, PCT_E_INVALID, "pct_selftest_math: function %d needs in2", func);
     if (n == 0) return PCT_OK;
     hipLaunchKernelGGL(k_selftest_math, dim3(pct_grid(n, 256)), dim3(256), 0, ctx->stream, func, in, in2, out, n);

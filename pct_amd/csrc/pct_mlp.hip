@@ -25,6 +25,7 @@
 #include "pct_internal.h"
 #include "pct_net_basis.h"
 #include "pct_net_dev.h"
+#include "pct_rdiv.h"
 
 namespace {
 
@@ -267,45 +268,6 @@ __device__ __forceinline__ void mlp_setup(const pct_geo& g, const pct_hist_soa& 
     m.pre3 = u_2 * (g.A0 + u_2 * (g.a1_2 + u_2 * (g.a2_3 + u_2 * (g.a3_4 + u_2 * (g.a4_5 + u_2 * g.a5_6)))));
     m.S = steps_below(u1tab, K, u_2 - g.mlp_u_step);
 }
-
-/* FP64 quotients with one reciprocal per denominator. The compiler's
- * correctly rounded num / den is: v_div_scale of both operands (a power-of-two
- * scaling when their exponents are extreme), r = v_rcp(den) refined by two
- * Newton steps, q = num r, e = fma(-den, q, num), v_div_fmas (fma(e, r, q)
- * when nothing was scaled), then v_div_fixup (zeros, infinities, NaNs,
- * overflow). For operands whose magnitudes lie in [2^-300, 2^300] (div_ok)
- * the scaling, the fmas' rescale and the fixup change nothing, so the same
- * arithmetic with r formed once per denominator gives the compiler's quotient
- * bit for bit; with a zero numerator and a positive denominator both give +0. */
-struct rdiv {
-    double den, r;
-};
-
-__device__ __forceinline__ rdiv rdiv_make(double den) {
-    double r = __builtin_amdgcn_rcp(den);
-    double e = __builtin_fma(-den, r, 1.0);
-    r = __builtin_fma(r, e, r);
-    e = __builtin_fma(-den, r, 1.0);
-    r = __builtin_fma(r, e, r);
-    return rdiv{den, r};
-}
-
-__device__ __forceinline__ double rdiv_q(const rdiv& d, double num) {
-    const double q = num * d.r;
-    const double e = __builtin_fma(-d.den, q, num);
-    return __builtin_fma(e, d.r, q);
-}
-
-__device__ __forceinline__ bool div_ok(double v) {
-    const double a = __builtin_fabs(v);
-    return (a >= 0x1p-300) & (a <= 0x1p300);
-}
-
-/* the logarithm's one division: f / (2 + f) with f in [-0.293, 0.415] or +0,
- * always inside the range above (or the zero case) */
-struct div_rcp {
-    __device__ double operator()(double a, double b) const { return rdiv_q(rdiv_make(b), a); }
-};
 
 /* the u_2-dependent part of one MLP step (.cu:5402-5472): (t_1, v_1) from the
  * u_1-only terms S1I* (Sigma_1^-1) and cst* (the common sigma_2 polynomials).

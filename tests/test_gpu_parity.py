@@ -48,6 +48,35 @@ def test_math_host_device_bit_equal(ctx_mini, func):
         f"{func}: {np.count_nonzero(host.view(np.uint64) != dev.view(np.uint64))} of {n} differ"
 
 
+def test_shared_reciprocal_quotients_are_ieee(ctx_mini):
+    """the exact MLP step's quotients with one reciprocal per denominator (the
+    compiler's division without its scale and fixup steps) equal the IEEE
+    quotient for operands of magnitude in [2^-300, 2^300], log-uniform with
+    both signs, including the operands near powers of two and with
+    significands of all ones; a zero numerator over a positive denominator
+    gives +0; the logarithm with that division equals pct_log"""
+    import oracle
+    import pct_amd
+    rng = np.random.default_rng(11)
+    n = 400_000
+    mag = lambda k: np.ldexp(rng.uniform(1.0, 2.0, k), rng.integers(-300, 300, k))  # noqa: E731
+    sign = lambda k: np.where(rng.random(k) < 0.5, -1.0, 1.0)  # noqa: E731
+    a, b = sign(n) * mag(n), sign(n) * mag(n)
+    edge = np.ldexp(1.0, rng.integers(-300, 300, 4000))
+    ones = np.nextafter(np.ldexp(2.0, rng.integers(-300, 299, 4000)), 0.0)
+    a = np.concatenate([a, edge, ones, np.zeros(100)])
+    b = np.concatenate([b, ones, edge, mag(100)])
+    host = a / b
+    assert np.array_equal(host.view(np.uint64), oracle.math_eval("div_shared_rcp", a, b).view(np.uint64))
+    dev = pct_amd.selftest_math(ctx_mini, oracle.MATH_FUNCS["div_shared_rcp"], a, b)
+    bad = np.count_nonzero(host.view(np.uint64) != dev.view(np.uint64))
+    assert bad == 0, f"{bad} of {a.size} quotients differ"
+    x = np.concatenate([rng.uniform(1e-4, 40, n // 2), 10 ** rng.uniform(-300, 300, n // 2), [1.0, 2.0, 0.5]])
+    h = oracle.math_eval("log", x)
+    d = pct_amd.selftest_math(ctx_mini, oracle.MATH_FUNCS["log_rcp_div"], x)
+    assert np.array_equal(h.view(np.uint64), d.view(np.uint64))
+
+
 def test_synth_device_equals_host(ctx_mini, mini_host):
     raw_dev = ctx_mini.generate(mini_host["cfg"]).numpy()
     for k, v in mini_host["raw"].items():
