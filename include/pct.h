@@ -337,7 +337,10 @@ int pct_drop_prepare(pct_ctx* ctx, const pct_hist_soa* hist, int block_size, int
  * x to its value before the call, marks the plan stale and returns
  * PCT_E_INVALID; the next call rebuilds the plan. With PCT_DROP_CACHED the
  * lists are not recomputed, so the caller must call pct_drop_invalidate (or
- * pct_drop_prepare) after changing histories in place. */
+ * pct_drop_prepare) after changing histories in place. A scan whose plan does
+ * not fit the device (pct_drop_plan_choice) runs the reference's structure
+ * (scatter + image update per block) over chunks of whole blocks instead;
+ * the research variants need the plan and return PCT_E_NOMEM then. */
 int pct_drop_iteration(pct_ctx* ctx, const pct_hist_soa* hist, float* x,
                        int block_size, float lambda, int mode);
 /* Marks the context's DROP plan stale (no counterpart in the reference, which

@@ -1126,12 +1126,13 @@ extern "C" int pct_drop_iteration(pct_ctx* ctx, const pct_hist_soa* hist, float*
         fresh = true;
     }
     /* the reference recomputes every MLP path in every iteration; so does the
-     * default mode, into the plan's list store (a list's capacity is its
-     * block's stride), checking every list against the plan's hash */
+     * default mode, beside the block loop, checking every list against the
+     * plan's hash (the block loop reads the plan's lists) */
     const bool recompute = !(mode & PCT_DROP_CACHED) && !fresh;
     if (recompute) {
         /* a recomputed list that differs from the plan is only known once the
-         * block loop has run on it: keep x to restore it then */
+         * block loop has run on the plan's lists (and the changed inputs'
+         * WEPL): keep x to restore it then */
         if ((rc = pct_ensure(ctx, ctx->x_saved, sizeof(float) * (size_t)ctx->g.V))) return rc;
         PCT_HIP_CHECK(ctx, hipMemcpyAsync(ctx->x_saved.ptr, x, sizeof(float) * (size_t)ctx->g.V,
                                           hipMemcpyDeviceToDevice, ctx->stream));
@@ -1141,9 +1142,10 @@ extern "C" int pct_drop_iteration(pct_ctx* ctx, const pct_hist_soa* hist, float*
     if (rc) return rc;
     int flags = 0;
     if ((rc = pct_take_status(ctx, &flags))) return rc;
-    /* recomputed lists that differ from the plan (or overflow a stride) have
-     * overwritten the list store: x goes back to its value before the call,
-     * and the plan is rebuilt on the next call */
+    /* recomputed lists that differ from the plan (or overflow a stride, or a
+     * wait of the waiting form that gave up): x goes back to its value before
+     * the call (the waiting form never wrote it), and the plan is rebuilt on
+     * the next call */
     if (flags) {
         pl->valid = false;
         if (recompute && (flags & (PCT_STATUS_PLAN | PCT_STATUS_CAPACITY | PCT_STATUS_TABLE | PCT_STATUS_OVERFLOW))) {
