@@ -389,6 +389,11 @@ int pct_drop_stats(pct_ctx* ctx, const pct_hist_soa* hist, int block_size, int64
                                      plan may take (default: free memory); a
                                      scan whose plan exceeds it runs in chunks
                                      of blocks (see pct_drop_plan_choice)     */
+#define PCT_TUNE_WAIT_GIVE_UP  7  /* diagnostics of block form 3: 1 makes its
+                                     waits target a stamp no update writes and
+                                     give up after two polls, so that the
+                                     failure path (PCT_E_HIP, x unchanged) can
+                                     be exercised; 0 (default) off            */
 int pct_set_tuning(pct_ctx* ctx, int key, int value);
 
 /* Capacity (no counterpart in the reference, which walks every path inside

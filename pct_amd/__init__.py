@@ -496,7 +496,8 @@ class Batch:
 
 TUNING_KEYS = {"mlp_waves": _lib.PCT_TUNE_MLP_WAVES, "chunk_first": _lib.PCT_TUNE_CHUNK_FIRST,
                "chunk_max": _lib.PCT_TUNE_CHUNK_MAX, "overlap": _lib.PCT_TUNE_OVERLAP,
-               "block_form": _lib.PCT_TUNE_BLOCK_FORM, "plan_limit_mb": _lib.PCT_TUNE_PLAN_LIMIT_MB}
+               "block_form": _lib.PCT_TUNE_BLOCK_FORM, "plan_limit_mb": _lib.PCT_TUNE_PLAN_LIMIT_MB,
+               "wait_give_up": _lib.PCT_TUNE_WAIT_GIVE_UP}
 
 
 def set_tuning(ctx: "Context", **knobs):
@@ -504,7 +505,8 @@ def set_tuning(ctx: "Context", **knobs):
     chunk_first, chunk_max, overlap (1 beside, 2 sequential), block_form (1 one
     launch per block recomputing the previous block's updates it needs, 2 two
     launches per block, 3 one launch per block waiting for the previous block's
-    stored updates); plan_limit_mb; 0 = default."""
+    stored updates); plan_limit_mb; wait_give_up (diagnostics of block form 3:
+    its waits give up, to exercise the failure path); 0 = default."""
     for k, v in knobs.items():
         if k not in TUNING_KEYS:
             raise PctError(f"unknown tuning knob {k}")

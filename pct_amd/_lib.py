@@ -208,6 +208,7 @@ PCT_TUNE_CHUNK_MAX = 3
 PCT_TUNE_OVERLAP = 4
 PCT_TUNE_BLOCK_FORM = 5
 PCT_TUNE_PLAN_LIMIT_MB = 6
+PCT_TUNE_WAIT_GIVE_UP = 7
 # the precision form: tabulated MLP with FP32 stepping + FP32/Kahan update
 PCT_PRECISION = PCT_MLP_TABLE | PCT_DROP_F32
 # the learned-path form: surrogate MLP (exact fallback) + FP32/Kahan update

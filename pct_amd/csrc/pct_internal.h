@@ -89,6 +89,7 @@ struct pct_ctx {
     /* execution knobs (pct_set_tuning; 0 = default) */
     int tune_mlp_waves = 0, tune_chunk_first = 0, tune_chunk_max = 0, tune_overlap = 0, tune_block_form = 0;
     int tune_plan_limit_mb = 0;
+    int tune_wait_give_up = 0;
     /* workspaces, grown on demand */
     /* research list iteration: sort keys, entries, lists, per-list factors,
      * block offsets, sort storage (pct_research.hip) */

@@ -64,6 +64,7 @@ struct pct_plan {
     bool graph_f32 = false;
     int graph_variant = 0;
     int graph_form = 0;
+    int graph_wait_give_up = 0;
     double graph_eta = 0.0;
     int graph_psi = 0;
 };

@@ -359,8 +359,8 @@ __global__ void __launch_bounds__(WAIT_THREADS)
 k_block_wait(const uint32_t* __restrict__ list, int stride, pct_blk::blk_pos pos, const float* __restrict__ wepl,
              int64_t h0, int64_t count, double lambda, void* __restrict__ delta_out,
              const uint64_t* __restrict__ segs_p, int64_t ns_p, const uint16_t* __restrict__ csc_p,
-             const void* __restrict__ delta_p, uint32_t stamp_p, unsigned na, uint64_t* __restrict__ xw,
-             int* __restrict__ status) {
+             const void* __restrict__ delta_p, uint32_t stamp_p, uint32_t stamp_wait, int polls, unsigned na,
+             uint64_t* __restrict__ xw, int* __restrict__ status) {
     if (blockIdx.x < na) {
         __builtin_amdgcn_s_setprio(3);
         const pct_blk::fused_prev<F32> pv{csc_p, delta_p, stamp_p};
@@ -371,7 +371,7 @@ k_block_wait(const uint32_t* __restrict__ list, int stride, pct_blk::blk_pos pos
     __builtin_amdgcn_s_setprio(2);
     const int64_t q = (int64_t)(blockIdx.x - na) * DELTA_WAVES + (threadIdx.x >> 6);
     if (q >= count) return;
-    pct_blk::wait_delta<F32>(list, stride, pos, wepl, h0, q, xw, stamp_p, status, lambda, delta_out,
+    pct_blk::wait_delta<F32>(list, stride, pos, wepl, h0, q, xw, stamp_wait, polls, status, lambda, delta_out,
                              threadIdx.x & 63);
 }
 
@@ -823,14 +823,17 @@ static int plan_block_loop(pct_ctx* ctx, pct_plan* pl, const pct_hist_soa* hist,
                 const unsigned na = nsp > 0 ? (pct_grid(nsp, WAIT_THREADS) + 7u) / 8u * 8u : 0u;
                 const unsigned nd = (unsigned)((cnt + DELTA_WAVES - 1) / DELTA_WAVES);
                 if (na + nd == 0) continue;
+                /* the diagnostics knob: a stamp no update writes, two polls */
+                const uint32_t stamp_wait = ctx->tune_wait_give_up ? stamp + 0x40000000u : stamp;
+                const int polls = ctx->tune_wait_give_up ? 2 : WAIT_POLLS;
                 if (f32)
                     hipLaunchKernelGGL(k_block_wait<true>, dim3(na + nd), dim3(WAIT_THREADS), 0, ctx->stream, lb,
                                        stride, pos_of(b), (const float*)hist->wepl, h0, cnt, lambda, dout, sp, nsp,
-                                       cp, dprev, stamp, na, xw, ctx->d_status);
+                                       cp, dprev, stamp, stamp_wait, polls, na, xw, ctx->d_status);
                 else
                     hipLaunchKernelGGL(k_block_wait<false>, dim3(na + nd), dim3(WAIT_THREADS), 0, ctx->stream, lb,
                                        stride, pos_of(b), (const float*)hist->wepl, h0, cnt, lambda, dout, sp, nsp,
-                                       cp, dprev, stamp, na, xw, ctx->d_status);
+                                       cp, dprev, stamp, stamp_wait, polls, na, xw, ctx->d_status);
                 continue;
             }
             const unsigned na = pct_grid(nsp, FUSED_THREADS) * (nsp > 0);
@@ -912,7 +915,7 @@ static int plan_ensure_graphs(pct_ctx* ctx, pct_plan* pl, const pct_hist_soa* hi
     const int64_t nchunks = (int64_t)pl->chunk_b.size() - 1;
     if ((int64_t)pl->graphs.size() == nchunks && pl->graph_x == x && pl->graph_lambda == lambda &&
         pl->graph_f32 == f32 && pl->graph_wepl == hist->wepl && pl->graph_variant == variant &&
-        pl->graph_form == block_form(ctx, variant) &&
+        pl->graph_form == block_form(ctx, variant) && pl->graph_wait_give_up == ctx->tune_wait_give_up &&
         (variant == 0 || (pl->graph_eta == ctx->p.eta && pl->graph_psi == ctx->p.psi_sign)))
         return PCT_OK;
     plan_release_graph(pl);
@@ -936,6 +939,7 @@ static int plan_ensure_graphs(pct_ctx* ctx, pct_plan* pl, const pct_hist_soa* hi
     pl->graph_wepl = hist->wepl;
     pl->graph_variant = variant;
     pl->graph_form = block_form(ctx, variant);
+    pl->graph_wait_give_up = ctx->tune_wait_give_up;
     pl->graph_eta = ctx->p.eta;
     pl->graph_psi = ctx->p.psi_sign;
     return PCT_OK;
@@ -1189,6 +1193,7 @@ extern "C" int pct_set_tuning(pct_ctx* ctx, int key, int value) {
     case PCT_TUNE_CHUNK_FIRST: ctx->tune_chunk_first = value; break;
     case PCT_TUNE_CHUNK_MAX: ctx->tune_chunk_max = value; break;
     case PCT_TUNE_PLAN_LIMIT_MB: ctx->tune_plan_limit_mb = value; break;
+    case PCT_TUNE_WAIT_GIVE_UP: ctx->tune_wait_give_up = value ? 1 : 0; break;
     case PCT_TUNE_BLOCK_FORM:
         if (value > 3) return pct_fail(ctx, PCT_E_INVALID, "pct_set_tuning: block form %d not in {1, 2, 3}", value);
         ctx->tune_block_form = value;

@@ -450,9 +450,10 @@ __device__ __forceinline__ void fused_apply(uint64_t r, uint64_t* __restrict__ x
 #define WAIT_POLLS 4096
 #define WAIT_FLAG 16   /* PCT_STATUS_WAIT */
 
-__device__ __forceinline__ uint64_t wait_word(const uint64_t* p, uint64_t w, uint32_t stamp, int* status) {
+__device__ __forceinline__ uint64_t wait_word(const uint64_t* p, uint64_t w, uint32_t stamp, int polls,
+                                              int* status) {
     for (int i = 1; xw_stamp(w) != stamp; ++i) {
-        if (i >= WAIT_POLLS) {
+        if (i >= polls) {
             atomicOr(status, WAIT_FLAG);
             break;
         }
@@ -474,8 +475,8 @@ __device__ __forceinline__ uint64_t slot_word(const uint64_t* __restrict__ xw, u
 template <bool F32>
 __device__ __forceinline__ void wait_delta(const uint32_t* __restrict__ list, int stride, blk_pos pos,
                                            const float* __restrict__ wepl, int64_t h0, int64_t q,
-                                           const uint64_t* __restrict__ xw, uint32_t stamp, int* status,
-                                           double lambda, void* __restrict__ delta, int lane) {
+                                           const uint64_t* __restrict__ xw, uint32_t stamp, int polls,
+                                           int* status, double lambda, void* __restrict__ delta, int lane) {
     const uint32_t k = pos.k[q];
     const uint32_t n = pos.len[q];
     const double Ld = pos.chord[q];
@@ -491,10 +492,11 @@ __device__ __forceinline__ void wait_delta(const uint32_t* __restrict__ list, in
     for (int r = 0; r < DELTA_SLOTS; ++r) w[r] = e[r] != LIST_SENTINEL ? slot_word(xw, e[r]) : 0ull;
 #pragma unroll
     for (int r = 0; r < DELTA_SLOTS; ++r)
-        if (e[r] != LIST_SENTINEL && (e[r] & LIST_TAG)) w[r] = wait_word(xw + LIST_VOX(e[r]), w[r], stamp, status);
+        if (e[r] != LIST_SENTINEL && (e[r] & LIST_TAG))
+            w[r] = wait_word(xw + LIST_VOX(e[r]), w[r], stamp, polls, status);
     auto tail_value = [&](uint32_t t) {
         uint64_t u = slot_word(xw, t);
-        if (t & LIST_TAG) u = wait_word(xw + LIST_VOX(t), u, stamp, status);
+        if (t & LIST_TAG) u = wait_word(xw + LIST_VOX(t), u, stamp, polls, status);
         return xw_value(u);
     };
     if (F32) {

@@ -206,6 +206,33 @@ def test_drop_modes_agree(ctx_mini, mini_host):
     assert np.max(np.abs(out["default"] - out["atomic"])) < 1e-5
 
 
+def test_waiting_form_wait_that_gives_up_leaves_x_unchanged(ctx_mini, mini_host):
+    """block form 3's bounded waits, forced from the host to give up (they
+    target a stamp no update writes, two polls each): the iteration returns
+    the error and x keeps its value; without the knob the next iteration runs
+    and equals the two-launch form's"""
+    import pct_amd
+    torch = ctx_mini.torch
+    d = _to_dev(mini_host["hr"])
+    x0 = mini_host["hull"].astype(np.float32)
+    lam = 1e-2
+    pct_amd.set_tuning(ctx_mini, block_form=3, wait_give_up=1)
+    try:
+        x = torch.from_numpy(x0.copy()).cuda()
+        with pytest.raises(pct_amd.PctError, match="gave up"):
+            ctx_mini.drop_iteration(d, x, lam=lam, mode=pct_amd.PCT_DROP_CACHED)
+        assert np.array_equal(x.cpu().numpy().view(np.uint32), x0.view(np.uint32))
+        pct_amd.set_tuning(ctx_mini, wait_give_up=0)
+        ctx_mini.drop_iteration(d, x, lam=lam, mode=pct_amd.PCT_DROP_CACHED)
+        got = x.cpu().numpy()
+    finally:
+        pct_amd.set_tuning(ctx_mini, block_form=0, wait_give_up=0)
+    x2 = torch.from_numpy(x0.copy()).cuda()
+    ctx_mini.drop_iteration(d, x2, lam=lam, mode=pct_amd.PCT_DROP_CACHED)
+    assert np.array_equal(got.view(np.uint32), x2.cpu().numpy().view(np.uint32))
+    assert not np.array_equal(got, x0)
+
+
 def test_graph_replays_match_eager(ctx_mini, mini_host):
     """The captured block loop replayed several times, with eager work and
     synchronisations in between, matches the eager launches every time."""

@@ -4,7 +4,9 @@ gives what pct_drop_block_update gives, through the C ABI: the per-voxel
 counts bit for bit, the float x_update sums to float-atomic tolerance (both
 scatter with hardware float atomics, whose order varies), on two calls in a
 row, the second on a smaller range of histories; sizes beyond its limits are
-refused."""
+refused; in the surrogate MLP mode, with part of the histories sent to the
+exact walk through the fallback list the fixed array also holds, the same
+holds."""
 import numpy as np
 import pytest
 
@@ -67,3 +69,35 @@ def configs_params_with_long_paths():
     p = configs.params("mini")
     p.max_intersections = 1024
     return p
+
+
+def test_noalloc_surrogate_with_fallbacks():
+    import pct_amd
+    from pct_amd import learned
+    p, ctx, hist, x0 = _mini_scan()
+    torch = ctx.torch
+    lam = 1e-2
+    # an untrained network over a range that excludes histories with t_in < 0:
+    # those take the exact walk from the fallback list
+    lo, hi = [-float("inf")] * 12, [float("inf")] * 12
+    lo[6] = 0.0
+    learned.install(ctx, learned.init_params(5, ctx.device), lo, hi)
+    for start, count in ((0, 8192), (2000, 900)):
+        print(f"noalloc surrogate block update: histories [{start}, {start + count})", flush=True)
+        out, fallbacks = [], []
+        for fn in (ctx.drop_block_update, ctx.drop_block_update_noalloc):
+            xu = torch.zeros(ctx.V, dtype=torch.float32, device=ctx.device)
+            cnt = torch.zeros(ctx.V, dtype=torch.int32, device=ctx.device)
+            learned.fallback_stats(ctx)
+            fn(hist, start, count, x0, xu, cnt, lam, pct_amd.PCT_MLP_SURROGATE)
+            ctx.sync()
+            fallbacks.append(learned.fallback_stats(ctx))
+            out.append((xu.cpu().numpy(), cnt.cpu().numpy()))
+        (xu_a, c_a), (xu_b, c_b) = out
+        fb, tot = fallbacks[0]
+        assert tot == count and 0 < fb < count and fallbacks[1] == fallbacks[0]
+        assert np.array_equal(c_a, c_b)
+        scale = float(np.max(np.abs(xu_a)))
+        assert scale > 0
+        np.testing.assert_allclose(xu_b, xu_a, rtol=1e-5, atol=1e-6 * scale)
+    ctx.close()
