@@ -811,7 +811,7 @@ def main():
                    "mean_path_voxels": round(sum_n / n_r, 2), "sum_block_voxels": sum_vb,
                    "drop_blocks": (n_r + B - 1) // B},
         "phases_ms": {"mlp_span": round(float(mlp_ms), 3), "block_loop_span": round(float(blk_ms), 3),
-                      "note": "the MLP pass and the block loop overlap chunk by chunk; alone they take "
+                      "note": "the MLP check runs beside the block loop and joins it at the end; alone they take "
                               f"{mlp_alone_ms:.2f} and {blk_alone_ms:.2f} ms",
                       "plan_setup_once": round(t_plan * 1e3, 1), "preprocess_once": round(t_pre * 1e3, 1)},
         "preprocess_stages": {"note": "each stage alone between two HIP events; bytes = the arrays the "

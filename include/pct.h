@@ -367,12 +367,18 @@ int pct_drop_stats(pct_ctx* ctx, const pct_hist_soa* hist, int block_size, int64
  * whose launch shapes are compile-time constants, data.h:377-378). They change
  * scheduling only, never results. value <= 0 restores the default. */
 #define PCT_TUNE_MLP_WAVES     1  /* resident MLP wavefronts per SIMD while the
-                                     block loop runs beside it (default: the
-                                     exact walk's occupancy limit, 2 for the
-                                     precision form, 1 for the surrogate)      */
-#define PCT_TUNE_CHUNK_FIRST   2  /* blocks in the first chunk (default 16)    */
-#define PCT_TUNE_CHUNK_MAX     3  /* largest chunk in blocks (default 256)     */
-#define PCT_TUNE_OVERLAP       4  /* 1: MLP beside the block loop (default),
+                                     block loop runs beside it (default: 3 for
+                                     the exact walk, 2 for the precision form,
+                                     1 for the surrogate)                      */
+#define PCT_TUNE_CHUNK_FIRST   2  /* blocks in the first chunk (default: all
+                                     blocks in one chunk; 16 with overlap 1)   */
+#define PCT_TUNE_CHUNK_MAX     3  /* largest chunk in blocks (default: all; 256
+                                     with overlap 1)                           */
+#define PCT_TUNE_OVERLAP       4  /* default: the check of the recomputed MLP
+                                     lists beside the block loop, which reads
+                                     the plan's lists and joins the check at
+                                     the end; 1: beside, the block loop's chunk
+                                     c waiting for the check of chunk c;
                                      2: MLP first, then the block loop         */
 #define PCT_TUNE_BLOCK_FORM    5  /* 1: one launch per block, the voxel updates
                                      of block b-1 beside the deltas of block b,

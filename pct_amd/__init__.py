@@ -502,7 +502,9 @@ TUNING_KEYS = {"mlp_waves": _lib.PCT_TUNE_MLP_WAVES, "chunk_first": _lib.PCT_TUN
 
 def set_tuning(ctx: "Context", **knobs):
     """Execution knobs of the DROP iteration (pct_set_tuning): mlp_waves,
-    chunk_first, chunk_max, overlap (1 beside, 2 sequential), block_form (1 one
+    chunk_first, chunk_max, overlap (0: the MLP check beside the block loop,
+    joined at the end; 1: beside, each chunk waiting for its check; 2:
+    sequential), block_form (1 one
     launch per block recomputing the previous block's updates it needs, 2 two
     launches per block, 3 one launch per block waiting for the previous block's
     stored updates); plan_limit_mb; wait_give_up (diagnostics of block form 3:

@@ -354,14 +354,20 @@ extern "C" int pct_batch_iteration(pct_batch* b, const pct_hist_soa* hists, floa
         PCT_HIP_CHECK(c0, hipEventRecord(c0->ev[1], c0->stream));
         PCT_HIP_CHECK(c0, hipEventRecord(c0->ev[2], c0->stream));
     }
+    /* the block steps read the plans' lists: by default they wait for the
+     * check of the recomputed lists only at the end; with overlap 1 of the
+     * first scan's context, chunk c waits for the check of its lists */
+    const bool chunk_waits = recompute && c0->tune_overlap == 1;
     for (int64_t c = 0; c < nchunks; ++c) {
-        if (recompute) PCT_HIP_CHECK(c0, hipStreamWaitEvent(c0->stream, b->ev_chunk[c], 0));
+        if (chunk_waits) PCT_HIP_CHECK(c0, hipStreamWaitEvent(c0->stream, b->ev_chunk[c], 0));
         if (eager) {
             if ((rc = batch_steps(b, xs, hists, lambdas, f32, b->chunk[c], b->chunk[c + 1]))) return rc;
         } else {
             PCT_HIP_CHECK(c0, hipGraphLaunch(b->graphs[c], c0->stream));
         }
     }
+    if (recompute && !chunk_waits && nchunks > 0)
+        PCT_HIP_CHECK(c0, hipStreamWaitEvent(c0->stream, b->ev_chunk[nchunks - 1], 0));
     if (c0->profile) {
         PCT_HIP_CHECK(c0, hipEventRecord(c0->ev[3], c0->stream));
         c0->ev_recorded = true;

@@ -394,9 +394,12 @@ __global__ void k_xw_unpack(const uint64_t* __restrict__ xw, int64_t V, float* _
 /* ------------------------------------------------------------------------- */
 /* host                                                                       */
 /* ------------------------------------------------------------------------- */
-/* chunks of the block loop (the unit of MLP / block-loop overlap): the first
- * is small so that the block loop starts early, later ones double up to
- * CHUNK_MAX blocks so that each MLP launch fills the GPU for many rounds */
+/* chunks of the block loop (one graph and one MLP launch each). By default
+ * the block loop reads the plan's lists and never waits for the check of the
+ * recomputed ones, which runs beside it as one launch (one chunk). With the
+ * per-chunk waits of overlap 1 the first chunk is small so that the block
+ * loop starts early, later ones double up to CHUNK_MAX blocks so that each
+ * MLP launch fills the GPU for many rounds. */
 #define CHUNK_FIRST 16
 #define CHUNK_MAX 256
 
@@ -898,8 +901,10 @@ static int plan_block_loop(pct_ctx* ctx, pct_plan* pl, const pct_hist_soa* hist,
 
 /* chunk boundaries from the current knobs; a change drops the graphs */
 static void plan_chunks(pct_ctx* ctx, pct_plan* pl) {
-    int64_t first = ctx->tune_chunk_first > 0 ? ctx->tune_chunk_first : CHUNK_FIRST;
-    int64_t most = ctx->tune_chunk_max > 0 ? ctx->tune_chunk_max : CHUNK_MAX;
+    const bool one = ctx->tune_overlap != 1 && ctx->tune_overlap != 2;
+    const int64_t all = std::max<int64_t>(1, pl->nb);
+    int64_t first = ctx->tune_chunk_first > 0 ? ctx->tune_chunk_first : (one ? all : CHUNK_FIRST);
+    int64_t most = ctx->tune_chunk_max > 0 ? ctx->tune_chunk_max : (one ? all : CHUNK_MAX);
     if (most < first) most = first;
     if (!pl->chunk_b.empty() && pl->chunk_first == first && pl->chunk_max == most) return;
     plan_release_graph(pl);
@@ -1006,8 +1011,12 @@ static int plan_iterate(pct_ctx* ctx, pct_plan* pl, const pct_hist_soa* hist, fl
         PCT_HIP_CHECK(ctx, hipEventRecord(ctx->ev[1], ctx->stream));
         PCT_HIP_CHECK(ctx, hipEventRecord(ctx->ev[2], ctx->stream));
     }
+    /* the block loop reads the plan's lists: by default it waits for the check
+     * of the recomputed lists only at the end (a mismatch restores x then);
+     * overlap 1 makes chunk c wait for the check of its own lists */
+    const bool chunk_waits = recompute && ctx->tune_overlap == 1;
     for (int64_t c = 0; c < nchunks; ++c) {
-        if (recompute) PCT_HIP_CHECK(ctx, hipStreamWaitEvent(ctx->stream, pl->ev_chunk[c], 0));
+        if (chunk_waits) PCT_HIP_CHECK(ctx, hipStreamWaitEvent(ctx->stream, pl->ev_chunk[c], 0));
         if (eager) {
             int64_t b0 = pl->chunk_b[c], b1 = pl->chunk_b[c + 1];
             if ((rc = plan_block_loop(ctx, pl, hist, x, lambda, f32, variant, b0, b1))) return rc;
@@ -1015,6 +1024,7 @@ static int plan_iterate(pct_ctx* ctx, pct_plan* pl, const pct_hist_soa* hist, fl
             PCT_HIP_CHECK(ctx, hipGraphLaunch(pl->graphs[c], ctx->stream));
         }
     }
+    if (recompute && !chunk_waits) PCT_HIP_CHECK(ctx, hipStreamWaitEvent(ctx->stream, pl->ev_chunk[nchunks - 1], 0));
     if (words)
         hipLaunchKernelGGL(k_xw_unpack, dim3(pct_grid(ctx->g.V, 256)), dim3(256), 0, ctx->stream,
                            (const uint64_t*)pl->xw.ptr, ctx->g.V, x, ctx->d_status);

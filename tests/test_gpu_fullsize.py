@@ -52,6 +52,7 @@ def test_full_size_schedules_agree(full):
     for name, mode, knobs in (("default", pct_amd.PCT_MLP_FP64, {}),
                               ("cached", pct_amd.PCT_MLP_FP64 | pct_amd.PCT_DROP_CACHED, {}),
                               ("sequential", pct_amd.PCT_MLP_FP64, {"overlap": 2}),
+                              ("per-chunk waits", pct_amd.PCT_MLP_FP64, {"overlap": 1}),
                               ("chunks", pct_amd.PCT_MLP_FP64, {"chunk_first": 5, "chunk_max": 40}),
                               ("one launch per block", pct_amd.PCT_MLP_FP64 | pct_amd.PCT_DROP_CACHED,
                                {"block_form": 1}),

@@ -306,7 +306,8 @@ def test_tuning_knobs_do_not_change_results(ctx_mini, mini_host):
     d = _to_dev(mini_host["hr"])
     x0 = mini_host["hull"].astype(np.float32)
     out = []
-    for knobs in ({}, {"overlap": 2}, {"chunk_first": 1, "chunk_max": 2}, {"chunk_first": 3, "mlp_waves": 1}):
+    for knobs in ({}, {"overlap": 1}, {"overlap": 2}, {"chunk_first": 1, "chunk_max": 2},
+                  {"overlap": 1, "chunk_first": 1, "chunk_max": 2}, {"chunk_first": 3, "mlp_waves": 1}):
         pct_amd.set_tuning(ctx_mini, overlap=0, chunk_first=0, chunk_max=0, mlp_waves=0)
         pct_amd.set_tuning(ctx_mini, **knobs)
         x = torch.from_numpy(x0.copy()).cuda()
