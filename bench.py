@@ -448,6 +448,8 @@ def main():
                     help="block loop: 1 one launch per block recomputing the previous block's updates, 2 two "
                          "launches per block, 3 one launch per block waiting for the previous block's stored "
                          "updates (0: the library's default)")
+    ap.add_argument("--mlp-waves", type=int, default=0,
+                    help="resident MLP wavefronts per SIMD beside the block loop (0: the library's default)")
     ap.add_argument("--no-cfg1", action="store_true", help="skip the 100x100x16 / 2 M CPU-and-GPU pipeline")
     ap.add_argument("--oracle-images", default=None,
                     help="directory of the CPU oracle's images of this scan after iterations 1, 6, 12 "
@@ -529,6 +531,8 @@ def main():
     B = p.drop_block_size
     if args.block_form:
         pct_amd.set_tuning(ctx, block_form=args.block_form)
+    if args.mlp_waves:
+        pct_amd.set_tuning(ctx, mlp_waves=args.mlp_waves)
     if args.quick and args.form == "walks":
         from pct_amd import learned
         learned.install(ctx, learned.init_params(1, dev), [-float("inf")] * 12, [float("inf")] * 12)

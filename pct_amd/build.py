@@ -24,6 +24,9 @@ HIPCC_FLAGS = [
     "-ffp-contract=off",
     # no packed FP32 instructions
     "-Xclang", "-target-feature", "-Xclang", "-packed-fp32-ops",
+    # the dispatch preloads leading scalar kernel arguments into SGPRs: a wave
+    # starts without a scalar load of its arguments (latency-bound kernels)
+    "-mllvm", "-amdgpu-kernarg-preload-count=16",
     "-fopenmp", "-Wno-unused-command-line-argument",
 ]
 

@@ -1239,10 +1239,10 @@ int pct_mlp_paths_fb(pct_ctx* ctx, hipStream_t stream, const pct_hist_soa* hist,
     /* a workgroup of 256 is one wavefront on each of the CU's 4 SIMDs. Beside
      * a single scan's block loop the walks leave room (wave slots and
      * registers) for the latency-bound block kernels (measured at 200x200x32:
-     * the exact walk best at 3 wavefronts per SIMD, the precision form at 2,
-     * the surrogate at 1; the batched form's steps do better with the full
-     * occupancy) */
-    static const int beside_default[3] = {3, 2, 1};
+     * the exact and the table walk best at 3 wavefronts per SIMD, the
+     * surrogate at 2, its most; the batched form's steps do better with the
+     * full occupancy) */
+    static const int beside_default[3] = {3, 3, 2};
     /* workgroups per CU of a walk launch: the walk's occupancy, capped beside
      * the block loop or by the knob; for the table walks the per-depth table
      * goes to LDS when the workgroups the launch keeps on a CU fit in its

@@ -262,31 +262,37 @@ __global__ void k_write_segments(const uint32_t* __restrict__ keys, const uint32
 /* ------------------------------------------------------------------------- */
 /* the block loop's kernels: bodies in pct_plan_kernels.h. Both raise their
  * wave priority: they are latency-bound and share the CUs with the
- * long-running MLP waves of the next chunks. */
+ * long-running MLP waves of the next chunks. Their leading arguments are the
+ * ones their first memory round needs, all scalars, so that the dispatch
+ * preloads them into SGPRs (the library is compiled with kernel-argument
+ * preloading); the rest are loaded while that round is in flight. */
 __global__ void __launch_bounds__(64 * DELTA_WAVES)
-k_block_delta(const uint32_t* __restrict__ list, int stride, pct_blk::blk_pos pos, const float* __restrict__ wepl,
-              int64_t h0, int64_t count, const float* __restrict__ x, double lambda, double* __restrict__ delta) {
+k_block_delta(const uint32_t* __restrict__ list, const uint16_t* __restrict__ pos_k,
+              const uint32_t* __restrict__ pos_len, const double* __restrict__ pos_chord, int stride, int64_t count,
+              const float* __restrict__ x, const float* __restrict__ wepl, int64_t h0, double lambda,
+              double* __restrict__ delta) {
     __builtin_amdgcn_s_setprio(3);
     const int64_t q = (int64_t)blockIdx.x * DELTA_WAVES + (threadIdx.x >> 6);
     if (q >= count) return;
-    pct_blk::delta_f64(list, stride, pos, wepl, h0, q, x, lambda, delta, threadIdx.x & 63);
+    pct_blk::delta_f64(list, stride, {pos_k, pos_len, pos_chord}, wepl, h0, q, x, lambda, delta, threadIdx.x & 63);
 }
 
 __global__ void __launch_bounds__(64 * DELTA_WAVES)
-k_block_delta_f32(const uint32_t* __restrict__ list, int stride, pct_blk::blk_pos pos,
-                  const float* __restrict__ wepl, int64_t h0, int64_t count, const float* __restrict__ x,
+k_block_delta_f32(const uint32_t* __restrict__ list, const uint16_t* __restrict__ pos_k,
+                  const uint32_t* __restrict__ pos_len, const double* __restrict__ pos_chord, int stride,
+                  int64_t count, const float* __restrict__ x, const float* __restrict__ wepl, int64_t h0,
                   float lambda, float* __restrict__ delta) {
     __builtin_amdgcn_s_setprio(3);
     const int64_t q = (int64_t)blockIdx.x * DELTA_WAVES + (threadIdx.x >> 6);
     if (q >= count) return;
-    pct_blk::delta_f32(list, stride, pos, wepl, h0, q, x, lambda, delta, threadIdx.x & 63);
+    pct_blk::delta_f32(list, stride, {pos_k, pos_len, pos_chord}, wepl, h0, q, x, lambda, delta, threadIdx.x & 63);
 }
 
 __global__ void __launch_bounds__(APPLY_THREADS)
 k_block_apply(const uint64_t* __restrict__ segs, int64_t s0, int64_t ns, const uint16_t* __restrict__ csc,
               const double* __restrict__ delta, float* __restrict__ x) {
     __builtin_amdgcn_s_setprio(3);
-    int64_t k = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    int64_t k = (int64_t)blockIdx.x * APPLY_THREADS + threadIdx.x;
     if (k >= ns) return;
     pct_blk::apply_f64(segs[s0 + k], csc, delta, x);
 }
@@ -295,7 +301,7 @@ __global__ void __launch_bounds__(APPLY_THREADS)
 k_block_apply_f32(const uint64_t* __restrict__ segs, int64_t s0, int64_t ns, const uint16_t* __restrict__ csc,
                   const float* __restrict__ delta, float* __restrict__ x) {
     __builtin_amdgcn_s_setprio(3);
-    int64_t k = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    int64_t k = (int64_t)blockIdx.x * APPLY_THREADS + threadIdx.x;
     if (k >= ns) return;
     pct_blk::apply_f32(segs[s0 + k], csc, delta, x);
 }
@@ -880,12 +886,15 @@ static int plan_block_loop(pct_ctx* ctx, pct_plan* pl, const pct_hist_soa* hist,
                                    cb, (const double*)out, B, chord + h0, lambda, eta, psi, x);
             continue;
         }
+        const pct_blk::blk_pos pb = pos_of(b);
         if (f32)
-            hipLaunchKernelGGL(k_block_delta_f32, dgrid, dim3(64 * DELTA_WAVES), 0, ctx->stream, lb, stride, pos_of(b),
-                               (const float*)hist->wepl, h0, cnt, (const float*)x, (float)lambda, (float*)delta);
+            hipLaunchKernelGGL(k_block_delta_f32, dgrid, dim3(64 * DELTA_WAVES), 0, ctx->stream, lb, pb.k, pb.len,
+                               pb.chord, stride, cnt, (const float*)x, (const float*)hist->wepl, h0, (float)lambda,
+                               (float*)delta);
         else
-            hipLaunchKernelGGL(k_block_delta, dgrid, dim3(64 * DELTA_WAVES), 0, ctx->stream, lb, stride, pos_of(b),
-                               (const float*)hist->wepl, h0, cnt, (const float*)x, lambda, (double*)delta);
+            hipLaunchKernelGGL(k_block_delta, dgrid, dim3(64 * DELTA_WAVES), 0, ctx->stream, lb, pb.k, pb.len,
+                               pb.chord, stride, cnt, (const float*)x, (const float*)hist->wepl, h0, lambda,
+                               (double*)delta);
         int64_t s0 = pl->blk_seg_h[b], ns = pl->blk_seg_h[b + 1] - s0;
         const uint16_t* cb = csc + pl->blk_entry_h[b];
         if (ns > 0 && f32)

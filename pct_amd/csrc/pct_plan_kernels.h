@@ -61,6 +61,16 @@ __device__ __forceinline__ void kahan_add(float& s, float& c, float v) {
     s = t;
 }
 
+/* x at a lane's list slots, every load issued before the first is used (one
+ * memory round): a sentinel slot loads x[0], which its caller ignores. A
+ * conditional load per slot would be compiled as a branch around each load
+ * and its use, one round per slot. */
+__device__ __forceinline__ void gather_slots(const float* __restrict__ x, const uint32_t (&v)[DELTA_SLOTS],
+                                             float (&xv)[DELTA_SLOTS]) {
+#pragma unroll
+    for (int r = 0; r < DELTA_SLOTS; ++r) xv[r] = x[v[r] != LIST_SENTINEL ? LIST_VOX(v[r]) : 0u];
+}
+
 /* The block's histories in placement order: position q's list is row q of the
  * block's rows (list + q * stride), and pos.k[q], pos.len[q], pos.chord[q]
  * are that history's block-local index, list length and chord length; the
@@ -88,10 +98,12 @@ __device__ __forceinline__ void delta_f64(const uint32_t* __restrict__ list, int
         int j = lane + 64 * r;
         v[r] = j < stride ? p[j] : LIST_SENTINEL;
     }
+    float xv[DELTA_SLOTS];
+    gather_slots(x, v, xv);
     double s = 0.0;
 #pragma unroll
     for (int r = 0; r < DELTA_SLOTS; ++r)
-        if (v[r] != LIST_SENTINEL) s += (double)x[LIST_VOX(v[r])];
+        if (v[r] != LIST_SENTINEL) s += (double)xv[r];
     for (int j = lane + 64 * DELTA_SLOTS; j < stride; j += 64) {
         uint32_t w = p[j];
         if (w != LIST_SENTINEL) s += (double)x[LIST_VOX(w)];
@@ -119,10 +131,12 @@ __device__ __forceinline__ void delta_f32(const uint32_t* __restrict__ list, int
         int j = lane + 64 * r;
         v[r] = j < stride ? p[j] : LIST_SENTINEL;
     }
+    float xv[DELTA_SLOTS];
+    gather_slots(x, v, xv);
     float s = 0.0f, c = 0.0f;
 #pragma unroll
     for (int r = 0; r < DELTA_SLOTS; ++r)
-        if (v[r] != LIST_SENTINEL) kahan_add(s, c, x[LIST_VOX(v[r])]);
+        if (v[r] != LIST_SENTINEL) kahan_add(s, c, xv[r]);
     for (int j = lane + 64 * DELTA_SLOTS; j < stride; j += 64) {
         uint32_t w = p[j];
         if (w != LIST_SENTINEL) kahan_add(s, c, x[LIST_VOX(w)]);
@@ -167,10 +181,12 @@ __device__ __forceinline__ void delta_variant(int variant, const uint32_t* __res
         int j = lane + 64 * r;
         v[r] = j < stride ? p[j] : LIST_SENTINEL;
     }
+    float xv[DELTA_SLOTS];
+    gather_slots(x, v, xv);
     double s = 0.0;
 #pragma unroll
     for (int r = 0; r < DELTA_SLOTS; ++r)
-        if (v[r] != LIST_SENTINEL) s += (double)x[LIST_VOX(v[r])];
+        if (v[r] != LIST_SENTINEL) s += (double)xv[r];
     for (int j = lane + 64 * DELTA_SLOTS; j < stride; j += 64) {
         uint32_t w = p[j];
         if (w != LIST_SENTINEL) s += (double)x[LIST_VOX(w)];
@@ -190,6 +206,15 @@ __device__ __forceinline__ void delta_variant(int variant, const uint32_t* __res
     }
 }
 
+/* history indices i .. i+CSC_BATCH-1 of a long segment (cnt > i), every load
+ * issued before the first is used: positions past the end load the last
+ * index again, which the caller ignores */
+__device__ __forceinline__ void csc_batch(const uint16_t* __restrict__ q, uint32_t i, uint32_t cnt,
+                                          uint32_t (&idx)[CSC_BATCH]) {
+#pragma unroll
+    for (int t = 0; t < CSC_BATCH; ++t) idx[t] = q[min(i + (uint32_t)t, cnt - 1u)];
+}
+
 /* the block-local history indices of segment record r, in history order */
 template <class F>
 __device__ __forceinline__ uint32_t for_each_history(uint64_t r, const uint16_t* __restrict__ csc, F f) {
@@ -204,8 +229,7 @@ __device__ __forceinline__ uint32_t for_each_history(uint64_t r, const uint16_t*
     const uint16_t* q = csc + ((r >> 24) & 0x7ffffffu);
     for (uint32_t i = 0; i < cnt; i += CSC_BATCH) {
         uint32_t idx[CSC_BATCH];
-#pragma unroll
-        for (int t = 0; t < CSC_BATCH; ++t) idx[t] = i + t < cnt ? q[i + t] : 0u;
+        csc_batch(q, i, cnt, idx);
 #pragma unroll
         for (int t = 0; t < CSC_BATCH; ++t)
             if (i + t < cnt) f(idx[t]);
@@ -262,10 +286,9 @@ __device__ __forceinline__ float applied_f64(float xv, uint64_t r, const uint16_
         for (uint32_t i = 0; i < cnt; i += CSC_BATCH) {
             uint32_t idx[CSC_BATCH];
             double d[CSC_BATCH];
+            csc_batch(q, i, cnt, idx);
 #pragma unroll
-            for (int t = 0; t < CSC_BATCH; ++t) idx[t] = i + t < cnt ? q[i + t] : 0u;
-#pragma unroll
-            for (int t = 0; t < CSC_BATCH; ++t) d[t] = i + t < cnt ? delta[idx[t]] : 0.0;
+            for (int t = 0; t < CSC_BATCH; ++t) d[t] = delta[idx[t]];
 #pragma unroll
             for (int t = 0; t < CSC_BATCH; ++t)
                 if (i + t < cnt) upd += d[t];
@@ -293,10 +316,9 @@ __device__ __forceinline__ float applied_f32(float xv, uint64_t r, const uint16_
         for (uint32_t i = 0; i < cnt; i += CSC_BATCH) {
             uint32_t idx[CSC_BATCH];
             float d[CSC_BATCH];
+            csc_batch(q, i, cnt, idx);
 #pragma unroll
-            for (int t = 0; t < CSC_BATCH; ++t) idx[t] = i + t < cnt ? q[i + t] : 0u;
-#pragma unroll
-            for (int t = 0; t < CSC_BATCH; ++t) d[t] = i + t < cnt ? delta[idx[t]] : 0.0f;
+            for (int t = 0; t < CSC_BATCH; ++t) d[t] = delta[idx[t]];
 #pragma unroll
             for (int t = 0; t < CSC_BATCH; ++t)
                 if (i + t < cnt) kahan_add(upd, comp, d[t]);
