@@ -37,7 +37,8 @@ struct pct_plan {
     pct_buffer len;                /* u32 per history: list length          */
     pct_buffer chord;              /* f64 per history: effective chord      */
     pct_buffer hash;               /* u32 per history: hash of its list     */
-    /* block transpose: per block the touched voxels sorted by voxel, each an
+    /* block transpose: per block the touched voxels, those with more than the
+     * inline number of histories first, each part sorted by voxel, each an
      * 8-byte record {voxel, count | history indices}; csc holds all
      * block-local history indices in (block, voxel, history) order for the
      * records with more than the inline number */
@@ -45,10 +46,16 @@ struct pct_plan {
     pct_buffer csc;                /* u16 per entry                         */
     pct_buffer blk_seg;            /* int64 per block+1: first segment      */
     pct_buffer blk_entry;          /* int64 per block+1: first csc entry    */
+    /* the segments with more than the inline number of histories come first
+     * in each block's range; long_rows holds, per such segment in segment
+     * order, its first eight history indices (16 bytes; at most 6 B per entry
+     * with the record, under the 8 B an inline record takes) */
+    pct_buffer long_rows;          /* 16 B per long segment                 */
+    pct_buffer blk_long;           /* int64 per block+1: first long segment */
     pct_buffer delta;              /* 2 f64 per history of two blocks       */
     pct_buffer xw;                 /* u64 per voxel: {x, stamp} words of the
                                       one-launch-per-block form             */
-    std::vector<int64_t> blk_entry_h, blk_seg_h, list_base_h;
+    std::vector<int64_t> blk_entry_h, blk_seg_h, blk_long_h, list_base_h;
     std::vector<int32_t> stride_h;
     /* the block loop in chunks of blocks [chunk_b[c], chunk_b[c+1]), one
      * hipGraph each; chunk c may start when the MLP lists of its histories are

@@ -10,10 +10,11 @@
  *                k * stride[b], stride[b] = the longest list of the block,
  *                unused slots hold a sentinel; lengths, effective chord
  *                lengths and a hash of every list;
- *   block CSC    per DROP block the distinct voxels it touches, sorted by
- *                voxel, as 8-byte records {voxel, count, up to 3 block-local
- *                history indices in history order}; longer segments point
- *                into a u16 array of all indices.
+ *   block CSC    per DROP block the distinct voxels it touches as 8-byte
+ *                records {voxel, count, up to 3 block-local history indices
+ *                in history order}; longer segments point into a u16 array
+ *                of all indices, come first in the block (each part sorted
+ *                by voxel) and keep their first 8 indices in a side row.
  *
  * It is built with one stable radix sort by (block, voxel) over chunks of
  * blocks. An iteration is then two gather-only kernels per block, with no
@@ -33,8 +34,8 @@
  * two: the delta kernel finds a history's list from the launch arguments
  * (block base + k * stride) and reads the list and the history's scalars in
  * one round, then gathers x; the apply kernel reads one 8-byte record per voxel
- * (voxel + inline history indices) and then the deltas and the old x[v]
- * together. The block loop is captured once into hipGraphs of chunks of
+ * (voxel + inline history indices; a long segment's side row beside it) and
+ * then the deltas and the old x[v] together. The block loop is captured once into hipGraphs of chunks of
  * blocks and replayed. In the default mode every iteration recomputes every MLP
  * voxel list (as the reference does, .cu:5822) into the plan's list store and
  * checks it against the plan's hash; that FP64-bound pass runs on a second
@@ -227,13 +228,32 @@ __global__ void k_head_flags(const uint32_t* __restrict__ keys, int64_t E, int32
     if (i < E) flags[i] = (i == 0 || keys[i] != keys[i - 1]) ? 1 : 0;
 }
 
+/* the number of entries of the segment whose head is entry i (flags: heads) */
+__device__ __forceinline__ uint32_t seg_count(const int32_t* __restrict__ flags, int64_t i, int64_t E) {
+    int64_t j = i + 1;
+    while (j < E && !flags[j]) ++j;               /* a new block always starts a new segment */
+    return (uint32_t)(j - i);
+}
+
+/* long[i] = 1 at the head of a segment of more than SEG_INLINE entries */
+__global__ void k_long_flags(const int32_t* __restrict__ flags, int64_t E, int32_t* __restrict__ lng) {
+    int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (i < E) lng[i] = (flags[i] && seg_count(flags, i, E) > SEG_INLINE) ? 1 : 0;
+}
+
 /* segments of the chunk: the csc values of every entry, and at each segment
- * head the 16-byte record and, for a block's first segment, blk_seg */
+ * head the 8-byte record; a block's long segments (segidx / lngidx: exclusive
+ * prefix counts of the heads / long heads over the chunk, nlong_chunk the
+ * chunk's long heads) take its first record slots and a row of long_rows
+ * each, its other segments the slots after them, each part in voxel order;
+ * for a block's first entry blk_seg and blk_long */
 __global__ void k_write_segments(const uint32_t* __restrict__ keys, const uint32_t* __restrict__ vals,
-                                 const int32_t* __restrict__ flags, const int64_t* __restrict__ segidx, int64_t E,
-                                 int vbits, int64_t seg0, int64_t b0, int64_t entry0,
+                                 const int32_t* __restrict__ flags, const int64_t* __restrict__ segidx,
+                                 const int64_t* __restrict__ lngidx, int64_t nlong_chunk, int64_t E, int vbits,
+                                 int64_t seg0, int64_t long0, int64_t b0, int64_t entry0,
                                  const int64_t* __restrict__ blk_entry, uint64_t* __restrict__ segs,
-                                 uint16_t* __restrict__ csc, int64_t* __restrict__ blk_seg) {
+                                 uint16_t* __restrict__ csc, pct_blk::long_row* __restrict__ long_rows,
+                                 int64_t* __restrict__ blk_seg, int64_t* __restrict__ blk_long) {
     int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
     if (i >= E) return;
     uint32_t k = keys[i];
@@ -241,20 +261,34 @@ __global__ void k_write_segments(const uint32_t* __restrict__ keys, const uint32
     int64_t gi = entry0 + i;
     csc[gi] = (uint16_t)vals[i];
     if (!flags[i]) return;
-    int64_t s = seg0 + segidx[i];
-    int64_t j = i + 1;
-    while (j < E && !flags[j]) ++j;               /* a new block always starts a new segment */
-    uint32_t c = (uint32_t)(j - i);
+    const int64_t eb = blk_entry[b] - entry0, eb1 = blk_entry[b + 1] - entry0;
+    const int64_t seg_first = segidx[eb], long_first = lngidx[eb];
+    const int64_t nlong = (eb1 < E ? lngidx[eb1] : nlong_chunk) - long_first;
+    uint32_t c = seg_count(flags, i, E);
     uint64_t r = (uint64_t)(k & ((1u << vbits) - 1u));
+    int64_t s;
     if (c <= SEG_INLINE) {
         r |= (uint64_t)c << SEG_VOX_BITS;
         for (uint32_t q = 0; q < c; ++q) r |= (uint64_t)(vals[i + q] & 0x1fffu) << (24 + SEG_IDX_BITS * q);
+        s = seg0 + seg_first + nlong + ((segidx[i] - lngidx[i]) - (seg_first - long_first));
     } else {                                        /* inline count 0: indices in csc */
         r |= (uint64_t)(gi - blk_entry[b]) << 24;   /* block-relative first entry */
         r |= (uint64_t)c << 51;
+        s = seg0 + seg_first + (lngidx[i] - long_first);
+        pct_blk::long_row row;
+#pragma unroll
+        for (int t = 0; t < LONG_IDX / 2; ++t) {
+            const uint32_t lo = vals[i + min(2u * t, c - 1u)];
+            const uint32_t hi = vals[i + min(2u * t + 1u, c - 1u)];
+            row.w[t] = (lo & 0xffffu) | (hi << 16);
+        }
+        long_rows[long0 + lngidx[i]] = row;
     }
     segs[s] = r;
-    if (gi == blk_entry[b]) blk_seg[b] = s;
+    if (i == eb) {
+        blk_seg[b] = seg0 + seg_first;
+        blk_long[b] = long0 + long_first;
+    }
 }
 
 /* ------------------------------------------------------------------------- */
@@ -288,22 +322,24 @@ k_block_delta_f32(const uint32_t* __restrict__ list, const uint16_t* __restrict_
     pct_blk::delta_f32(list, stride, {pos_k, pos_len, pos_chord}, wepl, h0, q, x, lambda, delta, threadIdx.x & 63);
 }
 
+/* segs: the block's records (its long segments first), rows: their side rows */
 __global__ void __launch_bounds__(APPLY_THREADS)
-k_block_apply(const uint64_t* __restrict__ segs, int64_t s0, int64_t ns, const uint16_t* __restrict__ csc,
-              const double* __restrict__ delta, float* __restrict__ x) {
+k_block_apply(const uint64_t* __restrict__ segs, const pct_blk::long_row* __restrict__ rows, int64_t nlong, int64_t ns,
+              const uint16_t* __restrict__ csc, const double* __restrict__ delta, float* __restrict__ x) {
     __builtin_amdgcn_s_setprio(3);
     int64_t k = (int64_t)blockIdx.x * APPLY_THREADS + threadIdx.x;
     if (k >= ns) return;
-    pct_blk::apply_f64(segs[s0 + k], csc, delta, x);
+    pct_blk::apply_seg<false>(k, nlong, segs[k], rows, csc, delta, x);
 }
 
 __global__ void __launch_bounds__(APPLY_THREADS)
-k_block_apply_f32(const uint64_t* __restrict__ segs, int64_t s0, int64_t ns, const uint16_t* __restrict__ csc,
-                  const float* __restrict__ delta, float* __restrict__ x) {
+k_block_apply_f32(const uint64_t* __restrict__ segs, const pct_blk::long_row* __restrict__ rows, int64_t nlong,
+                  int64_t ns, const uint16_t* __restrict__ csc, const float* __restrict__ delta,
+                  float* __restrict__ x) {
     __builtin_amdgcn_s_setprio(3);
     int64_t k = (int64_t)blockIdx.x * APPLY_THREADS + threadIdx.x;
     if (k >= ns) return;
-    pct_blk::apply_f32(segs[s0 + k], csc, delta, x);
+    pct_blk::apply_seg<true>(k, nlong, segs[k], rows, csc, delta, x);
 }
 
 /* the research variants of the FP64 update (pct_plan_kernels.h) */
@@ -425,8 +461,8 @@ void pct_plan_destroy(pct_plan* pl) {
     if (!pl) return;
     plan_release_graph(pl);
     pct_buffer* bufs[] = {&pl->list, &pl->prev, &pl->hist_off, &pl->len, &pl->chord, &pl->hash, &pl->segs,
-                          &pl->csc, &pl->blk_seg, &pl->blk_entry, &pl->delta, &pl->xw, &pl->pos_k, &pl->pos_len,
-                          &pl->pos_chord};
+                          &pl->csc, &pl->long_rows, &pl->blk_seg, &pl->blk_long, &pl->blk_entry, &pl->delta,
+                          &pl->xw, &pl->pos_k, &pl->pos_len, &pl->pos_chord};
     for (pct_buffer* b : bufs) release(*b);
     for (hipEvent_t e : pl->ev_chunk)
         if (e) (void)hipEventDestroy(e);
@@ -621,27 +657,36 @@ static int plan_build(pct_ctx* ctx, const pct_hist_soa* hist, int B, int mlp_mod
         uint32_t* v0 = nullptr;
         PCT_HIP_CHECK(ctx, rocprim::radix_sort_pairs(nullptr, sort_tmp, k0, k0, v0, v0, ce, 0, 32, ctx->stream));
     }
-    pct_buffer kin, kout, vin, vout, fl, sidx, tmp;
+    pct_buffer kin, kout, vin, vout, fl, sidx, lfl, lidx, tmp;
     auto free_all = [&]() {
-        for (pct_buffer* b : {&kin, &kout, &vin, &vout, &fl, &sidx, &tmp, &ex_off}) release(*b);
+        for (pct_buffer* b : {&kin, &kout, &vin, &vout, &fl, &sidx, &lfl, &lidx, &tmp, &ex_off}) release(*b);
     };
     if ((rc = pct_ensure(ctx, kin, 4 * ce)) || (rc = pct_ensure(ctx, kout, 4 * ce)) || (rc = pct_ensure(ctx, vin, 4 * ce)) ||
         (rc = pct_ensure(ctx, vout, 4 * ce)) || (rc = pct_ensure(ctx, fl, 4 * ce)) ||
-        (rc = pct_ensure(ctx, sidx, 8 * (ce + 1))) || (rc = pct_ensure(ctx, tmp, sort_tmp))) {
+        (rc = pct_ensure(ctx, sidx, 8 * (ce + 1))) || (rc = pct_ensure(ctx, lfl, 4 * ce)) ||
+        (rc = pct_ensure(ctx, lidx, 8 * (ce + 1))) || (rc = pct_ensure(ctx, tmp, sort_tmp))) {
         free_all();
         return rc;
     }
-    /* first pass counts the segments per chunk, the second writes them */
-    std::vector<int64_t> chunk_segs;
-    int64_t S_total = 0;
+    if ((rc = pct_ensure(ctx, pl->blk_long, sizeof(int64_t) * (nb + 1)))) {
+        free_all();
+        return rc;
+    }
+    /* first pass counts the segments and the long segments per chunk, the
+     * second writes them */
+    std::vector<int64_t> chunk_segs, chunk_long;
+    int64_t S_total = 0, L_total = 0;
     for (int pass = 0; pass < 2; ++pass) {
-        int64_t seg0 = 0;
+        int64_t seg0 = 0, long0 = 0;
         for (int64_t b0 = 0; b0 < nb; b0 += chunk) {
             int64_t b1 = b0 + chunk < nb ? b0 + chunk : nb;
             int64_t h0 = b0 * B, h1 = b1 * B < n ? b1 * B : n;
             int64_t entry0 = blk_entry_h[b0];
             int64_t Ec = blk_entry_h[b1] - entry0;
-            if (Ec == 0) { if (pass == 0) chunk_segs.push_back(0); continue; }
+            if (Ec == 0) {
+                if (pass == 0) { chunk_segs.push_back(0); chunk_long.push_back(0); }
+                continue;
+            }
             int endbit = vbits + vbits_for(b1 - b0 + 1);
             if (endbit > 32) endbit = 32;
             hipLaunchKernelGGL(k_emit_keys, dim3((unsigned)((h1 - h0 + 3) / 4)), dim3(256), 0, ctx->stream, list,
@@ -653,39 +698,56 @@ static int plan_build(pct_ctx* ctx, const pct_hist_soa* hist, int B, int mlp_mod
                                                          ctx->stream));
             hipLaunchKernelGGL(k_head_flags, dim3(pct_grid(Ec, 256)), dim3(256), 0, ctx->stream,
                                (const uint32_t*)kout.ptr, Ec, (int32_t*)fl.ptr);
-            int64_t Sc = 0;
+            int64_t Sc = 0, Lc = 0;
             if ((rc = pct_exclusive_scan_i32(ctx, (const int32_t*)fl.ptr, Ec, (int64_t*)sidx.ptr, &Sc))) {
+                free_all();
+                return rc;
+            }
+            hipLaunchKernelGGL(k_long_flags, dim3(pct_grid(Ec, 256)), dim3(256), 0, ctx->stream,
+                               (const int32_t*)fl.ptr, Ec, (int32_t*)lfl.ptr);
+            if ((rc = pct_exclusive_scan_i32(ctx, (const int32_t*)lfl.ptr, Ec, (int64_t*)lidx.ptr, &Lc))) {
                 free_all();
                 return rc;
             }
             if (pass == 0) {
                 chunk_segs.push_back(Sc);
+                chunk_long.push_back(Lc);
                 continue;
             }
             hipLaunchKernelGGL(k_write_segments, dim3(pct_grid(Ec, 256)), dim3(256), 0, ctx->stream,
                                (const uint32_t*)kout.ptr, (const uint32_t*)vout.ptr, (const int32_t*)fl.ptr,
-                               (const int64_t*)sidx.ptr, Ec, vbits, seg0, b0, entry0,
-                               (const int64_t*)pl->blk_entry.ptr, (uint64_t*)pl->segs.ptr, (uint16_t*)pl->csc.ptr,
-                               (int64_t*)pl->blk_seg.ptr);
+                               (const int64_t*)sidx.ptr, (const int64_t*)lidx.ptr, Lc, Ec, vbits, seg0, long0, b0,
+                               entry0, (const int64_t*)pl->blk_entry.ptr, (uint64_t*)pl->segs.ptr,
+                               (uint16_t*)pl->csc.ptr, (pct_blk::long_row*)pl->long_rows.ptr,
+                               (int64_t*)pl->blk_seg.ptr, (int64_t*)pl->blk_long.ptr);
             PCT_HIP_CHECK(ctx, hipGetLastError());
             seg0 += Sc;
+            long0 += Lc;
         }
         if (pass == 0) {
             S_total = 0;
+            L_total = 0;
             for (int64_t s : chunk_segs) S_total += s;
-            if ((rc = pct_ensure(ctx, pl->segs, sizeof(uint64_t) * (S_total > 0 ? S_total : 1)))) {
+            for (int64_t l : chunk_long) L_total += l;
+            if ((rc = pct_ensure(ctx, pl->segs, sizeof(uint64_t) * (S_total > 0 ? S_total : 1))) ||
+                (rc = pct_ensure(ctx, pl->long_rows, sizeof(pct_blk::long_row) * (L_total > 0 ? L_total : 1)))) {
                 free_all();
                 return rc;
             }
         }
     }
     free_all();
-    /* sentinel */
+    /* sentinels */
     PCT_HIP_CHECK(ctx, hipMemcpyAsync((int64_t*)pl->blk_seg.ptr + nb, &S_total, sizeof(int64_t), hipMemcpyHostToDevice,
                                       ctx->stream));
+    PCT_HIP_CHECK(ctx, hipMemcpyAsync((int64_t*)pl->blk_long.ptr + nb, &L_total, sizeof(int64_t),
+                                      hipMemcpyHostToDevice, ctx->stream));
     pl->blk_entry_h = blk_entry_h;
     pl->blk_seg_h.assign(nb + 1, 0);
+    pl->blk_long_h.assign(nb + 1, 0);
     PCT_HIP_CHECK(ctx, hipMemcpyAsync(pl->blk_seg_h.data(), pl->blk_seg.ptr, sizeof(int64_t) * (nb + 1),
+                                      hipMemcpyDeviceToHost, ctx->stream));
+    PCT_HIP_CHECK(ctx, hipMemcpyAsync(pl->blk_long_h.data(), pl->blk_long.ptr, sizeof(int64_t) * (nb + 1),
                                       hipMemcpyDeviceToHost, ctx->stream));
     PCT_HIP_CHECK(ctx, hipStreamSynchronize(ctx->stream));
     /* 5. the list store (unused slots the sentinel) and, for the
@@ -896,13 +958,15 @@ static int plan_block_loop(pct_ctx* ctx, pct_plan* pl, const pct_hist_soa* hist,
                                pb.chord, stride, cnt, (const float*)x, (const float*)hist->wepl, h0, lambda,
                                (double*)delta);
         int64_t s0 = pl->blk_seg_h[b], ns = pl->blk_seg_h[b + 1] - s0;
+        const int64_t l0 = pl->blk_long_h[b], nlong = pl->blk_long_h[b + 1] - l0;
+        const pct_blk::long_row* rows = (const pct_blk::long_row*)pl->long_rows.ptr + l0;
         const uint16_t* cb = csc + pl->blk_entry_h[b];
         if (ns > 0 && f32)
-            hipLaunchKernelGGL(k_block_apply_f32, dim3(pct_grid(ns, APPLY_THREADS)), dim3(APPLY_THREADS), 0, ctx->stream, segs, s0, ns, cb,
-                               (const float*)delta, x);
+            hipLaunchKernelGGL(k_block_apply_f32, dim3(pct_grid(ns, APPLY_THREADS)), dim3(APPLY_THREADS), 0,
+                               ctx->stream, segs + s0, rows, nlong, ns, cb, (const float*)delta, x);
         else if (ns > 0)
-            hipLaunchKernelGGL(k_block_apply, dim3(pct_grid(ns, APPLY_THREADS)), dim3(APPLY_THREADS), 0, ctx->stream, segs, s0, ns, cb,
-                               (const double*)delta, x);
+            hipLaunchKernelGGL(k_block_apply, dim3(pct_grid(ns, APPLY_THREADS)), dim3(APPLY_THREADS), 0, ctx->stream,
+                               segs + s0, rows, nlong, ns, cb, (const double*)delta, x);
     }
     PCT_HIP_CHECK(ctx, hipGetLastError());
     return PCT_OK;
@@ -1081,8 +1145,8 @@ static int64_t plan_budget(pct_ctx* ctx) {
     int64_t budget = hipMemGetInfo(&fr, &tot) == hipSuccess ? (int64_t)fr : 0;
     if (ctx->plan) {
         pct_plan* pl = ctx->plan;
-        for (pct_buffer* b : {&pl->list, &pl->prev, &pl->segs, &pl->csc, &pl->hist_off, &pl->pos_k, &pl->pos_len,
-                              &pl->pos_chord, &pl->len, &pl->chord, &pl->hash})
+        for (pct_buffer* b : {&pl->list, &pl->prev, &pl->segs, &pl->csc, &pl->long_rows, &pl->hist_off, &pl->pos_k,
+                              &pl->pos_len, &pl->pos_chord, &pl->len, &pl->chord, &pl->hash})
             budget += (int64_t)b->bytes;
     }
     budget -= (int64_t)1 << 30;
@@ -1093,7 +1157,8 @@ static int64_t plan_budget(pct_ctx* ctx) {
 static void plan_free_buffers(pct_plan* pl) {
     plan_release_graph(pl);
     for (pct_buffer* b : {&pl->list, &pl->prev, &pl->hist_off, &pl->len, &pl->chord, &pl->hash, &pl->segs, &pl->csc,
-                          &pl->blk_seg, &pl->blk_entry, &pl->delta, &pl->xw, &pl->pos_k, &pl->pos_len, &pl->pos_chord})
+                          &pl->long_rows, &pl->blk_seg, &pl->blk_long, &pl->blk_entry, &pl->delta, &pl->xw, &pl->pos_k,
+                          &pl->pos_len, &pl->pos_chord})
         release(*b);
     pl->valid = false;
     ++pl->generation;

@@ -327,6 +327,96 @@ __device__ __forceinline__ float applied_f32(float xv, uint64_t r, const uint16_
     return xv + upd / (float)cnt;
 }
 
+/* ------------------------------------------------------------------------- */
+/* Long segments (more than SEG_INLINE histories) come first in each block's  */
+/* segment range, and the first LONG_IDX of their block-local history indices */
+/* are also kept at a fixed stride in a side array (one 16-byte row per long  */
+/* segment, in segment order; the unused tail of a row repeats the last       */
+/* index). The two-launch apply kernels read a long segment's row beside its  */
+/* record, so its deltas load in the same round as an inline segment's; only  */
+/* indices past LONG_IDX go through csc.                                      */
+/* ------------------------------------------------------------------------- */
+#define LONG_IDX 8
+
+struct long_row {
+    uint32_t w[LONG_IDX / 2];
+};
+
+__device__ __forceinline__ uint32_t long_idx(const long_row& row, int t) {
+    return (row.w[t >> 1] >> (16 * (t & 1))) & 0xffffu;
+}
+
+/* the new value of a long segment's voxel: as applied_f64, with the first
+ * LONG_IDX history indices from its side row */
+__device__ __forceinline__ float applied_long_f64(float xv, uint64_t r, const long_row& row,
+                                                  const uint16_t* __restrict__ csc, const double* __restrict__ delta) {
+    const uint32_t cnt = (uint32_t)(r >> 51);
+    double d[LONG_IDX];
+#pragma unroll
+    for (int t = 0; t < LONG_IDX; ++t) d[t] = delta[long_idx(row, t)];
+    double upd = 0.0;
+#pragma unroll
+    for (int t = 0; t < LONG_IDX; ++t)
+        if ((uint32_t)t < cnt) upd += d[t];
+    if (cnt > LONG_IDX) {
+        const uint16_t* q = csc + ((r >> 24) & 0x7ffffffu);
+        for (uint32_t i = LONG_IDX; i < cnt; i += CSC_BATCH) {
+            uint32_t idx[CSC_BATCH];
+            double e[CSC_BATCH];
+            csc_batch(q, i, cnt, idx);
+#pragma unroll
+            for (int t = 0; t < CSC_BATCH; ++t) e[t] = delta[idx[t]];
+#pragma unroll
+            for (int t = 0; t < CSC_BATCH; ++t)
+                if (i + t < cnt) upd += e[t];
+        }
+    }
+    return (float)((double)xv + upd / (double)cnt);
+}
+
+__device__ __forceinline__ float applied_long_f32(float xv, uint64_t r, const long_row& row,
+                                                  const uint16_t* __restrict__ csc, const float* __restrict__ delta) {
+    const uint32_t cnt = (uint32_t)(r >> 51);
+    float d[LONG_IDX];
+#pragma unroll
+    for (int t = 0; t < LONG_IDX; ++t) d[t] = delta[long_idx(row, t)];
+    float upd = 0.0f, comp = 0.0f;
+#pragma unroll
+    for (int t = 0; t < LONG_IDX; ++t)
+        if ((uint32_t)t < cnt) kahan_add(upd, comp, d[t]);
+    if (cnt > LONG_IDX) {
+        const uint16_t* q = csc + ((r >> 24) & 0x7ffffffu);
+        for (uint32_t i = LONG_IDX; i < cnt; i += CSC_BATCH) {
+            uint32_t idx[CSC_BATCH];
+            float e[CSC_BATCH];
+            csc_batch(q, i, cnt, idx);
+#pragma unroll
+            for (int t = 0; t < CSC_BATCH; ++t) e[t] = delta[idx[t]];
+#pragma unroll
+            for (int t = 0; t < CSC_BATCH; ++t)
+                if (i + t < cnt) kahan_add(upd, comp, e[t]);
+        }
+    }
+    return xv + upd / (float)cnt;
+}
+
+/* segment k of a block (record r): long rows for k < nlong */
+template <bool F32>
+__device__ __forceinline__ void apply_seg(int64_t k, int64_t nlong, uint64_t r, const long_row* __restrict__ rows,
+                                          const uint16_t* __restrict__ csc, const void* __restrict__ delta,
+                                          float* __restrict__ x) {
+    const uint32_t v = (uint32_t)(r & (SEG_MAX_V - 1));
+    if (k < nlong) {
+        const long_row row = rows[k];
+        const float xv = x[v];
+        x[v] = F32 ? applied_long_f32(xv, r, row, csc, (const float*)delta)
+                   : applied_long_f64(xv, r, row, csc, (const double*)delta);
+    } else {
+        const float xv = x[v];
+        x[v] = F32 ? applied_f32(xv, r, csc, (const float*)delta) : applied_f64(xv, r, csc, (const double*)delta);
+    }
+}
+
 /* segment record r of a block; csc points at the block's first entry */
 __device__ __forceinline__ void apply_f64(uint64_t r, const uint16_t* __restrict__ csc,
                                           const double* __restrict__ delta, float* __restrict__ x) {
