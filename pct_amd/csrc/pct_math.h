@@ -76,11 +76,23 @@ PCT_HD double pct_round_int(double x) {
 /* natural logarithm; div(a, b) must return the IEEE quotient a / b for
  * a = f in [-0.293, 0.415] (or +0) and b = 2 + f, the one division */
 template <class Div>
+PCT_HD double pct_log_core(double x, int k, Div div);
+template <class Div>
 PCT_HD double pct_log_div(double x, Div div) {
     if (!(x > 0.0)) return x == 0.0 ? -__builtin_inf() : __builtin_nan("");
     if (pct_bits(x) >= 0x7ff0000000000000ULL) return x; /* +inf */
     int k = 0;
     if (x < 2.2250738585072014e-308) { x *= 18014398509481984.0; k = -54; } /* 2^54 */
+    return pct_log_core(x, k, div);
+}
+/* pct_log_div for a positive normal finite x: the same arithmetic without
+ * the special cases (the caller guarantees the range) */
+template <class Div>
+PCT_HD double pct_log_normal(double x, Div div) { return pct_log_core(x, 0, div); }
+/* log(x) + k ln 2 - k ln 2 of a positive normal x: the reduction and the
+ * series (k is the exponent offset of a pre-scaled subnormal) */
+template <class Div>
+PCT_HD double pct_log_core(double x, int k, Div div) {
     uint64_t u = pct_bits(x);
     k += (int)((u >> 52) & 0x7ff) - 1023;
     double m = pct_from_bits((u & 0x000fffffffffffffULL) | 0x3ff0000000000000ULL);
