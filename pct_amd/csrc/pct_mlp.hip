@@ -657,7 +657,10 @@ struct table_walk {
     __device__ int64_t fix(const pct_geo& g, const H& b, int k) const { return step(g, b, k); }
 };
 
-/* MLP paths. Each wavefront owns 64 consecutive histories: lane j computes the
+/* MLP paths, compiled for at least 5 wavefronts per SIMD (102 registers: the
+ * exact walk keeps a few per-history values in scratch, outside its step
+ * loop; 56.6 against 58.0 ms at 4 wavefronts over the 200x200x32 scan).
+ * Each wavefront owns 64 consecutive histories: lane j computes the
  * per-history constants of history j, then the wave walks the histories one
  * by one with the 64 lanes on 64 consecutive steps. With LDS_TAB the
  * per-depth table (7 FP64 fields per step, read by every step of every
@@ -669,7 +672,7 @@ struct table_walk {
  * per wavefront chosen from the list's length so that a short list spreads
  * over the waves instead of queueing in a few. */
 template <class W, bool LDS_TAB, bool IDX = false>
-__global__ void __launch_bounds__(256)
+__global__ void __launch_bounds__(256, 5)
 k_mlp_paths(pct_geo g, pct_hist_soa h, int64_t start, int64_t count, W walk,
             const int64_t* __restrict__ offsets, uint32_t* __restrict__ voxels, uint32_t* __restrict__ lengths,
             double* __restrict__ chord_out, int* __restrict__ status, uint32_t* __restrict__ hash_out,
@@ -874,14 +877,19 @@ struct sur_lds {
     float Q[40];              /* current history's cubics       */
 };
 
-/* setup of the 64 histories of a wavefront: lane j's MLP constants, its
- * network inputs into LDS, then the network for the whole wave when any lane
- * needs it */
-__device__ __forceinline__ void sur_wave_setup(const pct_geo& g, const pct_hist_soa& h, int64_t i, bool have,
-                                               const double* __restrict__ tab, int K, const pct_net_frag* __restrict__ nf,
-                                               const float* __restrict__ range, const uint8_t* __restrict__ cells,
-                                               bool use_cells, sur_lds& L, mlp_hist& m, bool& use, bool& in_range,
-                                               int& cell) {
+/* the network for the whole wave when any lane needs it (inputs in L.xs,
+ * outputs to L.out) */
+__device__ __forceinline__ void sur_wave_infer(const pct_net_frag* __restrict__ nf, sur_lds& L, bool use) {
+    if (__ballot(use)) pct_net_infer_wave(nf, L.xs, L.out);
+    wave_lds_sync();
+}
+
+/* setup of the 64 histories of a wavefront: lane j's MLP constants and its
+ * network inputs into LDS (sur_wave_infer then runs the network) */
+__device__ __forceinline__ void sur_wave_prep(const pct_geo& g, const pct_hist_soa& h, int64_t i, bool have,
+                                              const double* __restrict__ tab, int K, const float* __restrict__ range,
+                                              const uint8_t* __restrict__ cells, bool use_cells, sur_lds& L,
+                                              mlp_hist& m, bool& use, bool& in_range, int& cell) {
     const int lane = threadIdx.x & 63;
     float f[PCT_NET_IN];
 #pragma unroll
@@ -898,8 +906,6 @@ __device__ __forceinline__ void sur_wave_setup(const pct_geo& g, const pct_hist_
 #pragma unroll
     for (int k = 0; k < 16; ++k) L.xs[lane * 16 + k] = k < PCT_NET_IN ? (uint16_t)pct_f2bf(f[k]) : (uint16_t)0;
     wave_lds_sync();
-    if (__ballot(use)) pct_net_infer_wave(nf, L.xs, L.out);
-    wave_lds_sync();
 }
 
 /* Surrogate MLP paths: like k_mlp_paths, one wavefront per 64 histories; a
@@ -908,7 +914,10 @@ __device__ __forceinline__ void sur_wave_setup(const pct_geo& g, const pct_hist_
  * wavefront) for the exact walk's launch that follows (k_mlp_paths with the
  * list), so that this kernel holds no FP64 walk state. counts[0] +=
  * fallbacks, counts[1] += histories. */
-__global__ void __launch_bounds__(256)
+/* at least four wavefronts per SIMD: the register budget (128) is set by the
+ * walk; the network's pass over the wave's 64 histories, once per 64
+ * histories, keeps a few values in scratch to fit it */
+__global__ void __launch_bounds__(256, 4)
 k_mlp_paths_sur(pct_geo g, pct_hist_soa h, int64_t start, int64_t count, const double* __restrict__ tab, int K,
                 const float* __restrict__ u1f, grid_f gf, const pct_net_frag* __restrict__ nf,
                 const float* __restrict__ range, const uint8_t* __restrict__ cells, const int64_t* __restrict__ offsets,
@@ -928,23 +937,27 @@ k_mlp_paths_sur(pct_geo g, pct_hist_soa h, int64_t start, int64_t count, const d
     for (int64_t base = ((int64_t)blockIdx.x * blockDim.x + (threadIdx.x & ~63)); base < count;
          base += (int64_t)gridDim.x * blockDim.x) {
         const int64_t mine = base + lane;
-        mlp_hist m;
         bool use, in_range;
-        int cell;
-        sur_wave_setup(g, h, start + mine, mine < count, tab, K, nf, range, cells, true, L, m, use, in_range, cell);
         int mS = 0;
         uint32_t first = 0;
         int64_t off = 0, cap = 0;
         sur_hist sh{};
-        if (mine < count) {
-            mS = m.S;
-            first = h.first_voxel[start + mine];
-            off = voxels ? offsets[mine] : 0;
-            cap = voxels ? offsets[mine + 1] - off : ((int64_t)1 << 40);
-            if (chord_out) chord_out[mine] = m.chord;
-            if (mS >= K) atomicOr(status, PCT_STATUS_TABLE);
-            sh = sur_setup(m);
+        {
+            /* the FP64 constants end before the network runs */
+            mlp_hist m;
+            int cell;
+            sur_wave_prep(g, h, start + mine, mine < count, tab, K, range, cells, true, L, m, use, in_range, cell);
+            if (mine < count) {
+                mS = m.S;
+                first = h.first_voxel[start + mine];
+                off = voxels ? offsets[mine] : 0;
+                cap = voxels ? offsets[mine + 1] - off : ((int64_t)1 << 40);
+                if (chord_out) chord_out[mine] = m.chord;
+                if (mS >= K) atomicOr(status, PCT_STATUS_TABLE);
+                sh = sur_setup(m);
+            }
         }
+        sur_wave_infer(nf, L, use);
         const uint64_t umask = __ballot(use);
         const int nh = (int)((count - base) < 64 ? (count - base) : 64);
         tot += nh;
@@ -1016,8 +1029,8 @@ k_net_calib(pct_geo g, pct_hist_soa h, int64_t start, int64_t stride, int64_t ns
         mlp_hist m;
         bool use, in_range;
         int cell;
-        sur_wave_setup(g, h, start + q * stride, q < nsamp, tab, K, nf, range, nullptr, false, L, m, use, in_range,
-                       cell);
+        sur_wave_prep(g, h, start + q * stride, q < nsamp, tab, K, range, nullptr, false, L, m, use, in_range, cell);
+        sur_wave_infer(nf, L, use);
         int mS = 0;
         uint32_t first = 0;
         sur_hist sh{};
@@ -1264,10 +1277,11 @@ int pct_mlp_paths_fb(pct_ctx* ctx, hipStream_t stream, const pct_hist_soa* hist,
     /* a workgroup of 256 is one wavefront on each of the CU's 4 SIMDs. Beside
      * a single scan's block loop the walks leave room (wave slots and
      * registers) for the latency-bound block kernels (measured at 200x200x32:
-     * the exact and the table walk best at 3 wavefronts per SIMD, the
-     * surrogate at 2, its most; the batched form's steps do better with the
-     * full occupancy) */
-    static const int beside_default[3] = {3, 3, 2};
+     * the exact walk, compiled for 5 wavefronts per SIMD, best at 4 beside
+     * (62.2 ms per iteration against 65.2 at 3), the table walk at 3, the
+     * surrogate at 2 (55.4 ms against 56.2 at 3 and 68.3 at 4); the batched
+     * form's steps do better with the full occupancy) */
+    static const int beside_default[3] = {4, 3, 2};
     /* workgroups per CU of a walk launch: the walk's occupancy, capped beside
      * the block loop or by the knob; for the table walks the per-depth table
      * goes to LDS when the workgroups the launch keeps on a CU fit in its

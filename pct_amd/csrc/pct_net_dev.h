@@ -69,7 +69,13 @@ __device__ static inline void pct_net_infer_wave(const pct_net_frag* __restrict_
     const int lane = threadIdx.x & 63;
     const int g = lane >> 4, c16 = lane & 15;
     const net_f32x4 zero = {0.0f, 0.0f, 0.0f, 0.0f};
+#pragma nounroll
     for (int nt = 0; nt < 4; ++nt) {
+        /* a compiler-only barrier (no instruction): the weights are read again
+         * for each group of 16 histories instead of being held in registers
+         * across the groups, which would raise the register count of every
+         * kernel that runs the network */
+        __asm__ volatile("" ::: "memory");
         /* B of layer 1: inputs 4g..4g+3 of history 16 nt + l%16 */
         net_u32x2 xb = *(const net_u32x2*)(xs + (16 * nt + c16) * 16 + 4 * g);
         net_u32x4 b2op[2];
