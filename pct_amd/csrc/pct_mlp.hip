@@ -38,8 +38,6 @@ __device__ __forceinline__ int calc_voxel(double zero, double pos, double size) 
 }
 #define PCT_LDS_PER_CU (160u * 1024u)        /* LDS of one CDNA4 compute unit */
 #define PCT_MLP_LDS_TAB_MAX (64u * 1024u)   /* largest per-depth table staged in LDS */
-/* field stride of the LDS copy of a per-depth table of at most this many steps */
-#define PCT_MLP_LDS_KS 512
 
 /* calc_voxel with the division replaced by a product with the rounded
  * reciprocal. q = |pos - zero| * inv carries two roundings (of inv and of the
@@ -596,17 +594,11 @@ __device__ __forceinline__ int64_t mlp_step_table(const pct_geo& g, const grid_f
 }
 
 /* The two walks as policies of one paths kernel: setup of lane j's history,
- * its broadcast to the wave, and one step. The per-depth table holds K steps
- * per field; field f of step k is at tab[f * stride() + k]: stride K for the
- * table in global memory, the compile-time KS for a copy in LDS laid out
- * with a fixed field stride (every field's read then takes an immediate
- * offset from one address). */
-template <int KS = 0>
+ * its broadcast to the wave, and one step. */
 struct exact_walk {
     const double* tab;
     int K;
     typedef mlp_hist H;
-    __device__ int stride() const { return KS ? KS : K; }
     __device__ void setup(const pct_geo& g, const pct_hist_soa& h, int64_t i, H& m, double& chord, int& S) const {
         mlp_setup(g, h, i, tab + MLP_TAB_U1 * K, K, m);
         chord = m.chord;
@@ -621,11 +613,10 @@ struct exact_walk {
         b.pre3 = bcast_d(m.pre3, j);
         return b;
     }
-    __device__ int64_t step(const pct_geo& g, const H& b, int k) const { return mlp_step_r(g, b, tab, stride(), k); }
-    __device__ int64_t fix(const pct_geo& g, const H& b, int k) const { return mlp_step(g, b, tab, stride(), k); }
+    __device__ int64_t step(const pct_geo& g, const H& b, int k) const { return mlp_step_r(g, b, tab, K, k); }
+    __device__ int64_t fix(const pct_geo& g, const H& b, int k) const { return mlp_step(g, b, tab, K, k); }
 };
 
-template <int KS = 0>
 struct table_walk {
     const double* tab;
     int K;
@@ -633,7 +624,6 @@ struct table_walk {
     int Q;
     grid_f gf;
     typedef mlp_thist H;
-    __device__ int stride() const { return KS ? KS : K; }
     __device__ void setup(const pct_geo& g, const pct_hist_soa& h, int64_t i, H& t, double& chord, int& S) const {
         mlp_hist m;
         mlp_setup(g, h, i, tab + MLP_TAB_U1 * K, K, m);
@@ -652,7 +642,7 @@ struct table_walk {
         return b;
     }
     __device__ int64_t step(const pct_geo& g, const H& b, int k) const {
-        return mlp_step_table(g, gf, b, tab, stride(), scat, Q, k);
+        return mlp_step_table(g, gf, b, tab, K, scat, Q, k);
     }
     __device__ int64_t fix(const pct_geo& g, const H& b, int k) const { return step(g, b, k); }
 };
@@ -691,13 +681,8 @@ k_mlp_paths(pct_geo g, pct_hist_soa h, int64_t start, int64_t count, W walk,
         if ((int64_t)blockIdx.x * (blockDim.x >> 6) * G >= count) return;
     }
     if (LDS_TAB) {
-        /* the table from field stride K (global) to the walk's stride */
-        const int Kt = walk.K, ks = walk.stride();
-        const int n = MLP_TAB_FIELDS * Kt;
-        for (int i = threadIdx.x; i < n; i += blockDim.x) {
-            const int f = i / Kt;
-            s_mlp_tab[f * ks + (i - f * Kt)] = walk.tab[i];
-        }
+        const int n = MLP_TAB_FIELDS * walk.K;
+        for (int i = threadIdx.x; i < n; i += blockDim.x) s_mlp_tab[i] = walk.tab[i];
         __syncthreads();
         walk.tab = s_mlp_tab;
     }
@@ -1044,7 +1029,7 @@ k_net_calib(pct_geo g, pct_hist_soa h, int64_t start, int64_t stride, int64_t ns
             const int S = bcast_i(mS, j);
             const uint32_t carry = (uint32_t)bcast_i((int)first, j);
             sur_hist b = sur_bcast(sh, j);
-            exact_walk<> ew{tab, K};
+            exact_walk ew{tab, K};
             mlp_hist e = ew.bcast(m, j);
             if (lane < 40) L.Q[lane] = sur_coef(lane, b, L.out + 8 * j);
             wave_lds_sync();
@@ -1198,9 +1183,9 @@ static int mlp_occupancy(pct_ctx* ctx) {
     if (ctx->cus) return PCT_OK;
     PCT_HIP_CHECK(ctx, hipDeviceGetAttribute(&ctx->cus, hipDeviceAttributeMultiprocessorCount, ctx->device));
     PCT_HIP_CHECK(ctx, hipOccupancyMaxActiveBlocksPerMultiprocessor(&ctx->mlp_occ[PCT_MLP_FP64],
-                                                                    k_mlp_paths<exact_walk<>, false>, 256, 0));
+                                                                    k_mlp_paths<exact_walk, false>, 256, 0));
     PCT_HIP_CHECK(ctx, hipOccupancyMaxActiveBlocksPerMultiprocessor(&ctx->mlp_occ[PCT_MLP_TABLE],
-                                                                    k_mlp_paths<table_walk<>, false>, 256, 0));
+                                                                    k_mlp_paths<table_walk, false>, 256, 0));
     PCT_HIP_CHECK(ctx, hipOccupancyMaxActiveBlocksPerMultiprocessor(&ctx->mlp_occ[PCT_MLP_SURROGATE], k_mlp_paths_sur,
                                                                     256, sizeof(float) * ctx->mlp_tab_k));
     ctx->cus = std::max(1, ctx->cus);
@@ -1286,21 +1271,13 @@ int pct_mlp_paths_fb(pct_ctx* ctx, hipStream_t stream, const pct_hist_soa* hist,
      * the block loop or by the knob; for the table walks the per-depth table
      * goes to LDS when the workgroups the launch keeps on a CU fit in its
      * 160 KB beside their registers' limit */
-    auto walk_shape = [&](int walk_mode, int& per_cu, bool& lds, size_t& shm, bool& fixed) {
+    auto walk_shape = [&](int walk_mode, int& per_cu, bool& lds, size_t& shm) {
         per_cu = std::max(1, ctx->mlp_occ[walk_mode]);
         const int lim = ctx->tune_mlp_waves > 0 ? ctx->tune_mlp_waves : (beside ? beside_default[walk_mode] : 0);
         if (lim > 0 && lim < per_cu) per_cu = lim;
         lds = false;
-        fixed = false;
         shm = 0;
         if (walk_mode == PCT_MLP_SURROGATE) return;
-        /* the LDS copy with the fixed field stride when the table fits it */
-        const size_t fixed_bytes = sizeof(double) * MLP_TAB_FIELDS * (size_t)PCT_MLP_LDS_KS;
-        if (ctx->mlp_tab_k <= PCT_MLP_LDS_KS && fixed_bytes * (size_t)per_cu <= PCT_LDS_PER_CU) {
-            lds = fixed = true;
-            shm = fixed_bytes;
-            return;
-        }
         const size_t tab_bytes = sizeof(double) * MLP_TAB_FIELDS * (size_t)ctx->mlp_tab_k;
         lds = tab_bytes <= PCT_MLP_LDS_TAB_MAX && tab_bytes * (size_t)per_cu <= PCT_LDS_PER_CU;
         if (!lds && tab_bytes <= PCT_MLP_LDS_TAB_MAX && PCT_LDS_PER_CU / tab_bytes >= 3) {
@@ -1312,40 +1289,26 @@ int pct_mlp_paths_fb(pct_ctx* ctx, hipStream_t stream, const pct_hist_soa* hist,
     };
     auto exact_launch = [&](const uint32_t* idx, const unsigned long long* idx_count) {
         int per_cu;
-        bool lds, fixed;
+        bool lds;
         size_t shm;
-        walk_shape(PCT_MLP_FP64, per_cu, lds, shm, fixed);
+        walk_shape(PCT_MLP_FP64, per_cu, lds, shm);
         const unsigned tgrid = std::min(pct_grid(count, 256), (unsigned)(ctx->cus * per_cu));
-        const double* tab = (const double*)ctx->d_mlp_tab;
-        const int K = ctx->mlp_tab_k;
-        if (fixed) {
-            exact_walk<PCT_MLP_LDS_KS> w{tab, K};
-            if (idx)
-                hipLaunchKernelGGL((k_mlp_paths<exact_walk<PCT_MLP_LDS_KS>, true, true>), dim3(tgrid), dim3(256), shm,
-                                   stream, ctx->g, *hist, start, count, w, offsets, voxels, lengths, chord,
-                                   ctx->d_status, hash_out, hash_expect, idx, idx_count);
-            else
-                hipLaunchKernelGGL((k_mlp_paths<exact_walk<PCT_MLP_LDS_KS>, true>), dim3(tgrid), dim3(256), shm,
-                                   stream, ctx->g, *hist, start, count, w, offsets, voxels, lengths, chord,
-                                   ctx->d_status, hash_out, hash_expect, nullptr, nullptr);
-            return;
-        }
-        exact_walk<> w{tab, K};
+        exact_walk w{(const double*)ctx->d_mlp_tab, ctx->mlp_tab_k};
         if (idx)
-            hipLaunchKernelGGL((lds ? k_mlp_paths<exact_walk<>, true, true> : k_mlp_paths<exact_walk<>, false, true>),
+            hipLaunchKernelGGL((lds ? k_mlp_paths<exact_walk, true, true> : k_mlp_paths<exact_walk, false, true>),
                                dim3(tgrid), dim3(256), shm, stream, ctx->g, *hist, start, count, w, offsets, voxels,
                                lengths, chord, ctx->d_status, hash_out, hash_expect, idx, idx_count);
         else
-            hipLaunchKernelGGL((lds ? k_mlp_paths<exact_walk<>, true> : k_mlp_paths<exact_walk<>, false>), dim3(tgrid),
+            hipLaunchKernelGGL((lds ? k_mlp_paths<exact_walk, true> : k_mlp_paths<exact_walk, false>), dim3(tgrid),
                                dim3(256), shm, stream, ctx->g, *hist, start, count, w, offsets, voxels, lengths, chord,
                                ctx->d_status, hash_out, hash_expect, nullptr, nullptr);
     };
     if (mode == PCT_MLP_SURROGATE) {
         if (!fb_idx || !fb_count) return pct_fail(ctx, PCT_E_INVALID, "surrogate MLP paths without a fallback list");
         int per_cu;
-        bool lds, fixed;
+        bool lds;
         size_t shm;
-        walk_shape(PCT_MLP_SURROGATE, per_cu, lds, shm, fixed);
+        walk_shape(PCT_MLP_SURROGATE, per_cu, lds, shm);
         const unsigned grid = std::min(pct_grid(count, 256), (unsigned)(ctx->cus * per_cu));
         hipLaunchKernelGGL(k_zero_u64, dim3(1), dim3(64), 0, stream, fb_count);
         hipLaunchKernelGGL(k_mlp_paths_sur, dim3(grid), dim3(256), sizeof(float) * ctx->mlp_tab_k, stream, ctx->g, *hist,
@@ -1359,23 +1322,15 @@ int pct_mlp_paths_fb(pct_ctx* ctx, hipStream_t stream, const pct_hist_soa* hist,
         exact_launch(nullptr, nullptr);
     } else {
         int per_cu;
-        bool lds, fixed;
+        bool lds;
         size_t shm;
-        walk_shape(PCT_MLP_TABLE, per_cu, lds, shm, fixed);
+        walk_shape(PCT_MLP_TABLE, per_cu, lds, shm);
         const unsigned tgrid = std::min(pct_grid(count, 256), (unsigned)(ctx->cus * per_cu));
-        const double* tab = (const double*)ctx->d_mlp_tab;
-        const double* scat = (const double*)ctx->d_scat_tab;
-        if (fixed) {
-            table_walk<PCT_MLP_LDS_KS> w{tab, ctx->mlp_tab_k, scat, ctx->scat_q, make_grid_f(ctx->g)};
-            hipLaunchKernelGGL((k_mlp_paths<table_walk<PCT_MLP_LDS_KS>, true>), dim3(tgrid), dim3(256), shm, stream,
-                               ctx->g, *hist, start, count, w, offsets, voxels, lengths, chord, ctx->d_status,
-                               hash_out, hash_expect, nullptr, nullptr);
-        } else {
-            table_walk<> w{tab, ctx->mlp_tab_k, scat, ctx->scat_q, make_grid_f(ctx->g)};
-            hipLaunchKernelGGL((lds ? k_mlp_paths<table_walk<>, true> : k_mlp_paths<table_walk<>, false>), dim3(tgrid),
-                               dim3(256), shm, stream, ctx->g, *hist, start, count, w, offsets, voxels, lengths,
-                               chord, ctx->d_status, hash_out, hash_expect, nullptr, nullptr);
-        }
+        table_walk w{(const double*)ctx->d_mlp_tab, ctx->mlp_tab_k, (const double*)ctx->d_scat_tab, ctx->scat_q,
+                     make_grid_f(ctx->g)};
+        hipLaunchKernelGGL((lds ? k_mlp_paths<table_walk, true> : k_mlp_paths<table_walk, false>), dim3(tgrid),
+                           dim3(256), shm, stream, ctx->g, *hist, start, count, w, offsets, voxels, lengths, chord,
+                           ctx->d_status, hash_out, hash_expect, nullptr, nullptr);
     }
     PCT_HIP_CHECK(ctx, hipGetLastError());
     return PCT_OK;
