@@ -89,10 +89,19 @@ def batched_form(log):
         torch.cuda.synchronize()
         return e0.elapsed_time(e1) / steps
 
-    for name, mode in (("fp64", 0), ("precision", pct_amd.PCT_PRECISION)):
+    P = pct_amd.PCT_DROP_PERSISTENT
+    for name, mode in (("fp64", 0), ("precision", pct_amd.PCT_PRECISION), ("fp64_persistent", P),
+                       ("precision_persistent", pct_amd.PCT_PRECISION | P)):
         ms = timed_batch(mode)
         res[name] = {"ms_per_iteration": round(ms, 3), "histories_per_s": round(n / (ms / 1e3), 1)}
         log(f"batched {name}: {ms:.2f} ms/iteration, {n / (ms / 1e3) / 1e6:.1f} M histories/s")
+    # the block steps alone (cached lists): two launches per step (graphs)
+    # against one persistent launch per iteration
+    res["block_steps_cached_ms"] = {"two_launches_per_step": round(timed_batch(pct_amd.PCT_DROP_CACHED), 3),
+                                    "persistent": round(timed_batch(pct_amd.PCT_DROP_CACHED | P), 3)}
+    res["persistent_note"] = ("PCT_DROP_PERSISTENT: every block step of an iteration in one launch with grid "
+                              "barriers; its MLP check runs before the launch instead of beside the steps")
+    log(f"batched block steps alone (cached lists): {res['block_steps_cached_ms']}")
     seq = 0.0
     for ctx, h, x0, b in zip(ctxs, hists, x0s, B):
         x = x0.clone()

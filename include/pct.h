@@ -116,6 +116,14 @@ typedef struct pct_hist_soa {
                                     (.cu:7871-7901): the plain update scaled per
                                     voxel by N_v / (N_v + psi_v), N_v the sum of
                                     the block's L_i^2 through v                */
+#define PCT_DROP_PERSISTENT 0x4000 /* pct_batch_iteration only: every block step
+                                    of the iteration in one persistent launch
+                                    (grid barriers between the delta and update
+                                    phases); the MLP check runs before it, not
+                                    beside it. Results are identical to the
+                                    default two launches per step. A barrier
+                                    wait is bounded: one that gives up returns
+                                    PCT_E_HIP with every image unchanged.      */
 
 typedef struct pct_ctx pct_ctx;
 
@@ -395,11 +403,13 @@ int pct_drop_stats(pct_ctx* ctx, const pct_hist_soa* hist, int block_size, int64
                                      plan may take (default: free memory); a
                                      scan whose plan exceeds it runs in chunks
                                      of blocks (see pct_drop_plan_choice)     */
-#define PCT_TUNE_WAIT_GIVE_UP  7  /* diagnostics of block form 3: 1 makes its
-                                     waits target a stamp no update writes and
-                                     give up after two polls, so that the
-                                     failure path (PCT_E_HIP, x unchanged) can
-                                     be exercised; 0 (default) off            */
+#define PCT_TUNE_WAIT_GIVE_UP  7  /* diagnostics of block form 3 and of a
+                                     persistent batch (PCT_DROP_PERSISTENT,
+                                     set on the batch's first context): 1 makes
+                                     their waits target a value nothing writes
+                                     and give up after two polls, so that the
+                                     failure path (PCT_E_HIP, images unchanged)
+                                     can be exercised; 0 (default) off         */
 int pct_set_tuning(pct_ctx* ctx, int key, int value);
 
 /* Capacity (no counterpart in the reference, which walks every path inside
@@ -438,7 +448,7 @@ void       pct_batch_destroy(pct_batch* b);
 int        pct_batch_prepare(pct_batch* b, const pct_hist_soa* hists, const int* block_sizes, int mode);
 /* one DROP iteration of every scan: xs[s] is scan s's image, lambdas[s] its
  * relaxation; mode as pct_drop_iteration (MLP mode | PCT_DROP_F32 |
- * PCT_DROP_CACHED | PCT_DROP_NO_GRAPH), with the same contract for lists that
+ * PCT_DROP_CACHED | PCT_DROP_NO_GRAPH | PCT_DROP_PERSISTENT), with the same contract for lists that
  * differ from a plan (every such scan's image is restored). A context may
  * appear once in a batch (pct_batch_create returns NULL otherwise); a plan
  * rebuilt through its own context is picked up by the next batch call. */

@@ -12,11 +12,12 @@ from . import _lib
 from . import configs
 from ._lib import (PCT_LISTS_DROP, PCT_LISTS_DROP_RED, PCT_LISTS_ERROR, PCT_LISTS_ERROR_RED,
                    PCT_LISTS_ERROR_RED_SP, PCT_LISTS_REUSE, PCT_DROP_ATOMIC, PCT_DROP_CACHED, PCT_DROP_F32, PCT_DROP_NO_GRAPH, PCT_DROP_ROBUST,
-                   PCT_DROP_ROBUST2, PCT_LEARNED, PCT_MLP_FP64, PCT_MLP_SURROGATE, PCT_MLP_TABLE, PCT_PRECISION,
+                   PCT_DROP_ROBUST2, PCT_DROP_PERSISTENT, PCT_LEARNED, PCT_MLP_FP64, PCT_MLP_SURROGATE, PCT_MLP_TABLE, PCT_PRECISION,
                    PctHistSoa, PctParams, PctRawSoa, PctSynthCfg, PctSynthSoa)
 
 __all__ = ["Context", "Batch", "Histories", "RawScan", "configs", "PCT_MLP_FP64", "PCT_MLP_TABLE", "PCT_MLP_SURROGATE",
-           "PCT_PRECISION", "PCT_LEARNED", "PCT_DROP_ROBUST", "PCT_DROP_ROBUST2", "generate_host"]
+           "PCT_PRECISION", "PCT_LEARNED", "PCT_DROP_ROBUST", "PCT_DROP_ROBUST2", "PCT_DROP_PERSISTENT",
+           "generate_host"]
 
 
 class PctError(RuntimeError):
@@ -507,8 +508,9 @@ def set_tuning(ctx: "Context", **knobs):
     sequential), block_form (1 one
     launch per block recomputing the previous block's updates it needs, 2 two
     launches per block, 3 one launch per block waiting for the previous block's
-    stored updates); plan_limit_mb; wait_give_up (diagnostics of block form 3:
-    its waits give up, to exercise the failure path); 0 = default."""
+    stored updates); plan_limit_mb; wait_give_up (diagnostics of block form 3 and
+    of a persistent batch whose first context this is: their waits give up,
+    to exercise the failure path); 0 = default."""
     for k, v in knobs.items():
         if k not in TUNING_KEYS:
             raise PctError(f"unknown tuning knob {k}")

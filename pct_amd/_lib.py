@@ -201,6 +201,7 @@ PCT_DROP_ROBUST = 0x1000
 PCT_LISTS_DROP, PCT_LISTS_DROP_RED, PCT_LISTS_ERROR_RED, PCT_LISTS_ERROR, PCT_LISTS_ERROR_RED_SP = 0, 1, 2, 3, 4
 PCT_LISTS_REUSE = 0x100
 PCT_DROP_ROBUST2 = 0x2000
+PCT_DROP_PERSISTENT = 0x4000
 # pct_set_tuning keys
 PCT_TUNE_MLP_WAVES = 1
 PCT_TUNE_CHUNK_FIRST = 2

@@ -13,9 +13,12 @@
  * are recomputed on a second stream ahead of the chunk (as the single-scan
  * iteration does). The arithmetic is the single-scan kernels'
  * (pct_plan_kernels.h), so every scan's image is identical to its own
- * pct_drop_iteration.
+ * pct_drop_iteration. With PCT_DROP_PERSISTENT all block steps of an
+ * iteration run in one launch instead (k_batch_persist: grid barriers
+ * between the phases, bounded waits).
  */
 #include <algorithm>
+#include <cstdio>
 #include <vector>
 #include "pct_internal.h"
 #include "pct_plan.h"
@@ -85,6 +88,150 @@ k_batch_apply(batch_apply_args a) {
     else pct_blk::apply_f64(a.segs[s][k], a.csc[s], (const double*)a.delta[s], a.x[s]);
 }
 
+/* ------------------------------------------------------------------------- */
+/* Persistent form: every block step of an iteration in ONE launch. The grid  */
+/* is the device's resident capacity for this kernel (every workgroup is      */
+/* resident at once: the iteration runs its MLP check before the launch, so   */
+/* nothing of the batch runs beside it); block step j runs the deltas of      */
+/* block j of every scan, a grid barrier, the voxel updates, a grid barrier.  */
+/* A step's histories (segments) are numbered across the scans from per-step  */
+/* prefix counts; wavefront w takes histories w, w + W, ... (thread t         */
+/* segments t, t + T, ...). The arithmetic is the two-launch kernels', so     */
+/* every scan's image is bit-identical to the graph form's.                   */
+/* ------------------------------------------------------------------------- */
+struct batch_scan {                 /* one scan's plan and image */
+    const uint32_t* list;
+    const uint16_t* pos_k;
+    const uint32_t* pos_len;
+    const double* pos_chord;
+    const float* wepl;
+    float* x;
+    double lambda;
+    void* delta;
+    const uint64_t* segs;
+    const uint16_t* csc;
+    const pct_blk::long_row* rows;
+};
+
+struct batch_step {                 /* block j of scan s (a scan without one: cnt = ns = 0) */
+    int64_t list_base, h0, s0, entry, l0;
+    int32_t stride, cnt, ns, nlong;
+};
+
+struct batch_persist_args {
+    int n;
+    int64_t steps;
+    batch_scan sc[PCT_BATCH_MAX];
+    const batch_step* desc;         /* steps x n                                        */
+    const int32_t* pre;             /* per step: n+1 history prefix counts, n+1 segment */
+    unsigned* bar;                  /* barrier words, zero at the launch                */
+    int* status;
+    unsigned target_add;            /* added to every barrier's target (0; the          */
+    int polls;                      /*   diagnostics knob makes it unreachable)         */
+};
+
+#define PERSIST_THREADS 256
+/* workgroups per CU: the kernel's occupancy less one, at most 6 (every
+ * workgroup must be resident at once; the margin keeps that true when a
+ * CU holds a few other waves) */
+#define PERSIST_WG_PER_CU 6
+/* a barrier wait gives up after this long on the 100 MHz real-time counter
+ * (0.5 s; a barrier takes microseconds), or after `polls` polls */
+#define PERSIST_WAIT_TICKS 50000000ull
+#define PERSIST_POLLS (1 << 30)
+
+/* Grid barrier number `round` (1, 2, ...), two levels so that no counter
+ * takes more than a few dozen arrivals: workgroup w arrives (one thread, at
+ * agent scope, acquire-release) at the counter of its group w % PERSIST_GROUPS;
+ * the last arrival of a group arrives at the top counter; the last arrival
+ * there stores the round into the generation word, which every workgroup
+ * polls (a load at agent scope after a short sleep), then an agent-scope
+ * acquire. Each counter and the generation word have a 256-byte line of their
+ * own (bar[0] generation, bar[64] top, bar[128 + 64 g] group g). A poll count
+ * over `polls`, or the give-up flag set by another workgroup, sets the flag
+ * and ends the wait; the caller then returns. */
+#define PERSIST_GROUPS 32
+#define PERSIST_BAR_WORDS (128 + 64 * PERSIST_GROUPS)
+__device__ __forceinline__ bool persist_barrier(const batch_persist_args& a, unsigned round) {
+    __syncthreads();
+    if (threadIdx.x == 0) {
+        const unsigned G = gridDim.x, g = blockIdx.x % PERSIST_GROUPS;
+        const unsigned groups = G < PERSIST_GROUPS ? G : PERSIST_GROUPS;
+        const unsigned gsize = G / PERSIST_GROUPS + (g < G % PERSIST_GROUPS ? 1u : 0u);
+        unsigned* gen = a.bar;
+        unsigned* top = a.bar + 64;
+        unsigned* grp = a.bar + 128 + 64 * g;
+        if (__hip_atomic_fetch_add(grp, 1u, __ATOMIC_ACQ_REL, __HIP_MEMORY_SCOPE_AGENT) + 1u == round * gsize &&
+            __hip_atomic_fetch_add(top, 1u, __ATOMIC_ACQ_REL, __HIP_MEMORY_SCOPE_AGENT) + 1u == round * groups)
+            __hip_atomic_store(gen, round, __ATOMIC_RELEASE, __HIP_MEMORY_SCOPE_AGENT);
+        const unsigned target = round + a.target_add;
+        const uint64_t t0 = __builtin_amdgcn_s_memrealtime();
+        for (int i = 0; __hip_atomic_load(gen, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT) < target; ++i) {
+            if (i >= a.polls || __builtin_amdgcn_s_memrealtime() - t0 > PERSIST_WAIT_TICKS ||
+                ((i & 63) == 63 && (__hip_atomic_load(a.status, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT) &
+                                    PCT_STATUS_WAIT))) {
+                atomicOr(a.status, PCT_STATUS_WAIT);
+                break;
+            }
+            __builtin_amdgcn_s_sleep(1);
+        }
+        __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "agent");
+    }
+    __syncthreads();
+    /* every thread reads the flag: no LDS, so a workgroup needs none */
+    return !(__hip_atomic_load(a.status, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT) & PCT_STATUS_WAIT);
+}
+
+__device__ __forceinline__ int persist_scan_of(const int32_t* pre, int n, int64_t f) {
+    int s = 0;
+    while (s + 1 < n && f >= pre[s + 1]) ++s;
+    return s;
+}
+
+template <bool F32>
+__global__ void __launch_bounds__(PERSIST_THREADS)
+k_batch_persist(batch_persist_args a) {
+    __builtin_amdgcn_s_setprio(3);
+    const int lane = threadIdx.x & 63;
+    const int64_t nwaves = (int64_t)gridDim.x * (PERSIST_THREADS / 64);
+    const int64_t wave = (int64_t)blockIdx.x * (PERSIST_THREADS / 64) + (threadIdx.x >> 6);
+    const int64_t nthreads = (int64_t)gridDim.x * PERSIST_THREADS;
+    const int64_t tid = (int64_t)blockIdx.x * PERSIST_THREADS + threadIdx.x;
+    unsigned round = 0;
+    for (int64_t j = 0; j < a.steps; ++j) {
+        const int32_t* ph = a.pre + j * 2 * (a.n + 1);
+        const int32_t* ps = ph + (a.n + 1);
+        const batch_step* d = a.desc + j * a.n;
+        /* deltas of block j of every scan */
+        for (int64_t f = wave; f < ph[a.n]; f += nwaves) {
+            const int s = persist_scan_of(ph, a.n, f);
+            const batch_scan& c = a.sc[s];
+            const batch_step& e = d[s];
+            const pct_blk::blk_pos pos{c.pos_k + e.h0, c.pos_len + e.h0, c.pos_chord + e.h0};
+            if (F32)
+                pct_blk::delta_f32(c.list + e.list_base, e.stride, pos, c.wepl, e.h0, f - ph[s], c.x,
+                                   (float)c.lambda, (float*)c.delta, lane);
+            else
+                pct_blk::delta_f64(c.list + e.list_base, e.stride, pos, c.wepl, e.h0, f - ph[s], c.x, c.lambda,
+                                   (double*)c.delta, lane);
+        }
+        if (!persist_barrier(a, ++round)) return;
+        /* their voxel updates */
+        for (int64_t f = tid; f < ps[a.n]; f += nthreads) {
+            const int s = persist_scan_of(ps, a.n, f);
+            const batch_scan& c = a.sc[s];
+            const batch_step& e = d[s];
+            const int64_t k = f - ps[s];
+            pct_blk::apply_seg<F32>(k, e.nlong, c.segs[e.s0 + k], c.rows + e.l0, c.csc + e.entry, c.delta, c.x);
+        }
+        if (!persist_barrier(a, ++round)) return;
+    }
+}
+
+__global__ void k_zero_u32(unsigned* __restrict__ p, int n) {
+    for (int i = threadIdx.x; i < n; i += blockDim.x) p[i] = 0u;
+}
+
 }  // namespace
 
 /* chunks of block steps: small first, doubling up to BATCH_CHUNK_MAX */
@@ -108,6 +255,12 @@ struct pct_batch {
      * hold other buffers and offsets */
     std::vector<uint64_t> g_gen;
     bool g_f32 = false;
+    bool g_persist = false;
+    int g_wait_give_up = 0;
+    /* persistent form: step descriptors and prefix counts, barrier counter */
+    pct_buffer p_desc, p_bar;
+    int64_t p_pre_off = 0;          /* byte offset of the prefix counts in p_desc */
+    unsigned p_grid = 0;
 };
 
 static void batch_release_graphs(pct_batch* b) {
@@ -138,6 +291,8 @@ extern "C" void pct_batch_destroy(pct_batch* b) {
     if (!b->scans.empty() && b->scans[0]->stream) (void)hipStreamSynchronize(b->scans[0]->stream);
     if (b->aux) (void)hipStreamSynchronize(b->aux);
     batch_release_graphs(b);
+    for (pct_buffer* q : {&b->p_desc, &b->p_bar})
+        if (q->ptr) (void)hipFree(q->ptr);
     for (hipEvent_t e : b->ev_chunk)
         if (e) (void)hipEventDestroy(e);
     if (b->ev_start) (void)hipEventDestroy(b->ev_start);
@@ -250,19 +405,107 @@ static int batch_steps(pct_batch* b, float* const* xs, const pct_hist_soa* hists
     return PCT_OK;
 }
 
-static int batch_graphs(pct_batch* b, float* const* xs, const pct_hist_soa* hists, const float* lambdas, bool f32) {
+/* the persistent launch's arguments: per scan its plan and image, per block
+ * step its blocks' descriptors (uploaded to p_desc) */
+static int batch_persist_args_of(pct_batch* b, float* const* xs, const pct_hist_soa* hists, const float* lambdas,
+                                 bool f32, batch_persist_args& a) {
+    pct_ctx* c0 = b->scans[0];
+    const int n = (int)b->scans.size();
+    const int64_t steps = b->steps;
+    std::vector<batch_step> desc((size_t)(steps * n));
+    std::vector<int32_t> pre((size_t)(steps * 2 * (n + 1)), 0);
+    a = batch_persist_args{};
+    a.n = n;
+    a.steps = steps;
+    for (int s = 0; s < n; ++s) {
+        const pct_plan* pl = b->scans[s]->plan;
+        a.sc[s] = batch_scan{(const uint32_t*)pl->list.ptr, (const uint16_t*)pl->pos_k.ptr,
+                             (const uint32_t*)pl->pos_len.ptr, (const double*)pl->pos_chord.ptr, hists[s].wepl, xs[s],
+                             (double)lambdas[s], pl->delta.ptr, (const uint64_t*)pl->segs.ptr,
+                             (const uint16_t*)pl->csc.ptr, (const pct_blk::long_row*)pl->long_rows.ptr};
+    }
+    for (int64_t j = 0; j < steps; ++j) {
+        int32_t* ph = pre.data() + j * 2 * (n + 1);
+        int32_t* ps = ph + (n + 1);
+        for (int s = 0; s < n; ++s) {
+            const pct_plan* pl = b->scans[s]->plan;
+            batch_step e{};
+            if (j < pl->nb) {
+                e.h0 = j * pl->B;
+                e.cnt = (int32_t)std::min<int64_t>(pl->B, pl->n - e.h0);
+                e.list_base = pl->list_base_h[j];
+                e.stride = pl->stride_h[j];
+                e.s0 = pl->blk_seg_h[j];
+                e.ns = (int32_t)(pl->blk_seg_h[j + 1] - e.s0);
+                e.entry = pl->blk_entry_h[j];
+                e.l0 = pl->blk_long_h[j];
+                e.nlong = (int32_t)(pl->blk_long_h[j + 1] - e.l0);
+            }
+            desc[(size_t)(j * n + s)] = e;
+            ph[s + 1] = ph[s] + e.cnt;
+            ps[s + 1] = ps[s] + e.ns;
+        }
+    }
+    const size_t dbytes = sizeof(batch_step) * desc.size();
+    b->p_pre_off = (int64_t)((dbytes + 255) / 256 * 256);
+    int rc;
+    if ((rc = pct_ensure(c0, b->p_desc, (size_t)b->p_pre_off + sizeof(int32_t) * pre.size())) ||
+        (rc = pct_ensure(c0, b->p_bar, sizeof(unsigned) * PERSIST_BAR_WORDS)))
+        return rc;
+    PCT_HIP_CHECK(c0, hipMemcpy(b->p_desc.ptr, desc.data(), dbytes, hipMemcpyHostToDevice));
+    PCT_HIP_CHECK(c0, hipMemcpy((char*)b->p_desc.ptr + b->p_pre_off, pre.data(), sizeof(int32_t) * pre.size(),
+                                hipMemcpyHostToDevice));
+    a.desc = (const batch_step*)b->p_desc.ptr;
+    a.pre = (const int32_t*)((const char*)b->p_desc.ptr + b->p_pre_off);
+    a.bar = (unsigned*)b->p_bar.ptr;
+    a.status = c0->d_status;
+    /* the diagnostics knob: a target no arrival count reaches, two polls */
+    a.target_add = c0->tune_wait_give_up ? 0x40000000u : 0u;
+    a.polls = c0->tune_wait_give_up ? 2 : PERSIST_POLLS;
+    /* every workgroup resident at once: the kernel's occupancy on every CU */
+    int occ = 0;
+    PCT_HIP_CHECK(c0, hipOccupancyMaxActiveBlocksPerMultiprocessor(
+                          &occ, f32 ? (const void*)k_batch_persist<true> : (const void*)k_batch_persist<false>,
+                          PERSIST_THREADS, 0));
+    int cus = 0;
+    PCT_HIP_CHECK(c0, hipDeviceGetAttribute(&cus, hipDeviceAttributeMultiprocessorCount, c0->device));
+    b->p_grid = (unsigned)(std::max(1, cus) * std::max(1, std::min(occ - 1, PERSIST_WG_PER_CU)));
+    return PCT_OK;
+}
+
+/* the persistent form's launch of one iteration: zero the barrier counter
+ * (a kernel, so that the graph holds no memset node), then the loop */
+static int batch_persist_launch(pct_batch* b, const batch_persist_args& a, bool f32) {
+    pct_ctx* c0 = b->scans[0];
+    hipLaunchKernelGGL(k_zero_u32, dim3(1), dim3(256), 0, c0->stream, (unsigned*)b->p_bar.ptr, PERSIST_BAR_WORDS);
+    if (f32) hipLaunchKernelGGL(k_batch_persist<true>, dim3(b->p_grid), dim3(PERSIST_THREADS), 0, c0->stream, a);
+    else hipLaunchKernelGGL(k_batch_persist<false>, dim3(b->p_grid), dim3(PERSIST_THREADS), 0, c0->stream, a);
+    PCT_HIP_CHECK(c0, hipGetLastError());
+    return PCT_OK;
+}
+
+static int batch_graphs(pct_batch* b, float* const* xs, const pct_hist_soa* hists, const float* lambdas, bool f32,
+                        bool persist) {
     const size_t n = b->scans.size();
-    bool same = b->graphs.size() + 1 == b->chunk.size() && b->g_f32 == f32 && b->g_x.size() == n;
+    const size_t want = persist ? 1 : b->chunk.size() - 1;
+    bool same = b->graphs.size() == want && b->g_f32 == f32 && b->g_persist == persist && b->g_x.size() == n &&
+                b->g_wait_give_up == b->scans[0]->tune_wait_give_up;
     for (size_t s = 0; same && s < n; ++s)
         same = b->g_x[s] == xs[s] && b->g_wepl[s] == hists[s].wepl && b->g_lambda[s] == (double)lambdas[s] &&
                b->g_gen.size() == n && b->g_gen[s] == b->scans[s]->plan->generation;
     if (same) return PCT_OK;
     batch_release_graphs(b);
     pct_ctx* c0 = b->scans[0];
-    for (size_t c = 0; c + 1 < b->chunk.size(); ++c) {
+    batch_persist_args pa{};
+    if (persist) {
+        int rc = batch_persist_args_of(b, xs, hists, lambdas, f32, pa);
+        if (rc) return rc;
+    }
+    for (size_t c = 0; c < want; ++c) {
         hipGraph_t g = nullptr;
         PCT_HIP_CHECK(c0, hipStreamBeginCapture(c0->stream, hipStreamCaptureModeThreadLocal));
-        int rc = batch_steps(b, xs, hists, lambdas, f32, b->chunk[c], b->chunk[c + 1]);
+        int rc = persist ? batch_persist_launch(b, pa, f32)
+                         : batch_steps(b, xs, hists, lambdas, f32, b->chunk[c], b->chunk[c + 1]);
         hipError_t e = hipStreamEndCapture(c0->stream, &g);
         if (rc) { if (g) (void)hipGraphDestroy(g); batch_release_graphs(b); return rc; }
         if (e != hipSuccess) { batch_release_graphs(b); return pct_fail(c0, PCT_E_HIP, "graph capture: %s", hipGetErrorString(e)); }
@@ -282,6 +525,8 @@ static int batch_graphs(pct_batch* b, float* const* xs, const pct_hist_soa* hist
         b->g_lambda[s] = (double)lambdas[s];
     }
     b->g_f32 = f32;
+    b->g_persist = persist;
+    b->g_wait_give_up = c0->tune_wait_give_up;
     return PCT_OK;
 }
 
@@ -296,19 +541,21 @@ extern "C" int pct_batch_iteration(pct_batch* b, const pct_hist_soa* hists, floa
         if (!xs[s]) return pct_fail(c0, PCT_E_INVALID, "pct_batch_iteration: scan %d has no image", s);
     const int mlp = mode & PCT_MLP_MODE_MASK;
     const bool f32 = (mode & PCT_DROP_F32) != 0, eager = (mode & PCT_DROP_NO_GRAPH) != 0;
+    const bool persist = (mode & PCT_DROP_PERSISTENT) != 0;
     bool rebuilt;
     if ((rc = batch_plans(b, hists, block_sizes, mlp, &rebuilt))) return rc;
     const bool recompute = !(mode & PCT_DROP_CACHED) && !rebuilt;
-    if (!eager && (rc = batch_graphs(b, xs, hists, lambdas, f32))) return rc;
+    if (!eager && (rc = batch_graphs(b, xs, hists, lambdas, f32, persist))) return rc;
     const int64_t nchunks = (int64_t)b->chunk.size() - 1;
     while ((int64_t)b->ev_chunk.size() < nchunks) {
         hipEvent_t e = nullptr;
         PCT_HIP_CHECK(c0, hipEventCreateWithFlags(&e, hipEventDisableTiming));
         b->ev_chunk.push_back(e);
     }
-    if (recompute) {
+    if (recompute || persist) {
         /* lists that differ from a plan are only known after the block steps
-         * ran on them: keep every image to restore it then */
+         * ran on them, and a persistent launch whose barrier gave up stops
+         * midway: keep every image to restore it then */
         for (int s = 0; s < n; ++s) {
             pct_ctx* cs = b->scans[s];
             if ((rc = pct_ensure(cs, cs->x_saved, sizeof(float) * (size_t)cs->g.V))) {
@@ -319,7 +566,7 @@ extern "C" int pct_batch_iteration(pct_batch* b, const pct_hist_soa* hists, floa
                                              hipMemcpyDeviceToDevice, c0->stream));
             /* the surrogate's fallback list sized for the whole scan before
              * any chunk is queued (growing it later would wait for the device) */
-            if (mlp == PCT_MLP_SURROGATE && (rc = pct_mlp_fallback_reserve(cs, cs->plan->n))) {
+            if (recompute && mlp == PCT_MLP_SURROGATE && (rc = pct_mlp_fallback_reserve(cs, cs->plan->n))) {
                 pct_fail(c0, rc, "scan %d: %s", s, pct_last_error(cs));
                 return rc;
             }
@@ -357,8 +604,21 @@ extern "C" int pct_batch_iteration(pct_batch* b, const pct_hist_soa* hists, floa
     /* the block steps read the plans' lists: by default they wait for the
      * check of the recomputed lists only at the end; with overlap 1 of the
      * first scan's context, chunk c waits for the check of its lists */
-    const bool chunk_waits = recompute && c0->tune_overlap == 1;
-    for (int64_t c = 0; c < nchunks; ++c) {
+    const bool chunk_waits = recompute && c0->tune_overlap == 1 && !persist;
+    if (persist) {
+        /* the persistent launch needs every workgroup resident: it follows
+         * the MLP check instead of running beside it */
+        if (recompute && nchunks > 0) PCT_HIP_CHECK(c0, hipStreamWaitEvent(c0->stream, b->ev_chunk[nchunks - 1], 0));
+        if (eager) {
+            batch_persist_args pa{};
+            if ((rc = batch_persist_args_of(b, xs, hists, lambdas, f32, pa)) ||
+                (rc = batch_persist_launch(b, pa, f32)))
+                return rc;
+        } else {
+            PCT_HIP_CHECK(c0, hipGraphLaunch(b->graphs[0], c0->stream));
+        }
+    }
+    for (int64_t c = 0; !persist && c < nchunks; ++c) {
         if (chunk_waits) PCT_HIP_CHECK(c0, hipStreamWaitEvent(c0->stream, b->ev_chunk[c], 0));
         if (eager) {
             if ((rc = batch_steps(b, xs, hists, lambdas, f32, b->chunk[c], b->chunk[c + 1]))) return rc;
@@ -366,7 +626,7 @@ extern "C" int pct_batch_iteration(pct_batch* b, const pct_hist_soa* hists, floa
             PCT_HIP_CHECK(c0, hipGraphLaunch(b->graphs[c], c0->stream));
         }
     }
-    if (recompute && !chunk_waits && nchunks > 0)
+    if (recompute && !chunk_waits && !persist && nchunks > 0)
         PCT_HIP_CHECK(c0, hipStreamWaitEvent(c0->stream, b->ev_chunk[nchunks - 1], 0));
     if (c0->profile) {
         PCT_HIP_CHECK(c0, hipEventRecord(c0->ev[3], c0->stream));
@@ -375,20 +635,32 @@ extern "C" int pct_batch_iteration(pct_batch* b, const pct_hist_soa* hists, floa
     /* every kernel of the iteration has joined the batch stream */
     PCT_HIP_CHECK(c0, hipStreamSynchronize(c0->stream));
     int first_rc = PCT_OK;
+    /* a persistent launch reports a barrier that gave up in scans[0]'s status */
+    int flags0 = 0;
+    if ((rc = pct_take_status(c0, &flags0))) return rc;
+    const bool gave_up = persist && (flags0 & PCT_STATUS_WAIT);
+    char arrivals[96] = "";
+    if (gave_up && b->p_bar.ptr) {
+        /* what the barrier words held: how far the launch got */
+        unsigned w[128];
+        PCT_HIP_CHECK(c0, hipMemcpy(w, b->p_bar.ptr, sizeof(w), hipMemcpyDeviceToHost));
+        snprintf(arrivals, sizeof(arrivals), " (grid %u: generation %u, %u group arrivals)", b->p_grid, w[0], w[64]);
+    }
     for (int s = 0; s < n; ++s) {
         pct_ctx* cs = b->scans[s];
-        int flags = 0;
-        if ((rc = pct_take_status(cs, &flags))) return rc;
-        if (flags) {
-            cs->plan->valid = false;
-            if (recompute) {
-                PCT_HIP_CHECK(c0, hipMemcpyAsync(xs[s], cs->x_saved.ptr, sizeof(float) * (size_t)cs->g.V,
-                                                 hipMemcpyDeviceToDevice, c0->stream));
-                PCT_HIP_CHECK(c0, hipStreamSynchronize(c0->stream));
-            }
+        int flags = s == 0 ? flags0 : 0;
+        if (s > 0 && (rc = pct_take_status(cs, &flags))) return rc;
+        if (gave_up && s > 0) flags |= PCT_STATUS_WAIT;
+        /* a barrier that gave up leaves the plan intact; anything else the
+         * MLP check found invalidates it */
+        if (flags & ~PCT_STATUS_WAIT) cs->plan->valid = false;
+        if ((flags & PCT_STATUS_WAIT) || (flags && recompute)) {
+            PCT_HIP_CHECK(c0, hipMemcpyAsync(xs[s], cs->x_saved.ptr, sizeof(float) * (size_t)cs->g.V,
+                                             hipMemcpyDeviceToDevice, c0->stream));
+            PCT_HIP_CHECK(c0, hipStreamSynchronize(c0->stream));
         }
         if ((rc = pct_status_to_rc(cs, flags)) && first_rc == PCT_OK) {
-            pct_fail(c0, rc, "scan %d: %s", s, pct_last_error(cs));
+            pct_fail(c0, rc, "scan %d: %s%s", s, pct_last_error(cs), gave_up ? arrivals : "");
             first_rc = rc;
         }
     }

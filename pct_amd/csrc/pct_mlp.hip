@@ -1369,7 +1369,8 @@ int pct_status_to_rc(pct_ctx* ctx, int flags) {
         return pct_fail(ctx, PCT_E_INVALID, "recomputed MLP voxel lists differ from the DROP plan (inputs changed?)");
     if (flags & PCT_STATUS_WAIT)
         return pct_fail(ctx, PCT_E_HIP,
-                        "DROP block loop: a wait for the previous block's stored voxel updates gave up (x unchanged)");
+                        "DROP block loop: a bounded wait (the waiting one-launch form's wait for the previous block's "
+                        "stored updates, or a persistent batch's grid barrier) gave up (x unchanged)");
     if (flags & PCT_STATUS_TABLE) return pct_fail(ctx, PCT_E_OVERFLOW, "MLP chord longer than the per-depth table");
     if (flags & PCT_STATUS_CAPACITY) return pct_fail(ctx, PCT_E_OVERFLOW, "MLP path longer than its capacity");
     if (flags & PCT_STATUS_OVERFLOW)

@@ -31,18 +31,23 @@ def scans():
     return _small_scans()
 
 
-@pytest.mark.parametrize("mode_name", ["fp64", "precision", "fp64_eager", "fp64_cached"])
+@pytest.mark.parametrize("mode_name", ["fp64", "precision", "fp64_eager", "fp64_cached", "fp64_persistent",
+                                       "precision_persistent", "fp64_persistent_eager", "fp64_persistent_cached"])
 def test_batch_equals_single_scans(scans, mode_name):
     import pct_amd
+    P = pct_amd.PCT_DROP_PERSISTENT
     mode = {"fp64": 0, "precision": pct_amd.PCT_PRECISION, "fp64_eager": pct_amd.PCT_DROP_NO_GRAPH,
-            "fp64_cached": pct_amd.PCT_DROP_CACHED}[mode_name]
+            "fp64_cached": pct_amd.PCT_DROP_CACHED, "fp64_persistent": P,
+            "precision_persistent": pct_amd.PCT_PRECISION | P, "fp64_persistent_eager": P | pct_amd.PCT_DROP_NO_GRAPH,
+            "fp64_persistent_cached": P | pct_amd.PCT_DROP_CACHED}[mode_name]
+    single = mode & ~P
     sizes = [800, 1600, 3200, 6400][:len(scans)]
     lams = [1e-2, 2e-2, 5e-3, 1e-2][:len(scans)]
     ref = []
     for (ctx, hist, x0), B, lam in zip(scans, sizes, lams):
         x = x0.clone()
         for _ in range(3):
-            ctx.drop_iteration(hist, x, B, lam, mode)
+            ctx.drop_iteration(hist, x, B, lam, single)
         ref.append(x.cpu().numpy())
     batch = pct_amd.Batch([c for c, _, _ in scans])
     hists = [h for _, h, _ in scans]
@@ -107,3 +112,34 @@ def test_batch_follows_plans_rebuilt_by_their_own_context(scans):
     batch.close()
     h1.x_exit[:200].copy_(saved)
     ctxs[1].drop_invalidate()
+
+
+def test_persistent_barrier_that_gives_up_leaves_images_unchanged(scans):
+    """the persistent batch's grid barriers, forced from the host to give up
+    (a target no arrival count reaches, two polls): the iteration returns the
+    error and every image keeps its value; without the knob the next
+    iteration equals the two-launch form's"""
+    import pct_amd
+    sizes = [3200] * len(scans)
+    hists = [h for _, h, _ in scans]
+    c0 = scans[0][0]
+    batch = pct_amd.Batch([c for c, _, _ in scans])
+    xs = [x0.clone() for _, _, x0 in scans]
+    before = [x.cpu().numpy() for x in xs]
+    P = pct_amd.PCT_DROP_PERSISTENT | pct_amd.PCT_DROP_CACHED
+    pct_amd.set_tuning(c0, wait_give_up=1)
+    try:
+        with pytest.raises(pct_amd.PctError, match="gave up"):
+            batch.iteration(hists, xs, sizes, None, P)
+        for x, b in zip(xs, before):
+            assert np.array_equal(x.cpu().numpy().view(np.uint32), b.view(np.uint32))
+        pct_amd.set_tuning(c0, wait_give_up=0)
+        batch.iteration(hists, xs, sizes, None, P)
+    finally:
+        pct_amd.set_tuning(c0, wait_give_up=0)
+    ref = [x0.clone() for _, _, x0 in scans]
+    batch.iteration(hists, ref, sizes, None, pct_amd.PCT_DROP_CACHED)
+    for x, r, b in zip(xs, ref, before):
+        assert np.array_equal(x.cpu().numpy().view(np.uint32), r.cpu().numpy().view(np.uint32))
+        assert not np.array_equal(x.cpu().numpy(), b)
+    batch.close()
