@@ -300,6 +300,10 @@ __global__ void k_write_segments(const uint32_t* __restrict__ keys, const uint32
  * ones their first memory round needs, all scalars, so that the dispatch
  * preloads them into SGPRs (the library is compiled with kernel-argument
  * preloading); the rest are loaded while that round is in flight. */
+/* SLOTS: list slots per lane in the first round, 4 for blocks whose rows
+ * are at most 256 entries long (their launch then issues no slot past the
+ * row), DELTA_SLOTS otherwise */
+template <int SLOTS>
 __global__ void __launch_bounds__(64 * DELTA_WAVES)
 k_block_delta(const uint32_t* __restrict__ list, const uint16_t* __restrict__ pos_k,
               const uint32_t* __restrict__ pos_len, const double* __restrict__ pos_chord, int stride, int64_t count,
@@ -308,9 +312,11 @@ k_block_delta(const uint32_t* __restrict__ list, const uint16_t* __restrict__ po
     __builtin_amdgcn_s_setprio(3);
     const int64_t q = (int64_t)blockIdx.x * DELTA_WAVES + (threadIdx.x >> 6);
     if (q >= count) return;
-    pct_blk::delta_f64(list, stride, {pos_k, pos_len, pos_chord}, wepl, h0, q, x, lambda, delta, threadIdx.x & 63);
+    pct_blk::delta_f64<SLOTS>(list, stride, {pos_k, pos_len, pos_chord}, wepl, h0, q, x, lambda, delta,
+                              threadIdx.x & 63);
 }
 
+template <int SLOTS>
 __global__ void __launch_bounds__(64 * DELTA_WAVES)
 k_block_delta_f32(const uint32_t* __restrict__ list, const uint16_t* __restrict__ pos_k,
                   const uint32_t* __restrict__ pos_len, const double* __restrict__ pos_chord, int stride,
@@ -319,7 +325,8 @@ k_block_delta_f32(const uint32_t* __restrict__ list, const uint16_t* __restrict_
     __builtin_amdgcn_s_setprio(3);
     const int64_t q = (int64_t)blockIdx.x * DELTA_WAVES + (threadIdx.x >> 6);
     if (q >= count) return;
-    pct_blk::delta_f32(list, stride, {pos_k, pos_len, pos_chord}, wepl, h0, q, x, lambda, delta, threadIdx.x & 63);
+    pct_blk::delta_f32<SLOTS>(list, stride, {pos_k, pos_len, pos_chord}, wepl, h0, q, x, lambda, delta,
+                              threadIdx.x & 63);
 }
 
 /* segs: the block's records (its long segments first), rows: their side rows */
@@ -949,14 +956,15 @@ static int plan_block_loop(pct_ctx* ctx, pct_plan* pl, const pct_hist_soa* hist,
             continue;
         }
         const pct_blk::blk_pos pb = pos_of(b);
+        const bool short_rows = stride <= 4 * 64;
         if (f32)
-            hipLaunchKernelGGL(k_block_delta_f32, dgrid, dim3(64 * DELTA_WAVES), 0, ctx->stream, lb, pb.k, pb.len,
-                               pb.chord, stride, cnt, (const float*)x, (const float*)hist->wepl, h0, (float)lambda,
-                               (float*)delta);
+            hipLaunchKernelGGL((short_rows ? k_block_delta_f32<4> : k_block_delta_f32<DELTA_SLOTS>), dgrid,
+                               dim3(64 * DELTA_WAVES), 0, ctx->stream, lb, pb.k, pb.len, pb.chord, stride, cnt,
+                               (const float*)x, (const float*)hist->wepl, h0, (float)lambda, (float*)delta);
         else
-            hipLaunchKernelGGL(k_block_delta, dgrid, dim3(64 * DELTA_WAVES), 0, ctx->stream, lb, pb.k, pb.len,
-                               pb.chord, stride, cnt, (const float*)x, (const float*)hist->wepl, h0, lambda,
-                               (double*)delta);
+            hipLaunchKernelGGL((short_rows ? k_block_delta<4> : k_block_delta<DELTA_SLOTS>), dgrid,
+                               dim3(64 * DELTA_WAVES), 0, ctx->stream, lb, pb.k, pb.len, pb.chord, stride, cnt,
+                               (const float*)x, (const float*)hist->wepl, h0, lambda, (double*)delta);
         int64_t s0 = pl->blk_seg_h[b], ns = pl->blk_seg_h[b + 1] - s0;
         const int64_t l0 = pl->blk_long_h[b], nlong = pl->blk_long_h[b + 1] - l0;
         const pct_blk::long_row* rows = (const pct_blk::long_row*)pl->long_rows.ptr + l0;

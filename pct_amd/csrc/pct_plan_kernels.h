@@ -65,10 +65,11 @@ __device__ __forceinline__ void kahan_add(float& s, float& c, float v) {
  * memory round): a sentinel slot loads x[0], which its caller ignores. A
  * conditional load per slot would be compiled as a branch around each load
  * and its use, one round per slot. */
-__device__ __forceinline__ void gather_slots(const float* __restrict__ x, const uint32_t (&v)[DELTA_SLOTS],
-                                             float (&xv)[DELTA_SLOTS]) {
+template <int SLOTS>
+__device__ __forceinline__ void gather_slots(const float* __restrict__ x, const uint32_t (&v)[SLOTS],
+                                             float (&xv)[SLOTS]) {
 #pragma unroll
-    for (int r = 0; r < DELTA_SLOTS; ++r) xv[r] = x[v[r] != LIST_SENTINEL ? LIST_VOX(v[r]) : 0u];
+    for (int r = 0; r < SLOTS; ++r) xv[r] = x[v[r] != LIST_SENTINEL ? LIST_VOX(v[r]) : 0u];
 }
 
 /* The block's histories in placement order: position q's list is row q of the
@@ -83,7 +84,9 @@ struct blk_pos {
     const double* chord;
 };
 
-/* the history at position q of the block (one wavefront) */
+/* the history at position q of the block (one wavefront); SLOTS list slots
+ * per lane in the first round (rows up to 64 SLOTS long), the rest after */
+template <int SLOTS = DELTA_SLOTS>
 __device__ __forceinline__ void delta_f64(const uint32_t* __restrict__ list, int stride, blk_pos pos,
                                           const float* __restrict__ wepl, int64_t h0, int64_t q,
                                           const float* __restrict__ x, double lambda, double* __restrict__ delta,
@@ -92,19 +95,19 @@ __device__ __forceinline__ void delta_f64(const uint32_t* __restrict__ list, int
     const uint32_t n = pos.len[q];
     const double L = pos.chord[q];
     const uint32_t* p = list + q * stride;
-    uint32_t v[DELTA_SLOTS];
+    uint32_t v[SLOTS];
 #pragma unroll
-    for (int r = 0; r < DELTA_SLOTS; ++r) {
+    for (int r = 0; r < SLOTS; ++r) {
         int j = lane + 64 * r;
         v[r] = j < stride ? p[j] : LIST_SENTINEL;
     }
-    float xv[DELTA_SLOTS];
+    float xv[SLOTS];
     gather_slots(x, v, xv);
     double s = 0.0;
 #pragma unroll
-    for (int r = 0; r < DELTA_SLOTS; ++r)
+    for (int r = 0; r < SLOTS; ++r)
         if (v[r] != LIST_SENTINEL) s += (double)xv[r];
-    for (int j = lane + 64 * DELTA_SLOTS; j < stride; j += 64) {
+    for (int j = lane + 64 * SLOTS; j < stride; j += 64) {
         uint32_t w = p[j];
         if (w != LIST_SENTINEL) s += (double)x[LIST_VOX(w)];
     }
@@ -117,6 +120,7 @@ __device__ __forceinline__ void delta_f64(const uint32_t* __restrict__ list, int
     }
 }
 
+template <int SLOTS = DELTA_SLOTS>
 __device__ __forceinline__ void delta_f32(const uint32_t* __restrict__ list, int stride, blk_pos pos,
                                           const float* __restrict__ wepl, int64_t h0, int64_t q,
                                           const float* __restrict__ x, float lambda, float* __restrict__ delta,
@@ -125,19 +129,19 @@ __device__ __forceinline__ void delta_f32(const uint32_t* __restrict__ list, int
     const uint32_t n = pos.len[q];
     const float L = (float)pos.chord[q];
     const uint32_t* p = list + q * stride;
-    uint32_t v[DELTA_SLOTS];
+    uint32_t v[SLOTS];
 #pragma unroll
-    for (int r = 0; r < DELTA_SLOTS; ++r) {
+    for (int r = 0; r < SLOTS; ++r) {
         int j = lane + 64 * r;
         v[r] = j < stride ? p[j] : LIST_SENTINEL;
     }
-    float xv[DELTA_SLOTS];
+    float xv[SLOTS];
     gather_slots(x, v, xv);
     float s = 0.0f, c = 0.0f;
 #pragma unroll
-    for (int r = 0; r < DELTA_SLOTS; ++r)
+    for (int r = 0; r < SLOTS; ++r)
         if (v[r] != LIST_SENTINEL) kahan_add(s, c, xv[r]);
-    for (int j = lane + 64 * DELTA_SLOTS; j < stride; j += 64) {
+    for (int j = lane + 64 * SLOTS; j < stride; j += 64) {
         uint32_t w = p[j];
         if (w != LIST_SENTINEL) kahan_add(s, c, x[LIST_VOX(w)]);
     }
