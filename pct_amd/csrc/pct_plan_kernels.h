@@ -5,7 +5,7 @@
  *
  * delta_i = L (b_i - L sum_{v in path} x_v) / (n L^2) lambda
  *     (block_update_GPU, .cu:5822-5834), a.x summed per lane over the list
- *     slots then by a butterfly over the wavefront;
+ *     slots then across the wavefront by DPP exchanges;
  * x_v += (sum of the deltas of the block's histories through v, in history
  *     order) / count (DROP_blocks + DROP_update, .cu:8023-8055).
  * FP64 as the oracle, or FP32 with Kahan-compensated sums (precision form).
@@ -48,17 +48,70 @@
 
 namespace pct_blk {
 
-__device__ __forceinline__ double wave_sum_d(double v) {
-#pragma unroll
-    for (int d = 32; d >= 1; d >>= 1) v += __shfl_xor(v, d, 64);
-    return v;
-}
-
 __device__ __forceinline__ void kahan_add(float& s, float& c, float v) {
     float y = v - c;
     float t = s + y;
     c = (t - s) - y;
     s = t;
+}
+
+/* v of lane l's DPP source lane (both halves of the double moved alike) */
+template <int CTRL>
+__device__ __forceinline__ double dpp_d(double v) {
+    const int lo = __builtin_amdgcn_update_dpp(0, __double2loint(v), CTRL, 0xf, 0xf, false);
+    const int hi = __builtin_amdgcn_update_dpp(0, __double2hiint(v), CTRL, 0xf, 0xf, false);
+    return __hiloint2double(hi, lo);
+}
+
+template <int CTRL>
+__device__ __forceinline__ float dpp_f(float v) {
+    return __int_as_float(__builtin_amdgcn_update_dpp(0, __float_as_int(v), CTRL, 0xf, 0xf, false));
+}
+
+/* the Kahan-compensated sum (s, c) over the wavefront, valid in lane 0: the
+ * stages of wave_sum_d below, each combining two compensated partial sums */
+__device__ __forceinline__ void wave_kahan_sum(float& s, float& c) {
+    auto stage = [&](float os, float oc) {
+        c += oc;
+        kahan_add(s, c, os);
+    };
+    stage(dpp_f<0xB1>(s), dpp_f<0xB1>(c));
+    stage(dpp_f<0x4E>(s), dpp_f<0x4E>(c));
+    stage(dpp_f<0x141>(s), dpp_f<0x141>(c));
+    stage(dpp_f<0x140>(s), dpp_f<0x140>(c));
+    const float s1 = __int_as_float(__builtin_amdgcn_readlane(__float_as_int(s), 16));
+    const float c1 = __int_as_float(__builtin_amdgcn_readlane(__float_as_int(c), 16));
+    const float s2 = __int_as_float(__builtin_amdgcn_readlane(__float_as_int(s), 32));
+    const float c2 = __int_as_float(__builtin_amdgcn_readlane(__float_as_int(c), 32));
+    const float s3 = __int_as_float(__builtin_amdgcn_readlane(__float_as_int(s), 48));
+    const float c3 = __int_as_float(__builtin_amdgcn_readlane(__float_as_int(c), 48));
+    s = __int_as_float(__builtin_amdgcn_readlane(__float_as_int(s), 0));
+    c = __int_as_float(__builtin_amdgcn_readlane(__float_as_int(c), 0));
+    stage(s1, c1);
+    stage(s2, c2);
+    stage(s3, c3);
+}
+
+/* The sum over the wavefront of v, valid in lane 0: within each row of 16
+ * lanes by DPP exchanges (pairs, quads, half rows, rows: every lane of a
+ * group adds the same two operands, so each stage leaves one value per
+ * group), then the four rows' sums in order. Register moves instead of LDS
+ * permutes on the delta kernels' critical path. The a.x sums it serves are
+ * sums of floats in FP64, exact whatever the order. */
+__device__ __forceinline__ double wave_sum_d(double v) {
+    v += dpp_d<0xB1>(v);    /* quad_perm [1,0,3,2] */
+    v += dpp_d<0x4E>(v);    /* quad_perm [2,3,0,1] */
+    v += dpp_d<0x141>(v);   /* row_half_mirror     */
+    v += dpp_d<0x140>(v);   /* row_mirror          */
+    const double r0 = __hiloint2double(__builtin_amdgcn_readlane(__double2hiint(v), 0),
+                                       __builtin_amdgcn_readlane(__double2loint(v), 0));
+    const double r1 = __hiloint2double(__builtin_amdgcn_readlane(__double2hiint(v), 16),
+                                       __builtin_amdgcn_readlane(__double2loint(v), 16));
+    const double r2 = __hiloint2double(__builtin_amdgcn_readlane(__double2hiint(v), 32),
+                                       __builtin_amdgcn_readlane(__double2loint(v), 32));
+    const double r3 = __hiloint2double(__builtin_amdgcn_readlane(__double2hiint(v), 48),
+                                       __builtin_amdgcn_readlane(__double2loint(v), 48));
+    return ((r0 + r1) + r2) + r3;
 }
 
 /* x at a lane's list slots, every load issued before the first is used (one
@@ -145,12 +198,7 @@ __device__ __forceinline__ void delta_f32(const uint32_t* __restrict__ list, int
         uint32_t w = p[j];
         if (w != LIST_SENTINEL) kahan_add(s, c, x[LIST_VOX(w)]);
     }
-#pragma unroll
-    for (int d = 32; d >= 1; d >>= 1) {
-        float os = __shfl_xor(s, d, 64), oc = __shfl_xor(c, d, 64);
-        c += oc;
-        kahan_add(s, c, os);
-    }
+    wave_kahan_sum(s, c);
     const float b_i = wepl[h0 + k];
     if (lane == 0) {
         const float a_i_dot_x_k = s * L;
@@ -506,12 +554,7 @@ __device__ __forceinline__ void fused_delta(const uint32_t* __restrict__ list, c
             const uint32_t q = p[j];
             if (q != LIST_SENTINEL) kahan_add(s, c, fused_x<F32>(LIST_VOX(q), pr[j], xw, pv));
         }
-#pragma unroll
-        for (int d = 32; d >= 1; d >>= 1) {
-            float os = __shfl_xor(s, d, 64), oc = __shfl_xor(c, d, 64);
-            c += oc;
-            kahan_add(s, c, os);
-        }
+        wave_kahan_sum(s, c);
         const float b_i = wepl[h0 + k];
         if (lane == 0) {
             const float L = (float)Ld;
@@ -624,12 +667,7 @@ __device__ __forceinline__ void wait_delta(const uint32_t* __restrict__ list, in
             const uint32_t t = p[j];
             if (t != LIST_SENTINEL) kahan_add(s, c, tail_value(t));
         }
-#pragma unroll
-        for (int d = 32; d >= 1; d >>= 1) {
-            float os = __shfl_xor(s, d, 64), oc = __shfl_xor(c, d, 64);
-            c += oc;
-            kahan_add(s, c, os);
-        }
+        wave_kahan_sum(s, c);
         const float b_i = wepl[h0 + k];
         if (lane == 0) {
             const float L = (float)Ld;
