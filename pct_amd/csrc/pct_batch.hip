@@ -234,7 +234,10 @@ __global__ void k_zero_u32(unsigned* __restrict__ p, int n) {
 
 }  // namespace
 
-/* chunks of block steps: small first, doubling up to BATCH_CHUNK_MAX */
+/* chunks of block steps when chunk c waits for its own MLP check (overlap 1
+ * of the first scan's context): small first, doubling up to BATCH_CHUNK_MAX.
+ * Otherwise one chunk: the check of every scan is one MLP launch per scan,
+ * no launch a fraction of the resident grid. */
 #define BATCH_CHUNK_FIRST 16
 #define BATCH_CHUNK_MAX 256
 
@@ -243,6 +246,7 @@ struct pct_batch {
     hipStream_t aux = nullptr;
     int64_t steps = 0;
     std::vector<int64_t> chunk;            /* step boundaries */
+    int64_t chunk_first = 0, chunk_max = 0; /* what chunk was laid out with */
     std::vector<hipGraphExec_t> graphs;
     std::vector<hipEvent_t> ev_chunk;
     hipEvent_t ev_start = nullptr;
@@ -336,11 +340,19 @@ static int batch_plans(pct_batch* b, const pct_hist_soa* hists, const int* block
     for (pct_ctx* c : b->scans) steps = std::max(steps, c->plan->nb);
     for (size_t s = 0; !*rebuilt && s < b->scans.size() && s < b->g_gen.size(); ++s)
         if (b->g_gen[s] != b->scans[s]->plan->generation) batch_release_graphs(b);
-    if (*rebuilt || steps != b->steps || b->chunk.empty()) {
+    /* chunk layout from the first context's knobs, as pct_drop_iteration's */
+    const pct_ctx* c0 = b->scans[0];
+    const bool one = c0->tune_overlap != 1 && c0->tune_overlap != 2;
+    const int64_t all = std::max<int64_t>(1, steps);
+    const int64_t first = c0->tune_chunk_first > 0 ? c0->tune_chunk_first : (one ? all : BATCH_CHUNK_FIRST);
+    const int64_t most = std::max(first, c0->tune_chunk_max > 0 ? c0->tune_chunk_max : (one ? all : BATCH_CHUNK_MAX));
+    if (*rebuilt || steps != b->steps || b->chunk.empty() || first != b->chunk_first || most != b->chunk_max) {
         batch_release_graphs(b);
         b->steps = steps;
+        b->chunk_first = first;
+        b->chunk_max = most;
         b->chunk.assign(1, 0);
-        for (int64_t size = BATCH_CHUNK_FIRST; b->chunk.back() < steps; size = std::min<int64_t>(2 * size, BATCH_CHUNK_MAX))
+        for (int64_t size = first; b->chunk.back() < steps; size = std::min<int64_t>(2 * size, most))
             b->chunk.push_back(std::min(steps, b->chunk.back() + size));
     }
     return PCT_OK;

@@ -31,16 +31,37 @@ def scans():
     return _small_scans()
 
 
+# knobs of the first context that lay the batch's block steps out in chunks
+# (each with its own MLP launches; with overlap 1 each chunk waits for its
+# check) instead of the default single chunk
+CHUNK_KNOBS = {"fp64_chunks": dict(chunk_first=1, chunk_max=4),
+               "precision_chunks_overlap1": dict(overlap=1, chunk_first=1, chunk_max=2)}
+
+
 @pytest.mark.parametrize("mode_name", ["fp64", "precision", "fp64_eager", "fp64_cached", "fp64_persistent",
-                                       "precision_persistent", "fp64_persistent_eager", "fp64_persistent_cached"])
+                                       "precision_persistent", "fp64_persistent_eager", "fp64_persistent_cached",
+                                       "fp64_chunks", "precision_chunks_overlap1"])
 def test_batch_equals_single_scans(scans, mode_name):
     import pct_amd
     P = pct_amd.PCT_DROP_PERSISTENT
     mode = {"fp64": 0, "precision": pct_amd.PCT_PRECISION, "fp64_eager": pct_amd.PCT_DROP_NO_GRAPH,
             "fp64_cached": pct_amd.PCT_DROP_CACHED, "fp64_persistent": P,
             "precision_persistent": pct_amd.PCT_PRECISION | P, "fp64_persistent_eager": P | pct_amd.PCT_DROP_NO_GRAPH,
-            "fp64_persistent_cached": P | pct_amd.PCT_DROP_CACHED}[mode_name]
+            "fp64_persistent_cached": P | pct_amd.PCT_DROP_CACHED, "fp64_chunks": 0,
+            "precision_chunks_overlap1": pct_amd.PCT_PRECISION}[mode_name]
     single = mode & ~P
+    knobs = CHUNK_KNOBS.get(mode_name)
+    if knobs:
+        pct_amd.set_tuning(scans[0][0], **knobs)
+    try:
+        _batch_vs_single(scans, mode, single)
+    finally:
+        if knobs:
+            pct_amd.set_tuning(scans[0][0], **{k: 0 for k in knobs})
+
+
+def _batch_vs_single(scans, mode, single):
+    import pct_amd
     sizes = [800, 1600, 3200, 6400][:len(scans)]
     lams = [1e-2, 2e-2, 5e-3, 1e-2][:len(scans)]
     ref = []
