@@ -1260,13 +1260,15 @@ int pct_mlp_paths_fb(pct_ctx* ctx, hipStream_t stream, const pct_hist_soa* hist,
     if (count == 0) return PCT_OK;
     if ((rc = mlp_occupancy(ctx))) return rc;
     /* a workgroup of 256 is one wavefront on each of the CU's 4 SIMDs. Beside
-     * a single scan's block loop the walks leave room (wave slots and
-     * registers) for the latency-bound block kernels (measured at 200x200x32:
-     * the exact walk, compiled for 5 wavefronts per SIMD, best at 4 beside
-     * (62.2 ms per iteration against 65.2 at 3), the table walk at 3, the
-     * surrogate at 2 (55.4 ms against 56.2 at 3 and 68.3 at 4); the batched
-     * form's steps do better with the full occupancy) */
-    static const int beside_default[3] = {4, 3, 2};
+     * a single scan's block loop the table and surrogate walks leave room
+     * (wave slots and registers) for the latency-bound block kernels; the
+     * exact walk keeps its full occupancy (measured at 200x200x32: the exact
+     * walk, compiled for 5 wavefronts per SIMD, 61.7-62.1 ms per iteration at
+     * 5 beside against 62.5-62.8 at 4 and 64.7 at 3; the table walk best at
+     * 3, 55.9 ms against 62.5 at 2; the surrogate at 2, 54.0 ms against 54.7
+     * at 3 and 68.9 at 1; the batched form's steps do better with the full
+     * occupancy). 0: the walk's own occupancy. */
+    static const int beside_default[3] = {0, 3, 2};
     /* workgroups per CU of a walk launch: the walk's occupancy, capped beside
      * the block loop or by the knob; for the table walks the per-depth table
      * goes to LDS when the workgroups the launch keeps on a CU fit in its
