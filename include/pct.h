@@ -375,9 +375,9 @@ int pct_drop_stats(pct_ctx* ctx, const pct_hist_soa* hist, int block_size, int64
  * whose launch shapes are compile-time constants, data.h:377-378). They change
  * scheduling only, never results. value <= 0 restores the default. */
 #define PCT_TUNE_MLP_WAVES     1  /* resident MLP wavefronts per SIMD while the
-                                     block loop runs beside it (default: 3 for
-                                     the exact walk and the precision form, 2
-                                     for the surrogate)                        */
+                                     block loop runs beside it (default: the
+                                     exact walk's full occupancy, 3 for the
+                                     precision form, 2 for the surrogate)      */
 #define PCT_TUNE_CHUNK_FIRST   2  /* blocks in the first chunk (default: all
                                      blocks in one chunk; 16 with overlap 1)   */
 #define PCT_TUNE_CHUNK_MAX     3  /* largest chunk in blocks (default: all; 256
