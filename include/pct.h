@@ -423,7 +423,9 @@ int pct_set_tuning(pct_ctx* ctx, int key, int value);
  * scatter and a volume scan) over chunks of whole blocks whose MLP lists take
  * at most half the budget -- when the plan would exceed `budget` bytes (free
  * device memory, or PCT_TUNE_PLAN_LIMIT_MB), V > 2^22 or the block size >
- * 8191. *need receives the bytes. Pure functions (no device). */
+ * 8191 (V: the plan's voxel index space, the grid padded to whole tiles of
+ * 4 x 8 voxels of a slice). *need receives the bytes. Pure functions (no
+ * device). */
 int64_t pct_drop_plan_bytes(int64_t n, int64_t slots, int64_t list_slots, int64_t entries, int with_prev);
 int     pct_drop_plan_choice(int64_t n, int64_t slots, int64_t list_slots, int64_t entries, int64_t V,
                              int block_size, int64_t budget, int with_prev, int64_t* need);
@@ -436,9 +438,11 @@ int     pct_drop_plan_choice(int64_t n, int64_t slots, int64_t list_slots, int64
  * block j of every scan that has one in one pair of launches (one kernel for
  * the deltas, one for the voxel updates of all those blocks), so the
  * latency-bound chains of the scans share each launch. Each scan's MLP lists
- * are recomputed every iteration on a second stream, chunk of steps by chunk
- * of steps, ahead of the block steps that need them. The batch runs on the
- * stream of scans[0]; results are identical to pct_drop_iteration per scan.
+ * are recomputed every iteration on a second stream, one launch per scan
+ * beside the block steps (with PCT_TUNE_OVERLAP 1 on the first context: chunk
+ * of steps by chunk of steps, ahead of the block steps that need them). The
+ * batch runs on the stream of scans[0]; results are identical to
+ * pct_drop_iteration per scan.
  * At most PCT_BATCH_MAX scans. */
 #define PCT_BATCH_MAX 16
 typedef struct pct_batch pct_batch;

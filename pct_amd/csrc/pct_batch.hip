@@ -557,7 +557,16 @@ extern "C" int pct_batch_iteration(pct_batch* b, const pct_hist_soa* hists, floa
     bool rebuilt;
     if ((rc = batch_plans(b, hists, block_sizes, mlp, &rebuilt))) return rc;
     const bool recompute = !(mode & PCT_DROP_CACHED) && !rebuilt;
-    if (!eager && (rc = batch_graphs(b, xs, hists, lambdas, f32, persist))) return rc;
+    /* the block steps run on each plan's tiled image: x copied in first,
+     * back after the last step */
+    std::vector<float*> xts((size_t)n, nullptr);
+    for (int s = 0; s < n; ++s)
+        if ((rc = pct_plan_image_in(b->scans[s], c0->stream, xs[s], &xts[s]))) {
+            pct_fail(c0, rc, "scan %d: %s", s, pct_last_error(b->scans[s]));
+            return rc;
+        }
+    float* const* xt = xts.data();
+    if (!eager && (rc = batch_graphs(b, xt, hists, lambdas, f32, persist))) return rc;
     const int64_t nchunks = (int64_t)b->chunk.size() - 1;
     while ((int64_t)b->ev_chunk.size() < nchunks) {
         hipEvent_t e = nullptr;
@@ -623,7 +632,7 @@ extern "C" int pct_batch_iteration(pct_batch* b, const pct_hist_soa* hists, floa
         if (recompute && nchunks > 0) PCT_HIP_CHECK(c0, hipStreamWaitEvent(c0->stream, b->ev_chunk[nchunks - 1], 0));
         if (eager) {
             batch_persist_args pa{};
-            if ((rc = batch_persist_args_of(b, xs, hists, lambdas, f32, pa)) ||
+            if ((rc = batch_persist_args_of(b, xt, hists, lambdas, f32, pa)) ||
                 (rc = batch_persist_launch(b, pa, f32)))
                 return rc;
         } else {
@@ -633,13 +642,18 @@ extern "C" int pct_batch_iteration(pct_batch* b, const pct_hist_soa* hists, floa
     for (int64_t c = 0; !persist && c < nchunks; ++c) {
         if (chunk_waits) PCT_HIP_CHECK(c0, hipStreamWaitEvent(c0->stream, b->ev_chunk[c], 0));
         if (eager) {
-            if ((rc = batch_steps(b, xs, hists, lambdas, f32, b->chunk[c], b->chunk[c + 1]))) return rc;
+            if ((rc = batch_steps(b, xt, hists, lambdas, f32, b->chunk[c], b->chunk[c + 1]))) return rc;
         } else {
             PCT_HIP_CHECK(c0, hipGraphLaunch(b->graphs[c], c0->stream));
         }
     }
     if (recompute && !chunk_waits && !persist && nchunks > 0)
         PCT_HIP_CHECK(c0, hipStreamWaitEvent(c0->stream, b->ev_chunk[nchunks - 1], 0));
+    for (int s = 0; s < n; ++s)
+        if ((rc = pct_plan_image_out(b->scans[s], c0->stream, xs[s]))) {
+            pct_fail(c0, rc, "scan %d: %s", s, pct_last_error(b->scans[s]));
+            return rc;
+        }
     if (c0->profile) {
         PCT_HIP_CHECK(c0, hipEventRecord(c0->ev[3], c0->stream));
         c0->ev_recorded = true;

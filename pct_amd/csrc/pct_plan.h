@@ -53,8 +53,12 @@ struct pct_plan {
     pct_buffer long_rows;          /* 16 B per long segment                 */
     pct_buffer blk_long;           /* int64 per block+1: first long segment */
     pct_buffer delta;              /* 2 f64 per history of two blocks       */
-    pct_buffer xw;                 /* u64 per voxel: {x, stamp} words of the
-                                      one-launch-per-block form             */
+    pct_buffer xw;                 /* u64 per tiled voxel: {x, stamp} words
+                                      of the one-launch-per-block forms     */
+    /* the block loop's image in the tiled voxel order the lists and records
+     * use (Vt floats, padding voxels zero) */
+    pct_buffer xt;
+    int64_t Vt = 0;
     std::vector<int64_t> blk_entry_h, blk_seg_h, blk_long_h, list_base_h;
     std::vector<int32_t> stride_h;
     /* the block loop in chunks of blocks [chunk_b[c], chunk_b[c+1]), one
@@ -80,5 +84,9 @@ void pct_plan_destroy(pct_plan* pl);
 /* (re)build the plan of ctx for hist with blocks of B and the given MLP mode */
 int pct_plan_build(pct_ctx* ctx, const pct_hist_soa* hist, int B, int mlp_mode);
 bool pct_plan_ready(const pct_ctx* ctx, const pct_hist_soa* hist, int B, int mlp_mode);
+/* x (row order) into the plan's tiled image on stream s, *xt its address;
+ * and the tiled image back into x */
+int pct_plan_image_in(pct_ctx* ctx, hipStream_t s, const float* x, float** xt);
+int pct_plan_image_out(pct_ctx* ctx, hipStream_t s, float* x);
 
 #endif

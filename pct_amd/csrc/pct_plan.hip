@@ -173,8 +173,8 @@ k_seg_map(const uint64_t* __restrict__ segs, const int64_t* __restrict__ blk_seg
     }
 }
 
-/* one wavefront per history of [h0, h1): its voxel list into the
- * block-strided list store and, when prev is given, slot for slot the
+/* one wavefront per history of [h0, h1): its voxel list (tiled indices)
+ * into the block-strided list store and, when prev is given, slot for slot the
  * previous block's segment record of each voxel (0 when that block does not
  * touch it; looked up in the previous block's map); with tag, the slots whose
  * voxel the previous block touches carry LIST_TAG */
@@ -183,7 +183,7 @@ k_build_entries(const uint32_t* __restrict__ path, const int64_t* __restrict__ c
                 const uint32_t* __restrict__ len, const int64_t* __restrict__ hist_off, int64_t h0, int64_t h1,
                 int64_t B, const uint64_t* __restrict__ segs, const int64_t* __restrict__ blk_seg, int64_t M, int64_t V,
                 const uint64_t* __restrict__ map, uint32_t* __restrict__ list, uint64_t* __restrict__ prev,
-                int tag) {
+                int tag, pct_blk::vox_tiling tl) {
     const int64_t h = h0 + (int64_t)blockIdx.x * 4 + (threadIdx.x >> 6);
     const int lane = threadIdx.x & 63;
     if (h >= h1) return;
@@ -194,7 +194,7 @@ k_build_entries(const uint32_t* __restrict__ path, const int64_t* __restrict__ c
     const uint64_t* m = ((prev || tag) && b > 0) ? map + ((b - 1) % M) * V : nullptr;
     const int64_t sp = b > 0 ? blk_seg[b - 1] : 0;
     for (int j = lane; j < n; j += 64) {
-        const uint32_t v = src[j];
+        const uint32_t v = pct_blk::tiled_of(tl, src[j]);
         const uint64_t w = m ? m[v] : 0ull;
         const bool in_prev = m && (w >> 32) == (uint64_t)(b - 1);
         list[dst + j] = (tag && in_prev) ? (v | LIST_TAG) : v;
@@ -202,13 +202,14 @@ k_build_entries(const uint32_t* __restrict__ path, const int64_t* __restrict__ c
     }
 }
 
-/* one wavefront per history of the chunk: keys = (local block << vbits) | voxel,
- * values = block-local history index, at the history's exact entry offset */
+/* one wavefront per history of the chunk: keys = (local block << vbits) |
+ * tiled voxel, values = block-local history index, at the history's exact
+ * entry offset */
 __global__ void __launch_bounds__(256)
 k_emit_keys(const uint32_t* __restrict__ list, const int64_t* __restrict__ hist_off,
             const uint32_t* __restrict__ len, const int64_t* __restrict__ ex_off, int64_t h0, int64_t h1,
             int64_t B, int64_t b0, int vbits, int64_t entry0, uint32_t* __restrict__ keys,
-            uint32_t* __restrict__ vals) {
+            uint32_t* __restrict__ vals, pct_blk::vox_tiling tl) {
     int64_t h = h0 + (int64_t)blockIdx.x * 4 + (threadIdx.x >> 6);
     int lane = threadIdx.x & 63;
     if (h >= h1) return;
@@ -218,7 +219,7 @@ k_emit_keys(const uint32_t* __restrict__ list, const int64_t* __restrict__ hist_
     int n = (int)len[h];
     int64_t src = hist_off[h], dst = ex_off[h] - entry0;
     for (int j = lane; j < n; j += 64) {
-        keys[dst + j] = hi | list[src + j];
+        keys[dst + j] = hi | pct_blk::tiled_of(tl, list[src + j]);
         vals[dst + j] = local;
     }
 }
@@ -424,6 +425,21 @@ k_block_wait(const uint32_t* __restrict__ list, int stride, pct_blk::blk_pos pos
                              threadIdx.x & 63);
 }
 
+/* x (row order) into the tiled image and back: grid (ceil(C / 256), R, S) */
+__global__ void k_tile_in(const float* __restrict__ x, pct_blk::vox_tiling t, float* __restrict__ xt) {
+    const int c = blockIdx.x * blockDim.x + threadIdx.x;
+    if (c >= t.C) return;
+    const int r = blockIdx.y, z = blockIdx.z;
+    xt[pct_blk::tiled_xyz(t, c, r, z)] = x[((int64_t)z * t.R + r) * t.C + c];
+}
+
+__global__ void k_tile_out(const float* __restrict__ xt, pct_blk::vox_tiling t, float* __restrict__ x) {
+    const int c = blockIdx.x * blockDim.x + threadIdx.x;
+    if (c >= t.C) return;
+    const int r = blockIdx.y, z = blockIdx.z;
+    x[((int64_t)z * t.R + r) * t.C + c] = xt[pct_blk::tiled_xyz(t, c, r, z)];
+}
+
 /* the image as {value, stamp} words (stamp: no block yet) and back */
 __global__ void k_xw_pack(const float* __restrict__ x, int64_t V, uint64_t* __restrict__ xw) {
     const int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
@@ -469,7 +485,7 @@ void pct_plan_destroy(pct_plan* pl) {
     plan_release_graph(pl);
     pct_buffer* bufs[] = {&pl->list, &pl->prev, &pl->hist_off, &pl->len, &pl->chord, &pl->hash, &pl->segs,
                           &pl->csc, &pl->long_rows, &pl->blk_seg, &pl->blk_long, &pl->blk_entry, &pl->delta,
-                          &pl->xw, &pl->pos_k, &pl->pos_len, &pl->pos_chord};
+                          &pl->xw, &pl->xt, &pl->pos_k, &pl->pos_len, &pl->pos_chord};
     for (pct_buffer* b : bufs) release(*b);
     for (hipEvent_t e : pl->ev_chunk)
         if (e) (void)hipEventDestroy(e);
@@ -496,13 +512,16 @@ static int plan_build(pct_ctx* ctx, const pct_hist_soa* hist, int B, int mlp_mod
     bool& cap_fail = capacity ? *capacity : cap_dummy;
     cap_fail = false;
     if (n <= 0) return pct_fail(ctx, PCT_E_INVALID, "DROP plan: no histories");
-    if (B <= 0 || B > SEG_MAX_B || ctx->g.V > SEG_MAX_V) {
+    const pct_blk::vox_tiling tl = pct_blk::make_tiling(ctx->g.C, ctx->g.R, ctx->g.S);
+    const int64_t Vt = tl.Vt;
+    if (B <= 0 || B > SEG_MAX_B || Vt > SEG_MAX_V) {
         cap_fail = true;
         if (ctx->plan) plan_free_buffers(ctx->plan);
-        return pct_fail(ctx, PCT_E_INVALID, "no DROP plan for %lld voxels (at most %lld) in blocks of %d (at most %lld)",
-                        (long long)ctx->g.V, (long long)SEG_MAX_V, B, (long long)SEG_MAX_B);
+        return pct_fail(ctx, PCT_E_INVALID,
+                        "no DROP plan for %lld voxels (%lld in whole tiles, at most %lld) in blocks of %d (at most %lld)",
+                        (long long)ctx->g.V, (long long)Vt, (long long)SEG_MAX_V, B, (long long)SEG_MAX_B);
     }
-    const int vbits = vbits_for(ctx->g.V);
+    const int vbits = vbits_for(Vt);
     const bool with_prev = ctx->tune_block_form == 1;
     const bool with_tags = ctx->tune_block_form == 3;
     if (!ctx->plan) ctx->plan = new pct_plan();
@@ -519,7 +538,7 @@ static int plan_build(pct_ctx* ctx, const pct_hist_soa* hist, int B, int mlp_mod
     {
         const int64_t budget = plan_budget(ctx);
         int64_t need = 0;
-        if (!pct_drop_plan_choice(n, total, 0, 0, ctx->g.V, B, budget, with_prev, &need)) {
+        if (!pct_drop_plan_choice(n, total, 0, 0, Vt, B, budget, with_prev, &need)) {
             cap_fail = true;
             plan_free_buffers(pl);
             return pct_fail(ctx, PCT_E_NOMEM, "the DROP plan's build needs %.1f GB, %.1f GB available", need / 1e9,
@@ -623,7 +642,7 @@ static int plan_build(pct_ctx* ctx, const pct_hist_soa* hist, int B, int mlp_mod
     {
         const int64_t budget = plan_budget(ctx);
         int64_t need = 0;
-        if (!pct_drop_plan_choice(n, total, list_size, E, ctx->g.V, B, budget, with_prev, &need)) {
+        if (!pct_drop_plan_choice(n, total, list_size, E, Vt, B, budget, with_prev, &need)) {
             release(ex_off);
             release(ctx->path_store);
             cap_fail = true;
@@ -698,7 +717,7 @@ static int plan_build(pct_ctx* ctx, const pct_hist_soa* hist, int B, int mlp_mod
             if (endbit > 32) endbit = 32;
             hipLaunchKernelGGL(k_emit_keys, dim3((unsigned)((h1 - h0 + 3) / 4)), dim3(256), 0, ctx->stream, list,
                                cap_off, (const uint32_t*)len, (const int64_t*)ex_off.ptr, h0, h1, (int64_t)B, b0, vbits,
-                               entry0, (uint32_t*)kin.ptr, (uint32_t*)vin.ptr);
+                               entry0, (uint32_t*)kin.ptr, (uint32_t*)vin.ptr, tl);
             size_t tb = tmp.bytes;
             PCT_HIP_CHECK(ctx, rocprim::radix_sort_pairs(tmp.ptr, tb, (uint32_t*)kin.ptr, (uint32_t*)kout.ptr,
                                                          (uint32_t*)vin.ptr, (uint32_t*)vout.ptr, (size_t)Ec, 0, endbit,
@@ -771,7 +790,7 @@ static int plan_build(pct_ctx* ctx, const pct_hist_soa* hist, int B, int mlp_mod
         release(pl->prev);
     }
     {
-        const int64_t V = ctx->g.V;
+        const int64_t V = Vt;
         const bool maps = with_prev || with_tags;
         int64_t M = maps ? std::max<int64_t>(2, std::min<int64_t>(256, ((int64_t)1 << 31) / (8 * V))) : 256;
         M = std::min<int64_t>(M, nb + 1);
@@ -799,7 +818,7 @@ static int plan_build(pct_ctx* ctx, const pct_hist_soa* hist, int B, int mlp_mod
                                (const uint32_t*)ctx->path_store.ptr, cap_off, (const uint32_t*)len, hist_off, h0, h1,
                                (int64_t)B, (const uint64_t*)pl->segs.ptr, (const int64_t*)pl->blk_seg.ptr, M, V,
                                (const uint64_t*)map.ptr, (uint32_t*)pl->list.ptr,
-                               with_prev ? (uint64_t*)pl->prev.ptr : nullptr, with_tags ? 1 : 0);
+                               with_prev ? (uint64_t*)pl->prev.ptr : nullptr, with_tags ? 1 : 0, tl);
         }
         PCT_HIP_CHECK(ctx, hipGetLastError());
         PCT_HIP_CHECK(ctx, hipStreamSynchronize(ctx->stream));
@@ -814,6 +833,7 @@ static int plan_build(pct_ctx* ctx, const pct_hist_soa* hist, int B, int mlp_mod
     pl->n = n;
     pl->B = B;
     pl->nb = nb;
+    pl->Vt = Vt;
     pl->E = E;
     pl->S = S_total;
     pl->key = *hist;
@@ -1043,6 +1063,40 @@ static int plan_ensure_events(pct_ctx* ctx, pct_plan* pl, int64_t nchunks) {
     return PCT_OK;
 }
 
+/* the tiled image (Vt floats; zero on allocation, so the padding voxels,
+ * which no list or record names, hold zero) */
+static int plan_ensure_xt(pct_ctx* ctx, pct_plan* pl, hipStream_t s) {
+    if (pl->xt.ptr && pl->xt.bytes >= sizeof(float) * (size_t)pl->Vt) return PCT_OK;
+    int rc = pct_ensure(ctx, pl->xt, sizeof(float) * (size_t)pl->Vt);
+    if (rc) return rc;
+    hipLaunchKernelGGL(k_fill_u32, dim3(std::min(pct_grid(pl->Vt, 256), PCT_GRID_CAP)), dim3(256), 0, s,
+                       (uint32_t*)pl->xt.ptr, pl->Vt, 0u);
+    PCT_HIP_CHECK(ctx, hipGetLastError());
+    return PCT_OK;
+}
+
+static dim3 tile_grid(const pct_blk::vox_tiling& t) {
+    return dim3((unsigned)((t.C + 255) / 256), (unsigned)t.R, (unsigned)t.S);
+}
+
+int pct_plan_image_in(pct_ctx* ctx, hipStream_t s, const float* x, float** xt) {
+    pct_plan* pl = ctx->plan;
+    int rc = plan_ensure_xt(ctx, pl, s);
+    if (rc) return rc;
+    const pct_blk::vox_tiling tl = pct_blk::make_tiling(ctx->g.C, ctx->g.R, ctx->g.S);
+    hipLaunchKernelGGL(k_tile_in, tile_grid(tl), dim3(256), 0, s, x, tl, (float*)pl->xt.ptr);
+    PCT_HIP_CHECK(ctx, hipGetLastError());
+    *xt = (float*)pl->xt.ptr;
+    return PCT_OK;
+}
+
+int pct_plan_image_out(pct_ctx* ctx, hipStream_t s, float* x) {
+    const pct_blk::vox_tiling tl = pct_blk::make_tiling(ctx->g.C, ctx->g.R, ctx->g.S);
+    hipLaunchKernelGGL(k_tile_out, tile_grid(tl), dim3(256), 0, s, (const float*)ctx->plan->xt.ptr, tl, x);
+    PCT_HIP_CHECK(ctx, hipGetLastError());
+    return PCT_OK;
+}
+
 /* One iteration on the plan. With `recompute`, the MLP lists of every chunk
  * of blocks are recomputed on the context's second stream, ahead of the block
  * loop, which takes chunk c as soon as its lists are done: the FP64-bound MLP
@@ -1054,11 +1108,17 @@ static int plan_iterate(pct_ctx* ctx, pct_plan* pl, const pct_hist_soa* hist, fl
     int rc;
     plan_chunks(ctx, pl);
     const int64_t nchunks = (int64_t)pl->chunk_b.size() - 1;
-    if (words && (rc = pct_ensure(ctx, pl->xw, sizeof(uint64_t) * (size_t)ctx->g.V))) return rc;
-    if (!eager && (rc = plan_ensure_graphs(ctx, pl, hist, x, lambda, f32, variant))) return rc;
+    const int64_t Vt = pl->Vt;
+    if ((rc = plan_ensure_xt(ctx, pl, ctx->stream))) return rc;
+    float* xt = (float*)pl->xt.ptr;
+    if (words && (rc = pct_ensure(ctx, pl->xw, sizeof(uint64_t) * (size_t)Vt))) return rc;
+    if (!eager && (rc = plan_ensure_graphs(ctx, pl, hist, xt, lambda, f32, variant))) return rc;
+    const pct_blk::vox_tiling tl = pct_blk::make_tiling(ctx->g.C, ctx->g.R, ctx->g.S);
+    const dim3 tgrid = tile_grid(tl);
+    hipLaunchKernelGGL(k_tile_in, tgrid, dim3(256), 0, ctx->stream, (const float*)x, tl, xt);
     if (words)
-        hipLaunchKernelGGL(k_xw_pack, dim3(pct_grid(ctx->g.V, 256)), dim3(256), 0, ctx->stream, (const float*)x,
-                           ctx->g.V, (uint64_t*)pl->xw.ptr);
+        hipLaunchKernelGGL(k_xw_pack, dim3(pct_grid(Vt, 256)), dim3(256), 0, ctx->stream, (const float*)xt, Vt,
+                           (uint64_t*)pl->xw.ptr);
     if (ctx->profile) PCT_HIP_CHECK(ctx, hipEventRecord(ctx->ev[0], ctx->stream));
     if (recompute && ctx->tune_overlap == 2) {
         /* sequential: all lists first, then the block loop */
@@ -1100,15 +1160,16 @@ static int plan_iterate(pct_ctx* ctx, pct_plan* pl, const pct_hist_soa* hist, fl
         if (chunk_waits) PCT_HIP_CHECK(ctx, hipStreamWaitEvent(ctx->stream, pl->ev_chunk[c], 0));
         if (eager) {
             int64_t b0 = pl->chunk_b[c], b1 = pl->chunk_b[c + 1];
-            if ((rc = plan_block_loop(ctx, pl, hist, x, lambda, f32, variant, b0, b1))) return rc;
+            if ((rc = plan_block_loop(ctx, pl, hist, xt, lambda, f32, variant, b0, b1))) return rc;
         } else {
             PCT_HIP_CHECK(ctx, hipGraphLaunch(pl->graphs[c], ctx->stream));
         }
     }
     if (recompute && !chunk_waits) PCT_HIP_CHECK(ctx, hipStreamWaitEvent(ctx->stream, pl->ev_chunk[nchunks - 1], 0));
     if (words)
-        hipLaunchKernelGGL(k_xw_unpack, dim3(pct_grid(ctx->g.V, 256)), dim3(256), 0, ctx->stream,
-                           (const uint64_t*)pl->xw.ptr, ctx->g.V, x, ctx->d_status);
+        hipLaunchKernelGGL(k_xw_unpack, dim3(pct_grid(Vt, 256)), dim3(256), 0, ctx->stream,
+                           (const uint64_t*)pl->xw.ptr, Vt, xt, ctx->d_status);
+    hipLaunchKernelGGL(k_tile_out, tgrid, dim3(256), 0, ctx->stream, (const float*)xt, tl, x);
     if (ctx->profile) {
         PCT_HIP_CHECK(ctx, hipEventRecord(ctx->ev[3], ctx->stream));
         ctx->ev_recorded = true;
@@ -1165,8 +1226,8 @@ static int64_t plan_budget(pct_ctx* ctx) {
 static void plan_free_buffers(pct_plan* pl) {
     plan_release_graph(pl);
     for (pct_buffer* b : {&pl->list, &pl->prev, &pl->hist_off, &pl->len, &pl->chord, &pl->hash, &pl->segs, &pl->csc,
-                          &pl->long_rows, &pl->blk_seg, &pl->blk_long, &pl->blk_entry, &pl->delta, &pl->xw, &pl->pos_k,
-                          &pl->pos_len, &pl->pos_chord})
+                          &pl->long_rows, &pl->blk_seg, &pl->blk_long, &pl->blk_entry, &pl->delta, &pl->xw, &pl->xt,
+                          &pl->pos_k, &pl->pos_len, &pl->pos_chord})
         release(*b);
     pl->valid = false;
     ++pl->generation;

@@ -48,6 +48,45 @@
 
 namespace pct_blk {
 
+/* The plan's voxel index space: the image in tiles of TILE_X x TILE_Y voxels
+ * of one slice (32 FP32 values, one 128-byte line), tiles in row order within
+ * a slice, the grid padded to whole tiles. A path's consecutive voxels share
+ * lines in every in-plane direction, where row order shares them only along
+ * x: the delta kernels' x gathers touch about a third of the lines. Lists,
+ * segment records and the block loop's image use tiled indices; an iteration
+ * copies x into the tiled image and back. */
+#define TILE_X 4
+#define TILE_Y 8
+struct vox_tiling {
+    int32_t C, R, S;        /* grid columns, rows, slices                  */
+    int32_t tiles_x;        /* tiles per row of tiles: ceil(C / TILE_X)   */
+    int64_t per_slice;      /* tiled voxels per slice                      */
+    int64_t Vt;             /* tiled voxels                                */
+};
+
+inline vox_tiling make_tiling(int C, int R, int S) {
+    vox_tiling t;
+    t.C = C;
+    t.R = R;
+    t.S = S;
+    t.tiles_x = (C + TILE_X - 1) / TILE_X;
+    t.per_slice = (int64_t)t.tiles_x * ((R + TILE_Y - 1) / TILE_Y) * (TILE_X * TILE_Y);
+    t.Vt = t.per_slice * S;
+    return t;
+}
+
+__host__ __device__ __forceinline__ uint32_t tiled_xyz(const vox_tiling& t, int x, int y, int z) {
+    return (uint32_t)((int64_t)z * t.per_slice +
+                      ((int64_t)(y / TILE_Y) * t.tiles_x + x / TILE_X) * (TILE_X * TILE_Y) +
+                      (y % TILE_Y) * TILE_X + x % TILE_X);
+}
+
+/* the tiled index of row-order voxel v */
+__host__ __device__ __forceinline__ uint32_t tiled_of(const vox_tiling& t, uint32_t v) {
+    const uint32_t x = v % (uint32_t)t.C, r = v / (uint32_t)t.C;
+    return tiled_xyz(t, (int)x, (int)(r % (uint32_t)t.R), (int)(r / (uint32_t)t.R));
+}
+
 __device__ __forceinline__ void kahan_add(float& s, float& c, float v) {
     float y = v - c;
     float t = s + y;

@@ -193,3 +193,37 @@ def test_sm_research_chord_on_empty_and_degenerate_input(mini_host):
         assert np.array_equal(xd.cpu().numpy().view(np.uint32), xo.view(np.uint32)), variant
         assert int(cnt.sum()) == 0 and np.sum(xo != x0) > 0
     ctx.close()
+
+
+@pytest.mark.parametrize("block_form", [2, 1, 3])
+def test_drop_on_a_grid_of_partial_tiles(block_form):
+    """a 38x38x5 grid: neither dimension a multiple of the plan's 4 x 8 voxel
+    tiles, so the tiled image is padded; the image against oracle.drop for
+    every block-loop form"""
+    import oracle
+    import pct_amd
+    from pct_amd import configs
+    p = configs.params("mini", columns=38, rows=38, slices=5)
+    p.recon_cyl_height = p.slices * p.slice_thickness
+    cfg = configs.synth_cfg("mini", n_histories=12_000, seed=configs.SEED_BASE + 77)
+    raw = pct_amd.generate_host(cfg).numpy()
+    h_all, missed = oracle.recon_volume_intersections(p, dict(raw))
+    h = h_all.select(missed == 0)
+    hull = oracle.hull_init(p)
+    inter = oracle.mlp_endpoints(p, h, hull)
+    hr = h.select(inter == 1)
+    assert hr.n > 2000
+    x0 = hull.astype(np.float32)
+    lam, block = 1e-2, 700
+    exp = oracle.drop(p, hr, x0, iterations=2, block_size=block, lam=lam)
+    ctx = pct_amd.Context(p)
+    pct_amd.set_tuning(ctx, block_form=block_form)
+    d = _to_dev(hr)
+    x = ctx.torch.from_numpy(x0.copy()).cuda()
+    ctx.drop_iteration(d, x, block, lam)
+    ctx.drop_iteration(d, x, block, lam)
+    got = x.cpu().numpy()
+    assert np.abs(got - x0).max() > 1e-3
+    assert np.max(np.abs(got - exp)) <= 8 * np.spacing(np.float32(2.0))
+    assert np.array_equal(got != x0, exp != x0)
+    ctx.close()
