@@ -104,6 +104,7 @@ struct batch_scan {                 /* one scan's plan and image */
     const uint16_t* pos_k;
     const uint32_t* pos_len;
     const double* pos_chord;
+    const uint16_t* pos_r;
     const float* wepl;
     float* x;
     double lambda;
@@ -207,7 +208,7 @@ k_batch_persist(batch_persist_args a) {
             const int s = persist_scan_of(ph, a.n, f);
             const batch_scan& c = a.sc[s];
             const batch_step& e = d[s];
-            const pct_blk::blk_pos pos{c.pos_k + e.h0, c.pos_len + e.h0, c.pos_chord + e.h0};
+            const pct_blk::blk_pos pos{c.pos_k + e.h0, c.pos_len + e.h0, c.pos_chord + e.h0, c.pos_r + e.h0};
             if (F32)
                 pct_blk::delta_f32(c.list + e.list_base, e.stride, pos, c.wepl, e.h0, f - ph[s], c.x,
                                    (float)c.lambda, (float*)c.delta, lane);
@@ -384,7 +385,7 @@ static int batch_steps(pct_batch* b, float* const* xs, const pct_hist_soa* hists
             da.list[d] = (const uint32_t*)pl->list.ptr + pl->list_base_h[j];
             da.stride[d] = pl->stride_h[j];
             da.pos[d] = pct_blk::blk_pos{(const uint16_t*)pl->pos_k.ptr + h0, (const uint32_t*)pl->pos_len.ptr + h0,
-                                         (const double*)pl->pos_chord.ptr + h0};
+                                         (const double*)pl->pos_chord.ptr + h0, (const uint16_t*)pl->pos_r.ptr + h0};
             da.wepl[d] = hists[s].wepl;
             da.h0[d] = h0;
             da.count[d] = cnt;
@@ -432,7 +433,8 @@ static int batch_persist_args_of(pct_batch* b, float* const* xs, const pct_hist_
     for (int s = 0; s < n; ++s) {
         const pct_plan* pl = b->scans[s]->plan;
         a.sc[s] = batch_scan{(const uint32_t*)pl->list.ptr, (const uint16_t*)pl->pos_k.ptr,
-                             (const uint32_t*)pl->pos_len.ptr, (const double*)pl->pos_chord.ptr, hists[s].wepl, xs[s],
+                             (const uint32_t*)pl->pos_len.ptr, (const double*)pl->pos_chord.ptr,
+                             (const uint16_t*)pl->pos_r.ptr, hists[s].wepl, xs[s],
                              (double)lambdas[s], pl->delta.ptr, (const uint64_t*)pl->segs.ptr,
                              (const uint16_t*)pl->csc.ptr, (const pct_blk::long_row*)pl->long_rows.ptr};
     }

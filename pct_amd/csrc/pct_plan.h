@@ -34,6 +34,10 @@ struct pct_plan {
     pct_buffer pos_k;              /* u16 per history                       */
     pct_buffer pos_len;            /* u32 per history                       */
     pct_buffer pos_chord;          /* f64 per history                       */
+    pct_buffer pos_r;              /* u16 per history: lateral rank, the
+                                      slot of its delta                     */
+    pct_buffer rank;               /* u16 per history (build only): the
+                                      same by history                       */
     pct_buffer len;                /* u32 per history: list length          */
     pct_buffer chord;              /* f64 per history: effective chord      */
     pct_buffer hash;               /* u32 per history: hash of its list     */

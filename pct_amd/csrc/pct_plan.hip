@@ -117,12 +117,14 @@ __global__ void k_lateral_keys(pct_hist_soa h, int64_t n, int64_t B, uint64_t* _
  * block with (q / DELTA_WAVES) % 8 == g -- the positions the delta workgroups
  * of XCD g take (workgroup j runs on XCD j % 8) -- and band g fills those
  * positions in order. Writes per position the block-local index, the list
- * length and the chord length, and per history the offset of its row. */
+ * length, the chord length and the lateral rank r (the history's index in
+ * the sorted order of its block), and per history its rank and the offset of
+ * its row. */
 __global__ void k_positions(const uint32_t* __restrict__ sorted, int64_t n, int64_t B, int64_t nb,
                             const int64_t* __restrict__ list_base, const int32_t* __restrict__ stride,
                             const uint32_t* __restrict__ len, const double* __restrict__ chord,
                             uint16_t* __restrict__ pos_k, uint32_t* __restrict__ pos_len, double* __restrict__ pos_chord,
-                            int64_t* __restrict__ hist_off) {
+                            uint16_t* __restrict__ pos_r, uint16_t* __restrict__ rank, int64_t* __restrict__ hist_off) {
     const int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
     if (i >= n) return;
     const int64_t h = sorted[i];
@@ -143,6 +145,8 @@ __global__ void k_positions(const uint32_t* __restrict__ sorted, int64_t n, int6
     pos_k[at] = (uint16_t)(h - b * B);
     pos_len[at] = len[h];
     pos_chord[at] = chord[h];
+    pos_r[at] = (uint16_t)r;
+    rank[h] = (uint16_t)r;
     hist_off[h] = list_base[b] + (int64_t)q * stride[b];
     if (i == n - 1) hist_off[n] = list_base[nb];
 }
@@ -203,19 +207,20 @@ k_build_entries(const uint32_t* __restrict__ path, const int64_t* __restrict__ c
 }
 
 /* one wavefront per history of the chunk: keys = (local block << vbits) |
- * tiled voxel, values = block-local history index, at the history's exact
- * entry offset */
+ * tiled voxel, values = the history's lateral rank in its block, at the
+ * history's exact entry offset (histories in order, so a segment lists its
+ * histories in history order after the stable sort) */
 __global__ void __launch_bounds__(256)
 k_emit_keys(const uint32_t* __restrict__ list, const int64_t* __restrict__ hist_off,
             const uint32_t* __restrict__ len, const int64_t* __restrict__ ex_off, int64_t h0, int64_t h1,
-            int64_t B, int64_t b0, int vbits, int64_t entry0, uint32_t* __restrict__ keys,
-            uint32_t* __restrict__ vals, pct_blk::vox_tiling tl) {
+            int64_t B, int64_t b0, int vbits, int64_t entry0, const uint16_t* __restrict__ rank,
+            uint32_t* __restrict__ keys, uint32_t* __restrict__ vals, pct_blk::vox_tiling tl) {
     int64_t h = h0 + (int64_t)blockIdx.x * 4 + (threadIdx.x >> 6);
     int lane = threadIdx.x & 63;
     if (h >= h1) return;
     int64_t b = h / B;
     uint32_t hi = (uint32_t)(b - b0) << vbits;
-    uint32_t local = (uint32_t)(h - b * B);
+    uint32_t local = rank[h];
     int n = (int)len[h];
     int64_t src = hist_off[h], dst = ex_off[h] - entry0;
     for (int j = lane; j < n; j += 64) {
@@ -307,26 +312,26 @@ __global__ void k_write_segments(const uint32_t* __restrict__ keys, const uint32
 template <int SLOTS>
 __global__ void __launch_bounds__(64 * DELTA_WAVES)
 k_block_delta(const uint32_t* __restrict__ list, const uint16_t* __restrict__ pos_k,
-              const uint32_t* __restrict__ pos_len, const double* __restrict__ pos_chord, int stride, int64_t count,
-              const float* __restrict__ x, const float* __restrict__ wepl, int64_t h0, double lambda,
-              double* __restrict__ delta) {
+              const uint32_t* __restrict__ pos_len, const double* __restrict__ pos_chord,
+              const uint16_t* __restrict__ pos_r, int stride, int64_t count, const float* __restrict__ x,
+              const float* __restrict__ wepl, int64_t h0, double lambda, double* __restrict__ delta) {
     __builtin_amdgcn_s_setprio(3);
     const int64_t q = (int64_t)blockIdx.x * DELTA_WAVES + (threadIdx.x >> 6);
     if (q >= count) return;
-    pct_blk::delta_f64<SLOTS>(list, stride, {pos_k, pos_len, pos_chord}, wepl, h0, q, x, lambda, delta,
+    pct_blk::delta_f64<SLOTS>(list, stride, {pos_k, pos_len, pos_chord, pos_r}, wepl, h0, q, x, lambda, delta,
                               threadIdx.x & 63);
 }
 
 template <int SLOTS>
 __global__ void __launch_bounds__(64 * DELTA_WAVES)
 k_block_delta_f32(const uint32_t* __restrict__ list, const uint16_t* __restrict__ pos_k,
-                  const uint32_t* __restrict__ pos_len, const double* __restrict__ pos_chord, int stride,
-                  int64_t count, const float* __restrict__ x, const float* __restrict__ wepl, int64_t h0,
-                  float lambda, float* __restrict__ delta) {
+                  const uint32_t* __restrict__ pos_len, const double* __restrict__ pos_chord,
+                  const uint16_t* __restrict__ pos_r, int stride, int64_t count, const float* __restrict__ x,
+                  const float* __restrict__ wepl, int64_t h0, float lambda, float* __restrict__ delta) {
     __builtin_amdgcn_s_setprio(3);
     const int64_t q = (int64_t)blockIdx.x * DELTA_WAVES + (threadIdx.x >> 6);
     if (q >= count) return;
-    pct_blk::delta_f32<SLOTS>(list, stride, {pos_k, pos_len, pos_chord}, wepl, h0, q, x, lambda, delta,
+    pct_blk::delta_f32<SLOTS>(list, stride, {pos_k, pos_len, pos_chord, pos_r}, wepl, h0, q, x, lambda, delta,
                               threadIdx.x & 63);
 }
 
@@ -365,12 +370,12 @@ k_block_delta_var(const uint32_t* __restrict__ list, int stride, pct_blk::blk_po
 template <int V>
 __global__ void __launch_bounds__(APPLY_THREADS)
 k_block_apply_var(const uint64_t* __restrict__ segs, int64_t s0, int64_t ns, const uint16_t* __restrict__ csc,
-                  const double* __restrict__ out, int64_t B, const double* __restrict__ chord_b, double lambda,
-                  double eta, int psi_sign, float* __restrict__ x) {
+                  const double* __restrict__ out, int64_t B, double lambda, double eta, int psi_sign,
+                  float* __restrict__ x) {
     __builtin_amdgcn_s_setprio(3);
     int64_t k = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
     if (k >= ns) return;
-    pct_blk::apply_variant<V>(segs[s0 + k], csc, out, B, chord_b, lambda, eta, psi_sign, x);
+    pct_blk::apply_variant<V>(segs[s0 + k], csc, out, B, lambda, eta, psi_sign, x);
 }
 
 /* one launch per block b: the voxel updates of block b-1 (workgroups [0, na),
@@ -485,7 +490,7 @@ void pct_plan_destroy(pct_plan* pl) {
     plan_release_graph(pl);
     pct_buffer* bufs[] = {&pl->list, &pl->prev, &pl->hist_off, &pl->len, &pl->chord, &pl->hash, &pl->segs,
                           &pl->csc, &pl->long_rows, &pl->blk_seg, &pl->blk_long, &pl->blk_entry, &pl->delta,
-                          &pl->xw, &pl->xt, &pl->pos_k, &pl->pos_len, &pl->pos_chord};
+                          &pl->xw, &pl->xt, &pl->pos_k, &pl->pos_len, &pl->pos_chord, &pl->pos_r, &pl->rank};
     for (pct_buffer* b : bufs) release(*b);
     for (hipEvent_t e : pl->ev_chunk)
         if (e) (void)hipEventDestroy(e);
@@ -588,7 +593,9 @@ static int plan_build(pct_ctx* ctx, const pct_hist_soa* hist, int B, int mlp_mod
         /* placement of each block's rows: lateral bands per XCD */
         if ((rc = pct_ensure(ctx, pl->pos_k, sizeof(uint16_t) * (size_t)n)) ||
             (rc = pct_ensure(ctx, pl->pos_len, sizeof(uint32_t) * (size_t)n)) ||
-            (rc = pct_ensure(ctx, pl->pos_chord, sizeof(double) * (size_t)n))) {
+            (rc = pct_ensure(ctx, pl->pos_chord, sizeof(double) * (size_t)n)) ||
+            (rc = pct_ensure(ctx, pl->pos_r, sizeof(uint16_t) * (size_t)n)) ||
+            (rc = pct_ensure(ctx, pl->rank, sizeof(uint16_t) * (size_t)n))) {
             free_tmp();
             return rc;
         }
@@ -616,7 +623,8 @@ static int plan_build(pct_ctx* ctx, const pct_hist_soa* hist, int B, int mlp_mod
         hipLaunchKernelGGL(k_positions, dim3(pct_grid(n, 256)), dim3(256), 0, ctx->stream, (const uint32_t*)vout.ptr,
                            n, (int64_t)B, nb, (const int64_t*)tmp_base.ptr, (const int32_t*)tmp_stride.ptr,
                            (const uint32_t*)len, (const double*)pl->chord.ptr, (uint16_t*)pl->pos_k.ptr,
-                           (uint32_t*)pl->pos_len.ptr, (double*)pl->pos_chord.ptr, (int64_t*)pl->hist_off.ptr);
+                           (uint32_t*)pl->pos_len.ptr, (double*)pl->pos_chord.ptr, (uint16_t*)pl->pos_r.ptr,
+                           (uint16_t*)pl->rank.ptr, (int64_t*)pl->hist_off.ptr);
         PCT_HIP_CHECK(ctx, hipGetLastError());
         PCT_HIP_CHECK(ctx, hipStreamSynchronize(ctx->stream));
         free_sort();
@@ -717,7 +725,7 @@ static int plan_build(pct_ctx* ctx, const pct_hist_soa* hist, int B, int mlp_mod
             if (endbit > 32) endbit = 32;
             hipLaunchKernelGGL(k_emit_keys, dim3((unsigned)((h1 - h0 + 3) / 4)), dim3(256), 0, ctx->stream, list,
                                cap_off, (const uint32_t*)len, (const int64_t*)ex_off.ptr, h0, h1, (int64_t)B, b0, vbits,
-                               entry0, (uint32_t*)kin.ptr, (uint32_t*)vin.ptr, tl);
+                               entry0, (const uint16_t*)pl->rank.ptr, (uint32_t*)kin.ptr, (uint32_t*)vin.ptr, tl);
             size_t tb = tmp.bytes;
             PCT_HIP_CHECK(ctx, rocprim::radix_sort_pairs(tmp.ptr, tb, (uint32_t*)kin.ptr, (uint32_t*)kout.ptr,
                                                          (uint32_t*)vin.ptr, (uint32_t*)vout.ptr, (size_t)Ec, 0, endbit,
@@ -763,6 +771,7 @@ static int plan_build(pct_ctx* ctx, const pct_hist_soa* hist, int B, int mlp_mod
         }
     }
     free_all();
+    release(pl->rank);
     /* sentinels */
     PCT_HIP_CHECK(ctx, hipMemcpyAsync((int64_t*)pl->blk_seg.ptr + nb, &S_total, sizeof(int64_t), hipMemcpyHostToDevice,
                                       ctx->stream));
@@ -891,7 +900,7 @@ static int plan_block_loop(pct_ctx* ctx, pct_plan* pl, const pct_hist_soa* hist,
     const double* chord = (const double*)pl->chord.ptr;
     auto pos_of = [&](int64_t b) {
         return pct_blk::blk_pos{(const uint16_t*)pl->pos_k.ptr + b * B, (const uint32_t*)pl->pos_len.ptr + b * B,
-                                (const double*)pl->pos_chord.ptr + b * B};
+                                (const double*)pl->pos_chord.ptr + b * B, (const uint16_t*)pl->pos_r.ptr + b * B};
     };
     const uint64_t* segs = (const uint64_t*)pl->segs.ptr;
     const uint16_t* csc = (const uint16_t*)pl->csc.ptr;
@@ -969,21 +978,21 @@ static int plan_block_loop(pct_ctx* ctx, pct_plan* pl, const pct_hist_soa* hist,
             const uint16_t* cb = csc + pl->blk_entry_h[b];
             if (ns > 0 && variant == 1)
                 hipLaunchKernelGGL(k_block_apply_var<1>, dim3(pct_grid(ns, APPLY_THREADS)), dim3(APPLY_THREADS), 0, ctx->stream, segs, s0, ns,
-                                   cb, (const double*)out, B, chord + h0, lambda, eta, psi, x);
+                                   cb, (const double*)out, B, lambda, eta, psi, x);
             else if (ns > 0)
                 hipLaunchKernelGGL(k_block_apply_var<2>, dim3(pct_grid(ns, APPLY_THREADS)), dim3(APPLY_THREADS), 0, ctx->stream, segs, s0, ns,
-                                   cb, (const double*)out, B, chord + h0, lambda, eta, psi, x);
+                                   cb, (const double*)out, B, lambda, eta, psi, x);
             continue;
         }
         const pct_blk::blk_pos pb = pos_of(b);
         const bool short_rows = stride <= 4 * 64;
         if (f32)
             hipLaunchKernelGGL((short_rows ? k_block_delta_f32<4> : k_block_delta_f32<DELTA_SLOTS>), dgrid,
-                               dim3(64 * DELTA_WAVES), 0, ctx->stream, lb, pb.k, pb.len, pb.chord, stride, cnt,
+                               dim3(64 * DELTA_WAVES), 0, ctx->stream, lb, pb.k, pb.len, pb.chord, pb.r, stride, cnt,
                                (const float*)x, (const float*)hist->wepl, h0, (float)lambda, (float*)delta);
         else
             hipLaunchKernelGGL((short_rows ? k_block_delta<4> : k_block_delta<DELTA_SLOTS>), dgrid,
-                               dim3(64 * DELTA_WAVES), 0, ctx->stream, lb, pb.k, pb.len, pb.chord, stride, cnt,
+                               dim3(64 * DELTA_WAVES), 0, ctx->stream, lb, pb.k, pb.len, pb.chord, pb.r, stride, cnt,
                                (const float*)x, (const float*)hist->wepl, h0, lambda, (double*)delta);
         int64_t s0 = pl->blk_seg_h[b], ns = pl->blk_seg_h[b + 1] - s0;
         const int64_t l0 = pl->blk_long_h[b], nlong = pl->blk_long_h[b + 1] - l0;
@@ -1185,15 +1194,15 @@ static int plan_iterate(pct_ctx* ctx, pct_plan* pl, const pct_hist_soa* hist, fl
  * path lengths: the capacity-strided build store holds 4 B per slot),
  * `list_slots` slots of the block-strided list store (4 B each, 12 B with the
  * one-launch form's previous-block records), `entries` path entries (at most
- * one 8 B segment record and one 2 B csc index each), 38 B per history kept
- * (length, chord, hash, row offset, placement) and 32 B per history plus up
+ * one 8 B segment record and one 2 B csc index each), 40 B per history kept
+ * (length, chord, hash, row offset, placement, rank) and 34 B per history plus up
  * to 4 GiB of workspace while building (offsets, placement sort, the radix
  * sort of a chunk of blocks, the voxel maps). Before the MLP pass only n and
  * slots are known: list_slots = entries = 0 gives what the build needs to
  * start. */
 extern "C" int64_t pct_drop_plan_bytes(int64_t n, int64_t slots, int64_t list_slots, int64_t entries,
                                        int with_prev) {
-    return 4 * slots + (with_prev ? 12 : 4) * list_slots + 10 * entries + 70 * n + ((int64_t)4 << 30);
+    return 4 * slots + (with_prev ? 12 : 4) * list_slots + 10 * entries + 74 * n + ((int64_t)4 << 30);
 }
 
 /* 1: build a plan; 0: run the chunked reference structure instead -- when
@@ -1215,7 +1224,7 @@ static int64_t plan_budget(pct_ctx* ctx) {
     if (ctx->plan) {
         pct_plan* pl = ctx->plan;
         for (pct_buffer* b : {&pl->list, &pl->prev, &pl->segs, &pl->csc, &pl->long_rows, &pl->hist_off, &pl->pos_k,
-                              &pl->pos_len, &pl->pos_chord, &pl->len, &pl->chord, &pl->hash})
+                              &pl->pos_len, &pl->pos_chord, &pl->pos_r, &pl->len, &pl->chord, &pl->hash})
             budget += (int64_t)b->bytes;
     }
     budget -= (int64_t)1 << 30;
@@ -1227,7 +1236,7 @@ static void plan_free_buffers(pct_plan* pl) {
     plan_release_graph(pl);
     for (pct_buffer* b : {&pl->list, &pl->prev, &pl->hist_off, &pl->len, &pl->chord, &pl->hash, &pl->segs, &pl->csc,
                           &pl->long_rows, &pl->blk_seg, &pl->blk_long, &pl->blk_entry, &pl->delta, &pl->xw, &pl->xt,
-                          &pl->pos_k, &pl->pos_len, &pl->pos_chord})
+                          &pl->pos_k, &pl->pos_len, &pl->pos_chord, &pl->pos_r, &pl->rank})
         release(*b);
     pl->valid = false;
     ++pl->generation;

@@ -48,19 +48,22 @@
 
 namespace pct_blk {
 
-/* The plan's voxel index space: the image in tiles of TILE_X x TILE_Y voxels
- * of one slice (32 FP32 values, one 128-byte line), tiles in row order within
- * a slice, the grid padded to whole tiles. A path's consecutive voxels share
- * lines in every in-plane direction, where row order shares them only along
- * x: the delta kernels' x gathers touch about a third of the lines. Lists,
- * segment records and the block loop's image use tiled indices; an iteration
- * copies x into the tiled image and back. */
+/* The plan's voxel index space: the image in tiles of TILE_X x TILE_Y x
+ * TILE_Z voxels (32 FP32 values, one 128-byte line), tiles in row order, the
+ * grid padded to whole tiles. A path's consecutive voxels share lines in
+ * every in-plane direction, where row order shares them only along x: the
+ * delta kernels' x gathers touch about a third of the lines. Lists, segment
+ * records and the block loop's image use tiled indices; an iteration copies x
+ * into the tiled image and back. */
+#ifndef TILE_X
 #define TILE_X 4
 #define TILE_Y 8
+#define TILE_Z 1
+#endif
 struct vox_tiling {
     int32_t C, R, S;        /* grid columns, rows, slices                  */
     int32_t tiles_x;        /* tiles per row of tiles: ceil(C / TILE_X)   */
-    int64_t per_slice;      /* tiled voxels per slice                      */
+    int64_t per_slab;       /* tiled voxels per TILE_Z slices              */
     int64_t Vt;             /* tiled voxels                                */
 };
 
@@ -70,15 +73,15 @@ inline vox_tiling make_tiling(int C, int R, int S) {
     t.R = R;
     t.S = S;
     t.tiles_x = (C + TILE_X - 1) / TILE_X;
-    t.per_slice = (int64_t)t.tiles_x * ((R + TILE_Y - 1) / TILE_Y) * (TILE_X * TILE_Y);
-    t.Vt = t.per_slice * S;
+    t.per_slab = (int64_t)t.tiles_x * ((R + TILE_Y - 1) / TILE_Y) * (TILE_X * TILE_Y * TILE_Z);
+    t.Vt = t.per_slab * ((S + TILE_Z - 1) / TILE_Z);
     return t;
 }
 
 __host__ __device__ __forceinline__ uint32_t tiled_xyz(const vox_tiling& t, int x, int y, int z) {
-    return (uint32_t)((int64_t)z * t.per_slice +
-                      ((int64_t)(y / TILE_Y) * t.tiles_x + x / TILE_X) * (TILE_X * TILE_Y) +
-                      (y % TILE_Y) * TILE_X + x % TILE_X);
+    return (uint32_t)((int64_t)(z / TILE_Z) * t.per_slab +
+                      ((int64_t)(y / TILE_Y) * t.tiles_x + x / TILE_X) * (TILE_X * TILE_Y * TILE_Z) +
+                      ((z % TILE_Z) * TILE_Y + y % TILE_Y) * TILE_X + x % TILE_X);
 }
 
 /* the tiled index of row-order voxel v */
@@ -172,11 +175,15 @@ __device__ __forceinline__ void gather_slots(const float* __restrict__ x, const 
  * are that history's block-local index, list length and chord length; the
  * block's first history is h0. Rows are placed so that the workgroups on one
  * XCD get one lateral band of the block's paths (the plan build); results do
- * not depend on the placement, every history's delta goes to delta[k]. */
+ * not depend on the placement. pos.r[q] is the history's rank in lateral
+ * order within the block: its delta goes to delta[r], and the segment
+ * records name histories by rank (still listed in history order), so the
+ * histories through a compact group of voxels have nearby deltas. */
 struct blk_pos {
     const uint16_t* k;
     const uint32_t* len;
     const double* chord;
+    const uint16_t* r;
 };
 
 /* the history at position q of the block (one wavefront); SLOTS list slots
@@ -187,6 +194,7 @@ __device__ __forceinline__ void delta_f64(const uint32_t* __restrict__ list, int
                                           const float* __restrict__ x, double lambda, double* __restrict__ delta,
                                           int lane) {
     const uint32_t k = pos.k[q];
+    const uint32_t rk = pos.r[q];
     const uint32_t n = pos.len[q];
     const double L = pos.chord[q];
     const uint32_t* p = list + q * stride;
@@ -211,7 +219,7 @@ __device__ __forceinline__ void delta_f64(const uint32_t* __restrict__ list, int
     if (lane == 0) {
         const double a_i_dot_x_k = s * L;
         const double a_i_dot_a_i = (L * L) * n;
-        delta[k] = L * ((b_i - a_i_dot_x_k) / a_i_dot_a_i) * lambda;
+        delta[rk] = L * ((b_i - a_i_dot_x_k) / a_i_dot_a_i) * lambda;
     }
 }
 
@@ -221,6 +229,7 @@ __device__ __forceinline__ void delta_f32(const uint32_t* __restrict__ list, int
                                           const float* __restrict__ x, float lambda, float* __restrict__ delta,
                                           int lane) {
     const uint32_t k = pos.k[q];
+    const uint32_t rk = pos.r[q];
     const uint32_t n = pos.len[q];
     const float L = (float)pos.chord[q];
     const uint32_t* p = list + q * stride;
@@ -245,27 +254,30 @@ __device__ __forceinline__ void delta_f32(const uint32_t* __restrict__ list, int
     if (lane == 0) {
         const float a_i_dot_x_k = s * L;
         const float a_i_dot_a_i = (L * L) * (float)n;
-        delta[k] = L * ((b_i - a_i_dot_x_k) / a_i_dot_a_i) * lambda;
+        delta[rk] = L * ((b_i - a_i_dot_x_k) / a_i_dot_a_i) * lambda;
     }
 }
 
 /* The research variants of the FP64 update (the reference's CPU functions,
  * with its expressions and operation order):
- *   1 DROP_blocks_red + DROP_update (.cu:8005-8055): history k writes its
- *     residual r = b - L sum x to out[k] and a.a = L^2 n to out[B + k];
+ *   1 DROP_blocks_red + DROP_update (.cu:8005-8055): history k (rank rk)
+ *     writes its residual r = b - L sum x to out[rk] and a.a = L^2 n to
+ *     out[B + rk];
  *     voxel v adds, over the block's histories through it in history order,
  *     (L lambda) (r / (a.a + psi_v)), psi_v = ((1 - x_v) ETA) PSI_SIGN, and
  *     x_v += sum / count;
  *   2 DROP_blocks_robust2 + DROP_update_robust2 (.cu:7871-7901): history k
- *     writes (L lambda) (r / a.a) to out[k]; voxel v sums those and
+ *     writes (L lambda) (r / a.a) to out[rk]; voxel v sums those and
  *     N_v = sum L^2, and x_v += (N_v / (N_v + psi_v)) sum / count with
  *     psi_v = PSI_SIGN (1 - x_v) ETA, the first product in float.
- * x_v is the value at the block's start in both. */
+ * Both write the chord L to out[2B + rk] for the voxel updates. x_v is the
+ * value at the block's start in both. */
 __device__ __forceinline__ void delta_variant(int variant, const uint32_t* __restrict__ list, int stride,
                                               blk_pos pos, const float* __restrict__ wepl, int64_t h0, int64_t q,
                                               const float* __restrict__ x, double lambda, double* __restrict__ out,
                                               int64_t B, int lane) {
     const uint32_t k = pos.k[q];
+    const uint32_t rk = pos.r[q];
     const uint32_t n = pos.len[q];
     const double L = pos.chord[q];
     const uint32_t* p = list + q * stride;
@@ -292,11 +304,12 @@ __device__ __forceinline__ void delta_variant(int variant, const uint32_t* __res
         const double a_i_dot_a_i = (L * L) * n;
         const double residual = b_i - a_i_dot_x_k;
         if (variant == 1) {
-            out[k] = residual;
-            out[B + k] = a_i_dot_a_i;
+            out[rk] = residual;
+            out[B + rk] = a_i_dot_a_i;
         } else {
-            out[k] = (L * lambda) * (residual / a_i_dot_a_i);
+            out[rk] = (L * lambda) * (residual / a_i_dot_a_i);
         }
+        out[2 * B + rk] = L;
     }
 }
 
@@ -333,9 +346,9 @@ __device__ __forceinline__ uint32_t for_each_history(uint64_t r, const uint16_t*
 
 template <int VARIANT>
 __device__ __forceinline__ void apply_variant(uint64_t r, const uint16_t* __restrict__ csc,
-                                              const double* __restrict__ out, int64_t B,
-                                              const double* __restrict__ chord_b, double lambda, double eta,
+                                              const double* __restrict__ out, int64_t B, double lambda, double eta,
                                               int psi_sign, float* __restrict__ x) {
+    const double* __restrict__ chord_b = out + 2 * B;
     const uint32_t v = (uint32_t)(r & (SEG_MAX_V - 1));
     const float xv = x[v];
     double upd = 0.0;
@@ -575,6 +588,7 @@ __device__ __forceinline__ void fused_delta(const uint32_t* __restrict__ list, c
                                             int64_t q, const uint64_t* __restrict__ xw, const fused_prev<F32>& pv,
                                             double lambda, void* __restrict__ delta, int lane) {
     const uint32_t k = pos.k[q];
+    const uint32_t rk = pos.r[q];
     const uint32_t n = pos.len[q];
     const double Ld = pos.chord[q];
     const uint32_t* p = list + q * stride;
@@ -602,7 +616,7 @@ __device__ __forceinline__ void fused_delta(const uint32_t* __restrict__ list, c
             const float L = (float)Ld;
             const float a_i_dot_x_k = s * L;
             const float a_i_dot_a_i = (L * L) * (float)n;
-            ((float*)delta)[k] = L * ((b_i - a_i_dot_x_k) / a_i_dot_a_i) * (float)lambda;
+            ((float*)delta)[rk] = L * ((b_i - a_i_dot_x_k) / a_i_dot_a_i) * (float)lambda;
         }
     } else {
         double s = 0.0;
@@ -618,7 +632,7 @@ __device__ __forceinline__ void fused_delta(const uint32_t* __restrict__ list, c
         if (lane == 0) {
             const double a_i_dot_x_k = s * Ld;
             const double a_i_dot_a_i = (Ld * Ld) * n;
-            ((double*)delta)[k] = Ld * ((b_i - a_i_dot_x_k) / a_i_dot_a_i) * lambda;
+            ((double*)delta)[rk] = Ld * ((b_i - a_i_dot_x_k) / a_i_dot_a_i) * lambda;
         }
     }
 }
@@ -679,6 +693,7 @@ __device__ __forceinline__ void wait_delta(const uint32_t* __restrict__ list, in
                                            const uint64_t* __restrict__ xw, uint32_t stamp, int polls,
                                            int* status, double lambda, void* __restrict__ delta, int lane) {
     const uint32_t k = pos.k[q];
+    const uint32_t rk = pos.r[q];
     const uint32_t n = pos.len[q];
     const double Ld = pos.chord[q];
     const uint32_t* p = list + q * stride;
@@ -715,7 +730,7 @@ __device__ __forceinline__ void wait_delta(const uint32_t* __restrict__ list, in
             const float L = (float)Ld;
             const float a_i_dot_x_k = s * L;
             const float a_i_dot_a_i = (L * L) * (float)n;
-            ((float*)delta)[k] = L * ((b_i - a_i_dot_x_k) / a_i_dot_a_i) * (float)lambda;
+            ((float*)delta)[rk] = L * ((b_i - a_i_dot_x_k) / a_i_dot_a_i) * (float)lambda;
         }
     } else {
         double s = 0.0;
@@ -731,7 +746,7 @@ __device__ __forceinline__ void wait_delta(const uint32_t* __restrict__ list, in
         if (lane == 0) {
             const double a_i_dot_x_k = s * Ld;
             const double a_i_dot_a_i = (Ld * Ld) * n;
-            ((double*)delta)[k] = Ld * ((b_i - a_i_dot_x_k) / a_i_dot_a_i) * lambda;
+            ((double*)delta)[rk] = Ld * ((b_i - a_i_dot_x_k) / a_i_dot_a_i) * lambda;
         }
     }
 }
