@@ -608,9 +608,12 @@ extern "C" int pct_batch_iteration(pct_batch* b, const pct_hist_soa* hists, floa
                 const int64_t h0 = std::min(pl->n, b->chunk[c] * pl->B);
                 const int64_t h1 = std::min(pl->n, b->chunk[c + 1] * pl->B);
                 if (h1 <= h0) continue;
-                rc = pct_mlp_paths_on(cs, b->aux, &hists[s], h0, h1 - h0, (const int64_t*)pl->hist_off.ptr + h0,
-                                      nullptr, nullptr, nullptr, nullptr, (const uint32_t*)pl->hash.ptr + h0, mlp,
-                                      false);
+                rc = mlp == PCT_MLP_FP64
+                         ? pct_mlp_verify_lanes(cs, b->aux, &hists[s], h0, h1 - h0, (const uint16_t*)pl->lane_of.ptr,
+                                                (const uint32_t*)pl->hash.ptr + h0)
+                         : pct_mlp_paths_on(cs, b->aux, &hists[s], h0, h1 - h0, (const int64_t*)pl->hist_off.ptr + h0,
+                                            nullptr, nullptr, nullptr, nullptr, (const uint32_t*)pl->hash.ptr + h0, mlp,
+                                            false);
                 if (rc) {
                     pct_fail(c0, rc, "scan %d: %s", s, pct_last_error(cs));
                     return rc;

@@ -408,6 +408,25 @@ __device__ __forceinline__ int64_t mlp_step_r(const pct_geo& g, const mlp_hist& 
     return near ? -2 : voxel;
 }
 
+/* mlp_step_r with step k's table row given (u_1, S1I0, S1I1, S1I3, cst1,
+ * cst2, cst3): the same expressions, so the same voxel bit for bit */
+__device__ __forceinline__ int64_t mlp_step_r_row(const pct_geo& g, const mlp_hist& m, const double (&row)[MLP_TAB_FIELDS]) {
+    const double u_1 = row[MLP_TAB_U1];
+    double t_1, v_1;
+    if (!mlp_tv_terms<true>(g, m, u_1, row[MLP_TAB_S1I0], row[MLP_TAB_S1I1], row[MLP_TAB_S1I3], row[MLP_TAB_C1],
+                            row[MLP_TAB_C2], row[MLP_TAB_C3], t_1, v_1))
+        return -2;
+    double x_1 = (m.ca * (m.u_in + u_1)) - (m.sa * t_1);
+    double y_1 = (m.sa * (m.u_in + u_1)) + (m.ca * t_1);
+    bool near = false;
+    int vx = calc_voxel_r(g.x_zero, x_1, g.inv_width, near);
+    int vy = calc_voxel_r(g.y_zero, y_1, g.inv_height, near);
+    int vz = calc_voxel_r(g.z_zero, v_1, g.inv_thickness, near);
+    int64_t voxel = (int64_t)vx + (int64_t)vy * g.C + (int64_t)vz * g.C * g.R;
+    if (vx < 0 || vy < 0 || vz < 0 || voxel >= g.V) voxel = -1;
+    return near ? -2 : voxel;
+}
+
 /* position-aware mix of one list element; a list's hash is the sum over its
  * elements plus a length term, so it can be formed by a wave reduction */
 __device__ __forceinline__ uint32_t pct_list_mix(uint32_t v, uint32_t pos) {
@@ -740,6 +759,82 @@ k_mlp_paths(pct_geo g, pct_hist_soa h, int64_t start, int64_t count, W walk,
     }
     if (lane == 0 && local_flags) atomicOr(status, local_flags);
 }
+
+/* The verification pass of a recomputing DROP iteration with one lane per
+ * history: a wavefront takes 64 consecutive histories, its lanes in the
+ * order of their step counts (lane_of, from the plan: lanes whose walks have
+ * ended are then contiguous; any order that spreads a wave's histories over
+ * more than these 64 took the set-up reads out of L2 and slowed the block
+ * loop beside it), and walks them side by side, step k of all of them
+ * together. The per-depth table row of step k is then the same in every
+ * lane (one LDS address per step, broadcast reads), the dedup of
+ * .cu:5478-5485 is a compare with the lane's
+ * previous voxel, and no history's constants are broadcast. Nothing is
+ * stored: each list's hash (the position-aware mixes of its elements plus
+ * the length term, as walk_list forms it) is compared with the plan's. The
+ * steps are mlp_step_r's expressions (walk.fix where it defers), so the
+ * voxels are the lane-per-step walk's bit for bit. Only for the exact walk. */
+template <class W>
+__global__ void __launch_bounds__(256, 5)
+k_mlp_verify(pct_geo g, pct_hist_soa h, int64_t start, int64_t count, W walk, const uint16_t* __restrict__ lane_of,
+             const uint32_t* __restrict__ hash_expect, int* __restrict__ status) {
+    /* the per-depth table staged in LDS one row per step (the 7 fields of
+     * step k together: one address per step, the fields at fixed offsets) */
+    extern __shared__ double s_rows[];
+    const int K = walk.K;
+    for (int i = threadIdx.x; i < MLP_TAB_FIELDS * K; i += blockDim.x) {
+        const int f = i / K, k = i - f * K;
+        s_rows[k * MLP_TAB_FIELDS + f] = walk.tab[i];
+    }
+    __syncthreads();
+    const int lane = threadIdx.x & 63;
+    const int64_t waves = (int64_t)gridDim.x * (blockDim.x >> 6);
+    int flags = 0;
+    for (int64_t base = ((int64_t)blockIdx.x * (blockDim.x >> 6) + (threadIdx.x >> 6)) * 64; base < count;
+         base += waves * 64) {
+        const bool have = base + lane < count;
+        typename W::H m;
+        int S = 0;
+        uint32_t prev = 0, hix = 0;
+        if (have) {
+            /* the wave's 64 histories, lanes in step-count order */
+            hix = (uint32_t)(base + (lane_of ? lane_of[start + base + lane] : lane));
+            double chord;
+            walk.setup(g, h, start + hix, m, chord, S);
+            prev = h.first_voxel[start + hix];
+            if (S >= K) flags |= PCT_STATUS_TABLE;
+            if (S > K) S = K;
+        }
+        int smax = S;
+#pragma unroll
+        for (int d = 32; d >= 1; d >>= 1) smax = max(smax, __shfl_xor(smax, d, 64));
+        uint32_t n = 1, hsh = pct_list_mix(prev, 0);
+        for (int k = 0; k < smax; ++k) {
+            if (k < S) {
+                double row[MLP_TAB_FIELDS];
+#pragma unroll
+                for (int f = 0; f < MLP_TAB_FIELDS; ++f) row[f] = s_rows[k * MLP_TAB_FIELDS + f];
+                int64_t vox = mlp_step_r_row(g, m, row);
+                if (vox == -2) vox = walk.fix(g, m, k);
+                if (vox >= 0) {
+                    const uint32_t v = (uint32_t)vox;
+                    if (v != prev) {
+                        hsh += pct_list_mix(v, n);
+                        ++n;
+                    }
+                    prev = v;
+                }
+            }
+        }
+        if (have) {
+            hsh += n * 0x27d4eb2fu;
+            if (hash_expect[hix] != hsh) flags |= PCT_STATUS_PLAN;
+            if ((int64_t)n > g.max_intersections) flags |= PCT_STATUS_OVERFLOW;
+        }
+    }
+    if (flags) atomicOr(status, flags);
+}
+
 
 /* ------------------------------------------------------------------------- */
 /* Learned-path form (include/pct_net.h): the network's inputs, the FP32      */
@@ -1364,6 +1459,68 @@ int pct_mlp_paths_impl(pct_ctx* ctx, const pct_hist_soa* hist, int64_t start, in
                        const uint32_t* hash_expect, int mode) {
     return pct_mlp_paths_on(ctx, ctx->stream, hist, start, count, offsets, voxels, lengths, chord, hash_out,
                             hash_expect, mode, false);
+}
+
+/* lane_of for k_mlp_verify: per group of 64 consecutive histories (from
+ * history 0; a launch's start is a multiple of 64), lane_of[g * 64 + r] is
+ * the offset within the group of the history whose step count has rank r
+ * (ties by offset): one wavefront per group, each lane counting the lanes
+ * ranked before it */
+__global__ void k_verify_lanes_order(pct_geo g, pct_hist_soa h, int64_t count, const double* __restrict__ tab, int K,
+                                     uint16_t* __restrict__ lane_of) {
+    const int64_t base = ((int64_t)blockIdx.x * (blockDim.x >> 6) + (threadIdx.x >> 6)) * 64;
+    const int lane = threadIdx.x & 63;
+    if (base >= count) return;
+    const bool have = base + lane < count;
+    int S = 0x7fffffff;
+    if (have) {
+        mlp_hist m;
+        mlp_setup(g, h, base + lane, tab + MLP_TAB_U1 * K, K, m);
+        S = m.S;
+    }
+    int rank = 0;
+    for (int j = 0; j < 64; ++j) {
+        const int sj = __shfl(S, j, 64);
+        rank += (sj < S) || (sj == S && j < lane);
+    }
+    if (have) lane_of[base + rank] = (uint16_t)lane;
+}
+
+int pct_mlp_verify_order(pct_ctx* ctx, const pct_hist_soa* hist, int64_t count, uint16_t* lane_of) {
+    int rc = mlp_occupancy(ctx);
+    if (rc) return rc;
+    if (count <= 0) return PCT_OK;
+    const int64_t groups = (count + 63) / 64;
+    hipLaunchKernelGGL(k_verify_lanes_order, dim3((unsigned)((groups + 3) / 4)), dim3(256), 0, ctx->stream, ctx->g,
+                       *hist, count, (const double*)ctx->d_mlp_tab, ctx->mlp_tab_k, lane_of);
+    PCT_HIP_CHECK(ctx, hipGetLastError());
+    return PCT_OK;
+}
+
+/* the verification pass over histories [start, start + count) with one lane
+ * per history (FP64 exact walk), on `stream`; hash_expect[i] is history
+ * start + i's; lane_of (pct_mlp_verify_order of the whole scan) is used when
+ * the range is the whole scan, else the lanes keep history order */
+int pct_mlp_verify_lanes(pct_ctx* ctx, hipStream_t stream, const pct_hist_soa* hist, int64_t start, int64_t count,
+                         const uint16_t* lane_of, const uint32_t* hash_expect) {
+    int rc = mlp_occupancy(ctx);
+    if (rc) return rc;
+    if (count <= 0) return PCT_OK;
+    const size_t shm = sizeof(double) * MLP_TAB_FIELDS * (size_t)ctx->mlp_tab_k;
+    if (shm > PCT_MLP_LDS_TAB_MAX)   /* a table beyond LDS: the lane-per-step pass (no list store) */
+        return pct_mlp_paths_on(ctx, stream, hist, start, count, nullptr, nullptr, nullptr, nullptr, nullptr,
+                                hash_expect, PCT_MLP_FP64, true);
+    int occ = 0;
+    PCT_HIP_CHECK(ctx, hipOccupancyMaxActiveBlocksPerMultiprocessor(&occ, k_mlp_verify<exact_walk>, 256, shm));
+    int per_cu = std::max(1, occ);
+    if (ctx->tune_mlp_waves > 0 && ctx->tune_mlp_waves < per_cu) per_cu = ctx->tune_mlp_waves;
+    const unsigned grid = std::min(pct_grid((count + 63) / 64 * 64, 256), (unsigned)(ctx->cus * per_cu));
+    exact_walk w{(const double*)ctx->d_mlp_tab, ctx->mlp_tab_k};
+    if (start != 0 || count != hist->n) lane_of = nullptr;
+    hipLaunchKernelGGL(k_mlp_verify<exact_walk>, dim3(grid), dim3(256), shm, stream, ctx->g, *hist, start, count, w,
+                       lane_of, hash_expect, ctx->d_status);
+    PCT_HIP_CHECK(ctx, hipGetLastError());
+    return PCT_OK;
 }
 
 int pct_status_to_rc(pct_ctx* ctx, int flags) {

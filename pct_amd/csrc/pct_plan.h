@@ -41,6 +41,9 @@ struct pct_plan {
     pct_buffer len;                /* u32 per history: list length          */
     pct_buffer chord;              /* f64 per history: effective chord      */
     pct_buffer hash;               /* u32 per history: hash of its list     */
+    pct_buffer lane_of;            /* u16 per history: the verification
+                                      pass's lane order within each group of
+                                      64 (by step count)                    */
     /* block transpose: per block the touched voxels, those with more than the
      * inline number of histories first, each part sorted by voxel, each an
      * 8-byte record {voxel, count | history indices}; csc holds all
@@ -92,5 +95,10 @@ bool pct_plan_ready(const pct_ctx* ctx, const pct_hist_soa* hist, int B, int mlp
  * and the tiled image back into x */
 int pct_plan_image_in(pct_ctx* ctx, hipStream_t s, const float* x, float** xt);
 int pct_plan_image_out(pct_ctx* ctx, hipStream_t s, float* x);
+/* pct_mlp.hip: the verification pass of the FP64 walk with one lane per
+ * history */
+int pct_mlp_verify_order(pct_ctx* ctx, const pct_hist_soa* hist, int64_t count, uint16_t* lane_of);
+int pct_mlp_verify_lanes(pct_ctx* ctx, hipStream_t stream, const pct_hist_soa* hist, int64_t start, int64_t count,
+                         const uint16_t* lane_of, const uint32_t* hash_expect);
 
 #endif

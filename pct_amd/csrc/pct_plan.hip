@@ -488,7 +488,7 @@ static void release(pct_buffer& b) {
 void pct_plan_destroy(pct_plan* pl) {
     if (!pl) return;
     plan_release_graph(pl);
-    pct_buffer* bufs[] = {&pl->list, &pl->prev, &pl->hist_off, &pl->len, &pl->chord, &pl->hash, &pl->segs,
+    pct_buffer* bufs[] = {&pl->list, &pl->prev, &pl->hist_off, &pl->len, &pl->chord, &pl->hash, &pl->lane_of, &pl->segs,
                           &pl->csc, &pl->long_rows, &pl->blk_seg, &pl->blk_long, &pl->blk_entry, &pl->delta,
                           &pl->xw, &pl->xt, &pl->pos_k, &pl->pos_len, &pl->pos_chord, &pl->pos_r, &pl->rank};
     for (pct_buffer* b : bufs) release(*b);
@@ -839,6 +839,14 @@ static int plan_build(pct_ctx* ctx, const pct_hist_soa* hist, int B, int mlp_mod
     /* deltas of two blocks (the one-launch form reads block b-1's while writing
      * block b's), two doubles per history (ROBUST) */
     if ((rc = pct_ensure(ctx, pl->delta, sizeof(double) * 4 * (size_t)B))) return rc;
+    /* the verification pass's lane order */
+    if (mlp_mode == PCT_MLP_FP64) {
+        if ((rc = pct_ensure(ctx, pl->lane_of, sizeof(uint16_t) * (size_t)n)) ||
+            (rc = pct_mlp_verify_order(ctx, hist, n, (uint16_t*)pl->lane_of.ptr)))
+            return rc;
+    } else {
+        release(pl->lane_of);
+    }
     pl->n = n;
     pl->B = B;
     pl->nb = nb;
@@ -1131,8 +1139,11 @@ static int plan_iterate(pct_ctx* ctx, pct_plan* pl, const pct_hist_soa* hist, fl
     if (ctx->profile) PCT_HIP_CHECK(ctx, hipEventRecord(ctx->ev[0], ctx->stream));
     if (recompute && ctx->tune_overlap == 2) {
         /* sequential: all lists first, then the block loop */
-        rc = pct_mlp_paths_on(ctx, ctx->stream, hist, 0, pl->n, (const int64_t*)pl->hist_off.ptr, nullptr, nullptr,
-                              nullptr, nullptr, (const uint32_t*)pl->hash.ptr, mlp, false);
+        rc = mlp == PCT_MLP_FP64
+                 ? pct_mlp_verify_lanes(ctx, ctx->stream, hist, 0, pl->n, (const uint16_t*)pl->lane_of.ptr,
+                                        (const uint32_t*)pl->hash.ptr)
+                 : pct_mlp_paths_on(ctx, ctx->stream, hist, 0, pl->n, (const int64_t*)pl->hist_off.ptr, nullptr,
+                                    nullptr, nullptr, nullptr, (const uint32_t*)pl->hash.ptr, mlp, false);
         if (rc) return rc;
         recompute = false;
         if (ctx->profile) {
@@ -1150,8 +1161,13 @@ static int plan_iterate(pct_ctx* ctx, pct_plan* pl, const pct_hist_soa* hist, fl
         for (int64_t c = 0; c < nchunks; ++c) {
             int64_t h0 = pl->chunk_b[c] * pl->B;
             int64_t h1 = std::min(pl->n, pl->chunk_b[c + 1] * pl->B);
-            rc = pct_mlp_paths_on(ctx, aux, hist, h0, h1 - h0, (const int64_t*)pl->hist_off.ptr + h0, nullptr, nullptr,
-                                  nullptr, nullptr, (const uint32_t*)pl->hash.ptr + h0, mlp, true);
+            /* the exact walk's check: the lane-per-history pass */
+            if (mlp == PCT_MLP_FP64)
+                rc = pct_mlp_verify_lanes(ctx, aux, hist, h0, h1 - h0, (const uint16_t*)pl->lane_of.ptr,
+                                          (const uint32_t*)pl->hash.ptr + h0);
+            else
+                rc = pct_mlp_paths_on(ctx, aux, hist, h0, h1 - h0, (const int64_t*)pl->hist_off.ptr + h0, nullptr,
+                                      nullptr, nullptr, nullptr, (const uint32_t*)pl->hash.ptr + h0, mlp, true);
             if (rc) return rc;
             PCT_HIP_CHECK(ctx, hipEventRecord(pl->ev_chunk[c], aux));
             if (ctx->profile && c == 0) PCT_HIP_CHECK(ctx, hipEventRecord(ctx->ev[1], aux));
@@ -1194,15 +1210,16 @@ static int plan_iterate(pct_ctx* ctx, pct_plan* pl, const pct_hist_soa* hist, fl
  * path lengths: the capacity-strided build store holds 4 B per slot),
  * `list_slots` slots of the block-strided list store (4 B each, 12 B with the
  * one-launch form's previous-block records), `entries` path entries (at most
- * one 8 B segment record and one 2 B csc index each), 40 B per history kept
- * (length, chord, hash, row offset, placement, rank) and 34 B per history plus up
+ * one 8 B segment record and one 2 B csc index each), 42 B per history kept
+ * (length, chord, hash, row offset, placement, rank, verification lane
+ * order) and 34 B per history plus up
  * to 4 GiB of workspace while building (offsets, placement sort, the radix
  * sort of a chunk of blocks, the voxel maps). Before the MLP pass only n and
  * slots are known: list_slots = entries = 0 gives what the build needs to
  * start. */
 extern "C" int64_t pct_drop_plan_bytes(int64_t n, int64_t slots, int64_t list_slots, int64_t entries,
                                        int with_prev) {
-    return 4 * slots + (with_prev ? 12 : 4) * list_slots + 10 * entries + 74 * n + ((int64_t)4 << 30);
+    return 4 * slots + (with_prev ? 12 : 4) * list_slots + 10 * entries + 76 * n + ((int64_t)4 << 30);
 }
 
 /* 1: build a plan; 0: run the chunked reference structure instead -- when
@@ -1234,7 +1251,7 @@ static int64_t plan_budget(pct_ctx* ctx) {
 
 static void plan_free_buffers(pct_plan* pl) {
     plan_release_graph(pl);
-    for (pct_buffer* b : {&pl->list, &pl->prev, &pl->hist_off, &pl->len, &pl->chord, &pl->hash, &pl->segs, &pl->csc,
+    for (pct_buffer* b : {&pl->list, &pl->prev, &pl->hist_off, &pl->len, &pl->chord, &pl->hash, &pl->lane_of, &pl->segs, &pl->csc,
                           &pl->long_rows, &pl->blk_seg, &pl->blk_long, &pl->blk_entry, &pl->delta, &pl->xw, &pl->xt,
                           &pl->pos_k, &pl->pos_len, &pl->pos_chord, &pl->pos_r, &pl->rank})
         release(*b);
