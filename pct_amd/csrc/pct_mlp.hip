@@ -22,6 +22,7 @@
  */
 #include <algorithm>
 #include <vector>
+#include <rocprim/device/device_radix_sort.hpp>
 #include "pct_internal.h"
 #include "pct_net_basis.h"
 #include "pct_net_dev.h"
@@ -1459,6 +1460,91 @@ int pct_mlp_paths_impl(pct_ctx* ctx, const pct_hist_soa* hist, int64_t start, in
                        const uint32_t* hash_expect, int mode) {
     return pct_mlp_paths_on(ctx, ctx->stream, hist, start, count, offsets, voxels, lengths, chord, hash_out,
                             hash_expect, mode, false);
+}
+
+/* step count of history k (the key of the check's global order) and k */
+__global__ void k_verify_keys(pct_geo g, pct_hist_soa h, int64_t count, const double* __restrict__ tab, int K,
+                              uint32_t* __restrict__ keys, uint32_t* __restrict__ vals) {
+    const int64_t k = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (k >= count) return;
+    mlp_hist m;
+    mlp_setup(g, h, k, tab + MLP_TAB_U1 * K, K, m);
+    keys[k] = (uint32_t)(m.S < 0 ? 0 : (m.S > 0xffff ? 0xffff : m.S));
+    vals[k] = (uint32_t)k;
+}
+
+/* the histories [0, count) in the order of their step counts (order: count
+ * u32; keys and vals: count u32 each of workspace) */
+int pct_mlp_step_order(pct_ctx* ctx, const pct_hist_soa* hist, int64_t count, uint32_t* keys, uint32_t* vals,
+                       uint32_t* order) {
+    int rc = mlp_occupancy(ctx);
+    if (rc) return rc;
+    if (count <= 0) return PCT_OK;
+    hipLaunchKernelGGL(k_verify_keys, dim3(pct_grid(count, 256)), dim3(256), 0, ctx->stream, ctx->g, *hist, count,
+                       (const double*)ctx->d_mlp_tab, ctx->mlp_tab_k, keys, vals);
+    PCT_HIP_CHECK(ctx, hipGetLastError());
+    size_t tb = 0;
+    PCT_HIP_CHECK(ctx, rocprim::radix_sort_pairs(nullptr, tb, keys, keys, vals, order, (size_t)count, 0, 16,
+                                                 ctx->stream));
+    pct_buffer tmp, kout;
+    auto free_both = [&]() {
+        for (pct_buffer* b : {&tmp, &kout})
+            if (b->ptr) {
+                (void)hipFree(b->ptr);
+                b->ptr = nullptr;
+                b->bytes = 0;
+            }
+    };
+    if ((rc = pct_ensure(ctx, tmp, tb > 0 ? tb : 1)) || (rc = pct_ensure(ctx, kout, sizeof(uint32_t) * (size_t)count))) {
+        free_both();
+        return rc;
+    }
+    hipError_t e = rocprim::radix_sort_pairs(tmp.ptr, tb, keys, (uint32_t*)kout.ptr, vals, order, (size_t)count, 0,
+                                             16, ctx->stream);
+    if (e == hipSuccess) e = hipStreamSynchronize(ctx->stream);
+    free_both();
+    if (e != hipSuccess) return pct_fail(ctx, PCT_E_HIP, "step order: %s", hipGetErrorString(e));
+    return PCT_OK;
+}
+
+/* the check's inputs in step order: the fields the MLP set-up reads, the
+ * first voxel and the plan's hash of history order[i] into slot i (dst:
+ * 12 arrays of count u32, fields in pct_hist_soa order, then first voxel,
+ * then hash) */
+__global__ void k_gather_steps(pct_hist_soa h, const uint32_t* __restrict__ hash, const uint32_t* __restrict__ order,
+                               int64_t count, uint32_t* __restrict__ dst) {
+    const int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= count) return;
+    const uint32_t k = order[i];
+    const float* f[10] = {h.x_entry, h.y_entry, h.z_entry, h.x_exit, h.y_exit, h.z_exit,
+                          h.xy_entry_angle, h.xz_entry_angle, h.xy_exit_angle, h.xz_exit_angle};
+#pragma unroll
+    for (int a = 0; a < 10; ++a) dst[a * count + i] = __float_as_uint(f[a][k]);
+    dst[10 * count + i] = h.first_voxel[k];
+    dst[11 * count + i] = hash[k];
+}
+
+int pct_mlp_verify_lanes(pct_ctx* ctx, hipStream_t stream, const pct_hist_soa* hist, int64_t start, int64_t count,
+                         const uint16_t* lane_of, const uint32_t* hash_expect);
+
+/* the verification pass over all histories in step order: gather, then
+ * k_mlp_verify on the gathered copy (the inputs are read again every call,
+ * so a history changed in place still fails its hash) */
+int pct_mlp_verify_ordered(pct_ctx* ctx, hipStream_t stream, const pct_hist_soa* hist, const uint32_t* order,
+                           const uint32_t* hash_expect, uint32_t* work) {
+    const int64_t n = hist->n;
+    if (n <= 0) return PCT_OK;
+    hipLaunchKernelGGL(k_gather_steps, dim3(pct_grid(n, 256)), dim3(256), 0, stream, *hist, hash_expect, order, n,
+                       work);
+    PCT_HIP_CHECK(ctx, hipGetLastError());
+    pct_hist_soa s = *hist;
+    float** fp[10] = {&s.x_entry, &s.y_entry, &s.z_entry, &s.x_exit, &s.y_exit, &s.z_exit,
+                      &s.xy_entry_angle, &s.xz_entry_angle, &s.xy_exit_angle, &s.xz_exit_angle};
+    for (int a = 0; a < 10; ++a) *fp[a] = (float*)(work + a * n);
+    s.first_voxel = work + 10 * n;
+    s.wepl = nullptr;
+    s.bin_num = nullptr;
+    return pct_mlp_verify_lanes(ctx, stream, &s, 0, n, nullptr, work + 11 * n);
 }
 
 /* lane_of for k_mlp_verify: per group of 64 consecutive histories (from

@@ -44,6 +44,10 @@ struct pct_plan {
     pct_buffer lane_of;            /* u16 per history: the verification
                                       pass's lane order within each group of
                                       64 (by step count)                    */
+    pct_buffer vorder;             /* u32 per history: all histories by step
+                                      count (the one-launch check's order)  */
+    pct_buffer vwork;              /* 12 u32 per history: the check's inputs
+                                      gathered in that order each iteration */
     /* block transpose: per block the touched voxels, those with more than the
      * inline number of histories first, each part sorted by voxel, each an
      * 8-byte record {voxel, count | history indices}; csc holds all
@@ -98,6 +102,10 @@ int pct_plan_image_out(pct_ctx* ctx, hipStream_t s, float* x);
 /* pct_mlp.hip: the verification pass of the FP64 walk with one lane per
  * history */
 int pct_mlp_verify_order(pct_ctx* ctx, const pct_hist_soa* hist, int64_t count, uint16_t* lane_of);
+int pct_mlp_step_order(pct_ctx* ctx, const pct_hist_soa* hist, int64_t count, uint32_t* keys, uint32_t* vals,
+                       uint32_t* order);
+int pct_mlp_verify_ordered(pct_ctx* ctx, hipStream_t stream, const pct_hist_soa* hist, const uint32_t* order,
+                           const uint32_t* hash_expect, uint32_t* work);
 int pct_mlp_verify_lanes(pct_ctx* ctx, hipStream_t stream, const pct_hist_soa* hist, int64_t start, int64_t count,
                          const uint16_t* lane_of, const uint32_t* hash_expect);
 

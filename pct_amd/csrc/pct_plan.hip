@@ -488,7 +488,7 @@ static void release(pct_buffer& b) {
 void pct_plan_destroy(pct_plan* pl) {
     if (!pl) return;
     plan_release_graph(pl);
-    pct_buffer* bufs[] = {&pl->list, &pl->prev, &pl->hist_off, &pl->len, &pl->chord, &pl->hash, &pl->lane_of, &pl->segs,
+    pct_buffer* bufs[] = {&pl->list, &pl->prev, &pl->hist_off, &pl->len, &pl->chord, &pl->hash, &pl->lane_of, &pl->vorder, &pl->vwork, &pl->segs,
                           &pl->csc, &pl->long_rows, &pl->blk_seg, &pl->blk_long, &pl->blk_entry, &pl->delta,
                           &pl->xw, &pl->xt, &pl->pos_k, &pl->pos_len, &pl->pos_chord, &pl->pos_r, &pl->rank};
     for (pct_buffer* b : bufs) release(*b);
@@ -844,8 +844,17 @@ static int plan_build(pct_ctx* ctx, const pct_hist_soa* hist, int B, int mlp_mod
         if ((rc = pct_ensure(ctx, pl->lane_of, sizeof(uint16_t) * (size_t)n)) ||
             (rc = pct_mlp_verify_order(ctx, hist, n, (uint16_t*)pl->lane_of.ptr)))
             return rc;
+        /* the one-launch check's step order; its gather space doubles as
+         * the sort's workspace here */
+        if ((rc = pct_ensure(ctx, pl->vorder, sizeof(uint32_t) * (size_t)n)) ||
+            (rc = pct_ensure(ctx, pl->vwork, sizeof(uint32_t) * 12 * (size_t)n)) ||
+            (rc = pct_mlp_step_order(ctx, hist, n, (uint32_t*)pl->vwork.ptr, (uint32_t*)pl->vwork.ptr + n,
+                                     (uint32_t*)pl->vorder.ptr)))
+            return rc;
     } else {
         release(pl->lane_of);
+        release(pl->vorder);
+        release(pl->vwork);
     }
     pl->n = n;
     pl->B = B;
@@ -1162,7 +1171,10 @@ static int plan_iterate(pct_ctx* ctx, pct_plan* pl, const pct_hist_soa* hist, fl
             int64_t h0 = pl->chunk_b[c] * pl->B;
             int64_t h1 = std::min(pl->n, pl->chunk_b[c + 1] * pl->B);
             /* the exact walk's check: the lane-per-history pass */
-            if (mlp == PCT_MLP_FP64)
+            if (mlp == PCT_MLP_FP64 && nchunks == 1 && pl->vorder.ptr)
+                rc = pct_mlp_verify_ordered(ctx, aux, hist, (const uint32_t*)pl->vorder.ptr,
+                                            (const uint32_t*)pl->hash.ptr, (uint32_t*)pl->vwork.ptr);
+            else if (mlp == PCT_MLP_FP64)
                 rc = pct_mlp_verify_lanes(ctx, aux, hist, h0, h1 - h0, (const uint16_t*)pl->lane_of.ptr,
                                           (const uint32_t*)pl->hash.ptr + h0);
             else
@@ -1210,16 +1222,16 @@ static int plan_iterate(pct_ctx* ctx, pct_plan* pl, const pct_hist_soa* hist, fl
  * path lengths: the capacity-strided build store holds 4 B per slot),
  * `list_slots` slots of the block-strided list store (4 B each, 12 B with the
  * one-launch form's previous-block records), `entries` path entries (at most
- * one 8 B segment record and one 2 B csc index each), 42 B per history kept
- * (length, chord, hash, row offset, placement, rank, verification lane
- * order) and 34 B per history plus up
+ * one 8 B segment record and one 2 B csc index each), 94 B per history kept
+ * (length, chord, hash, row offset, placement, rank, the verification
+ * pass's orders and gather space) and 34 B per history plus up
  * to 4 GiB of workspace while building (offsets, placement sort, the radix
  * sort of a chunk of blocks, the voxel maps). Before the MLP pass only n and
  * slots are known: list_slots = entries = 0 gives what the build needs to
  * start. */
 extern "C" int64_t pct_drop_plan_bytes(int64_t n, int64_t slots, int64_t list_slots, int64_t entries,
                                        int with_prev) {
-    return 4 * slots + (with_prev ? 12 : 4) * list_slots + 10 * entries + 76 * n + ((int64_t)4 << 30);
+    return 4 * slots + (with_prev ? 12 : 4) * list_slots + 10 * entries + 128 * n + ((int64_t)4 << 30);
 }
 
 /* 1: build a plan; 0: run the chunked reference structure instead -- when
@@ -1251,7 +1263,7 @@ static int64_t plan_budget(pct_ctx* ctx) {
 
 static void plan_free_buffers(pct_plan* pl) {
     plan_release_graph(pl);
-    for (pct_buffer* b : {&pl->list, &pl->prev, &pl->hist_off, &pl->len, &pl->chord, &pl->hash, &pl->lane_of, &pl->segs, &pl->csc,
+    for (pct_buffer* b : {&pl->list, &pl->prev, &pl->hist_off, &pl->len, &pl->chord, &pl->hash, &pl->lane_of, &pl->vorder, &pl->vwork, &pl->segs, &pl->csc,
                           &pl->long_rows, &pl->blk_seg, &pl->blk_long, &pl->blk_entry, &pl->delta, &pl->xw, &pl->xt,
                           &pl->pos_k, &pl->pos_len, &pl->pos_chord, &pl->pos_r, &pl->rank})
         release(*b);

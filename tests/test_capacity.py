@@ -28,7 +28,7 @@ def test_plan_bytes_terms():
     assert L.pct_drop_plan_bytes(0, 0, 1000, 0, 0) - base == 4 * 1000          # list store
     assert L.pct_drop_plan_bytes(0, 0, 1000, 0, 1) - base == 12 * 1000         # + previous-block records
     assert L.pct_drop_plan_bytes(0, 0, 0, 1000, 0) - base == 10 * 1000         # segment record + csc index
-    assert L.pct_drop_plan_bytes(1000, 0, 0, 0, 0) - base == 76 * 1000         # per history
+    assert L.pct_drop_plan_bytes(1000, 0, 0, 0, 0) - base == 128 * 1000         # per history
 
 
 def _choice(L, n, slots, list_slots, entries, V, B, budget, prev=0):
