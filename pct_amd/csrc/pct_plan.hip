@@ -1148,7 +1148,10 @@ static int plan_iterate(pct_ctx* ctx, pct_plan* pl, const pct_hist_soa* hist, fl
     if (ctx->profile) PCT_HIP_CHECK(ctx, hipEventRecord(ctx->ev[0], ctx->stream));
     if (recompute && ctx->tune_overlap == 2) {
         /* sequential: all lists first, then the block loop */
-        rc = mlp == PCT_MLP_FP64
+        rc = mlp == PCT_MLP_FP64 && pl->vorder.ptr
+                 ? pct_mlp_verify_ordered(ctx, ctx->stream, hist, (const uint32_t*)pl->vorder.ptr,
+                                          (const uint32_t*)pl->hash.ptr, (uint32_t*)pl->vwork.ptr)
+                 : mlp == PCT_MLP_FP64
                  ? pct_mlp_verify_lanes(ctx, ctx->stream, hist, 0, pl->n, (const uint16_t*)pl->lane_of.ptr,
                                         (const uint32_t*)pl->hash.ptr)
                  : pct_mlp_paths_on(ctx, ctx->stream, hist, 0, pl->n, (const int64_t*)pl->hist_off.ptr, nullptr,
